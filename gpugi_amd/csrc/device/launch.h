@@ -1,0 +1,32 @@
+// Host-side launch wrappers of the HIP kernels (defined in the .hip files).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "gpugi_rt.h"
+#include "trace_ref.h"
+#include "wide_bvh.h"
+
+namespace gpugi {
+
+hipError_t LaunchTraceRef(const RefScene& s, int query, const gpugi_ray* rays, gpugi_hit* hits, size_t n,
+                          hipStream_t stream);
+hipError_t LaunchTraceRefCounts(const RefScene& s, int query, const gpugi_ray* rays, uint32_t* nbox,
+                                uint32_t* ntri, size_t n, hipStream_t stream);
+hipError_t LaunchTraceWide(const WideScene& s, int query, const gpugi_ray* rays, gpugi_hit* hits, size_t n,
+                           uint32_t* rayCounter, hipStream_t stream);
+// Resident blocks per CU of the wide kernel (sizes the persistent grid).
+hipError_t WideBlocksPerCU(int* blocks);
+
+hipError_t LaunchRaygenPrimary(const gpugi_camera& cam, int width, int height, uint32_t seed,
+                               gpugi_ray* rays, size_t n, hipStream_t stream);
+// Builds the list of primary rays that hit (in index order) and the diffuse / shadow sets.
+hipError_t LaunchHitCompaction(const gpugi_hit* hits, size_t n, uint32_t* list, uint32_t* count,
+                               void* temp, size_t* tempBytes, hipStream_t stream);
+hipError_t LaunchRaygenDiffuse(const RefScene& s, const gpugi_ray* prim, const gpugi_hit* hits,
+                               const uint32_t* list, const uint32_t* count, uint32_t seed, gpugi_ray* rays,
+                               size_t n, hipStream_t stream);
+hipError_t LaunchRaygenShadow(const gpugi_ray* prim, const gpugi_hit* hits, const uint32_t* list,
+                              const uint32_t* count, const float* lights, uint32_t numLights,
+                              gpugi_ray* rays, size_t n, hipStream_t stream);
+
+}  // namespace gpugi

@@ -1,0 +1,184 @@
+// Materials, BSDF and next-event estimation of the reference renderer (device
+// code).  Every expression keeps the operand order of the GLSL it restates,
+// and the translation unit is compiled with -ffp-contract=off, so a render
+// with matched random streams reproduces a CPU restatement bit for bit.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "../common/gshade.h"
+#include "gpugi_rt.h"
+
+namespace gpugi {
+
+// MaterialData (materials.glsl:9-18) for a constant material.
+struct MaterialData {
+  float refl[4];  // Reflectiveness, .w = Phong exponent
+  gg::V3 opacity;
+  gg::V3 diffuse;
+  gg::V3 emissivity;
+  gg::V3 fresnel0;
+  float refractionIndexAvg;
+  gg::V3 fresnel1;
+};
+
+__device__ __forceinline__ MaterialData load_material(const gpugi_material* __restrict__ m, int id) {
+  const gpugi_material& s = m[id];
+  MaterialData d;
+  d.refl[0] = s.reflectiveness[0]; d.refl[1] = s.reflectiveness[1];
+  d.refl[2] = s.reflectiveness[2]; d.refl[3] = s.reflectiveness[3];
+  d.opacity = gg::v3(s.opacity[0], s.opacity[1], s.opacity[2]);
+  d.diffuse = gg::v3(s.diffuse[0], s.diffuse[1], s.diffuse[2]);
+  d.emissivity = gg::v3(s.emissivity[0], s.emissivity[1], s.emissivity[2]);
+  d.fresnel0 = gg::v3(s.fresnel0[0], s.fresnel0[1], s.fresnel0[2]);
+  d.refractionIndexAvg = s.refraction_index_avg;
+  d.fresnel1 = gg::v3(s.fresnel1[0], s.fresnel1[1], s.fresnel1[2]);
+  return d;
+}
+
+// AvgProbability / Avg (materials.glsl:33-41).
+__device__ __forceinline__ float avg_probability(gg::V3 c) {
+  return (c.x + c.y + c.z + gg::kDivisorEpsilon) / (3.0f + gg::kDivisorEpsilon);
+}
+__device__ __forceinline__ float avg3(gg::V3 c) { return (c.x + c.y + c.z) / 3.0f; }
+__device__ __forceinline__ float signf_(float x) { return x > 0.0f ? 1.0f : (x < 0.0f ? -1.0f : 0.0f); }
+
+// GetBSDFDecisionPropabilities with USE_CONSTANT_REFLECTION (materials.glsl:43-63).
+__device__ __forceinline__ void decision_probabilities(const MaterialData& m, gg::V3& preflect, gg::V3& prefract,
+                                                       gg::V3& pdiffuse) {
+  preflect = m.fresnel0;
+  prefract = gg::v3s(1.0f) - gg::saturate(preflect);
+  pdiffuse = prefract * m.opacity;
+  prefract = prefract - pdiffuse;
+}
+
+// GetDiffuseProbability (materials.glsl:96-101).
+__device__ __forceinline__ gg::V3 diffuse_probability(const MaterialData& m) {
+  return (gg::v3s(1.0f) - gg::saturate(m.fresnel0)) * m.opacity;
+}
+
+// __SampleBSDF (materials.glsl:118-277).  Draws Random2 then Random.
+__device__ __forceinline__ gg::V3 sample_bsdf(gg::V3 inc, const MaterialData& m, uint32_t& seed, gg::V3 N,
+                                              gg::V3& tp, bool adjoint, float& pdf, bool& reflectedDiffuse) {
+  const float cosTheta = gg::dot(N, inc);
+  const float cosThetaAbs = gg::saturate(fabsf(cosTheta));
+  gg::V3 rrd = inc - (2.0f * cosTheta) * N;
+  const float r0 = gg::random01(seed);
+  const float r1 = gg::random01(seed);
+  gg::V3 preflect, prefract, pdiffuse;
+  decision_probabilities(m, preflect, prefract, pdiffuse);
+  const float avgPReflect = avg_probability(preflect);
+  const float avgPRefract = avg_probability(prefract);
+  const float avgPDiffuse = avg3(pdiffuse);
+  gg::V3 pphong = preflect;
+  float avgPPhong = avgPReflect;
+  const float pathDecisionVar = gg::random01(seed);
+  if (avgPDiffuse > pathDecisionVar) {
+    reflectedDiffuse = true;
+    tp = tp * ((m.diffuse * pdiffuse) / avgPDiffuse);
+    if (cosTheta > 0.0f) N = -N;
+    gg::V3 U, V;
+    gg::create_onb(N, U, V);
+    gg::V3 outDir = gg::sample_hemisphere_cosine(r0, r1, U, V, N);
+    const float cosThetaOut = gg::dot(N, outDir);
+    pdf = gg::saturate(cosThetaOut) / gg::kPI;
+    return outDir;
+  } else if (avgPDiffuse + avgPRefract > pathDecisionVar) {
+    pphong = prefract;
+    avgPPhong = avgPRefract;
+    const float eta = cosTheta < 0.0f ? 1.0f / m.refractionIndexAvg : m.refractionIndexAvg;
+    const float etaSq = eta * eta;
+    const float sinTheta2Sq = etaSq * (1.0f - cosTheta * cosTheta);
+    if (sinTheta2Sq < 1.0f) {
+      rrd = eta * inc + (signf_(cosTheta) * (sqrtf(gg::saturate(1.0f - sinTheta2Sq)) - eta * cosThetaAbs)) * N;
+      if (!adjoint) tp = tp / etaSq;
+    }
+  }
+  reflectedDiffuse = false;
+  rrd = gg::normalize(rrd);
+  gg::V3 RU, RV;
+  gg::create_onb(rrd, RU, RV);
+  // The rejection loop of materials.glsl:224-240 runs exactly once (plausible = true).
+  gg::V3 outDir = gg::sample_phong_lobe(r0, r1, m.refl[3], RU, RV, rrd);
+  if (!adjoint) tp = tp * (fabsf(gg::dot(N, outDir)) / fabsf(gg::dot(N, rrd)));
+  const float cosThetaOutAbs = fabsf(gg::dot(outDir, rrd));
+  pdf = (m.refl[3] + 1.0f) / gg::kPI_2 * gg::powf_(cosThetaOutAbs, m.refl[3]);
+  pdf = pdf * 1.0f;
+  tp = tp * (pphong / avgPPhong);
+  return outDir;
+}
+
+// __BSDF (materials.glsl:306-391).
+__device__ __forceinline__ gg::V3 eval_bsdf(gg::V3 inc, gg::V3 exc, const MaterialData& m, gg::V3 N, float& pdf,
+                                            bool adjoint, bool specularOnly) {
+  gg::V3 bsdf = gg::v3s(0.0f);
+  const float cosTheta = gg::dot(N, inc);
+  const float cosThetaAbs = gg::saturate(fabsf(cosTheta));
+  const float cosThetaOut = gg::dot(N, exc);
+  gg::V3 preflect, prefract, pdiffuse;
+  decision_probabilities(m, preflect, prefract, pdiffuse);
+  const float phongFactorBrdf = (m.refl[3] + 1.0f) / (2.0f * gg::kPI_2 * fabsf(cosThetaOut) + gg::kDivisorEpsilon);
+  const float phongNormalizationPdf = (m.refl[3] + 1.0f) / gg::kPI_2;
+  gg::V3 rrd = gg::normalize(inc - (2.0f * cosTheta) * N);
+  const float reflPow = gg::powf_(gg::saturate(gg::dot(rrd, exc)), m.refl[3]);
+  float brdfFactor = phongFactorBrdf * reflPow;
+  float pdfFactor = phongNormalizationPdf * reflPow;
+  bsdf = preflect * brdfFactor;
+  bsdf = bsdf * (fabsf(cosThetaOut) / cosThetaAbs);
+  pdf = avg_probability(preflect) * pdfFactor;
+  if (!specularOnly && cosTheta * cosThetaOut < 0.0f) {
+    bsdf = bsdf + pdiffuse * m.diffuse / gg::kPI;
+    pdf = pdf + avg3(pdiffuse) * gg::saturate(cosThetaOut) / gg::kPI;
+  }
+  const float eta = cosTheta < 0.0f ? 1.0f / m.refractionIndexAvg : m.refractionIndexAvg;
+  const float etaSq = eta * eta;
+  const float sinTheta2Sq = etaSq * (1.0f - cosTheta * cosTheta);
+  if (sinTheta2Sq < 1.0f) {
+    rrd = gg::normalize(eta * inc + (signf_(cosTheta) * (sqrtf(gg::saturate(1.0f - sinTheta2Sq)) - eta * cosThetaAbs)) * N);
+    const float refrPow = gg::powf_(gg::saturate(gg::dot(rrd, exc)), m.refl[3]);
+    brdfFactor = phongFactorBrdf * refrPow;
+    pdfFactor = phongNormalizationPdf * refrPow;
+    if (!adjoint) brdfFactor = brdfFactor / etaSq;
+    brdfFactor = brdfFactor * (fabsf(gg::dot(N, exc)) / fabsf(gg::dot(N, rrd)));
+  }
+  bsdf = bsdf + prefract * brdfFactor;
+  pdf = pdf + avg_probability(prefract) * pdfFactor;
+  return bsdf;
+}
+
+__device__ __forceinline__ gg::V3 bsdf_eye(gg::V3 inc, gg::V3 exc, const MaterialData& m, gg::V3 N, float& pdf) {
+  return eval_bsdf(inc, exc, m, N, pdf, false, false);
+}
+// AdjointBSDF (materials.glsl:405-408).
+__device__ __forceinline__ gg::V3 bsdf_adjoint(gg::V3 inc, gg::V3 exc, const MaterialData& m, gg::V3 N, float& pdf) {
+  return eval_bsdf(-exc, -inc, m, N, pdf, true, false);
+}
+
+// AdjointBSDFShadingNormalCorrectedOutDotN (materials.glsl:415-426).
+__device__ __forceinline__ float adjoint_corrected_out_dot_n(gg::V3 incomingLight, gg::V3 toCamera, gg::V3 gN,
+                                                             gg::V3 sN) {
+  if (gg::dot(incomingLight, gN) * gg::dot(incomingLight, sN) < 0.0f ||
+      gg::dot(toCamera, gN) * gg::dot(toCamera, sN) < 0.0f)
+    return 0.0f;
+  return gg::saturate(fabsf(gg::dot(incomingLight, sN) * gg::dot(toCamera, gN)) /
+                      fmaxf(fabsf(gg::dot(incomingLight, gN)), gg::kDivisorEpsilon));
+}
+// AdjointBSDFShadingNormalCorrection (materials.glsl:427-436).
+__device__ __forceinline__ float adjoint_correction(gg::V3 inDir, gg::V3 outDir, gg::V3 gN, gg::V3 sN) {
+  if (gg::dot(inDir, gN) * gg::dot(inDir, sN) < 0.0f || gg::dot(outDir, gN) * gg::dot(outDir, sN) < 0.0f)
+    return 0.0f;
+  return gg::saturate(fabsf(gg::dot(inDir, sN) * gg::dot(outDir, gN)) /
+                      fmaxf(fabsf(gg::dot(inDir, gN) * gg::dot(outDir, sN)), gg::kDivisorEpsilon));
+}
+
+// GetTriangleHitInfo (scenedata.glsl:94-105): interpolated shading normal.
+__device__ __forceinline__ gg::V3 shading_normal(const float4* __restrict__ vinfo, int4 tri, gg::V3 b) {
+  const float4 a = vinfo[tri.x], c = vinfo[tri.y], e = vinfo[tri.z];
+  return gg::normalize(gg::unpack_normal(a.x, a.y) * b.x + gg::unpack_normal(c.x, c.y) * b.y +
+                       gg::unpack_normal(e.x, e.y) * b.z);
+}
+__device__ __forceinline__ float2 hit_texcoord(const float4* __restrict__ vinfo, int4 tri, gg::V3 b) {
+  const float4 a = vinfo[tri.x], c = vinfo[tri.y], e = vinfo[tri.z];
+  return make_float2(a.z * b.x + c.z * b.y + e.z * b.z, a.w * b.x + c.w * b.y + e.w * b.z);
+}
+
+}  // namespace gpugi

@@ -1,0 +1,126 @@
+// Escape-pointer walk over the reference BVH2 layout (device code).
+//
+// This is the bit-exact traversal: same node order, same FP32 operations in
+// the same order as traceray.glsl:41-175 with intersectiontests.glsl:9-20,
+// 63-79, 100-111, so every ray gets the reference's hit slot, t and
+// barycentrics including exact ties (first visited wins) and box grazes.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "../common/gshade.h"
+
+namespace gpugi {
+
+struct RefScene {
+  const float4* __restrict__ nodes;     // 2 texels per node (HierachyBuffer, TBO 4)
+  const int4* __restrict__ slots;       // TriangleBuffer, TBO 1
+  const float* __restrict__ positions;  // VertexPositionBuffer RGB32F, TBO 2
+  const float4* __restrict__ vinfo;     // VertexInfoBuffer, TBO 3
+};
+
+struct RefHit {
+  float t;           // rayLength
+  int slot;          // _hitIndex.y
+  int node;          // _hitIndex.x
+  gg::V3 bary;       // barycentricCoord
+  gg::V3 triNormal;  // TRINORMAL_OUTPUT (not normalized)
+  int4 tri;          // outTriangle
+};
+
+__device__ __forceinline__ gg::V3 load_pos(const float* __restrict__ p, int v) {
+  return gg::v3(p[3 * v + 0], p[3 * v + 1], p[3 * v + 2]);
+}
+
+// IntersectBox (intersectiontests.glsl:9-20).
+__device__ __forceinline__ bool intersect_box(gg::V3 o, gg::V3 inv, gg::V3 bmin, gg::V3 bmax,
+                                              float& firstHit, float& lastHit) {
+  gg::V3 tbot = inv * (bmin - o);
+  gg::V3 ttop = inv * (bmax - o);
+  gg::V3 tmin = gg::vmin(ttop, tbot);
+  gg::V3 tmax = gg::vmax(ttop, tbot);
+  float tx = fmaxf(tmin.x, tmin.y), ty = fmaxf(tmin.x, tmin.z);
+  firstHit = fmaxf(0.0f, fmaxf(tx, ty));
+  tx = fminf(tmax.x, tmax.y);
+  ty = fminf(tmax.x, tmax.z);
+  lastHit = fminf(tx, ty);
+  return firstHit <= lastHit;
+}
+
+// IntersectTriangle (intersectiontests.glsl:63-79), the OptiX-style branchless
+// Moeller-Trumbore variant: not watertight, t > 1e-4, all barycentrics >= 0.
+__device__ __forceinline__ bool intersect_triangle(gg::V3 o, gg::V3 d, gg::V3 p0, gg::V3 p1, gg::V3 p2,
+                                                   float& hit, gg::V3& bary, gg::V3& n) {
+  gg::V3 e0 = p1 - p0;
+  gg::V3 e1 = p0 - p2;
+  n = gg::cross(e1, e0);
+  gg::V3 e2 = (1.0f / gg::dot(n, d)) * (p0 - o);
+  gg::V3 i = gg::cross(d, e2);
+  bary.y = gg::dot(i, e1);
+  bary.z = gg::dot(i, e0);
+  bary.x = 1.0f - (bary.z + bary.y);
+  hit = gg::dot(n, e2);
+  return (hit > gg::kIntersectEpsilon) && bary.x >= 0.0f && bary.y >= 0.0f && bary.z >= 0.0f;
+}
+
+// TraceRay / TraceRayAnyHit (traceray.glsl:18-180) for TRIANGLES_PER_LEAF == 2.
+// ANY: returns at the first accepted triangle.  COUNT: box/triangle visit
+// counters exactly as TRACERAY_DEBUG_VARS counts them.
+template <bool ANY, bool COUNT>
+__device__ __forceinline__ bool trace_ref(const RefScene& s, gg::V3 o, gg::V3 d, float& rayLength,
+                                          RefHit& out, uint32_t& nbox, uint32_t& ntri) {
+  int currentNodeIndex = 0;
+  int currentLeafIndex = 0;
+  int lastNodeIndex = 0;
+  if (!ANY) { out.slot = -1; out.node = -1; }
+  gg::V3 inv = gg::v3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+  bool nextIsLeafNode = false;
+  do {
+    if (!nextIsLeafNode) {
+      if (COUNT) ++nbox;
+      float4 a = s.nodes[2 * currentNodeIndex];
+      float4 b = s.nodes[2 * currentNodeIndex + 1];
+      uint32_t childCode = __float_as_uint(a.w);
+      int escape = __float_as_int(b.w);
+      float newHit, exitDist;
+      if (intersect_box(o, inv, gg::v3(a.x, a.y, a.z), gg::v3(b.x, b.y, b.z), newHit, exitDist) &&
+          newHit <= rayLength) {
+        lastNodeIndex = currentNodeIndex;
+        currentNodeIndex = (int)(childCode & 0x7fffffffu);
+        nextIsLeafNode = (uint32_t)currentNodeIndex != childCode;
+        if (nextIsLeafNode) {
+          currentLeafIndex = 2 * currentNodeIndex;
+          currentNodeIndex = escape;
+        }
+      } else {
+        currentNodeIndex = escape;
+      }
+    }
+    if (nextIsLeafNode) {
+      int4 tri = s.slots[currentLeafIndex];
+      if (tri.x == tri.y) {
+        nextIsLeafNode = false;
+      } else {
+        if (COUNT) ++ntri;
+        gg::V3 p0 = load_pos(s.positions, tri.x);
+        gg::V3 p1 = load_pos(s.positions, tri.y);
+        gg::V3 p2 = load_pos(s.positions, tri.z);
+        float newHit;
+        gg::V3 bary, n;
+        if (intersect_triangle(o, d, p0, p1, p2, newHit, bary, n) && newHit < rayLength) {
+          if (ANY) return true;
+          rayLength = newHit;
+          out.tri = tri;
+          out.bary = bary;
+          out.triNormal = n;
+          out.node = lastNodeIndex;
+          out.slot = currentLeafIndex;
+        }
+        ++currentLeafIndex;
+        nextIsLeafNode = (currentLeafIndex % 2) != 0;
+      }
+    }
+  } while (currentNodeIndex != 0 || nextIsLeafNode);
+  return false;
+}
+
+}  // namespace gpugi

@@ -1,0 +1,103 @@
+// Host-side scene: the content of a .rawscene file plus its material json, the
+// SAH sweep builder that produces it, and the reference GPU layout derived
+// from it.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "gpugi_rt.h"
+
+namespace gpugi {
+
+// FileDecl::Vertex (bvhmake/filedef.hpp:19-24): 32 bytes.
+struct RawVertex {
+  float position[3];
+  float normal[3];
+  float texcoord[2];
+};
+// FileDecl::Triangle (bvhmake/filedef.hpp:27-39): 16 bytes; invalid slot = all zero.
+struct RawTriangle {
+  uint32_t vertices[3];
+  uint32_t material;
+};
+// FileDecl::Node (bvhmake/filedef.hpp:64-69): 12 bytes.
+struct RawNode {
+  uint32_t parent;
+  uint32_t firstChild;  // MSB set: the remaining bits index a leaf block
+  uint32_t escape;      // next node when this one is done; 0 ends the walk
+};
+// ε::Box as stored in the "bounding_aabox" chunk: 24 bytes.
+struct RawBox {
+  float min[3];
+  float max[3];
+};
+struct PointLight {
+  float position[3];
+  float intensity[3];
+};
+struct LightTriangle {  // Scene::LightTriangle (scene.hpp:95-100)
+  float v0[3], v1[3], v2[3];
+  float luminance[3];
+};
+
+inline bool IsValidSlot(const RawTriangle& t) { return t.vertices[0] != t.vertices[1]; }
+
+class HostScene {
+ public:
+  std::vector<RawVertex> vertices;
+  uint32_t trianglesPerLeaf = 2;
+  std::vector<RawTriangle> leafSlots;  // numLeaves * trianglesPerLeaf
+  std::vector<RawNode> nodes;
+  std::vector<RawBox> boxes;
+  std::vector<gpugi_material_params> materials;
+  std::vector<PointLight> pointLights;
+
+  // Derived data, rebuilt by Finalize().
+  std::vector<float> gpuHierarchy;      // 8 floats per node
+  std::vector<int32_t> gpuTriangles;    // 4 ints per slot
+  std::vector<float> gpuPositions;      // 3 floats per vertex
+  std::vector<float> gpuVertexInfo;     // 4 floats per vertex
+  std::vector<gpugi_material> gpuMaterials;
+  std::vector<LightTriangle> lightTriangles;
+  std::vector<float> lightSummedArea;   // normalized to [0,1]
+  std::vector<float> pointLightSummedFlux;
+  float lightAreaSum = 0.0f;
+  float totalAreaLightFlux = 0.0f;
+  float totalPointLightFlux = 0.0f;
+  uint32_t numValidTriangles = 0;
+  uint32_t maxDepth = 0;
+  float bboxMin[3] = {0, 0, 0}, bboxMax[3] = {0, 0, 0};
+
+  uint32_t NumLeaves() const { return trianglesPerLeaf ? uint32_t(leafSlots.size() / trianglesPerLeaf) : 0; }
+  // Validates references and derives the GPU layout, material values and light tables.
+  // Returns an empty string on success, an error message otherwise.
+  std::string Finalize();
+  void ComputePointLightTable();
+  void Describe(gpugi_scene_desc* out) const;
+};
+
+// Scene::LoadMaterial (scene.cpp:132-187) for a constant-valued material.
+gpugi_material ResolveMaterial(const gpugi_material_params& p);
+// Default parameters of Scene::LoadMaterial when a json key is absent.
+gpugi_material_params DefaultMaterialParams(const char* name);
+// UNORM8 round trip of an RGB8 texel (the `_hdr == false` textures of scene.cpp:213).
+float QuantizeUnorm8(float x);
+
+// SAH sweep build (bvhmake/buildmethods/sweep.cpp:14-274).
+std::string BuildSweepBVH(const std::vector<RawVertex>& vertices,
+                          const std::vector<RawTriangle>& triangles, uint32_t trianglesPerLeaf,
+                          HostScene* out);
+
+// .rawscene + material json I/O.
+std::string LoadRawScene(const std::string& path, const std::string& jsonPath, HostScene* out, int* status);
+std::string SaveRawScene(const HostScene& scene, const std::string& path, const std::string& jsonPath);
+
+// Generators.
+void MakeCornellMesh(std::vector<RawVertex>* v, std::vector<RawTriangle>* t,
+                     std::vector<gpugi_material_params>* m);
+void MakeHeightfieldMesh(uint32_t cells, uint32_t seed, std::vector<RawVertex>* v,
+                         std::vector<RawTriangle>* t, std::vector<gpugi_material_params>* m);
+
+}  // namespace gpugi

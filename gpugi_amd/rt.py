@@ -1,0 +1,417 @@
+"""ctypes binding of the C ABI in include/gpugi_rt.h.
+
+This is glue for tests and the benchmark driver; the renderer itself is the
+native library.  Loading fails loudly when libgpugi_rt.so has not been built:
+there is no Python or CPU fallback for any entry point.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libgpugi_rt.so")
+
+GPUGI_OK = 0
+CLOSEST_HIT, ANY_HIT = 0, 1
+KERNEL_AUTO, KERNEL_REFERENCE, KERNEL_WIDE = 0, 1, 2
+PT, LT, BPT = 0, 1, 2
+RAY_MAX = np.float32(3.402823466e38)
+
+RAY_DTYPE = np.dtype([("origin", "<f4", 3), ("tmax", "<f4"), ("dir", "<f4", 3), ("pad", "<u4")])
+HIT_DTYPE = np.dtype([("slot", "<i4"), ("t", "<f4"), ("bary_y", "<f4"), ("bary_z", "<f4")])
+LIGHT_DTYPE = np.dtype([("position", "<f4", 3), ("normal_phi", "<f4"), ("intensity", "<f4", 3),
+                        ("normal_theta_cos", "<f4")])
+MATERIAL_DTYPE = np.dtype([("diffuse", "<f4", 3), ("opacity", "<f4", 3), ("reflectiveness", "<f4", 4),
+                           ("emissivity", "<f4", 3), ("fresnel0", "<f4", 3), ("refraction_index_avg", "<f4"),
+                           ("fresnel1", "<f4", 3)])
+
+
+class SceneDesc(ctypes.Structure):
+    _fields_ = [("hierarchy", ctypes.c_void_p), ("num_nodes", ctypes.c_uint32),
+                ("triangles", ctypes.c_void_p), ("num_leaves", ctypes.c_uint32),
+                ("triangles_per_leaf", ctypes.c_uint32), ("positions", ctypes.c_void_p),
+                ("vertex_info", ctypes.c_void_p), ("num_vertices", ctypes.c_uint32),
+                ("materials", ctypes.c_void_p), ("num_materials", ctypes.c_uint32)]
+
+
+class Camera(ctypes.Structure):
+    _fields_ = [("position", ctypes.c_float * 3), ("u", ctypes.c_float * 3), ("v", ctypes.c_float * 3),
+                ("w", ctypes.c_float * 3), ("pixel_area", ctypes.c_float)]
+
+    def as_dict(self):
+        return {k: list(getattr(self, k)) if k != "pixel_area" else self.pixel_area
+                for k, _ in self._fields_}
+
+
+class MaterialParams(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char * 32), ("albedo", ctypes.c_float * 3), ("specular_color", ctypes.c_float * 3),
+                ("roughness", ctypes.c_float), ("opacity", ctypes.c_float), ("emissivity", ctypes.c_float * 3),
+                ("refraction_index", ctypes.c_float)]
+
+
+class GpugiError(RuntimeError):
+    def __init__(self, func, code, msg):
+        super().__init__("%s failed with status %d: %s" % (func, code, msg))
+        self.code = code
+
+
+# Exported symbols with their argument types (restype int unless stated).
+_P = ctypes.c_void_p
+_SIGNATURES = {
+    "gpugi_create": [ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)],
+    "gpugi_destroy": ([_P], None),
+    "gpugi_last_error": ([_P], ctypes.c_char_p),
+    "gpugi_load_scene": [_P, ctypes.POINTER(SceneDesc)],
+    "gpugi_set_camera": [_P, ctypes.POINTER(Camera)],
+    "gpugi_set_screen": [_P, ctypes.c_int, ctypes.c_int],
+    "gpugi_set_light_samples": [_P, _P, ctypes.c_uint32],
+    "gpugi_set_max_path_length": [_P, ctypes.c_int],
+    "gpugi_set_integrator_kernel": [_P, ctypes.c_int],
+    "gpugi_draw": [_P, ctypes.c_int, ctypes.c_uint32],
+    "gpugi_reset_accum": [_P],
+    "gpugi_readback": [_P, _P, ctypes.c_size_t, ctypes.POINTER(ctypes.c_uint32)],
+    "gpugi_trace": [_P, ctypes.c_int, ctypes.c_int, _P, _P, ctypes.c_size_t, ctypes.c_int],
+    "gpugi_trace_counts": [_P, ctypes.c_int, _P, _P, _P, ctypes.c_size_t, ctypes.c_int],
+    "gpugi_sync": [_P],
+    "gpugi_alloc": [_P, ctypes.c_size_t, ctypes.POINTER(ctypes.c_void_p)],
+    "gpugi_free": [_P, _P],
+    "gpugi_upload": [_P, _P, _P, ctypes.c_size_t],
+    "gpugi_download": [_P, _P, _P, ctypes.c_size_t],
+    "gpugi_timer_begin": [_P],
+    "gpugi_timer_end": [_P, ctypes.POINTER(ctypes.c_float)],
+    "gpugi_raygen_primary": [_P, ctypes.POINTER(Camera), ctypes.c_int, ctypes.c_int, ctypes.c_uint32, _P,
+                             ctypes.c_size_t],
+    "gpugi_raygen_diffuse": [_P, _P, _P, ctypes.c_size_t, ctypes.c_uint32, _P, ctypes.c_size_t],
+    "gpugi_raygen_shadow": [_P, _P, _P, ctypes.c_size_t, _P, ctypes.c_uint32, _P, ctypes.c_size_t],
+    "gpugi_hostscene_build": [_P, _P, _P, ctypes.c_uint32, _P, _P, ctypes.c_uint32, _P, ctypes.c_uint32,
+                              ctypes.c_uint32, ctypes.POINTER(ctypes.c_void_p)],
+    "gpugi_hostscene_cornell": [ctypes.POINTER(ctypes.c_void_p)],
+    "gpugi_hostscene_heightfield": [ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(ctypes.c_void_p)],
+    "gpugi_hostscene_load": [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(ctypes.c_void_p)],
+    "gpugi_hostscene_save": [_P, ctypes.c_char_p, ctypes.c_char_p],
+    "gpugi_hostscene_desc": [_P, ctypes.POINTER(SceneDesc)],
+    "gpugi_hostscene_info": [_P, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32),
+                             ctypes.POINTER(ctypes.c_float * 3), ctypes.POINTER(ctypes.c_float * 3)],
+    "gpugi_hostscene_add_point_light": [_P, ctypes.POINTER(ctypes.c_float * 3), ctypes.POINTER(ctypes.c_float * 3)],
+    "gpugi_hostscene_destroy": ([_P], None),
+    "gpugi_camera_compute": [ctypes.POINTER(ctypes.c_float * 3), ctypes.POINTER(ctypes.c_float * 3),
+                             ctypes.POINTER(ctypes.c_float * 3), ctypes.c_float, ctypes.c_int, ctypes.c_int,
+                             ctypes.POINTER(Camera)],
+    "gpugi_lightsampler_create": [_P, ctypes.c_uint64, ctypes.POINTER(ctypes.c_void_p)],
+    "gpugi_lightsampler_generate": [_P, _P, ctypes.c_uint32, ctypes.c_float],
+    "gpugi_lightsampler_destroy": ([_P], None),
+    "gpugi_wanghash": ([ctypes.c_uint32], ctypes.c_uint32),
+}
+
+_lib = None
+
+
+def lib():
+    """Loads libgpugi_rt.so (built by __graft_entry__.build()); raises if absent."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libgpugi_rt.so is not built (%s); run __graft_entry__.build()" % LIB_PATH)
+        l = ctypes.CDLL(LIB_PATH)
+        for name, sig in _SIGNATURES.items():
+            f = getattr(l, name)
+            if isinstance(sig, tuple):
+                f.argtypes, f.restype = sig
+            else:
+                f.argtypes, f.restype = sig, ctypes.c_int
+        _lib = l
+    return _lib
+
+
+def _check(func, code, ctx=None):
+    if code != GPUGI_OK:
+        msg = lib().gpugi_last_error(ctx)
+        raise GpugiError(func, code, msg.decode() if msg else "")
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None else None
+
+
+def _v3(x):
+    return (ctypes.c_float * 3)(*[float(v) for v in x])
+
+
+def wanghash(x):
+    return int(lib().gpugi_wanghash(ctypes.c_uint32(x & 0xffffffff)))
+
+
+def camera(position, lookat, up, vfov_deg, width, height):
+    cam = Camera()
+    _check("gpugi_camera_compute", lib().gpugi_camera_compute(_v3(position), _v3(lookat), _v3(up),
+                                                              float(vfov_deg), int(width), int(height),
+                                                              ctypes.byref(cam)))
+    return cam
+
+
+class HostScene:
+    """A host-side scene (.rawscene content + materials + derived GPU layout)."""
+
+    def __init__(self, handle):
+        self._h = ctypes.c_void_p(handle)
+
+    @classmethod
+    def cornell(cls):
+        h = ctypes.c_void_p()
+        _check("gpugi_hostscene_cornell", lib().gpugi_hostscene_cornell(ctypes.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def heightfield(cls, cells, seed):
+        h = ctypes.c_void_p()
+        _check("gpugi_hostscene_heightfield", lib().gpugi_hostscene_heightfield(cells, seed, ctypes.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def load(cls, path, json_path=None):
+        h = ctypes.c_void_p()
+        _check("gpugi_hostscene_load", lib().gpugi_hostscene_load(
+            path.encode(), json_path.encode() if json_path else None, ctypes.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def build(cls, positions, indices, normals=None, texcoords=None, material_ids=None, materials=None,
+              triangles_per_leaf=2):
+        positions = np.ascontiguousarray(positions, dtype=np.float32).reshape(-1, 3)
+        indices = np.ascontiguousarray(indices, dtype=np.uint32).reshape(-1, 3)
+        normals = None if normals is None else np.ascontiguousarray(normals, dtype=np.float32).reshape(-1, 3)
+        texcoords = None if texcoords is None else np.ascontiguousarray(texcoords, dtype=np.float32).reshape(-1, 2)
+        mids = None if material_ids is None else np.ascontiguousarray(material_ids, dtype=np.uint32)
+        mats = materials or [default_material("Default")]
+        marr = (MaterialParams * len(mats))(*mats)
+        h = ctypes.c_void_p()
+        _check("gpugi_hostscene_build", lib().gpugi_hostscene_build(
+            _ptr(positions), _ptr(normals), _ptr(texcoords), len(positions), _ptr(indices), _ptr(mids),
+            len(indices), ctypes.cast(marr, ctypes.c_void_p), len(mats), triangles_per_leaf, ctypes.byref(h)))
+        return cls(h.value)
+
+    def save(self, path, json_path=None):
+        _check("gpugi_hostscene_save", lib().gpugi_hostscene_save(
+            self._h, path.encode(), json_path.encode() if json_path else None))
+
+    def add_point_light(self, position, intensity):
+        _check("gpugi_hostscene_add_point_light",
+               lib().gpugi_hostscene_add_point_light(self._h, _v3(position), _v3(intensity)))
+
+    def desc(self):
+        d = SceneDesc()
+        _check("gpugi_hostscene_desc", lib().gpugi_hostscene_desc(self._h, ctypes.byref(d)))
+        return d
+
+    def arrays(self):
+        """Copies of the reference-layout arrays as numpy arrays."""
+        d = self.desc()
+
+        def arr(ptr, count, dtype):
+            if count == 0:
+                return np.zeros(0, dtype=dtype)
+            buf = (ctypes.c_char * (count * np.dtype(dtype).itemsize)).from_address(ptr)
+            return np.frombuffer(buf, dtype=dtype).copy()
+
+        return {
+            "hierarchy": arr(d.hierarchy, d.num_nodes * 8, np.float32).reshape(-1, 8),
+            "triangles": arr(d.triangles, d.num_leaves * d.triangles_per_leaf * 4, np.int32).reshape(-1, 4),
+            "positions": arr(d.positions, d.num_vertices * 3, np.float32).reshape(-1, 3),
+            "vertex_info": arr(d.vertex_info, d.num_vertices * 4, np.float32).reshape(-1, 4),
+            "materials": arr(d.materials, d.num_materials, MATERIAL_DTYPE),
+            "triangles_per_leaf": d.triangles_per_leaf,
+        }
+
+    def info(self):
+        nt, dep = ctypes.c_uint32(), ctypes.c_uint32()
+        lo, hi = (ctypes.c_float * 3)(), (ctypes.c_float * 3)()
+        _check("gpugi_hostscene_info", lib().gpugi_hostscene_info(self._h, ctypes.byref(nt), ctypes.byref(dep),
+                                                                  ctypes.byref(lo), ctypes.byref(hi)))
+        return {"num_triangles": nt.value, "max_depth": dep.value, "bbox_min": list(lo), "bbox_max": list(hi)}
+
+    def close(self):
+        if self._h:
+            lib().gpugi_hostscene_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def default_material(name, albedo=(0.5, 0.5, 0.5), specular=(0.03, 0.03, 0.03), roughness=1.0, opacity=1.0,
+                     emissivity=(0.0, 0.0, 0.0), refraction_index=1.5):
+    m = MaterialParams()
+    m.name = name.encode()[:31]
+    m.albedo[:] = albedo
+    m.specular_color[:] = specular
+    m.roughness = roughness
+    m.opacity = opacity
+    m.emissivity[:] = emissivity
+    m.refraction_index = refraction_index
+    return m
+
+
+class LightSampler:
+    def __init__(self, scene, seed=123):
+        self._scene = scene  # keep alive
+        h = ctypes.c_void_p()
+        _check("gpugi_lightsampler_create", lib().gpugi_lightsampler_create(scene._h, seed, ctypes.byref(h)))
+        self._h = h
+
+    def generate(self, n, position_bias=1.0e-4):
+        out = np.zeros(n, dtype=LIGHT_DTYPE)
+        _check("gpugi_lightsampler_generate", lib().gpugi_lightsampler_generate(self._h, _ptr(out), n,
+                                                                                 float(position_bias)))
+        return out
+
+    def __del__(self):
+        try:
+            if self._h:
+                lib().gpugi_lightsampler_destroy(self._h)
+        except Exception:
+            pass
+
+
+class DeviceBuffer:
+    def __init__(self, ctx, nbytes):
+        self.ctx = ctx
+        self.nbytes = nbytes
+        p = ctypes.c_void_p()
+        ctx._call("gpugi_alloc", ctypes.c_size_t(nbytes), ctypes.byref(p))
+        self.ptr = p.value
+
+    def upload(self, arr):
+        arr = np.ascontiguousarray(arr)
+        assert arr.nbytes <= self.nbytes
+        self.ctx._call("gpugi_upload", ctypes.c_void_p(self.ptr), _ptr(arr), ctypes.c_size_t(arr.nbytes))
+
+    def download(self, dtype, count):
+        out = np.empty(count, dtype=dtype)
+        assert out.nbytes <= self.nbytes
+        self.ctx._call("gpugi_download", _ptr(out), ctypes.c_void_p(self.ptr), ctypes.c_size_t(out.nbytes))
+        return out
+
+    def free(self):
+        if self.ptr:
+            self.ctx._call("gpugi_free", ctypes.c_void_p(self.ptr))
+            self.ptr = None
+
+
+class Context:
+    """One device context (gpugi_ctx) on a gfx950 device."""
+
+    def __init__(self, device=0):
+        h = ctypes.c_void_p()
+        _check("gpugi_create", lib().gpugi_create(device, ctypes.byref(h)))
+        self._h = h
+        self._scene = None
+
+    def _call(self, name, *args):
+        _check(name, getattr(lib(), name)(self._h, *args), self._h)
+
+    def load_scene(self, scene):
+        d = scene.desc()
+        self._call("gpugi_load_scene", ctypes.byref(d))
+        self._scene = scene
+
+    def set_camera(self, cam):
+        self._call("gpugi_set_camera", ctypes.byref(cam))
+
+    def set_screen(self, w, h):
+        self._call("gpugi_set_screen", int(w), int(h))
+
+    def set_light_samples(self, samples):
+        samples = np.ascontiguousarray(samples, dtype=LIGHT_DTYPE)
+        self._call("gpugi_set_light_samples", _ptr(samples), len(samples))
+
+    def set_max_path_length(self, n):
+        self._call("gpugi_set_max_path_length", int(n))
+
+    def set_integrator_kernel(self, k):
+        self._call("gpugi_set_integrator_kernel", int(k))
+
+    def draw(self, integrator, frame_seed):
+        self._call("gpugi_draw", int(integrator), ctypes.c_uint32(frame_seed & 0xffffffff))
+
+    def reset_accum(self):
+        self._call("gpugi_reset_accum")
+
+    def readback(self, w, h):
+        out = np.zeros((h, w, 4), dtype=np.float32)
+        it = ctypes.c_uint32()
+        self._call("gpugi_readback", _ptr(out), ctypes.c_size_t(out.nbytes), ctypes.byref(it))
+        return out, it.value
+
+    def trace(self, rays, query=CLOSEST_HIT, kernel=KERNEL_AUTO):
+        rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
+        hits = np.zeros(len(rays), dtype=HIT_DTYPE)
+        self._call("gpugi_trace", int(query), int(kernel), _ptr(rays), _ptr(hits), ctypes.c_size_t(len(rays)), 0)
+        return hits
+
+    def trace_device(self, rays_buf, hits_buf, n, query=CLOSEST_HIT, kernel=KERNEL_AUTO):
+        self._call("gpugi_trace", int(query), int(kernel), ctypes.c_void_p(rays_buf.ptr),
+                   ctypes.c_void_p(hits_buf.ptr), ctypes.c_size_t(n), 1)
+
+    def trace_counts(self, rays, query=CLOSEST_HIT):
+        rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
+        nb = np.zeros(len(rays), dtype=np.uint32)
+        nt = np.zeros(len(rays), dtype=np.uint32)
+        self._call("gpugi_trace_counts", int(query), _ptr(rays), _ptr(nb), _ptr(nt), ctypes.c_size_t(len(rays)), 0)
+        return nb, nt
+
+    def trace_counts_device(self, rays_buf, nbox_buf, ntri_buf, n, query=CLOSEST_HIT):
+        self._call("gpugi_trace_counts", int(query), ctypes.c_void_p(rays_buf.ptr), ctypes.c_void_p(nbox_buf.ptr),
+                   ctypes.c_void_p(ntri_buf.ptr), ctypes.c_size_t(n), 1)
+
+    def alloc(self, nbytes):
+        return DeviceBuffer(self, nbytes)
+
+    def sync(self):
+        self._call("gpugi_sync")
+
+    def timer_begin(self):
+        self._call("gpugi_timer_begin")
+
+    def timer_end(self):
+        ms = ctypes.c_float()
+        self._call("gpugi_timer_end", ctypes.byref(ms))
+        return ms.value
+
+    def raygen_primary(self, cam, w, h, seed, rays_buf, n):
+        self._call("gpugi_raygen_primary", ctypes.byref(cam), int(w), int(h), ctypes.c_uint32(seed),
+                   ctypes.c_void_p(rays_buf.ptr), ctypes.c_size_t(n))
+
+    def raygen_diffuse(self, prim_buf, hits_buf, n_prim, seed, rays_buf, n):
+        self._call("gpugi_raygen_diffuse", ctypes.c_void_p(prim_buf.ptr), ctypes.c_void_p(hits_buf.ptr),
+                   ctypes.c_size_t(n_prim), ctypes.c_uint32(seed), ctypes.c_void_p(rays_buf.ptr), ctypes.c_size_t(n))
+
+    def raygen_shadow(self, prim_buf, hits_buf, n_prim, lights_xyz, rays_buf, n):
+        lights = np.ascontiguousarray(lights_xyz, dtype=np.float32).reshape(-1, 3)
+        self._call("gpugi_raygen_shadow", ctypes.c_void_p(prim_buf.ptr), ctypes.c_void_p(hits_buf.ptr),
+                   ctypes.c_size_t(n_prim), _ptr(lights), len(lights), ctypes.c_void_p(rays_buf.ptr),
+                   ctypes.c_size_t(n))
+
+    def close(self):
+        if self._h:
+            lib().gpugi_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def exported_symbols():
+    return sorted(_SIGNATURES)

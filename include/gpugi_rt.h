@@ -1,0 +1,287 @@
+/*
+ * gpugi_rt.h -- C ABI of the MI355X (gfx950) ray-tracing hot path.
+ *
+ * This is the drop-in boundary for the GLSL compute path of gpugi.  Every entry
+ * point names the reference interface it replaces (paths relative to the
+ * gpugi source tree).  Conventions:
+ *   - every function returns an int status (GPUGI_OK == 0, negative on error);
+ *     no C++ exception crosses this boundary, and gpugi_last_error() returns a
+ *     message for the last failing call on a context (thread-local for calls
+ *     without a context);
+ *   - a context owns one HIP stream and all device memory it allocates; it is
+ *     not thread-safe (one host thread per context), like the single GL context
+ *     of the reference (application.cpp:287-301);
+ *   - host inputs are copied during the call; calls are asynchronous on the
+ *     context stream except readback, trace with host pointers, timers and
+ *     gpugi_sync.
+ */
+#ifndef GPUGI_RT_H
+#define GPUGI_RT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GPUGI_ABI_VERSION 1
+
+enum gpugi_status {
+  GPUGI_OK = 0,
+  GPUGI_ERR_ARG = -1,         /* invalid argument (null pointer, bad size, bad enum) */
+  GPUGI_ERR_HIP = -2,         /* a HIP runtime call failed */
+  GPUGI_ERR_STATE = -3,       /* call order violated (e.g. trace before a scene is loaded) */
+  GPUGI_ERR_IO = -4,          /* file could not be opened/read/written */
+  GPUGI_ERR_FORMAT = -5,      /* malformed .rawscene / json content */
+  GPUGI_ERR_UNSUPPORTED = -6, /* valid request this build does not support */
+  GPUGI_ERR_NOMEM = -7,       /* host or device allocation failed */
+  GPUGI_ERR_NODEVICE = -8     /* no usable gfx950 device */
+};
+
+/* Which query a trace performs. */
+enum gpugi_query {
+  GPUGI_CLOSEST_HIT = 0,      /* TraceRay (traceray.glsl:25-38, HIT_INDEX_OUTPUT) */
+  GPUGI_ANY_HIT = 1           /* TraceRayAnyHit (traceray.glsl:19-23, ANY_HIT) */
+};
+
+/* Which traversal kernel a trace uses. */
+enum gpugi_kernel {
+  GPUGI_KERNEL_AUTO = 0,      /* fastest kernel available for the loaded scene */
+  GPUGI_KERNEL_REFERENCE = 1, /* escape-pointer walk on the reference BVH2 layout (bit-exact) */
+  GPUGI_KERNEL_WIDE = 2       /* ordered walk on the re-packed quantized 4-wide BVH */
+};
+
+/* Integrators a draw runs. */
+enum gpugi_integrator {
+  GPUGI_PT = 0,               /* Pathtracer::Draw (pathtracer.cpp:47-53) */
+  GPUGI_LT = 1,               /* LightPathtracer::Draw (lightpathtracer.cpp:45-51) */
+  GPUGI_BPT = 2               /* BidirectionalPathtracer::Draw (bidirectionalpathtracer.cpp:182-203) */
+};
+
+/* A ray query: 32 bytes.  tmax is the initial rayLength of TraceRay /
+ * TraceRayAnyHit (traceray.glsl:19-25); RAY_MAX = 3.402823466e+38
+ * (stdheader.glsl:6) means unbounded. */
+typedef struct gpugi_ray {
+  float origin[3];
+  float tmax;
+  float dir[3];
+  uint32_t pad;
+} gpugi_ray;
+
+/* A hit record: 16 bytes.  slot is the triangle-slot index of the closest hit
+ * (the _hitIndex.y of traceray.glsl:155, i.e. leaf * TRIANGLES_PER_LEAF +
+ * position in the leaf), -1 on a miss; t is the final rayLength (tmax on a
+ * miss); bary_y/bary_z are barycentricCoord.y/.z (intersectiontests.glsl:73-74).
+ * For GPUGI_ANY_HIT, slot is 1 when an occluder was found and 0 otherwise;
+ * t, bary_y and bary_z are left unspecified. */
+typedef struct gpugi_hit {
+  int32_t slot;
+  float t;
+  float bary_y;
+  float bary_z;
+} gpugi_hit;
+
+/* Resolved material: the values SampleMaterialData returns for a constant
+ * material (materials.glsl:19-30).  Diffuse and opacity are already quantized
+ * to UNORM8 the way Scene::LoadMaterial uploads them (scene.cpp:164,168). */
+typedef struct gpugi_material {
+  float diffuse[3];
+  float opacity[3];
+  float reflectiveness[4];    /* .w = Phong exponent */
+  float emissivity[3];
+  float fresnel0[3];
+  float refraction_index_avg;
+  float fresnel1[3];
+} gpugi_material;
+
+/* LightSampler::LightSample (lightsampler.hpp:17-23): 32 bytes. */
+typedef struct gpugi_light_sample {
+  float position[3];
+  float normal_phi;           /* atan2(n.y, n.x) */
+  float intensity[3];
+  float normal_theta_cos;     /* n.z; > 1.5 marks an omnidirectional point light */
+} gpugi_light_sample;
+
+/* Scene arrays in the reference GPU layout (renderersystem.cpp:138-147). */
+typedef struct gpugi_scene_desc {
+  const float* hierarchy;        /* num_nodes x 8 floats: TreeNode<Box> (scene.hpp:30-38):
+                                    {min.xyz, bits(firstChild)}, {max.xyz, bits(escape)} */
+  uint32_t num_nodes;
+  const int32_t* triangles;      /* num_leaves x triangles_per_leaf x {v0,v1,v2,material}
+                                    (scene.hpp:64-68); invalid slot has v0 == v1 */
+  uint32_t num_leaves;
+  uint32_t triangles_per_leaf;   /* must be 2 (scenedata.glsl:5) */
+  const float* positions;        /* num_vertices x 3 floats (RGB32F, scenedata.glsl:34) */
+  const float* vertex_info;      /* num_vertices x {atan2(ny,nx), nz, u, v} (scene.hpp:71-75) */
+  uint32_t num_vertices;
+  const gpugi_material* materials;
+  uint32_t num_materials;        /* at most 256 (globaluniforms.glsl:23-27) */
+} gpugi_scene_desc;
+
+/* Camera UBO (globaluniforms.glsl:7-15) without the raster-only ViewProjection. */
+typedef struct gpugi_camera {
+  float position[3];
+  float u[3];
+  float v[3];
+  float w[3];
+  float pixel_area;
+} gpugi_camera;
+
+/* ------------------------------------------------------------------------ */
+/* Device context                                                           */
+/* ------------------------------------------------------------------------ */
+typedef struct gpugi_ctx gpugi_ctx;
+
+/* Creates a context on HIP device `device` (RendererSystem::RendererSystem,
+ * renderersystem.cpp:23-48). */
+int gpugi_create(int device, gpugi_ctx** out);
+/* Releases all device memory of the context (RendererSystem::~RendererSystem). */
+void gpugi_destroy(gpugi_ctx* ctx);
+/* Message of the last failing call on ctx; ctx may be NULL for calls that
+ * have no context (host utilities, gpugi_create). */
+const char* gpugi_last_error(const gpugi_ctx* ctx);
+
+/* Uploads a scene and re-packs it for the wide kernel (Scene::Scene,
+ * UploadGeometry, UploadHierarchy, scene.cpp:15-130 + RendererSystem::SetScene,
+ * renderersystem.cpp:132-171).  Resets the accumulation buffer. */
+int gpugi_load_scene(gpugi_ctx* ctx, const gpugi_scene_desc* scene);
+/* Camera UBO update (RendererSystem::SetCamera, renderersystem.cpp:188-220). */
+int gpugi_set_camera(gpugi_ctx* ctx, const gpugi_camera* camera);
+/* Backbuffer size (RendererSystem::SetScreenSize, renderersystem.cpp:222-233).
+ * Allocates a zeroed RGBA32F accumulation buffer. */
+int gpugi_set_screen(gpugi_ctx* ctx, int width, int height);
+/* Initial light samples for the next draw (InitialLightSampleBuffer, TBO 5;
+ * RendererSystem::PerIterationBufferUpdate, renderersystem.cpp:95-116). */
+int gpugi_set_light_samples(gpugi_ctx* ctx, const gpugi_light_sample* samples, uint32_t n);
+/* MAX_PATHLENGTH define (pathtracer.cpp:60). */
+int gpugi_set_max_path_length(gpugi_ctx* ctx, int max_path_length);
+/* Traversal the integrators use: GPUGI_KERNEL_REFERENCE (bit-exact escape
+ * walk) or GPUGI_KERNEL_WIDE / GPUGI_KERNEL_AUTO (4-wide quantized BVH).
+ * Replaces the AABOX_BVH define of the shader compile (pathtracer.cpp:55-64). */
+int gpugi_set_integrator_kernel(gpugi_ctx* ctx, int kernel);
+/* One iteration of an integrator that adds into the accumulation buffer
+ * (Renderer::Draw, renderer.hpp:42; FrameSeed UBO, globaluniforms.glsl:17-21). */
+int gpugi_draw(gpugi_ctx* ctx, int integrator, uint32_t frame_seed);
+/* Clears the accumulation buffer (Texture2D::ClearToZero in SetRenderer,
+ * renderersystem.hpp:73-86). */
+int gpugi_reset_accum(gpugi_ctx* ctx);
+/* Reads the RGBA32F accumulation buffer (Texture2D::ReadImage in
+ * Application::SaveImage, application.cpp:394-414).  bytes must be
+ * >= width*height*16.  *iterations (optional) gets the number of draws since
+ * the last reset. */
+int gpugi_readback(gpugi_ctx* ctx, float* rgba, size_t bytes, uint32_t* iterations);
+
+/* Ray queries for parity and the traversal microbenchmark (the GLSL
+ * TraceRay/TraceRayAnyHit functions, traceray.glsl:18-39).  rays and hits
+ * are host pointers when rays_on_device == 0 (the call then blocks until the
+ * hits are copied back), device pointers otherwise (asynchronous). */
+int gpugi_trace(gpugi_ctx* ctx, int query, int kernel, const gpugi_ray* rays, gpugi_hit* hits,
+                size_t n, int rays_on_device);
+/* Reference escape-walk visit counters (TRACERAY_DEBUG_VARS,
+ * traceray.glsl:59-61,131-133): per ray, box tests and valid-slot triangle
+ * tests.  box_tests / tri_tests are device arrays of n uint32 each when
+ * rays_on_device != 0, host arrays otherwise. */
+int gpugi_trace_counts(gpugi_ctx* ctx, int query, const gpugi_ray* rays, uint32_t* box_tests,
+                       uint32_t* tri_tests, size_t n, int rays_on_device);
+/* Waits for all work queued on the context stream (glFinish). */
+int gpugi_sync(gpugi_ctx* ctx);
+
+/* Device memory owned by the context (freed by gpugi_destroy at the latest). */
+int gpugi_alloc(gpugi_ctx* ctx, size_t bytes, void** device_ptr);
+int gpugi_free(gpugi_ctx* ctx, void* device_ptr);
+int gpugi_upload(gpugi_ctx* ctx, void* device_dst, const void* host_src, size_t bytes);
+int gpugi_download(gpugi_ctx* ctx, void* host_dst, const void* device_src, size_t bytes);
+
+/* HIP-event timer on the context stream (replaces the glFinish + clock()
+ * bracket of RendererSystem::Draw, renderersystem.cpp:235-258). */
+int gpugi_timer_begin(gpugi_ctx* ctx);
+int gpugi_timer_end(gpugi_ctx* ctx, float* milliseconds);
+
+/* Ray generators of the traversal microbenchmark, all on the device.
+ * Primary: ray i -> pixel i mod (width*height), jitter Random2 of
+ * InitRandomSeed(seed, i), direction as pathtracer.comp:39-46.
+ * Diffuse: ray i starts at the (i mod n_primary)-th primary hit, skipping
+ * misses cyclically; origin = hit + 1e-4 * d, d cosine-distributed about the
+ * geometric normal facing the incoming ray (random.glsl:62-71), seed `seed`.
+ * Shadow: from the same hit points toward lights[i mod n_lights],
+ * origin + 1e-4 * d, tmax = dist - 2e-4 (lightsource.glsl:38-40). */
+int gpugi_raygen_primary(gpugi_ctx* ctx, const gpugi_camera* camera, int width, int height,
+                         uint32_t seed, gpugi_ray* rays_dev, size_t n);
+int gpugi_raygen_diffuse(gpugi_ctx* ctx, const gpugi_ray* primary_dev, const gpugi_hit* hits_dev,
+                         size_t n_primary, uint32_t seed, gpugi_ray* rays_dev, size_t n);
+int gpugi_raygen_shadow(gpugi_ctx* ctx, const gpugi_ray* primary_dev, const gpugi_hit* hits_dev,
+                        size_t n_primary, const float* lights_xyz, uint32_t n_lights,
+                        gpugi_ray* rays_dev, size_t n);
+
+/* ------------------------------------------------------------------------ */
+/* Host-side scene: .rawscene files, the SAH sweep builder, generators       */
+/* ------------------------------------------------------------------------ */
+typedef struct gpugi_hostscene gpugi_hostscene;
+
+/* Material parameters of the material json (Scene::LoadMaterial key names,
+ * scene.cpp:138-180): albedo, specularColor, roughness, opacity, emissivity,
+ * refractionIndex. */
+typedef struct gpugi_material_params {
+  char name[32];
+  float albedo[3];
+  float specular_color[3];
+  float roughness;
+  float opacity;
+  float emissivity[3];
+  float refraction_index;
+} gpugi_material_params;
+
+/* Builds a scene from a triangle soup with the SAH sweep builder (restating
+ * bvhmake/buildmethods/sweep.cpp:14-274 with leaf size triangles_per_leaf).
+ * positions/normals: num_vertices x 3 floats; texcoords may be NULL;
+ * indices: num_triangles x 3; material_ids: num_triangles. */
+int gpugi_hostscene_build(const float* positions, const float* normals, const float* texcoords,
+                          uint32_t num_vertices, const uint32_t* indices,
+                          const uint32_t* material_ids, uint32_t num_triangles,
+                          const gpugi_material_params* materials, uint32_t num_materials,
+                          uint32_t triangles_per_leaf, gpugi_hostscene** out);
+/* Synthetic Cornell room: 36 triangles, 10 walls (left red, right green),
+ * 2-triangle emissive ceiling quad, two rotated boxes. */
+int gpugi_hostscene_cornell(gpugi_hostscene** out);
+/* Heightfield of cells x cells quads (2 triangles each) over [0,1]^2, heights
+ * 0.2 * fBm (6 octaves of value noise on a wanghash lattice seeded `seed`). */
+int gpugi_hostscene_heightfield(uint32_t cells, uint32_t seed, gpugi_hostscene** out);
+/* Loads a .rawscene (documentation/fileformat.tex:37-168) and its material json. */
+int gpugi_hostscene_load(const char* rawscene_path, const char* json_path, gpugi_hostscene** out);
+/* Writes the .rawscene chunks the way bvhmake exports them
+ * (bvhbuilder.cpp:569-632) plus the material json. */
+int gpugi_hostscene_save(const gpugi_hostscene* scene, const char* rawscene_path,
+                         const char* json_path);
+/* Reference-layout arrays of the scene; pointers stay valid while the scene
+ * lives and until the next call that modifies it. */
+int gpugi_hostscene_desc(const gpugi_hostscene* scene, gpugi_scene_desc* out);
+/* Counts and bounds: num_triangles (valid slots), max tree depth. */
+int gpugi_hostscene_info(const gpugi_hostscene* scene, uint32_t* num_triangles,
+                         uint32_t* max_depth, float bbox_min[3], float bbox_max[3]);
+/* Scene::AddPointLight (scene.hpp:144). */
+int gpugi_hostscene_add_point_light(gpugi_hostscene* scene, const float position[3],
+                                    const float intensity[3]);
+void gpugi_hostscene_destroy(gpugi_hostscene* scene);
+
+/* Camera::ComputeCameraParams (camera.cpp:76-85) plus the PixelArea of
+ * RendererSystem::SetCamera (renderersystem.cpp:208). */
+int gpugi_camera_compute(const float position[3], const float lookat[3], const float up[3],
+                         float vfov_degrees, int width, int height, gpugi_camera* out);
+
+/* LightSampler (scene/lightsampler.cpp:18-95): area lights by summed-area CDF,
+ * point lights by summed log-flux CDF, xorshift64* seeded 123 by default. */
+typedef struct gpugi_lightsampler gpugi_lightsampler;
+int gpugi_lightsampler_create(const gpugi_hostscene* scene, uint64_t seed, gpugi_lightsampler** out);
+int gpugi_lightsampler_generate(gpugi_lightsampler* sampler, gpugi_light_sample* out, uint32_t n,
+                                float position_bias);
+void gpugi_lightsampler_destroy(gpugi_lightsampler* sampler);
+
+/* WangHash (utilities/random.cpp:5-13): the per-iteration FrameSeed. */
+uint32_t gpugi_wanghash(uint32_t x);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GPUGI_RT_H */

@@ -1,0 +1,82 @@
+/*
+ * CPU oracle of the gpugi ray-tracing hot path -- TEST INFRASTRUCTURE ONLY.
+ *
+ * A plain-C restatement of the reference algorithms, used by tests/, by
+ * __graft_entry__.smoke() and by bench.py's cpu_baseline leg as the checker.
+ * The product (gpugi_amd/) never links, loads or calls it.
+ *
+ * Compiled with gcc -O2 -ffp-contract=off (SSE scalar FP32, no FMA, no excess
+ * precision), so every FP32 expression rounds exactly as written.
+ *
+ * Pinning (see tests/): RNG against golden vectors from the reference's own
+ * utilities/random.cpp (built into oracle/_ref) and closed forms; math
+ * functions against libm / FP64 within stated ulp bounds; box and triangle
+ * tests against FP64 geometry and special cases; traversal against brute
+ * force over all triangles on small scenes; integrators against analytic
+ * cases (white furnace, single-bounce Lambertian, path-length slices).
+ */
+#ifndef GPUGI_ORACLE_H
+#define GPUGI_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+typedef struct {
+  const float* hierarchy;   /* num_nodes x 8 */
+  uint32_t num_nodes;
+  const int32_t* triangles; /* num_leaves x 2 x 4 */
+  uint32_t num_leaves;
+  const float* positions;   /* num_vertices x 3 */
+  const float* vertex_info; /* num_vertices x 4 */
+  uint32_t num_vertices;
+  const float* materials;   /* num_materials x 20 (gpugi_material) */
+  uint32_t num_materials;
+} or_scene;
+
+typedef struct {
+  float position[3], u[3], v[3], w[3], pixel_area;
+} or_camera;
+
+/* RNG (random.glsl:1-49, utilities/random.cpp:5-43) */
+uint32_t or_wanghash(uint32_t x);
+uint32_t or_xorshift32(uint32_t* s);
+float or_random01(uint32_t* s);
+uint64_t or_xorshift64star(uint64_t s);
+float or_xorshiftf(uint64_t* s);
+
+/* FP32 math (Cephes-style kernels; the semantics the renderer defines) */
+float or_sinf(float x);
+float or_cosf(float x);
+float or_log2f(float x);
+float or_exp2f(float x);
+float or_powf(float x, float y);
+float or_atanf(float x);
+float or_atan2f(float y, float x);
+
+/* Traversal (traceray.glsl:18-180).  rays: n x 8 floats {o, tmax, d, pad};
+ * hits: n x 4 int32 {slot, bits(t), bits(by), bits(bz)} (closest) or {flag,0,0,0}
+ * (any).  nbox/ntri may be NULL. */
+void or_trace(const or_scene* s, int any, const float* rays, int32_t* hits, uint32_t* nbox, uint32_t* ntri,
+              size_t n);
+/* Brute force: every valid slot with the reference triangle test, closest
+ * t > 1e-4 below tmax, ties to the smaller slot; any-hit: first accepted. */
+void or_bruteforce(const or_scene* s, int any, const float* rays, int32_t* hits, size_t n);
+/* FP64 classification of a ray against slots a and b (-1 = none).
+ * Returns flags: 1 = a within rel_eps of an edge, 2 = b within rel_eps of an
+ * edge, 4 = tie (|ta - tb| <= rel_eps * max), 8 = a or b within rel_eps of
+ * the t bounds (1e-4 or tmax), 16 = ray/triangle plane nearly parallel. */
+int or_classify(const or_scene* s, const float* ray, int32_t slot_a, int32_t slot_b, double rel_eps);
+
+/* Ray generators of the microbenchmark (restating the device generators). */
+void or_raygen_primary(const or_camera* c, int width, int height, uint32_t seed, float* rays, size_t first,
+                       size_t n);
+void or_raygen_diffuse(const or_scene* s, const float* prim, const int32_t* prim_hits, size_t n_prim,
+                       uint32_t seed, float* rays, size_t first, size_t n);
+void or_raygen_shadow(const float* prim, const int32_t* prim_hits, size_t n_prim, const float* lights,
+                      uint32_t n_lights, float* rays, size_t first, size_t n);
+
+/* Integrators: one draw adds into accum (width x height x 4 floats). */
+void or_pathtracer(const or_scene* s, const float* lights, uint32_t num_lights, const or_camera* c, int width,
+                   int height, uint32_t frame_seed, int max_path_length, float* accum, int row_begin, int row_end);
+
+#endif

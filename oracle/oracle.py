@@ -1,0 +1,184 @@
+"""ctypes binding of the CPU oracle -- TEST INFRASTRUCTURE ONLY.
+
+Imported by tests/, by __graft_entry__.smoke() and by bench.py's cpu_baseline
+leg as the checker; the product package never imports it.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+REF_LIB = os.path.join(HERE, "_ref", "librandom_ref.so")
+
+
+def build(reference=True):
+    """Builds liboracle.so (and the reference RNG library when the reference tree exists)."""
+    subprocess.run(["make", "-s", "-C", HERE, "liboracle.so"], check=True)
+    if reference and os.path.exists("/root/reference/gpugi/utilities/random.cpp"):
+        subprocess.run(["make", "-s", "-C", HERE, "ref"], check=True)
+
+
+class _Scene(ctypes.Structure):
+    _fields_ = [("hierarchy", ctypes.c_void_p), ("num_nodes", ctypes.c_uint32),
+                ("triangles", ctypes.c_void_p), ("num_leaves", ctypes.c_uint32),
+                ("positions", ctypes.c_void_p), ("vertex_info", ctypes.c_void_p),
+                ("num_vertices", ctypes.c_uint32), ("materials", ctypes.c_void_p),
+                ("num_materials", ctypes.c_uint32)]
+
+
+class _Camera(ctypes.Structure):
+    _fields_ = [("position", ctypes.c_float * 3), ("u", ctypes.c_float * 3), ("v", ctypes.c_float * 3),
+                ("w", ctypes.c_float * 3), ("pixel_area", ctypes.c_float)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build(reference=False)
+        l = ctypes.CDLL(LIB)
+        P = ctypes.c_void_p
+        l.or_wanghash.argtypes, l.or_wanghash.restype = [ctypes.c_uint32], ctypes.c_uint32
+        l.or_xorshift32.argtypes, l.or_xorshift32.restype = [ctypes.POINTER(ctypes.c_uint32)], ctypes.c_uint32
+        l.or_random01.argtypes, l.or_random01.restype = [ctypes.POINTER(ctypes.c_uint32)], ctypes.c_float
+        l.or_xorshift64star.argtypes, l.or_xorshift64star.restype = [ctypes.c_uint64], ctypes.c_uint64
+        l.or_xorshiftf.argtypes, l.or_xorshiftf.restype = [ctypes.POINTER(ctypes.c_uint64)], ctypes.c_float
+        for f in ("or_sinf", "or_cosf", "or_log2f", "or_exp2f", "or_atanf"):
+            getattr(l, f).argtypes, getattr(l, f).restype = [ctypes.c_float], ctypes.c_float
+        for f in ("or_powf", "or_atan2f"):
+            getattr(l, f).argtypes, getattr(l, f).restype = [ctypes.c_float, ctypes.c_float], ctypes.c_float
+        l.or_trace.argtypes = [P, ctypes.c_int, P, P, P, P, ctypes.c_size_t]
+        l.or_trace.restype = None
+        l.or_bruteforce.argtypes = [P, ctypes.c_int, P, P, ctypes.c_size_t]
+        l.or_bruteforce.restype = None
+        l.or_classify.argtypes = [P, P, ctypes.c_int32, ctypes.c_int32, ctypes.c_double]
+        l.or_classify.restype = ctypes.c_int
+        l.or_raygen_primary.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_uint32, P, ctypes.c_size_t,
+                                        ctypes.c_size_t]
+        l.or_raygen_primary.restype = None
+        l.or_raygen_diffuse.argtypes = [P, P, P, ctypes.c_size_t, ctypes.c_uint32, P, ctypes.c_size_t, ctypes.c_size_t]
+        l.or_raygen_diffuse.restype = None
+        l.or_raygen_shadow.argtypes = [P, P, ctypes.c_size_t, P, ctypes.c_uint32, P, ctypes.c_size_t, ctypes.c_size_t]
+        l.or_raygen_shadow.restype = None
+        l.or_pathtracer.argtypes = [P, P, ctypes.c_uint32, P, ctypes.c_int, ctypes.c_int, ctypes.c_uint32,
+                                    ctypes.c_int, P, ctypes.c_int, ctypes.c_int]
+        l.or_pathtracer.restype = None
+        _lib = l
+    return _lib
+
+
+def _p(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+class Scene:
+    """Holds numpy copies of the reference-layout arrays and the C view of them."""
+
+    def __init__(self, arrays):
+        self.a = {k: (np.ascontiguousarray(v) if isinstance(v, np.ndarray) else v) for k, v in arrays.items()}
+        mats = self.a["materials"]
+        self.mat_floats = np.ascontiguousarray(mats.view(np.float32).reshape(len(mats), -1))
+        assert self.mat_floats.shape[1] == 20
+        self.c = _Scene(_p(self.a["hierarchy"]), len(self.a["hierarchy"]), _p(self.a["triangles"]),
+                        len(self.a["triangles"]) // 2, _p(self.a["positions"]), _p(self.a["vertex_info"]),
+                        len(self.a["positions"]), _p(self.mat_floats), len(mats))
+
+    @property
+    def ptr(self):
+        return ctypes.byref(self.c)
+
+
+def camera(cam):
+    c = _Camera()
+    for k in ("position", "u", "v", "w"):
+        getattr(c, k)[:] = list(getattr(cam, k))
+    c.pixel_area = cam.pixel_area
+    return c
+
+
+def _rays_f32(rays):
+    return np.ascontiguousarray(rays).view(np.float32).reshape(-1, 8)
+
+
+def trace(scene, rays, any_hit=False, counts=False):
+    r = _rays_f32(rays)
+    n = len(r)
+    hits = np.zeros((n, 4), dtype=np.int32)
+    nb = np.zeros(n, dtype=np.uint32) if counts else None
+    nt = np.zeros(n, dtype=np.uint32) if counts else None
+    lib().or_trace(scene.ptr, int(any_hit), _p(r), _p(hits), _p(nb) if counts else None,
+                   _p(nt) if counts else None, n)
+    return (hits, nb, nt) if counts else hits
+
+
+def bruteforce(scene, rays, any_hit=False):
+    r = _rays_f32(rays)
+    hits = np.zeros((len(r), 4), dtype=np.int32)
+    lib().or_bruteforce(scene.ptr, int(any_hit), _p(r), _p(hits), len(r))
+    return hits
+
+
+def classify(scene, rays, slots_a, slots_b, rel_eps=1e-6):
+    r = _rays_f32(rays)
+    out = np.zeros(len(r), dtype=np.int32)
+    for i in range(len(r)):
+        row = np.ascontiguousarray(r[i])
+        out[i] = lib().or_classify(scene.ptr, _p(row), int(slots_a[i]), int(slots_b[i]), rel_eps)
+    return out
+
+
+def raygen_primary(cam, w, h, seed, n, first=0):
+    out = np.zeros((n, 8), dtype=np.float32)
+    c = camera(cam)
+    lib().or_raygen_primary(ctypes.byref(c), w, h, seed, _p(out), first, n)
+    return out
+
+
+def raygen_diffuse(scene, prim, prim_hits, seed, n, first=0):
+    out = np.zeros((n, 8), dtype=np.float32)
+    p = _rays_f32(prim)
+    h = np.ascontiguousarray(prim_hits, dtype=np.int32).reshape(-1, 4)
+    lib().or_raygen_diffuse(scene.ptr, _p(p), _p(h), len(p), seed, _p(out), first, n)
+    return out
+
+
+def raygen_shadow(prim, prim_hits, lights, n, first=0):
+    out = np.zeros((n, 8), dtype=np.float32)
+    p = _rays_f32(prim)
+    h = np.ascontiguousarray(prim_hits, dtype=np.int32).reshape(-1, 4)
+    l = np.ascontiguousarray(lights, dtype=np.float32).reshape(-1, 3)
+    lib().or_raygen_shadow(_p(p), _p(h), len(p), _p(l), len(l), _p(out), first, n)
+    return out
+
+
+def pathtrace(scene, lights, cam, w, h, frame_seed, max_path_length, accum=None, rows=None):
+    accum = np.zeros((h, w, 4), dtype=np.float32) if accum is None else accum
+    lt = np.ascontiguousarray(lights).view(np.float32).reshape(-1, 8)
+    c = camera(cam)
+    r0, r1 = rows if rows else (0, h)
+    lib().or_pathtracer(scene.ptr, _p(lt), len(lt), ctypes.byref(c), w, h, frame_seed, max_path_length,
+                        _p(accum), r0, r1)
+    return accum
+
+
+# ---- reference RNG built from the reference's own utilities/random.cpp ----
+_ref = None
+
+
+def ref_lib():
+    """The reference RNG library (oracle/_ref), or None where it was not built."""
+    global _ref
+    if _ref is None and os.path.exists(REF_LIB):
+        l = ctypes.CDLL(REF_LIB)
+        l._Z8WangHashj.argtypes, l._Z8WangHashj.restype = [ctypes.c_uint32], ctypes.c_uint32
+        l._Z8Xorshiftj.argtypes, l._Z8Xorshiftj.restype = [ctypes.c_uint32], ctypes.c_uint32
+        l._Z8Xorshiftm.argtypes, l._Z8Xorshiftm.restype = [ctypes.c_uint64], ctypes.c_uint64
+        l._Z9XorshiftFRm.argtypes, l._Z9XorshiftFRm.restype = [ctypes.POINTER(ctypes.c_uint64)], ctypes.c_float
+        _ref = l
+    return _ref

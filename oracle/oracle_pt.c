@@ -1,0 +1,227 @@
+/*
+ * CPU oracle: materials, BSDF, next-event estimation and the path tracer.
+ * TEST INFRASTRUCTURE ONLY (see oracle.h).  Restates materials.glsl,
+ * lightsource.glsl, scenedata.glsl:94-105 and pathtracer.comp.
+ */
+#include <string.h>
+
+#include "oracle.h"
+#include "oracle_vec.h"
+
+typedef struct {
+  float refl[4];
+  vec3 opacity, diffuse, emissivity, fresnel0;
+  float rIndex;
+  vec3 fresnel1;
+} or_material;
+
+static or_material mat_of(const or_scene* s, int id) {
+  const float* m = s->materials + (size_t)id * 20;
+  or_material r;
+  r.diffuse = v3(m[0], m[1], m[2]);
+  r.opacity = v3(m[3], m[4], m[5]);
+  r.refl[0] = m[6]; r.refl[1] = m[7]; r.refl[2] = m[8]; r.refl[3] = m[9];
+  r.emissivity = v3(m[10], m[11], m[12]);
+  r.fresnel0 = v3(m[13], m[14], m[15]);
+  r.rIndex = m[16];
+  r.fresnel1 = v3(m[17], m[18], m[19]);
+  return r;
+}
+
+/* AvgProbability / Avg (materials.glsl:33-41) */
+static float avg_p(vec3 c) { return (c.x + c.y + c.z + 1e-5f) / (3.0f + 1e-5f); }
+static float avg(vec3 c) { return (c.x + c.y + c.z) / 3.0f; }
+static float sgn(float x) { return x > 0.0f ? 1.0f : (x < 0.0f ? -1.0f : 0.0f); }
+
+/* GetBSDFDecisionPropabilities, USE_CONSTANT_REFLECTION (materials.glsl:43-63) */
+static void decision(const or_material* m, vec3* pr, vec3* pt, vec3* pd) {
+  *pr = m->fresnel0;
+  vec3 t = vsub(v3(1.0f, 1.0f, 1.0f), vsat(*pr));
+  *pd = vmul(t, m->opacity);
+  *pt = vsub(t, *pd);
+}
+
+/* __SampleBSDF (materials.glsl:118-277) */
+vec3 or_sample_bsdf(vec3 inc, const or_material* m, uint32_t* seed, vec3 N, vec3* tp, int adjoint, float* pdf,
+                    int* diffuseOut) {
+  float cosTheta = vdot(N, inc);
+  float cosThetaAbs = fsat(fabsf(cosTheta));
+  vec3 rrd = vsub(inc, vscale(2.0f * cosTheta, N));
+  float r0 = or_random01(seed);
+  float r1 = or_random01(seed);
+  vec3 pr, pt, pd;
+  decision(m, &pr, &pt, &pd);
+  float aR = avg_p(pr), aT = avg_p(pt), aD = avg(pd);
+  vec3 pphong = pr;
+  float aPhong = aR;
+  float xi = or_random01(seed);
+  if (aD > xi) {
+    *diffuseOut = 1;
+    *tp = vmul(*tp, vdiv(vmul(m->diffuse, pd), aD));
+    if (cosTheta > 0.0f) N = vneg(N);
+    vec3 U, V;
+    or_create_onb(N, &U, &V);
+    vec3 out = or_hemisphere_cosine(r0, r1, U, V, N);
+    *pdf = fsat(vdot(N, out)) / OR_PI;
+    return out;
+  } else if (aD + aT > xi) {
+    pphong = pt;
+    aPhong = aT;
+    float eta = cosTheta < 0.0f ? 1.0f / m->rIndex : m->rIndex;
+    float etaSq = eta * eta;
+    float s2 = etaSq * (1.0f - cosTheta * cosTheta);
+    if (s2 < 1.0f) {
+      rrd = vadd(vscale(eta, inc), vscale(sgn(cosTheta) * (sqrtf(fsat(1.0f - s2)) - eta * cosThetaAbs), N));
+      if (!adjoint) *tp = vdiv(*tp, etaSq);
+    }
+  }
+  *diffuseOut = 0;
+  rrd = vnormalize(rrd);
+  vec3 RU, RV;
+  or_create_onb(rrd, &RU, &RV);
+  vec3 out = or_phong_lobe(r0, r1, m->refl[3], RU, RV, rrd);
+  if (!adjoint) *tp = vscalev(*tp, fabsf(vdot(N, out)) / fabsf(vdot(N, rrd)));
+  float cAbs = fabsf(vdot(out, rrd));
+  *pdf = (m->refl[3] + 1.0f) / OR_PI_2 * or_powf(cAbs, m->refl[3]);
+  *pdf = *pdf * 1.0f;
+  *tp = vmul(*tp, vdiv(pphong, aPhong));
+  return out;
+}
+
+/* __BSDF (materials.glsl:306-391) */
+vec3 or_bsdf(vec3 inc, vec3 exc, const or_material* m, vec3 N, float* pdf, int adjoint, int specularOnly) {
+  float cosTheta = vdot(N, inc);
+  float cosThetaAbs = fsat(fabsf(cosTheta));
+  float cosThetaOut = vdot(N, exc);
+  vec3 pr, pt, pd;
+  decision(m, &pr, &pt, &pd);
+  float fB = (m->refl[3] + 1.0f) / (2.0f * OR_PI_2 * fabsf(cosThetaOut) + 1e-5f);
+  float fP = (m->refl[3] + 1.0f) / OR_PI_2;
+  vec3 rrd = vnormalize(vsub(inc, vscale(2.0f * cosTheta, N)));
+  float pw = or_powf(fsat(vdot(rrd, exc)), m->refl[3]);
+  float bF = fB * pw, pF = fP * pw;
+  vec3 bsdf = vscalev(pr, bF);
+  bsdf = vscalev(bsdf, fabsf(cosThetaOut) / cosThetaAbs);
+  *pdf = avg_p(pr) * pF;
+  if (!specularOnly && cosTheta * cosThetaOut < 0.0f) {
+    bsdf = vadd(bsdf, vdiv(vmul(pd, m->diffuse), OR_PI));
+    *pdf = *pdf + avg(pd) * fsat(cosThetaOut) / OR_PI;
+  }
+  float eta = cosTheta < 0.0f ? 1.0f / m->rIndex : m->rIndex;
+  float etaSq = eta * eta;
+  float s2 = etaSq * (1.0f - cosTheta * cosTheta);
+  if (s2 < 1.0f) {
+    rrd = vnormalize(vadd(vscale(eta, inc), vscale(sgn(cosTheta) * (sqrtf(fsat(1.0f - s2)) - eta * cosThetaAbs), N)));
+    float pw2 = or_powf(fsat(vdot(rrd, exc)), m->refl[3]);
+    bF = fB * pw2;
+    pF = fP * pw2;
+    if (!adjoint) bF = bF / etaSq;
+    bF = bF * (fabsf(vdot(N, exc)) / fabsf(vdot(N, rrd)));
+  }
+  bsdf = vadd(bsdf, vscalev(pt, bF));
+  *pdf = *pdf + avg_p(pt) * pF;
+  return bsdf;
+}
+
+/* GetTriangleHitInfo (scenedata.glsl:94-105) */
+static vec3 or_shading_normal(const or_scene* s, const int32_t* tri, vec3 b) {
+  const float* a = s->vertex_info + (size_t)tri[0] * 4;
+  const float* c = s->vertex_info + (size_t)tri[1] * 4;
+  const float* e = s->vertex_info + (size_t)tri[2] * 4;
+  vec3 n = vadd(vadd(vscalev(or_unpack_normal(a[0], a[1]), b.x), vscalev(or_unpack_normal(c[0], c[1]), b.y)),
+                vscalev(or_unpack_normal(e[0], e[1]), b.z));
+  return vnormalize(n);
+}
+
+static int any_hit(const or_scene* s, vec3 o, vec3 d, float len) {
+  int32_t slot;
+  vec3 b;
+  return or_trace_one(s, 1, o, d, &len, &slot, &b, NULL, NULL, NULL, NULL);
+}
+
+/* EstimateDirectLight (lightsource.glsl:3-93) */
+static vec3 or_direct_light(const or_scene* s, const float* lights, vec3 pos, vec3 normal, int idx, vec3 view,
+                            const or_material* m, int sampleArea) {
+  const float* ls = lights + (size_t)idx * 8;
+  vec3 dir = vsub(v3(ls[0], ls[1], ls[2]), pos);
+  float distSq = vdot(dir, dir) + 1e-5f;
+  float dist = sqrtf(distSq);
+  dir = vdiv(dir, dist);
+  float surfaceCos = fsat(vdot(dir, normal));
+  if (surfaceCos <= 0.0f) return v3(0, 0, 0);
+  vec3 I = v3(ls[4], ls[5], ls[6]);
+  if (ls[7] > 1.5f) {
+    vec3 o = vadd(vscale(0.0001f, dir), pos);
+    if (!any_hit(s, o, dir, dist - 0.0001f * 2.0f)) {
+      vec3 irr = vscale(surfaceCos / distSq, I);
+      float pdf;
+      vec3 f = or_bsdf(view, dir, m, normal, &pdf, 0, 0);
+      return vmul(irr, f);
+    }
+  } else if (sampleArea) {
+    vec3 ln = or_unpack_normal(ls[3], ls[7]);
+    float k = fsat(vdot(ln, vneg(dir)));
+    if (k > 0.0f) {
+      vec3 o = vadd(vscale(0.0001f, dir), pos);
+      if (!any_hit(s, o, dir, dist - 0.0001f * 2.0f)) {
+        float pdf;
+        vec3 li = vscalev(I, k);
+        vec3 f = or_bsdf(view, dir, m, normal, &pdf, 0, 0);
+        vec3 irr = vscale(surfaceCos / distSq, li);
+        return vmul(irr, f);
+      }
+    }
+  }
+  return v3(0, 0, 0);
+}
+
+/* pathtracer.comp:16-110, rows [row_begin, row_end) of the truncated (W/8*8) x (H/8*8) grid */
+void or_pathtracer(const or_scene* s, const float* lights, uint32_t numLights, const or_camera* c, int W, int H,
+                   uint32_t frameSeed, int maxLen, float* accum, int row_begin, int row_end) {
+  const int gw = (W / 8) * 8, gh = (H / 8) * 8;
+  vec3 U = v3(c->u[0], c->u[1], c->u[2]), V = v3(c->v[0], c->v[1], c->v[2]), Wd = v3(c->w[0], c->w[1], c->w[2]);
+  const float lsw = (float)numLights / 1.0f;
+  if (row_end > gh) row_end = gh;
+  for (int y = row_begin; y < row_end; ++y) {
+    for (int x = 0; x < gw; ++x) {
+      uint32_t seed = or_wanghash(frameSeed + (uint32_t)(x + y * W));
+      vec3 color = v3(0, 0, 0), thr = v3(1, 1, 1);
+      int didNEE = 0;
+      float j0 = or_random01(&seed), j1 = or_random01(&seed);
+      vec3 o = v3(c->position[0], c->position[1], c->position[2]);
+      vec3 d = or_primary_dir(j0, j1, x, y, W, H, U, V, Wd);
+      for (int i = 0; i < maxLen; ++i) {
+        float t = 3.402823466e+38f;
+        int32_t slot, tri[4];
+        vec3 b, gn;
+        or_trace_one(s, 0, o, d, &t, &slot, &b, &gn, tri, NULL, NULL);
+        if (t == 3.402823466e+38f) {
+          color = vadd(color, vmul(thr, v3(0, 0, 0)));
+          break;
+        }
+        o = vadd(o, vscale(t, d));
+        vec3 n = or_shading_normal(s, tri, b);
+        or_material m = mat_of(s, tri[3]);
+        if (!didNEE) color = vadd(color, vmul(thr, m.emissivity));
+        didNEE = m.refl[3] < 500.0f;
+        int idx = (int)(or_xorshift32(&seed) % numLights);
+        vec3 rad = or_direct_light(s, lights, o, n, idx, d, &m, didNEE);
+        color = vadd(color, vscalev(vmul(rad, thr), lsw));
+        vec3 tp = v3(1, 1, 1);
+        float pdf;
+        int dif;
+        d = or_sample_bsdf(d, &m, &seed, n, &tp, 0, &pdf, &dif);
+        o = vadd(o, vscalev(d, 0.0001f));
+        thr = vmul(thr, tp);
+        float q = fsat(or_luminance(tp));
+        if (or_random01(&seed) >= q) break;
+        thr = vdivv(thr, v3(q, q, q));
+      }
+      float* px = accum + ((size_t)y * W + x) * 4;
+      px[0] = color.x + px[0];
+      px[1] = color.y + px[1];
+      px[2] = color.z + px[2];
+      px[3] = 0.0f + px[3];
+    }
+  }
+}
