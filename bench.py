@@ -37,6 +37,9 @@ def parse():
     p.add_argument("--cpu-seconds", type=float, default=12.0, help="bounded CPU-baseline sample duration")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--extra", action="store_true", help="also time coherent / shadow sets and the reference walk")
+    p.add_argument("--no-autotune", action="store_true", help="keep the default wide-kernel configuration")
+    p.add_argument("--calibrate", action="store_true",
+                   help="append 3 device-to-device copies of 1 GiB (counter calibration for rocprofv3 --pmc)")
     return p.parse_args()
 
 
@@ -83,6 +86,15 @@ def main():
     R = workloads.HeightfieldRays(ctx, n=n)
     kernel = rt.KERNEL_WIDE if args.kernel == "wide" else rt.KERNEL_REFERENCE
 
+    # Autotune the wide kernel on a separate incoherent sample (other seed), untimed.
+    tuned = None
+    if kernel == rt.KERNEL_WIDE and not args.no_autotune:
+        m = min(n, 1 << 22)
+        T = workloads.HeightfieldRays(ctx, n=m, seed=(workloads.MASTER_SEED + 7) & 0xffffffff)
+        tuned_ms = ctx.autotune_trace(T.incoherent, T.hits, m, rt.CLOSEST_HIT, repeats=2)
+        tuned = dict(ctx.wide_config(), sample_rays=m, sample_ms=round(tuned_ms, 4))
+        T.free()
+
     # Algorithmic bytes: reference escape-walk visit counts of every ray.
     nbox_b, ntri_b = ctx.alloc(n * 4), ctx.alloc(n * 4)
     ctx.trace_counts_device(R.incoherent, nbox_b, ntri_b, n, rt.CLOSEST_HIT)
@@ -109,7 +121,7 @@ def main():
     extra = {"setup_s": round(setup_s, 2), "wall_s_timed": round(wall, 4),
              "hit_fraction": float((hits[:, 0] >= 0).mean()),
              "mean_box_tests": float(nbox.mean()), "mean_tri_tests": float(ntri.mean()),
-             "alg_bytes_per_ray": alg_bytes / n}
+             "alg_bytes_per_ray": alg_bytes / n, "wide_config": tuned or None}
     if args.extra:
         for name, buf, q in (("coherent_closest", R.primary, rt.CLOSEST_HIT), ("shadow_any", R.shadow, rt.ANY_HIT)):
             ctx.trace_device(buf, R.hits, n, q, kernel)
@@ -122,6 +134,13 @@ def main():
         ctx.trace_device(R.incoherent, R.hits, n, rt.CLOSEST_HIT, rt.KERNEL_REFERENCE)
         extra["reference_walk_incoherent_grays"] = n / (ctx.timer_end() * 1e-3) / 1e9
 
+    if args.calibrate:
+        a, b = ctx.alloc(1 << 30), ctx.alloc(1 << 30)
+        for _ in range(3):
+            b.copy_from(a, 1 << 30)
+        ctx.sync()
+        a.free()
+        b.free()
     traffic = traffic_from_profile()
     result = {
         "metric": "incoherent closest-hit Grays/s",
@@ -144,7 +163,7 @@ def main():
         "extra": extra,
     }
     if not args.no_cpu_baseline:
-        rays_host = R.incoherent.download(np.float32, min(n, 1 << 22) * 8).reshape(-1, 8)
+        rays_host = R.incoherent.download(np.float32, n * 8).reshape(-1, 8)
         result["cpu_baseline"] = cpu_baseline(hs.arrays(), rays_host, args.cpu_seconds)
     else:
         result["cpu_baseline"] = None

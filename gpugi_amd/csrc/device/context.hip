@@ -35,7 +35,14 @@ struct gpugi_ctx {
   void* dWideTris = nullptr;
   void* dSpill = nullptr;
   void* dCounters = nullptr;  // [0] ray counter, [1] error flag, [2] hit count
-  uint32_t wideThreads = 0, spillDepth = 0, wideMaxStack = 0;
+  size_t spillBytes = 0;
+  uint32_t wideMaxStack = 0;
+  int wideLeaf = 4;     // triangles per merged wide leaf
+  int wideStack = 16;   // LDS stack entries per lane of the trace kernel
+  // host copy of the reference layout (re-packing after a configuration change)
+  std::vector<float> hHier, hPos;
+  std::vector<int32_t> hTris;
+  uint32_t hNumLeaves = 0, hNumVertices = 0;
   // image state
   int width = 0, height = 0;
   void* dAccum = nullptr;
@@ -46,6 +53,8 @@ struct gpugi_ctx {
   uint32_t numLights = 0, lightCap = 0;
   int maxPathLength = 16;  // default pathLength (application.cpp:123)
   int integratorKernel = GPUGI_KERNEL_AUTO;
+  int bptCapacity = -1;
+  int bptAverageLength = 0;
   IntegratorState integ;
   // scratch
   void* dScratch = nullptr;
@@ -82,13 +91,15 @@ RefScene MakeRef(const gpugi_ctx* c) {
   return s;
 }
 
-WideScene MakeWide(const gpugi_ctx* c) {
+// Wide-scene view for a kernel whose LDS stack holds `stack` entries; the
+// spill slice per thread covers the rest of the tree's worst case.
+WideScene MakeWide(const gpugi_ctx* c, int stack) {
   WideScene s;
   s.nodes = (const uint4*)c->dWideNodes;
   s.tris = (const float4*)c->dWideTris;
   s.spill = (uint2*)c->dSpill;
-  s.spillDepth = c->spillDepth;
-  s.numThreads = c->wideThreads;
+  s.spillDepth = c->wideMaxStack > (uint32_t)stack ? c->wideMaxStack - (uint32_t)stack : 0;
+  s.numThreads = 0;
   s.errorFlag = (uint32_t*)c->dCounters + 1;
   return s;
 }
@@ -107,9 +118,61 @@ int SetDev(gpugi_ctx* ctx) {
   return GPUGI_OK;
 }
 
+// Persistent grid and spill buffer for the trace kernel variant `stack`.
+int PrepareTraceWide(gpugi_ctx* ctx, int stack, WideScene* out) {
+  int blocksPerCU = 0;
+  HIP_TRY(ctx, WideBlocksPerCU(stack, &blocksPerCU));
+  if (blocksPerCU < 1) blocksPerCU = 1;
+  WideScene s = MakeWide(ctx, stack);
+  s.numThreads = (uint32_t)(ctx->numCUs * blocksPerCU * kWideBlock);
+  const size_t bytes = (size_t)s.numThreads * s.spillDepth * sizeof(uint2);
+  if (bytes > ctx->spillBytes) {
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    FreeDev(ctx->dSpill);
+    ctx->spillBytes = 0;
+    HIP_TRY(ctx, hipMalloc(&ctx->dSpill, bytes));
+    ctx->spillBytes = bytes;
+  }
+  s.spill = (uint2*)ctx->dSpill;
+  *out = s;
+  return GPUGI_OK;
+}
+
+// Re-packs the host copy of the reference BVH into the wide layout with the
+// context's leaf size and uploads it.
+int RebuildWide(gpugi_ctx* ctx) {
+  gpugi_scene_desc d{};
+  d.hierarchy = ctx->hHier.data();
+  d.num_nodes = ctx->numNodes;
+  d.triangles = ctx->hTris.data();
+  d.num_leaves = ctx->hNumLeaves;
+  d.triangles_per_leaf = 2;
+  d.positions = ctx->hPos.data();
+  d.num_vertices = ctx->hNumVertices;
+  WideBVHHost wide;
+  std::string err = BuildWideBVH(d, (uint32_t)ctx->wideLeaf, &wide);
+  if (!err.empty()) return Fail(ctx, GPUGI_ERR_FORMAT, "wide BVH re-pack: " + err);
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  FreeDev(ctx->dWideNodes);
+  FreeDev(ctx->dWideTris);
+  ctx->hasWide = false;
+  HIP_TRY(ctx, hipMalloc(&ctx->dWideNodes, wide.nodes.size() * sizeof(WideNodeHost)));
+  HIP_TRY(ctx, hipMalloc(&ctx->dWideTris, std::max<size_t>(1, wide.tris.size()) * sizeof(WideTriHost)));
+  HIP_TRY(ctx, hipMemcpy(ctx->dWideNodes, wide.nodes.data(), wide.nodes.size() * sizeof(WideNodeHost),
+                         hipMemcpyHostToDevice));
+  if (!wide.tris.empty())
+    HIP_TRY(ctx, hipMemcpy(ctx->dWideTris, wide.tris.data(), wide.tris.size() * sizeof(WideTriHost),
+                           hipMemcpyHostToDevice));
+  ctx->wideMaxStack = wide.maxStack;
+  ctx->hasWide = true;
+  return GPUGI_OK;
+}
+
 }  // namespace
 
 const RefScene* gpugi_internal_refscene(const gpugi_ctx* c, RefScene* tmp) { *tmp = MakeRef(c); return tmp; }
+
+static DrawParams MakeDrawParams(gpugi_ctx* ctx, uint32_t frame_seed);
 
 extern "C" {
 
@@ -186,14 +249,11 @@ int gpugi_load_scene(gpugi_ctx* ctx, const gpugi_scene_desc* d) {
       return Fail(ctx, GPUGI_ERR_FORMAT, "gpugi_load_scene: material index out of range");
   }
   if (SetDev(ctx)) return GPUGI_ERR_HIP;
-  WideBVHHost wide;
-  std::string err = BuildWideBVH(*d, 4, &wide);
-  if (!err.empty()) return Fail(ctx, GPUGI_ERR_FORMAT, "gpugi_load_scene: " + err);
-
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   for (void** p : {&ctx->dNodes, &ctx->dSlots, &ctx->dPos, &ctx->dVinfo, &ctx->dMats, &ctx->dWideNodes,
                    &ctx->dWideTris, &ctx->dSpill})
     FreeDev(*p);
+  ctx->spillBytes = 0;
   ctx->hasScene = ctx->hasWide = false;
   const size_t bNodes = (size_t)nn * 32, bSlots = (size_t)nl * 2 * 16, bPos = (size_t)nv * 12 + 4,
                bInfo = (size_t)nv * 16, bMats = (size_t)d->num_materials * sizeof(gpugi_material);
@@ -212,30 +272,23 @@ int gpugi_load_scene(gpugi_ctx* ctx, const gpugi_scene_desc* d) {
   ctx->numVertices = nv;
   ctx->numMaterials = d->num_materials;
   ctx->hasScene = true;
-
-  // Wide layout + persistent-grid spill stack.
-  int blocksPerCU = 0;
-  HIP_TRY(ctx, WideBlocksPerCU(&blocksPerCU));
-  if (blocksPerCU < 1) blocksPerCU = 1;
-  ctx->wideThreads = (uint32_t)(ctx->numCUs * blocksPerCU * kWideBlock);
-  ctx->wideMaxStack = wide.maxStack;
-  ctx->spillDepth = wide.maxStack > (uint32_t)kWideLdsStack ? wide.maxStack - (uint32_t)kWideLdsStack : 0;
-  HIP_TRY(ctx, hipMalloc(&ctx->dWideNodes, wide.nodes.size() * sizeof(WideNodeHost)));
-  HIP_TRY(ctx, hipMalloc(&ctx->dWideTris, std::max<size_t>(1, wide.tris.size()) * sizeof(WideTriHost)));
-  HIP_TRY(ctx, hipMemcpy(ctx->dWideNodes, wide.nodes.data(), wide.nodes.size() * sizeof(WideNodeHost),
-                         hipMemcpyHostToDevice));
-  if (!wide.tris.empty())
-    HIP_TRY(ctx, hipMemcpy(ctx->dWideTris, wide.tris.data(), wide.tris.size() * sizeof(WideTriHost),
-                           hipMemcpyHostToDevice));
-  if (ctx->spillDepth > 0)
-    HIP_TRY(ctx, hipMalloc(&ctx->dSpill, (size_t)ctx->wideThreads * ctx->spillDepth * sizeof(uint2)));
-  ctx->hasWide = true;
+  try {
+    ctx->hHier.assign(d->hierarchy, d->hierarchy + (size_t)nn * 8);
+    ctx->hTris.assign(d->triangles, d->triangles + (size_t)nl * 8);
+    ctx->hPos.assign(d->positions, d->positions + (size_t)nv * 3);
+  } catch (const std::bad_alloc&) {
+    return Fail(ctx, GPUGI_ERR_NOMEM, "gpugi_load_scene: host copy of the scene");
+  }
+  ctx->hNumLeaves = nl;
+  ctx->hNumVertices = nv;
+  if (int rc = RebuildWide(ctx)) return rc;
   HIP_TRY(ctx, hipMemset(ctx->dCounters, 0, 64));
   if (ctx->dAccum) {
     HIP_TRY(ctx, hipMemset(ctx->dAccum, 0, (size_t)ctx->width * ctx->height * 16));
     ctx->iterations = 0;
   }
   InvalidateIntegratorCaches(&ctx->integ);
+  ctx->bptCapacity = -1;
   return GPUGI_OK;
 }
 
@@ -263,6 +316,7 @@ int gpugi_set_screen(gpugi_ctx* ctx, int width, int height) {
   ctx->height = height;
   ctx->iterations = 0;
   InvalidateIntegratorCaches(&ctx->integ);
+  ctx->bptCapacity = -1;
   return GPUGI_OK;
 }
 
@@ -305,9 +359,19 @@ int gpugi_draw(gpugi_ctx* ctx, int integrator, uint32_t frame_seed) {
     return Fail(ctx, GPUGI_ERR_STATE, "gpugi_draw: scene, screen, camera and light samples must be set first");
   if (integrator < GPUGI_PT || integrator > GPUGI_BPT) return Fail(ctx, GPUGI_ERR_ARG, "gpugi_draw: unknown integrator");
   if (SetDev(ctx)) return GPUGI_ERR_HIP;
+  DrawParams p = MakeDrawParams(ctx, frame_seed);
+  std::string err;
+  hipError_t e = DrawIntegrator(&ctx->integ, integrator, p, ctx->stream, &err);
+  if (e != hipSuccess) return Fail(ctx, GPUGI_ERR_HIP, "gpugi_draw: " + err + hipGetErrorString(e));
+  if (!err.empty()) return Fail(ctx, GPUGI_ERR_STATE, "gpugi_draw: " + err);
+  ++ctx->iterations;
+  return GPUGI_OK;
+}
+
+static DrawParams MakeDrawParams(gpugi_ctx* ctx, uint32_t frame_seed) {
   DrawParams p;
   p.ref = MakeRef(ctx);
-  p.wide = MakeWide(ctx);
+  p.wide = MakeWide(ctx, kWideLdsStack);
   p.useWide = ctx->hasWide && ctx->integratorKernel != GPUGI_KERNEL_REFERENCE;
   p.materials = (const gpugi_material*)ctx->dMats;
   p.lights = (const gpugi_light_sample*)ctx->dLights;
@@ -319,11 +383,47 @@ int gpugi_draw(gpugi_ctx* ctx, int integrator, uint32_t frame_seed) {
   p.frameSeed = frame_seed;
   p.maxPathLength = ctx->maxPathLength;
   p.counters = (uint32_t*)ctx->dCounters;
-  std::string err;
-  hipError_t e = DrawIntegrator(&ctx->integ, integrator, p, ctx->stream, &err);
-  if (e != hipSuccess) return Fail(ctx, GPUGI_ERR_HIP, "gpugi_draw: " + err + hipGetErrorString(e));
-  if (!err.empty()) return Fail(ctx, GPUGI_ERR_STATE, "gpugi_draw: " + err);
-  ++ctx->iterations;
+  p.bptCapacity = ctx->bptCapacity;
+  return p;
+}
+
+int gpugi_bpt_warmup(gpugi_ctx* ctx, uint32_t frame_seed, uint64_t* num_light_cache_entries,
+                     uint64_t* light_path_length_sum) {
+  if (!ctx || !num_light_cache_entries || !light_path_length_sum)
+    return Fail(ctx, GPUGI_ERR_ARG, "gpugi_bpt_warmup: null argument");
+  if (!ctx->hasScene || !ctx->dAccum || !ctx->hasCamera || ctx->numLights == 0)
+    return Fail(ctx, GPUGI_ERR_STATE, "gpugi_bpt_warmup: scene, screen, camera and light samples must be set first");
+  if (SetDev(ctx)) return GPUGI_ERR_HIP;
+  BptWarmupCounts c{0, 0};
+  HIP_TRY(ctx, BptWarmup(&ctx->integ, MakeDrawParams(ctx, frame_seed), ctx->stream, &c));
+  *num_light_cache_entries = c.numLightCacheEntries;
+  *light_path_length_sum = c.lightPathLengthSum;
+  return GPUGI_OK;
+}
+
+int gpugi_bpt_set_light_cache(gpugi_ctx* ctx, int32_t capacity, int32_t average_light_path_length) {
+  if (!ctx || capacity < 1) return Fail(ctx, GPUGI_ERR_ARG, "gpugi_bpt_set_light_cache: capacity must be positive");
+  ctx->bptCapacity = capacity;
+  ctx->bptAverageLength = average_light_path_length;
+  return GPUGI_OK;
+}
+
+int gpugi_bpt_read_light_cache(gpugi_ctx* ctx, void* entries, uint32_t max_entries, uint32_t* num_entries) {
+  if (!ctx || !num_entries || (max_entries > 0 && !entries))
+    return Fail(ctx, GPUGI_ERR_ARG, "gpugi_bpt_read_light_cache: null argument");
+  if (!ctx->integ.lastTotal || !ctx->integ.lightCache)
+    return Fail(ctx, GPUGI_ERR_STATE, "gpugi_bpt_read_light_cache: no BPT draw since the last scene/screen change");
+  if (SetDev(ctx)) return GPUGI_ERR_HIP;
+  uint32_t total = 0;
+  HIP_TRY(ctx, hipMemcpyAsync(&total, ctx->integ.lastTotal, 4, hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  const uint32_t avail = (uint32_t)std::min<int64_t>((int64_t)total, (int64_t)ctx->bptCapacity);
+  const uint32_t n = std::min(avail, max_entries);
+  if (n > 0) {
+    HIP_TRY(ctx, hipMemcpyAsync(entries, ctx->integ.lightCache, (size_t)n * 64, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  }
+  *num_entries = total;
   return GPUGI_OK;
 }
 
@@ -366,8 +466,13 @@ int gpugi_trace(gpugi_ctx* ctx, int query, int kernel, const gpugi_ray* rays, gp
     dh = (gpugi_hit*)((char*)ctx->dScratch + n * sizeof(gpugi_ray));
     HIP_TRY(ctx, hipMemcpyAsync((void*)dr, rays, n * sizeof(gpugi_ray), hipMemcpyHostToDevice, ctx->stream));
   }
-  if (wide) HIP_TRY(ctx, LaunchTraceWide(MakeWide(ctx), query, dr, dh, n, (uint32_t*)ctx->dCounters, ctx->stream));
-  else HIP_TRY(ctx, LaunchTraceRef(MakeRef(ctx), query, dr, dh, n, ctx->stream));
+  if (wide) {
+    WideScene ws;
+    if (int rc = PrepareTraceWide(ctx, ctx->wideStack, &ws)) return rc;
+    HIP_TRY(ctx, LaunchTraceWide(ws, ctx->wideStack, query, dr, dh, n, (uint32_t*)ctx->dCounters, ctx->stream));
+  } else {
+    HIP_TRY(ctx, LaunchTraceRef(MakeRef(ctx), query, dr, dh, n, ctx->stream));
+  }
   if (!rays_on_device) {
     HIP_TRY(ctx, hipMemcpyAsync(hits, dh, n * sizeof(gpugi_hit), hipMemcpyDeviceToHost, ctx->stream));
     uint32_t flag = 0;
@@ -375,6 +480,66 @@ int gpugi_trace(gpugi_ctx* ctx, int query, int kernel, const gpugi_ray* rays, gp
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     if (flag) return Fail(ctx, GPUGI_ERR_STATE, "gpugi_trace: traversal stack overflow (bug in the stack bound)");
   }
+  return GPUGI_OK;
+}
+
+int gpugi_set_wide_config(gpugi_ctx* ctx, int max_leaf_triangles, int lds_stack_entries) {
+  if (!ctx) return Fail(ctx, GPUGI_ERR_ARG, "gpugi_set_wide_config: null context");
+  bool stackOk = false;
+  for (int v : kWideStackVariants) stackOk |= v == lds_stack_entries;
+  if (!stackOk || max_leaf_triangles < 1 || max_leaf_triangles > (int)kWideMaxLeafTris)
+    return Fail(ctx, GPUGI_ERR_ARG, "gpugi_set_wide_config: leaf size in [1, 16], stack one of 8/12/16/24/32");
+  ctx->wideStack = lds_stack_entries;
+  if (max_leaf_triangles != ctx->wideLeaf) {
+    ctx->wideLeaf = max_leaf_triangles;
+    if (ctx->hasScene) {
+      if (SetDev(ctx)) return GPUGI_ERR_HIP;
+      if (int rc = RebuildWide(ctx)) return rc;
+    }
+  }
+  return GPUGI_OK;
+}
+
+int gpugi_get_wide_config(gpugi_ctx* ctx, int* max_leaf_triangles, int* lds_stack_entries, uint32_t* max_stack) {
+  if (!ctx) return Fail(ctx, GPUGI_ERR_ARG, "gpugi_get_wide_config: null context");
+  if (max_leaf_triangles) *max_leaf_triangles = ctx->wideLeaf;
+  if (lds_stack_entries) *lds_stack_entries = ctx->wideStack;
+  if (max_stack) *max_stack = ctx->wideMaxStack;
+  return GPUGI_OK;
+}
+
+int gpugi_autotune_trace(gpugi_ctx* ctx, int query, const gpugi_ray* rays_dev, gpugi_hit* hits_dev, size_t n,
+                         int repeats, float* best_ms) {
+  if (!ctx || !rays_dev || !hits_dev || n == 0 || repeats < 1)
+    return Fail(ctx, GPUGI_ERR_ARG, "gpugi_autotune_trace: invalid argument");
+  if (!ctx->hasScene) return Fail(ctx, GPUGI_ERR_STATE, "gpugi_autotune_trace: no scene loaded");
+  if (query != GPUGI_CLOSEST_HIT && query != GPUGI_ANY_HIT)
+    return Fail(ctx, GPUGI_ERR_ARG, "gpugi_autotune_trace: bad query");
+  if (SetDev(ctx)) return GPUGI_ERR_HIP;
+  const int leaves[] = {1, 2, 4, 8};
+  float best = INFINITY;
+  int bestLeaf = ctx->wideLeaf, bestStack = ctx->wideStack;
+  for (int leaf : leaves) {
+    if (int rc = gpugi_set_wide_config(ctx, leaf, ctx->wideStack)) return rc;
+    for (int stack : kWideStackVariants) {
+      ctx->wideStack = stack;
+      WideScene ws;
+      if (int rc = PrepareTraceWide(ctx, stack, &ws)) return rc;
+      // one untimed launch, then the median-free average of `repeats`
+      HIP_TRY(ctx, LaunchTraceWide(ws, stack, query, rays_dev, hits_dev, n, (uint32_t*)ctx->dCounters, ctx->stream));
+      HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+      for (int r = 0; r < repeats; ++r)
+        HIP_TRY(ctx, LaunchTraceWide(ws, stack, query, rays_dev, hits_dev, n, (uint32_t*)ctx->dCounters, ctx->stream));
+      HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+      HIP_TRY(ctx, hipEventSynchronize(ctx->ev1));
+      float ms = 0.0f;
+      HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+      ms /= (float)repeats;
+      if (ms < best) { best = ms; bestLeaf = leaf; bestStack = stack; }
+    }
+  }
+  if (int rc = gpugi_set_wide_config(ctx, bestLeaf, bestStack)) return rc;
+  if (best_ms) *best_ms = best;
   return GPUGI_OK;
 }
 
@@ -447,6 +612,13 @@ int gpugi_download(gpugi_ctx* ctx, void* host_dst, const void* device_src, size_
   if (SetDev(ctx)) return GPUGI_ERR_HIP;
   HIP_TRY(ctx, hipMemcpyAsync(host_dst, device_src, bytes, hipMemcpyDeviceToHost, ctx->stream));
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return GPUGI_OK;
+}
+
+int gpugi_copy_device(gpugi_ctx* ctx, void* device_dst, const void* device_src, size_t bytes) {
+  if (!ctx || !device_dst || !device_src) return Fail(ctx, GPUGI_ERR_ARG, "gpugi_copy_device: null argument");
+  if (SetDev(ctx)) return GPUGI_ERR_HIP;
+  HIP_TRY(ctx, hipMemcpyAsync(device_dst, device_src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
   return GPUGI_OK;
 }
 

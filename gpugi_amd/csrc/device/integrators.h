@@ -23,7 +23,29 @@ struct DrawParams {
   uint32_t frameSeed = 0;
   int maxPathLength = 16;
   uint32_t* counters = nullptr;
+  int bptCapacity = -1;                   // LightCacheCapacity (bidirectionalpathtracer.cpp:110-112)
 };
+
+// LightCacheEntry (bidirpathtracer/lightcache.glsl:1-16): 64 bytes.
+struct LightCacheEntryDev {
+  float flux[3];
+  int32_t materialIndex;
+  float position[3];
+  float normal0;
+  float texcoord[2];
+  float normal1;
+  int32_t pathLength;
+  float incidentPacked[2];
+  float chainedMisN;
+  float chainedMisT;
+};
+static_assert(sizeof(LightCacheEntryDev) == 64, "light cache entry is 64 bytes");
+
+// NumRaysPerLightSample (lightpathtracer.cpp:29-31): every 64-wide block works on one light sample.
+inline int RaysPerLightSample(int width, int height, uint32_t numLightSamples) {
+  int r = (int)(((uint32_t)(width * height) / numLightSamples / 64u) * 64u);
+  return r < 64 ? 64 : r;
+}
 
 // Device buffers an integrator keeps between draws (spill stacks, the BPT
 // light cache and its capacity estimate).
@@ -34,9 +56,34 @@ struct IntegratorState {
   size_t lightCacheBytes = 0;
   void* scratch = nullptr;
   size_t scratchBytes = 0;
-  int64_t lightCacheCapacity = -1;  // -1: estimate on the next BPT draw
-  int averageLightPathLength = 0;
+  void* staging = nullptr;          // BPT per-path vertex staging (deterministic append)
+  size_t stagingBytes = 0;
+  void* pathCounts = nullptr;       // per light path: stored vertices; then exclusive offsets
+  size_t pathCountsBytes = 0;
+  uint32_t* lastTotal = nullptr;    // device NumLightCacheEntries of the last BPT draw
 };
+
+// Grows a device buffer to at least `bytes` (contents are not preserved).
+inline hipError_t EnsureBuffer(void** buf, size_t* cap, size_t bytes) {
+  if (*cap >= bytes) return hipSuccess;
+  if (*buf) (void)hipFree(*buf);
+  *buf = nullptr;
+  *cap = 0;
+  hipError_t e = hipMalloc(buf, bytes);
+  if (e == hipSuccess) *cap = bytes;
+  return e;
+}
+
+hipError_t DrawLT(IntegratorState* st, const DrawParams& p, hipStream_t stream);
+hipError_t DrawBPT(IntegratorState* st, const DrawParams& p, hipStream_t stream, std::string* err);
+
+// Result of a BPT warm-up light pass (the two SSBO counters read back by
+// CreateLightCacheWithCapacityEstimate, bidirectionalpathtracer.cpp:75-104).
+struct BptWarmupCounts {
+  uint64_t numLightCacheEntries;  // NumRaysPerLightSample * N + stored vertices
+  uint64_t lightPathLengthSum;    // stored vertices
+};
+hipError_t BptWarmup(IntegratorState* st, const DrawParams& p, hipStream_t stream, BptWarmupCounts* out);
 
 void FreeIntegratorState(IntegratorState* s);
 void InvalidateIntegratorCaches(IntegratorState* s);

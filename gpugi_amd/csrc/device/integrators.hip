@@ -6,114 +6,9 @@
 // pixel when the reference traversal is selected.
 #include <hip/hip_runtime.h>
 
-#include "integrators.h"
-#include "shading.h"
-#include "trace_wide.h"
+#include "integrator_common.h"
 
 namespace gpugi {
-
-constexpr int kTile = 8;                 // local size 8x8 of pathtracer.comp:14-15
-constexpr int kTileThreads = kTile * kTile;
-
-// Closest / any hit through either traversal.  The wide walk needs a per-lane
-// LDS stack (S x 64 entries per block) and a global spill slice.
-struct TraceCtx {
-  const DrawParams* p;
-  uint32_t* stkNode;
-  float* stkT;
-  int tid;
-  uint2* spill;
-};
-
-struct ClosestHit {
-  bool hit;
-  float t;
-  int4 tri;
-  gg::V3 bary;
-  gg::V3 geoNormal;  // cross(e1, e0), not normalized (TRINORMAL_OUTPUT)
-};
-
-template <bool WIDE>
-__device__ __forceinline__ ClosestHit trace_closest(const TraceCtx& c, gg::V3 o, gg::V3 d) {
-  ClosestHit h;
-  if (WIDE) {
-    WideResult r = trace_wide_ray<false, kTileThreads, kWideLdsStack>(c.p->wide, o, d, gg::kRayMax, c.stkNode, c.stkT,
-                                                                      c.tid, c.spill);
-    h.hit = r.slot >= 0;
-    h.t = r.t;
-    if (h.hit) {
-      h.tri = c.p->ref.slots[r.slot];
-      h.bary = gg::v3(1.0f - (r.bz + r.by), r.by, r.bz);
-      gg::V3 p0 = load_pos(c.p->ref.positions, h.tri.x), p1 = load_pos(c.p->ref.positions, h.tri.y),
-             p2 = load_pos(c.p->ref.positions, h.tri.z);
-      h.geoNormal = gg::cross(p0 - p2, p1 - p0);
-    }
-  } else {
-    float rayLength = gg::kRayMax;
-    RefHit rh;
-    uint32_t nb = 0, nt = 0;
-    trace_ref<false, false>(c.p->ref, o, d, rayLength, rh, nb, nt);
-    h.hit = rayLength != gg::kRayMax;  // pathtracer.comp:56 tests rayHit == RAY_MAX
-    h.t = rayLength;
-    if (h.hit) {
-      h.tri = rh.tri;
-      h.bary = rh.bary;
-      h.geoNormal = rh.triNormal;
-    }
-  }
-  return h;
-}
-
-template <bool WIDE>
-__device__ __forceinline__ bool trace_any(const TraceCtx& c, gg::V3 o, gg::V3 d, float len) {
-  if (WIDE) {
-    WideResult r = trace_wide_ray<true, kTileThreads, kWideLdsStack>(c.p->wide, o, d, len, c.stkNode, c.stkT, c.tid,
-                                                                     c.spill);
-    return r.any;
-  }
-  float rayLength = len;
-  RefHit rh;
-  uint32_t nb = 0, nt = 0;
-  return trace_ref<true, false>(c.p->ref, o, d, rayLength, rh, nb, nt);
-}
-
-// EstimateDirectLight (lightsource.glsl:3-93) without importance break / cache.
-template <bool WIDE>
-__device__ __forceinline__ gg::V3 estimate_direct_light(const TraceCtx& c, gg::V3 pos, gg::V3 normal, int idx,
-                                                        gg::V3 viewDir, const MaterialData& m, bool sampleAreaLights) {
-  const gpugi_light_sample& ls = c.p->lights[idx];
-  const gg::V3 lpos = gg::v3(ls.position[0], ls.position[1], ls.position[2]);
-  const gg::V3 lint = gg::v3(ls.intensity[0], ls.intensity[1], ls.intensity[2]);
-  gg::V3 dir = lpos - pos;
-  const float lightDistSq = gg::dot(dir, dir) + gg::kDivisorEpsilon;
-  const float lightDist = sqrtf(lightDistSq);
-  dir = dir / lightDist;
-  const float surfaceCos = gg::saturate(gg::dot(dir, normal));
-  if (surfaceCos <= 0.0f) return gg::v3s(0.0f);
-  if (ls.normal_theta_cos > 1.5f) {
-    const gg::V3 o = gg::kRayHitEpsilon * dir + pos;
-    if (!trace_any<WIDE>(c, o, dir, lightDist - gg::kRayHitEpsilon * 2.0f)) {
-      const gg::V3 irradiance = (surfaceCos / lightDistSq) * lint;
-      float pdf;
-      const gg::V3 bsdf = bsdf_eye(viewDir, dir, m, normal, pdf);
-      return irradiance * bsdf;
-    }
-  } else if (sampleAreaLights) {
-    const gg::V3 lightNormal = gg::unpack_normal(ls.normal_phi, ls.normal_theta_cos);
-    const float f = gg::saturate(gg::dot(lightNormal, -dir));
-    if (f > 0.0f) {
-      const gg::V3 o = gg::kRayHitEpsilon * dir + pos;
-      if (!trace_any<WIDE>(c, o, dir, lightDist - gg::kRayHitEpsilon * 2.0f)) {
-        float pdf;
-        const gg::V3 intensity = lint * f;
-        const gg::V3 bsdf = bsdf_eye(viewDir, dir, m, normal, pdf);
-        const gg::V3 irradiance = (surfaceCos / lightDistSq) * intensity;
-        return irradiance * bsdf;
-      }
-    }
-  }
-  return gg::v3s(0.0f);
-}
 
 // pathtracer.comp:16-110 (NUM_LIGHT_SAMPLES 1, RUSSIAN_ROULETTE, black environment).
 template <bool WIDE>
@@ -167,24 +62,14 @@ __global__ __launch_bounds__(kTileThreads) void k_pathtracer(DrawParams p) {
   *px = make_float4(color.x + v.x, color.y + v.y, color.z + v.z, 0.0f + v.w);
 }
 
-static hipError_t EnsureBuffer(void** buf, size_t* cap, size_t bytes) {
-  if (*cap >= bytes) return hipSuccess;
-  if (*buf) hipFree(*buf);
-  *buf = nullptr;
-  *cap = 0;
-  hipError_t e = hipMalloc(buf, bytes);
-  if (e == hipSuccess) *cap = bytes;
-  return e;
-}
-
 void FreeIntegratorState(IntegratorState* s) {
-  for (void** b : {&s->spill, &s->lightCache, &s->scratch})
-    if (*b) { hipFree(*b); *b = nullptr; }
-  s->spillBytes = s->lightCacheBytes = s->scratchBytes = 0;
-  s->lightCacheCapacity = -1;
+  for (void** b : {&s->spill, &s->lightCache, &s->scratch, &s->staging, &s->pathCounts})
+    if (*b) { (void)hipFree(*b); *b = nullptr; }
+  s->spillBytes = s->lightCacheBytes = s->scratchBytes = s->stagingBytes = s->pathCountsBytes = 0;
+  s->lastTotal = nullptr;
 }
 
-void InvalidateIntegratorCaches(IntegratorState* s) { s->lightCacheCapacity = -1; }
+void InvalidateIntegratorCaches(IntegratorState* s) { s->lastTotal = nullptr; }
 
 static hipError_t DrawPT(IntegratorState* st, const DrawParams& p0, hipStream_t stream) {
   DrawParams p = p0;
@@ -208,6 +93,10 @@ hipError_t DrawIntegrator(IntegratorState* st, int integrator, const DrawParams&
   switch (integrator) {
     case GPUGI_PT:
       return DrawPT(st, p, stream);
+    case GPUGI_LT:
+      return DrawLT(st, p, stream);
+    case GPUGI_BPT:
+      return DrawBPT(st, p, stream, err);
     default:
       *err = "integrator not available in this build";
       return hipSuccess;

@@ -12,10 +12,13 @@ hipError_t LaunchTraceRef(const RefScene& s, int query, const gpugi_ray* rays, g
                           hipStream_t stream);
 hipError_t LaunchTraceRefCounts(const RefScene& s, int query, const gpugi_ray* rays, uint32_t* nbox,
                                 uint32_t* ntri, size_t n, hipStream_t stream);
-hipError_t LaunchTraceWide(const WideScene& s, int query, const gpugi_ray* rays, gpugi_hit* hits, size_t n,
-                           uint32_t* rayCounter, hipStream_t stream);
-// Resident blocks per CU of the wide kernel (sizes the persistent grid).
-hipError_t WideBlocksPerCU(int* blocks);
+// Wide kernel with an LDS stack of `stack` entries per lane (one of kWideStackVariants).
+hipError_t LaunchTraceWide(const WideScene& s, int stack, int query, const gpugi_ray* rays, gpugi_hit* hits,
+                           size_t n, uint32_t* rayCounter, hipStream_t stream);
+// Resident blocks per CU of the wide kernel variant (sizes the persistent grid).
+hipError_t WideBlocksPerCU(int stack, int* blocks);
+constexpr int kNumWideStackVariants = 5;
+extern const int kWideStackVariants[kNumWideStackVariants];
 
 hipError_t LaunchRaygenPrimary(const gpugi_camera& cam, int width, int height, uint32_t seed,
                                gpugi_ray* rays, size_t n, hipStream_t stream);

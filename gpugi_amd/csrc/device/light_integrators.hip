@@ -1,0 +1,455 @@
+// Light tracer (lighttracer.comp + showlightcaches.comp) and bidirectional path
+// tracer (lighttracer.comp with SAVE_LIGHT_CACHE + pathtracer_bidir.comp).
+//
+// Compiled with -ffp-contract=off; every kernel keeps the random-number draw
+// order of the GLSL it restates.  Camera splats use global float atomics
+// (the reference's imageAtomicAdd on an r32f alias of the backbuffer,
+// screenreproject.glsl:238-241), so per-pixel sums are exact up to the
+// order of the additions.
+//
+// The BPT light cache is appended deterministically: every light path stages
+// its stored vertices in a private slice, an exclusive scan over the per-path
+// counts assigns each path a contiguous range, and a scatter writes the cache
+// in (path, bounce) order behind the R*N initial vertices.  This is one of the
+// orders the reference's atomicAdd append (lighttracer.comp:120) can produce,
+// fixed so that renders are reproducible and comparable with a CPU
+// restatement.
+#include <hip/hip_runtime.h>
+
+#include "integrator_common.h"
+#include "scan.h"
+
+namespace gpugi {
+
+constexpr int kLightBlock = 64;  // local size of lighttracer.comp:48
+
+__device__ __forceinline__ float mis_heuristic(float p) { return fminf(1e18f, p); }     // lightcache.glsl:61-73
+__device__ __forceinline__ float distance_mis(float d, float c) { return mis_heuristic(d * d); }  // :84-89
+
+// ProjectToScreen without DEPTH_BUFFER_OCCLUSION_TEST / CHAINED_MIS and with
+// the box reconstruction filter (screenreproject.glsl:55-227).
+template <bool WIDE>
+__device__ __forceinline__ void project_to_screen(const TraceCtx& c, gg::V3 rayO, gg::V3 rayD, gg::V3 gN,
+                                                  gg::V3 sN, const MaterialData& m, gg::V3 tp, bool computeMIS) {
+  const DrawParams& p = *c.p;
+  const gg::V3 U = gg::v3(p.camera.u[0], p.camera.u[1], p.camera.u[2]);
+  const gg::V3 V = gg::v3(p.camera.v[0], p.camera.v[1], p.camera.v[2]);
+  const gg::V3 W = gg::v3(p.camera.w[0], p.camera.w[1], p.camera.w[2]);
+  const gg::V3 cp = gg::v3(p.camera.position[0], p.camera.position[1], p.camera.position[2]);
+  gg::V3 crd = cp - rayO;
+  const float camViewDotRay = -gg::dot(W, crd);
+  if (!(camViewDotRay > 0.0f)) return;
+  float sx = -gg::dot(U, crd) / camViewDotRay;
+  float sy = -gg::dot(V, crd) / camViewDotRay;
+  sx = sx / gg::dot(U, U);
+  sy = sy / gg::dot(V, V);
+  if (!(sx >= -1.0f && sx < 1.0f && sy > -1.0f && sy <= 1.0f)) return;
+  const float cameraDistSq = gg::dot(crd, crd);
+  const float cameraDist = sqrtf(cameraDistSq);
+  crd = crd / cameraDist;
+  const gg::V3 co = crd * gg::kRayHitEpsilon + rayO;
+  const float ppx = (sx * 0.5f + 0.5f) * (float)p.width;
+  const float ppy = (sy * 0.5f + 0.5f) * (float)p.height;
+  const int ix = (int)roundf(ppx), iy = (int)roundf(ppy);
+  if (trace_any<WIDE>(c, co, crd, cameraDist)) return;
+  const float cosAtCamera = gg::saturate(camViewDotRay / cameraDist);
+  const float f2i = adjoint_corrected_out_dot_n(rayD, crd, gN, sN) /
+                    (cosAtCamera * cosAtCamera * cosAtCamera * p.camera.pixel_area * cameraDistSq);
+  float pdf;
+  gg::V3 bsdf = bsdf_adjoint(rayD, crd, m, sN, pdf);
+  if (computeMIS) {
+    const float eyeToLight = 1.0f;  // InitialMIS (lightcache.glsl:76-79)
+    const float lightToEye = mis_heuristic(pdf);
+    const float mis = eyeToLight / fmaxf(lightToEye + eyeToLight, gg::kDivisorEpsilon);
+    bsdf = bsdf * mis;
+  }
+  const gg::V3 color = tp * f2i * bsdf;
+  // Splats outside the image are dropped, as image atomics outside the image are.
+  if (ix < 0 || ix >= p.width || iy < 0 || iy >= p.height) return;
+  float* px = reinterpret_cast<float*>(&p.accum[(size_t)iy * p.width + ix]);
+  atomicAdd(px + 0, color.x);
+  atomicAdd(px + 1, color.y);
+  atomicAdd(px + 2, color.z);
+}
+
+enum LightMode { kModeLT = 0, kModeBptLight = 1, kModeBptWarmup = 2 };
+
+__device__ __forceinline__ LightCacheEntryDev make_entry(gg::V3 flux, int mat, gg::V3 pos, float n0, float n1,
+                                                         float u, float v, gg::V3 incident, int len) {
+  LightCacheEntryDev e;
+  e.flux[0] = flux.x; e.flux[1] = flux.y; e.flux[2] = flux.z;
+  e.materialIndex = mat;
+  e.position[0] = pos.x; e.position[1] = pos.y; e.position[2] = pos.z;
+  e.normal0 = n0;
+  e.texcoord[0] = u; e.texcoord[1] = v;
+  e.normal1 = n1;
+  e.pathLength = len;
+  gg::pack_normal(incident, e.incidentPacked[0], e.incidentPacked[1]);
+  e.chainedMisN = 0.0f;
+  e.chainedMisT = 0.0f;
+  return e;
+}
+
+__device__ __forceinline__ void store_entry(LightCacheEntryDev* dst, const LightCacheEntryDev& e) {
+  const float4* s = reinterpret_cast<const float4*>(&e);
+  float4* d = reinterpret_cast<float4*>(dst);
+  d[0] = s[0]; d[1] = s[1]; d[2] = s[2]; d[3] = s[3];
+}
+
+// lighttracer.comp:49-193.  MODE selects LT, BPT light pass or BPT warm-up.
+template <bool WIDE, int MODE>
+__global__ __launch_bounds__(kLightBlock) void k_lighttracer(DrawParams p, int raysPerSample,
+                                                             LightCacheEntryDev* __restrict__ cache,
+                                                             LightCacheEntryDev* __restrict__ staging,
+                                                             uint32_t* __restrict__ pathCounts) {
+  __shared__ uint32_t stkNode[WIDE ? kWideLdsStack * kLightBlock : 1];
+  __shared__ float stkT[WIDE ? kWideLdsStack * kLightBlock : 1];
+  const int tid = threadIdx.x;
+  const uint32_t gid = blockIdx.x * kLightBlock + tid;
+  TraceCtx c{&p, stkNode, stkT, tid, WIDE ? p.wide.spill + (size_t)gid * p.wide.spillDepth : nullptr};
+  uint32_t seed = gg::init_random_seed(p.frameSeed, gid);
+  const int lsIndex = (int)(gid / (uint32_t)raysPerSample);
+  const gpugi_light_sample& ls = p.lights[lsIndex];
+  const gg::V3 lpos = gg::v3(ls.position[0], ls.position[1], ls.position[2]);
+  const gg::V3 lint = gg::v3(ls.intensity[0], ls.intensity[1], ls.intensity[2]);
+  if (MODE == kModeBptLight && (int)gid < p.bptCapacity) {
+    // Initial light vertex of this path (lighttracer.comp:56-61).
+    const gg::V3 flux = (lint / gg::kPI_2) / (float)raysPerSample;
+    store_entry(&cache[gid], make_entry(flux, -1, lpos, ls.normal_phi, ls.normal_theta_cos, 0.0f, 0.0f,
+                                        gg::v3s(0.0f), 0));
+  }
+  gg::V3 tp = lint * (gg::kPI / (float)raysPerSample);
+  gg::V3 o = lpos;
+  gg::V3 d;
+  if (ls.normal_theta_cos > 1.5f) {
+    const float r0 = gg::random01(seed);
+    const float r1 = gg::random01(seed);
+    d = gg::sample_direction(r0, r1);
+    tp = tp * 4.0f;
+  } else {
+    const gg::V3 n = gg::unpack_normal(ls.normal_phi, ls.normal_theta_cos);
+    gg::V3 U, V;
+    gg::create_onb(n, U, V);
+    const float r0 = gg::random01(seed);
+    const float r1 = gg::random01(seed);
+    d = gg::sample_hemisphere_cosine(r0, r1, U, V, n);
+  }
+  uint32_t stored = 0;
+  for (int i = 0; i < p.maxPathLength; ++i) {
+    const ClosestHit h = trace_closest<WIDE>(c, o, d);
+    if (!h.hit) break;
+    o = o + h.t * d;
+    const gg::V3 sN = shading_normal(p.ref.vinfo, h.tri, h.bary);
+    const float2 uv = hit_texcoord(p.ref.vinfo, h.tri, h.bary);
+    const MaterialData m = load_material(p.materials, h.tri.w);
+    if (MODE != kModeLT && m.refl[3] < 100.0f) {
+      if (MODE == kModeBptLight) {
+        float n0, n1;
+        gg::pack_normal(sN, n0, n1);
+        store_entry(&staging[(size_t)gid * p.maxPathLength + stored],
+                    make_entry(tp, h.tri.w, o, n0, n1, uv.x, uv.y, d, i + 1));
+      }
+      ++stored;
+    }
+    if (MODE != kModeBptWarmup) project_to_screen<WIDE>(c, o, d, h.geoNormal, sN, m, tp, MODE == kModeBptLight);
+    gg::V3 t1 = gg::v3s(1.0f);
+    float pdf;
+    bool diffuse;
+    const gg::V3 nd = sample_bsdf(d, m, seed, sN, t1, true, pdf, diffuse);
+    t1 = t1 * adjoint_correction(d, nd, h.geoNormal, sN);
+    tp = tp * t1;
+    const float q = gg::saturate(gg::luminance(t1));
+    if (gg::random01(seed) >= q) break;
+    tp = tp / q;
+    d = nd;
+    o = o + d * gg::kRayHitEpsilon;
+  }
+  if (MODE != kModeLT) pathCounts[gid] = stored;
+}
+
+// showlightcaches.comp:9-65 for exactly N light samples.
+template <bool WIDE>
+__global__ __launch_bounds__(kLightBlock) void k_showlightcaches(DrawParams p) {
+  __shared__ uint32_t stkNode[WIDE ? kWideLdsStack * kLightBlock : 1];
+  __shared__ float stkT[WIDE ? kWideLdsStack * kLightBlock : 1];
+  const int tid = threadIdx.x;
+  const uint32_t i = blockIdx.x * kLightBlock + tid;
+  if (i >= p.numLights) return;
+  TraceCtx c{&p, stkNode, stkT, tid, WIDE ? p.wide.spill + (size_t)i * p.wide.spillDepth : nullptr};
+  const gpugi_light_sample& ls = p.lights[i];
+  const gg::V3 lpos = gg::v3(ls.position[0], ls.position[1], ls.position[2]);
+  const gg::V3 U = gg::v3(p.camera.u[0], p.camera.u[1], p.camera.u[2]);
+  const gg::V3 V = gg::v3(p.camera.v[0], p.camera.v[1], p.camera.v[2]);
+  const gg::V3 W = gg::v3(p.camera.w[0], p.camera.w[1], p.camera.w[2]);
+  const gg::V3 cp = gg::v3(p.camera.position[0], p.camera.position[1], p.camera.position[2]);
+  gg::V3 crd = cp - lpos;
+  const float camViewDotRay = -gg::dot(W, crd);
+  if (!(camViewDotRay > 0.0f)) return;
+  float sx = -gg::dot(U, crd) / camViewDotRay;
+  float sy = -gg::dot(V, crd) / camViewDotRay;
+  sx = sx / gg::dot(U, U);
+  sy = sy / gg::dot(V, V);
+  if (!(sx >= -1.0f && sx < 1.0f && sy > -1.0f && sy <= 1.0f)) return;
+  const float cameraDistSq = gg::dot(crd, crd);
+  const float cameraDist = sqrtf(cameraDistSq);
+  crd = crd / cameraDist;
+  const gg::V3 co = crd * gg::kRayHitEpsilon + lpos;
+  if (trace_any<WIDE>(c, co, crd, cameraDist)) return;
+  const gg::V3 n = gg::unpack_normal(ls.normal_phi, ls.normal_theta_cos);
+  const float cosTheta = gg::dot(n, crd);
+  const float cosAtCamera = gg::saturate(camViewDotRay / cameraDist);
+  const float i2r = cosTheta / (gg::kPI_2 * cosAtCamera * cosAtCamera * cosAtCamera * p.camera.pixel_area * cameraDistSq);
+  const gg::V3 color = gg::v3(ls.intensity[0], ls.intensity[1], ls.intensity[2]) * i2r;
+  const int ix = (int)((sx * 0.5f + 0.5f) * (float)p.width);
+  const int iy = (int)((sy * 0.5f + 0.5f) * (float)p.height);
+  if (ix < 0 || ix >= p.width || iy < 0 || iy >= p.height) return;
+  // The reference does an unsynchronized load/store here (showlightcaches.comp:54-55);
+  // atomics give the intended sum.
+  float* px = reinterpret_cast<float*>(&p.accum[(size_t)iy * p.width + ix]);
+  atomicAdd(px + 0, color.x);
+  atomicAdd(px + 1, color.y);
+  atomicAdd(px + 2, color.z);
+}
+
+// Scatter of the staged BPT vertices behind the initial ones.
+__global__ __launch_bounds__(256) void k_bpt_scatter(const LightCacheEntryDev* __restrict__ staging,
+                                                     const uint32_t* __restrict__ counts,
+                                                     const uint32_t* __restrict__ offsets, uint32_t numPaths,
+                                                     int maxLen, int capacity, LightCacheEntryDev* __restrict__ cache) {
+  const uint32_t gid = blockIdx.x * 256u + threadIdx.x;
+  if (gid >= numPaths) return;
+  const uint32_t n = counts[gid];
+  const uint64_t base = (uint64_t)numPaths + offsets[gid];
+  for (uint32_t k = 0; k < n; ++k) {
+    const uint64_t dst = base + k;
+    if (dst < (uint64_t)capacity) {
+      const float4* s = reinterpret_cast<const float4*>(&staging[(size_t)gid * maxLen + k]);
+      float4* d = reinterpret_cast<float4*>(&cache[dst]);
+      d[0] = s[0]; d[1] = s[1]; d[2] = s[2]; d[3] = s[3];
+    }
+  }
+}
+
+// pathtracer_bidir.comp:16-251 without CHAINED_MIS.
+template <bool WIDE>
+__global__ __launch_bounds__(kTileThreads) void k_bpt_eye(DrawParams p, const LightCacheEntryDev* __restrict__ cache,
+                                                          const uint32_t* __restrict__ numEntriesPtr) {
+  __shared__ uint32_t stkNode[WIDE ? kWideLdsStack * kTileThreads : 1];
+  __shared__ float stkT[WIDE ? kWideLdsStack * kTileThreads : 1];
+  const int tid = threadIdx.x;
+  const int gx = blockIdx.x * kTile + (tid % kTile);
+  const int gy = blockIdx.y * kTile + (tid / kTile);
+  const uint32_t linear = (blockIdx.y * gridDim.x + blockIdx.x) * kTileThreads + tid;
+  TraceCtx c{&p, stkNode, stkT, tid, WIDE ? p.wide.spill + (size_t)linear * p.wide.spillDepth : nullptr};
+  uint32_t seed = gg::init_random_seed(p.frameSeed, (uint32_t)(gx + gy * p.width));
+  const float j0 = gg::random01(seed);
+  const float j1 = gg::random01(seed);
+  const gg::V3 U = gg::v3(p.camera.u[0], p.camera.u[1], p.camera.u[2]);
+  const gg::V3 V = gg::v3(p.camera.v[0], p.camera.v[1], p.camera.v[2]);
+  const gg::V3 W = gg::v3(p.camera.w[0], p.camera.w[1], p.camera.w[2]);
+  gg::V3 o = gg::v3(p.camera.position[0], p.camera.position[1], p.camera.position[2]);
+  gg::V3 d = gg::primary_direction(j0, j1, gx, gy, p.width, p.height, U, V, W);
+  gg::V3 color = gg::v3s(0.0f);
+  gg::V3 tp = gg::v3s(1.0f);
+  const int numEntries = (int)*numEntriesPtr;
+  const uint32_t actualN = (uint32_t)(numEntries < p.bptCapacity ? numEntries : p.bptCapacity);
+  const float lightSampleWeight = (float)actualN;
+  float lastConnectionProbability = 1.0f;  // InitialMIS
+  float lastCosAbs = 1.0f;
+  float mis = 0.0f;
+  for (int i = 0; i < p.maxPathLength; ++i) {
+    const ClosestHit h = trace_closest<WIDE>(c, o, d);
+    if (!h.hit) break;
+    const float rayHit = h.t;
+    o = o + rayHit * d;
+    const gg::V3 n = shading_normal(p.ref.vinfo, h.tri, h.bary);
+    const MaterialData m = load_material(p.materials, h.tri.w);
+    const float incidentSurfaceCosSat = gg::saturate(-gg::dot(n, d));
+    const float connectionLightToEyePath = distance_mis(rayHit, lastCosAbs) * mis_heuristic(incidentSurfaceCosSat / gg::kPI);
+    mis = (i == 0) ? 1.0f
+                   : (lastConnectionProbability /
+                      fmaxf(lastConnectionProbability + connectionLightToEyePath, gg::kDivisorEpsilon));
+    color = color + mis * tp * m.emissivity;
+    // Connection to one random light-cache vertex.
+    const uint32_t idx = gg::xorshift32(seed) % actualN;
+    const LightCacheEntryDev& e = cache[idx];
+    const int pathLength = e.pathLength + i + 1;
+    if (pathLength <= p.maxPathLength) {
+      const gg::V3 epos = gg::v3(e.position[0], e.position[1], e.position[2]);
+      gg::V3 ld = epos - o;
+      const float lightDistSq = gg::dot(ld, ld);
+      const float lightDist = sqrtf(lightDistSq);
+      ld = ld / fmaxf(lightDist, gg::kDivisorEpsilon);
+      const gg::V3 lo = gg::kRayHitEpsilon * ld + o;
+      if (!trace_any<WIDE>(c, lo, ld, lightDist - gg::kRayHitEpsilon * 2.0f)) {
+        const int lmat = e.materialIndex;
+        const gg::V3 flux = gg::v3(e.flux[0], e.flux[1], e.flux[2]);
+        gg::V3 lnormal = gg::v3(0.0f, 1.0f, 0.0f);
+        float cosLight = 0.0f;
+        gg::V3 lintensity;
+        const float cosCamera = fabsf(gg::dot(ld, n));
+        if (e.normal1 < 1.5f) {
+          lnormal = gg::unpack_normal_raw(e.normal0, e.normal1);
+          cosLight = fabsf(gg::dot(ld, lnormal));
+        }
+        float pdf;
+        const gg::V3 bsdf = bsdf_eye(d, ld, m, n, pdf);
+        float eyeToLight = mis_heuristic(pdf);
+        if (i > 0) {
+          const float distMIS = distance_mis(rayHit, lastCosAbs);
+          eyeToLight = eyeToLight * distMIS;
+          mis = lastConnectionProbability / fmaxf(lastConnectionProbability + eyeToLight, gg::kDivisorEpsilon);
+          if (m.refl[3] >= 100.0f) mis = 1.0f;
+        } else {
+          mis = eyeToLight / fmaxf(lastConnectionProbability + eyeToLight, gg::kDivisorEpsilon);
+        }
+        float misLight = 1.0f;
+        if (e.normal1 < 1.5f) {
+          float lightToEye;
+          if (lmat != -1) {
+            const MaterialData lm = load_material(p.materials, lmat);
+            const gg::V3 incidentDir = gg::unpack_normal(e.incidentPacked[0], e.incidentPacked[1]);
+            const gg::V3 lbsdf = bsdf_adjoint(ld, -incidentDir, lm, lnormal, pdf);
+            lintensity = flux * lbsdf * cosLight;
+            lightToEye = mis_heuristic(pdf) * distance_mis(lightDist, cosCamera);
+          } else {
+            lintensity = flux * (cosLight * gg::kPI_2);
+            lightToEye = mis_heuristic(cosLight / gg::kPI) * distance_mis(lightDist, cosCamera);
+          }
+          misLight = lightToEye / fmaxf(lightToEye + eyeToLight, gg::kDivisorEpsilon);
+        } else {
+          lintensity = flux * gg::kPI_2;
+          misLight = 1.0f;
+        }
+        mis = mis * misLight;
+        const gg::V3 irradiance = (mis * lightSampleWeight * cosCamera / lightDistSq) * lintensity;
+        color = color + irradiance * tp * bsdf;
+      }
+    }
+    float samplePDF;
+    gg::V3 t1 = gg::v3s(1.0f);
+    bool diffuse;
+    d = sample_bsdf(d, m, seed, n, t1, false, samplePDF, diffuse);
+    o = o + d * gg::kRayHitEpsilon;
+    tp = tp * t1;
+    const float q = gg::saturate(gg::luminance(t1));
+    if (gg::random01(seed) >= q) break;
+    tp = tp / q;
+    float reverse = mis_heuristic(samplePDF);
+    if (i == 0) {
+      mis = reverse / fmaxf(reverse + lastConnectionProbability, gg::kDivisorEpsilon);
+    } else {
+      const float distMIS = distance_mis(rayHit, lastCosAbs);
+      reverse = reverse * distMIS;
+      mis = lastConnectionProbability / fmaxf(lastConnectionProbability + distMIS * reverse, gg::kDivisorEpsilon);
+      if (m.refl[3] >= 100.0f) mis = 1.0f;
+    }
+    tp = tp * mis;
+    lastConnectionProbability = reverse;
+    lastCosAbs = fabsf(gg::dot(d, n));
+  }
+  float4* px = &p.accum[(size_t)gy * p.width + gx];
+  const float4 v = *px;
+  *px = make_float4(color.x + v.x, color.y + v.y, color.z + v.z, 1.0f + v.w);
+}
+
+// ---------------------------------------------------------------------------
+static hipError_t PrepareSpill(IntegratorState* st, DrawParams* p, size_t threads) {
+  if (!p->useWide) return hipSuccess;
+  const size_t bytes = std::max<size_t>(8, threads * p->wide.spillDepth * sizeof(uint2));
+  hipError_t e = EnsureBuffer(&st->spill, &st->spillBytes, bytes);
+  if (e == hipSuccess) p->wide.spill = (uint2*)st->spill;
+  return e;
+}
+
+template <int MODE>
+static void LaunchLight(const DrawParams& p, int R, uint32_t numPaths, LightCacheEntryDev* cache,
+                        LightCacheEntryDev* staging, uint32_t* counts, hipStream_t stream) {
+  dim3 grid(numPaths / kLightBlock);
+  if (p.useWide)
+    hipLaunchKernelGGL((k_lighttracer<true, MODE>), grid, dim3(kLightBlock), 0, stream, p, R, cache, staging, counts);
+  else
+    hipLaunchKernelGGL((k_lighttracer<false, MODE>), grid, dim3(kLightBlock), 0, stream, p, R, cache, staging, counts);
+}
+
+hipError_t DrawLT(IntegratorState* st, const DrawParams& p0, hipStream_t stream) {
+  DrawParams p = p0;
+  const int R = RaysPerLightSample(p.width, p.height, p.numLights);
+  const uint32_t numPaths = p.numLights * (uint32_t)R;
+  hipError_t e = PrepareSpill(st, &p, std::max<size_t>(numPaths, p.numLights));
+  if (e != hipSuccess) return e;
+  LaunchLight<kModeLT>(p, R, numPaths, nullptr, nullptr, nullptr, stream);
+  // LightPathtracer::Draw then projects the light samples themselves (lightpathtracer.cpp:50).
+  dim3 grid((p.numLights + kLightBlock - 1) / kLightBlock);
+  if (p.useWide) hipLaunchKernelGGL(k_showlightcaches<true>, grid, dim3(kLightBlock), 0, stream, p);
+  else hipLaunchKernelGGL(k_showlightcaches<false>, grid, dim3(kLightBlock), 0, stream, p);
+  return hipGetLastError();
+}
+
+hipError_t BptWarmup(IntegratorState* st, const DrawParams& p0, hipStream_t stream, BptWarmupCounts* out) {
+  DrawParams p = p0;
+  const int R = RaysPerLightSample(p.width, p.height, p.numLights);
+  const uint32_t numPaths = p.numLights * (uint32_t)R;
+  hipError_t e = PrepareSpill(st, &p, numPaths);
+  if (e != hipSuccess) return e;
+  e = EnsureBuffer(&st->pathCounts, &st->pathCountsBytes, (size_t)numPaths * 4);
+  if (e != hipSuccess) return e;
+  size_t scanTemp = ExclusiveScanTempBytes(numPaths);
+  e = EnsureBuffer(&st->scratch, &st->scratchBytes, scanTemp + 256);
+  if (e != hipSuccess) return e;
+  uint32_t* counts = (uint32_t*)st->pathCounts;
+  uint32_t* total = (uint32_t*)((char*)st->scratch + scanTemp);
+  LaunchLight<kModeBptWarmup>(p, R, numPaths, nullptr, nullptr, counts, stream);
+  e = LaunchExclusiveScanU32(counts, counts, numPaths, st->scratch, total, stream);
+  if (e != hipSuccess) return e;
+  uint32_t stored = 0;
+  e = hipMemcpyAsync(&stored, total, 4, hipMemcpyDeviceToHost, stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(stream);
+  if (e != hipSuccess) return e;
+  out->lightPathLengthSum = stored;
+  out->numLightCacheEntries = (uint64_t)numPaths + stored;
+  return hipSuccess;
+}
+
+hipError_t DrawBPT(IntegratorState* st, const DrawParams& p0, hipStream_t stream, std::string* err) {
+  DrawParams p = p0;
+  if (p.bptCapacity < 0) {
+    *err = "the light cache capacity is not set (run the warm-up first)";
+    return hipSuccess;
+  }
+  const int R = RaysPerLightSample(p.width, p.height, p.numLights);
+  const uint32_t numPaths = p.numLights * (uint32_t)R;
+  const size_t eyeThreads = (size_t)(p.width / kTile) * (p.height / kTile) * kTileThreads;
+  hipError_t e = PrepareSpill(st, &p, std::max<size_t>(numPaths, eyeThreads));
+  if (e != hipSuccess) return e;
+  e = EnsureBuffer(&st->lightCache, &st->lightCacheBytes, std::max<size_t>(64, (size_t)p.bptCapacity * 64));
+  if (e != hipSuccess) return e;
+  e = EnsureBuffer(&st->staging, &st->stagingBytes, (size_t)numPaths * p.maxPathLength * 64);
+  if (e != hipSuccess) return e;
+  e = EnsureBuffer(&st->pathCounts, &st->pathCountsBytes, (size_t)numPaths * 8);
+  if (e != hipSuccess) return e;
+  const size_t scanTemp = ExclusiveScanTempBytes(numPaths);
+  e = EnsureBuffer(&st->scratch, &st->scratchBytes, scanTemp + 256);
+  if (e != hipSuccess) return e;
+  LightCacheEntryDev* cache = (LightCacheEntryDev*)st->lightCache;
+  LightCacheEntryDev* staging = (LightCacheEntryDev*)st->staging;
+  uint32_t* counts = (uint32_t*)st->pathCounts;
+  uint32_t* offsets = counts + numPaths;
+  uint32_t* total = (uint32_t*)((char*)st->scratch + scanTemp);
+  LaunchLight<kModeBptLight>(p, R, numPaths, cache, staging, counts, stream);
+  e = LaunchExclusiveScanU32(counts, offsets, numPaths, st->scratch, total, stream);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_bpt_scatter, dim3((numPaths + 255) / 256), dim3(256), 0, stream, staging, counts, offsets,
+                     numPaths, p.maxPathLength, p.bptCapacity, cache);
+  // NumLightCacheEntries = R*N + appended (resetcounter.comp:9 plus the appends)
+  AddConstantU32(total, numPaths, stream);
+  st->lastTotal = total;
+  dim3 grid(p.width / kTile, p.height / kTile);
+  if (grid.x && grid.y) {
+    if (p.useWide) hipLaunchKernelGGL(k_bpt_eye<true>, grid, dim3(kTileThreads), 0, stream, p, cache, total);
+    else hipLaunchKernelGGL(k_bpt_eye<false>, grid, dim3(kTileThreads), 0, stream, p, cache, total);
+  }
+  return hipGetLastError();
+}
+
+}  // namespace gpugi

@@ -22,6 +22,9 @@ RAY_DTYPE = np.dtype([("origin", "<f4", 3), ("tmax", "<f4"), ("dir", "<f4", 3), 
 HIT_DTYPE = np.dtype([("slot", "<i4"), ("t", "<f4"), ("bary_y", "<f4"), ("bary_z", "<f4")])
 LIGHT_DTYPE = np.dtype([("position", "<f4", 3), ("normal_phi", "<f4"), ("intensity", "<f4", 3),
                         ("normal_theta_cos", "<f4")])
+LIGHT_CACHE_DTYPE = np.dtype([("flux", "<f4", 3), ("material", "<i4"), ("position", "<f4", 3), ("normal0", "<f4"),
+                              ("texcoord", "<f4", 2), ("normal1", "<f4"), ("path_length", "<i4"),
+                              ("incident", "<f4", 2), ("mis_n", "<f4"), ("mis_t", "<f4")])
 MATERIAL_DTYPE = np.dtype([("diffuse", "<f4", 3), ("opacity", "<f4", 3), ("reflectiveness", "<f4", 4),
                            ("emissivity", "<f4", 3), ("fresnel0", "<f4", 3), ("refraction_index_avg", "<f4"),
                            ("fresnel1", "<f4", 3)])
@@ -69,15 +72,23 @@ _SIGNATURES = {
     "gpugi_set_max_path_length": [_P, ctypes.c_int],
     "gpugi_set_integrator_kernel": [_P, ctypes.c_int],
     "gpugi_draw": [_P, ctypes.c_int, ctypes.c_uint32],
+    "gpugi_bpt_warmup": [_P, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)],
+    "gpugi_bpt_set_light_cache": [_P, ctypes.c_int32, ctypes.c_int32],
+    "gpugi_bpt_read_light_cache": [_P, _P, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32)],
     "gpugi_reset_accum": [_P],
     "gpugi_readback": [_P, _P, ctypes.c_size_t, ctypes.POINTER(ctypes.c_uint32)],
     "gpugi_trace": [_P, ctypes.c_int, ctypes.c_int, _P, _P, ctypes.c_size_t, ctypes.c_int],
     "gpugi_trace_counts": [_P, ctypes.c_int, _P, _P, _P, ctypes.c_size_t, ctypes.c_int],
+    "gpugi_set_wide_config": [_P, ctypes.c_int, ctypes.c_int],
+    "gpugi_get_wide_config": [_P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                              ctypes.POINTER(ctypes.c_uint32)],
+    "gpugi_autotune_trace": [_P, ctypes.c_int, _P, _P, ctypes.c_size_t, ctypes.c_int, ctypes.POINTER(ctypes.c_float)],
     "gpugi_sync": [_P],
     "gpugi_alloc": [_P, ctypes.c_size_t, ctypes.POINTER(ctypes.c_void_p)],
     "gpugi_free": [_P, _P],
     "gpugi_upload": [_P, _P, _P, ctypes.c_size_t],
     "gpugi_download": [_P, _P, _P, ctypes.c_size_t],
+    "gpugi_copy_device": [_P, _P, _P, ctypes.c_size_t],
     "gpugi_timer_begin": [_P],
     "gpugi_timer_end": [_P, ctypes.POINTER(ctypes.c_float)],
     "gpugi_raygen_primary": [_P, ctypes.POINTER(Camera), ctypes.c_int, ctypes.c_int, ctypes.c_uint32, _P,
@@ -295,6 +306,11 @@ class DeviceBuffer:
         self.ctx._call("gpugi_download", _ptr(out), ctypes.c_void_p(self.ptr), ctypes.c_size_t(out.nbytes))
         return out
 
+    def copy_from(self, other, nbytes):
+        assert nbytes <= self.nbytes and nbytes <= other.nbytes
+        self.ctx._call("gpugi_copy_device", ctypes.c_void_p(self.ptr), ctypes.c_void_p(other.ptr),
+                       ctypes.c_size_t(nbytes))
+
     def free(self):
         if self.ptr:
             self.ctx._call("gpugi_free", ctypes.c_void_p(self.ptr))
@@ -337,6 +353,20 @@ class Context:
     def draw(self, integrator, frame_seed):
         self._call("gpugi_draw", int(integrator), ctypes.c_uint32(frame_seed & 0xffffffff))
 
+    def bpt_warmup(self, frame_seed):
+        a, b = ctypes.c_uint64(), ctypes.c_uint64()
+        self._call("gpugi_bpt_warmup", ctypes.c_uint32(frame_seed & 0xffffffff), ctypes.byref(a), ctypes.byref(b))
+        return a.value, b.value
+
+    def bpt_set_light_cache(self, capacity, average_path_length=0):
+        self._call("gpugi_bpt_set_light_cache", int(capacity), int(average_path_length))
+
+    def bpt_read_light_cache(self, max_entries):
+        out = np.zeros(max_entries, dtype=LIGHT_CACHE_DTYPE)
+        n = ctypes.c_uint32()
+        self._call("gpugi_bpt_read_light_cache", _ptr(out), int(max_entries), ctypes.byref(n))
+        return out, n.value
+
     def reset_accum(self):
         self._call("gpugi_reset_accum")
 
@@ -355,6 +385,20 @@ class Context:
     def trace_device(self, rays_buf, hits_buf, n, query=CLOSEST_HIT, kernel=KERNEL_AUTO):
         self._call("gpugi_trace", int(query), int(kernel), ctypes.c_void_p(rays_buf.ptr),
                    ctypes.c_void_p(hits_buf.ptr), ctypes.c_size_t(n), 1)
+
+    def set_wide_config(self, max_leaf_triangles, lds_stack_entries):
+        self._call("gpugi_set_wide_config", int(max_leaf_triangles), int(lds_stack_entries))
+
+    def wide_config(self):
+        a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_uint32()
+        self._call("gpugi_get_wide_config", ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
+        return {"max_leaf_triangles": a.value, "lds_stack_entries": b.value, "max_stack": c.value}
+
+    def autotune_trace(self, rays_buf, hits_buf, n, query=CLOSEST_HIT, repeats=3):
+        ms = ctypes.c_float()
+        self._call("gpugi_autotune_trace", int(query), ctypes.c_void_p(rays_buf.ptr), ctypes.c_void_p(hits_buf.ptr),
+                   ctypes.c_size_t(n), int(repeats), ctypes.byref(ms))
+        return ms.value
 
     def trace_counts(self, rays, query=CLOSEST_HIT):
         rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)

@@ -163,6 +163,22 @@ int gpugi_set_integrator_kernel(gpugi_ctx* ctx, int kernel);
 /* One iteration of an integrator that adds into the accumulation buffer
  * (Renderer::Draw, renderer.hpp:42; FrameSeed UBO, globaluniforms.glsl:17-21). */
 int gpugi_draw(gpugi_ctx* ctx, int integrator, uint32_t frame_seed);
+/* One warm-up light pass of the bidirectional path tracer with the current
+ * light samples (lighttracer.comp with SAVE_LIGHT_CACHE_WARMUP): returns the
+ * NumLightCacheEntries and LightPathLengthSum counters the reference reads
+ * back in CreateLightCacheWithCapacityEstimate (bidirectionalpathtracer.cpp:65-104).
+ * Leaves the accumulation buffer untouched.  Blocks until the counts are known. */
+int gpugi_bpt_warmup(gpugi_ctx* ctx, uint32_t frame_seed, uint64_t* num_light_cache_entries,
+                     uint64_t* light_path_length_sum);
+/* LightCacheCapacity and AverageLightPathLength of the LightPathTrace UBO
+ * (bidirectionalpathtracer.cpp:106-132); allocates capacity x 64-byte entries.
+ * A BPT draw fails with GPUGI_ERR_STATE until this is set; changing the screen
+ * or the scene clears it. */
+int gpugi_bpt_set_light_cache(gpugi_ctx* ctx, int32_t capacity, int32_t average_light_path_length);
+/* Copies the light cache of the last BPT draw (LightCacheEntry, 64 bytes each,
+ * bidirpathtracer/lightcache.glsl:1-16): up to max_entries entries and the
+ * NumLightCacheEntries counter (which may exceed the capacity). */
+int gpugi_bpt_read_light_cache(gpugi_ctx* ctx, void* entries, uint32_t max_entries, uint32_t* num_entries);
 /* Clears the accumulation buffer (Texture2D::ClearToZero in SetRenderer,
  * renderersystem.hpp:73-86). */
 int gpugi_reset_accum(gpugi_ctx* ctx);
@@ -178,6 +194,18 @@ int gpugi_readback(gpugi_ctx* ctx, float* rgba, size_t bytes, uint32_t* iteratio
  * hits are copied back), device pointers otherwise (asynchronous). */
 int gpugi_trace(gpugi_ctx* ctx, int query, int kernel, const gpugi_ray* rays, gpugi_hit* hits,
                 size_t n, int rays_on_device);
+/* Configuration of the wide traversal: triangles per merged leaf (1..16; the
+ * BVH is re-packed when it changes) and LDS stack entries per lane of the
+ * trace kernel (8, 12, 16, 24 or 32; deeper entries spill to global memory).
+ * No reference counterpart: the GLSL path has one fixed layout
+ * (TRIANGLES_PER_LEAF, scenedata.glsl:5). */
+int gpugi_set_wide_config(gpugi_ctx* ctx, int max_leaf_triangles, int lds_stack_entries);
+int gpugi_get_wide_config(gpugi_ctx* ctx, int* max_leaf_triangles, int* lds_stack_entries, uint32_t* max_stack);
+/* Autotuner: times every (leaf size, stack depth) pair on the given device
+ * ray set (one warm-up launch + `repeats` timed launches each), keeps the
+ * fastest configuration active and returns its average launch time. */
+int gpugi_autotune_trace(gpugi_ctx* ctx, int query, const gpugi_ray* rays_dev, gpugi_hit* hits_dev, size_t n,
+                         int repeats, float* best_ms);
 /* Reference escape-walk visit counters (TRACERAY_DEBUG_VARS,
  * traceray.glsl:59-61,131-133): per ray, box tests and valid-slot triangle
  * tests.  box_tests / tri_tests are device arrays of n uint32 each when
@@ -192,6 +220,8 @@ int gpugi_alloc(gpugi_ctx* ctx, size_t bytes, void** device_ptr);
 int gpugi_free(gpugi_ctx* ctx, void* device_ptr);
 int gpugi_upload(gpugi_ctx* ctx, void* device_dst, const void* host_src, size_t bytes);
 int gpugi_download(gpugi_ctx* ctx, void* host_dst, const void* device_src, size_t bytes);
+/* Device-to-device copy on the context stream (asynchronous). */
+int gpugi_copy_device(gpugi_ctx* ctx, void* device_dst, const void* device_src, size_t bytes);
 
 /* HIP-event timer on the context stream (replaces the glFinish + clock()
  * bracket of RendererSystem::Draw, renderersystem.cpp:235-258). */

@@ -79,4 +79,20 @@ void or_raygen_shadow(const float* prim, const int32_t* prim_hits, size_t n_prim
 void or_pathtracer(const or_scene* s, const float* lights, uint32_t num_lights, const or_camera* c, int width,
                    int height, uint32_t frame_seed, int max_path_length, float* accum, int row_begin, int row_end);
 
+/* Light tracer draw: light paths then the light-sample splat (LightPathtracer::Draw). */
+void or_lighttracer(const or_scene* s, const float* lights, uint32_t num_lights, const or_camera* c, int width,
+                    int height, uint32_t frame_seed, int max_path_length, float* accum);
+/* BPT warm-up counters (bidirectionalpathtracer.cpp:65-104). */
+void or_bpt_warmup(const or_scene* s, const float* lights, uint32_t num_lights, const or_camera* c, int width,
+                   int height, uint32_t frame_seed, int max_path_length, uint64_t* num_entries,
+                   uint64_t* path_length_sum);
+/* BPT light pass: splats into accum, fills cache (capacity x 16 floats); returns NumLightCacheEntries. */
+uint32_t or_bpt_light_pass(const or_scene* s, const float* lights, uint32_t num_lights, const or_camera* c,
+                           int width, int height, uint32_t frame_seed, int max_path_length, int capacity,
+                           float* accum, float* cache);
+/* BPT eye pass over rows [row_begin, row_end). */
+void or_bpt_eye(const or_scene* s, const or_camera* c, int width, int height, uint32_t frame_seed,
+                int max_path_length, int capacity, const float* cache, uint32_t num_entries, float* accum,
+                int row_begin, int row_end);
+
 #endif

@@ -69,6 +69,18 @@ def lib():
         l.or_pathtracer.argtypes = [P, P, ctypes.c_uint32, P, ctypes.c_int, ctypes.c_int, ctypes.c_uint32,
                                     ctypes.c_int, P, ctypes.c_int, ctypes.c_int]
         l.or_pathtracer.restype = None
+        l.or_lighttracer.argtypes = [P, P, ctypes.c_uint32, P, ctypes.c_int, ctypes.c_int, ctypes.c_uint32,
+                                     ctypes.c_int, P]
+        l.or_lighttracer.restype = None
+        l.or_bpt_warmup.argtypes = [P, P, ctypes.c_uint32, P, ctypes.c_int, ctypes.c_int, ctypes.c_uint32,
+                                    ctypes.c_int, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)]
+        l.or_bpt_warmup.restype = None
+        l.or_bpt_light_pass.argtypes = [P, P, ctypes.c_uint32, P, ctypes.c_int, ctypes.c_int, ctypes.c_uint32,
+                                        ctypes.c_int, ctypes.c_int, P, P]
+        l.or_bpt_light_pass.restype = ctypes.c_uint32
+        l.or_bpt_eye.argtypes = [P, P, ctypes.c_int, ctypes.c_int, ctypes.c_uint32, ctypes.c_int, ctypes.c_int, P,
+                                 ctypes.c_uint32, P, ctypes.c_int, ctypes.c_int]
+        l.or_bpt_eye.restype = None
         _lib = l
     return _lib
 
@@ -165,6 +177,40 @@ def pathtrace(scene, lights, cam, w, h, frame_seed, max_path_length, accum=None,
     lib().or_pathtracer(scene.ptr, _p(lt), len(lt), ctypes.byref(c), w, h, frame_seed, max_path_length,
                         _p(accum), r0, r1)
     return accum
+
+
+def _lights(lights):
+    return np.ascontiguousarray(lights).view(np.float32).reshape(-1, 8)
+
+
+def lighttrace(scene, lights, cam, w, h, frame_seed, max_path_length, accum=None):
+    accum = np.zeros((h, w, 4), dtype=np.float32) if accum is None else accum
+    lt = _lights(lights)
+    c = camera(cam)
+    lib().or_lighttracer(scene.ptr, _p(lt), len(lt), ctypes.byref(c), w, h, frame_seed, max_path_length, _p(accum))
+    return accum
+
+
+def bpt_warmup(scene, lights, cam, w, h, frame_seed, max_path_length):
+    lt = _lights(lights)
+    c = camera(cam)
+    a, b = ctypes.c_uint64(), ctypes.c_uint64()
+    lib().or_bpt_warmup(scene.ptr, _p(lt), len(lt), ctypes.byref(c), w, h, frame_seed, max_path_length,
+                        ctypes.byref(a), ctypes.byref(b))
+    return a.value, b.value
+
+
+def bpt_draw(scene, lights, cam, w, h, frame_seed, max_path_length, capacity, accum=None):
+    """One BPT draw: light pass then eye pass.  Returns (accum, cache, NumLightCacheEntries)."""
+    accum = np.zeros((h, w, 4), dtype=np.float32) if accum is None else accum
+    lt = _lights(lights)
+    c = camera(cam)
+    cache = np.zeros((capacity, 16), dtype=np.float32)
+    n = lib().or_bpt_light_pass(scene.ptr, _p(lt), len(lt), ctypes.byref(c), w, h, frame_seed, max_path_length,
+                                capacity, _p(accum), _p(cache))
+    lib().or_bpt_eye(scene.ptr, ctypes.byref(c), w, h, frame_seed, max_path_length, capacity, _p(cache), n,
+                     _p(accum), 0, h)
+    return accum, cache, int(n)
 
 
 # ---- reference RNG built from the reference's own utilities/random.cpp ----

@@ -8,14 +8,7 @@
 #include "oracle.h"
 #include "oracle_vec.h"
 
-typedef struct {
-  float refl[4];
-  vec3 opacity, diffuse, emissivity, fresnel0;
-  float rIndex;
-  vec3 fresnel1;
-} or_material;
-
-static or_material mat_of(const or_scene* s, int id) {
+or_material or_mat_of(const or_scene* s, int id) {
   const float* m = s->materials + (size_t)id * 20;
   or_material r;
   r.diffuse = v3(m[0], m[1], m[2]);
@@ -124,7 +117,7 @@ vec3 or_bsdf(vec3 inc, vec3 exc, const or_material* m, vec3 N, float* pdf, int a
 }
 
 /* GetTriangleHitInfo (scenedata.glsl:94-105) */
-static vec3 or_shading_normal(const or_scene* s, const int32_t* tri, vec3 b) {
+vec3 or_shading_normal(const or_scene* s, const int32_t* tri, vec3 b) {
   const float* a = s->vertex_info + (size_t)tri[0] * 4;
   const float* c = s->vertex_info + (size_t)tri[1] * 4;
   const float* e = s->vertex_info + (size_t)tri[2] * 4;
@@ -133,7 +126,7 @@ static vec3 or_shading_normal(const or_scene* s, const int32_t* tri, vec3 b) {
   return vnormalize(n);
 }
 
-static int any_hit(const or_scene* s, vec3 o, vec3 d, float len) {
+int or_any_hit(const or_scene* s, vec3 o, vec3 d, float len) {
   int32_t slot;
   vec3 b;
   return or_trace_one(s, 1, o, d, &len, &slot, &b, NULL, NULL, NULL, NULL);
@@ -152,7 +145,7 @@ static vec3 or_direct_light(const or_scene* s, const float* lights, vec3 pos, ve
   vec3 I = v3(ls[4], ls[5], ls[6]);
   if (ls[7] > 1.5f) {
     vec3 o = vadd(vscale(0.0001f, dir), pos);
-    if (!any_hit(s, o, dir, dist - 0.0001f * 2.0f)) {
+    if (!or_any_hit(s, o, dir, dist - 0.0001f * 2.0f)) {
       vec3 irr = vscale(surfaceCos / distSq, I);
       float pdf;
       vec3 f = or_bsdf(view, dir, m, normal, &pdf, 0, 0);
@@ -163,7 +156,7 @@ static vec3 or_direct_light(const or_scene* s, const float* lights, vec3 pos, ve
     float k = fsat(vdot(ln, vneg(dir)));
     if (k > 0.0f) {
       vec3 o = vadd(vscale(0.0001f, dir), pos);
-      if (!any_hit(s, o, dir, dist - 0.0001f * 2.0f)) {
+      if (!or_any_hit(s, o, dir, dist - 0.0001f * 2.0f)) {
         float pdf;
         vec3 li = vscalev(I, k);
         vec3 f = or_bsdf(view, dir, m, normal, &pdf, 0, 0);
@@ -201,7 +194,7 @@ void or_pathtracer(const or_scene* s, const float* lights, uint32_t numLights, c
         }
         o = vadd(o, vscale(t, d));
         vec3 n = or_shading_normal(s, tri, b);
-        or_material m = mat_of(s, tri[3]);
+        or_material m = or_mat_of(s, tri[3]);
         if (!didNEE) color = vadd(color, vmul(thr, m.emissivity));
         didNEE = m.refl[3] < 500.0f;
         int idx = (int)(or_xorshift32(&seed) % numLights);

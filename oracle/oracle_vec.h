@@ -92,6 +92,20 @@ static inline vec3 or_primary_dir(float j0, float j1, int px, int py, int w, int
   return vnormalize(vadd(vadd(vscale(sx, U), vscale(sy, V)), W));
 }
 
+typedef struct {
+  float refl[4];
+  vec3 opacity, diffuse, emissivity, fresnel0;
+  float rIndex;
+  vec3 fresnel1;
+} or_material;
+
+or_material or_mat_of(const or_scene* s, int id);
+vec3 or_sample_bsdf(vec3 inc, const or_material* m, uint32_t* seed, vec3 N, vec3* tp, int adjoint, float* pdf,
+                    int* diffuseOut);
+vec3 or_bsdf(vec3 inc, vec3 exc, const or_material* m, vec3 N, float* pdf, int adjoint, int specularOnly);
+vec3 or_shading_normal(const or_scene* s, const int32_t* tri, vec3 b);
+int or_any_hit(const or_scene* s, vec3 o, vec3 d, float len);
+
 int or_triangle(vec3 o, vec3 d, vec3 p0, vec3 p1, vec3 p2, float* t, vec3* b, vec3* n);
 int or_trace_one(const or_scene* s, int any, vec3 o, vec3 d, float* rayLength, int32_t* slot, vec3* bary,
                  vec3* triNormal, int32_t tri[4], uint32_t* nbox, uint32_t* ntri);
