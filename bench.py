@@ -38,6 +38,8 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--extra", action="store_true", help="also time coherent / shadow sets and the reference walk")
     p.add_argument("--no-autotune", action="store_true", help="keep the default wide-kernel configuration")
+    p.add_argument("--sweep", action="store_true",
+                   help="print Grays/s of every wide-kernel configuration on the three ray sets and exit")
     p.add_argument("--calibrate", action="store_true",
                    help="append 3 device-to-device copies of 1 GiB (counter calibration for rocprofv3 --pmc)")
     return p.parse_args()
@@ -85,6 +87,25 @@ def main():
     n = args.rays
     R = workloads.HeightfieldRays(ctx, n=n)
     kernel = rt.KERNEL_WIDE if args.kernel == "wide" else rt.KERNEL_REFERENCE
+
+    if args.sweep:
+        sets = (("incoherent", R.incoherent, rt.CLOSEST_HIT), ("coherent", R.primary, rt.CLOSEST_HIT),
+                ("shadow_any", R.shadow, rt.ANY_HIT))
+        for leaf in (1, 2, 4, 8):
+            for stack in (4, 8, 12, 16, 24):
+                for refill in (8, 16, 32, 64):
+                    ctx.set_wide_config(leaf, stack, refill)
+                    row = {}
+                    for name, buf, q in sets:
+                        ctx.trace_device(buf, R.hits, n, q, rt.KERNEL_WIDE)
+                        ctx.sync()
+                        ctx.timer_begin()
+                        for _ in range(3):
+                            ctx.trace_device(buf, R.hits, n, q, rt.KERNEL_WIDE)
+                        row[name] = round(n / (ctx.timer_end() / 3 * 1e-3) / 1e9, 3)
+                    print(json.dumps(dict(ctx.wide_config(), **row)), flush=True)
+        ctx.close()
+        return
 
     # Autotune the wide kernel on a separate incoherent sample (other seed), untimed.
     tuned = None

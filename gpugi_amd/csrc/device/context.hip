@@ -37,8 +37,9 @@ struct gpugi_ctx {
   void* dCounters = nullptr;  // [0] ray counter, [1] error flag, [2] hit count
   size_t spillBytes = 0;
   uint32_t wideMaxStack = 0;
-  int wideLeaf = 4;     // triangles per merged wide leaf
-  int wideStack = 16;   // LDS stack entries per lane of the trace kernel
+  int wideLeaf = 2;     // triangles per merged wide leaf
+  int wideStack = 8;    // LDS stack entries per lane of the trace kernel
+  int wideRefill = 16;  // idle lanes that trigger a wave-level ray refill
   // host copy of the reference layout (re-packing after a configuration change)
   std::vector<float> hHier, hPos;
   std::vector<int32_t> hTris;
@@ -124,7 +125,9 @@ int PrepareTraceWide(gpugi_ctx* ctx, int stack, WideScene* out) {
   HIP_TRY(ctx, WideBlocksPerCU(stack, &blocksPerCU));
   if (blocksPerCU < 1) blocksPerCU = 1;
   WideScene s = MakeWide(ctx, stack);
-  s.numThreads = (uint32_t)(ctx->numCUs * blocksPerCU * kWideBlock);
+  // whole blocks per XCD for the static schedule of the trace kernel
+  const uint32_t blocks = std::max<uint32_t>(8, (uint32_t)(ctx->numCUs * blocksPerCU) / 8 * 8);
+  s.numThreads = blocks * (uint32_t)kWideBlock;
   const size_t bytes = (size_t)s.numThreads * s.spillDepth * sizeof(uint2);
   if (bytes > ctx->spillBytes) {
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
@@ -469,7 +472,7 @@ int gpugi_trace(gpugi_ctx* ctx, int query, int kernel, const gpugi_ray* rays, gp
   if (wide) {
     WideScene ws;
     if (int rc = PrepareTraceWide(ctx, ctx->wideStack, &ws)) return rc;
-    HIP_TRY(ctx, LaunchTraceWide(ws, ctx->wideStack, query, dr, dh, n, (uint32_t*)ctx->dCounters, ctx->stream));
+    HIP_TRY(ctx, LaunchTraceWide(ws, ctx->wideStack, query, dr, dh, n, ctx->wideRefill, ctx->stream));
   } else {
     HIP_TRY(ctx, LaunchTraceRef(MakeRef(ctx), query, dr, dh, n, ctx->stream));
   }
@@ -483,13 +486,16 @@ int gpugi_trace(gpugi_ctx* ctx, int query, int kernel, const gpugi_ray* rays, gp
   return GPUGI_OK;
 }
 
-int gpugi_set_wide_config(gpugi_ctx* ctx, int max_leaf_triangles, int lds_stack_entries) {
+int gpugi_set_wide_config(gpugi_ctx* ctx, int max_leaf_triangles, int lds_stack_entries, int refill_lanes) {
   if (!ctx) return Fail(ctx, GPUGI_ERR_ARG, "gpugi_set_wide_config: null context");
   bool stackOk = false;
   for (int v : kWideStackVariants) stackOk |= v == lds_stack_entries;
-  if (!stackOk || max_leaf_triangles < 1 || max_leaf_triangles > (int)kWideMaxLeafTris)
-    return Fail(ctx, GPUGI_ERR_ARG, "gpugi_set_wide_config: leaf size in [1, 16], stack one of 8/12/16/24/32");
+  if (!stackOk || max_leaf_triangles < 1 || max_leaf_triangles > (int)kWideMaxLeafTris || refill_lanes < 1 ||
+      refill_lanes > 64)
+    return Fail(ctx, GPUGI_ERR_ARG,
+                "gpugi_set_wide_config: leaf size in [1, 16], stack one of 4/8/12/16/24, refill in [1, 64]");
   ctx->wideStack = lds_stack_entries;
+  ctx->wideRefill = refill_lanes;
   if (max_leaf_triangles != ctx->wideLeaf) {
     ctx->wideLeaf = max_leaf_triangles;
     if (ctx->hasScene) {
@@ -500,10 +506,12 @@ int gpugi_set_wide_config(gpugi_ctx* ctx, int max_leaf_triangles, int lds_stack_
   return GPUGI_OK;
 }
 
-int gpugi_get_wide_config(gpugi_ctx* ctx, int* max_leaf_triangles, int* lds_stack_entries, uint32_t* max_stack) {
+int gpugi_get_wide_config(gpugi_ctx* ctx, int* max_leaf_triangles, int* lds_stack_entries, int* refill_lanes,
+                          uint32_t* max_stack) {
   if (!ctx) return Fail(ctx, GPUGI_ERR_ARG, "gpugi_get_wide_config: null context");
   if (max_leaf_triangles) *max_leaf_triangles = ctx->wideLeaf;
   if (lds_stack_entries) *lds_stack_entries = ctx->wideStack;
+  if (refill_lanes) *refill_lanes = ctx->wideRefill;
   if (max_stack) *max_stack = ctx->wideMaxStack;
   return GPUGI_OK;
 }
@@ -517,28 +525,30 @@ int gpugi_autotune_trace(gpugi_ctx* ctx, int query, const gpugi_ray* rays_dev, g
     return Fail(ctx, GPUGI_ERR_ARG, "gpugi_autotune_trace: bad query");
   if (SetDev(ctx)) return GPUGI_ERR_HIP;
   const int leaves[] = {1, 2, 4, 8};
+  const int refills[] = {8, 16, 32, 64};
   float best = INFINITY;
-  int bestLeaf = ctx->wideLeaf, bestStack = ctx->wideStack;
+  int bestLeaf = ctx->wideLeaf, bestStack = ctx->wideStack, bestRefill = ctx->wideRefill;
   for (int leaf : leaves) {
-    if (int rc = gpugi_set_wide_config(ctx, leaf, ctx->wideStack)) return rc;
+    if (int rc = gpugi_set_wide_config(ctx, leaf, ctx->wideStack, ctx->wideRefill)) return rc;
     for (int stack : kWideStackVariants) {
-      ctx->wideStack = stack;
       WideScene ws;
       if (int rc = PrepareTraceWide(ctx, stack, &ws)) return rc;
-      // one untimed launch, then the median-free average of `repeats`
-      HIP_TRY(ctx, LaunchTraceWide(ws, stack, query, rays_dev, hits_dev, n, (uint32_t*)ctx->dCounters, ctx->stream));
-      HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-      for (int r = 0; r < repeats; ++r)
-        HIP_TRY(ctx, LaunchTraceWide(ws, stack, query, rays_dev, hits_dev, n, (uint32_t*)ctx->dCounters, ctx->stream));
-      HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
-      HIP_TRY(ctx, hipEventSynchronize(ctx->ev1));
-      float ms = 0.0f;
-      HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
-      ms /= (float)repeats;
-      if (ms < best) { best = ms; bestLeaf = leaf; bestStack = stack; }
+      for (int refill : refills) {
+        // one untimed launch, then the average of `repeats`
+        HIP_TRY(ctx, LaunchTraceWide(ws, stack, query, rays_dev, hits_dev, n, refill, ctx->stream));
+        HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+        for (int r = 0; r < repeats; ++r)
+          HIP_TRY(ctx, LaunchTraceWide(ws, stack, query, rays_dev, hits_dev, n, refill, ctx->stream));
+        HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+        HIP_TRY(ctx, hipEventSynchronize(ctx->ev1));
+        float ms = 0.0f;
+        HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        ms /= (float)repeats;
+        if (ms < best) { best = ms; bestLeaf = leaf; bestStack = stack; bestRefill = refill; }
+      }
     }
   }
-  if (int rc = gpugi_set_wide_config(ctx, bestLeaf, bestStack)) return rc;
+  if (int rc = gpugi_set_wide_config(ctx, bestLeaf, bestStack, bestRefill)) return rc;
   if (best_ms) *best_ms = best;
   return GPUGI_OK;
 }

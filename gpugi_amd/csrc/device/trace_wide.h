@@ -1,5 +1,11 @@
 // Ordered traversal of the 4-wide quantized BVH (wide_bvh.h), one ray per lane.
 //
+// The walk is a state machine: wide_step() visits one node (four child boxes,
+// the triangles of hit leaf children, ordered push of hit inner children) and
+// reports whether the ray is finished.  trace_wide_ray() runs it to the end for
+// one ray; the persistent trace kernel interleaves steps with wave-level ray
+// refills instead.
+//
 // Per lane, a short stack of (node, entry distance) pairs lives in LDS laid
 // out [depth][thread], so lanes at different depths touch different banks;
 // deeper entries spill to a per-thread slice of global memory whose size the
@@ -29,6 +35,17 @@ struct WideResult {
   bool any;    // any-hit result
 };
 
+struct WideState {
+  gg::V3 o, d;
+  float ix, iy, iz;
+  float tbest;
+  int bestSlot;
+  float bu, bv;
+  uint32_t node;
+  int sp;
+  bool found;
+};
+
 __device__ __forceinline__ float safe_inv(float d) {
   const float tiny = 1e-20f;
   return 1.0f / (fabsf(d) > tiny ? d : copysignf(tiny, d));
@@ -36,101 +53,111 @@ __device__ __forceinline__ float safe_inv(float d) {
 
 __device__ __forceinline__ float byte_f(uint32_t w, int k) { return (float)((w >> (8 * k)) & 0xffu); }
 
-// stkNode/stkT: LDS arrays of S * B entries; tid in [0, B).
+__device__ __forceinline__ void wide_init(WideState& st, gg::V3 o, gg::V3 d, float tmax) {
+  st.o = o;
+  st.d = d;
+  st.ix = safe_inv(d.x);
+  st.iy = safe_inv(d.y);
+  st.iz = safe_inv(d.z);
+  st.tbest = tmax;
+  st.bestSlot = -1;
+  st.bu = st.bv = 0.0f;
+  st.node = 0;
+  st.sp = 0;
+  st.found = false;
+}
+
+// One node visit.  Returns true when the ray is finished.
 template <bool ANY, int B, int S>
-__device__ __forceinline__ WideResult trace_wide_ray(const WideScene& s, gg::V3 o, gg::V3 d, float tmax,
-                                                     uint32_t* stkNode, float* stkT, int tid, uint2* spill) {
-  const float ix = safe_inv(d.x), iy = safe_inv(d.y), iz = safe_inv(d.z);
-  float tbest = tmax;
-  int bestSlot = -1;
-  float bu = 0.0f, bv = 0.0f;
-  bool found = false;
-  uint32_t node = 0;
-  int sp = 0;
-  for (;;) {
-    const uint4 q0 = s.nodes[4 * (size_t)node + 0];
-    const uint4 q1 = s.nodes[4 * (size_t)node + 1];
-    const uint4 q2 = s.nodes[4 * (size_t)node + 2];
-    const uint4 q3 = s.nodes[4 * (size_t)node + 3];
-    const float ox = __uint_as_float(q0.x), oy = __uint_as_float(q0.y), oz = __uint_as_float(q0.z);
-    const float sx = __uint_as_float((q0.w & 0xffu) << 23);
-    const float sy = __uint_as_float(((q0.w >> 8) & 0xffu) << 23);
-    const float sz = __uint_as_float(((q0.w >> 16) & 0xffu) << 23);
-    const int nchild = (int)(q0.w >> 24);
-    const uint32_t childs[4] = {q1.x, q1.y, q1.z, q1.w};
-    float tn[4];
-    bool hit[4];
+__device__ __forceinline__ bool wide_step(const WideScene& s, WideState& st, uint32_t* stkNode, float* stkT, int tid,
+                                          uint2* spill) {
+  const gg::V3 o = st.o, d = st.d;
+  const uint32_t node = st.node;
+  const uint4 q0 = s.nodes[4 * (size_t)node + 0];
+  const uint4 q1 = s.nodes[4 * (size_t)node + 1];
+  const uint4 q2 = s.nodes[4 * (size_t)node + 2];
+  const uint4 q3 = s.nodes[4 * (size_t)node + 3];
+  const float ox = __uint_as_float(q0.x), oy = __uint_as_float(q0.y), oz = __uint_as_float(q0.z);
+  const float sx = __uint_as_float((q0.w & 0xffu) << 23);
+  const float sy = __uint_as_float(((q0.w >> 8) & 0xffu) << 23);
+  const float sz = __uint_as_float(((q0.w >> 16) & 0xffu) << 23);
+  const int nchild = (int)(q0.w >> 24);
+  const uint32_t childs[4] = {q1.x, q1.y, q1.z, q1.w};
+  float tn[4];
+  bool hit[4];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      const float lox = __builtin_fmaf(byte_f(q2.x, k), sx, ox);
-      const float loy = __builtin_fmaf(byte_f(q2.y, k), sy, oy);
-      const float loz = __builtin_fmaf(byte_f(q2.z, k), sz, oz);
-      const float hix = __builtin_fmaf(byte_f(q2.w, k), sx, ox);
-      const float hiy = __builtin_fmaf(byte_f(q3.x, k), sy, oy);
-      const float hiz = __builtin_fmaf(byte_f(q3.y, k), sz, oz);
-      const float tx0 = (lox - o.x) * ix, tx1 = (hix - o.x) * ix;
-      const float ty0 = (loy - o.y) * iy, ty1 = (hiy - o.y) * iy;
-      const float tz0 = (loz - o.z) * iz, tz1 = (hiz - o.z) * iz;
-      const float tnear = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.0f));
-      const float tfar = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1)) * 1.00000036f;
-      tn[k] = tnear;
-      hit[k] = (k < nchild) && tnear <= tfar && tnear <= tbest;
-    }
-    // Leaf children first: they can only shorten tbest.
+  for (int k = 0; k < 4; ++k) {
+    const float lox = __builtin_fmaf(byte_f(q2.x, k), sx, ox);
+    const float loy = __builtin_fmaf(byte_f(q2.y, k), sy, oy);
+    const float loz = __builtin_fmaf(byte_f(q2.z, k), sz, oz);
+    const float hix = __builtin_fmaf(byte_f(q2.w, k), sx, ox);
+    const float hiy = __builtin_fmaf(byte_f(q3.x, k), sy, oy);
+    const float hiz = __builtin_fmaf(byte_f(q3.y, k), sz, oz);
+    const float tx0 = (lox - o.x) * st.ix, tx1 = (hix - o.x) * st.ix;
+    const float ty0 = (loy - o.y) * st.iy, ty1 = (hiy - o.y) * st.iy;
+    const float tz0 = (loz - o.z) * st.iz, tz1 = (hiz - o.z) * st.iz;
+    const float tnear = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.0f));
+    const float tfar = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1)) * 1.00000036f;
+    tn[k] = tnear;
+    hit[k] = (k < nchild) && tnear <= tfar && tnear <= st.tbest;
+  }
+  // Leaf children first: they can only shorten tbest.
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      if (hit[k] && (childs[k] & kWideLeafFlag)) {
-        const uint32_t first = childs[k] & kWideLeafFirstMask;
-        const uint32_t cnt = ((childs[k] >> 27) & 0xfu) + 1u;
-        for (uint32_t m = 0; m < cnt; ++m) {
-          const float4 a = s.tris[3 * (size_t)(first + m) + 0];
-          const float4 b = s.tris[3 * (size_t)(first + m) + 1];
-          const float4 c = s.tris[3 * (size_t)(first + m) + 2];
-          const gg::V3 p0 = gg::v3(a.x, a.y, a.z);
-          const gg::V3 e0 = gg::v3(b.x, b.y, b.z);
-          const gg::V3 e1 = gg::v3(c.x, c.y, c.z);
-          const int slot = __float_as_int(a.w);
-          const gg::V3 nn = gg::cross(e1, e0);
-          const gg::V3 e2 = (1.0f / gg::dot(nn, d)) * (p0 - o);
-          const gg::V3 ii = gg::cross(d, e2);
-          const float by = gg::dot(ii, e1);
-          const float bz = gg::dot(ii, e0);
-          const float bx = 1.0f - (bz + by);
-          const float t = gg::dot(nn, e2);
-          const bool ok = t > 0.0001f && bx >= 0.0f && by >= 0.0f && bz >= 0.0f;
-          if (ANY) {
-            if (ok && t < tbest) { found = true; break; }
-          } else if (ok && (t < tbest || (t == tbest && slot < bestSlot))) {
-            tbest = t;
-            bestSlot = slot;
-            bu = by;
-            bv = bz;
-          }
+  for (int k = 0; k < 4; ++k) {
+    if (hit[k] && (childs[k] & kWideLeafFlag)) {
+      const uint32_t first = childs[k] & kWideLeafFirstMask;
+      const uint32_t cnt = ((childs[k] >> 27) & 0xfu) + 1u;
+      for (uint32_t m = 0; m < cnt; ++m) {
+        const float4 a = s.tris[3 * (size_t)(first + m) + 0];
+        const float4 b = s.tris[3 * (size_t)(first + m) + 1];
+        const float4 c = s.tris[3 * (size_t)(first + m) + 2];
+        const gg::V3 p0 = gg::v3(a.x, a.y, a.z);
+        const gg::V3 e0 = gg::v3(b.x, b.y, b.z);
+        const gg::V3 e1 = gg::v3(c.x, c.y, c.z);
+        const int slot = __float_as_int(a.w);
+        const gg::V3 nn = gg::cross(e1, e0);
+        const gg::V3 e2 = (1.0f / gg::dot(nn, d)) * (p0 - o);
+        const gg::V3 ii = gg::cross(d, e2);
+        const float by = gg::dot(ii, e1);
+        const float bz = gg::dot(ii, e0);
+        const float bx = 1.0f - (bz + by);
+        const float t = gg::dot(nn, e2);
+        const bool ok = t > 0.0001f && bx >= 0.0f && by >= 0.0f && bz >= 0.0f;
+        if (ANY) {
+          if (ok && t < st.tbest) { st.found = true; break; }
+        } else if (ok && (t < st.tbest || (t == st.tbest && slot < st.bestSlot))) {
+          st.tbest = t;
+          st.bestSlot = slot;
+          st.bu = by;
+          st.bv = bz;
         }
       }
-      if (ANY && found) break;
     }
-    if (ANY && found) break;
-    // Inner children, nearest first.
-    float t0 = (hit[0] && !(childs[0] & kWideLeafFlag) && tn[0] <= tbest) ? tn[0] : INFINITY;
-    float t1 = (hit[1] && !(childs[1] & kWideLeafFlag) && tn[1] <= tbest) ? tn[1] : INFINITY;
-    float t2 = (hit[2] && !(childs[2] & kWideLeafFlag) && tn[2] <= tbest) ? tn[2] : INFINITY;
-    float t3 = (hit[3] && !(childs[3] & kWideLeafFlag) && tn[3] <= tbest) ? tn[3] : INFINITY;
-    uint32_t c0 = childs[0], c1 = childs[1], c2 = childs[2], c3 = childs[3];
-    if (!ANY) {
-#define GPUGI_CSWAP(ta, ca, tb, cb)     \
-  if (tb < ta) {                        \
-    float tt = ta; ta = tb; tb = tt;    \
+    if (ANY && st.found) break;
+  }
+  if (ANY && st.found) return true;
+  // Inner children, nearest first.
+  const float tb = st.tbest;
+  float t0 = (hit[0] && !(childs[0] & kWideLeafFlag) && tn[0] <= tb) ? tn[0] : INFINITY;
+  float t1 = (hit[1] && !(childs[1] & kWideLeafFlag) && tn[1] <= tb) ? tn[1] : INFINITY;
+  float t2 = (hit[2] && !(childs[2] & kWideLeafFlag) && tn[2] <= tb) ? tn[2] : INFINITY;
+  float t3 = (hit[3] && !(childs[3] & kWideLeafFlag) && tn[3] <= tb) ? tn[3] : INFINITY;
+  uint32_t c0 = childs[0], c1 = childs[1], c2 = childs[2], c3 = childs[3];
+  if (!ANY) {
+#define GPUGI_CSWAP(ta, ca, tb_, cb)    \
+  if (tb_ < ta) {                       \
+    float tt = ta; ta = tb_; tb_ = tt;  \
     uint32_t cc = ca; ca = cb; cb = cc; \
   }
-      GPUGI_CSWAP(t0, c0, t1, c1);
-      GPUGI_CSWAP(t2, c2, t3, c3);
-      GPUGI_CSWAP(t0, c0, t2, c2);
-      GPUGI_CSWAP(t1, c1, t3, c3);
-      GPUGI_CSWAP(t1, c1, t2, c2);
+    GPUGI_CSWAP(t0, c0, t1, c1);
+    GPUGI_CSWAP(t2, c2, t3, c3);
+    GPUGI_CSWAP(t0, c0, t2, c2);
+    GPUGI_CSWAP(t1, c1, t3, c3);
+    GPUGI_CSWAP(t1, c1, t2, c2);
 #undef GPUGI_CSWAP
-    }
-    bool overflow = false;
+  }
+  bool overflow = false;
+  int sp = st.sp;
 #define GPUGI_PUSH(nd, tv)                                   \
   do {                                                       \
     if (sp < S) {                                            \
@@ -144,44 +171,62 @@ __device__ __forceinline__ WideResult trace_wide_ray(const WideScene& s, gg::V3 
       overflow = true;                                       \
     }                                                        \
   } while (0)
-    // Sorted (closest hit): finite entries form a prefix, c0 is the nearest.
-    // Unsorted (any hit): the first finite entry is taken next.
-    uint32_t next = 0xffffffffu;
-    if (t3 != INFINITY) { if (ANY && next == 0xffffffffu) next = c3; else GPUGI_PUSH(c3, t3); }
-    if (t2 != INFINITY) { if (ANY && next == 0xffffffffu) next = c2; else GPUGI_PUSH(c2, t2); }
-    if (t1 != INFINITY) { if (ANY && next == 0xffffffffu) next = c1; else GPUGI_PUSH(c1, t1); }
-    if (t0 != INFINITY) { if (next == 0xffffffffu) next = c0; else GPUGI_PUSH(c0, t0); }
+  // Sorted (closest hit): finite entries form a prefix, c0 is the nearest.
+  // Unsorted (any hit): the first finite entry is taken next.
+  uint32_t next = 0xffffffffu;
+  if (t3 != INFINITY) { if (ANY && next == 0xffffffffu) next = c3; else GPUGI_PUSH(c3, t3); }
+  if (t2 != INFINITY) { if (ANY && next == 0xffffffffu) next = c2; else GPUGI_PUSH(c2, t2); }
+  if (t1 != INFINITY) { if (ANY && next == 0xffffffffu) next = c1; else GPUGI_PUSH(c1, t1); }
+  if (t0 != INFINITY) { if (next == 0xffffffffu) next = c0; else GPUGI_PUSH(c0, t0); }
 #undef GPUGI_PUSH
-    if (overflow) {  // impossible when the host sized the spill from the tree; fail loudly
-      atomicOr(s.errorFlag, 1u);
-      bestSlot = -2;
-      break;
+  if (overflow) {  // impossible when the host sized the spill from the tree; fail loudly
+    atomicOr(s.errorFlag, 1u);
+    st.bestSlot = -2;
+    st.sp = sp;
+    return true;
+  }
+  if (next != 0xffffffffu) {
+    st.node = next;
+    st.sp = sp;
+    return false;
+  }
+  // Pop the nearest pending node still in front of the closest hit.
+  while (sp > 0) {
+    --sp;
+    uint32_t nd;
+    float t;
+    if (sp < S) {
+      nd = stkNode[sp * B + tid];
+      t = stkT[sp * B + tid];
+    } else {
+      const uint2 e = spill[sp - S];
+      nd = e.x;
+      t = __uint_as_float(e.y);
     }
-    if (next != 0xffffffffu) { node = next; continue; }
-    // Pop the nearest pending node still in front of the closest hit.
-    bool have = false;
-    while (sp > 0) {
-      --sp;
-      uint32_t nd;
-      float t;
-      if (sp < S) {
-        nd = stkNode[sp * B + tid];
-        t = stkT[sp * B + tid];
-      } else {
-        const uint2 e = spill[sp - S];
-        nd = e.x;
-        t = __uint_as_float(e.y);
-      }
-      if (t <= tbest) { node = nd; have = true; break; }
+    if (t <= st.tbest) {
+      st.node = nd;
+      st.sp = sp;
+      return false;
     }
-    if (!have) break;
+  }
+  st.sp = 0;
+  return true;
+}
+
+// stkNode/stkT: LDS arrays of S * B entries; tid in [0, B).
+template <bool ANY, int B, int S>
+__device__ __forceinline__ WideResult trace_wide_ray(const WideScene& s, gg::V3 o, gg::V3 d, float tmax,
+                                                     uint32_t* stkNode, float* stkT, int tid, uint2* spill) {
+  WideState st;
+  wide_init(st, o, d, tmax);
+  while (!wide_step<ANY, B, S>(s, st, stkNode, stkT, tid, spill)) {
   }
   WideResult r;
-  r.t = tbest;
-  r.slot = bestSlot;
-  r.by = bu;
-  r.bz = bv;
-  r.any = found;
+  r.t = st.tbest;
+  r.slot = st.bestSlot;
+  r.by = st.bu;
+  r.bz = st.bv;
+  r.any = st.found;
   return r;
 }
 

@@ -1,11 +1,17 @@
 // Persistent-wave ray queries on the 4-wide quantized BVH.
 //
-// Each wave takes 64 rays at a time from a global counter (one atomic per
-// wave), so the grid is sized to the machine rather than to the ray count and
-// the per-thread overflow stack has a fixed size.  Hit records are written as
-// one 16-byte store per lane, 1 KiB contiguous per wave.  The LDS stack depth
-// is a template parameter: deeper stacks spill less but cap the waves per CU
-// (LDS is 160 KiB per CU); the autotuner picks the depth per scene.
+// Scheduling is static and atomic-free: workgroups are dispatched round-robin
+// over the 8 XCDs, so block b runs on XCD b % 8; XCD x owns the x-th
+// contiguous eighth of the ray array (neighbouring rays start close together,
+// so they share that XCD's L2), and within it every wave walks its own
+// interleaved sequence of 64-ray batches.
+//
+// Each wave steps its 64 lanes one node visit at a time.  When at least
+// `refill` lanes have finished (ballot + popcount), the idle lanes take the
+// next rays of the wave's sequence in one go, so a single long ray no longer
+// holds 63 idle lanes; refill = 64 reproduces batch-at-a-time processing.
+// The LDS stack depth is a template parameter: deeper stacks spill less but
+// cap the waves per CU (160 KiB LDS per CU); the autotuner picks it.
 // Compiled with -ffp-contract=off: the triangle test must round exactly like
 // the reference formula.
 #include <hip/hip_runtime.h>
@@ -15,32 +21,62 @@
 
 namespace gpugi {
 
+constexpr uint32_t kNumXcd = 8;
+
 template <bool ANY, int B, int S>
 __global__ __launch_bounds__(B) void k_trace_wide(WideScene s, const float4* __restrict__ rays,
-                                                  int4* __restrict__ hits, uint32_t n,
-                                                  uint32_t* __restrict__ counter) {
+                                                  int4* __restrict__ hits, uint32_t n, int refill) {
   __shared__ uint32_t stkNode[S * B];
   __shared__ float stkT[S * B];
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const uint32_t gtid = blockIdx.x * B + tid;
   uint2* spill = s.spill + (size_t)gtid * s.spillDepth;
+
+  const uint32_t wavesPerBlock = B / 64;
+  const uint32_t xcd = blockIdx.x % kNumXcd;
+  const uint32_t wavesX = (gridDim.x / kNumXcd) * wavesPerBlock;
+  const uint32_t waveX = (blockIdx.x / kNumXcd) * wavesPerBlock + (uint32_t)(tid >> 6);
+  const uint32_t batches = (n + 63u) / 64u;
+  const uint32_t perX = (batches + kNumXcd - 1) / kNumXcd;
+  const uint32_t bBegin = xcd * perX;
+  const uint32_t bEnd = min(batches, bBegin + perX);
+  const uint64_t below = (1ull << lane) - 1ull;
+
+  uint32_t cursor = 0;  // rays of this wave's sequence handed out so far (wave-uniform)
+  bool exhausted = bBegin + waveX >= bEnd;
+  bool active = false;
+  uint32_t ri = 0;
+  WideState st;
   for (;;) {
-    uint32_t base = 0;
-    if (lane == 0) base = atomicAdd(counter, 64u);
-    base = __shfl(base, 0);
-    if (base >= n) break;
-    const uint32_t ri = base + (uint32_t)lane;
-    if (ri >= n) continue;
-    const float4 r0 = rays[2 * (size_t)ri], r1 = rays[2 * (size_t)ri + 1];
-    const WideResult r = trace_wide_ray<ANY, B, S>(s, gg::v3(r0.x, r0.y, r0.z), gg::v3(r1.x, r1.y, r1.z), r0.w,
-                                                   stkNode, stkT, tid, spill);
-    if (ANY) hits[ri] = make_int4(r.any ? 1 : 0, 0, 0, 0);
-    else hits[ri] = make_int4(r.slot, __float_as_int(r.t), __float_as_int(r.by), __float_as_int(r.bz));
+    const uint64_t idle = __ballot(!active);
+    const int nidle = __popcll(idle);
+    if (!exhausted && (nidle >= refill || nidle == 64)) {
+      if (!active) {
+        const uint32_t pos = cursor + (uint32_t)__popcll(idle & below);
+        const uint32_t batch = bBegin + waveX + (pos / 64u) * wavesX;
+        const uint32_t r = batch * 64u + (pos % 64u);
+        if (batch < bEnd && r < n) {
+          const float4 r0 = rays[2 * (size_t)r], r1 = rays[2 * (size_t)r + 1];
+          wide_init(st, gg::v3(r0.x, r0.y, r0.z), gg::v3(r1.x, r1.y, r1.z), r0.w);
+          ri = r;
+          active = true;
+        }
+      }
+      cursor += (uint32_t)nidle;
+      if (bBegin + waveX + (cursor / 64u) * wavesX >= bEnd) exhausted = true;
+    }
+    if (__ballot(active) == 0) {
+      if (exhausted) break;
+      continue;
+    }
+    if (active && wide_step<ANY, B, S>(s, st, stkNode, stkT, tid, spill)) {
+      if (ANY) hits[ri] = make_int4(st.found ? 1 : 0, 0, 0, 0);
+      else hits[ri] = make_int4(st.bestSlot, __float_as_int(st.tbest), __float_as_int(st.bu), __float_as_int(st.bv));
+      active = false;
+    }
   }
 }
-
-__global__ void k_zero_u32(uint32_t* p) { *p = 0u; }
 
 namespace {
 
@@ -57,28 +93,28 @@ Variant MakeVariant() {
 }
 
 const Variant* FindVariant(int stack) {
-  static const Variant table[] = {MakeVariant<8>(), MakeVariant<12>(), MakeVariant<16>(), MakeVariant<24>(),
-                                  MakeVariant<32>()};
+  static const Variant table[] = {MakeVariant<4>(), MakeVariant<8>(), MakeVariant<12>(), MakeVariant<16>(),
+                                  MakeVariant<24>()};
   for (const Variant& v : table)
     if (v.stack == stack) return &v;
   return nullptr;
 }
 
 template <int S>
-void LaunchS(const WideScene& s, int query, const gpugi_ray* rays, gpugi_hit* hits, size_t n, uint32_t* counter,
+void LaunchS(const WideScene& s, int query, const gpugi_ray* rays, gpugi_hit* hits, size_t n, int refill,
              hipStream_t stream) {
   dim3 grid(s.numThreads / kWideBlock);
   if (query == GPUGI_ANY_HIT)
     hipLaunchKernelGGL((k_trace_wide<true, kWideBlock, S>), grid, dim3(kWideBlock), 0, stream, s,
-                       (const float4*)rays, (int4*)hits, (uint32_t)n, counter);
+                       (const float4*)rays, (int4*)hits, (uint32_t)n, refill);
   else
     hipLaunchKernelGGL((k_trace_wide<false, kWideBlock, S>), grid, dim3(kWideBlock), 0, stream, s,
-                       (const float4*)rays, (int4*)hits, (uint32_t)n, counter);
+                       (const float4*)rays, (int4*)hits, (uint32_t)n, refill);
 }
 
 }  // namespace
 
-const int kWideStackVariants[kNumWideStackVariants] = {8, 12, 16, 24, 32};
+const int kWideStackVariants[kNumWideStackVariants] = {4, 8, 12, 16, 24};
 
 hipError_t WideBlocksPerCU(int stack, int* blocks) {
   const Variant* v = FindVariant(stack);
@@ -93,17 +129,18 @@ hipError_t WideBlocksPerCU(int stack, int* blocks) {
 }
 
 hipError_t LaunchTraceWide(const WideScene& s, int stack, int query, const gpugi_ray* rays, gpugi_hit* hits,
-                           size_t n, uint32_t* rayCounter, hipStream_t stream) {
+                           size_t n, int refill, hipStream_t stream) {
   if (n == 0) return hipSuccess;
   if (n > 0xffffffffull - 64) return hipErrorInvalidValue;
-  if (s.numThreads == 0 || s.numThreads % kWideBlock != 0) return hipErrorInvalidConfiguration;
-  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, stream, rayCounter);
+  // the static schedule needs a whole number of blocks per XCD
+  if (s.numThreads == 0 || s.numThreads % (kWideBlock * kNumXcd) != 0) return hipErrorInvalidConfiguration;
+  if (refill < 1 || refill > 64) return hipErrorInvalidValue;
   switch (stack) {
-    case 8: LaunchS<8>(s, query, rays, hits, n, rayCounter, stream); break;
-    case 12: LaunchS<12>(s, query, rays, hits, n, rayCounter, stream); break;
-    case 16: LaunchS<16>(s, query, rays, hits, n, rayCounter, stream); break;
-    case 24: LaunchS<24>(s, query, rays, hits, n, rayCounter, stream); break;
-    case 32: LaunchS<32>(s, query, rays, hits, n, rayCounter, stream); break;
+    case 4: LaunchS<4>(s, query, rays, hits, n, refill, stream); break;
+    case 8: LaunchS<8>(s, query, rays, hits, n, refill, stream); break;
+    case 12: LaunchS<12>(s, query, rays, hits, n, refill, stream); break;
+    case 16: LaunchS<16>(s, query, rays, hits, n, refill, stream); break;
+    case 24: LaunchS<24>(s, query, rays, hits, n, refill, stream); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

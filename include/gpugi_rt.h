@@ -195,13 +195,15 @@ int gpugi_readback(gpugi_ctx* ctx, float* rgba, size_t bytes, uint32_t* iteratio
 int gpugi_trace(gpugi_ctx* ctx, int query, int kernel, const gpugi_ray* rays, gpugi_hit* hits,
                 size_t n, int rays_on_device);
 /* Configuration of the wide traversal: triangles per merged leaf (1..16; the
- * BVH is re-packed when it changes) and LDS stack entries per lane of the
- * trace kernel (8, 12, 16, 24 or 32; deeper entries spill to global memory).
+ * BVH is re-packed when it changes), LDS stack entries per lane of the trace
+ * kernel (4, 8, 12, 16 or 24; deeper entries spill to global memory) and the
+ * number of idle lanes (1..64) at which a wave refills them with new rays.
  * No reference counterpart: the GLSL path has one fixed layout
  * (TRIANGLES_PER_LEAF, scenedata.glsl:5). */
-int gpugi_set_wide_config(gpugi_ctx* ctx, int max_leaf_triangles, int lds_stack_entries);
-int gpugi_get_wide_config(gpugi_ctx* ctx, int* max_leaf_triangles, int* lds_stack_entries, uint32_t* max_stack);
-/* Autotuner: times every (leaf size, stack depth) pair on the given device
+int gpugi_set_wide_config(gpugi_ctx* ctx, int max_leaf_triangles, int lds_stack_entries, int refill_lanes);
+int gpugi_get_wide_config(gpugi_ctx* ctx, int* max_leaf_triangles, int* lds_stack_entries, int* refill_lanes,
+                          uint32_t* max_stack);
+/* Autotuner: times every (leaf size, stack depth, refill) triple on the given device
  * ray set (one warm-up launch + `repeats` timed launches each), keeps the
  * fastest configuration active and returns its average launch time. */
 int gpugi_autotune_trace(gpugi_ctx* ctx, int query, const gpugi_ray* rays_dev, gpugi_hit* hits_dev, size_t n,
