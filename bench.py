@@ -37,7 +37,13 @@ def parse():
     p.add_argument("--cpu-seconds", type=float, default=12.0, help="bounded CPU-baseline sample duration")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--extra", action="store_true", help="also time coherent / shadow sets and the reference walk")
-    p.add_argument("--no-autotune", action="store_true", help="keep the default wide-kernel configuration")
+    p.add_argument("--autotune", action="store_true",
+                   help="autotune the wide kernel on a separate incoherent sample before timing "
+                        "(default: the library's default configuration)")
+    p.add_argument("--summarize", nargs=4, metavar=("KT_DIR", "FETCH_DIR", "WRITE_DIR", "KT_LOG"),
+                   help="write the profiles/ summaries from rocprofv3 output directories and exit")
+    p.add_argument("--summarize-counters", nargs="+", metavar="NAME_THEN_DIRS",
+                   help="write profiles/<NAME>.json from rocprofv3 --pmc output directories and exit")
     p.add_argument("--sweep", action="store_true",
                    help="print Grays/s of every wide-kernel configuration on the three ray sets and exit")
     p.add_argument("--calibrate", action="store_true",
@@ -58,6 +64,128 @@ def traffic_from_profile():
         return None
 
 
+PROFILE_WORKLOAD = "heightfield 1024x1024 cells (2,097,152 triangles), 2^24 incoherent rays, closest hit"
+DOMINANT_KERNEL = "void gpugi::k_trace_wide<false"
+PROFILE_COMMANDS = {
+    "kt": "cd /tmp && export TMPDIR=/tmp && timeout -k 10 300 rocprofv3 --kernel-trace --stats -d <out>/prof_kt "
+          "-o kt --output-format csv -- python3 <repo>/bench.py > <out>/prof_kt.log",
+    "fetch": "cd /tmp && export TMPDIR=/tmp && timeout -s KILL 150 rocprofv3 --pmc FETCH_SIZE -d <out>/prof_fetch "
+             "-o f --output-format csv -- python3 <repo>/bench.py --steps 3 --warmup 1 --no-cpu-baseline --calibrate",
+    "write": "cd /tmp && export TMPDIR=/tmp && timeout -s KILL 150 rocprofv3 --pmc WRITE_SIZE -d <out>/prof_write "
+             "-o w --output-format csv -- python3 <repo>/bench.py --steps 3 --warmup 1 --no-cpu-baseline --calibrate",
+    "summary": "python3 <repo>/bench.py --summarize <out>/prof_kt <out>/prof_fetch <out>/prof_write <out>/prof_kt.log",
+}
+
+
+def _csv_rows(directory, suffix):
+    import csv
+    import glob
+    files = sorted(glob.glob(os.path.join(directory, "*" + suffix)))
+    if not files:
+        raise SystemExit("bench.py --summarize: no *%s under %s" % (suffix, directory))
+    with open(files[0], newline="") as f:
+        return list(csv.DictReader(f))
+
+
+def _pmc_per_launch(directory, counter):
+    """(average counter value of the dominant kernel, average of the three 1 GiB calibration copies)."""
+    rows = [r for r in _csv_rows(directory, "_counter_collection.csv") if r["Counter_Name"] == counter]
+    kern = [float(r["Counter_Value"]) for r in rows if r["Kernel_Name"].startswith(DOMINANT_KERNEL)]
+    copies = [float(r["Counter_Value"]) for r in rows if r["Kernel_Name"] == "__amd_rocclr_copyBuffer"]
+    copies = sorted(copies)[-3:]
+    if not kern or len(copies) < 3:
+        raise SystemExit("bench.py --summarize: %s rows missing in %s" % (counter, directory))
+    return sum(kern) / len(kern), sum(copies) / len(copies)
+
+
+def summarize(kt_dir, fetch_dir, write_dir, kt_log):
+    """Writes profiles/trace_wide_heightfield_{kernel_stats.txt,pmc.json} from rocprofv3 CSV output.
+
+    HBM bytes per launch = FETCH_SIZE * ff + WRITE_SIZE * fw (KiB -> bytes), where ff and fw make the
+    counters read 1 GiB each for a 1 GiB device-to-device copy (the --calibrate copies of the same runs).
+    """
+    stats = _csv_rows(kt_dir, "_kernel_stats.csv")
+    stats.sort(key=lambda r: -float(r["TotalDurationNs"]))
+    dom = [r for r in stats if r["Name"].startswith(DOMINANT_KERNEL)]
+    if not dom:
+        raise SystemExit("bench.py --summarize: dominant kernel not in the kernel statistics")
+    bench_line = None
+    with open(kt_log) as f:
+        for line in f:
+            if line.startswith("{"):
+                bench_line = json.loads(line)
+    lines = ["rocprofv3 kernel-trace statistics: traversal benchmark (%s), MI355X (gfx950), one GPU." % PROFILE_WORKLOAD,
+             "",
+             "command: " + PROFILE_COMMANDS["kt"],
+             "summary: " + PROFILE_COMMANDS["summary"]]
+    if bench_line:
+        lines.append("bench.py printed (same process): ms_per_step %.4f (HIP events over %d launches), value %.4f %s, "
+                     "wide_config %s" % (bench_line["ms_per_step"], bench_line["steps"], bench_line["value"],
+                                         bench_line["unit"], json.dumps(bench_line["extra"].get("wide_config"))))
+    lines += ["", "%-72s %6s %14s %14s %14s" % ("kernel", "calls", "avg_ns", "min_ns", "max_ns")]
+    for r in stats[:14]:
+        lines.append("%-72s %6s %14d %14d %14d" % (r["Name"][:72], r["Calls"], round(float(r["AverageNs"])),
+                                                   int(float(r["MinNs"])), int(float(r["MaxNs"]))))
+    with open(os.path.join(ROOT, "profiles", "trace_wide_heightfield_kernel_stats.txt"), "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+    fk, fc = _pmc_per_launch(fetch_dir, "FETCH_SIZE")
+    wk, wc = _pmc_per_launch(write_dir, "WRITE_SIZE")
+    gib_kib = float(1 << 20)
+    ff, fw = gib_kib / fc, gib_kib / wc
+    hbm = (fk * ff + wk * fw) * 1024.0
+    avg_ns = float(dom[0]["AverageNs"])
+    out = {"kernel": dom[0]["Name"], "workload": PROFILE_WORKLOAD,
+           "commands": [PROFILE_COMMANDS["fetch"], PROFILE_COMMANDS["write"], PROFILE_COMMANDS["summary"]],
+           "calibration": {"copy_bytes": 1 << 30, "fetch_size_kib_read": fc, "write_size_kib_read": wc,
+                           "fetch_factor": ff, "write_factor": fw},
+           "fetch_size_kib_per_launch": fk, "write_size_kib_per_launch": wk,
+           "hbm_bytes_per_launch": hbm, "launch_ns_avg_kernel_trace": avg_ns,
+           "hbm_gbs": hbm / avg_ns}
+    with open(os.path.join(ROOT, "profiles", "trace_wide_heightfield_pmc.json"), "w") as f:
+        json.dump(out, f, indent=1)
+        f.write("\n")
+    print(json.dumps(out))
+
+
+def summarize_counters(name, *dirs):
+    """Writes profiles/<name>.json: per-launch averages of every counter that rocprofv3 --pmc collected for
+    the dominant kernel in the given output directories, plus derived ratios when their inputs are there."""
+    per = {}
+    for d in dirs:
+        rows = _csv_rows(d, "_counter_collection.csv")
+        acc = {}
+        for r in rows:
+            if r["Kernel_Name"].startswith(DOMINANT_KERNEL):
+                acc.setdefault(r["Counter_Name"], []).append(float(r["Counter_Value"]))
+        for k, v in acc.items():
+            per[k] = sum(v) / len(v)
+    rays = float(1 << 24)
+    derived = {}
+    if "TCC_HIT_sum" in per and "TCC_MISS_sum" in per:
+        derived["tcc_hit_rate"] = per["TCC_HIT_sum"] / (per["TCC_HIT_sum"] + per["TCC_MISS_sum"])
+    if "SQ_INSTS_VALU" in per:
+        derived["valu_wave_instructions_per_ray"] = per["SQ_INSTS_VALU"] / rays
+    if "SQ_INSTS_VMEM_RD" in per:
+        derived["vmem_read_wave_instructions_per_ray"] = per["SQ_INSTS_VMEM_RD"] / rays
+    if "SQ_THREAD_CYCLES_VALU" in per and "SQ_ACTIVE_INST_VALU" in per:
+        derived["valu_thread_cycles_per_active_inst"] = per["SQ_THREAD_CYCLES_VALU"] / per["SQ_ACTIVE_INST_VALU"]
+    if "SQ_ACTIVE_INST_VALU" in per and "GRBM_GUI_ACTIVE" in per:
+        # GRBM_GUI_ACTIVE is summed over the 8 XCDs; 1024 SIMDs issue at most one VALU instruction per cycle
+        derived["valu_issue_utilization"] = per["SQ_ACTIVE_INST_VALU"] / (per["GRBM_GUI_ACTIVE"] / 8.0 * 1024.0)
+    out = {"kernel_prefix": DOMINANT_KERNEL, "workload": PROFILE_WORKLOAD,
+           "commands": ["cd /tmp && export TMPDIR=/tmp && timeout -s KILL 120 rocprofv3 --pmc <counters of one pass> "
+                        "-d <out>/<pass> -o <pass> --output-format csv -- python3 <repo>/bench.py --steps 2 --warmup 1 "
+                        "--no-cpu-baseline",
+                        "python3 <repo>/bench.py --summarize-counters %s %s" % (name, " ".join(
+                            "<out>/" + os.path.basename(os.path.normpath(d)) for d in dirs))],
+           "per_launch": per, "derived": derived}
+    with open(os.path.join(ROOT, "profiles", name + ".json"), "w") as f:
+        json.dump(out, f, indent=1)
+        f.write("\n")
+    print(json.dumps(out))
+
+
 def cpu_baseline(scene_arrays, rays_host, seconds):
     from oracle import oracle
     sc = oracle.Scene(scene_arrays)
@@ -74,6 +202,12 @@ def cpu_baseline(scene_arrays, rays_host, seconds):
 
 def main():
     args = parse()
+    if args.summarize:
+        summarize(*args.summarize)
+        return
+    if args.summarize_counters:
+        summarize_counters(*args.summarize_counters)
+        return
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; --gpus %d is not supported (rays shard with zero "
               "communication, see DESIGN.md)" % args.gpus, file=sys.stderr)
@@ -109,7 +243,7 @@ def main():
 
     # Autotune the wide kernel on a separate incoherent sample (other seed), untimed.
     tuned = None
-    if kernel == rt.KERNEL_WIDE and not args.no_autotune:
+    if kernel == rt.KERNEL_WIDE and args.autotune:
         m = min(n, 1 << 22)
         T = workloads.HeightfieldRays(ctx, n=m, seed=(workloads.MASTER_SEED + 7) & 0xffffffff)
         tuned_ms = ctx.autotune_trace(T.incoherent, T.hits, m, rt.CLOSEST_HIT, repeats=2)
@@ -142,7 +276,8 @@ def main():
     extra = {"setup_s": round(setup_s, 2), "wall_s_timed": round(wall, 4),
              "hit_fraction": float((hits[:, 0] >= 0).mean()),
              "mean_box_tests": float(nbox.mean()), "mean_tri_tests": float(ntri.mean()),
-             "alg_bytes_per_ray": alg_bytes / n, "wide_config": tuned or None}
+             "alg_bytes_per_ray": alg_bytes / n,
+             "wide_config": tuned or ctx.wide_config()}
     if args.extra:
         for name, buf, q in (("coherent_closest", R.primary, rt.CLOSEST_HIT), ("shadow_any", R.shadow, rt.ANY_HIT)):
             ctx.trace_device(buf, R.hits, n, q, kernel)
@@ -154,6 +289,13 @@ def main():
         ctx.timer_begin()
         ctx.trace_device(R.incoherent, R.hits, n, rt.CLOSEST_HIT, rt.KERNEL_REFERENCE)
         extra["reference_walk_incoherent_grays"] = n / (ctx.timer_end() * 1e-3) / 1e9
+        # host buffers through the same entry point: upload of the rays, the launch and the hit download
+        rays_np = R.incoherent.download(np.float32, n * 8).view(rt.RAY_DTYPE)
+        ctx.trace(rays_np[: 1 << 20], rt.CLOSEST_HIT, kernel)
+        t0 = time.perf_counter()
+        ctx.trace(rays_np, rt.CLOSEST_HIT, kernel)
+        extra["host_buffers_pcie_inclusive_grays"] = n / (time.perf_counter() - t0) / 1e9
+        del rays_np
 
     if args.calibrate:
         a, b = ctx.alloc(1 << 30), ctx.alloc(1 << 30)
@@ -163,6 +305,9 @@ def main():
         a.free()
         b.free()
     traffic = traffic_from_profile()
+    if traffic:
+        # HBM traffic the counters measured for one launch, over this run's launch time
+        extra["hbm_gbs_measured"] = round(traffic / (ms * 1e-3) / 1e9, 2)
     result = {
         "metric": "incoherent closest-hit Grays/s",
         "value": round(grays, 4),

@@ -38,7 +38,7 @@ struct gpugi_ctx {
   size_t spillBytes = 0;
   uint32_t wideMaxStack = 0;
   int wideLeaf = 2;     // triangles per merged wide leaf
-  int wideStack = 8;    // LDS stack entries per lane of the trace kernel
+  int wideStack = 12;   // LDS stack entries per lane of the trace kernel
   int wideRefill = 16;  // idle lanes that trigger a wave-level ray refill
   // host copy of the reference layout (re-packing after a configuration change)
   std::vector<float> hHier, hPos;

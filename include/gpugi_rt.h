@@ -198,7 +198,8 @@ int gpugi_trace(gpugi_ctx* ctx, int query, int kernel, const gpugi_ray* rays, gp
  * BVH is re-packed when it changes), LDS stack entries per lane of the trace
  * kernel (4, 8, 12, 16 or 24; deeper entries spill to global memory) and the
  * number of idle lanes (1..64) at which a wave refills them with new rays.
- * No reference counterpart: the GLSL path has one fixed layout
+ * Default (2, 12, 16): the fastest incoherent closest-hit configuration on
+ * the 2,097,152-triangle heightfield.  No reference counterpart: the GLSL path has one fixed layout
  * (TRIANGLES_PER_LEAF, scenedata.glsl:5). */
 int gpugi_set_wide_config(gpugi_ctx* ctx, int max_leaf_triangles, int lds_stack_entries, int refill_lanes);
 int gpugi_get_wide_config(gpugi_ctx* ctx, int* max_leaf_triangles, int* lds_stack_entries, int* refill_lanes,
