@@ -14,6 +14,8 @@ PKG = os.path.join(ROOT, "gpugi_amd")
 CSRC = os.path.join(PKG, "csrc")
 BUILD = os.path.join(ROOT, "_build")
 LIB = os.path.join(PKG, "lib", "libgpugi_rt.so")
+# script-driven front end over the C++ renderer layer (links libgpugi_rt.so)
+APP = os.path.join(PKG, "bin", "gpugi_render")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 COMMON = [
@@ -31,6 +33,11 @@ def sources():
     host = sorted(os.path.join(CSRC, "host", f) for f in os.listdir(os.path.join(CSRC, "host")) if f.endswith(".cpp"))
     dev = sorted(os.path.join(CSRC, "device", f) for f in os.listdir(os.path.join(CSRC, "device")) if f.endswith(".hip"))
     return host, dev
+
+
+def app_sources():
+    d = os.path.join(CSRC, "renderer")
+    return sorted(os.path.join(d, f) for f in os.listdir(d) if f.endswith(".cpp"))
 
 
 def _compile(src, is_device):
@@ -66,6 +73,18 @@ def build(verbose=False, jobs=None):
     if r.returncode != 0:
         raise RuntimeError("link failed: %s\n%s\n%s" % (" ".join(cmd), r.stdout, r.stderr))
     os.replace(tmp, LIB)
+
+    # the renderer layer uses nothing but the C ABI: plain host C++ linked against the library
+    with concurrent.futures.ThreadPoolExecutor(max_workers=jobs) as ex:
+        app_objs = [o for o, _ in ex.map(lambda s: _compile(s, False), app_sources())]
+    os.makedirs(os.path.dirname(APP), exist_ok=True)
+    tmp = APP + ".tmp"
+    cmd = [HIPCC, "-o", tmp] + app_objs + ["-L" + os.path.dirname(LIB), "-lgpugi_rt",
+                                          "-Wl,-rpath,$ORIGIN/../lib"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("link failed: %s\n%s\n%s" % (" ".join(cmd), r.stdout, r.stderr))
+    os.replace(tmp, APP)
     return LIB
 
 

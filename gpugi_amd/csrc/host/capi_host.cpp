@@ -210,6 +210,15 @@ int gpugi_lightsampler_generate(gpugi_lightsampler* sampler, gpugi_light_sample*
   return GPUGI_OK;
 }
 
+int gpugi_lightsampler_set_scene(gpugi_lightsampler* sampler, const gpugi_hostscene* scene) {
+  if (!sampler || !scene) {
+    SetGlobalError("gpugi_lightsampler_set_scene: invalid argument");
+    return GPUGI_ERR_ARG;
+  }
+  sampler->scene = scene;
+  return GPUGI_OK;
+}
+
 void gpugi_lightsampler_destroy(gpugi_lightsampler* sampler) { delete sampler; }
 
 }  // extern "C"

@@ -111,6 +111,7 @@ _SIGNATURES = {
                              ctypes.POINTER(Camera)],
     "gpugi_lightsampler_create": [_P, ctypes.c_uint64, ctypes.POINTER(ctypes.c_void_p)],
     "gpugi_lightsampler_generate": [_P, _P, ctypes.c_uint32, ctypes.c_float],
+    "gpugi_lightsampler_set_scene": [_P, _P],
     "gpugi_lightsampler_destroy": ([_P], None),
     "gpugi_wanghash": ([ctypes.c_uint32], ctypes.c_uint32),
 }
@@ -278,6 +279,10 @@ class LightSampler:
         _check("gpugi_lightsampler_generate", lib().gpugi_lightsampler_generate(self._h, _ptr(out), n,
                                                                                  float(position_bias)))
         return out
+
+    def set_scene(self, scene):
+        _check("gpugi_lightsampler_set_scene", lib().gpugi_lightsampler_set_scene(self._h, scene._h))
+        self._scene = scene
 
     def __del__(self):
         try:

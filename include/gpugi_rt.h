@@ -308,6 +308,9 @@ typedef struct gpugi_lightsampler gpugi_lightsampler;
 int gpugi_lightsampler_create(const gpugi_hostscene* scene, uint64_t seed, gpugi_lightsampler** out);
 int gpugi_lightsampler_generate(gpugi_lightsampler* sampler, gpugi_light_sample* out, uint32_t n,
                                 float position_bias);
+/* LightSampler::SetScene (scene/lightsampler.cpp:13-16): samples another
+ * scene from now on; the random state carries over. */
+int gpugi_lightsampler_set_scene(gpugi_lightsampler* sampler, const gpugi_hostscene* scene);
 void gpugi_lightsampler_destroy(gpugi_lightsampler* sampler);
 
 /* WangHash (utilities/random.cpp:5-13): the per-iteration FrameSeed. */
