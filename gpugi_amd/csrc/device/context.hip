@@ -11,6 +11,7 @@
 #include "../host/host_api.hpp"
 #include "integrators.h"
 #include "launch.h"
+#include "mlp.h"
 
 using namespace gpugi;
 
@@ -57,6 +58,13 @@ struct gpugi_ctx {
   int bptCapacity = -1;
   int bptAverageLength = 0;
   IntegratorState integ;
+  // radiance-cache MLP
+  bool hasMlp = false;
+  MlpDev mlp{};
+  uint32_t mlpStep = 0;
+  float* dMlpLoss = nullptr;  // loss of the last gradient / training step
+  void* dMlpIo = nullptr;     // host-pointer staging of features / targets / outputs
+  size_t mlpIoCap = 0;
   // scratch
   void* dScratch = nullptr;
   size_t scratchCap = 0;
@@ -214,6 +222,10 @@ void gpugi_destroy(gpugi_ctx* ctx) {
                    &ctx->dWideTris, &ctx->dSpill, &ctx->dCounters, &ctx->dAccum, &ctx->dLights, &ctx->dScratch})
     FreeDev(*p);
   FreeIntegratorState(&ctx->integ);
+  for (void** p : {(void**)&ctx->mlp.w, (void**)&ctx->mlp.m, (void**)&ctx->mlp.v, (void**)&ctx->mlp.wb,
+                   (void**)&ctx->mlp.wtb, (void**)&ctx->mlp.partials, (void**)&ctx->mlp.grad,
+                   (void**)&ctx->mlp.loss, (void**)&ctx->dMlpLoss, &ctx->dMlpIo})
+    FreeDev(*p);
   for (void* p : ctx->userAllocs) hipFree(p);
   if (ctx->ev0) hipEventDestroy(ctx->ev0);
   if (ctx->ev1) hipEventDestroy(ctx->ev1);
@@ -707,3 +719,189 @@ int gpugi_raygen_shadow(gpugi_ctx* ctx, const gpugi_ray* primary_dev, const gpug
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------- radiance-cache MLP
+
+namespace {
+
+// He-uniform initial weights from a xorshift64* stream; rows of padded outputs are 0.
+void MlpInitialWeights(uint64_t seed, std::vector<float>* w) {
+  w->assign(kMlpParams, 0.0f);
+  uint64_t x = seed ? seed : 0x9E3779B97F4A7C15ull;
+  for (int l = 0; l < kMlpLayers; ++l) {
+    const int in = kMlpLayerIn[l], out = kMlpLayerOut[l];
+    const int used = l == kMlpLayers - 1 ? kMlpOutUsed : out;
+    const float bound = (float)std::sqrt(6.0 / (l == 0 ? (double)kMlpInUsed : (double)in));
+    for (int o = 0; o < used; ++o)
+      for (int i = 0; i < in; ++i) {
+        x ^= x >> 12;
+        x ^= x << 25;
+        x ^= x >> 27;
+        const uint64_t r = x * 0x2545F4914F6CDD1Dull;
+        const double u = (double)(r >> 11) * (1.0 / 9007199254740992.0);
+        (*w)[kMlpWOff[l] + o * in + i] = (float)((2.0 * u - 1.0) * bound);
+      }
+  }
+}
+
+int MlpEnsureTiles(gpugi_ctx* ctx, uint32_t tiles) {
+  if (ctx->mlp.partialTiles >= tiles) return GPUGI_OK;
+  FreeDev(*(void**)&ctx->mlp.partials);
+  FreeDev(*(void**)&ctx->mlp.loss);
+  ctx->mlp.partialTiles = 0;
+  HIP_TRY(ctx, hipMalloc((void**)&ctx->mlp.partials, (size_t)tiles * kMlpParams * sizeof(float)));
+  HIP_TRY(ctx, hipMalloc((void**)&ctx->mlp.loss, (size_t)tiles * sizeof(float)));
+  ctx->mlp.partialTiles = tiles;
+  return GPUGI_OK;
+}
+
+int MlpEnsureIo(gpugi_ctx* ctx, size_t bytes) {
+  if (ctx->mlpIoCap >= bytes) return GPUGI_OK;
+  FreeDev(ctx->dMlpIo);
+  ctx->mlpIoCap = 0;
+  HIP_TRY(ctx, hipMalloc(&ctx->dMlpIo, bytes));
+  ctx->mlpIoCap = bytes;
+  return GPUGI_OK;
+}
+
+int MlpUpload(gpugi_ctx* ctx, const float* w) {
+  const size_t bytes = kMlpParams * sizeof(float);
+  HIP_TRY(ctx, hipMemcpyAsync(ctx->mlp.w, w, bytes, hipMemcpyHostToDevice, ctx->stream));
+  HIP_TRY(ctx, hipMemsetAsync(ctx->mlp.m, 0, bytes, ctx->stream));
+  HIP_TRY(ctx, hipMemsetAsync(ctx->mlp.v, 0, bytes, ctx->stream));
+  HIP_TRY(ctx, LaunchMlpRefreshBf16(ctx->mlp, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->mlpStep = 0;
+  return GPUGI_OK;
+}
+
+int MlpCheck(gpugi_ctx* ctx, const char* fn, size_t n) {
+  if (!ctx) return Fail(ctx, GPUGI_ERR_ARG, std::string(fn) + ": null context");
+  if (!ctx->hasMlp) return Fail(ctx, GPUGI_ERR_STATE, std::string(fn) + ": call gpugi_mlp_init first");
+  if (n > (size_t)1 << 28) return Fail(ctx, GPUGI_ERR_ARG, std::string(fn) + ": at most 2^28 samples per call");
+  return SetDev(ctx);
+}
+
+}  // namespace
+
+int gpugi_mlp_init(gpugi_ctx* ctx, uint64_t seed) {
+  if (!ctx) return Fail(ctx, GPUGI_ERR_ARG, "gpugi_mlp_init: null context");
+  if (SetDev(ctx)) return GPUGI_ERR_HIP;
+  if (!ctx->hasMlp) {
+    const size_t fb = kMlpParams * sizeof(float);
+    HIP_TRY(ctx, hipMalloc((void**)&ctx->mlp.w, fb));
+    HIP_TRY(ctx, hipMalloc((void**)&ctx->mlp.m, fb));
+    HIP_TRY(ctx, hipMalloc((void**)&ctx->mlp.v, fb));
+    HIP_TRY(ctx, hipMalloc((void**)&ctx->mlp.grad, fb));
+    HIP_TRY(ctx, hipMalloc((void**)&ctx->mlp.wb, kMlpParams * sizeof(uint16_t)));
+    HIP_TRY(ctx, hipMalloc((void**)&ctx->mlp.wtb, kMlpWTSize * sizeof(uint16_t)));
+    HIP_TRY(ctx, hipMemset(ctx->mlp.wtb, 0, kMlpWTSize * sizeof(uint16_t)));
+    HIP_TRY(ctx, hipMalloc((void**)&ctx->dMlpLoss, 256));
+    ctx->hasMlp = true;
+  }
+  std::vector<float> w;
+  MlpInitialWeights(seed, &w);
+  return MlpUpload(ctx, w.data());
+}
+
+int gpugi_mlp_set_weights(gpugi_ctx* ctx, const float* weights, size_t n) {
+  if (int rc = MlpCheck(ctx, "gpugi_mlp_set_weights", 0)) return rc;
+  if (!weights || n != (size_t)kMlpParams)
+    return Fail(ctx, GPUGI_ERR_ARG, "gpugi_mlp_set_weights: need GPUGI_MLP_PARAMS floats");
+  return MlpUpload(ctx, weights);
+}
+
+int gpugi_mlp_get_weights(gpugi_ctx* ctx, float* weights, size_t n) {
+  if (int rc = MlpCheck(ctx, "gpugi_mlp_get_weights", 0)) return rc;
+  if (!weights || n != (size_t)kMlpParams)
+    return Fail(ctx, GPUGI_ERR_ARG, "gpugi_mlp_get_weights: need GPUGI_MLP_PARAMS floats");
+  HIP_TRY(ctx, hipMemcpyAsync(weights, ctx->mlp.w, kMlpParams * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return GPUGI_OK;
+}
+
+int gpugi_mlp_encode(gpugi_ctx* ctx, const float* features, uint16_t* encoded, size_t n) {
+  if (int rc = MlpCheck(ctx, "gpugi_mlp_encode", n)) return rc;
+  if (n == 0) return GPUGI_OK;
+  if (!features || !encoded) return Fail(ctx, GPUGI_ERR_ARG, "gpugi_mlp_encode: null argument");
+  const size_t fb = n * kMlpRaw * sizeof(float), eb = n * kMlpIn * sizeof(uint16_t);
+  if (int rc = MlpEnsureIo(ctx, fb + eb)) return rc;
+  float* df = (float*)ctx->dMlpIo;
+  uint16_t* de = (uint16_t*)((char*)ctx->dMlpIo + fb);
+  HIP_TRY(ctx, hipMemcpyAsync(df, features, fb, hipMemcpyHostToDevice, ctx->stream));
+  HIP_TRY(ctx, LaunchMlpEncode(df, de, n, ctx->stream));
+  HIP_TRY(ctx, hipMemcpyAsync(encoded, de, eb, hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return GPUGI_OK;
+}
+
+int gpugi_mlp_infer(gpugi_ctx* ctx, const float* features, float* radiance, size_t n, int on_device) {
+  if (int rc = MlpCheck(ctx, "gpugi_mlp_infer", n)) return rc;
+  if (n == 0) return GPUGI_OK;
+  if (!features || !radiance) return Fail(ctx, GPUGI_ERR_ARG, "gpugi_mlp_infer: null argument");
+  const float* df = features;
+  float* dr = radiance;
+  const size_t fb = n * kMlpRaw * sizeof(float), rb = n * 4 * sizeof(float);
+  if (!on_device) {
+    if (int rc = MlpEnsureIo(ctx, fb + rb)) return rc;
+    df = (const float*)ctx->dMlpIo;
+    dr = (float*)((char*)ctx->dMlpIo + fb);
+    HIP_TRY(ctx, hipMemcpyAsync((void*)df, features, fb, hipMemcpyHostToDevice, ctx->stream));
+  }
+  HIP_TRY(ctx, LaunchMlpInfer(ctx->mlp, df, dr, n, ctx->stream));
+  if (!on_device) {
+    HIP_TRY(ctx, hipMemcpyAsync(radiance, dr, rb, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  }
+  return GPUGI_OK;
+}
+
+namespace {
+
+// Stages features/targets for host pointers and runs the gradient kernels.
+int MlpGradientCommon(gpugi_ctx* ctx, const char* fn, const float* features, const float* targets, size_t n,
+                      int on_device) {
+  if (n == 0 || !features || !targets) return Fail(ctx, GPUGI_ERR_ARG, std::string(fn) + ": need samples");
+  if (int rc = MlpEnsureTiles(ctx, MlpTiles(n))) return rc;
+  const float* df = features;
+  const float* dt = targets;
+  const size_t fb = n * kMlpRaw * sizeof(float), tb = n * 4 * sizeof(float);
+  if (!on_device) {
+    if (int rc = MlpEnsureIo(ctx, fb + tb + 256)) return rc;
+    df = (const float*)ctx->dMlpIo;
+    dt = (const float*)((char*)ctx->dMlpIo + fb);
+    HIP_TRY(ctx, hipMemcpyAsync((void*)df, features, fb, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(ctx, hipMemcpyAsync((void*)dt, targets, tb, hipMemcpyHostToDevice, ctx->stream));
+  }
+  HIP_TRY(ctx, LaunchMlpGradient(ctx->mlp, df, dt, n, ctx->dMlpLoss, ctx->stream));
+  return GPUGI_OK;
+}
+
+}  // namespace
+
+int gpugi_mlp_gradient(gpugi_ctx* ctx, const float* features, const float* targets, size_t n, float* grad,
+                       float* loss, int on_device) {
+  if (int rc = MlpCheck(ctx, "gpugi_mlp_gradient", n)) return rc;
+  if (!grad) return Fail(ctx, GPUGI_ERR_ARG, "gpugi_mlp_gradient: null gradient buffer");
+  if (int rc = MlpGradientCommon(ctx, "gpugi_mlp_gradient", features, targets, n, on_device)) return rc;
+  HIP_TRY(ctx, hipMemcpyAsync(grad, ctx->mlp.grad, kMlpParams * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+  float l = 0.0f;
+  HIP_TRY(ctx, hipMemcpyAsync(&l, ctx->dMlpLoss, sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  if (loss) *loss = l;
+  return GPUGI_OK;
+}
+
+int gpugi_mlp_train_step(gpugi_ctx* ctx, const float* features, const float* targets, size_t n,
+                         float learning_rate, float* loss, int on_device) {
+  if (int rc = MlpCheck(ctx, "gpugi_mlp_train_step", n)) return rc;
+  if (!(learning_rate > 0.0f)) return Fail(ctx, GPUGI_ERR_ARG, "gpugi_mlp_train_step: learning rate must be > 0");
+  if (int rc = MlpGradientCommon(ctx, "gpugi_mlp_train_step", features, targets, n, on_device)) return rc;
+  AdamParams p{learning_rate, 0.9f, 0.99f, 1e-8f, ++ctx->mlpStep};
+  HIP_TRY(ctx, LaunchMlpAdam(ctx->mlp, p, ctx->stream));
+  if (loss) {
+    HIP_TRY(ctx, hipMemcpyAsync(loss, ctx->dMlpLoss, sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  }
+  return GPUGI_OK;
+}

@@ -114,7 +114,19 @@ _SIGNATURES = {
     "gpugi_lightsampler_set_scene": [_P, _P],
     "gpugi_lightsampler_destroy": ([_P], None),
     "gpugi_wanghash": ([ctypes.c_uint32], ctypes.c_uint32),
+    "gpugi_mlp_init": [_P, ctypes.c_uint64],
+    "gpugi_mlp_set_weights": [_P, _P, ctypes.c_size_t],
+    "gpugi_mlp_get_weights": [_P, _P, ctypes.c_size_t],
+    "gpugi_mlp_encode": [_P, _P, _P, ctypes.c_size_t],
+    "gpugi_mlp_infer": [_P, _P, _P, ctypes.c_size_t, ctypes.c_int],
+    "gpugi_mlp_gradient": [_P, _P, _P, ctypes.c_size_t, _P, ctypes.POINTER(ctypes.c_float), ctypes.c_int],
+    "gpugi_mlp_train_step": [_P, _P, _P, ctypes.c_size_t, ctypes.c_float, ctypes.POINTER(ctypes.c_float),
+                             ctypes.c_int],
 }
+
+MLP_PARAMS = 19456
+MLP_FEATURES = 16
+MLP_INPUTS = 96
 
 _lib = None
 
@@ -444,6 +456,48 @@ class Context:
         self._call("gpugi_raygen_shadow", ctypes.c_void_p(prim_buf.ptr), ctypes.c_void_p(hits_buf.ptr),
                    ctypes.c_size_t(n_prim), _ptr(lights), len(lights), ctypes.c_void_p(rays_buf.ptr),
                    ctypes.c_size_t(n))
+
+    # -- radiance-cache MLP
+    def mlp_init(self, seed=1):
+        self._call("gpugi_mlp_init", ctypes.c_uint64(seed))
+
+    def mlp_set_weights(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        assert w.size == MLP_PARAMS
+        self._call("gpugi_mlp_set_weights", _ptr(w), ctypes.c_size_t(w.size))
+
+    def mlp_get_weights(self):
+        w = np.zeros(MLP_PARAMS, np.float32)
+        self._call("gpugi_mlp_get_weights", _ptr(w), ctypes.c_size_t(w.size))
+        return w
+
+    def mlp_encode(self, features):
+        f = np.ascontiguousarray(features, dtype=np.float32).reshape(-1, MLP_FEATURES)
+        out = np.zeros((len(f), MLP_INPUTS), np.uint16)
+        self._call("gpugi_mlp_encode", _ptr(f), _ptr(out), ctypes.c_size_t(len(f)))
+        return out
+
+    def mlp_infer(self, features):
+        f = np.ascontiguousarray(features, dtype=np.float32).reshape(-1, MLP_FEATURES)
+        out = np.zeros((len(f), 4), np.float32)
+        self._call("gpugi_mlp_infer", _ptr(f), _ptr(out), ctypes.c_size_t(len(f)), 0)
+        return out
+
+    def mlp_gradient(self, features, targets):
+        f = np.ascontiguousarray(features, dtype=np.float32).reshape(-1, MLP_FEATURES)
+        t = np.ascontiguousarray(targets, dtype=np.float32).reshape(-1, 4)
+        g = np.zeros(MLP_PARAMS, np.float32)
+        loss = ctypes.c_float()
+        self._call("gpugi_mlp_gradient", _ptr(f), _ptr(t), ctypes.c_size_t(len(f)), _ptr(g), ctypes.byref(loss), 0)
+        return g, loss.value
+
+    def mlp_train_step(self, features, targets, lr=1e-3):
+        f = np.ascontiguousarray(features, dtype=np.float32).reshape(-1, MLP_FEATURES)
+        t = np.ascontiguousarray(targets, dtype=np.float32).reshape(-1, 4)
+        loss = ctypes.c_float()
+        self._call("gpugi_mlp_train_step", _ptr(f), _ptr(t), ctypes.c_size_t(len(f)), ctypes.c_float(lr),
+                   ctypes.byref(loss), 0)
+        return loss.value
 
     def close(self):
         if self._h:

@@ -313,6 +313,42 @@ int gpugi_lightsampler_generate(gpugi_lightsampler* sampler, gpugi_light_sample*
 int gpugi_lightsampler_set_scene(gpugi_lightsampler* sampler, const gpugi_hostscene* scene);
 void gpugi_lightsampler_destroy(gpugi_lightsampler* sampler);
 
+/* ------------------------------------------------------------------------ */
+/* Radiance-cache MLP (no reference counterpart: the learned cache that      */
+/* lets paths terminate early; the plain path tracer stays the judge)       */
+/* ------------------------------------------------------------------------ */
+/* 96 encoded inputs -> 4 hidden layers of 64 (ReLU) -> 3 outputs, no biases;
+ * BF16 weights on the matrix cores, FP32 accumulation, FP32 master weights
+ * and Adam (beta1 0.9, beta2 0.99, eps 1e-8).  Weights are the five matrices
+ * W_l[out][in] row-major, concatenated: 64x96, 64x64, 64x64, 64x64, 16x64
+ * (rows 3..15 of the last are padding and stay 0). */
+#define GPUGI_MLP_PARAMS 19456
+/* Per-sample raw features (16 floats): position in [0,1]^3 (scene bounds),
+ * direction, shading normal, diffuse albedo, log2(1 + Phong exponent) / 16,
+ * 3 unused.  The network encodes them into 96 inputs: sin and cos of
+ * 2^k * pi * position for k < 12, then the other 10 values, then zeros. */
+#define GPUGI_MLP_FEATURES 16
+#define GPUGI_MLP_INPUTS 96
+/* Allocates the cache on the context and sets He-uniform initial weights from
+ * a xorshift64* stream seeded `seed`; resets Adam. */
+int gpugi_mlp_init(gpugi_ctx* ctx, uint64_t seed);
+/* FP32 master weights in and out (n must be GPUGI_MLP_PARAMS); set resets Adam. */
+int gpugi_mlp_set_weights(gpugi_ctx* ctx, const float* weights, size_t n);
+int gpugi_mlp_get_weights(gpugi_ctx* ctx, float* weights, size_t n);
+/* The BF16 input encoding of n samples (host buffers; n x 96 BF16 out). */
+int gpugi_mlp_encode(gpugi_ctx* ctx, const float* features, uint16_t* encoded, size_t n);
+/* Predicted radiance, n x 4 floats (rgb, 0). */
+int gpugi_mlp_infer(gpugi_ctx* ctx, const float* features, float* radiance, size_t n, int on_device);
+/* Gradient (GPUGI_MLP_PARAMS floats, host) of the relative L2 loss
+ * mean over samples and rgb of (y - t)^2 / (y^2 + 0.01), the denominator held
+ * constant, and the loss itself; targets are n x 4 floats.  No update. */
+int gpugi_mlp_gradient(gpugi_ctx* ctx, const float* features, const float* targets, size_t n, float* grad,
+                       float* loss, int on_device);
+/* One Adam step on that gradient with the given learning rate; *loss
+ * (optional) receives the loss before the step. */
+int gpugi_mlp_train_step(gpugi_ctx* ctx, const float* features, const float* targets, size_t n,
+                         float learning_rate, float* loss, int on_device);
+
 /* WangHash (utilities/random.cpp:5-13): the per-iteration FrameSeed. */
 uint32_t gpugi_wanghash(uint32_t x);
 
