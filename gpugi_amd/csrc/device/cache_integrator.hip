@@ -1,0 +1,240 @@
+// Path tracer with the radiance-cache MLP (no reference counterpart).
+//
+// Each pixel traces the reference path tracer's estimator (pathtracer.comp:
+// same random-number order, NEE, emission only after a vertex without NEE,
+// Russian roulette) until the path's footprint has spread: with
+//   a0 = sqrt(|x1 - x0|^2 / (4 pi |cos theta_1|))           at the primary hit,
+//   a  = sum_{i>=2} sqrt(|x_i - x_{i-1}|^2 / (pdf_i |cos theta_i|)),
+// the path stops at the first vertex i >= 2 (second hit on) where a > c * a0
+// (Mueller et al. 2021, "Real-time Neural Radiance Caching for Path
+// Tracing").  There the cache's prediction of the reflected radiance, weighted
+// by the path throughput, replaces the rest of the path.
+//
+// One pixel in every `trainStride` also traces a training path: the same path
+// continued `extraBounces` vertices past the termination, where the cache
+// stands in for the remainder (self-training).  Every vertex i of a training
+// path yields a sample whose target is the radiance reflected at x_i toward
+// x_{i-1}: (C + T_tail * L_tail - S_i) / T_i, with C the path's collected
+// radiance, S_i what was collected up to and including x_i's emission and T_i
+// the throughput at x_i.  Records live at fixed per-pixel / per-path slots, so
+// a frame is deterministic.
+#include <hip/hip_runtime.h>
+
+#include "integrator_common.h"
+#include "mlp.h"
+
+namespace gpugi {
+
+namespace {
+
+constexpr int kTrainVertFloats = 24;  // features[16], T[4], S[4]
+
+__device__ __forceinline__ void write_features(float* dst, const CacheParams& cp, gg::V3 x, gg::V3 wo, gg::V3 n,
+                                               const MaterialData& m) {
+  float4* d4 = reinterpret_cast<float4*>(dst);
+  const float px = gg::saturate((x.x - cp.bboxMin[0]) * cp.bboxInvExtent[0]);
+  const float py = gg::saturate((x.y - cp.bboxMin[1]) * cp.bboxInvExtent[1]);
+  const float pz = gg::saturate((x.z - cp.bboxMin[2]) * cp.bboxInvExtent[2]);
+  d4[0] = make_float4(px, py, pz, wo.x);
+  d4[1] = make_float4(wo.y, wo.z, n.x, n.y);
+  d4[2] = make_float4(n.z, m.diffuse.x, m.diffuse.y, m.diffuse.z);
+  d4[3] = make_float4(log2f(1.0f + m.refl[3]) * (1.0f / 16.0f), 0.0f, 0.0f, 0.0f);
+}
+
+__device__ __forceinline__ void zero_features(float* dst) {
+  float4* d4 = reinterpret_cast<float4*>(dst);
+  for (int k = 0; k < 4; ++k) d4[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
+template <bool WIDE>
+__global__ __launch_bounds__(kTileThreads) void k_cached_paths(DrawParams p, CacheParams cp) {
+  __shared__ uint32_t stkNode[WIDE ? kWideLdsStack * kTileThreads : 1];
+  __shared__ float stkT[WIDE ? kWideLdsStack * kTileThreads : 1];
+  const int tid = threadIdx.x;
+  const int gx = blockIdx.x * kTile + (tid % kTile);
+  const int gy = blockIdx.y * kTile + (tid / kTile);
+  const uint32_t linear = (blockIdx.y * gridDim.x + blockIdx.x) * kTileThreads + tid;
+  TraceCtx c{&p, stkNode, stkT, tid, WIDE ? p.wide.spill + (size_t)linear * p.wide.spillDepth : nullptr};
+  const uint32_t pix = (uint32_t)gy * (uint32_t)p.width + (uint32_t)gx;
+  const bool isTrain = pix % cp.trainStride == cp.trainOffset;
+  const uint32_t trainIdx = pix / cp.trainStride;
+  float* tv = cp.trainVert + (size_t)trainIdx * p.maxPathLength * kTrainVertFloats;
+
+  const float lightSampleWeight = (float)p.numLights / 1.0f;
+  uint32_t seed = gg::init_random_seed(p.frameSeed, pix);
+  gg::V3 color = gg::v3s(0.0f), cacc = gg::v3s(0.0f), throughput = gg::v3s(1.0f), tailT = gg::v3s(0.0f);
+  bool didNEE = false, term = false, tail = false;
+  int termDepth = 0;
+  uint32_t nTrain = 0;
+  float a0 = 0.0f, spread = 0.0f, pdfPrev = 1.0f;
+  const float j0 = gg::random01(seed);
+  const float j1 = gg::random01(seed);
+  const gg::V3 U = gg::v3(p.camera.u[0], p.camera.u[1], p.camera.u[2]);
+  const gg::V3 V = gg::v3(p.camera.v[0], p.camera.v[1], p.camera.v[2]);
+  const gg::V3 W = gg::v3(p.camera.w[0], p.camera.w[1], p.camera.w[2]);
+  gg::V3 o = gg::v3(p.camera.position[0], p.camera.position[1], p.camera.position[2]);
+  gg::V3 d = gg::primary_direction(j0, j1, gx, gy, p.width, p.height, U, V, W);
+  for (int i = 0; i < p.maxPathLength; ++i) {
+    const ClosestHit h = trace_closest<WIDE>(c, o, d);
+    if (!h.hit) break;  // black environment
+    o = o + h.t * d;
+    const gg::V3 n = shading_normal(p.ref.vinfo, h.tri, h.bary);
+    const MaterialData m = load_material(p.materials, h.tri.w);
+    const float cosIn = fmaxf(fabsf(gg::dot(n, d)), 1e-4f);
+    const float dist2 = h.t * h.t;
+    if (i == 0) a0 = sqrtf(dist2 / (4.0f * gg::kPI * cosIn));
+    else spread += sqrtf(dist2 / (fmaxf(pdfPrev, 1e-8f) * cosIn));
+    const gg::V3 te = throughput * (didNEE ? gg::v3s(0.0f) : m.emissivity);
+    if (!term && i >= 1 && spread > cp.spreadC * a0) {
+      color = color + te;
+      write_features(cp.queryFeat + (size_t)pix * kMlpRaw, cp, o, -d, n, m);
+      reinterpret_cast<float4*>(cp.queryWeight)[pix] = make_float4(throughput.x, throughput.y, throughput.z, 1.0f);
+      term = true;
+      termDepth = i;
+      if (!isTrain) break;
+    }
+    if (isTrain && term && i == termDepth + cp.extraBounces) {
+      // the cache stands in for what this vertex reflects
+      cacc = cacc + te;
+      write_features(cp.tailFeat + (size_t)trainIdx * kMlpRaw, cp, o, -d, n, m);
+      tailT = throughput;
+      tail = true;
+      break;
+    }
+    if (isTrain) {
+      float* r = tv + (size_t)nTrain * kTrainVertFloats;
+      write_features(r, cp, o, -d, n, m);
+      const gg::V3 s = cacc + te;
+      reinterpret_cast<float4*>(r)[4] = make_float4(throughput.x, throughput.y, throughput.z, 0.0f);
+      reinterpret_cast<float4*>(r)[5] = make_float4(s.x, s.y, s.z, 0.0f);
+      ++nTrain;
+    }
+    cacc = cacc + te;
+    if (!term) color = color + te;
+    didNEE = m.refl[3] < 500.0f;
+    const int sampleIndex = (int)(gg::xorshift32(seed) % p.numLights);
+    const gg::V3 radiance = estimate_direct_light<WIDE>(c, o, n, sampleIndex, d, m, didNEE);
+    const gg::V3 contrib = radiance * throughput * lightSampleWeight;
+    cacc = cacc + contrib;
+    if (!term) color = color + contrib;
+    gg::V3 tp = gg::v3s(1.0f);
+    float pdf;
+    bool diffuse;
+    d = sample_bsdf(d, m, seed, n, tp, false, pdf, diffuse);
+    pdfPrev = pdf;
+    o = o + d * gg::kRayHitEpsilon;
+    throughput = throughput * tp;
+    const float q = gg::saturate(gg::luminance(tp));
+    if (gg::random01(seed) >= q) break;
+    throughput = throughput / q;
+  }
+  reinterpret_cast<float4*>(cp.pixelColor)[pix] = make_float4(color.x, color.y, color.z, 0.0f);
+  if (!term) {
+    reinterpret_cast<float4*>(cp.queryWeight)[pix] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    zero_features(cp.queryFeat + (size_t)pix * kMlpRaw);
+  }
+  if (isTrain) {
+    cp.trainCount[trainIdx] = nTrain;
+    float4* tp4 = reinterpret_cast<float4*>(cp.trainPath + (size_t)trainIdx * 8);
+    tp4[0] = make_float4(cacc.x, cacc.y, cacc.z, 0.0f);
+    tp4[1] = make_float4(tailT.x, tailT.y, tailT.z, tail ? 1.0f : 0.0f);
+    if (!tail) zero_features(cp.tailFeat + (size_t)trainIdx * kMlpRaw);
+  }
+}
+
+__global__ __launch_bounds__(kTileThreads) void k_cache_resolve(DrawParams p, CacheParams cp) {
+  const int tid = threadIdx.x;
+  const int gx = blockIdx.x * kTile + (tid % kTile);
+  const int gy = blockIdx.y * kTile + (tid / kTile);
+  const size_t pix = (size_t)gy * p.width + gx;
+  float4 col = reinterpret_cast<const float4*>(cp.pixelColor)[pix];
+  const float4 w = reinterpret_cast<const float4*>(cp.queryWeight)[pix];
+  if (w.w > 0.5f) {
+    const float4 y = reinterpret_cast<const float4*>(cp.queryPred)[pix];
+    col.x += w.x * fmaxf(y.x, 0.0f);
+    col.y += w.y * fmaxf(y.y, 0.0f);
+    col.z += w.z * fmaxf(y.z, 0.0f);
+  }
+  float4* px = &p.accum[pix];
+  const float4 v = *px;
+  *px = make_float4(col.x + v.x, col.y + v.y, col.z + v.z, 0.0f + v.w);
+}
+
+__device__ __forceinline__ float target_channel(float C, float Tend, float pred, float S, float T) {
+  if (!(T > 1e-6f)) return 0.0f;
+  const float L = (C + Tend * pred - S) / T;
+  return fminf(fmaxf(L, 0.0f), 1e4f);
+}
+
+__global__ void k_cache_targets(CacheParams cp, uint32_t numPaths, int maxLen, const uint32_t* __restrict__ offsets,
+                                uint32_t maxSamples, float* __restrict__ feat, float* __restrict__ targets) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t j = (uint32_t)(t / (uint32_t)maxLen);
+  const uint32_t i = (uint32_t)(t % (uint32_t)maxLen);
+  if (j >= numPaths || i >= cp.trainCount[j]) return;
+  const uint32_t off = offsets[j] + i;
+  if (off >= maxSamples) return;
+  const float* r = cp.trainVert + ((size_t)j * maxLen + i) * kTrainVertFloats;
+  for (int k = 0; k < 4; ++k)
+    reinterpret_cast<float4*>(feat + (size_t)off * kMlpRaw)[k] = reinterpret_cast<const float4*>(r)[k];
+  const float4 T = reinterpret_cast<const float4*>(r)[4];
+  const float4 S = reinterpret_cast<const float4*>(r)[5];
+  const float4 C = reinterpret_cast<const float4*>(cp.trainPath + (size_t)j * 8)[0];
+  const float4 Te = reinterpret_cast<const float4*>(cp.trainPath + (size_t)j * 8)[1];
+  float4 pr = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  if (Te.w > 0.5f) {
+    pr = reinterpret_cast<const float4*>(cp.tailPred)[j];
+    pr.x = fmaxf(pr.x, 0.0f);
+    pr.y = fmaxf(pr.y, 0.0f);
+    pr.z = fmaxf(pr.z, 0.0f);
+  }
+  reinterpret_cast<float4*>(targets)[off] =
+      make_float4(target_channel(C.x, Te.x, pr.x, S.x, T.x), target_channel(C.y, Te.y, pr.y, S.y, T.y),
+                  target_channel(C.z, Te.z, pr.z, S.z, T.z), 0.0f);
+}
+
+__global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t v, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+}  // namespace
+
+hipError_t LaunchFillU32(uint32_t* p, uint32_t value, size_t n, hipStream_t stream) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, p, value, n);
+  return hipGetLastError();
+}
+
+hipError_t LaunchCachedPaths(const DrawParams& p, const CacheParams& cp, void* spill, hipStream_t stream) {
+  dim3 grid(p.width / kTile, p.height / kTile);
+  if (grid.x == 0 || grid.y == 0) return hipSuccess;
+  if (cp.trainStride == 0 || cp.trainOffset >= cp.trainStride) return hipErrorInvalidValue;
+  if (p.useWide) {
+    DrawParams q = p;
+    q.wide.spill = (uint2*)spill;
+    hipLaunchKernelGGL(k_cached_paths<true>, grid, dim3(kTileThreads), 0, stream, q, cp);
+  } else {
+    hipLaunchKernelGGL(k_cached_paths<false>, grid, dim3(kTileThreads), 0, stream, p, cp);
+  }
+  return hipGetLastError();
+}
+
+hipError_t LaunchCacheResolve(const DrawParams& p, const CacheParams& cp, hipStream_t stream) {
+  dim3 grid(p.width / kTile, p.height / kTile);
+  if (grid.x == 0 || grid.y == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_cache_resolve, grid, dim3(kTileThreads), 0, stream, p, cp);
+  return hipGetLastError();
+}
+
+hipError_t LaunchCacheTargets(const DrawParams& p, const CacheParams& cp, const uint32_t* offsets, uint32_t maxSamples,
+                              float* feat, float* targets, hipStream_t stream) {
+  const uint32_t paths = CacheTrainPaths(p.width, p.height, cp.trainStride);
+  const size_t threads = (size_t)paths * p.maxPathLength;
+  if (threads == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_cache_targets, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, stream, cp, paths,
+                     p.maxPathLength, offsets, maxSamples, feat, targets);
+  return hipGetLastError();
+}
+
+}  // namespace gpugi

@@ -12,6 +12,7 @@
 #include "integrators.h"
 #include "launch.h"
 #include "mlp.h"
+#include "scan.h"
 
 using namespace gpugi;
 
@@ -62,6 +63,16 @@ struct gpugi_ctx {
   bool hasMlp = false;
   MlpDev mlp{};
   uint32_t mlpStep = 0;
+  float bboxMin[3] = {0, 0, 0}, bboxMax[3] = {0, 0, 0};
+  // cached path tracer (GPUGI_PT_CACHE)
+  CacheParams cacheCfg;
+  uint32_t cacheMaxSamples = 65536;
+  int cacheTrainSteps = 4;
+  float cacheLr = 1e-3f;
+  uint32_t cacheLastSamples = 0;
+  float cacheLastLoss = 0.0f;
+  void* dCache = nullptr;
+  size_t cacheBytes = 0;
   float* dMlpLoss = nullptr;  // loss of the last gradient / training step
   void* dMlpIo = nullptr;     // host-pointer staging of features / targets / outputs
   size_t mlpIoCap = 0;
@@ -224,7 +235,7 @@ void gpugi_destroy(gpugi_ctx* ctx) {
   FreeIntegratorState(&ctx->integ);
   for (void** p : {(void**)&ctx->mlp.w, (void**)&ctx->mlp.m, (void**)&ctx->mlp.v, (void**)&ctx->mlp.wb,
                    (void**)&ctx->mlp.wtb, (void**)&ctx->mlp.partials, (void**)&ctx->mlp.grad,
-                   (void**)&ctx->mlp.loss, (void**)&ctx->dMlpLoss, &ctx->dMlpIo})
+                   (void**)&ctx->mlp.loss, (void**)&ctx->dMlpLoss, &ctx->dMlpIo, &ctx->dCache})
     FreeDev(*p);
   for (void* p : ctx->userAllocs) hipFree(p);
   if (ctx->ev0) hipEventDestroy(ctx->ev0);
@@ -296,6 +307,16 @@ int gpugi_load_scene(gpugi_ctx* ctx, const gpugi_scene_desc* d) {
   }
   ctx->hNumLeaves = nl;
   ctx->hNumVertices = nv;
+  for (int a = 0; a < 3; ++a) {
+    ctx->bboxMin[a] = INFINITY;
+    ctx->bboxMax[a] = -INFINITY;
+  }
+  for (uint32_t v = 0; v < nv; ++v)
+    for (int a = 0; a < 3; ++a) {
+      const float x = d->positions[(size_t)v * 3 + a];
+      ctx->bboxMin[a] = std::min(ctx->bboxMin[a], x);
+      ctx->bboxMax[a] = std::max(ctx->bboxMax[a], x);
+    }
   if (int rc = RebuildWide(ctx)) return rc;
   HIP_TRY(ctx, hipMemset(ctx->dCounters, 0, 64));
   if (ctx->dAccum) {
@@ -368,12 +389,114 @@ int gpugi_set_integrator_kernel(gpugi_ctx* ctx, int kernel) {
   return GPUGI_OK;
 }
 
+namespace {
+
+size_t Align256(size_t b) { return (b + 255) & ~(size_t)255; }
+int MlpEnsureTiles(gpugi_ctx* ctx, uint32_t tiles);
+
+// Scene bounds as the cache's position mapping: min and 1 / extent per axis.
+int SceneBounds(gpugi_ctx* ctx, float mn[3], float invExtent[3]) {
+  if (!ctx->hasScene) return Fail(ctx, GPUGI_ERR_STATE, "no scene loaded");
+  for (int a = 0; a < 3; ++a) {
+    const float e = ctx->bboxMax[a] - ctx->bboxMin[a];
+    mn[a] = ctx->bboxMin[a];
+    invExtent[a] = e > 0.0f ? 1.0f / e : 0.0f;
+  }
+  return GPUGI_OK;
+}
+
+// One frame of the cached path tracer: short paths and training paths, cache
+// inference at the termination and tail vertices, resolve into the
+// accumulation buffer, training targets, then `cacheTrainSteps` Adam steps.
+int DrawCached(gpugi_ctx* ctx, uint32_t frameSeed) {
+  if (!ctx->hasMlp) return Fail(ctx, GPUGI_ERR_STATE, "gpugi_draw(PT_CACHE): call gpugi_mlp_init first");
+  DrawParams p = MakeDrawParams(ctx, frameSeed);
+  CacheParams cp = ctx->cacheCfg;
+  const size_t P = (size_t)ctx->width * ctx->height;
+  const uint32_t NT = CacheTrainPaths(ctx->width, ctx->height, cp.trainStride);
+  const size_t L = (size_t)p.maxPathLength;
+  const uint32_t S = ctx->cacheMaxSamples;
+  size_t off[14], total = 0;
+  const size_t sizes[14] = {P * 16, P * 64, P * 16, P * 16, (size_t)NT * L * 96, (size_t)NT * 4, (size_t)NT * 4,
+                            (size_t)NT * 32, (size_t)NT * 64, (size_t)NT * 16, ExclusiveScanTempBytes(NT), 4,
+                            (size_t)S * 64, (size_t)S * 16};
+  for (int i = 0; i < 14; ++i) {
+    off[i] = total;
+    total += Align256(sizes[i]);
+  }
+  if (ctx->cacheBytes < total) {
+    FreeDev(ctx->dCache);
+    ctx->cacheBytes = 0;
+    HIP_TRY(ctx, hipMalloc(&ctx->dCache, total));
+    ctx->cacheBytes = total;
+  }
+  char* b = (char*)ctx->dCache;
+  cp.pixelColor = (float*)(b + off[0]);
+  cp.queryFeat = (float*)(b + off[1]);
+  cp.queryWeight = (float*)(b + off[2]);
+  float* queryPred = (float*)(b + off[3]);
+  cp.queryPred = queryPred;
+  cp.trainVert = (float*)(b + off[4]);
+  cp.trainCount = (uint32_t*)(b + off[5]);
+  uint32_t* offsets = (uint32_t*)(b + off[6]);
+  cp.trainPath = (float*)(b + off[7]);
+  cp.tailFeat = (float*)(b + off[8]);
+  float* tailPred = (float*)(b + off[9]);
+  cp.tailPred = tailPred;
+  void* scanTemp = b + off[10];
+  uint32_t* dTotal = (uint32_t*)(b + off[11]);
+  float* feat = (float*)(b + off[12]);
+  float* targets = (float*)(b + off[13]);
+  cp.trainOffset = frameSeed % cp.trainStride;
+  if (int rc = SceneBounds(ctx, cp.bboxMin, cp.bboxInvExtent)) return rc;
+
+  HIP_TRY(ctx, LaunchFillU32(cp.trainCount, 0u, NT, ctx->stream));
+  void* spill = nullptr;
+  if (p.useWide) {
+    const size_t threads = (size_t)(ctx->width / 8) * (ctx->height / 8) * 64;
+    HIP_TRY(ctx, EnsureBuffer(&ctx->integ.spill, &ctx->integ.spillBytes,
+                              std::max<size_t>(8, threads * p.wide.spillDepth * sizeof(uint2))));
+    spill = ctx->integ.spill;
+  }
+  HIP_TRY(ctx, LaunchCachedPaths(p, cp, spill, ctx->stream));
+  HIP_TRY(ctx, LaunchMlpInfer(ctx->mlp, cp.queryFeat, queryPred, P, ctx->stream));
+  HIP_TRY(ctx, LaunchMlpInfer(ctx->mlp, cp.tailFeat, tailPred, NT, ctx->stream));
+  HIP_TRY(ctx, LaunchCacheResolve(p, cp, ctx->stream));
+  HIP_TRY(ctx, LaunchExclusiveScanU32(cp.trainCount, offsets, NT, scanTemp, dTotal, ctx->stream));
+  HIP_TRY(ctx, LaunchCacheTargets(p, cp, offsets, S, feat, targets, ctx->stream));
+  uint32_t count = 0;
+  HIP_TRY(ctx, hipMemcpyAsync(&count, dTotal, 4, hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  const uint32_t n = count < S ? count : S;
+  ctx->cacheLastSamples = n;
+  const int steps = ctx->cacheTrainSteps;
+  const uint32_t chunk = steps > 0 ? n / (uint32_t)steps : 0;
+  if (chunk >= 32) {
+    if (int rc = MlpEnsureTiles(ctx, MlpTiles(chunk))) return rc;
+    for (int s = 0; s < steps; ++s) {
+      HIP_TRY(ctx, LaunchMlpGradient(ctx->mlp, feat + (size_t)s * chunk * kMlpRaw, targets + (size_t)s * chunk * 4,
+                                     chunk, ctx->dMlpLoss, ctx->stream));
+      AdamParams ap{ctx->cacheLr, 0.9f, 0.99f, 1e-8f, ++ctx->mlpStep};
+      HIP_TRY(ctx, LaunchMlpAdam(ctx->mlp, ap, ctx->stream));
+    }
+  }
+  return GPUGI_OK;
+}
+
+}  // namespace
+
 int gpugi_draw(gpugi_ctx* ctx, int integrator, uint32_t frame_seed) {
   if (!ctx) return Fail(ctx, GPUGI_ERR_ARG, "gpugi_draw: null context");
   if (!ctx->hasScene || !ctx->dAccum || !ctx->hasCamera || ctx->numLights == 0)
     return Fail(ctx, GPUGI_ERR_STATE, "gpugi_draw: scene, screen, camera and light samples must be set first");
-  if (integrator < GPUGI_PT || integrator > GPUGI_BPT) return Fail(ctx, GPUGI_ERR_ARG, "gpugi_draw: unknown integrator");
+  if (integrator < GPUGI_PT || integrator > GPUGI_PT_CACHE)
+    return Fail(ctx, GPUGI_ERR_ARG, "gpugi_draw: unknown integrator");
   if (SetDev(ctx)) return GPUGI_ERR_HIP;
+  if (integrator == GPUGI_PT_CACHE) {
+    if (int rc = DrawCached(ctx, frame_seed)) return rc;
+    ++ctx->iterations;
+    return GPUGI_OK;
+  }
   DrawParams p = MakeDrawParams(ctx, frame_seed);
   std::string err;
   hipError_t e = DrawIntegrator(&ctx->integ, integrator, p, ctx->stream, &err);
@@ -902,6 +1025,34 @@ int gpugi_mlp_train_step(gpugi_ctx* ctx, const float* features, const float* tar
   if (loss) {
     HIP_TRY(ctx, hipMemcpyAsync(loss, ctx->dMlpLoss, sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  }
+  return GPUGI_OK;
+}
+
+int gpugi_cache_config(gpugi_ctx* ctx, float spread_c, int extra_bounces, uint32_t train_stride,
+                       uint32_t max_train_samples, int train_steps, float learning_rate) {
+  if (!ctx) return Fail(ctx, GPUGI_ERR_ARG, "gpugi_cache_config: null context");
+  if (!(spread_c > 0.0f) || extra_bounces < 0 || extra_bounces > 16 || train_stride == 0 || max_train_samples == 0 ||
+      max_train_samples > (1u << 24) || train_steps < 0 || train_steps > 64 || !(learning_rate > 0.0f))
+    return Fail(ctx, GPUGI_ERR_ARG, "gpugi_cache_config: invalid argument");
+  ctx->cacheCfg.spreadC = spread_c;
+  ctx->cacheCfg.extraBounces = extra_bounces;
+  ctx->cacheCfg.trainStride = train_stride;
+  ctx->cacheMaxSamples = max_train_samples;
+  ctx->cacheTrainSteps = train_steps;
+  ctx->cacheLr = learning_rate;
+  return GPUGI_OK;
+}
+
+int gpugi_cache_stats(gpugi_ctx* ctx, uint32_t* train_samples, float* loss) {
+  if (!ctx) return Fail(ctx, GPUGI_ERR_ARG, "gpugi_cache_stats: null context");
+  if (train_samples) *train_samples = ctx->cacheLastSamples;
+  if (loss) {
+    *loss = 0.0f;
+    if (ctx->hasMlp) {
+      HIP_TRY(ctx, hipMemcpyAsync(loss, ctx->dMlpLoss, sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+      HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    }
   }
   return GPUGI_OK;
 }

@@ -85,6 +85,41 @@ struct BptWarmupCounts {
 };
 hipError_t BptWarmup(IntegratorState* st, const DrawParams& p, hipStream_t stream, BptWarmupCounts* out);
 
+// ---------------------------------------------------------------- radiance cache
+
+// Query / training records of the cached path tracer.  Feature vectors are the
+// 16-float raw cache inputs (mlp.h).
+struct CacheParams {
+  float spreadC = 0.01f;          // terminate when the path spread exceeds spreadC * a0
+  int extraBounces = 2;           // training paths continue this many vertices past termination
+  uint32_t trainStride = 32;      // one training path per trainStride pixels
+  uint32_t trainOffset = 0;       // which pixel of each stride (varies per frame)
+  float bboxMin[3] = {0, 0, 0};
+  float bboxInvExtent[3] = {1, 1, 1};
+  float* pixelColor = nullptr;    // W*H float4: the short path's own contributions
+  float* queryFeat = nullptr;     // W*H x 16: features at the termination vertex
+  float* queryWeight = nullptr;   // W*H float4: throughput there (w = 1 when a query exists)
+  float* trainVert = nullptr;     // numTrainPaths * maxPathLength x 24: features, T_i, S_i
+  uint32_t* trainCount = nullptr; // training vertices per path
+  float* trainPath = nullptr;     // numTrainPaths x 8: accumulated C, tail throughput (w = tail valid)
+  float* tailFeat = nullptr;      // numTrainPaths x 16
+  const float* queryPred = nullptr;  // W*H float4 predictions (resolve)
+  const float* tailPred = nullptr;   // numTrainPaths float4 predictions (targets)
+};
+inline uint32_t CacheTrainPaths(int width, int height, uint32_t stride) {
+  return (uint32_t)(((size_t)width * height + stride - 1) / stride);
+}
+// p[0..n) = value (a kernel, so the call can be recorded into a graph).
+hipError_t LaunchFillU32(uint32_t* p, uint32_t value, size_t n, hipStream_t stream);
+// Traces the short paths (and the extended training paths) of one frame.
+hipError_t LaunchCachedPaths(const DrawParams& p, const CacheParams& cp, void* spill, hipStream_t stream);
+// accum += pixelColor + queryWeight * max(queryPred, 0).
+hipError_t LaunchCacheResolve(const DrawParams& p, const CacheParams& cp, hipStream_t stream);
+// Dense training samples: features n x 16 and targets n x 4 at the exclusive
+// offsets `offsets` of the per-path vertex counts, at most maxSamples.
+hipError_t LaunchCacheTargets(const DrawParams& p, const CacheParams& cp, const uint32_t* offsets, uint32_t maxSamples,
+                              float* feat, float* targets, hipStream_t stream);
+
 void FreeIntegratorState(IntegratorState* s);
 void InvalidateIntegratorCaches(IntegratorState* s);
 hipError_t DrawIntegrator(IntegratorState* st, int integrator, const DrawParams& p, hipStream_t stream,

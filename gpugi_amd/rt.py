@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(_HERE, "lib", "libgpugi_rt.so")
 GPUGI_OK = 0
 CLOSEST_HIT, ANY_HIT = 0, 1
 KERNEL_AUTO, KERNEL_REFERENCE, KERNEL_WIDE = 0, 1, 2
-PT, LT, BPT = 0, 1, 2
+PT, LT, BPT, PT_CACHE = 0, 1, 2, 3
 RAY_MAX = np.float32(3.402823466e38)
 
 RAY_DTYPE = np.dtype([("origin", "<f4", 3), ("tmax", "<f4"), ("dir", "<f4", 3), ("pad", "<u4")])
@@ -114,6 +114,9 @@ _SIGNATURES = {
     "gpugi_lightsampler_set_scene": [_P, _P],
     "gpugi_lightsampler_destroy": ([_P], None),
     "gpugi_wanghash": ([ctypes.c_uint32], ctypes.c_uint32),
+    "gpugi_cache_config": [_P, ctypes.c_float, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int,
+                           ctypes.c_float],
+    "gpugi_cache_stats": [_P, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_float)],
     "gpugi_mlp_init": [_P, ctypes.c_uint64],
     "gpugi_mlp_set_weights": [_P, _P, ctypes.c_size_t],
     "gpugi_mlp_get_weights": [_P, _P, ctypes.c_size_t],
@@ -458,6 +461,17 @@ class Context:
                    ctypes.c_size_t(n))
 
     # -- radiance-cache MLP
+    def cache_config(self, spread_c=0.01, extra_bounces=2, train_stride=32, max_train_samples=65536,
+                     train_steps=4, learning_rate=1e-3):
+        self._call("gpugi_cache_config", ctypes.c_float(spread_c), int(extra_bounces), int(train_stride),
+                   int(max_train_samples), int(train_steps), ctypes.c_float(learning_rate))
+
+    def cache_stats(self):
+        n = ctypes.c_uint32()
+        loss = ctypes.c_float()
+        self._call("gpugi_cache_stats", ctypes.byref(n), ctypes.byref(loss))
+        return n.value, loss.value
+
     def mlp_init(self, seed=1):
         self._call("gpugi_mlp_init", ctypes.c_uint64(seed))
 

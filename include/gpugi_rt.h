@@ -56,7 +56,9 @@ enum gpugi_kernel {
 enum gpugi_integrator {
   GPUGI_PT = 0,               /* Pathtracer::Draw (pathtracer.cpp:47-53) */
   GPUGI_LT = 1,               /* LightPathtracer::Draw (lightpathtracer.cpp:45-51) */
-  GPUGI_BPT = 2               /* BidirectionalPathtracer::Draw (bidirectionalpathtracer.cpp:182-203) */
+  GPUGI_BPT = 2,              /* BidirectionalPathtracer::Draw (bidirectionalpathtracer.cpp:182-203) */
+  GPUGI_PT_CACHE = 3          /* path tracer terminating into the radiance-cache MLP, trained online
+                                 each frame (no reference counterpart; needs gpugi_mlp_init) */
 };
 
 /* A ray query: 32 bytes.  tmax is the initial rayLength of TraceRay /
@@ -348,6 +350,18 @@ int gpugi_mlp_gradient(gpugi_ctx* ctx, const float* features, const float* targe
  * (optional) receives the loss before the step. */
 int gpugi_mlp_train_step(gpugi_ctx* ctx, const float* features, const float* targets, size_t n,
                          float learning_rate, float* loss, int on_device);
+
+/* Cached path tracer (GPUGI_PT_CACHE): a path terminates into the cache at
+ * the first vertex from the second hit on whose footprint spread exceeds
+ * spread_c times the primary one (default 0.01); one pixel in train_stride
+ * (default 32) also traces a training path extra_bounces (default 2) vertices
+ * further; up to max_train_samples (default 65536) vertices of those paths
+ * train the cache with train_steps (default 4) Adam steps of
+ * learning_rate (default 1e-3) after every frame. */
+int gpugi_cache_config(gpugi_ctx* ctx, float spread_c, int extra_bounces, uint32_t train_stride,
+                       uint32_t max_train_samples, int train_steps, float learning_rate);
+/* Training samples and loss of the last cached frame. */
+int gpugi_cache_stats(gpugi_ctx* ctx, uint32_t* train_samples, float* loss);
 
 /* WangHash (utilities/random.cpp:5-13): the per-iteration FrameSeed. */
 uint32_t gpugi_wanghash(uint32_t x);
