@@ -46,6 +46,11 @@ def parse():
                    help="write profiles/<NAME>.json from rocprofv3 --pmc output directories and exit")
     p.add_argument("--sweep", action="store_true",
                    help="print Grays/s of every wide-kernel configuration on the three ray sets and exit")
+    p.add_argument("--mlp", action="store_true",
+                   help="time-to-equal-RMSE of the cached path tracer against the plain one (heightfield, 1920x1080)")
+    p.add_argument("--mlp-ref-spp", type=int, default=4096, help="samples per pixel of the plain reference render")
+    p.add_argument("--mlp-frames", type=int, default=256, help="largest frame count of the two progressive renders")
+    p.add_argument("--mlp-warmup", type=int, default=32, help="cached frames drawn before accumulation starts")
     p.add_argument("--calibrate", action="store_true",
                    help="append 3 device-to-device copies of 1 GiB (counter calibration for rocprofv3 --pmc)")
     return p.parse_args()
@@ -186,6 +191,100 @@ def summarize_counters(name, *dirs):
     print(json.dumps(out))
 
 
+MLP_PATH_LENGTH = 8
+MLP_LIGHT_INTENSITY = 0.5
+
+
+def mlp_benchmark(ref_spp, max_frames, warmup=32):
+    """Plain vs cached path tracing on the heightfield at 1920 x 1080.
+
+    Both render progressively from frame 0; the cache is trained online from
+    scratch and its first `warmup` frames are drawn but not accumulated (their
+    time is charged to the cached render).  RMSE is taken against a plain render of
+    `ref_spp` samples per pixel drawn with other frame seeds.  For each cached
+    frame count F the plain sample count N with the same RMSE is interpolated
+    (log-log) and the GPU-time ratio N * t_plain / (F * t_cached) reported.
+    """
+    from gpugi_amd import rt, workloads
+    W, H = workloads.HF_WIDTH, workloads.HF_HEIGHT
+    hs = rt.HostScene.heightfield(workloads.HF_CELLS, workloads.MASTER_SEED)
+    for pos in workloads.heightfield_lights():
+        hs.add_point_light(pos, (MLP_LIGHT_INTENSITY,) * 3)
+    ctx = rt.Context(0)
+    ctx.load_scene(hs)
+    ctx.set_screen(W, H)
+    ctx.set_camera(workloads.heightfield_camera(W, H))
+    ctx.set_max_path_length(MLP_PATH_LENGTH)
+    ctx.set_integrator_kernel(rt.KERNEL_WIDE)
+    sampler = rt.LightSampler(hs, 123)
+    # light-sample batches (128 per frame, as the path tracer uses) cycled through a fixed pool,
+    # so frame k of every render sees the same batch
+    pool = [sampler.generate(128) for _ in range(64)]
+
+    def run(integrator, frames, first, checkpoints, warmup=0):
+        """Frames `warmup`.. are accumulated; returns images at the checkpoints and ms per frame."""
+        ctx.reset_accum()
+        out, ms = {}, 0.0
+        for k in range(warmup + frames):
+            if k == warmup:
+                ctx.reset_accum()
+            ctx.set_light_samples(pool[(first + k) % len(pool)])
+            ctx.sync()
+            ctx.timer_begin()
+            ctx.draw(integrator, rt.wanghash(first + k))
+            ms += ctx.timer_end()
+            f = k + 1 - warmup
+            if f in checkpoints:
+                img, _ = ctx.readback(W, H)
+                out[f] = (img[..., :3] / np.float32(f)).copy()
+        return out, ms / (warmup + frames)
+
+    t0 = time.perf_counter()
+    ref, ref_ms = run(rt.PT, ref_spp, 1 << 20, {ref_spp})
+    ref = ref[ref_spp].astype(np.float64)
+    checks = sorted({1 << i for i in range(0, 16) if (1 << i) <= max_frames} | {max_frames})
+    plain, plain_ms = run(rt.PT, max_frames, 0, set(checks))
+    ctx.mlp_init(1)
+    ctx.cache_config()
+    cached, cached_ms = run(rt.PT_CACHE, max_frames, 0, set(checks), warmup)
+    samples, loss = ctx.cache_stats()
+
+    def rmse(img):
+        return float(np.sqrt(np.mean((img.astype(np.float64) - ref) ** 2)))
+
+    rp = {n: rmse(plain[n]) for n in checks}
+    rc = {n: rmse(cached[n]) for n in checks}
+    ln = np.log([float(n) for n in checks])
+    lp = np.log([rp[n] for n in checks])
+    rows = []
+    for f in checks:
+        target = rc[f]
+        # plain RMSE falls monotonically with spp; interpolate log(spp) at log(target)
+        if target >= rp[checks[0]]:
+            n_eq = float(checks[0]) * (rp[checks[0]] / target) ** 2
+        elif target <= rp[checks[-1]]:
+            n_eq = float(checks[-1]) * (rp[checks[-1]] / target) ** 2  # 1/sqrt(N) beyond the last point
+        else:
+            n_eq = float(np.exp(np.interp(-np.log(target), -lp, ln)))
+        rows.append({"frames": f, "rmse_cached": round(rc[f], 6), "plain_spp_equal_rmse": round(n_eq, 2),
+                     "gpu_time_ratio": round(n_eq * plain_ms / ((warmup + f) * cached_ms), 3)})
+    ctx.close()
+    best = max(rows, key=lambda r: r["gpu_time_ratio"])
+    return {
+        "metric": "time-to-equal-RMSE, MLP cache vs plain PT (GPU-time ratio, higher is better)",
+        "value": best["gpu_time_ratio"], "unit": "x", "higher_is_better": True,
+        "config": {"workload": "heightfield 1024x1024 cells, 4 point lights, 1920x1080, path length %d, "
+                               "wide traversal" % MLP_PATH_LENGTH,
+                   "reference_spp": ref_spp, "frames": max_frames, "cache": "96-64x4-3 BF16 MLP, spread c 0.01, "
+                   "1 training path per 32 pixels, 65536 samples x 4 Adam steps per frame, lr 1e-3"},
+        "ms_per_frame_plain": round(plain_ms, 3), "ms_per_frame_cached": round(cached_ms, 3),
+        "reference_ms_per_spp": round(ref_ms, 3), "warmup_frames": warmup,
+        "rmse_plain": {str(n): round(rp[n], 6) for n in checks},
+        "rows": rows, "last_train_samples": samples, "last_loss": loss,
+        "wall_s": round(time.perf_counter() - t0, 1),
+    }
+
+
 def cpu_baseline(scene_arrays, rays_host, seconds):
     from oracle import oracle
     sc = oracle.Scene(scene_arrays)
@@ -209,9 +308,12 @@ def main():
         summarize_counters(*args.summarize_counters)
         return
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU; --gpus %d is not supported (rays shard with zero "
-              "communication, see DESIGN.md)" % args.gpus, file=sys.stderr)
+        print("bench.py: this build measures one GPU; --gpus %d is not supported (ray sets and pixels would "
+              "shard across GPUs with no communication during traversal)" % args.gpus, file=sys.stderr)
         sys.exit(2)
+    if args.mlp:
+        print(json.dumps(mlp_benchmark(args.mlp_ref_spp, args.mlp_frames, args.mlp_warmup)))
+        return
     from gpugi_amd import rt, workloads
 
     t_setup = time.perf_counter()

@@ -124,9 +124,11 @@ __global__ __launch_bounds__(kTileThreads) void k_cached_paths(DrawParams p, Cac
     pdfPrev = pdf;
     o = o + d * gg::kRayHitEpsilon;
     throughput = throughput * tp;
-    const float q = gg::saturate(gg::luminance(tp));
-    if (gg::random01(seed) >= q) break;
-    throughput = throughput / q;
+    if (i >= p.rrStartDepth) {  // Russian roulette (pathtracer.comp:99-104) from rrStartDepth on
+      const float q = gg::saturate(gg::luminance(tp));
+      if (gg::random01(seed) >= q) break;
+      throughput = throughput / q;
+    }
   }
   reinterpret_cast<float4*>(cp.pixelColor)[pix] = make_float4(color.x, color.y, color.z, 0.0f);
   if (!term) {

@@ -55,6 +55,7 @@ struct gpugi_ctx {
   void* dLights = nullptr;
   uint32_t numLights = 0, lightCap = 0;
   int maxPathLength = 16;  // default pathLength (application.cpp:123)
+  int rrStartDepth = 0;
   int integratorKernel = GPUGI_KERNEL_AUTO;
   int bptCapacity = -1;
   int bptAverageLength = 0;
@@ -520,6 +521,7 @@ static DrawParams MakeDrawParams(gpugi_ctx* ctx, uint32_t frame_seed) {
   p.accum = (float4*)ctx->dAccum;
   p.frameSeed = frame_seed;
   p.maxPathLength = ctx->maxPathLength;
+  p.rrStartDepth = ctx->rrStartDepth;
   p.counters = (uint32_t*)ctx->dCounters;
   p.bptCapacity = ctx->bptCapacity;
   return p;
@@ -1054,5 +1056,12 @@ int gpugi_cache_stats(gpugi_ctx* ctx, uint32_t* train_samples, float* loss) {
       HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     }
   }
+  return GPUGI_OK;
+}
+
+int gpugi_set_rr_start_depth(gpugi_ctx* ctx, int depth) {
+  if (!ctx) return Fail(ctx, GPUGI_ERR_ARG, "gpugi_set_rr_start_depth: null context");
+  if (depth < 0 || depth > 64) return Fail(ctx, GPUGI_ERR_ARG, "gpugi_set_rr_start_depth: depth in [0, 64]");
+  ctx->rrStartDepth = depth;
   return GPUGI_OK;
 }

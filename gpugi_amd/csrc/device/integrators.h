@@ -22,6 +22,7 @@ struct DrawParams {
   float4* accum = nullptr;
   uint32_t frameSeed = 0;
   int maxPathLength = 16;
+  int rrStartDepth = 0;                   // Russian roulette from this bounce on (0: every bounce, as the reference)
   uint32_t* counters = nullptr;
   int bptCapacity = -1;                   // LightCacheCapacity (bidirectionalpathtracer.cpp:110-112)
 };

@@ -53,9 +53,11 @@ __global__ __launch_bounds__(kTileThreads) void k_pathtracer(DrawParams p) {
     d = sample_bsdf(d, m, seed, n, tp, false, pdf, diffuse);
     o = o + d * gg::kRayHitEpsilon;
     throughput = throughput * tp;
-    const float q = gg::saturate(gg::luminance(tp));
-    if (gg::random01(seed) >= q) break;
-    throughput = throughput / q;
+    if (i >= p.rrStartDepth) {  // Russian roulette (pathtracer.comp:99-104) from rrStartDepth on
+      const float q = gg::saturate(gg::luminance(tp));
+      if (gg::random01(seed) >= q) break;
+      throughput = throughput / q;
+    }
   }
   float4* px = &p.accum[(size_t)gy * p.width + gx];
   float4 v = *px;

@@ -158,9 +158,11 @@ __global__ __launch_bounds__(kLightBlock) void k_lighttracer(DrawParams p, int r
     const gg::V3 nd = sample_bsdf(d, m, seed, sN, t1, true, pdf, diffuse);
     t1 = t1 * adjoint_correction(d, nd, h.geoNormal, sN);
     tp = tp * t1;
-    const float q = gg::saturate(gg::luminance(t1));
-    if (gg::random01(seed) >= q) break;
-    tp = tp / q;
+    if (i >= p.rrStartDepth) {  // Russian roulette (lighttracer.comp:183-188) from rrStartDepth on
+      const float q = gg::saturate(gg::luminance(t1));
+      if (gg::random01(seed) >= q) break;
+      tp = tp / q;
+    }
     d = nd;
     o = o + d * gg::kRayHitEpsilon;
   }
@@ -332,9 +334,11 @@ __global__ __launch_bounds__(kTileThreads) void k_bpt_eye(DrawParams p, const Li
     d = sample_bsdf(d, m, seed, n, t1, false, samplePDF, diffuse);
     o = o + d * gg::kRayHitEpsilon;
     tp = tp * t1;
-    const float q = gg::saturate(gg::luminance(t1));
-    if (gg::random01(seed) >= q) break;
-    tp = tp / q;
+    if (i >= p.rrStartDepth) {  // Russian roulette from rrStartDepth on
+      const float q = gg::saturate(gg::luminance(t1));
+      if (gg::random01(seed) >= q) break;
+      tp = tp / q;
+    }
     float reverse = mis_heuristic(samplePDF);
     if (i == 0) {
       mis = reverse / fmaxf(reverse + lastConnectionProbability, gg::kDivisorEpsilon);

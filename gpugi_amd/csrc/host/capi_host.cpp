@@ -116,6 +116,27 @@ int gpugi_hostscene_heightfield(uint32_t cells, uint32_t seed, gpugi_hostscene**
   GPUGI_GUARD_END
 }
 
+int gpugi_hostscene_clutter(uint32_t spheres, uint32_t seed, gpugi_hostscene** out) {
+  GPUGI_GUARD_BEGIN
+  if (!out || spheres > 100000) {
+    SetGlobalError("gpugi_hostscene_clutter: spheres must be in [0, 100000]");
+    return GPUGI_ERR_ARG;
+  }
+  std::vector<gpugi::RawVertex> v;
+  std::vector<gpugi::RawTriangle> t;
+  std::vector<gpugi::PointLight> lights;
+  auto* hs = new gpugi_hostscene();
+  gpugi::MakeClutterMesh(spheres, seed, &v, &t, &hs->scene.materials, &lights);
+  std::string err = gpugi::BuildSweepBVH(v, t, 2, &hs->scene);
+  int rc = FinishScene(hs, err, out);
+  if (rc == GPUGI_OK) {
+    (*out)->scene.pointLights = lights;
+    (*out)->scene.ComputePointLightTable();
+  }
+  return rc;
+  GPUGI_GUARD_END
+}
+
 int gpugi_hostscene_load(const char* rawscene_path, const char* json_path, gpugi_hostscene** out) {
   GPUGI_GUARD_BEGIN
   if (!rawscene_path || !out) return GPUGI_ERR_ARG;

@@ -95,6 +95,11 @@ std::string LoadRawScene(const std::string& path, const std::string& jsonPath, H
 std::string SaveRawScene(const HostScene& scene, const std::string& path, const std::string& jsonPath);
 
 // Generators.
+// 20 x 20 x 8 room with `spheres` icospheres (subdivision 3, 1280 triangles each),
+// radii log-uniform in [0.05, 0.8]; 60% Lambertian / 40% Phong-glossy materials in
+// 240 bins; a 4-triangle ceiling area light and 3 point lights (returned in `lights`).
+void MakeClutterMesh(uint32_t spheres, uint32_t seed, std::vector<RawVertex>* v, std::vector<RawTriangle>* t,
+                     std::vector<gpugi_material_params>* m, std::vector<PointLight>* lights);
 void MakeCornellMesh(std::vector<RawVertex>* v, std::vector<RawTriangle>* t,
                      std::vector<gpugi_material_params>* m);
 void MakeHeightfieldMesh(uint32_t cells, uint32_t seed, std::vector<RawVertex>* v,

@@ -1,7 +1,10 @@
-// Deterministic synthetic scenes: the Cornell-style room (36 triangles) and the
-// fBm heightfield of the traversal microbenchmark.
+// Deterministic synthetic scenes: the Cornell-style room (36 triangles), the
+// fBm heightfield of the traversal microbenchmark and the clutter room of
+// icospheres (5,120,016 triangles at 4000 spheres).
 #include <cmath>
+#include <cstdio>
 #include <cstring>
+#include <utility>
 
 #include "../common/gmath.h"
 #include "scene.hpp"
@@ -83,7 +86,139 @@ double Fbm(double x, double z, uint32_t seed) {
   return sum / norm;
 }
 
+// Unit icosphere: the icosahedron subdivided `levels` times (20 * 4^levels
+// triangles), vertices projected onto the sphere; midpoints are shared.
+void Icosphere(int levels, std::vector<P3>* verts, std::vector<uint32_t>* idx) {
+  const double g = (1.0 + std::sqrt(5.0)) / 2.0;
+  *verts = {{-1, g, 0}, {1, g, 0}, {-1, -g, 0}, {1, -g, 0}, {0, -1, g}, {0, 1, g},
+            {0, -1, -g}, {0, 1, -g}, {g, 0, -1}, {g, 0, 1}, {-g, 0, -1}, {-g, 0, 1}};
+  *idx = {0, 11, 5, 0, 5, 1, 0, 1, 7, 0, 7, 10, 0, 10, 11, 1, 5, 9, 5, 11, 4, 11, 10, 2, 10, 7, 6, 7, 1, 8,
+          3, 9, 4, 3, 4, 2, 3, 2, 6, 3, 6, 8, 3, 8, 9, 4, 9, 5, 2, 4, 11, 6, 2, 10, 8, 6, 7, 9, 8, 1};
+  auto norm = [](P3 p) {
+    const double l = std::sqrt(p.x * p.x + p.y * p.y + p.z * p.z);
+    return P3{p.x / l, p.y / l, p.z / l};
+  };
+  for (P3& p : *verts) p = norm(p);
+  for (int l = 0; l < levels; ++l) {
+    std::vector<uint32_t> next;
+    next.reserve(idx->size() * 4);
+    std::vector<std::pair<uint64_t, uint32_t>> mids;  // (edge key, vertex), small per level
+    auto midpoint = [&](uint32_t a, uint32_t b) {
+      const uint64_t key = a < b ? ((uint64_t)a << 32 | b) : ((uint64_t)b << 32 | a);
+      for (const auto& m : mids)
+        if (m.first == key) return m.second;
+      const P3 pa = (*verts)[a], pb = (*verts)[b];
+      verts->push_back(norm(P3{(pa.x + pb.x) / 2, (pa.y + pb.y) / 2, (pa.z + pb.z) / 2}));
+      mids.push_back({key, (uint32_t)verts->size() - 1});
+      return (uint32_t)verts->size() - 1;
+    };
+    for (size_t i = 0; i < idx->size(); i += 3) {
+      const uint32_t a = (*idx)[i], b = (*idx)[i + 1], c = (*idx)[i + 2];
+      const uint32_t ab = midpoint(a, b), bc = midpoint(b, c), ca = midpoint(c, a);
+      next.insert(next.end(), {a, ab, ca, b, bc, ab, c, ca, bc, ab, bc, ca});
+    }
+    *idx = std::move(next);
+  }
+}
+
+// Deterministic uniform doubles in [0, 1) from a xorshift64* stream.
+struct Rng {
+  uint64_t s;
+  double next() {
+    s ^= s >> 12;
+    s ^= s << 25;
+    s ^= s >> 27;
+    return (double)((s * 0x2545F4914F6CDD1Dull) >> 11) * (1.0 / 9007199254740992.0);
+  }
+};
+
 }  // namespace
+
+void MakeClutterMesh(uint32_t spheres, uint32_t seed, std::vector<RawVertex>* v, std::vector<RawTriangle>* t,
+                     std::vector<gpugi_material_params>* m, std::vector<PointLight>* lights) {
+  // Room [-10,10] x [0,8] x [-10,10]; materials: 0 room, 1 light, 2..121 Lambertian bins
+  // (albedo uniform in [0.2, 0.9] per channel), 122..241 Phong-glossy bins (exponent
+  // log-uniform in [10, 1000]).  Spheres pick a Lambertian bin with probability 0.6.
+  const int kBins = 120;
+  Rng rng{0x9E3779B97F4A7C15ull ^ ((uint64_t)seed << 1 | 1)};
+  m->clear();
+  m->push_back(Mat("Room", 0.7f, 0.7f, 0.7f));
+  gpugi_material_params light = Mat("Light", 0.0f, 0.0f, 0.0f);
+  light.emissivity[0] = light.emissivity[1] = light.emissivity[2] = 40.0f;
+  m->push_back(light);
+  for (int i = 0; i < kBins; ++i) {
+    char name[32];
+    std::snprintf(name, sizeof(name), "Lambert%03d", i);
+    m->push_back(Mat(name, (float)(0.2 + 0.7 * rng.next()), (float)(0.2 + 0.7 * rng.next()),
+                     (float)(0.2 + 0.7 * rng.next())));
+  }
+  for (int i = 0; i < kBins; ++i) {
+    char name[32];
+    std::snprintf(name, sizeof(name), "Glossy%03d", i);
+    const double albedo = 0.2 + 0.7 * rng.next();
+    gpugi_material_params g = Mat(name, (float)(0.3 * albedo), (float)(0.3 * albedo), (float)(0.3 * albedo));
+    const double exponent = std::exp(std::log(10.0) + (std::log(1000.0) - std::log(10.0)) * rng.next());
+    g.roughness = (float)(1.0 / std::sqrt(exponent));  // exponent = 1 / roughness^2 (scene.cpp:160,172)
+    for (int c = 0; c < 3; ++c) g.specular_color[c] = (float)(0.5 * albedo + 0.3);
+    m->push_back(g);
+  }
+  v->clear();
+  t->clear();
+  AddQuad(v, t, {-10, 0, 10}, {10, 0, 10}, {10, 0, -10}, {-10, 0, -10}, {0, 1, 0}, 0);    // floor
+  AddQuad(v, t, {-10, 8, -10}, {10, 8, -10}, {10, 8, 10}, {-10, 8, 10}, {0, -1, 0}, 0);   // ceiling
+  AddQuad(v, t, {-10, 0, -10}, {10, 0, -10}, {10, 8, -10}, {-10, 8, -10}, {0, 0, 1}, 0);  // back
+  AddQuad(v, t, {10, 0, 10}, {-10, 0, 10}, {-10, 8, 10}, {10, 8, 10}, {0, 0, -1}, 0);     // front
+  AddQuad(v, t, {-10, 0, 10}, {-10, 0, -10}, {-10, 8, -10}, {-10, 8, 10}, {1, 0, 0}, 0);  // left
+  AddQuad(v, t, {10, 0, -10}, {10, 0, 10}, {10, 8, 10}, {10, 8, -10}, {-1, 0, 0}, 0);     // right
+  // Area light: a 2 x 2 quad just below the ceiling as a 4-triangle fan around its centre,
+  // wound so that cross(v0 - v1, v0 - v2) points down into the room.
+  {
+    const uint32_t base = (uint32_t)v->size();
+    const P3 c[5] = {{0, 7.99, 0}, {-1, 7.99, -1}, {1, 7.99, -1}, {1, 7.99, 1}, {-1, 7.99, 1}};
+    for (const P3& p : c) {
+      RawVertex rv;
+      rv.position[0] = (float)p.x; rv.position[1] = (float)p.y; rv.position[2] = (float)p.z;
+      rv.normal[0] = 0.0f; rv.normal[1] = -1.0f; rv.normal[2] = 0.0f;
+      rv.texcoord[0] = rv.texcoord[1] = 0.0f;
+      v->push_back(rv);
+    }
+    for (uint32_t k = 0; k < 4; ++k)
+      t->push_back(RawTriangle{{base, base + 1 + (k + 1) % 4, base + 1 + k}, 1});
+  }
+  std::vector<P3> sv;
+  std::vector<uint32_t> si;
+  Icosphere(3, &sv, &si);
+  v->reserve(v->size() + (size_t)spheres * sv.size());
+  t->reserve(t->size() + (size_t)spheres * si.size() / 3);
+  for (uint32_t s = 0; s < spheres; ++s) {
+    const double r = std::exp(std::log(0.05) + (std::log(0.8) - std::log(0.05)) * rng.next());
+    const double cx = -10.0 + r + (20.0 - 2 * r) * rng.next();
+    const double cy = r + (8.0 - 2 * r) * rng.next();
+    const double cz = -10.0 + r + (20.0 - 2 * r) * rng.next();
+    const bool lambert = rng.next() < 0.6;
+    const uint32_t bin = (uint32_t)(rng.next() * kBins) % kBins;
+    const uint32_t mat = (lambert ? 2u : 2u + kBins) + bin;
+    const uint32_t base = (uint32_t)v->size();
+    for (const P3& p : sv) {
+      RawVertex rv;
+      rv.position[0] = (float)(cx + r * p.x); rv.position[1] = (float)(cy + r * p.y);
+      rv.position[2] = (float)(cz + r * p.z);
+      rv.normal[0] = (float)p.x; rv.normal[1] = (float)p.y; rv.normal[2] = (float)p.z;
+      rv.texcoord[0] = rv.texcoord[1] = 0.0f;
+      v->push_back(rv);
+    }
+    for (size_t i = 0; i < si.size(); i += 3)
+      t->push_back(RawTriangle{{base + si[i], base + si[i + 1], base + si[i + 2]}, mat});
+  }
+  lights->clear();
+  const float lp[3][3] = {{-6.0f, 6.5f, -5.0f}, {5.0f, 5.5f, 6.0f}, {6.0f, 7.0f, -6.0f}};
+  for (int k = 0; k < 3; ++k) {
+    PointLight pl;
+    std::memcpy(pl.position, lp[k], 12);
+    pl.intensity[0] = pl.intensity[1] = pl.intensity[2] = 30.0f;
+    lights->push_back(pl);
+  }
+}
 
 void MakeCornellMesh(std::vector<RawVertex>* v, std::vector<RawTriangle>* t,
                      std::vector<gpugi_material_params>* m) {

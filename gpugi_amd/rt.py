@@ -71,6 +71,7 @@ _SIGNATURES = {
     "gpugi_set_light_samples": [_P, _P, ctypes.c_uint32],
     "gpugi_set_max_path_length": [_P, ctypes.c_int],
     "gpugi_set_integrator_kernel": [_P, ctypes.c_int],
+    "gpugi_set_rr_start_depth": [_P, ctypes.c_int],
     "gpugi_draw": [_P, ctypes.c_int, ctypes.c_uint32],
     "gpugi_bpt_warmup": [_P, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)],
     "gpugi_bpt_set_light_cache": [_P, ctypes.c_int32, ctypes.c_int32],
@@ -99,6 +100,7 @@ _SIGNATURES = {
                               ctypes.c_uint32, ctypes.POINTER(ctypes.c_void_p)],
     "gpugi_hostscene_cornell": [ctypes.POINTER(ctypes.c_void_p)],
     "gpugi_hostscene_heightfield": [ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(ctypes.c_void_p)],
+    "gpugi_hostscene_clutter": [ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(ctypes.c_void_p)],
     "gpugi_hostscene_load": [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(ctypes.c_void_p)],
     "gpugi_hostscene_save": [_P, ctypes.c_char_p, ctypes.c_char_p],
     "gpugi_hostscene_desc": [_P, ctypes.POINTER(SceneDesc)],
@@ -193,6 +195,12 @@ class HostScene:
     def heightfield(cls, cells, seed):
         h = ctypes.c_void_p()
         _check("gpugi_hostscene_heightfield", lib().gpugi_hostscene_heightfield(cells, seed, ctypes.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def clutter(cls, spheres, seed):
+        h = ctypes.c_void_p()
+        _check("gpugi_hostscene_clutter", lib().gpugi_hostscene_clutter(spheres, seed, ctypes.byref(h)))
         return cls(h.value)
 
     @classmethod
@@ -369,6 +377,9 @@ class Context:
 
     def set_integrator_kernel(self, k):
         self._call("gpugi_set_integrator_kernel", int(k))
+
+    def set_rr_start_depth(self, depth):
+        self._call("gpugi_set_rr_start_depth", int(depth))
 
     def draw(self, integrator, frame_seed):
         self._call("gpugi_draw", int(integrator), ctypes.c_uint32(frame_seed & 0xffffffff))

@@ -158,6 +158,10 @@ int gpugi_set_screen(gpugi_ctx* ctx, int width, int height);
 int gpugi_set_light_samples(gpugi_ctx* ctx, const gpugi_light_sample* samples, uint32_t n);
 /* MAX_PATHLENGTH define (pathtracer.cpp:60). */
 int gpugi_set_max_path_length(gpugi_ctx* ctx, int max_path_length);
+/* Bounce from which Russian roulette runs (default 0: every bounce, as
+ * pathtracer.comp:99-104 and lighttracer.comp:183-188 do; 3 gives the
+ * "roulette after depth 3" variant, which has no reference counterpart). */
+int gpugi_set_rr_start_depth(gpugi_ctx* ctx, int depth);
 /* Traversal the integrators use: GPUGI_KERNEL_REFERENCE (bit-exact escape
  * walk) or GPUGI_KERNEL_WIDE / GPUGI_KERNEL_AUTO (4-wide quantized BVH).
  * Replaces the AABOX_BVH define of the shader compile (pathtracer.cpp:55-64). */
@@ -282,6 +286,14 @@ int gpugi_hostscene_cornell(gpugi_hostscene** out);
 /* Heightfield of cells x cells quads (2 triangles each) over [0,1]^2, heights
  * 0.2 * fBm (6 octaves of value noise on a wanghash lattice seeded `seed`). */
 int gpugi_hostscene_heightfield(uint32_t cells, uint32_t seed, gpugi_hostscene** out);
+/* Clutter room: a 20 x 20 x 8 room holding `spheres` seeded icospheres
+ * (subdivision 3, 1280 triangles each, radii log-uniform in [0.05, 0.8];
+ * 4000 spheres give 5,120,016 triangles), 60% Lambertian (albedo uniform in
+ * [0.2, 0.9]) and 40% Phong-glossy (exponent log-uniform in [10, 1000])
+ * materials binned into 240, one 4-triangle ceiling area light and 3 point
+ * lights.  No reference counterpart: a synthetic scene of the integrator
+ * benchmark. */
+int gpugi_hostscene_clutter(uint32_t spheres, uint32_t seed, gpugi_hostscene** out);
 /* Loads a .rawscene (documentation/fileformat.tex:37-168) and its material json. */
 int gpugi_hostscene_load(const char* rawscene_path, const char* json_path, gpugi_hostscene** out);
 /* Writes the .rawscene chunks the way bvhmake exports them
