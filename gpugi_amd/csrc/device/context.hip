@@ -56,6 +56,7 @@ struct gpugi_ctx {
   uint32_t numLights = 0, lightCap = 0;
   int maxPathLength = 16;  // default pathLength (application.cpp:123)
   int rrStartDepth = 0;
+  int triangleTest = GPUGI_TRIANGLE_REFERENCE;
   int integratorKernel = GPUGI_KERNEL_AUTO;
   int bptCapacity = -1;
   int bptAverageLength = 0;
@@ -102,6 +103,8 @@ void FreeDev(void*& p) {
   if (p) hipFree(p);
   p = nullptr;
 }
+
+bool Watertight(const gpugi_ctx* c) { return c->triangleTest == GPUGI_TRIANGLE_WATERTIGHT; }
 
 RefScene MakeRef(const gpugi_ctx* c) {
   RefScene s;
@@ -173,7 +176,7 @@ int RebuildWide(gpugi_ctx* ctx) {
   d.positions = ctx->hPos.data();
   d.num_vertices = ctx->hNumVertices;
   WideBVHHost wide;
-  std::string err = BuildWideBVH(d, (uint32_t)ctx->wideLeaf, &wide);
+  std::string err = BuildWideBVH(d, (uint32_t)ctx->wideLeaf, &wide, ctx->triangleTest == GPUGI_TRIANGLE_WATERTIGHT);
   if (!err.empty()) return Fail(ctx, GPUGI_ERR_FORMAT, "wide BVH re-pack: " + err);
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   FreeDev(ctx->dWideNodes);
@@ -522,6 +525,7 @@ static DrawParams MakeDrawParams(gpugi_ctx* ctx, uint32_t frame_seed) {
   p.frameSeed = frame_seed;
   p.maxPathLength = ctx->maxPathLength;
   p.rrStartDepth = ctx->rrStartDepth;
+  p.watertight = Watertight(ctx);
   p.counters = (uint32_t*)ctx->dCounters;
   p.bptCapacity = ctx->bptCapacity;
   return p;
@@ -609,9 +613,10 @@ int gpugi_trace(gpugi_ctx* ctx, int query, int kernel, const gpugi_ray* rays, gp
   if (wide) {
     WideScene ws;
     if (int rc = PrepareTraceWide(ctx, ctx->wideStack, &ws)) return rc;
-    HIP_TRY(ctx, LaunchTraceWide(ws, ctx->wideStack, query, dr, dh, n, ctx->wideRefill, ctx->stream));
+    HIP_TRY(ctx, LaunchTraceWide(ws, ctx->wideStack, query, dr, dh, n, ctx->wideRefill, Watertight(ctx),
+                                 ctx->stream));
   } else {
-    HIP_TRY(ctx, LaunchTraceRef(MakeRef(ctx), query, dr, dh, n, ctx->stream));
+    HIP_TRY(ctx, LaunchTraceRef(MakeRef(ctx), query, dr, dh, n, Watertight(ctx), ctx->stream));
   }
   if (!rays_on_device) {
     HIP_TRY(ctx, hipMemcpyAsync(hits, dh, n * sizeof(gpugi_hit), hipMemcpyDeviceToHost, ctx->stream));
@@ -672,10 +677,10 @@ int gpugi_autotune_trace(gpugi_ctx* ctx, int query, const gpugi_ray* rays_dev, g
       if (int rc = PrepareTraceWide(ctx, stack, &ws)) return rc;
       for (int refill : refills) {
         // one untimed launch, then the average of `repeats`
-        HIP_TRY(ctx, LaunchTraceWide(ws, stack, query, rays_dev, hits_dev, n, refill, ctx->stream));
+        HIP_TRY(ctx, LaunchTraceWide(ws, stack, query, rays_dev, hits_dev, n, refill, Watertight(ctx), ctx->stream));
         HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
         for (int r = 0; r < repeats; ++r)
-          HIP_TRY(ctx, LaunchTraceWide(ws, stack, query, rays_dev, hits_dev, n, refill, ctx->stream));
+          HIP_TRY(ctx, LaunchTraceWide(ws, stack, query, rays_dev, hits_dev, n, refill, Watertight(ctx), ctx->stream));
         HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
         HIP_TRY(ctx, hipEventSynchronize(ctx->ev1));
         float ms = 0.0f;
@@ -1063,5 +1068,18 @@ int gpugi_set_rr_start_depth(gpugi_ctx* ctx, int depth) {
   if (!ctx) return Fail(ctx, GPUGI_ERR_ARG, "gpugi_set_rr_start_depth: null context");
   if (depth < 0 || depth > 64) return Fail(ctx, GPUGI_ERR_ARG, "gpugi_set_rr_start_depth: depth in [0, 64]");
   ctx->rrStartDepth = depth;
+  return GPUGI_OK;
+}
+
+int gpugi_set_triangle_test(gpugi_ctx* ctx, int mode) {
+  if (!ctx) return Fail(ctx, GPUGI_ERR_ARG, "gpugi_set_triangle_test: null context");
+  if (mode != GPUGI_TRIANGLE_REFERENCE && mode != GPUGI_TRIANGLE_WATERTIGHT)
+    return Fail(ctx, GPUGI_ERR_ARG, "gpugi_set_triangle_test: unknown mode");
+  if (mode == ctx->triangleTest) return GPUGI_OK;
+  ctx->triangleTest = mode;
+  if (ctx->hasScene) {
+    if (SetDev(ctx)) return GPUGI_ERR_HIP;
+    if (int rc = RebuildWide(ctx)) return rc;
+  }
   return GPUGI_OK;
 }

@@ -29,12 +29,18 @@ struct ClosestHit {
   gg::V3 geoNormal;  // cross(e1, e0), not normalized (TRINORMAL_OUTPUT)
 };
 
+// With DrawParams::watertight the watertight triangle test replaces the
+// reference one in both traversals (the geometric normal is then rebuilt
+// from the vertices with the reference formula cross(p0 - p2, p1 - p0)).
 template <bool WIDE>
 __device__ __forceinline__ ClosestHit trace_closest(const TraceCtx& c, gg::V3 o, gg::V3 d) {
   ClosestHit h;
   if (WIDE) {
-    WideResult r = trace_wide_ray<false, kTileThreads, kWideLdsStack>(c.p->wide, o, d, gg::kRayMax, c.stkNode, c.stkT,
-                                                                      c.tid, c.spill);
+    WideResult r = c.p->watertight
+                       ? trace_wide_ray<false, kTileThreads, kWideLdsStack, true>(c.p->wide, o, d, gg::kRayMax,
+                                                                                  c.stkNode, c.stkT, c.tid, c.spill)
+                       : trace_wide_ray<false, kTileThreads, kWideLdsStack, false>(c.p->wide, o, d, gg::kRayMax,
+                                                                                   c.stkNode, c.stkT, c.tid, c.spill);
     h.hit = r.slot >= 0;
     h.t = r.t;
     if (h.hit) {
@@ -48,13 +54,20 @@ __device__ __forceinline__ ClosestHit trace_closest(const TraceCtx& c, gg::V3 o,
     float rayLength = gg::kRayMax;
     RefHit rh;
     uint32_t nb = 0, nt = 0;
-    trace_ref<false, false>(c.p->ref, o, d, rayLength, rh, nb, nt);
+    if (c.p->watertight) trace_ref<false, false, true>(c.p->ref, o, d, rayLength, rh, nb, nt);
+    else trace_ref<false, false, false>(c.p->ref, o, d, rayLength, rh, nb, nt);
     h.hit = rayLength != gg::kRayMax;  // pathtracer.comp:56 tests rayHit == RAY_MAX
     h.t = rayLength;
     if (h.hit) {
       h.tri = rh.tri;
       h.bary = rh.bary;
-      h.geoNormal = rh.triNormal;
+      if (c.p->watertight) {
+        const gg::V3 p0 = load_pos(c.p->ref.positions, h.tri.x), p1 = load_pos(c.p->ref.positions, h.tri.y),
+                     p2 = load_pos(c.p->ref.positions, h.tri.z);
+        h.geoNormal = gg::cross(p0 - p2, p1 - p0);
+      } else {
+        h.geoNormal = rh.triNormal;
+      }
     }
   }
   return h;
@@ -63,14 +76,18 @@ __device__ __forceinline__ ClosestHit trace_closest(const TraceCtx& c, gg::V3 o,
 template <bool WIDE>
 __device__ __forceinline__ bool trace_any(const TraceCtx& c, gg::V3 o, gg::V3 d, float len) {
   if (WIDE) {
-    WideResult r = trace_wide_ray<true, kTileThreads, kWideLdsStack>(c.p->wide, o, d, len, c.stkNode, c.stkT, c.tid,
-                                                                     c.spill);
+    WideResult r = c.p->watertight
+                       ? trace_wide_ray<true, kTileThreads, kWideLdsStack, true>(c.p->wide, o, d, len, c.stkNode,
+                                                                                 c.stkT, c.tid, c.spill)
+                       : trace_wide_ray<true, kTileThreads, kWideLdsStack, false>(c.p->wide, o, d, len, c.stkNode,
+                                                                                  c.stkT, c.tid, c.spill);
     return r.any;
   }
   float rayLength = len;
   RefHit rh;
   uint32_t nb = 0, nt = 0;
-  return trace_ref<true, false>(c.p->ref, o, d, rayLength, rh, nb, nt);
+  if (c.p->watertight) return trace_ref<true, false, true>(c.p->ref, o, d, rayLength, rh, nb, nt);
+  return trace_ref<true, false, false>(c.p->ref, o, d, rayLength, rh, nb, nt);
 }
 
 // EstimateDirectLight (lightsource.glsl:3-93) without importance break / cache.

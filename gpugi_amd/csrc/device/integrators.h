@@ -23,6 +23,7 @@ struct DrawParams {
   uint32_t frameSeed = 0;
   int maxPathLength = 16;
   int rrStartDepth = 0;                   // Russian roulette from this bounce on (0: every bounce, as the reference)
+  bool watertight = false;                // watertight triangle test instead of IntersectTriangle
   uint32_t* counters = nullptr;
   int bptCapacity = -1;                   // LightCacheCapacity (bidirectionalpathtracer.cpp:110-112)
 };

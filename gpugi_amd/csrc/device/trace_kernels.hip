@@ -9,7 +9,7 @@ namespace gpugi {
 
 // ---------------------------------------------------------------------------
 // Reference layout: one thread per ray.
-template <bool ANY>
+template <bool ANY, bool WT>
 __global__ __launch_bounds__(256) void k_trace_ref(RefScene s, const float4* __restrict__ rays,
                                                    int4* __restrict__ hits, uint32_t n) {
   const uint32_t i = blockIdx.x * 256u + threadIdx.x;
@@ -19,7 +19,7 @@ __global__ __launch_bounds__(256) void k_trace_ref(RefScene s, const float4* __r
   float rayLength = r0.w;
   RefHit h;
   uint32_t nb = 0, nt = 0;
-  bool any = trace_ref<ANY, false>(s, o, d, rayLength, h, nb, nt);
+  bool any = trace_ref<ANY, false, WT>(s, o, d, rayLength, h, nb, nt);
   if (ANY) {
     hits[i] = make_int4(any ? 1 : 0, 0, 0, 0);
   } else {
@@ -45,15 +45,19 @@ __global__ __launch_bounds__(256) void k_trace_ref_counts(RefScene s, const floa
 }
 
 hipError_t LaunchTraceRef(const RefScene& s, int query, const gpugi_ray* rays, gpugi_hit* hits, size_t n,
-                          hipStream_t stream) {
+                          bool watertight, hipStream_t stream) {
   if (n == 0) return hipSuccess;
   dim3 grid((unsigned)((n + 255) / 256));
-  if (query == GPUGI_ANY_HIT)
-    hipLaunchKernelGGL(k_trace_ref<true>, grid, dim3(256), 0, stream, s, (const float4*)rays, (int4*)hits,
-                       (uint32_t)n);
-  else
-    hipLaunchKernelGGL(k_trace_ref<false>, grid, dim3(256), 0, stream, s, (const float4*)rays, (int4*)hits,
-                       (uint32_t)n);
+  const float4* r = (const float4*)rays;
+  int4* h = (int4*)hits;
+  const uint32_t m = (uint32_t)n;
+  if (query == GPUGI_ANY_HIT) {
+    if (watertight) hipLaunchKernelGGL((k_trace_ref<true, true>), grid, dim3(256), 0, stream, s, r, h, m);
+    else hipLaunchKernelGGL((k_trace_ref<true, false>), grid, dim3(256), 0, stream, s, r, h, m);
+  } else {
+    if (watertight) hipLaunchKernelGGL((k_trace_ref<false, true>), grid, dim3(256), 0, stream, s, r, h, m);
+    else hipLaunchKernelGGL((k_trace_ref<false, false>), grid, dim3(256), 0, stream, s, r, h, m);
+  }
   return hipGetLastError();
 }
 

@@ -62,12 +62,75 @@ __device__ __forceinline__ bool intersect_triangle(gg::V3 o, gg::V3 d, gg::V3 p0
   return (hit > gg::kIntersectEpsilon) && bary.x >= 0.0f && bary.y >= 0.0f && bary.z >= 0.0f;
 }
 
+// Watertight ray/triangle test (Woop, Benthin and Wald 2013, "Watertight
+// Ray/Triangle Intersection", JCGT 2(1)); no reference counterpart.  The
+// vertices are sheared into a frame whose z axis is the ray's dominant axis
+// and the 2D edge functions are differences of products in Kahan's fma form,
+// whose sign is exact: an edge shared by two triangles gets opposite signs in
+// both, so no ray passes between them.  b.x, b.y, b.z weight p0, p1, p2.
+struct WtRay {
+  int kx, ky, kz;
+  float sx, sy, sz;
+};
+
+__device__ __forceinline__ float comp(gg::V3 v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
+
+__device__ __forceinline__ WtRay wt_setup(gg::V3 d) {
+  WtRay r;
+  const float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+  r.kz = (ax >= ay && ax >= az) ? 0 : (ay >= az ? 1 : 2);
+  r.kx = r.kz == 2 ? 0 : r.kz + 1;
+  r.ky = r.kx == 2 ? 0 : r.kx + 1;
+  if (comp(d, r.kz) < 0.0f) {
+    const int t = r.kx;
+    r.kx = r.ky;
+    r.ky = t;
+  }
+  r.sx = comp(d, r.kx) / comp(d, r.kz);
+  r.sy = comp(d, r.ky) / comp(d, r.kz);
+  r.sz = 1.0f / comp(d, r.kz);
+  return r;
+}
+
+// a * b - c * d with an exact sign: w = c*d and e = c*d - w exactly.
+__device__ __forceinline__ float diff_of_products(float a, float b, float c, float d) {
+  const float w = c * d;
+  const float e = __builtin_fmaf(-c, d, w);
+  const float f = __builtin_fmaf(a, b, -w);
+  return f + e;
+}
+
+__device__ __forceinline__ bool intersect_triangle_wt(gg::V3 o, const WtRay& r, gg::V3 p0, gg::V3 p1, gg::V3 p2,
+                                                      float& t, gg::V3& bary) {
+  const gg::V3 A = p0 - o, B = p1 - o, C = p2 - o;
+  const float Akz = comp(A, r.kz), Bkz = comp(B, r.kz), Ckz = comp(C, r.kz);
+  const float Ax = comp(A, r.kx) - r.sx * Akz, Ay = comp(A, r.ky) - r.sy * Akz;
+  const float Bx = comp(B, r.kx) - r.sx * Bkz, By = comp(B, r.ky) - r.sy * Bkz;
+  const float Cx = comp(C, r.kx) - r.sx * Ckz, Cy = comp(C, r.ky) - r.sy * Ckz;
+  const float U = diff_of_products(Cx, By, Cy, Bx);
+  const float V = diff_of_products(Ax, Cy, Ay, Cx);
+  const float W = diff_of_products(Bx, Ay, By, Ax);
+  if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
+  const float det = U + V + W;
+  if (det == 0.0f) return false;
+  const float Az = r.sz * Akz, Bz = r.sz * Bkz, Cz = r.sz * Ckz;
+  const float T = U * Az + V * Bz + W * Cz;
+  const float inv = 1.0f / det;
+  t = T * inv;
+  bary.x = U * inv;
+  bary.y = V * inv;
+  bary.z = W * inv;
+  return t > gg::kIntersectEpsilon;
+}
+
 // TraceRay / TraceRayAnyHit (traceray.glsl:18-180) for TRIANGLES_PER_LEAF == 2.
 // ANY: returns at the first accepted triangle.  COUNT: box/triangle visit
-// counters exactly as TRACERAY_DEBUG_VARS counts them.
-template <bool ANY, bool COUNT>
+// counters exactly as TRACERAY_DEBUG_VARS counts them.  WT: the watertight
+// triangle test in place of IntersectTriangle (triNormal is then not set).
+template <bool ANY, bool COUNT, bool WT = false>
 __device__ __forceinline__ bool trace_ref(const RefScene& s, gg::V3 o, gg::V3 d, float& rayLength,
                                           RefHit& out, uint32_t& nbox, uint32_t& ntri) {
+  const WtRay wr = WT ? wt_setup(d) : WtRay{};
   int currentNodeIndex = 0;
   int currentLeafIndex = 0;
   int lastNodeIndex = 0;
@@ -105,8 +168,10 @@ __device__ __forceinline__ bool trace_ref(const RefScene& s, gg::V3 o, gg::V3 d,
         gg::V3 p1 = load_pos(s.positions, tri.y);
         gg::V3 p2 = load_pos(s.positions, tri.z);
         float newHit;
-        gg::V3 bary, n;
-        if (intersect_triangle(o, d, p0, p1, p2, newHit, bary, n) && newHit < rayLength) {
+        gg::V3 bary, n = gg::v3s(0.0f);
+        const bool ok = WT ? intersect_triangle_wt(o, wr, p0, p1, p2, newHit, bary)
+                           : intersect_triangle(o, d, p0, p1, p2, newHit, bary, n);
+        if (ok && newHit < rayLength) {
           if (ANY) return true;
           rayLength = newHit;
           out.tri = tri;

@@ -24,6 +24,7 @@
 #include <hip/hip_runtime.h>
 
 #include "../common/gvec.h"
+#include "trace_ref.h"
 #include "wide_bvh.h"
 
 namespace gpugi {
@@ -44,6 +45,7 @@ struct WideState {
   uint32_t node;
   int sp;
   bool found;
+  WtRay wr;  // watertight mode only
 };
 
 __device__ __forceinline__ float safe_inv(float d) {
@@ -53,6 +55,7 @@ __device__ __forceinline__ float safe_inv(float d) {
 
 __device__ __forceinline__ float byte_f(uint32_t w, int k) { return (float)((w >> (8 * k)) & 0xffu); }
 
+template <bool WT = false>
 __device__ __forceinline__ void wide_init(WideState& st, gg::V3 o, gg::V3 d, float tmax) {
   st.o = o;
   st.d = d;
@@ -65,10 +68,12 @@ __device__ __forceinline__ void wide_init(WideState& st, gg::V3 o, gg::V3 d, flo
   st.node = 0;
   st.sp = 0;
   st.found = false;
+  if (WT) st.wr = wt_setup(d);
 }
 
-// One node visit.  Returns true when the ray is finished.
-template <bool ANY, int B, int S>
+// One node visit.  Returns true when the ray is finished.  WT: the watertight
+// triangle test on the raw-vertex triangle layout {p0, slot}{p1, mat}{p2, 0}.
+template <bool ANY, int B, int S, bool WT = false>
 __device__ __forceinline__ bool wide_step(const WideScene& s, WideState& st, uint32_t* stkNode, float* stkT, int tid,
                                           uint2* spill) {
   const gg::V3 o = st.o, d = st.d;
@@ -112,17 +117,26 @@ __device__ __forceinline__ bool wide_step(const WideScene& s, WideState& st, uin
         const float4 b = s.tris[3 * (size_t)(first + m) + 1];
         const float4 c = s.tris[3 * (size_t)(first + m) + 2];
         const gg::V3 p0 = gg::v3(a.x, a.y, a.z);
-        const gg::V3 e0 = gg::v3(b.x, b.y, b.z);
-        const gg::V3 e1 = gg::v3(c.x, c.y, c.z);
         const int slot = __float_as_int(a.w);
-        const gg::V3 nn = gg::cross(e1, e0);
-        const gg::V3 e2 = (1.0f / gg::dot(nn, d)) * (p0 - o);
-        const gg::V3 ii = gg::cross(d, e2);
-        const float by = gg::dot(ii, e1);
-        const float bz = gg::dot(ii, e0);
-        const float bx = 1.0f - (bz + by);
-        const float t = gg::dot(nn, e2);
-        const bool ok = t > 0.0001f && bx >= 0.0f && by >= 0.0f && bz >= 0.0f;
+        float t, by, bz;
+        bool ok;
+        if (WT) {
+          gg::V3 bary;
+          ok = intersect_triangle_wt(o, st.wr, p0, gg::v3(b.x, b.y, b.z), gg::v3(c.x, c.y, c.z), t, bary);
+          by = bary.y;
+          bz = bary.z;
+        } else {
+          const gg::V3 e0 = gg::v3(b.x, b.y, b.z);
+          const gg::V3 e1 = gg::v3(c.x, c.y, c.z);
+          const gg::V3 nn = gg::cross(e1, e0);
+          const gg::V3 e2 = (1.0f / gg::dot(nn, d)) * (p0 - o);
+          const gg::V3 ii = gg::cross(d, e2);
+          by = gg::dot(ii, e1);
+          bz = gg::dot(ii, e0);
+          const float bx = 1.0f - (bz + by);
+          t = gg::dot(nn, e2);
+          ok = t > 0.0001f && bx >= 0.0f && by >= 0.0f && bz >= 0.0f;
+        }
         if (ANY) {
           if (ok && t < st.tbest) { st.found = true; break; }
         } else if (ok && (t < st.tbest || (t == st.tbest && slot < st.bestSlot))) {
@@ -214,12 +228,12 @@ __device__ __forceinline__ bool wide_step(const WideScene& s, WideState& st, uin
 }
 
 // stkNode/stkT: LDS arrays of S * B entries; tid in [0, B).
-template <bool ANY, int B, int S>
+template <bool ANY, int B, int S, bool WT = false>
 __device__ __forceinline__ WideResult trace_wide_ray(const WideScene& s, gg::V3 o, gg::V3 d, float tmax,
                                                      uint32_t* stkNode, float* stkT, int tid, uint2* spill) {
   WideState st;
-  wide_init(st, o, d, tmax);
-  while (!wide_step<ANY, B, S>(s, st, stkNode, stkT, tid, spill)) {
+  wide_init<WT>(st, o, d, tmax);
+  while (!wide_step<ANY, B, S, WT>(s, st, stkNode, stkT, tid, spill)) {
   }
   WideResult r;
   r.t = st.tbest;

@@ -23,7 +23,7 @@ namespace gpugi {
 
 constexpr uint32_t kNumXcd = 8;
 
-template <bool ANY, int B, int S>
+template <bool ANY, int B, int S, bool WT>
 __global__ __launch_bounds__(B) void k_trace_wide(WideScene s, const float4* __restrict__ rays,
                                                   int4* __restrict__ hits, uint32_t n, int refill) {
   __shared__ uint32_t stkNode[S * B];
@@ -58,7 +58,7 @@ __global__ __launch_bounds__(B) void k_trace_wide(WideScene s, const float4* __r
         const uint32_t r = batch * 64u + (pos % 64u);
         if (batch < bEnd && r < n) {
           const float4 r0 = rays[2 * (size_t)r], r1 = rays[2 * (size_t)r + 1];
-          wide_init(st, gg::v3(r0.x, r0.y, r0.z), gg::v3(r1.x, r1.y, r1.z), r0.w);
+          wide_init<WT>(st, gg::v3(r0.x, r0.y, r0.z), gg::v3(r1.x, r1.y, r1.z), r0.w);
           ri = r;
           active = true;
         }
@@ -70,7 +70,7 @@ __global__ __launch_bounds__(B) void k_trace_wide(WideScene s, const float4* __r
       if (exhausted) break;
       continue;
     }
-    if (active && wide_step<ANY, B, S>(s, st, stkNode, stkT, tid, spill)) {
+    if (active && wide_step<ANY, B, S, WT>(s, st, stkNode, stkT, tid, spill)) {
       if (ANY) hits[ri] = make_int4(st.found ? 1 : 0, 0, 0, 0);
       else hits[ri] = make_int4(st.bestSlot, __float_as_int(st.tbest), __float_as_int(st.bu), __float_as_int(st.bv));
       active = false;
@@ -88,8 +88,8 @@ struct Variant {
 
 template <int S>
 Variant MakeVariant() {
-  return Variant{S, reinterpret_cast<const void*>(&k_trace_wide<false, kWideBlock, S>),
-                 reinterpret_cast<const void*>(&k_trace_wide<true, kWideBlock, S>)};
+  return Variant{S, reinterpret_cast<const void*>(&k_trace_wide<false, kWideBlock, S, false>),
+                 reinterpret_cast<const void*>(&k_trace_wide<true, kWideBlock, S, false>)};
 }
 
 const Variant* FindVariant(int stack) {
@@ -100,16 +100,23 @@ const Variant* FindVariant(int stack) {
   return nullptr;
 }
 
-template <int S>
+template <int S, bool WT>
 void LaunchS(const WideScene& s, int query, const gpugi_ray* rays, gpugi_hit* hits, size_t n, int refill,
              hipStream_t stream) {
   dim3 grid(s.numThreads / kWideBlock);
   if (query == GPUGI_ANY_HIT)
-    hipLaunchKernelGGL((k_trace_wide<true, kWideBlock, S>), grid, dim3(kWideBlock), 0, stream, s,
+    hipLaunchKernelGGL((k_trace_wide<true, kWideBlock, S, WT>), grid, dim3(kWideBlock), 0, stream, s,
                        (const float4*)rays, (int4*)hits, (uint32_t)n, refill);
   else
-    hipLaunchKernelGGL((k_trace_wide<false, kWideBlock, S>), grid, dim3(kWideBlock), 0, stream, s,
+    hipLaunchKernelGGL((k_trace_wide<false, kWideBlock, S, WT>), grid, dim3(kWideBlock), 0, stream, s,
                        (const float4*)rays, (int4*)hits, (uint32_t)n, refill);
+}
+
+template <int S>
+void LaunchS(const WideScene& s, int query, const gpugi_ray* rays, gpugi_hit* hits, size_t n, int refill,
+             bool watertight, hipStream_t stream) {
+  if (watertight) LaunchS<S, true>(s, query, rays, hits, n, refill, stream);
+  else LaunchS<S, false>(s, query, rays, hits, n, refill, stream);
 }
 
 }  // namespace
@@ -129,18 +136,18 @@ hipError_t WideBlocksPerCU(int stack, int* blocks) {
 }
 
 hipError_t LaunchTraceWide(const WideScene& s, int stack, int query, const gpugi_ray* rays, gpugi_hit* hits,
-                           size_t n, int refill, hipStream_t stream) {
+                           size_t n, int refill, bool watertight, hipStream_t stream) {
   if (n == 0) return hipSuccess;
   if (n > 0xffffffffull - 64) return hipErrorInvalidValue;
   // the static schedule needs a whole number of blocks per XCD
   if (s.numThreads == 0 || s.numThreads % (kWideBlock * kNumXcd) != 0) return hipErrorInvalidConfiguration;
   if (refill < 1 || refill > 64) return hipErrorInvalidValue;
   switch (stack) {
-    case 4: LaunchS<4>(s, query, rays, hits, n, refill, stream); break;
-    case 8: LaunchS<8>(s, query, rays, hits, n, refill, stream); break;
-    case 12: LaunchS<12>(s, query, rays, hits, n, refill, stream); break;
-    case 16: LaunchS<16>(s, query, rays, hits, n, refill, stream); break;
-    case 24: LaunchS<24>(s, query, rays, hits, n, refill, stream); break;
+    case 4: LaunchS<4>(s, query, rays, hits, n, refill, watertight, stream); break;
+    case 8: LaunchS<8>(s, query, rays, hits, n, refill, watertight, stream); break;
+    case 12: LaunchS<12>(s, query, rays, hits, n, refill, watertight, stream); break;
+    case 16: LaunchS<16>(s, query, rays, hits, n, refill, watertight, stream); break;
+    case 24: LaunchS<24>(s, query, rays, hits, n, refill, watertight, stream); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

@@ -10,7 +10,8 @@
 // host rounds lo down and hi up and checks in FP32 that the decoded box
 // contains the exact box, so the quantized tree is conservative.
 // Triangle: 48 bytes {p0, slot}, {e0 = p1-p0, material}, {e1 = p0-p2, 0}: the
-// operands of the reference triangle test, computed once in FP32.
+// operands of the reference triangle test, computed once in FP32 (in
+// watertight mode the raw vertices {p0, slot}, {p1, material}, {p2, 0}).
 #pragma once
 #include <stdint.h>
 
@@ -58,8 +59,11 @@ struct WideBVHHost {
 
 // Re-packs the reference BVH2 (any arity written in preorder with escape
 // pointers) into the 4-wide quantized layout.  Leaves of up to
-// `maxLeafTris` triangles are formed from subtrees that small.
-std::string BuildWideBVH(const gpugi_scene_desc& desc, uint32_t maxLeafTris, WideBVHHost* out);
+// `maxLeafTris` triangles are formed from subtrees that small.  rawVertices:
+// triangles store {p0, slot}{p1, mat}{p2, 0} (watertight test) instead of the
+// reference test's operands {p0, slot}{p1 - p0, mat}{p0 - p2, 0}.
+std::string BuildWideBVH(const gpugi_scene_desc& desc, uint32_t maxLeafTris, WideBVHHost* out,
+                         bool rawVertices = false);
 
 // Launch shape of the wide kernel: threads per block and LDS stack entries per thread.
 constexpr int kWideBlock = 256;

@@ -45,7 +45,7 @@ float ScaleOf(uint32_t biased) { uint32_t u = biased << 23; float f; std::memcpy
 
 }  // namespace
 
-std::string BuildWideBVH(const gpugi_scene_desc& desc, uint32_t maxLeafTris, WideBVHHost* out) {
+std::string BuildWideBVH(const gpugi_scene_desc& desc, uint32_t maxLeafTris, WideBVHHost* out, bool rawVertices) {
   if (maxLeafTris < 1 || maxLeafTris > kWideMaxLeafTris) return "maxLeafTris must be in [1, 16]";
   Bvh2View v{&desc, desc.num_nodes};
   if (v.n == 0) return "empty hierarchy";
@@ -98,8 +98,9 @@ std::string BuildWideBVH(const gpugi_scene_desc& desc, uint32_t maxLeafTris, Wid
           WideTriHost w;
           for (int a = 0; a < 3; ++a) {
             w.p0[a] = p0[a];
-            w.e0[a] = p1[a] - p0[a];
-            w.e1[a] = p0[a] - p2[a];
+            // reference test operands e0 = p1 - p0, e1 = p0 - p2, or the raw vertices
+            w.e0[a] = rawVertices ? p1[a] : p1[a] - p0[a];
+            w.e1[a] = rawVertices ? p2[a] : p0[a] - p2[a];
           }
           w.slot = (int32_t)slot;
           w.material = t[3];

@@ -16,6 +16,7 @@ GPUGI_OK = 0
 CLOSEST_HIT, ANY_HIT = 0, 1
 KERNEL_AUTO, KERNEL_REFERENCE, KERNEL_WIDE = 0, 1, 2
 PT, LT, BPT, PT_CACHE = 0, 1, 2, 3
+TRIANGLE_REFERENCE, TRIANGLE_WATERTIGHT = 0, 1
 RAY_MAX = np.float32(3.402823466e38)
 
 RAY_DTYPE = np.dtype([("origin", "<f4", 3), ("tmax", "<f4"), ("dir", "<f4", 3), ("pad", "<u4")])
@@ -72,6 +73,7 @@ _SIGNATURES = {
     "gpugi_set_max_path_length": [_P, ctypes.c_int],
     "gpugi_set_integrator_kernel": [_P, ctypes.c_int],
     "gpugi_set_rr_start_depth": [_P, ctypes.c_int],
+    "gpugi_set_triangle_test": [_P, ctypes.c_int],
     "gpugi_draw": [_P, ctypes.c_int, ctypes.c_uint32],
     "gpugi_bpt_warmup": [_P, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)],
     "gpugi_bpt_set_light_cache": [_P, ctypes.c_int32, ctypes.c_int32],
@@ -380,6 +382,9 @@ class Context:
 
     def set_rr_start_depth(self, depth):
         self._call("gpugi_set_rr_start_depth", int(depth))
+
+    def set_triangle_test(self, mode):
+        self._call("gpugi_set_triangle_test", int(mode))
 
     def draw(self, integrator, frame_seed):
         self._call("gpugi_draw", int(integrator), ctypes.c_uint32(frame_seed & 0xffffffff))

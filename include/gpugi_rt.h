@@ -52,6 +52,14 @@ enum gpugi_kernel {
   GPUGI_KERNEL_WIDE = 2       /* ordered walk on the re-packed quantized 4-wide BVH */
 };
 
+/* Which ray/triangle test traversal uses. */
+enum gpugi_triangle_test {
+  GPUGI_TRIANGLE_REFERENCE = 0,  /* IntersectTriangle (intersectiontests.glsl:63-79): not watertight */
+  GPUGI_TRIANGLE_WATERTIGHT = 1  /* Woop, Benthin, Wald 2013 with exact-sign edge functions (no
+                                    reference counterpart): no ray passes between triangles that
+                                    share an edge or vertex */
+};
+
 /* Integrators a draw runs. */
 enum gpugi_integrator {
   GPUGI_PT = 0,               /* Pathtracer::Draw (pathtracer.cpp:47-53) */
@@ -158,6 +166,11 @@ int gpugi_set_screen(gpugi_ctx* ctx, int width, int height);
 int gpugi_set_light_samples(gpugi_ctx* ctx, const gpugi_light_sample* samples, uint32_t n);
 /* MAX_PATHLENGTH define (pathtracer.cpp:60). */
 int gpugi_set_max_path_length(gpugi_ctx* ctx, int max_path_length);
+/* Triangle test of every trace and integrator (default
+ * GPUGI_TRIANGLE_REFERENCE).  Watertight mode re-packs the wide triangles
+ * as raw vertices; barycentrics keep the reference meaning (bary_y, bary_z
+ * weight the second and third vertex). */
+int gpugi_set_triangle_test(gpugi_ctx* ctx, int mode);
 /* Bounce from which Russian roulette runs (default 0: every bounce, as
  * pathtracer.comp:99-104 and lighttracer.comp:183-188 do; 3 gives the
  * "roulette after depth 3" variant, which has no reference counterpart). */

@@ -61,6 +61,11 @@ void or_trace(const or_scene* s, int any, const float* rays, int32_t* hits, uint
 /* Brute force: every valid slot with the reference triangle test, closest
  * t > 1e-4 below tmax, ties to the smaller slot; any-hit: first accepted. */
 void or_bruteforce(const or_scene* s, int any, const float* rays, int32_t* hits, size_t n);
+/* The same with the watertight triangle test when wt != 0 (Woop et al. 2013;
+ * no reference counterpart). */
+void or_trace_mode(const or_scene* s, int any, int wt, const float* rays, int32_t* hits, uint32_t* nbox,
+                   uint32_t* ntri, size_t n);
+void or_bruteforce_mode(const or_scene* s, int any, int wt, const float* rays, int32_t* hits, size_t n);
 /* FP64 classification of a ray against slots a and b (-1 = none).
  * Returns flags: 1 = a within rel_eps of an edge, 2 = b within rel_eps of an
  * edge, 4 = tie (|ta - tb| <= rel_eps * max), 8 = a or b within rel_eps of

@@ -57,6 +57,10 @@ def lib():
         l.or_trace.restype = None
         l.or_bruteforce.argtypes = [P, ctypes.c_int, P, P, ctypes.c_size_t]
         l.or_bruteforce.restype = None
+        l.or_trace_mode.argtypes = [P, ctypes.c_int, ctypes.c_int, P, P, P, P, ctypes.c_size_t]
+        l.or_trace_mode.restype = None
+        l.or_bruteforce_mode.argtypes = [P, ctypes.c_int, ctypes.c_int, P, P, ctypes.c_size_t]
+        l.or_bruteforce_mode.restype = None
         l.or_classify.argtypes = [P, P, ctypes.c_int32, ctypes.c_int32, ctypes.c_double]
         l.or_classify.restype = ctypes.c_int
         l.or_raygen_primary.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_uint32, P, ctypes.c_size_t,
@@ -118,21 +122,21 @@ def _rays_f32(rays):
     return np.ascontiguousarray(rays).view(np.float32).reshape(-1, 8)
 
 
-def trace(scene, rays, any_hit=False, counts=False):
+def trace(scene, rays, any_hit=False, counts=False, watertight=False):
     r = _rays_f32(rays)
     n = len(r)
     hits = np.zeros((n, 4), dtype=np.int32)
     nb = np.zeros(n, dtype=np.uint32) if counts else None
     nt = np.zeros(n, dtype=np.uint32) if counts else None
-    lib().or_trace(scene.ptr, int(any_hit), _p(r), _p(hits), _p(nb) if counts else None,
-                   _p(nt) if counts else None, n)
+    lib().or_trace_mode(scene.ptr, int(any_hit), int(watertight), _p(r), _p(hits), _p(nb) if counts else None,
+                        _p(nt) if counts else None, n)
     return (hits, nb, nt) if counts else hits
 
 
-def bruteforce(scene, rays, any_hit=False):
+def bruteforce(scene, rays, any_hit=False, watertight=False):
     r = _rays_f32(rays)
     hits = np.zeros((len(r), 4), dtype=np.int32)
-    lib().or_bruteforce(scene.ptr, int(any_hit), _p(r), _p(hits), len(r))
+    lib().or_bruteforce_mode(scene.ptr, int(any_hit), int(watertight), _p(r), _p(hits), len(r))
     return hits
 
 

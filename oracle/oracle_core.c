@@ -176,9 +176,71 @@ int or_triangle(vec3 o, vec3 d, vec3 p0, vec3 p1, vec3 p2, float* t, vec3* b, ve
   return (*t > 0.0001f) && b->x >= 0.0f && b->y >= 0.0f && b->z >= 0.0f;
 }
 
-/* TraceRay / TraceRayAnyHit, TRIANGLES_PER_LEAF = 2 (traceray.glsl:41-175). */
-int or_trace_one(const or_scene* s, int any, vec3 o, vec3 d, float* rayLength, int32_t* slot, vec3* bary,
-                 vec3* triNormal, int32_t tri[4], uint32_t* nbox, uint32_t* ntri) {
+/* Watertight ray/triangle test of Woop, Benthin and Wald (2013), "Watertight
+ * Ray/Triangle Intersection", JCGT 2(1): vertices are sheared into a frame
+ * whose z axis is the ray's dominant axis; the 2D edge functions U, V, W are
+ * differences of products evaluated with Kahan's fma form, whose sign is
+ * exact, so an edge shared by two triangles gets opposite signs in both and
+ * no ray slips through.  Not a reference function (the reference test is
+ * or_triangle); the HIP kernels' watertight mode restates this one.
+ * Barycentrics: b.x, b.y, b.z weight p0, p1, p2; hits need t > 1e-4. */
+static float comp_of(vec3 v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
+
+or_wtray or_wt_setup(vec3 d) {
+  or_wtray r;
+  const float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+  r.kz = (ax >= ay && ax >= az) ? 0 : (ay >= az ? 1 : 2);
+  r.kx = r.kz == 2 ? 0 : r.kz + 1;
+  r.ky = r.kx == 2 ? 0 : r.kx + 1;
+  if (comp_of(d, r.kz) < 0.0f) { int t = r.kx; r.kx = r.ky; r.ky = t; }
+  r.sx = comp_of(d, r.kx) / comp_of(d, r.kz);
+  r.sy = comp_of(d, r.ky) / comp_of(d, r.kz);
+  r.sz = 1.0f / comp_of(d, r.kz);
+  return r;
+}
+
+/* a * b - c * d with an exact sign (Kahan): w = c*d, e = c*d - w exactly. */
+static float diff_of_products(float a, float b, float c, float d) {
+  const float w = c * d;
+  const float e = fmaf(-c, d, w);
+  const float f = fmaf(a, b, -w);
+  return f + e;
+}
+
+int or_triangle_wt(vec3 o, const or_wtray* r, vec3 p0, vec3 p1, vec3 p2, float* t, vec3* b) {
+  const vec3 A = vsub(p0, o), B = vsub(p1, o), C = vsub(p2, o);
+  const float Akz = comp_of(A, r->kz), Bkz = comp_of(B, r->kz), Ckz = comp_of(C, r->kz);
+  const float Ax = comp_of(A, r->kx) - r->sx * Akz, Ay = comp_of(A, r->ky) - r->sy * Akz;
+  const float Bx = comp_of(B, r->kx) - r->sx * Bkz, By = comp_of(B, r->ky) - r->sy * Bkz;
+  const float Cx = comp_of(C, r->kx) - r->sx * Ckz, Cy = comp_of(C, r->ky) - r->sy * Ckz;
+  const float U = diff_of_products(Cx, By, Cy, Bx);
+  const float V = diff_of_products(Ax, Cy, Ay, Cx);
+  const float W = diff_of_products(Bx, Ay, By, Ax);
+  if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return 0;
+  const float det = U + V + W;
+  if (det == 0.0f) return 0;
+  const float Az = r->sz * Akz, Bz = r->sz * Bkz, Cz = r->sz * Ckz;
+  const float T = U * Az + V * Bz + W * Cz;
+  const float inv = 1.0f / det;
+  *t = T * inv;
+  b->x = U * inv;
+  b->y = V * inv;
+  b->z = W * inv;
+  return *t > 0.0001f;
+}
+
+static int tri_test(int wt, vec3 o, vec3 d, const or_wtray* r, vec3 p0, vec3 p1, vec3 p2, float* t, vec3* b,
+                    vec3* n) {
+  if (!wt) return or_triangle(o, d, p0, p1, p2, t, b, n);
+  *n = v3(0, 0, 0);
+  return or_triangle_wt(o, r, p0, p1, p2, t, b);
+}
+
+/* TraceRay / TraceRayAnyHit, TRIANGLES_PER_LEAF = 2 (traceray.glsl:41-175),
+ * with the reference triangle test (wt = 0) or the watertight one. */
+static int trace_one(const or_scene* s, int any, int wt, vec3 o, vec3 d, float* rayLength, int32_t* slot,
+                     vec3* bary, vec3* triNormal, int32_t tri[4], uint32_t* nbox, uint32_t* ntri) {
+  const or_wtray wr = or_wt_setup(d);
   int node = 0, leaf = 0;
   int nextIsLeaf = 0;
   vec3 inv = v3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
@@ -209,7 +271,7 @@ int or_trace_one(const or_scene* s, int any, vec3 o, vec3 d, float* rayLength, i
         float hit;
         vec3 b, n;
         if (ntri) ++*ntri;
-        if (or_triangle(o, d, pos_of(s, t[0]), pos_of(s, t[1]), pos_of(s, t[2]), &hit, &b, &n) &&
+        if (tri_test(wt, o, d, &wr, pos_of(s, t[0]), pos_of(s, t[1]), pos_of(s, t[2]), &hit, &b, &n) &&
             hit < *rayLength) {
           if (any) return 1;
           *rayLength = hit;
@@ -226,8 +288,18 @@ int or_trace_one(const or_scene* s, int any, vec3 o, vec3 d, float* rayLength, i
   return 0;
 }
 
+int or_trace_one(const or_scene* s, int any, vec3 o, vec3 d, float* rayLength, int32_t* slot, vec3* bary,
+                 vec3* triNormal, int32_t tri[4], uint32_t* nbox, uint32_t* ntri) {
+  return trace_one(s, any, 0, o, d, rayLength, slot, bary, triNormal, tri, nbox, ntri);
+}
+
 void or_trace(const or_scene* s, int any, const float* rays, int32_t* hits, uint32_t* nbox, uint32_t* ntri,
               size_t n) {
+  or_trace_mode(s, any, 0, rays, hits, nbox, ntri, n);
+}
+
+void or_trace_mode(const or_scene* s, int any, int wt, const float* rays, int32_t* hits, uint32_t* nbox,
+                   uint32_t* ntri, size_t n) {
   for (size_t i = 0; i < n; ++i) {
     const float* r = rays + i * 8;
     vec3 o = v3(r[0], r[1], r[2]), d = v3(r[4], r[5], r[6]);
@@ -235,7 +307,7 @@ void or_trace(const or_scene* s, int any, const float* rays, int32_t* hits, uint
     int32_t slot;
     vec3 b = v3(0, 0, 0);
     uint32_t cb = 0, ct = 0;
-    int a = or_trace_one(s, any, o, d, &len, &slot, &b, NULL, NULL, nbox ? &cb : NULL, ntri ? &ct : NULL);
+    int a = trace_one(s, any, wt, o, d, &len, &slot, &b, NULL, NULL, nbox ? &cb : NULL, ntri ? &ct : NULL);
     int32_t* h = hits + i * 4;
     if (any) {
       h[0] = a; h[1] = h[2] = h[3] = 0;
@@ -251,10 +323,15 @@ void or_trace(const or_scene* s, int any, const float* rays, int32_t* hits, uint
 }
 
 void or_bruteforce(const or_scene* s, int any, const float* rays, int32_t* hits, size_t n) {
+  or_bruteforce_mode(s, any, 0, rays, hits, n);
+}
+
+void or_bruteforce_mode(const or_scene* s, int any, int wt, const float* rays, int32_t* hits, size_t n) {
   const size_t slots = (size_t)s->num_leaves * 2;
   for (size_t i = 0; i < n; ++i) {
     const float* r = rays + i * 8;
     vec3 o = v3(r[0], r[1], r[2]), d = v3(r[4], r[5], r[6]);
+    const or_wtray wr = or_wt_setup(d);
     float best = r[3];
     int32_t bestSlot = -1;
     vec3 bb = v3(0, 0, 0);
@@ -264,7 +341,7 @@ void or_bruteforce(const or_scene* s, int any, const float* rays, int32_t* hits,
       if (t[0] == t[1]) continue;
       float hit;
       vec3 b, nn;
-      if (!or_triangle(o, d, pos_of(s, t[0]), pos_of(s, t[1]), pos_of(s, t[2]), &hit, &b, &nn)) continue;
+      if (!tri_test(wt, o, d, &wr, pos_of(s, t[0]), pos_of(s, t[1]), pos_of(s, t[2]), &hit, &b, &nn)) continue;
       if (any) {
         if (hit < best) found = 1;
       } else if (hit < best) {  /* slots ascend, so the first of equal t is kept */
