@@ -51,6 +51,9 @@ def parse():
     p.add_argument("--mlp-ref-spp", type=int, default=4096, help="samples per pixel of the plain reference render")
     p.add_argument("--mlp-frames", type=int, default=256, help="largest frame count of the two progressive renders")
     p.add_argument("--mlp-warmup", type=int, default=32, help="cached frames drawn before accumulation starts")
+    p.add_argument("--batch", action="store_true",
+                   help="batched rendering of 16 scenes: per-scene launches vs persistent launch vs hipGraph replay")
+    p.add_argument("--batch-frames", type=int, default=10, help="frames timed per batched mode")
     p.add_argument("--calibrate", action="store_true",
                    help="append 3 device-to-device copies of 1 GiB (counter calibration for rocprofv3 --pmc)")
     return p.parse_args()
@@ -191,6 +194,42 @@ def summarize_counters(name, *dirs):
     print(json.dumps(out))
 
 
+def batch_benchmark(frames):
+    """16 heightfield scenes (10k..2M triangles, 256^2..1024^2 pixels) rendered by the
+    path tracer: per-scene launches vs one persistent launch vs its hipGraph replay.
+    Aggregate Grays/s = ray queries (closest + any hit) / GPU time."""
+    from gpugi_amd import rt, workloads
+    t0 = time.perf_counter()
+    scenes = workloads.batch_scenes()
+    b = rt.Batch(0)
+    for hs, cam, w, h, lights in scenes:
+        b.add(hs, cam, w, h, lights, workloads.BATCH_PATH_LENGTH)
+    b.finalize()
+    setup_s = time.perf_counter() - t0
+    tuned = b.autotune(2)
+    out = {}
+    for name, mode in (("separate", 0), ("persistent", 1), ("graph", 2)):
+        b.reset(0)
+        b.run(mode, 1)  # warm-up (and capture for the graph)
+        b.reset(0)
+        ms = b.run(mode, frames)
+        rays = b.rays()
+        out[name] = {"ms_per_frame": round(ms / frames, 3), "grays": round(rays / (ms * 1e-3) / 1e9, 4),
+                     "rays_per_frame": rays // frames}
+    b.close()
+    return {
+        "metric": "batched path tracing Grays/s (16 scenes, one persistent launch)",
+        "value": out["graph"]["grays"], "unit": "Grays/s", "higher_is_better": True,
+        "speedup_vs_separate": round(out["separate"]["ms_per_frame"] / out["graph"]["ms_per_frame"], 3),
+        "modes": out, "autotuned": tuned, "frames": frames,
+        "config": {"workload": "16 heightfields, triangles log-uniform in [10k, 2M], square 256..1024 px, "
+                               "4 point lights, path length %d" % workloads.BATCH_PATH_LENGTH,
+                   "triangles": [int(s[0].info()["num_triangles"]) for s in scenes],
+                   "pixels": [int(s[2] * s[3]) for s in scenes]},
+        "setup_s": round(setup_s, 2),
+    }
+
+
 MLP_PATH_LENGTH = 8
 MLP_LIGHT_INTENSITY = 0.5
 
@@ -313,6 +352,9 @@ def main():
         sys.exit(2)
     if args.mlp:
         print(json.dumps(mlp_benchmark(args.mlp_ref_spp, args.mlp_frames, args.mlp_warmup)))
+        return
+    if args.batch:
+        print(json.dumps(batch_benchmark(args.batch_frames)))
         return
     from gpugi_amd import rt, workloads
 

@@ -19,6 +19,7 @@ struct TraceCtx {
   float* stkT;
   int tid;
   uint2* spill;
+  uint32_t* count = nullptr;  // when set, incremented per ray query
 };
 
 struct ClosestHit {
@@ -35,6 +36,7 @@ struct ClosestHit {
 template <bool WIDE>
 __device__ __forceinline__ ClosestHit trace_closest(const TraceCtx& c, gg::V3 o, gg::V3 d) {
   ClosestHit h;
+  if (c.count) ++*c.count;
   if (WIDE) {
     WideResult r = c.p->watertight
                        ? trace_wide_ray<false, kTileThreads, kWideLdsStack, true>(c.p->wide, o, d, gg::kRayMax,
@@ -75,6 +77,7 @@ __device__ __forceinline__ ClosestHit trace_closest(const TraceCtx& c, gg::V3 o,
 
 template <bool WIDE>
 __device__ __forceinline__ bool trace_any(const TraceCtx& c, gg::V3 o, gg::V3 d, float len) {
+  if (c.count) ++*c.count;
   if (WIDE) {
     WideResult r = c.p->watertight
                        ? trace_wide_ray<true, kTileThreads, kWideLdsStack, true>(c.p->wide, o, d, len, c.stkNode,
@@ -126,6 +129,62 @@ __device__ __forceinline__ gg::V3 estimate_direct_light(const TraceCtx& c, gg::V
     }
   }
   return gg::v3s(0.0f);
+}
+
+// pathtracer.comp:16-110 for pixel (gx, gy) (NUM_LIGHT_SAMPLES 1,
+// RUSSIAN_ROULETTE, black environment): adds vec4(color, 0) to the
+// accumulation buffer.  rays (optional) counts the rays this path traced.
+template <bool WIDE>
+__device__ __forceinline__ void pt_pixel(const DrawParams& p, TraceCtx c, int gx, int gy, unsigned long long* rays) {
+  const float lightSampleWeight = (float)p.numLights / 1.0f;
+  uint32_t seed = gg::init_random_seed(p.frameSeed, (uint32_t)(gx + gy * p.width));
+  gg::V3 color = gg::v3s(0.0f);
+  gg::V3 throughput = gg::v3s(1.0f);
+  bool didNEE = false;
+  uint32_t traced = 0;
+  if (rays) c.count = &traced;
+  const float j0 = gg::random01(seed);
+  const float j1 = gg::random01(seed);
+  const gg::V3 U = gg::v3(p.camera.u[0], p.camera.u[1], p.camera.u[2]);
+  const gg::V3 V = gg::v3(p.camera.v[0], p.camera.v[1], p.camera.v[2]);
+  const gg::V3 W = gg::v3(p.camera.w[0], p.camera.w[1], p.camera.w[2]);
+  gg::V3 o = gg::v3(p.camera.position[0], p.camera.position[1], p.camera.position[2]);
+  gg::V3 d = gg::primary_direction(j0, j1, gx, gy, p.width, p.height, U, V, W);
+  for (int i = 0; i < p.maxPathLength; ++i) {
+    const ClosestHit h = trace_closest<WIDE>(c, o, d);
+    if (!h.hit) {
+      color = color + throughput * gg::v3s(0.0f);  // black environment map (renderersystem.cpp:39-47)
+      break;
+    }
+    o = o + h.t * d;
+    const gg::V3 n = shading_normal(p.ref.vinfo, h.tri, h.bary);
+    const MaterialData m = load_material(p.materials, h.tri.w);
+    if (!didNEE) color = color + throughput * m.emissivity;
+    didNEE = m.refl[3] < 500.0f;
+    const int sampleIndex = (int)(gg::xorshift32(seed) % p.numLights);
+    const gg::V3 radiance = estimate_direct_light<WIDE>(c, o, n, sampleIndex, d, m, didNEE);
+    color = color + radiance * throughput * lightSampleWeight;
+    gg::V3 tp = gg::v3s(1.0f);
+    float pdf;
+    bool diffuse;
+    d = sample_bsdf(d, m, seed, n, tp, false, pdf, diffuse);
+    o = o + d * gg::kRayHitEpsilon;
+    throughput = throughput * tp;
+    if (i >= p.rrStartDepth) {  // Russian roulette (pathtracer.comp:99-104) from rrStartDepth on
+      const float q = gg::saturate(gg::luminance(tp));
+      if (gg::random01(seed) >= q) break;
+      throughput = throughput / q;
+    }
+  }
+  float4* px = &p.accum[(size_t)gy * p.width + gx];
+  float4 v = *px;
+  *px = make_float4(color.x + v.x, color.y + v.y, color.z + v.z, 0.0f + v.w);
+  if (rays) {
+    // one atomic per wave
+    uint32_t sum = traced;
+    for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off);
+    if ((threadIdx.x & 63) == 0) atomicAdd(rays, (unsigned long long)sum);
+  }
 }
 
 }  // namespace gpugi

@@ -10,7 +10,7 @@
 
 namespace gpugi {
 
-// pathtracer.comp:16-110 (NUM_LIGHT_SAMPLES 1, RUSSIAN_ROULETTE, black environment).
+// pathtracer.comp:16-110 for one 8x8 tile per block (the per-pixel body is pt_pixel).
 template <bool WIDE>
 __global__ __launch_bounds__(kTileThreads) void k_pathtracer(DrawParams p) {
   __shared__ uint32_t stkNode[WIDE ? kWideLdsStack * kTileThreads : 1];
@@ -20,48 +20,7 @@ __global__ __launch_bounds__(kTileThreads) void k_pathtracer(DrawParams p) {
   const int gy = blockIdx.y * kTile + (tid / kTile);
   const uint32_t linear = (blockIdx.y * gridDim.x + blockIdx.x) * kTileThreads + tid;
   TraceCtx c{&p, stkNode, stkT, tid, WIDE ? p.wide.spill + (size_t)linear * p.wide.spillDepth : nullptr};
-
-  const float lightSampleWeight = (float)p.numLights / 1.0f;
-  uint32_t seed = gg::init_random_seed(p.frameSeed, (uint32_t)(gx + gy * p.width));
-  gg::V3 color = gg::v3s(0.0f);
-  gg::V3 throughput = gg::v3s(1.0f);
-  bool didNEE = false;
-  const float j0 = gg::random01(seed);
-  const float j1 = gg::random01(seed);
-  const gg::V3 U = gg::v3(p.camera.u[0], p.camera.u[1], p.camera.u[2]);
-  const gg::V3 V = gg::v3(p.camera.v[0], p.camera.v[1], p.camera.v[2]);
-  const gg::V3 W = gg::v3(p.camera.w[0], p.camera.w[1], p.camera.w[2]);
-  gg::V3 o = gg::v3(p.camera.position[0], p.camera.position[1], p.camera.position[2]);
-  gg::V3 d = gg::primary_direction(j0, j1, gx, gy, p.width, p.height, U, V, W);
-  for (int i = 0; i < p.maxPathLength; ++i) {
-    const ClosestHit h = trace_closest<WIDE>(c, o, d);
-    if (!h.hit) {
-      color = color + throughput * gg::v3s(0.0f);  // black environment map (renderersystem.cpp:39-47)
-      break;
-    }
-    o = o + h.t * d;
-    const gg::V3 n = shading_normal(p.ref.vinfo, h.tri, h.bary);
-    const MaterialData m = load_material(p.materials, h.tri.w);
-    if (!didNEE) color = color + throughput * m.emissivity;
-    didNEE = m.refl[3] < 500.0f;
-    const int sampleIndex = (int)(gg::xorshift32(seed) % p.numLights);
-    const gg::V3 radiance = estimate_direct_light<WIDE>(c, o, n, sampleIndex, d, m, didNEE);
-    color = color + radiance * throughput * lightSampleWeight;
-    gg::V3 tp = gg::v3s(1.0f);
-    float pdf;
-    bool diffuse;
-    d = sample_bsdf(d, m, seed, n, tp, false, pdf, diffuse);
-    o = o + d * gg::kRayHitEpsilon;
-    throughput = throughput * tp;
-    if (i >= p.rrStartDepth) {  // Russian roulette (pathtracer.comp:99-104) from rrStartDepth on
-      const float q = gg::saturate(gg::luminance(tp));
-      if (gg::random01(seed) >= q) break;
-      throughput = throughput / q;
-    }
-  }
-  float4* px = &p.accum[(size_t)gy * p.width + gx];
-  float4 v = *px;
-  *px = make_float4(color.x + v.x, color.y + v.y, color.z + v.z, 0.0f + v.w);
+  pt_pixel<WIDE>(p, c, gx, gy, nullptr);
 }
 
 void FreeIntegratorState(IntegratorState* s) {

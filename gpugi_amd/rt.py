@@ -118,6 +118,22 @@ _SIGNATURES = {
     "gpugi_lightsampler_set_scene": [_P, _P],
     "gpugi_lightsampler_destroy": ([_P], None),
     "gpugi_wanghash": ([ctypes.c_uint32], ctypes.c_uint32),
+    "gpugi_batch_create": [ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)],
+    "gpugi_batch_destroy": ([_P], None),
+    "gpugi_batch_last_error": ([_P], ctypes.c_char_p),
+    "gpugi_batch_add": [_P, ctypes.POINTER(SceneDesc), ctypes.POINTER(Camera), ctypes.c_int, ctypes.c_int, _P,
+                        ctypes.c_uint32, ctypes.c_int],
+    "gpugi_batch_finalize": [_P],
+    "gpugi_batch_config": [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int],
+    "gpugi_batch_reset": [_P, ctypes.c_uint32],
+    "gpugi_batch_draw": [_P, ctypes.c_int],
+    "gpugi_batch_capture": [_P],
+    "gpugi_batch_replay": [_P, ctypes.c_int],
+    "gpugi_batch_run": [_P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_float)],
+    "gpugi_batch_readback": [_P, ctypes.c_int, _P, ctypes.c_size_t],
+    "gpugi_batch_rays": [_P, ctypes.POINTER(ctypes.c_uint64)],
+    "gpugi_batch_autotune": [_P, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                             ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_float)],
     "gpugi_cache_config": [_P, ctypes.c_float, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int,
                            ctypes.c_float],
     "gpugi_cache_stats": [_P, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_float)],
@@ -539,6 +555,81 @@ class Context:
 
     def __exit__(self, *a):
         self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Many scenes rendered by one persistent launch (gpugi_batch_*)."""
+
+    def __init__(self, device=0):
+        h = ctypes.c_void_p()
+        code = lib().gpugi_batch_create(device, ctypes.byref(h))
+        if code != GPUGI_OK:
+            raise GpugiError("gpugi_batch_create", code, "no batch")
+        self._h = h
+        self._scenes = []
+
+    def _call(self, name, *args):
+        code = getattr(lib(), name)(self._h, *args)
+        if code != GPUGI_OK:
+            raise GpugiError(name, code, lib().gpugi_batch_last_error(self._h).decode())
+
+    def add(self, scene, cam, width, height, lights, max_path_length=5):
+        lights = np.ascontiguousarray(lights, dtype=LIGHT_DTYPE)
+        self._call("gpugi_batch_add", ctypes.byref(scene.desc()), ctypes.byref(cam), int(width), int(height),
+                   _ptr(lights), len(lights), int(max_path_length))
+        self._scenes.append((width, height))
+        return len(self._scenes) - 1
+
+    def finalize(self):
+        self._call("gpugi_batch_finalize")
+
+    def config(self, kernel=KERNEL_WIDE, tiles_per_grab=2, grid_multiplier=2):
+        self._call("gpugi_batch_config", int(kernel), int(tiles_per_grab), int(grid_multiplier))
+
+    def reset(self, frame_index=0):
+        self._call("gpugi_batch_reset", ctypes.c_uint32(frame_index))
+
+    def draw(self, persistent=True):
+        self._call("gpugi_batch_draw", int(bool(persistent)))
+
+    def capture(self):
+        self._call("gpugi_batch_capture")
+
+    def replay(self, times=1):
+        self._call("gpugi_batch_replay", int(times))
+
+    def run(self, mode, frames):
+        ms = ctypes.c_float()
+        self._call("gpugi_batch_run", int(mode), int(frames), ctypes.byref(ms))
+        return ms.value
+
+    def readback(self, scene):
+        w, h = self._scenes[scene]
+        out = np.zeros((h, w, 4), np.float32)
+        self._call("gpugi_batch_readback", int(scene), _ptr(out), ctypes.c_size_t(out.nbytes))
+        return out
+
+    def rays(self):
+        r = ctypes.c_uint64()
+        self._call("gpugi_batch_rays", ctypes.byref(r))
+        return r.value
+
+    def autotune(self, frames=2):
+        k, g, m, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_float()
+        self._call("gpugi_batch_autotune", int(frames), ctypes.byref(k), ctypes.byref(g), ctypes.byref(m),
+                   ctypes.byref(ms))
+        return {"kernel": k.value, "tiles_per_grab": g.value, "grid_multiplier": m.value, "ms_per_frame": ms.value}
+
+    def close(self):
+        if self._h:
+            lib().gpugi_batch_destroy(self._h)
+            self._h = None
 
     def __del__(self):
         try:

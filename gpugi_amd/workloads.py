@@ -74,3 +74,32 @@ class HeightfieldRays:
     def free(self):
         for b in (self.primary, self.primary_hits, self.incoherent, self.shadow, self.hits):
             b.free()
+
+
+BATCH_SCENES = 16
+BATCH_PATH_LENGTH = 5
+
+
+def batch_scenes(count=BATCH_SCENES, seed=MASTER_SEED, max_triangles=2_000_000, min_triangles=10_000):
+    """Scenes of the batched-rendering benchmark: heightfields whose triangle
+    counts are log-uniform in [min_triangles, max_triangles] (cells =
+    sqrt(triangles / 2)), square resolutions of 256..1024 pixels in steps of
+    128, a camera at a seeded azimuth and 4 seeded point lights each (128
+    light samples).  Returns [(HostScene, camera, width, height, lights)]."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        tris = math.exp(math.log(min_triangles) + (math.log(max_triangles) - math.log(min_triangles)) * rng.random())
+        cells = max(2, int(round(math.sqrt(tris / 2.0))))
+        size = int(256 + 128 * rng.integers(0, 7))
+        hs = rt.HostScene.heightfield(cells, (seed + 977 * i) & 0xffffffff)
+        for pos in heightfield_lights((seed + 31 * i) & 0xffffffff):
+            hs.add_point_light(pos, (0.5, 0.5, 0.5))
+        az = math.radians(360.0 * rng.random())
+        e = math.radians(HF_ELEVATION_DEG)
+        eye = [HF_TARGET[0] + HF_DISTANCE * math.cos(e) * math.sin(az), HF_TARGET[1] + HF_DISTANCE * math.sin(e),
+               HF_TARGET[2] + HF_DISTANCE * math.cos(e) * math.cos(az)]
+        cam = rt.camera(eye, HF_TARGET, (0.0, 1.0, 0.0), 40.0, size, size)
+        lights = rt.LightSampler(hs, 123).generate(128)
+        out.append((hs, cam, size, size, lights))
+    return out

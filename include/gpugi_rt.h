@@ -388,6 +388,48 @@ int gpugi_cache_config(gpugi_ctx* ctx, float spread_c, int extra_bounces, uint32
 /* Training samples and loss of the last cached frame. */
 int gpugi_cache_stats(gpugi_ctx* ctx, uint32_t* train_samples, float* loss);
 
+/* ------------------------------------------------------------------------ */
+/* Batched rendering (no reference counterpart: the reference renders one   */
+/* scene per GL context)                                                    */
+/* ------------------------------------------------------------------------ */
+/* Many scenes, each with its own BVH, camera, resolution, light samples and
+ * accumulation buffer, rendered by the path tracer (pathtracer.comp) in one
+ * persistent launch whose workgroups take 8x8 pixel tiles of all scenes from
+ * one shared queue.  A frame k draws with FrameSeed = WangHash(k) for every
+ * scene, derived on the device, so one captured frame replays as a hipGraph.
+ * A batch has its own HIP stream; it is not thread-safe. */
+typedef struct gpugi_batch gpugi_batch;
+int gpugi_batch_create(int device, gpugi_batch** out);
+void gpugi_batch_destroy(gpugi_batch* batch);
+const char* gpugi_batch_last_error(const gpugi_batch* batch);
+/* Adds a scene (arrays are copied) before gpugi_batch_finalize. */
+int gpugi_batch_add(gpugi_batch* batch, const gpugi_scene_desc* scene, const gpugi_camera* camera, int width,
+                    int height, const gpugi_light_sample* lights, uint32_t num_lights, int max_path_length);
+/* Packs and uploads all scenes; then gpugi_batch_reset(batch, 0). */
+int gpugi_batch_finalize(gpugi_batch* batch);
+/* Traversal (GPUGI_KERNEL_REFERENCE: the BVH2 escape walk, GPUGI_KERNEL_WIDE:
+ * the 4-wide BVH), tiles taken per queue access and the persistent grid in
+ * quarters of full residency (1..16).  Defaults: WIDE, 2, 2. */
+int gpugi_batch_config(gpugi_batch* batch, int kernel, int tiles_per_grab, int grid_multiplier);
+/* Clears every accumulation buffer and the ray counter; the next frame is frame_index. */
+int gpugi_batch_reset(gpugi_batch* batch, uint32_t frame_index);
+/* One frame: persistent != 0 as one persistent launch, else one launch per scene. */
+int gpugi_batch_draw(gpugi_batch* batch, int persistent);
+/* Captures one persistent frame as a hipGraph; gpugi_batch_replay launches it `times` times. */
+int gpugi_batch_capture(gpugi_batch* batch);
+int gpugi_batch_replay(gpugi_batch* batch, int times);
+/* Runs `frames` frames (mode 0: per-scene launches, 1: persistent, 2: graph
+ * replay of a persistent frame) and returns their GPU time in *ms. */
+int gpugi_batch_run(gpugi_batch* batch, int mode, int frames, float* ms);
+/* Accumulation buffer (RGBA32F, width x height) of scene `scene`. */
+int gpugi_batch_readback(gpugi_batch* batch, int scene, float* rgba, size_t bytes);
+/* Ray queries (closest + any hit) traced since the last reset. */
+int gpugi_batch_rays(gpugi_batch* batch, uint64_t* rays);
+/* Times every (kernel, tiles per grab in {1,2,4,8}, grid multiplier in
+ * {1,2,4}) on `frames` persistent frames and keeps the fastest. */
+int gpugi_batch_autotune(gpugi_batch* batch, int frames, int* kernel, int* tiles_per_grab, int* grid_multiplier,
+                         float* best_ms);
+
 /* WangHash (utilities/random.cpp:5-13): the per-iteration FrameSeed. */
 uint32_t gpugi_wanghash(uint32_t x);
 
