@@ -54,6 +54,9 @@ def parse():
     p.add_argument("--batch", action="store_true",
                    help="batched rendering of 16 scenes: per-scene launches vs persistent launch vs hipGraph replay")
     p.add_argument("--batch-frames", type=int, default=10, help="frames timed per batched mode")
+    p.add_argument("--clutter", action="store_true",
+                   help="5.12M-triangle clutter room: trace roofline and PT/LT/BPT iteration times")
+    p.add_argument("--clutter-iterations", type=int, default=4, help="timed iterations per integrator")
     p.add_argument("--calibrate", action="store_true",
                    help="append 3 device-to-device copies of 1 GiB (counter calibration for rocprofv3 --pmc)")
     return p.parse_args()
@@ -230,6 +233,87 @@ def batch_benchmark(frames):
     }
 
 
+def clutter_benchmark(steps, iterations):
+    """The 5,120,016-triangle clutter room: incoherent closest-hit Grays/s with its
+    roofline (its BVH and triangles exceed the 256 MB MALL), then PT, LT and BPT
+    iterations at 1920 x 1080, path length 8, roulette from bounce 3."""
+    from gpugi_amd import rt, workloads
+    t0 = time.perf_counter()
+    hs = rt.HostScene.clutter(workloads.CLUTTER_SPHERES, workloads.MASTER_SEED)
+    ctx = rt.Context(0)
+    ctx.load_scene(hs)
+    W, H = workloads.HF_WIDTH, workloads.HF_HEIGHT
+    n = 1 << 24
+    R = workloads.SceneRays(ctx, workloads.clutter_camera(W, H), workloads.CLUTTER_LIGHTS, n,
+                            workloads.MASTER_SEED, W, H)
+    nbox_b, ntri_b = ctx.alloc(n * 4), ctx.alloc(n * 4)
+    ctx.trace_counts_device(R.incoherent, nbox_b, ntri_b, n, rt.CLOSEST_HIT)
+    nbox = nbox_b.download(np.uint32, n).astype(np.float64)
+    ntri = ntri_b.download(np.uint32, n).astype(np.float64)
+    nbox_b.free()
+    ntri_b.free()
+    alg_bytes = float(n * (32 + 16) + 32.0 * nbox.sum() + 52.0 * ntri.sum())
+    setup_s = time.perf_counter() - t0
+    trace = {}
+    for name, buf, q in (("incoherent", R.incoherent, rt.CLOSEST_HIT), ("coherent", R.primary, rt.CLOSEST_HIT),
+                         ("shadow_any", R.shadow, rt.ANY_HIT)):
+        ctx.trace_device(buf, R.hits, n, q, rt.KERNEL_WIDE)
+        ctx.sync()
+        ctx.timer_begin()
+        for _ in range(steps):
+            ctx.trace_device(buf, R.hits, n, q, rt.KERNEL_WIDE)
+        ms = ctx.timer_end() / steps
+        trace[name] = {"ms": round(ms, 4), "grays": round(n / (ms * 1e-3) / 1e9, 4)}
+    hits = R.hits.download(np.int32, n * 4).reshape(n, 4)
+    R.free()
+    ms_inc = trace["incoherent"]["ms"]
+    roof = {"bound": "hbm", "achieved": round(alg_bytes / (ms_inc * 1e-3) / 1e9, 2), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "alg_bytes_per_ray": alg_bytes / n, "mean_box_tests": float(nbox.mean()),
+            "mean_tri_tests": float(ntri.mean())}
+    roof["frac"] = round(roof["achieved"] / HBM_PEAK_GBS, 4)
+
+    ctx.set_screen(W, H)
+    ctx.set_camera(workloads.clutter_camera(W, H))
+    ctx.set_max_path_length(workloads.CLUTTER_PATH_LENGTH)
+    ctx.set_rr_start_depth(workloads.CLUTTER_RR_START)
+    ctx.set_integrator_kernel(rt.KERNEL_WIDE)
+    sampler = rt.LightSampler(hs, 123)
+    integ = {}
+    for name, integrator, nls in (("pt", rt.PT, 128), ("lt", rt.LT, 256), ("bpt", rt.BPT, 128)):
+        if integrator == rt.BPT:
+            entries = 0
+            for k in range(8):  # capacity estimate of bidirectionalpathtracer.cpp:65-112
+                ctx.set_light_samples(sampler.generate(nls))
+                entries += ctx.bpt_warmup(rt.wanghash(k))[0] & 0xffffffff
+            mean = entries // 8
+            ctx.bpt_set_light_cache(mean + mean // 10)
+        ctx.reset_accum()
+        ctx.set_light_samples(sampler.generate(nls))
+        ctx.draw(integrator, rt.wanghash(0))
+        ctx.sync()
+        ms = 0.0
+        for k in range(iterations):
+            ctx.set_light_samples(sampler.generate(nls))
+            ctx.timer_begin()
+            ctx.draw(integrator, rt.wanghash(k + 1))
+            ms += ctx.timer_end()
+        img, _ = ctx.readback(W, H)
+        integ[name] = {"ms_per_iteration": round(ms / iterations, 3),
+                       "image_mean": float(img[..., :3].mean() / (iterations + 1))}
+    ctx.close()
+    info = hs.info()
+    return {
+        "metric": "incoherent closest-hit Grays/s (clutter room)", "value": trace["incoherent"]["grays"],
+        "unit": "Grays/s", "higher_is_better": True,
+        "config": {"workload": "clutter room, %d icospheres, %d triangles, 1920x1080, 2^24 rays per set"
+                               % (workloads.CLUTTER_SPHERES, info["num_triangles"]),
+                   "integrators": "path length %d, roulette from bounce %d, wide traversal"
+                                  % (workloads.CLUTTER_PATH_LENGTH, workloads.CLUTTER_RR_START)},
+        "trace": trace, "roofline": roof, "hit_fraction": float((hits[:, 0] >= 0).mean()),
+        "integrators": integ, "setup_s": round(setup_s, 2),
+    }
+
+
 MLP_PATH_LENGTH = 8
 MLP_LIGHT_INTENSITY = 0.5
 
@@ -355,6 +439,9 @@ def main():
         return
     if args.batch:
         print(json.dumps(batch_benchmark(args.batch_frames)))
+        return
+    if args.clutter:
+        print(json.dumps(clutter_benchmark(args.steps, args.clutter_iterations)))
         return
     from gpugi_amd import rt, workloads
 

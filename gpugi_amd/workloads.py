@@ -7,6 +7,8 @@ heightfield: 1024 x 1024 cells (2,097,152 triangles) over [0,1]^2 with
   coherent primary rays (seed S), incoherent cosine-distributed rays from the
   primary hits (seed S + 1), shadow rays toward 4 seeded point lights.
 cornell: the 36-triangle room, 256 x 256, vertical FOV 40 degrees, path length 5.
+clutter: the 20 x 20 x 8 room with 4000 icospheres (5,120,016 triangles), 1920 x 1080,
+  vertical FOV 60 degrees, path length 8, Russian roulette from bounce 3.
 """
 import math
 
@@ -53,13 +55,16 @@ def cornell_camera(size=CORNELL_SIZE):
     return rt.camera((0.0, 1.0, 3.7), (0.0, 1.0, 0.0), (0.0, 1.0, 0.0), 40.0, size, size)
 
 
-class HeightfieldRays:
-    """Generates the three ray sets on the device; buffers stay resident in HBM."""
+class SceneRays:
+    """The three ray sets of a camera and point lights, generated on the device;
+    buffers stay resident in HBM.  Coherent: primary rays of pixel i mod
+    (width * height) with seeded jitter (seed); incoherent: cosine-distributed
+    from the i-th primary hit (seed + 1); shadow: toward light i mod count."""
 
-    def __init__(self, ctx, n=HF_RAYS, seed=MASTER_SEED, width=HF_WIDTH, height=HF_HEIGHT):
+    def __init__(self, ctx, cam, lights, n, seed, width, height):
         self.ctx, self.n = ctx, n
-        self.cam = heightfield_camera(width, height)
-        self.lights = heightfield_lights(seed)
+        self.cam = cam
+        self.lights = np.ascontiguousarray(lights, dtype=np.float32)
         self.primary = ctx.alloc(n * 32)
         self.primary_hits = ctx.alloc(n * 16)
         self.incoherent = ctx.alloc(n * 32)
@@ -74,6 +79,24 @@ class HeightfieldRays:
     def free(self):
         for b in (self.primary, self.primary_hits, self.incoherent, self.shadow, self.hits):
             b.free()
+
+
+class HeightfieldRays(SceneRays):
+    """The heightfield benchmark's ray sets (camera and lights above)."""
+
+    def __init__(self, ctx, n=HF_RAYS, seed=MASTER_SEED, width=HF_WIDTH, height=HF_HEIGHT):
+        super().__init__(ctx, heightfield_camera(width, height), heightfield_lights(seed), n, seed, width, height)
+
+
+CLUTTER_SPHERES = 4000
+CLUTTER_PATH_LENGTH = 8
+CLUTTER_RR_START = 3
+CLUTTER_LIGHTS = np.array([[-6.0, 6.5, -5.0], [5.0, 5.5, 6.0], [6.0, 7.0, -6.0]], np.float32)
+
+
+def clutter_camera(width=HF_WIDTH, height=HF_HEIGHT):
+    """Inside the 20 x 20 x 8 clutter room, near the +z wall, looking at its middle."""
+    return rt.camera((0.0, 4.0, 9.5), (0.0, 3.0, 0.0), (0.0, 1.0, 0.0), 60.0, width, height)
 
 
 BATCH_SCENES = 16
