@@ -88,23 +88,38 @@ __device__ __forceinline__ bool wide_step(const WideScene& s, WideState& st, uin
   const float sz = __uint_as_float(((q0.w >> 16) & 0xffu) << 23);
   const int nchild = (int)(q0.w >> 24);
   const uint32_t childs[4] = {q1.x, q1.y, q1.z, q1.w};
+  // Slab distances straight from the quantized bytes: along axis a,
+  //   t(q) = (origin_a + q * s_a - o_a) / d_a ~ q * A_a + B_a,
+  //   A_a = s_a * inv_a (exact: s_a is a power of two),  B_a = (origin_a - o_a) * inv_a.
+  // B_a carries at most ~3 roundings of |B_a|, so near distances use B_a - e_a and
+  // far ones B_a + e_a with e_a = |B_a| * 2^-21; the byte word giving the near
+  // plane (lo or hi) is picked per axis from the sign of inv_a.  The host checks
+  // in extended precision that origin + q * s encloses every child box, so with
+  // the far distance widened by 1 + 2^-20 a box that holds the closest hit is
+  // never rejected (the same argument as Ize 2013, "Robust BVH ray traversal").
+  const float Ax = sx * st.ix, Ay = sy * st.iy, Az = sz * st.iz;
+  const float Bx = (ox - o.x) * st.ix, By = (oy - o.y) * st.iy, Bz = (oz - o.z) * st.iz;
+  const float ex = fabsf(Bx) * 4.76837158e-7f, ey = fabsf(By) * 4.76837158e-7f, ez = fabsf(Bz) * 4.76837158e-7f;
+  const float bnx = Bx - ex, bny = By - ey, bnz = Bz - ez;
+  const float bfx = Bx + ex, bfy = By + ey, bfz = Bz + ez;
+  const uint32_t nearX = st.ix >= 0.0f ? q2.x : q2.w, farX = st.ix >= 0.0f ? q2.w : q2.x;
+  const uint32_t nearY = st.iy >= 0.0f ? q2.y : q3.x, farY = st.iy >= 0.0f ? q3.x : q2.y;
+  const uint32_t nearZ = st.iz >= 0.0f ? q2.z : q3.y, farZ = st.iz >= 0.0f ? q3.y : q2.z;
+  const float tbScaled = st.tbest * 1.00000095f;
   float tn[4];
   bool hit[4];
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
-    const float lox = __builtin_fmaf(byte_f(q2.x, k), sx, ox);
-    const float loy = __builtin_fmaf(byte_f(q2.y, k), sy, oy);
-    const float loz = __builtin_fmaf(byte_f(q2.z, k), sz, oz);
-    const float hix = __builtin_fmaf(byte_f(q2.w, k), sx, ox);
-    const float hiy = __builtin_fmaf(byte_f(q3.x, k), sy, oy);
-    const float hiz = __builtin_fmaf(byte_f(q3.y, k), sz, oz);
-    const float tx0 = (lox - o.x) * st.ix, tx1 = (hix - o.x) * st.ix;
-    const float ty0 = (loy - o.y) * st.iy, ty1 = (hiy - o.y) * st.iy;
-    const float tz0 = (loz - o.z) * st.iz, tz1 = (hiz - o.z) * st.iz;
-    const float tnear = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.0f));
-    const float tfar = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1)) * 1.00000036f;
+    const float nx = __builtin_fmaf(byte_f(nearX, k), Ax, bnx);
+    const float ny = __builtin_fmaf(byte_f(nearY, k), Ay, bny);
+    const float nz = __builtin_fmaf(byte_f(nearZ, k), Az, bnz);
+    const float fx = __builtin_fmaf(byte_f(farX, k), Ax, bfx);
+    const float fy = __builtin_fmaf(byte_f(farY, k), Ay, bfy);
+    const float fz = __builtin_fmaf(byte_f(farZ, k), Az, bfz);
+    const float tnear = fmaxf(fmaxf(nx, ny), fmaxf(nz, 0.0f));
+    const float tfar = fminf(fminf(fx, fy), fz) * 1.00000095f;
     tn[k] = tnear;
-    hit[k] = (k < nchild) && tnear <= tfar && tnear <= st.tbest;
+    hit[k] = (k < nchild) && tnear <= tfar && tnear <= tbScaled;
   }
   // Leaf children first: they can only shorten tbest.
 #pragma unroll
@@ -151,7 +166,7 @@ __device__ __forceinline__ bool wide_step(const WideScene& s, WideState& st, uin
   }
   if (ANY && st.found) return true;
   // Inner children, nearest first.
-  const float tb = st.tbest;
+  const float tb = st.tbest * 1.00000095f;  // entry distances are lower bounds with the same slack
   float t0 = (hit[0] && !(childs[0] & kWideLeafFlag) && tn[0] <= tb) ? tn[0] : INFINITY;
   float t1 = (hit[1] && !(childs[1] & kWideLeafFlag) && tn[1] <= tb) ? tn[1] : INFINITY;
   float t2 = (hit[2] && !(childs[2] & kWideLeafFlag) && tn[2] <= tb) ? tn[2] : INFINITY;
@@ -217,7 +232,7 @@ __device__ __forceinline__ bool wide_step(const WideScene& s, WideState& st, uin
       nd = e.x;
       t = __uint_as_float(e.y);
     }
-    if (t <= st.tbest) {
+    if (t <= st.tbest * 1.00000095f) {
       st.node = nd;
       st.sp = sp;
       return false;

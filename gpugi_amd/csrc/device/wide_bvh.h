@@ -7,8 +7,8 @@
 //   q2 = {lo.x[4], lo.y[4], lo.z[4], hi.x[4]}   one byte per child
 //   q3 = {hi.y[4], hi.z[4], 0, 0}
 // child box along axis a = [origin_a + lo*scale_a, origin_a + hi*scale_a]; the
-// host rounds lo down and hi up and checks in FP32 that the decoded box
-// contains the exact box, so the quantized tree is conservative.
+// host rounds lo down and hi up and checks in extended precision that the
+// exact origin + q * scale encloses the child box, so the quantized tree is conservative.
 // Triangle: 48 bytes {p0, slot}, {e0 = p1-p0, material}, {e1 = p0-p2, 0}: the
 // operands of the reference triangle test, computed once in FP32 (in
 // watertight mode the raw vertices {p0, slot}, {p1, material}, {p2, 0}).

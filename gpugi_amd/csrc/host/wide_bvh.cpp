@@ -40,7 +40,11 @@ bool Children(const Bvh2View& v, uint32_t i, std::vector<uint32_t>* out, std::st
   return false;
 }
 
-float Decode(float origin, uint32_t q, float scale) { return origin + (float)q * scale; }
+// The exact real origin + q * scale (extended precision: the kernel never rounds a
+// decoded bound, it folds q * scale into the slab distance).
+long double Decode(float origin, uint32_t q, float scale) {
+  return (long double)origin + (long double)q * (long double)scale;
+}
 float ScaleOf(uint32_t biased) { uint32_t u = biased << 23; float f; std::memcpy(&f, &u, 4); return f; }
 
 }  // namespace
