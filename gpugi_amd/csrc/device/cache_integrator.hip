@@ -168,14 +168,33 @@ __device__ __forceinline__ float target_channel(float C, float Tend, float pred,
   return fminf(fmaxf(L, 0.0f), 1e4f);
 }
 
+// When the frame yields more training vertices (T) than the cache trains on
+// (M), vertex `off` of the raster-ordered list is kept iff it is the first to
+// map to floor(off * M / T), which spreads the M samples evenly over the
+// image.  The kept sample k then goes to (k % steps) * (M / steps) + k / steps,
+// so every training step sees the whole image rather than a band of it.
 __global__ void k_cache_targets(CacheParams cp, uint32_t numPaths, int maxLen, const uint32_t* __restrict__ offsets,
-                                uint32_t maxSamples, float* __restrict__ feat, float* __restrict__ targets) {
+                                const uint32_t* __restrict__ total, uint32_t maxSamples, uint32_t steps,
+                                float* __restrict__ feat, float* __restrict__ targets) {
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t j = (uint32_t)(t / (uint32_t)maxLen);
   const uint32_t i = (uint32_t)(t % (uint32_t)maxLen);
   if (j >= numPaths || i >= cp.trainCount[j]) return;
-  const uint32_t off = offsets[j] + i;
-  if (off >= maxSamples) return;
+  const uint64_t totalV = *total;
+  const uint64_t M = maxSamples;
+  const uint64_t rank = offsets[j] + i;
+  uint64_t k = rank;
+  if (totalV > M) {
+    if ((rank * M) % totalV >= M) return;
+    k = rank * M / totalV;
+  }
+  const uint64_t n = totalV < M ? totalV : M;
+  const uint64_t per = steps > 0 ? n / steps : n;
+  if (steps > 1 && per > 0) {
+    if (k >= per * steps) return;  // the remainder does not fill a whole step
+    k = (k % steps) * per + k / steps;
+  }
+  const uint32_t off = (uint32_t)k;
   const float* r = cp.trainVert + ((size_t)j * maxLen + i) * kTrainVertFloats;
   for (int k = 0; k < 4; ++k)
     reinterpret_cast<float4*>(feat + (size_t)off * kMlpRaw)[k] = reinterpret_cast<const float4*>(r)[k];
@@ -229,13 +248,14 @@ hipError_t LaunchCacheResolve(const DrawParams& p, const CacheParams& cp, hipStr
   return hipGetLastError();
 }
 
-hipError_t LaunchCacheTargets(const DrawParams& p, const CacheParams& cp, const uint32_t* offsets, uint32_t maxSamples,
-                              float* feat, float* targets, hipStream_t stream) {
+hipError_t LaunchCacheTargets(const DrawParams& p, const CacheParams& cp, const uint32_t* offsets,
+                              const uint32_t* total, uint32_t maxSamples, uint32_t steps, float* feat, float* targets,
+                              hipStream_t stream) {
   const uint32_t paths = CacheTrainPaths(p.width, p.height, cp.trainStride);
   const size_t threads = (size_t)paths * p.maxPathLength;
   if (threads == 0) return hipSuccess;
   hipLaunchKernelGGL(k_cache_targets, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, stream, cp, paths,
-                     p.maxPathLength, offsets, maxSamples, feat, targets);
+                     p.maxPathLength, offsets, total, maxSamples, steps, feat, targets);
   return hipGetLastError();
 }
 

@@ -467,7 +467,8 @@ int DrawCached(gpugi_ctx* ctx, uint32_t frameSeed) {
   HIP_TRY(ctx, LaunchMlpInfer(ctx->mlp, cp.tailFeat, tailPred, NT, ctx->stream));
   HIP_TRY(ctx, LaunchCacheResolve(p, cp, ctx->stream));
   HIP_TRY(ctx, LaunchExclusiveScanU32(cp.trainCount, offsets, NT, scanTemp, dTotal, ctx->stream));
-  HIP_TRY(ctx, LaunchCacheTargets(p, cp, offsets, S, feat, targets, ctx->stream));
+  HIP_TRY(ctx, LaunchCacheTargets(p, cp, offsets, dTotal, S, (uint32_t)std::max(ctx->cacheTrainSteps, 1), feat,
+                                  targets, ctx->stream));
   uint32_t count = 0;
   HIP_TRY(ctx, hipMemcpyAsync(&count, dTotal, 4, hipMemcpyDeviceToHost, ctx->stream));
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));

@@ -117,10 +117,12 @@ hipError_t LaunchFillU32(uint32_t* p, uint32_t value, size_t n, hipStream_t stre
 hipError_t LaunchCachedPaths(const DrawParams& p, const CacheParams& cp, void* spill, hipStream_t stream);
 // accum += pixelColor + queryWeight * max(queryPred, 0).
 hipError_t LaunchCacheResolve(const DrawParams& p, const CacheParams& cp, hipStream_t stream);
-// Dense training samples: features n x 16 and targets n x 4 at the exclusive
-// offsets `offsets` of the per-path vertex counts, at most maxSamples.
-hipError_t LaunchCacheTargets(const DrawParams& p, const CacheParams& cp, const uint32_t* offsets, uint32_t maxSamples,
-                              float* feat, float* targets, hipStream_t stream);
+// Dense training samples (features n x 16, targets n x 4), n = min(*total, maxSamples):
+// the vertices at the exclusive offsets `offsets` of the per-path counts, evenly
+// thinned to maxSamples and interleaved over `steps` equal training steps.
+hipError_t LaunchCacheTargets(const DrawParams& p, const CacheParams& cp, const uint32_t* offsets,
+                              const uint32_t* total, uint32_t maxSamples, uint32_t steps, float* feat, float* targets,
+                              hipStream_t stream);
 
 void FreeIntegratorState(IntegratorState* s);
 void InvalidateIntegratorCaches(IntegratorState* s);
