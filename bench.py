@@ -57,20 +57,21 @@ def parse():
     p.add_argument("--clutter", action="store_true",
                    help="5.12M-triangle clutter room: trace roofline and PT/LT/BPT iteration times")
     p.add_argument("--clutter-iterations", type=int, default=4, help="timed iterations per integrator")
+    p.add_argument("--clutter-incoherent-only", action="store_true", help="only the incoherent trace (profiling)")
     p.add_argument("--calibrate", action="store_true",
                    help="append 3 device-to-device copies of 1 GiB (counter calibration for rocprofv3 --pmc)")
     return p.parse_args()
 
 
-def traffic_from_profile():
-    """HBM bytes per launch from the committed rocprofv3 --pmc summary, if any."""
-    path = os.path.join(ROOT, "profiles", "trace_wide_heightfield_pmc.json")
+def traffic_from_profile(name="trace_wide_heightfield_pmc.json"):
+    """HBM bytes per launch from a committed rocprofv3 --pmc summary, if any."""
+    path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
     try:
         with open(path) as f:
             d = json.load(f)
-        return float(d["hbm_bytes_per_launch"])
+        return float(d["hbm_bytes_per_launch"] if "hbm_bytes_per_launch" in d else d["derived"]["hbm_bytes_per_launch"])
     except (OSError, KeyError, ValueError):
         return None
 
@@ -159,6 +160,10 @@ def summarize(kt_dir, fetch_dir, write_dir, kt_log):
     print(json.dumps(out))
 
 
+def _has_counter(directory, counter):
+    return any(r["Counter_Name"] == counter for r in _csv_rows(directory, "_counter_collection.csv"))
+
+
 def summarize_counters(name, *dirs):
     """Writes profiles/<name>.json: per-launch averages of every counter that rocprofv3 --pmc collected for
     the dominant kernel in the given output directories, plus derived ratios when their inputs are there."""
@@ -181,13 +186,23 @@ def summarize_counters(name, *dirs):
         derived["vmem_read_wave_instructions_per_ray"] = per["SQ_INSTS_VMEM_RD"] / rays
     if "SQ_THREAD_CYCLES_VALU" in per and "SQ_ACTIVE_INST_VALU" in per:
         derived["valu_thread_cycles_per_active_inst"] = per["SQ_THREAD_CYCLES_VALU"] / per["SQ_ACTIVE_INST_VALU"]
-    if "SQ_ACTIVE_INST_VALU" in per and "GRBM_GUI_ACTIVE" in per:
-        # GRBM_GUI_ACTIVE is summed over the 8 XCDs; 1024 SIMDs issue at most one VALU instruction per cycle
-        derived["valu_issue_utilization"] = per["SQ_ACTIVE_INST_VALU"] / (per["GRBM_GUI_ACTIVE"] / 8.0 * 1024.0)
-    out = {"kernel_prefix": DOMINANT_KERNEL, "workload": PROFILE_WORKLOAD,
+    if "SQ_INSTS_VALU" in per and "GRBM_GUI_ACTIVE" in per:
+        # GRBM_GUI_ACTIVE is summed over the 8 XCDs; each of the 1024 SIMDs needs 4 cycles per
+        # wave64 FP32 VALU instruction, so this is the fraction of SIMD cycles spent issuing VALU
+        derived["valu_issue_fraction"] = 4.0 * per["SQ_INSTS_VALU"] / (per["GRBM_GUI_ACTIVE"] / 8.0 * 1024.0)
+    clutter = "clutter" in name
+    workload = ("clutter room, 4000 icospheres (5,120,016 triangles), 2^24 incoherent rays, closest hit"
+                if clutter else PROFILE_WORKLOAD)
+    run = "--clutter --clutter-incoherent-only --steps 3 --calibrate" if clutter else "--steps 2 --warmup 1 --no-cpu-baseline"
+    if "FETCH_SIZE" in per and "WRITE_SIZE" in per:
+        gib_kib = float(1 << 20)
+        fc = _pmc_per_launch([d for d in dirs if _has_counter(d, "FETCH_SIZE")][0], "FETCH_SIZE")[1]
+        wc = _pmc_per_launch([d for d in dirs if _has_counter(d, "WRITE_SIZE")][0], "WRITE_SIZE")[1]
+        derived["calibration"] = {"fetch_factor": gib_kib / fc, "write_factor": gib_kib / wc}
+        derived["hbm_bytes_per_launch"] = (per["FETCH_SIZE"] * gib_kib / fc + per["WRITE_SIZE"] * gib_kib / wc) * 1024.0
+    out = {"kernel_prefix": DOMINANT_KERNEL, "workload": workload,
            "commands": ["cd /tmp && export TMPDIR=/tmp && timeout -s KILL 120 rocprofv3 --pmc <counters of one pass> "
-                        "-d <out>/<pass> -o <pass> --output-format csv -- python3 <repo>/bench.py --steps 2 --warmup 1 "
-                        "--no-cpu-baseline",
+                        "-d <out>/<pass> -o <pass> --output-format csv -- python3 <repo>/bench.py " + run,
                         "python3 <repo>/bench.py --summarize-counters %s %s" % (name, " ".join(
                             "<out>/" + os.path.basename(os.path.normpath(d)) for d in dirs))],
            "per_launch": per, "derived": derived}
@@ -233,10 +248,11 @@ def batch_benchmark(frames):
     }
 
 
-def clutter_benchmark(steps, iterations):
+def clutter_benchmark(steps, iterations, incoherent_only=False, calibrate=False):
     """The 5,120,016-triangle clutter room: incoherent closest-hit Grays/s with its
     roofline (its BVH and triangles exceed the 256 MB MALL), then PT, LT and BPT
-    iterations at 1920 x 1080, path length 8, roulette from bounce 3."""
+    iterations at 1920 x 1080, path length 8, roulette from bounce 3.
+    incoherent_only: only the incoherent trace (the counter-profiling run)."""
     from gpugi_amd import rt, workloads
     t0 = time.perf_counter()
     hs = rt.HostScene.clutter(workloads.CLUTTER_SPHERES, workloads.MASTER_SEED)
@@ -255,8 +271,9 @@ def clutter_benchmark(steps, iterations):
     alg_bytes = float(n * (32 + 16) + 32.0 * nbox.sum() + 52.0 * ntri.sum())
     setup_s = time.perf_counter() - t0
     trace = {}
-    for name, buf, q in (("incoherent", R.incoherent, rt.CLOSEST_HIT), ("coherent", R.primary, rt.CLOSEST_HIT),
-                         ("shadow_any", R.shadow, rt.ANY_HIT)):
+    sets = (("incoherent", R.incoherent, rt.CLOSEST_HIT), ("coherent", R.primary, rt.CLOSEST_HIT),
+            ("shadow_any", R.shadow, rt.ANY_HIT))
+    for name, buf, q in sets[:1] if incoherent_only else sets:
         ctx.trace_device(buf, R.hits, n, q, rt.KERNEL_WIDE)
         ctx.sync()
         ctx.timer_begin()
@@ -271,6 +288,21 @@ def clutter_benchmark(steps, iterations):
             "unit": "GB/s", "alg_bytes_per_ray": alg_bytes / n, "mean_box_tests": float(nbox.mean()),
             "mean_tri_tests": float(ntri.mean())}
     roof["frac"] = round(roof["achieved"] / HBM_PEAK_GBS, 4)
+    traffic = traffic_from_profile("trace_wide_clutter_pmc.json")
+    if traffic:
+        roof["traffic"] = traffic
+        roof["hbm_gbs_measured"] = round(traffic / (ms_inc * 1e-3) / 1e9, 2)
+    if incoherent_only:
+        if calibrate:  # 3 device-to-device copies of 1 GiB (counter calibration)
+            a, b = ctx.alloc(1 << 30), ctx.alloc(1 << 30)
+            for _ in range(3):
+                b.copy_from(a, 1 << 30)
+            ctx.sync()
+            a.free()
+            b.free()
+        ctx.close()
+        return {"metric": "incoherent closest-hit Grays/s (clutter room)", "value": trace["incoherent"]["grays"],
+                "unit": "Grays/s", "trace": trace, "roofline": roof}
 
     ctx.set_screen(W, H)
     ctx.set_camera(workloads.clutter_camera(W, H))
@@ -441,7 +473,8 @@ def main():
         print(json.dumps(batch_benchmark(args.batch_frames)))
         return
     if args.clutter:
-        print(json.dumps(clutter_benchmark(args.steps, args.clutter_iterations)))
+        print(json.dumps(clutter_benchmark(args.steps, args.clutter_iterations, args.clutter_incoherent_only,
+                                           args.calibrate)))
         return
     from gpugi_amd import rt, workloads
 
