@@ -13,10 +13,11 @@
 // nearest first, and a popped entry whose entry distance lies beyond the
 // current hit is dropped without fetching its node.
 //
-// Box tests use the quantized boxes the host made conservative, with the far
-// distance widened by 1 + 3*2^-23 (Ize 2013, "Robust BVH ray traversal") and
-// axis-parallel direction components replaced by +-1e-20 so that 0 * inf never
-// occurs: a box that contains the closest hit is never rejected.  The
+// Box tests use the quantized boxes the host made conservative, slab distances
+// computed with one fma per plane and a slack that covers their rounding (see
+// wide_step), and axis-parallel direction components replaced by +-1e-20 so
+// that 0 * inf never occurs: a box that contains the closest hit is never
+// rejected.  The
 // triangle test is the reference formula (intersectiontests.glsl:63-79), and
 // equal distances resolve to the smaller reference slot index, which is the
 // triangle the reference's preorder walk meets first.
@@ -121,48 +122,59 @@ __device__ __forceinline__ bool wide_step(const WideScene& s, WideState& st, uin
     tn[k] = tnear;
     hit[k] = (k < nchild) && tnear <= tfar && tnear <= tbScaled;
   }
-  // Leaf children first: they can only shorten tbest.
+  // Leaf children first: they can only shorten tbest.  The triangles of all
+  // hit leaf children form one list per lane and each lane walks its own, so a
+  // wave spends max over lanes of their triangle counts, not the sum over the
+  // four child slots of per-slot maxima.
+  uint32_t leafMask = 0;
 #pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    if (hit[k] && (childs[k] & kWideLeafFlag)) {
-      const uint32_t first = childs[k] & kWideLeafFirstMask;
-      const uint32_t cnt = ((childs[k] >> 27) & 0xfu) + 1u;
-      for (uint32_t m = 0; m < cnt; ++m) {
-        const float4 a = s.tris[3 * (size_t)(first + m) + 0];
-        const float4 b = s.tris[3 * (size_t)(first + m) + 1];
-        const float4 c = s.tris[3 * (size_t)(first + m) + 2];
-        const gg::V3 p0 = gg::v3(a.x, a.y, a.z);
-        const int slot = __float_as_int(a.w);
-        float t, by, bz;
-        bool ok;
-        if (WT) {
-          gg::V3 bary;
-          ok = intersect_triangle_wt(o, st.wr, p0, gg::v3(b.x, b.y, b.z), gg::v3(c.x, c.y, c.z), t, bary);
-          by = bary.y;
-          bz = bary.z;
-        } else {
-          const gg::V3 e0 = gg::v3(b.x, b.y, b.z);
-          const gg::V3 e1 = gg::v3(c.x, c.y, c.z);
-          const gg::V3 nn = gg::cross(e1, e0);
-          const gg::V3 e2 = (1.0f / gg::dot(nn, d)) * (p0 - o);
-          const gg::V3 ii = gg::cross(d, e2);
-          by = gg::dot(ii, e1);
-          bz = gg::dot(ii, e0);
-          const float bx = 1.0f - (bz + by);
-          t = gg::dot(nn, e2);
-          ok = t > 0.0001f && bx >= 0.0f && by >= 0.0f && bz >= 0.0f;
-        }
-        if (ANY) {
-          if (ok && t < st.tbest) { st.found = true; break; }
-        } else if (ok && (t < st.tbest || (t == st.tbest && slot < st.bestSlot))) {
-          st.tbest = t;
-          st.bestSlot = slot;
-          st.bu = by;
-          st.bv = bz;
-        }
-      }
+  for (int k = 0; k < 4; ++k) leafMask |= (hit[k] && (childs[k] & kWideLeafFlag)) ? (1u << k) : 0u;
+  uint32_t m = 0;
+  while (leafMask) {
+    const int k = __builtin_ctz(leafMask);
+    const uint32_t ck = k == 0 ? childs[0] : (k == 1 ? childs[1] : (k == 2 ? childs[2] : childs[3]));
+    const uint32_t first = ck & kWideLeafFirstMask;
+    const uint32_t cnt = ((ck >> 27) & 0xfu) + 1u;
+    const size_t ti = 3 * (size_t)(first + m);
+    const float4 a = s.tris[ti + 0];
+    const float4 b = s.tris[ti + 1];
+    const float4 c = s.tris[ti + 2];
+    const gg::V3 p0 = gg::v3(a.x, a.y, a.z);
+    const int slot = __float_as_int(a.w);
+    float t, by, bz;
+    bool ok;
+    if (WT) {
+      gg::V3 bary;
+      ok = intersect_triangle_wt(o, st.wr, p0, gg::v3(b.x, b.y, b.z), gg::v3(c.x, c.y, c.z), t, bary);
+      by = bary.y;
+      bz = bary.z;
+    } else {
+      const gg::V3 e0 = gg::v3(b.x, b.y, b.z);
+      const gg::V3 e1 = gg::v3(c.x, c.y, c.z);
+      const gg::V3 nn = gg::cross(e1, e0);
+      const gg::V3 e2 = (1.0f / gg::dot(nn, d)) * (p0 - o);
+      const gg::V3 ii = gg::cross(d, e2);
+      by = gg::dot(ii, e1);
+      bz = gg::dot(ii, e0);
+      const float bx = 1.0f - (bz + by);
+      t = gg::dot(nn, e2);
+      ok = t > 0.0001f && bx >= 0.0f && by >= 0.0f && bz >= 0.0f;
     }
-    if (ANY && st.found) break;
+    if (ANY) {
+      if (ok && t < st.tbest) {
+        st.found = true;
+        break;
+      }
+    } else if (ok && (t < st.tbest || (t == st.tbest && slot < st.bestSlot))) {
+      st.tbest = t;
+      st.bestSlot = slot;
+      st.bu = by;
+      st.bv = bz;
+    }
+    if (++m == cnt) {
+      m = 0;
+      leafMask &= leafMask - 1u;
+    }
   }
   if (ANY && st.found) return true;
   // Inner children, nearest first.
