@@ -14,6 +14,7 @@ cpu_baseline: the CPU oracle's escape walk (single thread) on a bounded
 sample of the same rays.
 """
 import argparse
+import itertools
 import json
 import os
 import sys
@@ -489,19 +490,18 @@ def main():
     if args.sweep:
         sets = (("incoherent", R.incoherent, rt.CLOSEST_HIT), ("coherent", R.primary, rt.CLOSEST_HIT),
                 ("shadow_any", R.shadow, rt.ANY_HIT))
-        for leaf in (1, 2, 4, 8):
-            for stack in (4, 8, 12, 16, 24):
-                for refill in (8, 16, 32, 64):
-                    ctx.set_wide_config(leaf, stack, refill)
-                    row = {}
-                    for name, buf, q in sets:
-                        ctx.trace_device(buf, R.hits, n, q, rt.KERNEL_WIDE)
-                        ctx.sync()
-                        ctx.timer_begin()
-                        for _ in range(3):
-                            ctx.trace_device(buf, R.hits, n, q, rt.KERNEL_WIDE)
-                        row[name] = round(n / (ctx.timer_end() / 3 * 1e-3) / 1e9, 3)
-                    print(json.dumps(dict(ctx.wide_config(), **row)), flush=True)
+        for leaf, stack, refill, wait in itertools.product((1, 2, 4), (8, 12, 16, 24), (8, 16, 32),
+                                                           (1, 8, 16, 32)):
+            ctx.set_wide_config(leaf, stack, refill, wait)
+            row = {}
+            for name, buf, q in sets:
+                ctx.trace_device(buf, R.hits, n, q, rt.KERNEL_WIDE)
+                ctx.sync()
+                ctx.timer_begin()
+                for _ in range(3):
+                    ctx.trace_device(buf, R.hits, n, q, rt.KERNEL_WIDE)
+                row[name] = round(n / (ctx.timer_end() / 3 * 1e-3) / 1e9, 3)
+            print(json.dumps(dict(ctx.wide_config(), **row)), flush=True)
         ctx.close()
         return
 

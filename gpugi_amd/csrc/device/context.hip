@@ -40,8 +40,9 @@ struct gpugi_ctx {
   size_t spillBytes = 0;
   uint32_t wideMaxStack = 0;
   int wideLeaf = 2;     // triangles per merged wide leaf
-  int wideStack = 12;   // LDS stack entries per lane of the trace kernel
-  int wideRefill = 16;  // idle lanes that trigger a wave-level ray refill
+  int wideStack = 8;    // LDS stack entries per lane of the trace kernel
+  int wideRefill = 8;   // idle lanes that trigger a wave-level ray refill
+  int wideWait = 16;    // lanes waiting with a pending leaf that trigger a leaf pass
   // host copy of the reference layout (re-packing after a configuration change)
   std::vector<float> hHier, hPos;
   std::vector<int32_t> hTris;
@@ -614,8 +615,8 @@ int gpugi_trace(gpugi_ctx* ctx, int query, int kernel, const gpugi_ray* rays, gp
   if (wide) {
     WideScene ws;
     if (int rc = PrepareTraceWide(ctx, ctx->wideStack, &ws)) return rc;
-    HIP_TRY(ctx, LaunchTraceWide(ws, ctx->wideStack, query, dr, dh, n, ctx->wideRefill, Watertight(ctx),
-                                 ctx->stream));
+    HIP_TRY(ctx, LaunchTraceWide(ws, ctx->wideStack, query, dr, dh, n, ctx->wideRefill, ctx->wideWait,
+                                 Watertight(ctx), ctx->stream));
   } else {
     HIP_TRY(ctx, LaunchTraceRef(MakeRef(ctx), query, dr, dh, n, Watertight(ctx), ctx->stream));
   }
@@ -629,16 +630,18 @@ int gpugi_trace(gpugi_ctx* ctx, int query, int kernel, const gpugi_ray* rays, gp
   return GPUGI_OK;
 }
 
-int gpugi_set_wide_config(gpugi_ctx* ctx, int max_leaf_triangles, int lds_stack_entries, int refill_lanes) {
+int gpugi_set_wide_config(gpugi_ctx* ctx, int max_leaf_triangles, int lds_stack_entries, int refill_lanes,
+                          int wait_lanes) {
   if (!ctx) return Fail(ctx, GPUGI_ERR_ARG, "gpugi_set_wide_config: null context");
   bool stackOk = false;
   for (int v : kWideStackVariants) stackOk |= v == lds_stack_entries;
   if (!stackOk || max_leaf_triangles < 1 || max_leaf_triangles > (int)kWideMaxLeafTris || refill_lanes < 1 ||
-      refill_lanes > 64)
+      refill_lanes > 64 || wait_lanes < 1 || wait_lanes > 64)
     return Fail(ctx, GPUGI_ERR_ARG,
-                "gpugi_set_wide_config: leaf size in [1, 16], stack one of 4/8/12/16/24, refill in [1, 64]");
+                "gpugi_set_wide_config: leaf size in [1, 16], stack one of 4/8/12/16/24, refill and wait in [1, 64]");
   ctx->wideStack = lds_stack_entries;
   ctx->wideRefill = refill_lanes;
+  ctx->wideWait = wait_lanes;
   if (max_leaf_triangles != ctx->wideLeaf) {
     ctx->wideLeaf = max_leaf_triangles;
     if (ctx->hasScene) {
@@ -650,11 +653,12 @@ int gpugi_set_wide_config(gpugi_ctx* ctx, int max_leaf_triangles, int lds_stack_
 }
 
 int gpugi_get_wide_config(gpugi_ctx* ctx, int* max_leaf_triangles, int* lds_stack_entries, int* refill_lanes,
-                          uint32_t* max_stack) {
+                          int* wait_lanes, uint32_t* max_stack) {
   if (!ctx) return Fail(ctx, GPUGI_ERR_ARG, "gpugi_get_wide_config: null context");
   if (max_leaf_triangles) *max_leaf_triangles = ctx->wideLeaf;
   if (lds_stack_entries) *lds_stack_entries = ctx->wideStack;
   if (refill_lanes) *refill_lanes = ctx->wideRefill;
+  if (wait_lanes) *wait_lanes = ctx->wideWait;
   if (max_stack) *max_stack = ctx->wideMaxStack;
   return GPUGI_OK;
 }
@@ -669,29 +673,45 @@ int gpugi_autotune_trace(gpugi_ctx* ctx, int query, const gpugi_ray* rays_dev, g
   if (SetDev(ctx)) return GPUGI_ERR_HIP;
   const int leaves[] = {1, 2, 4, 8};
   const int refills[] = {8, 16, 32, 64};
+  const int waits[] = {1, 8, 16, 24, 32, 48};
   float best = INFINITY;
-  int bestLeaf = ctx->wideLeaf, bestStack = ctx->wideStack, bestRefill = ctx->wideRefill;
+  int bestLeaf = ctx->wideLeaf, bestStack = ctx->wideStack, bestRefill = ctx->wideRefill, bestWait = ctx->wideWait;
+  // one untimed launch, then the average of `repeats`
+  auto time = [&](const WideScene& ws, int stack, int refill, int wait, float* ms) -> int {
+    HIP_TRY(ctx, LaunchTraceWide(ws, stack, query, rays_dev, hits_dev, n, refill, wait, Watertight(ctx), ctx->stream));
+    HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    for (int r = 0; r < repeats; ++r)
+      HIP_TRY(ctx, LaunchTraceWide(ws, stack, query, rays_dev, hits_dev, n, refill, wait, Watertight(ctx), ctx->stream));
+    HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    HIP_TRY(ctx, hipEventSynchronize(ctx->ev1));
+    HIP_TRY(ctx, hipEventElapsedTime(ms, ctx->ev0, ctx->ev1));
+    *ms /= (float)repeats;
+    return GPUGI_OK;
+  };
+  // leaf size x stack depth x refill at the current wait, then the wait threshold
   for (int leaf : leaves) {
-    if (int rc = gpugi_set_wide_config(ctx, leaf, ctx->wideStack, ctx->wideRefill)) return rc;
+    if (int rc = gpugi_set_wide_config(ctx, leaf, ctx->wideStack, ctx->wideRefill, ctx->wideWait)) return rc;
     for (int stack : kWideStackVariants) {
       WideScene ws;
       if (int rc = PrepareTraceWide(ctx, stack, &ws)) return rc;
       for (int refill : refills) {
-        // one untimed launch, then the average of `repeats`
-        HIP_TRY(ctx, LaunchTraceWide(ws, stack, query, rays_dev, hits_dev, n, refill, Watertight(ctx), ctx->stream));
-        HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-        for (int r = 0; r < repeats; ++r)
-          HIP_TRY(ctx, LaunchTraceWide(ws, stack, query, rays_dev, hits_dev, n, refill, Watertight(ctx), ctx->stream));
-        HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
-        HIP_TRY(ctx, hipEventSynchronize(ctx->ev1));
         float ms = 0.0f;
-        HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
-        ms /= (float)repeats;
+        if (int rc = time(ws, stack, refill, ctx->wideWait, &ms)) return rc;
         if (ms < best) { best = ms; bestLeaf = leaf; bestStack = stack; bestRefill = refill; }
       }
     }
   }
-  if (int rc = gpugi_set_wide_config(ctx, bestLeaf, bestStack, bestRefill)) return rc;
+  if (int rc = gpugi_set_wide_config(ctx, bestLeaf, bestStack, bestRefill, bestWait)) return rc;
+  {
+    WideScene ws;
+    if (int rc = PrepareTraceWide(ctx, bestStack, &ws)) return rc;
+    for (int wait : waits) {
+      float ms = 0.0f;
+      if (int rc = time(ws, bestStack, bestRefill, wait, &ms)) return rc;
+      if (ms < best) { best = ms; bestWait = wait; }
+    }
+  }
+  if (int rc = gpugi_set_wide_config(ctx, bestLeaf, bestStack, bestRefill, bestWait)) return rc;
   if (best_ms) *best_ms = best;
   return GPUGI_OK;
 }

@@ -15,9 +15,10 @@ hipError_t LaunchTraceRefCounts(const RefScene& s, int query, const gpugi_ray* r
                                 uint32_t* ntri, size_t n, hipStream_t stream);
 // Wide kernel with an LDS stack of `stack` entries per lane (one of kWideStackVariants).
 // refill: a wave reloads idle lanes once at least this many (1..64) are idle.
+// wait: a wave runs a leaf pass once at least this many (1..64) lanes wait for one.
 // watertight: the watertight triangle test (needs the raw-vertex triangle layout).
 hipError_t LaunchTraceWide(const WideScene& s, int stack, int query, const gpugi_ray* rays, gpugi_hit* hits,
-                           size_t n, int refill, bool watertight, hipStream_t stream);
+                           size_t n, int refill, int wait, bool watertight, hipStream_t stream);
 // Resident blocks per CU of the wide kernel variant (sizes the persistent grid).
 hipError_t WideBlocksPerCU(int stack, int* blocks);
 constexpr int kNumWideStackVariants = 5;

@@ -1,26 +1,35 @@
 // Ordered traversal of the 4-wide quantized BVH (wide_bvh.h), one ray per lane.
 //
-// The walk is a state machine: wide_step() visits one node (four child boxes,
-// the triangles of hit leaf children, ordered push of hit inner children) and
-// reports whether the ray is finished.  trace_wide_ray() runs it to the end for
-// one ray; the persistent trace kernel interleaves steps with wave-level ray
-// refills instead.
+// A ray's state holds the next inner node to visit, at most one pending leaf
+// and a stack of (child word, entry distance) pairs.  Work comes in two kinds
+// of wave-wide passes:
+//   box pass   every lane with a node visits it: four child box tests, the
+//              nearest hit inner child becomes the next node, the nearest hit
+//              leaf becomes the pending leaf if that slot is free, the other
+//              hit children are pushed farthest first;
+//   leaf pass  every lane with a pending leaf tests its triangles.
+// A lane whose pending slot is taken keeps traversing (speculative traversal,
+// Aila and Laine 2009, "Understanding the efficiency of ray traversal on
+// GPUs"), so a wave runs a leaf pass only once enough lanes wait for one and
+// does not pay a divergent triangle loop on every node visit.  The caller
+// chooses the pass from two ballots (wide_choose_leaf_pass).  Lanes left
+// without a node pop: entries beyond the current hit are dropped, a leaf
+// entry fills the free pending slot, an inner entry becomes the next node.
+// The result does not depend on the order of the work: the closest hit is the
+// minimum distance, ties resolved to the smaller reference slot index (the
+// triangle the reference's preorder walk meets first).
 //
-// Per lane, a short stack of (node, entry distance) pairs lives in LDS laid
-// out [depth][thread], so lanes at different depths touch different banks;
-// deeper entries spill to a per-thread slice of global memory whose size the
-// host derives from the exact worst case of the tree.  Children are visited
-// nearest first, and a popped entry whose entry distance lies beyond the
-// current hit is dropped without fetching its node.
+// Per lane, the first S stack entries live in LDS laid out [depth][thread],
+// so lanes at different depths touch different banks; deeper entries spill to
+// a per-thread slice of global memory whose size the host derives from the
+// exact worst case of the tree.
 //
 // Box tests use the quantized boxes the host made conservative, slab distances
 // computed with one fma per plane and a slack that covers their rounding (see
-// wide_step), and axis-parallel direction components replaced by +-1e-20 so
+// wide_visit), and axis-parallel direction components replaced by +-1e-20 so
 // that 0 * inf never occurs: a box that contains the closest hit is never
-// rejected.  The
-// triangle test is the reference formula (intersectiontests.glsl:63-79), and
-// equal distances resolve to the smaller reference slot index, which is the
-// triangle the reference's preorder walk meets first.
+// rejected.  The triangle test is the reference formula
+// (intersectiontests.glsl:63-79) or, in watertight mode, the watertight test.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -29,6 +38,8 @@
 #include "wide_bvh.h"
 
 namespace gpugi {
+
+constexpr uint32_t kWideNoNode = 0xffffffffu;
 
 struct WideResult {
   float t;     // final ray length (tmax on a miss)
@@ -43,7 +54,8 @@ struct WideState {
   float tbest;
   int bestSlot;
   float bu, bv;
-  uint32_t node;
+  uint32_t node;  // next inner node to visit, kWideNoNode if none
+  uint32_t leaf;  // pending leaf child word (leaf flag set), 0 if none
   int sp;
   bool found;
   WtRay wr;  // watertight mode only
@@ -67,28 +79,28 @@ __device__ __forceinline__ void wide_init(WideState& st, gg::V3 o, gg::V3 d, flo
   st.bestSlot = -1;
   st.bu = st.bv = 0.0f;
   st.node = 0;
+  st.leaf = 0;
   st.sp = 0;
   st.found = false;
   if (WT) st.wr = wt_setup(d);
 }
 
-// One node visit.  Returns true when the ray is finished.  WT: the watertight
-// triangle test on the raw-vertex triangle layout {p0, slot}{p1, mat}{p2, 0}.
-template <bool ANY, int B, int S, bool WT = false>
-__device__ __forceinline__ bool wide_step(const WideScene& s, WideState& st, uint32_t* stkNode, float* stkT, int tid,
-                                          uint2* spill) {
-  const gg::V3 o = st.o, d = st.d;
-  const uint32_t node = st.node;
-  const uint4 q0 = s.nodes[4 * (size_t)node + 0];
-  const uint4 q1 = s.nodes[4 * (size_t)node + 1];
-  const uint4 q2 = s.nodes[4 * (size_t)node + 2];
-  const uint4 q3 = s.nodes[4 * (size_t)node + 3];
+// Box pass for a lane with st.node != kWideNoNode.  Returns false if the stack
+// would overflow (impossible when the host sized the spill from the tree).
+template <bool ANY, int B, int S>
+__device__ __forceinline__ bool wide_visit(const WideScene& s, WideState& st, uint32_t* stkNode, float* stkT,
+                                           int tid, uint2* spill) {
+  const gg::V3 o = st.o;
+  const uint32_t nodeIndex = st.node;
+  const uint4 q0 = s.nodes[4 * (size_t)nodeIndex + 0];
+  const uint4 q1 = s.nodes[4 * (size_t)nodeIndex + 1];
+  const uint4 q2 = s.nodes[4 * (size_t)nodeIndex + 2];
+  const uint4 q3 = s.nodes[4 * (size_t)nodeIndex + 3];
   const float ox = __uint_as_float(q0.x), oy = __uint_as_float(q0.y), oz = __uint_as_float(q0.z);
   const float sx = __uint_as_float((q0.w & 0xffu) << 23);
   const float sy = __uint_as_float(((q0.w >> 8) & 0xffu) << 23);
   const float sz = __uint_as_float(((q0.w >> 16) & 0xffu) << 23);
   const int nchild = (int)(q0.w >> 24);
-  const uint32_t childs[4] = {q1.x, q1.y, q1.z, q1.w};
   // Slab distances straight from the quantized bytes: along axis a,
   //   t(q) = (origin_a + q * s_a - o_a) / d_a ~ q * A_a + B_a,
   //   A_a = s_a * inv_a (exact: s_a is a power of two),  B_a = (origin_a - o_a) * inv_a.
@@ -108,7 +120,6 @@ __device__ __forceinline__ bool wide_step(const WideScene& s, WideState& st, uin
   const uint32_t nearZ = st.iz >= 0.0f ? q2.z : q3.y, farZ = st.iz >= 0.0f ? q3.y : q2.z;
   const float tbScaled = st.tbest * 1.00000095f;
   float tn[4];
-  bool hit[4];
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     const float nx = __builtin_fmaf(byte_f(nearX, k), Ax, bnx);
@@ -119,22 +130,80 @@ __device__ __forceinline__ bool wide_step(const WideScene& s, WideState& st, uin
     const float fz = __builtin_fmaf(byte_f(farZ, k), Az, bfz);
     const float tnear = fmaxf(fmaxf(nx, ny), fmaxf(nz, 0.0f));
     const float tfar = fminf(fminf(fx, fy), fz) * 1.00000095f;
-    tn[k] = tnear;
-    hit[k] = (k < nchild) && tnear <= tfar && tnear <= tbScaled;
+    tn[k] = ((k < nchild) && tnear <= tfar && tnear <= tbScaled) ? tnear : INFINITY;
   }
-  // Leaf children first: they can only shorten tbest.  The triangles of all
-  // hit leaf children form one list per lane and each lane walks its own, so a
-  // wave spends max over lanes of their triangle counts, not the sum over the
-  // four child slots of per-slot maxima.
-  uint32_t leafMask = 0;
-#pragma unroll
-  for (int k = 0; k < 4; ++k) leafMask |= (hit[k] && (childs[k] & kWideLeafFlag)) ? (1u << k) : 0u;
-  uint32_t m = 0;
-  while (leafMask) {
-    const int k = __builtin_ctz(leafMask);
-    const uint32_t ck = k == 0 ? childs[0] : (k == 1 ? childs[1] : (k == 2 ? childs[2] : childs[3]));
-    const uint32_t first = ck & kWideLeafFirstMask;
-    const uint32_t cnt = ((ck >> 27) & 0xfu) + 1u;
+  float t0 = tn[0], t1 = tn[1], t2 = tn[2], t3 = tn[3];
+  uint32_t c0 = q1.x, c1 = q1.y, c2 = q1.z, c3 = q1.w;
+  if (!ANY) {  // nearest first; missed children (INFINITY) sink to the end
+#define GPUGI_CSWAP(ta, ca, tb_, cb)    \
+  if (tb_ < ta) {                       \
+    float tt = ta; ta = tb_; tb_ = tt;  \
+    uint32_t cc = ca; ca = cb; cb = cc; \
+  }
+    GPUGI_CSWAP(t0, c0, t1, c1);
+    GPUGI_CSWAP(t2, c2, t3, c3);
+    GPUGI_CSWAP(t0, c0, t2, c2);
+    GPUGI_CSWAP(t1, c1, t3, c3);
+    GPUGI_CSWAP(t1, c1, t2, c2);
+#undef GPUGI_CSWAP
+  }
+  // The nearest hit inner child is visited next, the nearest hit leaf fills a
+  // free pending slot; p_k: child k goes on the stack.
+  uint32_t node = kWideNoNode, leaf = st.leaf;
+#define GPUGI_CLAIM(t, c, pk)                   \
+  bool pk = t != INFINITY;                      \
+  if (pk) {                                     \
+    if (c & kWideLeafFlag) {                    \
+      if (leaf == 0u) { leaf = c; pk = false; } \
+    } else if (node == kWideNoNode) {           \
+      node = c;                                 \
+      pk = false;                               \
+    }                                           \
+  }
+  GPUGI_CLAIM(t0, c0, p0)
+  GPUGI_CLAIM(t1, c1, p1)
+  GPUGI_CLAIM(t2, c2, p2)
+  GPUGI_CLAIM(t3, c3, p3)
+#undef GPUGI_CLAIM
+  bool overflow = false;
+  int sp = st.sp;
+#define GPUGI_PUSH(nd, tv)                                   \
+  do {                                                       \
+    if (sp < S) {                                            \
+      stkNode[sp * B + tid] = (nd);                          \
+      stkT[sp * B + tid] = (tv);                             \
+      ++sp;                                                  \
+    } else if ((uint32_t)(sp - S) < s.spillDepth) {          \
+      spill[sp - S] = make_uint2((nd), __float_as_uint(tv)); \
+      ++sp;                                                  \
+    } else {                                                 \
+      overflow = true;                                       \
+    }                                                        \
+  } while (0)
+  if (p3) GPUGI_PUSH(c3, t3);
+  if (p2) GPUGI_PUSH(c2, t2);
+  if (p1) GPUGI_PUSH(c1, t1);
+  if (p0) GPUGI_PUSH(c0, t0);
+#undef GPUGI_PUSH
+  st.node = node;
+  st.leaf = leaf;
+  st.sp = sp;
+  if (overflow) {
+    atomicOr(s.errorFlag, 1u);
+    st.bestSlot = -2;
+    return false;
+  }
+  return true;
+}
+
+// Leaf pass for a lane with a pending leaf: tests its triangles and frees the slot.
+template <bool ANY, bool WT>
+__device__ __forceinline__ void wide_leaf(const WideScene& s, WideState& st) {
+  const gg::V3 o = st.o, d = st.d;
+  const uint32_t first = st.leaf & kWideLeafFirstMask;
+  const uint32_t cnt = ((st.leaf >> 27) & 0xfu) + 1u;
+  st.leaf = 0;
+  for (uint32_t m = 0; m < cnt; ++m) {
     const size_t ti = 3 * (size_t)(first + m);
     const float4 a = s.tris[ti + 0];
     const float4 b = s.tris[ti + 1];
@@ -163,7 +232,7 @@ __device__ __forceinline__ bool wide_step(const WideScene& s, WideState& st, uin
     if (ANY) {
       if (ok && t < st.tbest) {
         st.found = true;
-        break;
+        return;
       }
     } else if (ok && (t < st.tbest || (t == st.tbest && slot < st.bestSlot))) {
       st.tbest = t;
@@ -171,96 +240,91 @@ __device__ __forceinline__ bool wide_step(const WideScene& s, WideState& st, uin
       st.bu = by;
       st.bv = bz;
     }
-    if (++m == cnt) {
-      m = 0;
-      leafMask &= leafMask - 1u;
-    }
   }
-  if (ANY && st.found) return true;
-  // Inner children, nearest first.
-  const float tb = st.tbest * 1.00000095f;  // entry distances are lower bounds with the same slack
-  float t0 = (hit[0] && !(childs[0] & kWideLeafFlag) && tn[0] <= tb) ? tn[0] : INFINITY;
-  float t1 = (hit[1] && !(childs[1] & kWideLeafFlag) && tn[1] <= tb) ? tn[1] : INFINITY;
-  float t2 = (hit[2] && !(childs[2] & kWideLeafFlag) && tn[2] <= tb) ? tn[2] : INFINITY;
-  float t3 = (hit[3] && !(childs[3] & kWideLeafFlag) && tn[3] <= tb) ? tn[3] : INFINITY;
-  uint32_t c0 = childs[0], c1 = childs[1], c2 = childs[2], c3 = childs[3];
-  if (!ANY) {
-#define GPUGI_CSWAP(ta, ca, tb_, cb)    \
-  if (tb_ < ta) {                       \
-    float tt = ta; ta = tb_; tb_ = tt;  \
-    uint32_t cc = ca; ca = cb; cb = cc; \
-  }
-    GPUGI_CSWAP(t0, c0, t1, c1);
-    GPUGI_CSWAP(t2, c2, t3, c3);
-    GPUGI_CSWAP(t0, c0, t2, c2);
-    GPUGI_CSWAP(t1, c1, t3, c3);
-    GPUGI_CSWAP(t1, c1, t2, c2);
-#undef GPUGI_CSWAP
-  }
-  bool overflow = false;
+}
+
+// For a lane without a next node: pops entries still in front of the closest
+// hit until one yields a node, or a leaf meets a taken pending slot (the entry
+// stays), or the stack is empty.
+template <int B, int S>
+__device__ __forceinline__ void wide_pop(WideState& st, const uint32_t* stkNode, const float* stkT, int tid,
+                                         const uint2* spill) {
+  const float tb = st.tbest * 1.00000095f;  // entry distances are lower bounds with this slack
   int sp = st.sp;
-#define GPUGI_PUSH(nd, tv)                                   \
-  do {                                                       \
-    if (sp < S) {                                            \
-      stkNode[sp * B + tid] = (nd);                          \
-      stkT[sp * B + tid] = (tv);                             \
-      ++sp;                                                  \
-    } else if ((uint32_t)(sp - S) < s.spillDepth) {          \
-      spill[sp - S] = make_uint2((nd), __float_as_uint(tv)); \
-      ++sp;                                                  \
-    } else {                                                 \
-      overflow = true;                                       \
-    }                                                        \
-  } while (0)
-  // Sorted (closest hit): finite entries form a prefix, c0 is the nearest.
-  // Unsorted (any hit): the first finite entry is taken next.
-  uint32_t next = 0xffffffffu;
-  if (t3 != INFINITY) { if (ANY && next == 0xffffffffu) next = c3; else GPUGI_PUSH(c3, t3); }
-  if (t2 != INFINITY) { if (ANY && next == 0xffffffffu) next = c2; else GPUGI_PUSH(c2, t2); }
-  if (t1 != INFINITY) { if (ANY && next == 0xffffffffu) next = c1; else GPUGI_PUSH(c1, t1); }
-  if (t0 != INFINITY) { if (next == 0xffffffffu) next = c0; else GPUGI_PUSH(c0, t0); }
-#undef GPUGI_PUSH
-  if (overflow) {  // impossible when the host sized the spill from the tree; fail loudly
-    atomicOr(s.errorFlag, 1u);
-    st.bestSlot = -2;
-    st.sp = sp;
-    return true;
-  }
-  if (next != 0xffffffffu) {
-    st.node = next;
-    st.sp = sp;
-    return false;
-  }
-  // Pop the nearest pending node still in front of the closest hit.
+  uint32_t leaf = st.leaf;
+  uint32_t node = kWideNoNode;
   while (sp > 0) {
-    --sp;
+    const int top = sp - 1;
     uint32_t nd;
     float t;
-    if (sp < S) {
-      nd = stkNode[sp * B + tid];
-      t = stkT[sp * B + tid];
+    if (top < S) {
+      nd = stkNode[top * B + tid];
+      t = stkT[top * B + tid];
     } else {
-      const uint2 e = spill[sp - S];
+      const uint2 e = spill[top - S];
       nd = e.x;
       t = __uint_as_float(e.y);
     }
-    if (t <= st.tbest * 1.00000095f) {
-      st.node = nd;
-      st.sp = sp;
-      return false;
+    if (t > tb) {
+      sp = top;
+      continue;
     }
+    if (nd & kWideLeafFlag) {
+      if (leaf != 0u) break;
+      leaf = nd;
+      sp = top;
+      continue;
+    }
+    node = nd;
+    sp = top;
+    break;
   }
-  st.sp = 0;
-  return true;
+  st.sp = sp;
+  st.leaf = leaf;
+  st.node = node;
 }
 
-// stkNode/stkT: LDS arrays of S * B entries; tid in [0, B).
+// Wave-uniform pass choice from ballots over the working lanes: a leaf pass
+// when no lane has a node to visit or at least `waitLanes` lanes have only a
+// pending leaf.
+__device__ __forceinline__ bool wide_choose_leaf_pass(bool working, const WideState& st, int waitLanes) {
+  const uint64_t box = __ballot(working && st.node != kWideNoNode);
+  const uint64_t wait = __ballot(working && st.node == kWideNoNode);
+  return box == 0 || __popcll(wait) >= waitLanes;
+}
+
+// One pass for a working lane.  Returns true when the ray is finished.
+template <bool ANY, int B, int S, bool WT>
+__device__ __forceinline__ bool wide_pass(const WideScene& s, WideState& st, bool leafPass, uint32_t* stkNode,
+                                          float* stkT, int tid, uint2* spill) {
+  if (leafPass) {
+    if (st.leaf == 0u) return false;  // still traversing; unaffected by this pass
+    wide_leaf<ANY, WT>(s, st);
+    if (ANY && st.found) return true;
+  } else {
+    if (st.node == kWideNoNode) return false;  // waiting for a leaf pass
+    if (!wide_visit<ANY, B, S>(s, st, stkNode, stkT, tid, spill)) return true;  // overflow, slot -2
+  }
+  if (st.node == kWideNoNode) {
+    wide_pop<B, S>(st, stkNode, stkT, tid, spill);
+    return st.node == kWideNoNode && st.leaf == 0u;
+  }
+  return false;
+}
+
+// Traces one ray per calling lane to the end; the calling lanes form the wave
+// for pass choices.  stkNode/stkT: LDS arrays of S * B entries; tid in [0, B).
 template <bool ANY, int B, int S, bool WT = false>
 __device__ __forceinline__ WideResult trace_wide_ray(const WideScene& s, gg::V3 o, gg::V3 d, float tmax,
                                                      uint32_t* stkNode, float* stkT, int tid, uint2* spill) {
   WideState st;
   wide_init<WT>(st, o, d, tmax);
-  while (!wide_step<ANY, B, S, WT>(s, st, stkNode, stkT, tid, spill)) {
+  bool live = true;
+  for (;;) {
+    const uint64_t act = __ballot(live);
+    if (act == 0) break;
+    const bool leafPass = wide_choose_leaf_pass(live, st, (__popcll(act) + 1) / 2);
+    if (live && wide_pass<ANY, B, S, WT>(s, st, leafPass, stkNode, stkT, tid, spill)) live = false;
   }
   WideResult r;
   r.t = st.tbest;

@@ -6,10 +6,12 @@
 // so they share that XCD's L2), and within it every wave walks its own
 // interleaved sequence of 64-ray batches.
 //
-// Each wave steps its 64 lanes one node visit at a time.  When at least
-// `refill` lanes have finished (ballot + popcount), the idle lanes take the
-// next rays of the wave's sequence in one go, so a single long ray no longer
-// holds 63 idle lanes; refill = 64 reproduces batch-at-a-time processing.
+// Each wave runs its 64 lanes one box or leaf pass at a time (trace_wide.h);
+// a leaf pass is chosen when at least `wait` lanes have only a pending leaf.
+// When at least `refill` lanes have finished (ballot + popcount), the idle
+// lanes take the next rays of the wave's sequence in one go, so a single long
+// ray no longer holds 63 idle lanes; refill = 64 reproduces batch-at-a-time
+// processing.
 // The LDS stack depth is a template parameter: deeper stacks spill less but
 // cap the waves per CU (160 KiB LDS per CU); the autotuner picks it.
 // Compiled with -ffp-contract=off: the triangle test must round exactly like
@@ -25,7 +27,7 @@ constexpr uint32_t kNumXcd = 8;
 
 template <bool ANY, int B, int S, bool WT>
 __global__ __launch_bounds__(B) void k_trace_wide(WideScene s, const float4* __restrict__ rays,
-                                                  int4* __restrict__ hits, uint32_t n, int refill) {
+                                                  int4* __restrict__ hits, uint32_t n, int refill, int wait) {
   __shared__ uint32_t stkNode[S * B];
   __shared__ float stkT[S * B];
   const int tid = threadIdx.x;
@@ -70,7 +72,8 @@ __global__ __launch_bounds__(B) void k_trace_wide(WideScene s, const float4* __r
       if (exhausted) break;
       continue;
     }
-    if (active && wide_step<ANY, B, S, WT>(s, st, stkNode, stkT, tid, spill)) {
+    const bool leafPass = wide_choose_leaf_pass(active, st, wait);
+    if (active && wide_pass<ANY, B, S, WT>(s, st, leafPass, stkNode, stkT, tid, spill)) {
       if (ANY) hits[ri] = make_int4(st.found ? 1 : 0, 0, 0, 0);
       else hits[ri] = make_int4(st.bestSlot, __float_as_int(st.tbest), __float_as_int(st.bu), __float_as_int(st.bv));
       active = false;
@@ -102,21 +105,21 @@ const Variant* FindVariant(int stack) {
 
 template <int S, bool WT>
 void LaunchS(const WideScene& s, int query, const gpugi_ray* rays, gpugi_hit* hits, size_t n, int refill,
-             hipStream_t stream) {
+             int wait, hipStream_t stream) {
   dim3 grid(s.numThreads / kWideBlock);
   if (query == GPUGI_ANY_HIT)
     hipLaunchKernelGGL((k_trace_wide<true, kWideBlock, S, WT>), grid, dim3(kWideBlock), 0, stream, s,
-                       (const float4*)rays, (int4*)hits, (uint32_t)n, refill);
+                       (const float4*)rays, (int4*)hits, (uint32_t)n, refill, wait);
   else
     hipLaunchKernelGGL((k_trace_wide<false, kWideBlock, S, WT>), grid, dim3(kWideBlock), 0, stream, s,
-                       (const float4*)rays, (int4*)hits, (uint32_t)n, refill);
+                       (const float4*)rays, (int4*)hits, (uint32_t)n, refill, wait);
 }
 
 template <int S>
 void LaunchS(const WideScene& s, int query, const gpugi_ray* rays, gpugi_hit* hits, size_t n, int refill,
-             bool watertight, hipStream_t stream) {
-  if (watertight) LaunchS<S, true>(s, query, rays, hits, n, refill, stream);
-  else LaunchS<S, false>(s, query, rays, hits, n, refill, stream);
+             int wait, bool watertight, hipStream_t stream) {
+  if (watertight) LaunchS<S, true>(s, query, rays, hits, n, refill, wait, stream);
+  else LaunchS<S, false>(s, query, rays, hits, n, refill, wait, stream);
 }
 
 }  // namespace
@@ -136,18 +139,18 @@ hipError_t WideBlocksPerCU(int stack, int* blocks) {
 }
 
 hipError_t LaunchTraceWide(const WideScene& s, int stack, int query, const gpugi_ray* rays, gpugi_hit* hits,
-                           size_t n, int refill, bool watertight, hipStream_t stream) {
+                           size_t n, int refill, int wait, bool watertight, hipStream_t stream) {
   if (n == 0) return hipSuccess;
   if (n > 0xffffffffull - 64) return hipErrorInvalidValue;
   // the static schedule needs a whole number of blocks per XCD
   if (s.numThreads == 0 || s.numThreads % (kWideBlock * kNumXcd) != 0) return hipErrorInvalidConfiguration;
-  if (refill < 1 || refill > 64) return hipErrorInvalidValue;
+  if (refill < 1 || refill > 64 || wait < 1 || wait > 64) return hipErrorInvalidValue;
   switch (stack) {
-    case 4: LaunchS<4>(s, query, rays, hits, n, refill, watertight, stream); break;
-    case 8: LaunchS<8>(s, query, rays, hits, n, refill, watertight, stream); break;
-    case 12: LaunchS<12>(s, query, rays, hits, n, refill, watertight, stream); break;
-    case 16: LaunchS<16>(s, query, rays, hits, n, refill, watertight, stream); break;
-    case 24: LaunchS<24>(s, query, rays, hits, n, refill, watertight, stream); break;
+    case 4: LaunchS<4>(s, query, rays, hits, n, refill, wait, watertight, stream); break;
+    case 8: LaunchS<8>(s, query, rays, hits, n, refill, wait, watertight, stream); break;
+    case 12: LaunchS<12>(s, query, rays, hits, n, refill, wait, watertight, stream); break;
+    case 16: LaunchS<16>(s, query, rays, hits, n, refill, wait, watertight, stream); break;
+    case 24: LaunchS<24>(s, query, rays, hits, n, refill, wait, watertight, stream); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

@@ -53,7 +53,7 @@ static_assert(sizeof(WideTriHost) == 48, "wide triangle is 48 bytes");
 struct WideBVHHost {
   std::vector<WideNodeHost> nodes;
   std::vector<WideTriHost> tris;
-  uint32_t maxStack = 0;   // worst-case traversal stack entries (sum of siblings along a path)
+  uint32_t maxStack = 0;   // worst-case traversal stack entries (siblings summed along a path)
   uint32_t depth = 0;
 };
 

@@ -224,7 +224,10 @@ std::string BuildWideBVH(const gpugi_scene_desc& desc, uint32_t maxLeafTris, Wid
     node.pad[0] = innerKids;  // used for the stack bound only
     out->nodes[p.wideIndex] = node;
   }
-  // Worst-case stack: need(n) = (inner children - 1) + max need(child); wide
+  // Worst-case stack.  A visit pushes every hit child except the nearest inner
+  // one, which is visited next (leaf children are pushed when the lane's
+  // pending-leaf slot is taken), so need(n) = (children - [n has an inner
+  // child]) + max need(inner child); popped leaf entries push nothing.  Wide
   // nodes are numbered parent-before-child, so a reverse sweep is bottom-up.
   uint32_t depthMax = 1;
   std::vector<uint32_t> depth(out->nodes.size(), 1);
@@ -233,8 +236,9 @@ std::string BuildWideBVH(const gpugi_scene_desc& desc, uint32_t maxLeafTris, Wid
   for (size_t i = out->nodes.size(); i-- > 0;) {
     uint32_t m = 0;
     for (uint32_t c : wideKids[i]) m = std::max(m, needStack[c]);
-    uint32_t k = out->nodes[i].pad[0];
-    needStack[i] = (k > 0 ? k - 1 : 0) + m;
+    const uint32_t k = out->nodes[i].pad[0];
+    const uint32_t children = out->nodes[i].exps >> 24;
+    needStack[i] = (k > 0 ? children - 1 : children) + m;
     out->nodes[i].pad[0] = 0;
   }
   out->maxStack = needStack[0];

@@ -82,9 +82,10 @@ _SIGNATURES = {
     "gpugi_readback": [_P, _P, ctypes.c_size_t, ctypes.POINTER(ctypes.c_uint32)],
     "gpugi_trace": [_P, ctypes.c_int, ctypes.c_int, _P, _P, ctypes.c_size_t, ctypes.c_int],
     "gpugi_trace_counts": [_P, ctypes.c_int, _P, _P, _P, ctypes.c_size_t, ctypes.c_int],
-    "gpugi_set_wide_config": [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int],
+    "gpugi_set_wide_config": [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int],
     "gpugi_get_wide_config": [_P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
-                              ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_uint32)],
+                              ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                              ctypes.POINTER(ctypes.c_uint32)],
     "gpugi_autotune_trace": [_P, ctypes.c_int, _P, _P, ctypes.c_size_t, ctypes.c_int, ctypes.POINTER(ctypes.c_float)],
     "gpugi_sync": [_P],
     "gpugi_alloc": [_P, ctypes.c_size_t, ctypes.POINTER(ctypes.c_void_p)],
@@ -438,14 +439,16 @@ class Context:
         self._call("gpugi_trace", int(query), int(kernel), ctypes.c_void_p(rays_buf.ptr),
                    ctypes.c_void_p(hits_buf.ptr), ctypes.c_size_t(n), 1)
 
-    def set_wide_config(self, max_leaf_triangles, lds_stack_entries, refill_lanes=16):
-        self._call("gpugi_set_wide_config", int(max_leaf_triangles), int(lds_stack_entries), int(refill_lanes))
+    def set_wide_config(self, max_leaf_triangles, lds_stack_entries, refill_lanes=8, wait_lanes=16):
+        self._call("gpugi_set_wide_config", int(max_leaf_triangles), int(lds_stack_entries), int(refill_lanes),
+                   int(wait_lanes))
 
     def wide_config(self):
-        a, b, r, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_uint32()
-        self._call("gpugi_get_wide_config", ctypes.byref(a), ctypes.byref(b), ctypes.byref(r), ctypes.byref(c))
+        a, b, r, w, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_uint32()
+        self._call("gpugi_get_wide_config", ctypes.byref(a), ctypes.byref(b), ctypes.byref(r), ctypes.byref(w),
+                   ctypes.byref(c))
         return {"max_leaf_triangles": a.value, "lds_stack_entries": b.value, "refill_lanes": r.value,
-                "max_stack": c.value}
+                "wait_lanes": w.value, "max_stack": c.value}
 
     def autotune_trace(self, rays_buf, hits_buf, n, query=CLOSEST_HIT, repeats=3):
         ms = ctypes.c_float()

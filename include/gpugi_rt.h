@@ -215,17 +215,21 @@ int gpugi_trace(gpugi_ctx* ctx, int query, int kernel, const gpugi_ray* rays, gp
                 size_t n, int rays_on_device);
 /* Configuration of the wide traversal: triangles per merged leaf (1..16; the
  * BVH is re-packed when it changes), LDS stack entries per lane of the trace
- * kernel (4, 8, 12, 16 or 24; deeper entries spill to global memory) and the
- * number of idle lanes (1..64) at which a wave refills them with new rays.
- * Default (2, 12, 16): the fastest incoherent closest-hit configuration on
- * the 2,097,152-triangle heightfield.  No reference counterpart: the GLSL path has one fixed layout
+ * kernel (4, 8, 12, 16 or 24; deeper entries spill to global memory), the
+ * number of idle lanes (1..64) at which a wave refills them with new rays and
+ * the number of lanes (1..64) waiting with a pending leaf at which a wave runs
+ * a triangle pass instead of a box pass.  Default (2, 8, 8, 16): the fastest
+ * incoherent closest-hit configuration on the 2,097,152-triangle heightfield.  No
+ * reference counterpart: the GLSL path has one fixed layout
  * (TRIANGLES_PER_LEAF, scenedata.glsl:5). */
-int gpugi_set_wide_config(gpugi_ctx* ctx, int max_leaf_triangles, int lds_stack_entries, int refill_lanes);
+int gpugi_set_wide_config(gpugi_ctx* ctx, int max_leaf_triangles, int lds_stack_entries, int refill_lanes,
+                          int wait_lanes);
 int gpugi_get_wide_config(gpugi_ctx* ctx, int* max_leaf_triangles, int* lds_stack_entries, int* refill_lanes,
-                          uint32_t* max_stack);
+                          int* wait_lanes, uint32_t* max_stack);
 /* Autotuner: times every (leaf size, stack depth, refill) triple on the given device
- * ray set (one warm-up launch + `repeats` timed launches each), keeps the
- * fastest configuration active and returns its average launch time. */
+ * ray set (one warm-up launch + `repeats` timed launches each), then the wait
+ * threshold at the best triple, keeps the fastest configuration active and
+ * returns its average launch time. */
 int gpugi_autotune_trace(gpugi_ctx* ctx, int query, const gpugi_ray* rays_dev, gpugi_hit* hits_dev, size_t n,
                          int repeats, float* best_ms);
 /* Reference escape-walk visit counters (TRACERAY_DEBUG_VARS,

@@ -165,3 +165,29 @@ def test_full_size_incoherent_set_properties(rt, oracle, ctx):
     sc = oracle.Scene(a)
     check_wide(oracle, sc, sample, ref, h1[:: 4099], 1e-3)
     R.free()
+
+
+@pytest.mark.parametrize("leaf,stack,refill,wait", [(1, 4, 64, 1), (8, 4, 1, 64), (2, 24, 16, 32), (4, 8, 8, 16)])
+def test_wide_configurations(rt, oracle, ctx, leaf, stack, refill, wait):
+    """Every wide configuration gives the same answers: a 4-entry LDS stack
+    spills to global memory on most rays, wait 1 runs a leaf pass as soon as a
+    lane waits, wait 64 only when no lane can traverse."""
+    print("case: wide config", leaf, stack, refill, wait, flush=True)
+    hs = rt.HostScene.heightfield(64, workloads.MASTER_SEED)
+    ctx.load_scene(hs)
+    sc, prim, ph, diff = scene_rays(rt, oracle, hs, workloads.heightfield_camera(), 1920, 1080, 32768)
+    ctx.set_wide_config(leaf, stack, refill, wait)
+    cfg = ctx.wide_config()
+    assert (cfg["max_leaf_triangles"], cfg["lds_stack_entries"], cfg["refill_lanes"], cfg["wait_lanes"]) == \
+        (leaf, stack, refill, wait)
+    if stack == 4:
+        assert cfg["max_stack"] > stack  # the spill path is exercised
+    for rays in (prim, diff):
+        want = oracle.trace(sc, rays)
+        got = hits_int(ctx.trace(rays_struct(rt, rays), rt.CLOSEST_HIT, rt.KERNEL_WIDE))
+        check_wide(oracle, sc, rays, want, got, 1e-3)
+    shadow = oracle.raygen_shadow(prim, ph, workloads.heightfield_lights(), 32768)
+    want_any = oracle.trace(sc, shadow, any_hit=True)[:, 0]
+    got_any = hits_int(ctx.trace(rays_struct(rt, shadow), rt.ANY_HIT, rt.KERNEL_WIDE))[:, 0]
+    assert (got_any != want_any).sum() <= max(2, 1e-3 * len(shadow))
+    ctx.set_wide_config(2, 8, 8, 16)
