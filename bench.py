@@ -490,7 +490,7 @@ def main():
     if args.sweep:
         sets = (("incoherent", R.incoherent, rt.CLOSEST_HIT), ("coherent", R.primary, rt.CLOSEST_HIT),
                 ("shadow_any", R.shadow, rt.ANY_HIT))
-        for leaf, stack, refill, wait in itertools.product((1, 2, 4), (8, 12, 16, 24), (8, 16, 32),
+        for leaf, stack, refill, wait in itertools.product((1, 2, 4), (8, 10, 12, 16), (8, 16, 32),
                                                            (1, 8, 16, 32)):
             ctx.set_wide_config(leaf, stack, refill, wait)
             row = {}

@@ -40,7 +40,7 @@ struct gpugi_ctx {
   size_t spillBytes = 0;
   uint32_t wideMaxStack = 0;
   int wideLeaf = 2;     // triangles per merged wide leaf
-  int wideStack = 8;    // LDS stack entries per lane of the trace kernel
+  int wideStack = 10;   // LDS stack entries per lane of the trace kernel
   int wideRefill = 8;   // idle lanes that trigger a wave-level ray refill
   int wideWait = 16;    // lanes waiting with a pending leaf that trigger a leaf pass
   // host copy of the reference layout (re-packing after a configuration change)
@@ -638,7 +638,7 @@ int gpugi_set_wide_config(gpugi_ctx* ctx, int max_leaf_triangles, int lds_stack_
   if (!stackOk || max_leaf_triangles < 1 || max_leaf_triangles > (int)kWideMaxLeafTris || refill_lanes < 1 ||
       refill_lanes > 64 || wait_lanes < 1 || wait_lanes > 64)
     return Fail(ctx, GPUGI_ERR_ARG,
-                "gpugi_set_wide_config: leaf size in [1, 16], stack one of 4/8/12/16/24, refill and wait in [1, 64]");
+                "gpugi_set_wide_config: leaf size in [1, 16], stack one of 4/8/10/12/16/24, refill and wait in [1, 64]");
   ctx->wideStack = lds_stack_entries;
   ctx->wideRefill = refill_lanes;
   ctx->wideWait = wait_lanes;

@@ -21,7 +21,7 @@ hipError_t LaunchTraceWide(const WideScene& s, int stack, int query, const gpugi
                            size_t n, int refill, int wait, bool watertight, hipStream_t stream);
 // Resident blocks per CU of the wide kernel variant (sizes the persistent grid).
 hipError_t WideBlocksPerCU(int stack, int* blocks);
-constexpr int kNumWideStackVariants = 5;
+constexpr int kNumWideStackVariants = 6;
 extern const int kWideStackVariants[kNumWideStackVariants];
 
 hipError_t LaunchRaygenPrimary(const gpugi_camera& cam, int width, int height, uint32_t seed,

@@ -21,8 +21,8 @@
 //
 // Per lane, the first S stack entries live in LDS laid out [depth][thread],
 // so lanes at different depths touch different banks; deeper entries spill to
-// a per-thread slice of global memory whose size the host derives from the
-// exact worst case of the tree.
+// global memory (WideStack), sized by the host from the exact worst case of
+// the tree.
 //
 // Box tests use the quantized boxes the host made conservative, slab distances
 // computed with one fma per plane and a slack that covers their rounding (see
@@ -46,6 +46,19 @@ struct WideResult {
   int slot;    // reference slot index, -1 on a miss, -2 on stack overflow
   float by, bz;
   bool any;    // any-hit result
+};
+
+// A lane's traversal stack: entry i < S at LDS node[i * B + tid] / t[i * B + tid],
+// deeper entries at spill[(i - S) * spillStride] as {child word, distance bits}
+// (spillStride = number of threads sharing the spill buffer when it is laid out
+// [depth][thread], so a wave's spills at one depth are contiguous; 1 for a
+// per-thread slice).
+struct WideStack {
+  uint32_t* node;
+  float* t;
+  int tid;
+  uint2* spill;
+  uint32_t spillStride;
 };
 
 struct WideState {
@@ -88,8 +101,7 @@ __device__ __forceinline__ void wide_init(WideState& st, gg::V3 o, gg::V3 d, flo
 // Box pass for a lane with st.node != kWideNoNode.  Returns false if the stack
 // would overflow (impossible when the host sized the spill from the tree).
 template <bool ANY, int B, int S>
-__device__ __forceinline__ bool wide_visit(const WideScene& s, WideState& st, uint32_t* stkNode, float* stkT,
-                                           int tid, uint2* spill) {
+__device__ __forceinline__ bool wide_visit(const WideScene& s, WideState& st, const WideStack& stk) {
   const gg::V3 o = st.o;
   const uint32_t nodeIndex = st.node;
   const uint4 q0 = s.nodes[4 * (size_t)nodeIndex + 0];
@@ -167,18 +179,18 @@ __device__ __forceinline__ bool wide_visit(const WideScene& s, WideState& st, ui
 #undef GPUGI_CLAIM
   bool overflow = false;
   int sp = st.sp;
-#define GPUGI_PUSH(nd, tv)                                   \
-  do {                                                       \
-    if (sp < S) {                                            \
-      stkNode[sp * B + tid] = (nd);                          \
-      stkT[sp * B + tid] = (tv);                             \
-      ++sp;                                                  \
-    } else if ((uint32_t)(sp - S) < s.spillDepth) {          \
-      spill[sp - S] = make_uint2((nd), __float_as_uint(tv)); \
-      ++sp;                                                  \
-    } else {                                                 \
-      overflow = true;                                       \
-    }                                                        \
+#define GPUGI_PUSH(nd, tv)                                                                \
+  do {                                                                                    \
+    if (sp < S) {                                                                         \
+      stk.node[sp * B + stk.tid] = (nd);                                                  \
+      stk.t[sp * B + stk.tid] = (tv);                                                     \
+      ++sp;                                                                               \
+    } else if ((uint32_t)(sp - S) < s.spillDepth) {                                       \
+      stk.spill[(size_t)(sp - S) * stk.spillStride] = make_uint2((nd), __float_as_uint(tv));\
+      ++sp;                                                                               \
+    } else {                                                                              \
+      overflow = true;                                                                    \
+    }                                                                                     \
   } while (0)
   if (p3) GPUGI_PUSH(c3, t3);
   if (p2) GPUGI_PUSH(c2, t2);
@@ -247,8 +259,7 @@ __device__ __forceinline__ void wide_leaf(const WideScene& s, WideState& st) {
 // hit until one yields a node, or a leaf meets a taken pending slot (the entry
 // stays), or the stack is empty.
 template <int B, int S>
-__device__ __forceinline__ void wide_pop(WideState& st, const uint32_t* stkNode, const float* stkT, int tid,
-                                         const uint2* spill) {
+__device__ __forceinline__ void wide_pop(WideState& st, const WideStack& stk) {
   const float tb = st.tbest * 1.00000095f;  // entry distances are lower bounds with this slack
   int sp = st.sp;
   uint32_t leaf = st.leaf;
@@ -258,10 +269,10 @@ __device__ __forceinline__ void wide_pop(WideState& st, const uint32_t* stkNode,
     uint32_t nd;
     float t;
     if (top < S) {
-      nd = stkNode[top * B + tid];
-      t = stkT[top * B + tid];
+      nd = stk.node[top * B + stk.tid];
+      t = stk.t[top * B + stk.tid];
     } else {
-      const uint2 e = spill[top - S];
+      const uint2 e = stk.spill[(size_t)(top - S) * stk.spillStride];
       nd = e.x;
       t = __uint_as_float(e.y);
     }
@@ -295,28 +306,29 @@ __device__ __forceinline__ bool wide_choose_leaf_pass(bool working, const WideSt
 
 // One pass for a working lane.  Returns true when the ray is finished.
 template <bool ANY, int B, int S, bool WT>
-__device__ __forceinline__ bool wide_pass(const WideScene& s, WideState& st, bool leafPass, uint32_t* stkNode,
-                                          float* stkT, int tid, uint2* spill) {
+__device__ __forceinline__ bool wide_pass(const WideScene& s, WideState& st, bool leafPass, const WideStack& stk) {
   if (leafPass) {
     if (st.leaf == 0u) return false;  // still traversing; unaffected by this pass
     wide_leaf<ANY, WT>(s, st);
     if (ANY && st.found) return true;
   } else {
     if (st.node == kWideNoNode) return false;  // waiting for a leaf pass
-    if (!wide_visit<ANY, B, S>(s, st, stkNode, stkT, tid, spill)) return true;  // overflow, slot -2
+    if (!wide_visit<ANY, B, S>(s, st, stk)) return true;  // overflow, slot -2
   }
   if (st.node == kWideNoNode) {
-    wide_pop<B, S>(st, stkNode, stkT, tid, spill);
+    wide_pop<B, S>(st, stk);
     return st.node == kWideNoNode && st.leaf == 0u;
   }
   return false;
 }
 
 // Traces one ray per calling lane to the end; the calling lanes form the wave
-// for pass choices.  stkNode/stkT: LDS arrays of S * B entries; tid in [0, B).
+// for pass choices.  stkNode/stkT: LDS arrays of S * B entries; tid in [0, B);
+// spill: this thread's own slice of s.spillDepth entries.
 template <bool ANY, int B, int S, bool WT = false>
 __device__ __forceinline__ WideResult trace_wide_ray(const WideScene& s, gg::V3 o, gg::V3 d, float tmax,
                                                      uint32_t* stkNode, float* stkT, int tid, uint2* spill) {
+  const WideStack stk{stkNode, stkT, tid, spill, 1u};
   WideState st;
   wide_init<WT>(st, o, d, tmax);
   bool live = true;
@@ -324,7 +336,7 @@ __device__ __forceinline__ WideResult trace_wide_ray(const WideScene& s, gg::V3 
     const uint64_t act = __ballot(live);
     if (act == 0) break;
     const bool leafPass = wide_choose_leaf_pass(live, st, (__popcll(act) + 1) / 2);
-    if (live && wide_pass<ANY, B, S, WT>(s, st, leafPass, stkNode, stkT, tid, spill)) live = false;
+    if (live && wide_pass<ANY, B, S, WT>(s, st, leafPass, stk)) live = false;
   }
   WideResult r;
   r.t = st.tbest;

@@ -33,7 +33,8 @@ __global__ __launch_bounds__(B) void k_trace_wide(WideScene s, const float4* __r
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const uint32_t gtid = blockIdx.x * B + tid;
-  uint2* spill = s.spill + (size_t)gtid * s.spillDepth;
+  // spill laid out [depth][thread] over the whole grid (s.numThreads == gridDim.x * B)
+  const WideStack stk{stkNode, stkT, tid, s.spill + gtid, gridDim.x * (uint32_t)B};
 
   const uint32_t wavesPerBlock = B / 64;
   const uint32_t xcd = blockIdx.x % kNumXcd;
@@ -73,7 +74,7 @@ __global__ __launch_bounds__(B) void k_trace_wide(WideScene s, const float4* __r
       continue;
     }
     const bool leafPass = wide_choose_leaf_pass(active, st, wait);
-    if (active && wide_pass<ANY, B, S, WT>(s, st, leafPass, stkNode, stkT, tid, spill)) {
+    if (active && wide_pass<ANY, B, S, WT>(s, st, leafPass, stk)) {
       if (ANY) hits[ri] = make_int4(st.found ? 1 : 0, 0, 0, 0);
       else hits[ri] = make_int4(st.bestSlot, __float_as_int(st.tbest), __float_as_int(st.bu), __float_as_int(st.bv));
       active = false;
@@ -96,8 +97,8 @@ Variant MakeVariant() {
 }
 
 const Variant* FindVariant(int stack) {
-  static const Variant table[] = {MakeVariant<4>(), MakeVariant<8>(), MakeVariant<12>(), MakeVariant<16>(),
-                                  MakeVariant<24>()};
+  static const Variant table[] = {MakeVariant<4>(),  MakeVariant<8>(),  MakeVariant<10>(),
+                                  MakeVariant<12>(), MakeVariant<16>(), MakeVariant<24>()};
   for (const Variant& v : table)
     if (v.stack == stack) return &v;
   return nullptr;
@@ -124,7 +125,7 @@ void LaunchS(const WideScene& s, int query, const gpugi_ray* rays, gpugi_hit* hi
 
 }  // namespace
 
-const int kWideStackVariants[kNumWideStackVariants] = {4, 8, 12, 16, 24};
+const int kWideStackVariants[kNumWideStackVariants] = {4, 8, 10, 12, 16, 24};
 
 hipError_t WideBlocksPerCU(int stack, int* blocks) {
   const Variant* v = FindVariant(stack);
@@ -148,6 +149,7 @@ hipError_t LaunchTraceWide(const WideScene& s, int stack, int query, const gpugi
   switch (stack) {
     case 4: LaunchS<4>(s, query, rays, hits, n, refill, wait, watertight, stream); break;
     case 8: LaunchS<8>(s, query, rays, hits, n, refill, wait, watertight, stream); break;
+    case 10: LaunchS<10>(s, query, rays, hits, n, refill, wait, watertight, stream); break;
     case 12: LaunchS<12>(s, query, rays, hits, n, refill, wait, watertight, stream); break;
     case 16: LaunchS<16>(s, query, rays, hits, n, refill, wait, watertight, stream); break;
     case 24: LaunchS<24>(s, query, rays, hits, n, refill, wait, watertight, stream); break;

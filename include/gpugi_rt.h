@@ -215,10 +215,10 @@ int gpugi_trace(gpugi_ctx* ctx, int query, int kernel, const gpugi_ray* rays, gp
                 size_t n, int rays_on_device);
 /* Configuration of the wide traversal: triangles per merged leaf (1..16; the
  * BVH is re-packed when it changes), LDS stack entries per lane of the trace
- * kernel (4, 8, 12, 16 or 24; deeper entries spill to global memory), the
+ * kernel (4, 8, 10, 12, 16 or 24; deeper entries spill to global memory), the
  * number of idle lanes (1..64) at which a wave refills them with new rays and
  * the number of lanes (1..64) waiting with a pending leaf at which a wave runs
- * a triangle pass instead of a box pass.  Default (2, 8, 8, 16): the fastest
+ * a triangle pass instead of a box pass.  Default (2, 10, 8, 16): the fastest
  * incoherent closest-hit configuration on the 2,097,152-triangle heightfield.  No
  * reference counterpart: the GLSL path has one fixed layout
  * (TRIANGLES_PER_LEAF, scenedata.glsl:5). */

@@ -190,4 +190,4 @@ def test_wide_configurations(rt, oracle, ctx, leaf, stack, refill, wait):
     want_any = oracle.trace(sc, shadow, any_hit=True)[:, 0]
     got_any = hits_int(ctx.trace(rays_struct(rt, shadow), rt.ANY_HIT, rt.KERNEL_WIDE))[:, 0]
     assert (got_any != want_any).sum() <= max(2, 1e-3 * len(shadow))
-    ctx.set_wide_config(2, 8, 8, 16)
+    ctx.set_wide_config(2, 10, 8, 16)
