@@ -60,14 +60,14 @@ __global__ void k_fill_f4(float4* p, size_t n) {
 }
 
 template <bool WIDE>
-__device__ __forceinline__ void render_tile(const BatchSlot& sl, uint32_t local, uint32_t seed, uint32_t* stkNode,
-                                            float* stkT, uint2* spill, BatchCounters* c) {
+__device__ __forceinline__ void render_tile(const BatchSlot& sl, uint32_t local, uint32_t seed, uint2* stk,
+                                            uint2* spill, BatchCounters* c) {
   DrawParams p = sl.p;
   p.frameSeed = seed;
   const int tid = threadIdx.x;
   const int gx = (int)(local % sl.tilesX) * kTile + (tid % kTile);
   const int gy = (int)(local / sl.tilesX) * kTile + (tid / kTile);
-  TraceCtx ctx{&p, stkNode, stkT, tid, spill};
+  TraceCtx ctx{&p, stk, tid, spill};
   pt_pixel<WIDE>(p, ctx, gx, gy, &c->rays);
 }
 
@@ -76,8 +76,7 @@ __global__ __launch_bounds__(kTileThreads) void k_batch_persistent(const BatchSl
                                                                    uint32_t numScenes, uint32_t totalTiles,
                                                                    BatchCounters* c, uint2* spillBase,
                                                                    uint32_t spillDepth, uint32_t grab) {
-  __shared__ uint32_t stkNode[WIDE ? kWideLdsStack * kTileThreads : 1];
-  __shared__ float stkT[WIDE ? kWideLdsStack * kTileThreads : 1];
+  __shared__ uint2 stk[WIDE ? kWideLdsStack * kTileThreads : 1];
   __shared__ uint32_t sBase;
   const int tid = threadIdx.x;
   uint2* spill = WIDE ? spillBase + ((size_t)blockIdx.x * kTileThreads + tid) * spillDepth : nullptr;
@@ -92,7 +91,7 @@ __global__ __launch_bounds__(kTileThreads) void k_batch_persistent(const BatchSl
     for (uint32_t t = base; t < end; ++t) {
       uint32_t s = 0;
       while (s + 1 < numScenes && slots[s + 1].tileBase <= t) ++s;
-      render_tile<WIDE>(slots[s], t - slots[s].tileBase, seed, stkNode, stkT, spill, c);
+      render_tile<WIDE>(slots[s], t - slots[s].tileBase, seed, stk, spill, c);
     }
   }
 }
@@ -101,10 +100,9 @@ __global__ __launch_bounds__(kTileThreads) void k_batch_persistent(const BatchSl
 template <bool WIDE>
 __global__ __launch_bounds__(kTileThreads) void k_batch_scene(const BatchSlot* __restrict__ slot, BatchCounters* c,
                                                               uint2* spillBase, uint32_t spillDepth) {
-  __shared__ uint32_t stkNode[WIDE ? kWideLdsStack * kTileThreads : 1];
-  __shared__ float stkT[WIDE ? kWideLdsStack * kTileThreads : 1];
+  __shared__ uint2 stk[WIDE ? kWideLdsStack * kTileThreads : 1];
   uint2* spill = WIDE ? spillBase + ((size_t)blockIdx.x * kTileThreads + threadIdx.x) * spillDepth : nullptr;
-  render_tile<WIDE>(*slot, blockIdx.x, c->seed, stkNode, stkT, spill, c);
+  render_tile<WIDE>(*slot, blockIdx.x, c->seed, stk, spill, c);
 }
 
 }  // namespace

@@ -48,13 +48,12 @@ __device__ __forceinline__ void zero_features(float* dst) {
 
 template <bool WIDE>
 __global__ __launch_bounds__(kTileThreads) void k_cached_paths(DrawParams p, CacheParams cp) {
-  __shared__ uint32_t stkNode[WIDE ? kWideLdsStack * kTileThreads : 1];
-  __shared__ float stkT[WIDE ? kWideLdsStack * kTileThreads : 1];
+  __shared__ uint2 stk[WIDE ? kWideLdsStack * kTileThreads : 1];
   const int tid = threadIdx.x;
   const int gx = blockIdx.x * kTile + (tid % kTile);
   const int gy = blockIdx.y * kTile + (tid / kTile);
   const uint32_t linear = (blockIdx.y * gridDim.x + blockIdx.x) * kTileThreads + tid;
-  TraceCtx c{&p, stkNode, stkT, tid, WIDE ? p.wide.spill + (size_t)linear * p.wide.spillDepth : nullptr};
+  TraceCtx c{&p, stk, tid, WIDE ? p.wide.spill + (size_t)linear * p.wide.spillDepth : nullptr};
   const uint32_t pix = (uint32_t)gy * (uint32_t)p.width + (uint32_t)gx;
   const bool isTrain = pix % cp.trainStride == cp.trainOffset;
   const uint32_t trainIdx = pix / cp.trainStride;

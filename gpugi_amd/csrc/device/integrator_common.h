@@ -15,8 +15,7 @@ constexpr int kTileThreads = kTile * kTile;
 // LDS stack (S x 64 entries per block) and a global spill slice.
 struct TraceCtx {
   const DrawParams* p;
-  uint32_t* stkNode;
-  float* stkT;
+  uint2* stk;
   int tid;
   uint2* spill;
   uint32_t* count = nullptr;  // when set, incremented per ray query
@@ -40,9 +39,9 @@ __device__ __forceinline__ ClosestHit trace_closest(const TraceCtx& c, gg::V3 o,
   if (WIDE) {
     WideResult r = c.p->watertight
                        ? trace_wide_ray<false, kTileThreads, kWideLdsStack, true>(c.p->wide, o, d, gg::kRayMax,
-                                                                                  c.stkNode, c.stkT, c.tid, c.spill)
+                                                                                  c.stk, c.tid, c.spill)
                        : trace_wide_ray<false, kTileThreads, kWideLdsStack, false>(c.p->wide, o, d, gg::kRayMax,
-                                                                                   c.stkNode, c.stkT, c.tid, c.spill);
+                                                                                   c.stk, c.tid, c.spill);
     h.hit = r.slot >= 0;
     h.t = r.t;
     if (h.hit) {
@@ -80,10 +79,10 @@ __device__ __forceinline__ bool trace_any(const TraceCtx& c, gg::V3 o, gg::V3 d,
   if (c.count) ++*c.count;
   if (WIDE) {
     WideResult r = c.p->watertight
-                       ? trace_wide_ray<true, kTileThreads, kWideLdsStack, true>(c.p->wide, o, d, len, c.stkNode,
-                                                                                 c.stkT, c.tid, c.spill)
-                       : trace_wide_ray<true, kTileThreads, kWideLdsStack, false>(c.p->wide, o, d, len, c.stkNode,
-                                                                                  c.stkT, c.tid, c.spill);
+                       ? trace_wide_ray<true, kTileThreads, kWideLdsStack, true>(c.p->wide, o, d, len, c.stk,
+                                                                                 c.tid, c.spill)
+                       : trace_wide_ray<true, kTileThreads, kWideLdsStack, false>(c.p->wide, o, d, len, c.stk,
+                                                                                  c.tid, c.spill);
     return r.any;
   }
   float rayLength = len;

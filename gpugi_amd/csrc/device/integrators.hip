@@ -13,13 +13,12 @@ namespace gpugi {
 // pathtracer.comp:16-110 for one 8x8 tile per block (the per-pixel body is pt_pixel).
 template <bool WIDE>
 __global__ __launch_bounds__(kTileThreads) void k_pathtracer(DrawParams p) {
-  __shared__ uint32_t stkNode[WIDE ? kWideLdsStack * kTileThreads : 1];
-  __shared__ float stkT[WIDE ? kWideLdsStack * kTileThreads : 1];
+  __shared__ uint2 stk[WIDE ? kWideLdsStack * kTileThreads : 1];
   const int tid = threadIdx.x;
   const int gx = blockIdx.x * kTile + (tid % kTile);
   const int gy = blockIdx.y * kTile + (tid / kTile);
   const uint32_t linear = (blockIdx.y * gridDim.x + blockIdx.x) * kTileThreads + tid;
-  TraceCtx c{&p, stkNode, stkT, tid, WIDE ? p.wide.spill + (size_t)linear * p.wide.spillDepth : nullptr};
+  TraceCtx c{&p, stk, tid, WIDE ? p.wide.spill + (size_t)linear * p.wide.spillDepth : nullptr};
   pt_pixel<WIDE>(p, c, gx, gy, nullptr);
 }
 

@@ -22,6 +22,7 @@
 namespace gpugi {
 
 constexpr int kLightBlock = 64;  // local size of lighttracer.comp:48
+static_assert(kLightBlock == kTileThreads, "the wide traversal stack is laid out for kTileThreads lanes");
 
 __device__ __forceinline__ float mis_heuristic(float p) { return fminf(1e18f, p); }     // lightcache.glsl:61-73
 __device__ __forceinline__ float distance_mis(float d, float c) { return mis_heuristic(d * d); }  // :84-89
@@ -102,11 +103,10 @@ __global__ __launch_bounds__(kLightBlock) void k_lighttracer(DrawParams p, int r
                                                              LightCacheEntryDev* __restrict__ cache,
                                                              LightCacheEntryDev* __restrict__ staging,
                                                              uint32_t* __restrict__ pathCounts) {
-  __shared__ uint32_t stkNode[WIDE ? kWideLdsStack * kLightBlock : 1];
-  __shared__ float stkT[WIDE ? kWideLdsStack * kLightBlock : 1];
+  __shared__ uint2 stk[WIDE ? kWideLdsStack * kLightBlock : 1];
   const int tid = threadIdx.x;
   const uint32_t gid = blockIdx.x * kLightBlock + tid;
-  TraceCtx c{&p, stkNode, stkT, tid, WIDE ? p.wide.spill + (size_t)gid * p.wide.spillDepth : nullptr};
+  TraceCtx c{&p, stk, tid, WIDE ? p.wide.spill + (size_t)gid * p.wide.spillDepth : nullptr};
   uint32_t seed = gg::init_random_seed(p.frameSeed, gid);
   const int lsIndex = (int)(gid / (uint32_t)raysPerSample);
   const gpugi_light_sample& ls = p.lights[lsIndex];
@@ -172,12 +172,11 @@ __global__ __launch_bounds__(kLightBlock) void k_lighttracer(DrawParams p, int r
 // showlightcaches.comp:9-65 for exactly N light samples.
 template <bool WIDE>
 __global__ __launch_bounds__(kLightBlock) void k_showlightcaches(DrawParams p) {
-  __shared__ uint32_t stkNode[WIDE ? kWideLdsStack * kLightBlock : 1];
-  __shared__ float stkT[WIDE ? kWideLdsStack * kLightBlock : 1];
+  __shared__ uint2 stk[WIDE ? kWideLdsStack * kLightBlock : 1];
   const int tid = threadIdx.x;
   const uint32_t i = blockIdx.x * kLightBlock + tid;
   if (i >= p.numLights) return;
-  TraceCtx c{&p, stkNode, stkT, tid, WIDE ? p.wide.spill + (size_t)i * p.wide.spillDepth : nullptr};
+  TraceCtx c{&p, stk, tid, WIDE ? p.wide.spill + (size_t)i * p.wide.spillDepth : nullptr};
   const gpugi_light_sample& ls = p.lights[i];
   const gg::V3 lpos = gg::v3(ls.position[0], ls.position[1], ls.position[2]);
   const gg::V3 U = gg::v3(p.camera.u[0], p.camera.u[1], p.camera.u[2]);
@@ -236,13 +235,12 @@ __global__ __launch_bounds__(256) void k_bpt_scatter(const LightCacheEntryDev* _
 template <bool WIDE>
 __global__ __launch_bounds__(kTileThreads) void k_bpt_eye(DrawParams p, const LightCacheEntryDev* __restrict__ cache,
                                                           const uint32_t* __restrict__ numEntriesPtr) {
-  __shared__ uint32_t stkNode[WIDE ? kWideLdsStack * kTileThreads : 1];
-  __shared__ float stkT[WIDE ? kWideLdsStack * kTileThreads : 1];
+  __shared__ uint2 stk[WIDE ? kWideLdsStack * kTileThreads : 1];
   const int tid = threadIdx.x;
   const int gx = blockIdx.x * kTile + (tid % kTile);
   const int gy = blockIdx.y * kTile + (tid / kTile);
   const uint32_t linear = (blockIdx.y * gridDim.x + blockIdx.x) * kTileThreads + tid;
-  TraceCtx c{&p, stkNode, stkT, tid, WIDE ? p.wide.spill + (size_t)linear * p.wide.spillDepth : nullptr};
+  TraceCtx c{&p, stk, tid, WIDE ? p.wide.spill + (size_t)linear * p.wide.spillDepth : nullptr};
   uint32_t seed = gg::init_random_seed(p.frameSeed, (uint32_t)(gx + gy * p.width));
   const float j0 = gg::random01(seed);
   const float j1 = gg::random01(seed);

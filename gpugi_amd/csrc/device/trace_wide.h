@@ -48,14 +48,13 @@ struct WideResult {
   bool any;    // any-hit result
 };
 
-// A lane's traversal stack: entry i < S at LDS node[i * B + tid] / t[i * B + tid],
-// deeper entries at spill[(i - S) * spillStride] as {child word, distance bits}
+// A lane's traversal stack of {child word, entry distance bits}: entry i < S at
+// LDS lds[i * B + tid], deeper entries at spill[(i - S) * spillStride]
 // (spillStride = number of threads sharing the spill buffer when it is laid out
 // [depth][thread], so a wave's spills at one depth are contiguous; 1 for a
 // per-thread slice).
 struct WideStack {
-  uint32_t* node;
-  float* t;
+  uint2* lds;
   int tid;
   uint2* spill;
   uint32_t spillStride;
@@ -147,10 +146,15 @@ __device__ __forceinline__ bool wide_visit(const WideScene& s, WideState& st, co
   float t0 = tn[0], t1 = tn[1], t2 = tn[2], t3 = tn[3];
   uint32_t c0 = q1.x, c1 = q1.y, c2 = q1.z, c3 = q1.w;
   if (!ANY) {  // nearest first; missed children (INFINITY) sink to the end
-#define GPUGI_CSWAP(ta, ca, tb_, cb)    \
-  if (tb_ < ta) {                       \
-    float tt = ta; ta = tb_; tb_ = tt;  \
-    uint32_t cc = ca; ca = cb; cb = cc; \
+#define GPUGI_CSWAP(ta, ca, tb_, cb)                       \
+  {                                                        \
+    const bool sw = tb_ < ta;                              \
+    const float lo = sw ? tb_ : ta, hi = sw ? ta : tb_;    \
+    const uint32_t cl = sw ? cb : ca, ch = sw ? ca : cb;   \
+    ta = lo;                                               \
+    tb_ = hi;                                              \
+    ca = cl;                                               \
+    cb = ch;                                               \
   }
     GPUGI_CSWAP(t0, c0, t1, c1);
     GPUGI_CSWAP(t2, c2, t3, c3);
@@ -160,43 +164,55 @@ __device__ __forceinline__ bool wide_visit(const WideScene& s, WideState& st, co
 #undef GPUGI_CSWAP
   }
   // The nearest hit inner child is visited next, the nearest hit leaf fills a
-  // free pending slot; p_k: child k goes on the stack.
-  uint32_t node = kWideNoNode, leaf = st.leaf;
-#define GPUGI_CLAIM(t, c, pk)                   \
-  bool pk = t != INFINITY;                      \
-  if (pk) {                                     \
-    if (c & kWideLeafFlag) {                    \
-      if (leaf == 0u) { leaf = c; pk = false; } \
-    } else if (node == kWideNoNode) {           \
-      node = c;                                 \
-      pk = false;                               \
-    }                                           \
-  }
-  GPUGI_CLAIM(t0, c0, p0)
-  GPUGI_CLAIM(t1, c1, p1)
-  GPUGI_CLAIM(t2, c2, p2)
-  GPUGI_CLAIM(t3, c3, p3)
-#undef GPUGI_CLAIM
+  // free pending slot, every other hit child is pushed (p_k).  The conditions
+  // are lane masks; each claim is a chain of selects from the farthest child in.
+  const bool h0 = t0 != INFINITY, h1 = t1 != INFINITY, h2 = t2 != INFINITY, h3 = t3 != INFINITY;
+  const bool f0 = (int32_t)c0 < 0, f1 = (int32_t)c1 < 0, f2 = (int32_t)c2 < 0, f3 = (int32_t)c3 < 0;
+  const bool in0 = h0 && !f0, in1 = h1 && !f1, in2 = h2 && !f2, in3 = h3 && !f3;
+  const bool freeSlot = st.leaf == 0u;
+  const bool lf0 = freeSlot && h0 && f0, lf1 = freeSlot && h1 && f1, lf2 = freeSlot && h2 && f2,
+             lf3 = freeSlot && h3 && f3;
+  uint32_t node = in3 ? c3 : kWideNoNode;
+  node = in2 ? c2 : node;
+  node = in1 ? c1 : node;
+  node = in0 ? c0 : node;
+  uint32_t leaf = lf3 ? c3 : st.leaf;
+  leaf = lf2 ? c2 : leaf;
+  leaf = lf1 ? c1 : leaf;
+  leaf = lf0 ? c0 : leaf;
+  const bool p0 = h0 && !in0 && !lf0;
+  const bool p1 = h1 && !(in1 && !in0) && !(lf1 && !lf0);
+  const bool p2 = h2 && !(in2 && !in0 && !in1) && !(lf2 && !lf0 && !lf1);
+  const bool p3 = h3 && !(in3 && !in0 && !in1 && !in2) && !(lf3 && !lf0 && !lf1 && !lf2);
   bool overflow = false;
   int sp = st.sp;
-#define GPUGI_PUSH(nd, tv)                                                                \
-  do {                                                                                    \
-    if (sp < S) {                                                                         \
-      stk.node[sp * B + stk.tid] = (nd);                                                  \
-      stk.t[sp * B + stk.tid] = (tv);                                                     \
-      ++sp;                                                                               \
-    } else if ((uint32_t)(sp - S) < s.spillDepth) {                                       \
-      stk.spill[(size_t)(sp - S) * stk.spillStride] = make_uint2((nd), __float_as_uint(tv));\
-      ++sp;                                                                               \
-    } else {                                                                              \
-      overflow = true;                                                                    \
-    }                                                                                     \
+  if (sp + 4 <= S) {  // every push stays in LDS: one store and one increment each
+    uint2* e = stk.lds + sp * B + stk.tid;
+    int pushed = 0;
+    if (p3) { e[pushed * B] = make_uint2(c3, __float_as_uint(t3)); ++pushed; }
+    if (p2) { e[pushed * B] = make_uint2(c2, __float_as_uint(t2)); ++pushed; }
+    if (p1) { e[pushed * B] = make_uint2(c1, __float_as_uint(t1)); ++pushed; }
+    if (p0) { e[pushed * B] = make_uint2(c0, __float_as_uint(t0)); ++pushed; }
+    sp += pushed;
+  } else {
+#define GPUGI_PUSH(nd, tv)                                                                  \
+  do {                                                                                      \
+    if (sp < S) {                                                                           \
+      stk.lds[sp * B + stk.tid] = make_uint2((nd), __float_as_uint(tv));                    \
+      ++sp;                                                                                 \
+    } else if ((uint32_t)(sp - S) < s.spillDepth) {                                         \
+      stk.spill[(size_t)(sp - S) * stk.spillStride] = make_uint2((nd), __float_as_uint(tv)); \
+      ++sp;                                                                                 \
+    } else {                                                                                \
+      overflow = true;                                                                      \
+    }                                                                                       \
   } while (0)
-  if (p3) GPUGI_PUSH(c3, t3);
-  if (p2) GPUGI_PUSH(c2, t2);
-  if (p1) GPUGI_PUSH(c1, t1);
-  if (p0) GPUGI_PUSH(c0, t0);
+    if (p3) GPUGI_PUSH(c3, t3);
+    if (p2) GPUGI_PUSH(c2, t2);
+    if (p1) GPUGI_PUSH(c1, t1);
+    if (p0) GPUGI_PUSH(c0, t0);
 #undef GPUGI_PUSH
+  }
   st.node = node;
   st.leaf = leaf;
   st.sp = sp;
@@ -269,8 +285,9 @@ __device__ __forceinline__ void wide_pop(WideState& st, const WideStack& stk) {
     uint32_t nd;
     float t;
     if (top < S) {
-      nd = stk.node[top * B + stk.tid];
-      t = stk.t[top * B + stk.tid];
+      const uint2 e = stk.lds[top * B + stk.tid];
+      nd = e.x;
+      t = __uint_as_float(e.y);
     } else {
       const uint2 e = stk.spill[(size_t)(top - S) * stk.spillStride];
       nd = e.x;
@@ -295,13 +312,12 @@ __device__ __forceinline__ void wide_pop(WideState& st, const WideStack& stk) {
   st.node = node;
 }
 
-// Wave-uniform pass choice from ballots over the working lanes: a leaf pass
-// when no lane has a node to visit or at least `waitLanes` lanes have only a
-// pending leaf.
-__device__ __forceinline__ bool wide_choose_leaf_pass(bool working, const WideState& st, int waitLanes) {
-  const uint64_t box = __ballot(working && st.node != kWideNoNode);
-  const uint64_t wait = __ballot(working && st.node == kWideNoNode);
-  return box == 0 || __popcll(wait) >= waitLanes;
+// Wave-uniform pass choice for the working lanes (mask `working`, a ballot
+// taken by the caller): a leaf pass when no working lane has a node to visit
+// or at least `waitLanes` of them have only a pending leaf.
+__device__ __forceinline__ bool wide_choose_leaf_pass(uint64_t working, const WideState& st, int waitLanes) {
+  const uint64_t hasNode = __ballot(st.node != kWideNoNode);
+  return (working & hasNode) == 0 || __popcll(working & ~hasNode) >= waitLanes;
 }
 
 // One pass for a working lane.  Returns true when the ray is finished.
@@ -323,19 +339,19 @@ __device__ __forceinline__ bool wide_pass(const WideScene& s, WideState& st, boo
 }
 
 // Traces one ray per calling lane to the end; the calling lanes form the wave
-// for pass choices.  stkNode/stkT: LDS arrays of S * B entries; tid in [0, B);
+// for pass choices.  lds: LDS array of S * B stack entries; tid in [0, B);
 // spill: this thread's own slice of s.spillDepth entries.
 template <bool ANY, int B, int S, bool WT = false>
 __device__ __forceinline__ WideResult trace_wide_ray(const WideScene& s, gg::V3 o, gg::V3 d, float tmax,
-                                                     uint32_t* stkNode, float* stkT, int tid, uint2* spill) {
-  const WideStack stk{stkNode, stkT, tid, spill, 1u};
+                                                     uint2* lds, int tid, uint2* spill) {
+  const WideStack stk{lds, tid, spill, 1u};
   WideState st;
   wide_init<WT>(st, o, d, tmax);
   bool live = true;
   for (;;) {
     const uint64_t act = __ballot(live);
     if (act == 0) break;
-    const bool leafPass = wide_choose_leaf_pass(live, st, (__popcll(act) + 1) / 2);
+    const bool leafPass = wide_choose_leaf_pass(act, st, (__popcll(act) + 1) / 2);
     if (live && wide_pass<ANY, B, S, WT>(s, st, leafPass, stk)) live = false;
   }
   WideResult r;

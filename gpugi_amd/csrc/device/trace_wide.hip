@@ -28,13 +28,12 @@ constexpr uint32_t kNumXcd = 8;
 template <bool ANY, int B, int S, bool WT>
 __global__ __launch_bounds__(B) void k_trace_wide(WideScene s, const float4* __restrict__ rays,
                                                   int4* __restrict__ hits, uint32_t n, int refill, int wait) {
-  __shared__ uint32_t stkNode[S * B];
-  __shared__ float stkT[S * B];
+  __shared__ uint2 lds[S * B];
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const uint32_t gtid = blockIdx.x * B + tid;
   // spill laid out [depth][thread] over the whole grid (s.numThreads == gridDim.x * B)
-  const WideStack stk{stkNode, stkT, tid, s.spill + gtid, gridDim.x * (uint32_t)B};
+  const WideStack stk{lds, tid, s.spill + gtid, gridDim.x * (uint32_t)B};
 
   const uint32_t wavesPerBlock = B / 64;
   const uint32_t xcd = blockIdx.x % kNumXcd;
@@ -52,7 +51,8 @@ __global__ __launch_bounds__(B) void k_trace_wide(WideScene s, const float4* __r
   uint32_t ri = 0;
   WideState st;
   for (;;) {
-    const uint64_t idle = __ballot(!active);
+    uint64_t act = __ballot(active);
+    const uint64_t idle = ~act;  // every lane of the wave runs this loop
     const int nidle = __popcll(idle);
     if (!exhausted && (nidle >= refill || nidle == 64)) {
       if (!active) {
@@ -68,12 +68,13 @@ __global__ __launch_bounds__(B) void k_trace_wide(WideScene s, const float4* __r
       }
       cursor += (uint32_t)nidle;
       if (bBegin + waveX + (cursor / 64u) * wavesX >= bEnd) exhausted = true;
+      act = __ballot(active);
     }
-    if (__ballot(active) == 0) {
+    if (act == 0) {
       if (exhausted) break;
       continue;
     }
-    const bool leafPass = wide_choose_leaf_pass(active, st, wait);
+    const bool leafPass = wide_choose_leaf_pass(act, st, wait);
     if (active && wide_pass<ANY, B, S, WT>(s, st, leafPass, stk)) {
       if (ANY) hits[ri] = make_int4(st.found ? 1 : 0, 0, 0, 0);
       else hits[ri] = make_int4(st.bestSlot, __float_as_int(st.tbest), __float_as_int(st.bu), __float_as_int(st.bv));
