@@ -2,22 +2,38 @@
 //
 // Each pixel traces the reference path tracer's estimator (pathtracer.comp:
 // same random-number order, NEE, emission only after a vertex without NEE,
-// Russian roulette) until the path's footprint has spread: with
+// Russian roulette) until it queries the cache, whose prediction, weighted by
+// the path throughput, replaces the rest of the path.  Two query rules:
+//
+// GPUGI_CACHE_SPREAD: once the path's footprint has spread; with
 //   a0 = sqrt(|x1 - x0|^2 / (4 pi |cos theta_1|))           at the primary hit,
 //   a  = sum_{i>=2} sqrt(|x_i - x_{i-1}|^2 / (pdf_i |cos theta_i|)),
 // the path stops at the first vertex i >= 2 (second hit on) where a > c * a0
 // (Mueller et al. 2021, "Real-time Neural Radiance Caching for Path
-// Tracing").  There the cache's prediction of the reflected radiance, weighted
-// by the path throughput, replaces the rest of the path.
+// Tracing"); the cache predicts the radiance that vertex reflects (its NEE
+// included).
+// GPUGI_CACHE_PRIMARY_INDIRECT: at the primary hit, after its emission and
+// NEE; the cache predicts the radiance the vertex reflects of light arriving
+// by its BSDF-sampled bounce (everything the path would collect from the next
+// vertex on), so direct light stays an exact estimate and the bounce noise
+// goes away.
 //
 // One pixel in every `trainStride` also traces a training path: the same path
-// continued `extraBounces` vertices past the termination, where the cache
-// stands in for the remainder (self-training).  Every vertex i of a training
-// path yields a sample whose target is the radiance reflected at x_i toward
-// x_{i-1}: (C + T_tail * L_tail - S_i) / T_i, with C the path's collected
-// radiance, S_i what was collected up to and including x_i's emission and T_i
-// the throughput at x_i.  Records live at fixed per-pixel / per-path slots, so
-// a frame is deterministic.
+// continued `extraBounces` vertices past the query vertex, where the cache
+// stands in for the remainder (self-training; with PRIMARY_INDIRECT the tail
+// vertex does its NEE first).  Every vertex i of a training path yields a
+// sample whose target is what the cache predicts there:
+// (C + T_tail * L_tail - S_i) / T_i, with C the path's collected radiance, T_i
+// the throughput at x_i and S_i what was collected before the predicted part
+// (up to x_i's emission for SPREAD, up to x_i's NEE for PRIMARY_INDIRECT).
+// Records live at fixed per-pixel / per-path slots, so a frame is
+// deterministic.
+//
+// With PRIMARY_INDIRECT every pixel's path is only the primary hit, so it
+// runs as wavefront passes over all pixels (ray generation, the persistent
+// closest-hit trace, shading with the NEE setup, the persistent shadow trace,
+// the sum), with the arithmetic of the one-kernel path; the training paths
+// alone run in k_cached_paths, one thread per path.
 #include <hip/hip_runtime.h>
 
 #include "integrator_common.h"
@@ -46,15 +62,28 @@ __device__ __forceinline__ void zero_features(float* dst) {
   for (int k = 0; k < 4; ++k) d4[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
+// SPREAD: an 8x8 tile of pixels per block.  PRIMARY_INDIRECT: a 1-D grid, one
+// thread per training path (the wavefront passes render the pixels).
 template <bool WIDE>
 __global__ __launch_bounds__(kTileThreads) void k_cached_paths(DrawParams p, CacheParams cp) {
   __shared__ uint2 stk[WIDE ? kWideLdsStack * kTileThreads : 1];
   const int tid = threadIdx.x;
-  const int gx = blockIdx.x * kTile + (tid % kTile);
-  const int gy = blockIdx.y * kTile + (tid / kTile);
-  const uint32_t linear = (blockIdx.y * gridDim.x + blockIdx.x) * kTileThreads + tid;
+  const bool atPrimary = cp.query == GPUGI_CACHE_PRIMARY_INDIRECT;
+  int gx, gy;
+  uint32_t linear, pix;
+  if (atPrimary) {
+    linear = blockIdx.x * kTileThreads + tid;
+    pix = linear * cp.trainStride + cp.trainOffset;
+    if (pix >= (uint32_t)p.width * (uint32_t)p.height) return;
+    gx = (int)(pix % (uint32_t)p.width);
+    gy = (int)(pix / (uint32_t)p.width);
+  } else {
+    gx = blockIdx.x * kTile + (tid % kTile);
+    gy = blockIdx.y * kTile + (tid / kTile);
+    linear = (blockIdx.y * gridDim.x + blockIdx.x) * kTileThreads + tid;
+    pix = (uint32_t)gy * (uint32_t)p.width + (uint32_t)gx;
+  }
   TraceCtx c{&p, stk, tid, WIDE ? p.wide.spill + (size_t)linear * p.wide.spillDepth : nullptr};
-  const uint32_t pix = (uint32_t)gy * (uint32_t)p.width + (uint32_t)gx;
   const bool isTrain = pix % cp.trainStride == cp.trainOffset;
   const uint32_t trainIdx = pix / cp.trainStride;
   float* tv = cp.trainVert + (size_t)trainIdx * p.maxPathLength * kTrainVertFloats;
@@ -84,7 +113,7 @@ __global__ __launch_bounds__(kTileThreads) void k_cached_paths(DrawParams p, Cac
     if (i == 0) a0 = sqrtf(dist2 / (4.0f * gg::kPI * cosIn));
     else spread += sqrtf(dist2 / (fmaxf(pdfPrev, 1e-8f) * cosIn));
     const gg::V3 te = throughput * (didNEE ? gg::v3s(0.0f) : m.emissivity);
-    if (!term && i >= 1 && spread > cp.spreadC * a0) {
+    if (!atPrimary && !term && i >= 1 && spread > cp.spreadC * a0) {
       color = color + te;
       write_features(cp.queryFeat + (size_t)pix * kMlpRaw, cp, o, -d, n, m);
       reinterpret_cast<float4*>(cp.queryWeight)[pix] = make_float4(throughput.x, throughput.y, throughput.z, 1.0f);
@@ -93,15 +122,21 @@ __global__ __launch_bounds__(kTileThreads) void k_cached_paths(DrawParams p, Cac
       if (!isTrain) break;
     }
     if (isTrain && term && i == termDepth + cp.extraBounces) {
-      // the cache stands in for what this vertex reflects
+      // the cache stands in for what this vertex reflects (of its bounce, after its NEE)
       cacc = cacc + te;
+      if (atPrimary) {
+        const bool nee = m.refl[3] < 500.0f;
+        const int sampleIndex = (int)(gg::xorshift32(seed) % p.numLights);
+        cacc = cacc + estimate_direct_light<WIDE>(c, o, n, sampleIndex, d, m, nee) * throughput * lightSampleWeight;
+      }
       write_features(cp.tailFeat + (size_t)trainIdx * kMlpRaw, cp, o, -d, n, m);
       tailT = throughput;
       tail = true;
       break;
     }
+    float* r = nullptr;
     if (isTrain) {
-      float* r = tv + (size_t)nTrain * kTrainVertFloats;
+      r = tv + (size_t)nTrain * kTrainVertFloats;
       write_features(r, cp, o, -d, n, m);
       const gg::V3 s = cacc + te;
       reinterpret_cast<float4*>(r)[4] = make_float4(throughput.x, throughput.y, throughput.z, 0.0f);
@@ -116,6 +151,13 @@ __global__ __launch_bounds__(kTileThreads) void k_cached_paths(DrawParams p, Cac
     const gg::V3 contrib = radiance * throughput * lightSampleWeight;
     cacc = cacc + contrib;
     if (!term) color = color + contrib;
+    if (atPrimary) {
+      if (r) reinterpret_cast<float4*>(r)[5] = make_float4(cacc.x, cacc.y, cacc.z, 0.0f);
+      if (i == 0) {  // the cache stands in for everything the bounce from x1 would add
+        term = true;
+        termDepth = 0;
+      }
+    }
     gg::V3 tp = gg::v3s(1.0f);
     float pdf;
     bool diffuse;
@@ -129,10 +171,12 @@ __global__ __launch_bounds__(kTileThreads) void k_cached_paths(DrawParams p, Cac
       throughput = throughput / q;
     }
   }
-  reinterpret_cast<float4*>(cp.pixelColor)[pix] = make_float4(color.x, color.y, color.z, 0.0f);
-  if (!term) {
-    reinterpret_cast<float4*>(cp.queryWeight)[pix] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    zero_features(cp.queryFeat + (size_t)pix * kMlpRaw);
+  if (!atPrimary) {  // with PRIMARY_INDIRECT the wavefront passes write the pixel and its query
+    reinterpret_cast<float4*>(cp.pixelColor)[pix] = make_float4(color.x, color.y, color.z, 0.0f);
+    if (!term) {
+      reinterpret_cast<float4*>(cp.queryWeight)[pix] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      zero_features(cp.queryFeat + (size_t)pix * kMlpRaw);
+    }
   }
   if (isTrain) {
     cp.trainCount[trainIdx] = nTrain;
@@ -141,6 +185,97 @@ __global__ __launch_bounds__(kTileThreads) void k_cached_paths(DrawParams p, Cac
     tp4[1] = make_float4(tailT.x, tailT.y, tailT.z, tail ? 1.0f : 0.0f);
     if (!tail) zero_features(cp.tailFeat + (size_t)trainIdx * kMlpRaw);
   }
+}
+
+// ---- PRIMARY_INDIRECT wavefront passes: one thread per ray r, every buffer but
+// the per-pixel ones indexed by r.  Ray r belongs to pixel
+//   pix(r) = (r mod Q) * 8 + r / Q,  Q = W*H / 8,
+// so each eighth of the ray array, which the trace kernel hands to one XCD,
+// samples the whole image (every 8th pixel) and the XCDs get equal work
+// (pix(r) = r when W*H is not a multiple of 8).
+__device__ __forceinline__ uint32_t wave_pixel(uint32_t r, uint32_t P) {
+  if (P & 7u) return r;
+  const uint32_t Q = P / 8u;
+  return (r % Q) * 8u + r / Q;
+}
+
+// Primary rays {origin, tmax}{dir, 0} and each pixel's random state after them.
+__global__ void k_pi_raygen(DrawParams p, float4* __restrict__ rays, uint32_t* __restrict__ seeds) {
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t P = (uint32_t)p.width * (uint32_t)p.height;
+  if (r >= P) return;
+  const uint32_t pix = wave_pixel(r, P);
+  const int gx = (int)(pix % (uint32_t)p.width), gy = (int)(pix / (uint32_t)p.width);
+  uint32_t seed = gg::init_random_seed(p.frameSeed, pix);
+  const float j0 = gg::random01(seed);
+  const float j1 = gg::random01(seed);
+  const gg::V3 U = gg::v3(p.camera.u[0], p.camera.u[1], p.camera.u[2]);
+  const gg::V3 V = gg::v3(p.camera.v[0], p.camera.v[1], p.camera.v[2]);
+  const gg::V3 W = gg::v3(p.camera.w[0], p.camera.w[1], p.camera.w[2]);
+  const gg::V3 d = gg::primary_direction(j0, j1, gx, gy, p.width, p.height, U, V, W);
+  rays[2 * (size_t)r] = make_float4(p.camera.position[0], p.camera.position[1], p.camera.position[2], gg::kRayMax);
+  rays[2 * (size_t)r + 1] = make_float4(d.x, d.y, d.z, 0.0f);
+  seeds[r] = seed;
+}
+
+// Primary hit: emission into pixelColor, the cache query, the NEE shadow ray
+// (in place of the primary ray; tmax 0 when there is nothing to connect) and
+// its unoccluded contribution.
+__global__ void k_pi_shade(DrawParams p, CacheParams cp, float4* __restrict__ rays, const int4* __restrict__ hits,
+                           const uint32_t* __restrict__ seeds, float4* __restrict__ nee) {
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t P = (uint32_t)p.width * (uint32_t)p.height;
+  if (r >= P) return;
+  const uint32_t pix = wave_pixel(r, P);
+  const float4 r0 = rays[2 * (size_t)r], r1 = rays[2 * (size_t)r + 1];
+  const int4 rec = hits[r];
+  const ClosestHit h = closest_from_record(p, rec.x, __int_as_float(rec.y), __int_as_float(rec.z),
+                                           __int_as_float(rec.w));
+  const float lightSampleWeight = (float)p.numLights / 1.0f;
+  const gg::V3 throughput = gg::v3s(1.0f);
+  gg::V3 color = gg::v3s(0.0f), val = gg::v3s(0.0f);
+  float test = 0.0f;
+  float4 so = make_float4(0.0f, 0.0f, 0.0f, 0.0f), sd = make_float4(0.0f, 0.0f, 1.0f, 0.0f);
+  if (h.hit) {
+    const gg::V3 d = gg::v3(r1.x, r1.y, r1.z);
+    const gg::V3 o = gg::v3(r0.x, r0.y, r0.z) + h.t * d;
+    const gg::V3 n = shading_normal(p.ref.vinfo, h.tri, h.bary);
+    const MaterialData m = load_material(p.materials, h.tri.w);
+    color = color + throughput * m.emissivity;  // the path's first vertex follows the camera, no NEE before it
+    const bool didNEE = m.refl[3] < 500.0f;
+    uint32_t seed = seeds[r];
+    const int sampleIndex = (int)(gg::xorshift32(seed) % p.numLights);
+    const NeeSample s = nee_sample(p, o, n, sampleIndex, d, m, didNEE);
+    if (s.test) {
+      so = make_float4(s.o.x, s.o.y, s.o.z, s.len);
+      sd = make_float4(s.dir.x, s.dir.y, s.dir.z, 0.0f);
+      val = s.value * throughput * lightSampleWeight;
+      test = 1.0f;
+    }
+    write_features(cp.queryFeat + (size_t)pix * kMlpRaw, cp, o, -d, n, m);
+    reinterpret_cast<float4*>(cp.queryWeight)[pix] = make_float4(throughput.x, throughput.y, throughput.z, 1.0f);
+  } else {
+    zero_features(cp.queryFeat + (size_t)pix * kMlpRaw);
+    reinterpret_cast<float4*>(cp.queryWeight)[pix] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  }
+  rays[2 * (size_t)r] = so;
+  rays[2 * (size_t)r + 1] = sd;
+  nee[r] = make_float4(val.x, val.y, val.z, test);
+  reinterpret_cast<float4*>(cp.pixelColor)[pix] = make_float4(color.x, color.y, color.z, 0.0f);
+}
+
+// pixelColor += the NEE contribution (zero when there was nothing to connect
+// or the shadow ray is occluded, as the one-kernel path adds it).
+__global__ void k_pi_finish(DrawParams p, CacheParams cp, const int4* __restrict__ occluded,
+                            const float4* __restrict__ nee) {
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t P = (uint32_t)p.width * (uint32_t)p.height;
+  if (r >= P) return;
+  float4 v = nee[r];
+  if (v.w == 0.0f || occluded[r].x != 0) v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  float4* c = reinterpret_cast<float4*>(cp.pixelColor) + wave_pixel(r, P);
+  const float4 col = *c;
+  *c = make_float4(col.x + v.x, col.y + v.y, col.z + v.z, col.w);
 }
 
 __global__ __launch_bounds__(kTileThreads) void k_cache_resolve(DrawParams p, CacheParams cp) {
@@ -230,6 +365,8 @@ hipError_t LaunchCachedPaths(const DrawParams& p, const CacheParams& cp, void* s
   dim3 grid(p.width / kTile, p.height / kTile);
   if (grid.x == 0 || grid.y == 0) return hipSuccess;
   if (cp.trainStride == 0 || cp.trainOffset >= cp.trainStride) return hipErrorInvalidValue;
+  if (cp.query == GPUGI_CACHE_PRIMARY_INDIRECT)  // one thread per training path
+    grid = dim3((CacheTrainPaths(p.width, p.height, cp.trainStride) + kTileThreads - 1) / kTileThreads);
   if (p.useWide) {
     DrawParams q = p;
     q.wide.spill = (uint2*)spill;
@@ -237,6 +374,30 @@ hipError_t LaunchCachedPaths(const DrawParams& p, const CacheParams& cp, void* s
   } else {
     hipLaunchKernelGGL(k_cached_paths<false>, grid, dim3(kTileThreads), 0, stream, p, cp);
   }
+  return hipGetLastError();
+}
+
+hipError_t LaunchPrimaryRaygen(const DrawParams& p, float4* rays, uint32_t* seeds, hipStream_t stream) {
+  const size_t P = (size_t)p.width * p.height;
+  if (P == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_pi_raygen, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, stream, p, rays, seeds);
+  return hipGetLastError();
+}
+
+hipError_t LaunchPrimaryShade(const DrawParams& p, const CacheParams& cp, float4* rays, const int4* hits,
+                              const uint32_t* seeds, float4* nee, hipStream_t stream) {
+  const size_t P = (size_t)p.width * p.height;
+  if (P == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_pi_shade, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, stream, p, cp, rays, hits, seeds,
+                     nee);
+  return hipGetLastError();
+}
+
+hipError_t LaunchPrimaryFinish(const DrawParams& p, const CacheParams& cp, const int4* occluded, const float4* nee,
+                               hipStream_t stream) {
+  const size_t P = (size_t)p.width * p.height;
+  if (P == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_pi_finish, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, stream, p, cp, occluded, nee);
   return hipGetLastError();
 }
 

@@ -21,6 +21,10 @@ struct gpugi_ctx {
   int numCUs = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // second stream (default priority) for work that overlaps the main stream's,
+  // forked and joined with the two events; created on first use
+  hipStream_t side = nullptr;
+  hipEvent_t evFork = nullptr, evJoin = nullptr;
   bool timerStarted = false;
   std::string error;
   // reference layout
@@ -37,6 +41,7 @@ struct gpugi_ctx {
   void* dWideTris = nullptr;
   void* dSpill = nullptr;
   void* dCounters = nullptr;  // [0] ray counter, [1] error flag, [2] hit count
+  void* dWideQueue = nullptr;  // the wide trace kernel's batch counters
   size_t spillBytes = 0;
   uint32_t wideMaxStack = 0;
   int wideLeaf = 2;     // triangles per merged wide leaf
@@ -72,7 +77,7 @@ struct gpugi_ctx {
   uint32_t cacheMaxSamples = 65536;
   int cacheTrainSteps = 4;
   float cacheLr = 1e-3f;
-  uint32_t cacheLastSamples = 0;
+  const uint32_t* dCacheCount = nullptr;  // training samples written by the last cached draw (device)
   float cacheLastLoss = 0.0f;
   void* dCache = nullptr;
   size_t cacheBytes = 0;
@@ -126,6 +131,7 @@ WideScene MakeWide(const gpugi_ctx* c, int stack) {
   s.spillDepth = c->wideMaxStack > (uint32_t)stack ? c->wideMaxStack - (uint32_t)stack : 0;
   s.numThreads = 0;
   s.errorFlag = (uint32_t*)c->dCounters + 1;
+  s.queue = (uint32_t*)c->dWideQueue;
   return s;
 }
 
@@ -222,7 +228,8 @@ int gpugi_create(int device, gpugi_ctx** out) {
   ctx->numCUs = prop.multiProcessorCount;
   if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess ||
-      hipMalloc(&ctx->dCounters, 64) != hipSuccess || hipMemset(ctx->dCounters, 0, 64) != hipSuccess) {
+      hipMalloc(&ctx->dCounters, 64) != hipSuccess || hipMemset(ctx->dCounters, 0, 64) != hipSuccess ||
+      hipMalloc(&ctx->dWideQueue, kWideQueueWords * sizeof(uint32_t)) != hipSuccess) {
     gpugi_destroy(ctx);
     return Fail(nullptr, GPUGI_ERR_HIP, "gpugi_create: HIP stream/event/counter setup failed");
   }
@@ -233,9 +240,11 @@ int gpugi_create(int device, gpugi_ctx** out) {
 void gpugi_destroy(gpugi_ctx* ctx) {
   if (!ctx) return;
   hipSetDevice(ctx->device);
+  if (ctx->side) hipStreamSynchronize(ctx->side);
   if (ctx->stream) hipStreamSynchronize(ctx->stream);
   for (void** p : {&ctx->dNodes, &ctx->dSlots, &ctx->dPos, &ctx->dVinfo, &ctx->dMats, &ctx->dWideNodes,
-                   &ctx->dWideTris, &ctx->dSpill, &ctx->dCounters, &ctx->dAccum, &ctx->dLights, &ctx->dScratch})
+                   &ctx->dWideTris, &ctx->dSpill, &ctx->dCounters, &ctx->dWideQueue, &ctx->dAccum, &ctx->dLights,
+                   &ctx->dScratch})
     FreeDev(*p);
   FreeIntegratorState(&ctx->integ);
   for (void** p : {(void**)&ctx->mlp.w, (void**)&ctx->mlp.m, (void**)&ctx->mlp.v, (void**)&ctx->mlp.wb,
@@ -245,6 +254,9 @@ void gpugi_destroy(gpugi_ctx* ctx) {
   for (void* p : ctx->userAllocs) hipFree(p);
   if (ctx->ev0) hipEventDestroy(ctx->ev0);
   if (ctx->ev1) hipEventDestroy(ctx->ev1);
+  if (ctx->evFork) hipEventDestroy(ctx->evFork);
+  if (ctx->evJoin) hipEventDestroy(ctx->evJoin);
+  if (ctx->side) hipStreamDestroy(ctx->side);
   if (ctx->stream) hipStreamDestroy(ctx->stream);
   delete ctx;
 }
@@ -421,15 +433,18 @@ int DrawCached(gpugi_ctx* ctx, uint32_t frameSeed) {
   const uint32_t NT = CacheTrainPaths(ctx->width, ctx->height, cp.trainStride);
   const size_t L = (size_t)p.maxPathLength;
   const uint32_t S = ctx->cacheMaxSamples;
-  size_t off[14], total = 0;
-  const size_t sizes[14] = {P * 16, P * 64, P * 16, P * 16, (size_t)NT * L * 96, (size_t)NT * 4, (size_t)NT * 4,
+  const bool atPrimary = cp.query == GPUGI_CACHE_PRIMARY_INDIRECT;
+  const size_t PW = atPrimary ? P : 0;  // wavefront buffers: rays, hit records, random states, NEE
+  size_t off[18], total = 0;
+  const size_t sizes[18] = {P * 16, P * 64, P * 16, P * 16, (size_t)NT * L * 96, (size_t)NT * 4, (size_t)NT * 4,
                             (size_t)NT * 32, (size_t)NT * 64, (size_t)NT * 16, ExclusiveScanTempBytes(NT), 4,
-                            (size_t)S * 64, (size_t)S * 16};
-  for (int i = 0; i < 14; ++i) {
+                            (size_t)S * 64, (size_t)S * 16, PW * 32, PW * 16, PW * 4, PW * 16};
+  for (int i = 0; i < 18; ++i) {
     off[i] = total;
     total += Align256(sizes[i]);
   }
   if (ctx->cacheBytes < total) {
+    ctx->dCacheCount = nullptr;
     FreeDev(ctx->dCache);
     ctx->cacheBytes = 0;
     HIP_TRY(ctx, hipMalloc(&ctx->dCache, total));
@@ -452,10 +467,27 @@ int DrawCached(gpugi_ctx* ctx, uint32_t frameSeed) {
   uint32_t* dTotal = (uint32_t*)(b + off[11]);
   float* feat = (float*)(b + off[12]);
   float* targets = (float*)(b + off[13]);
+  float4* wRays = (float4*)(b + off[14]);
+  int4* wHits = (int4*)(b + off[15]);
+  uint32_t* wSeeds = (uint32_t*)(b + off[16]);
+  float4* wNee = (float4*)(b + off[17]);
   cp.trainOffset = frameSeed % cp.trainStride;
   if (int rc = SceneBounds(ctx, cp.bboxMin, cp.bboxInvExtent)) return rc;
 
-  HIP_TRY(ctx, LaunchFillU32(cp.trainCount, 0u, NT, ctx->stream));
+  // With PRIMARY_INDIRECT the training paths (few, latency-bound threads) run
+  // on the side stream beside the wavefront passes and join before the targets.
+  hipStream_t trainStream = ctx->stream;
+  if (atPrimary) {
+    if (!ctx->side) {
+      HIP_TRY(ctx, hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking));
+      HIP_TRY(ctx, hipEventCreateWithFlags(&ctx->evFork, hipEventDisableTiming));
+      HIP_TRY(ctx, hipEventCreateWithFlags(&ctx->evJoin, hipEventDisableTiming));
+    }
+    HIP_TRY(ctx, hipEventRecord(ctx->evFork, ctx->stream));
+    HIP_TRY(ctx, hipStreamWaitEvent(ctx->side, ctx->evFork, 0));
+    trainStream = ctx->side;
+  }
+  HIP_TRY(ctx, LaunchFillU32(cp.trainCount, 0u, NT, trainStream));
   void* spill = nullptr;
   if (p.useWide) {
     const size_t threads = (size_t)(ctx->width / 8) * (ctx->height / 8) * 64;
@@ -463,27 +495,46 @@ int DrawCached(gpugi_ctx* ctx, uint32_t frameSeed) {
                               std::max<size_t>(8, threads * p.wide.spillDepth * sizeof(uint2))));
     spill = ctx->integ.spill;
   }
-  HIP_TRY(ctx, LaunchCachedPaths(p, cp, spill, ctx->stream));
+  HIP_TRY(ctx, LaunchCachedPaths(p, cp, spill, trainStream));
+  if (atPrimary) {
+    HIP_TRY(ctx, hipEventRecord(ctx->evJoin, ctx->side));
+    // every pixel's primary hit and NEE as wavefront passes through the trace kernels
+    WideScene ws;
+    if (p.useWide)
+      if (int rc = PrepareTraceWide(ctx, ctx->wideStack, &ws)) return rc;
+    auto trace = [&](int query) -> int {
+      if (p.useWide)
+        HIP_TRY(ctx, LaunchTraceWide(ws, ctx->wideStack, query, (const gpugi_ray*)wRays, (gpugi_hit*)wHits, P,
+                                     ctx->wideRefill, ctx->wideWait, Watertight(ctx), ctx->stream));
+      else
+        HIP_TRY(ctx, LaunchTraceRef(MakeRef(ctx), query, (const gpugi_ray*)wRays, (gpugi_hit*)wHits, P,
+                                    Watertight(ctx), ctx->stream));
+      return GPUGI_OK;
+    };
+    HIP_TRY(ctx, LaunchPrimaryRaygen(p, wRays, wSeeds, ctx->stream));
+    if (int rc = trace(GPUGI_CLOSEST_HIT)) return rc;
+    HIP_TRY(ctx, LaunchPrimaryShade(p, cp, wRays, wHits, wSeeds, wNee, ctx->stream));
+    if (int rc = trace(GPUGI_ANY_HIT)) return rc;
+    HIP_TRY(ctx, LaunchPrimaryFinish(p, cp, wHits, wNee, ctx->stream));
+  }
   HIP_TRY(ctx, LaunchMlpInfer(ctx->mlp, cp.queryFeat, queryPred, P, ctx->stream));
-  HIP_TRY(ctx, LaunchMlpInfer(ctx->mlp, cp.tailFeat, tailPred, NT, ctx->stream));
   HIP_TRY(ctx, LaunchCacheResolve(p, cp, ctx->stream));
+  if (atPrimary) HIP_TRY(ctx, hipStreamWaitEvent(ctx->stream, ctx->evJoin, 0));
+  HIP_TRY(ctx, LaunchMlpInfer(ctx->mlp, cp.tailFeat, tailPred, NT, ctx->stream));
   HIP_TRY(ctx, LaunchExclusiveScanU32(cp.trainCount, offsets, NT, scanTemp, dTotal, ctx->stream));
   HIP_TRY(ctx, LaunchCacheTargets(p, cp, offsets, dTotal, S, (uint32_t)std::max(ctx->cacheTrainSteps, 1), feat,
                                   targets, ctx->stream));
-  uint32_t count = 0;
-  HIP_TRY(ctx, hipMemcpyAsync(&count, dTotal, 4, hipMemcpyDeviceToHost, ctx->stream));
-  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-  const uint32_t n = count < S ? count : S;
-  ctx->cacheLastSamples = n;
+  // The sample count stays on the device: each step's kernels read it and
+  // take their chunk of min(count, S) samples, so the host does not wait.
+  ctx->dCacheCount = dTotal;
   const int steps = ctx->cacheTrainSteps;
-  const uint32_t chunk = steps > 0 ? n / (uint32_t)steps : 0;
-  if (chunk >= 32) {
-    if (int rc = MlpEnsureTiles(ctx, MlpTiles(chunk))) return rc;
+  if (steps > 0) {
+    if (int rc = MlpEnsureTiles(ctx, MlpTiles(S / (uint32_t)steps))) return rc;
     for (int s = 0; s < steps; ++s) {
-      HIP_TRY(ctx, LaunchMlpGradient(ctx->mlp, feat + (size_t)s * chunk * kMlpRaw, targets + (size_t)s * chunk * 4,
-                                     chunk, ctx->dMlpLoss, ctx->stream));
+      const MlpBatch batch{dTotal, S, (uint32_t)s, (uint32_t)steps};
+      HIP_TRY(ctx, LaunchMlpGradient(ctx->mlp, feat, targets, batch, ctx->dMlpLoss, ctx->stream));
       AdamParams ap{ctx->cacheLr, 0.9f, 0.99f, 1e-8f, ++ctx->mlpStep};
-      HIP_TRY(ctx, LaunchMlpAdam(ctx->mlp, ap, ctx->stream));
+      HIP_TRY(ctx, LaunchMlpAdam(ctx->mlp, ap, batch, ctx->stream));
     }
   }
   return GPUGI_OK;
@@ -1024,7 +1075,7 @@ int MlpGradientCommon(gpugi_ctx* ctx, const char* fn, const float* features, con
     HIP_TRY(ctx, hipMemcpyAsync((void*)df, features, fb, hipMemcpyHostToDevice, ctx->stream));
     HIP_TRY(ctx, hipMemcpyAsync((void*)dt, targets, tb, hipMemcpyHostToDevice, ctx->stream));
   }
-  HIP_TRY(ctx, LaunchMlpGradient(ctx->mlp, df, dt, n, ctx->dMlpLoss, ctx->stream));
+  HIP_TRY(ctx, LaunchMlpGradient(ctx->mlp, df, dt, MlpBatch{nullptr, (uint32_t)n, 0, 1}, ctx->dMlpLoss, ctx->stream));
   return GPUGI_OK;
 }
 
@@ -1049,7 +1100,7 @@ int gpugi_mlp_train_step(gpugi_ctx* ctx, const float* features, const float* tar
   if (!(learning_rate > 0.0f)) return Fail(ctx, GPUGI_ERR_ARG, "gpugi_mlp_train_step: learning rate must be > 0");
   if (int rc = MlpGradientCommon(ctx, "gpugi_mlp_train_step", features, targets, n, on_device)) return rc;
   AdamParams p{learning_rate, 0.9f, 0.99f, 1e-8f, ++ctx->mlpStep};
-  HIP_TRY(ctx, LaunchMlpAdam(ctx->mlp, p, ctx->stream));
+  HIP_TRY(ctx, LaunchMlpAdam(ctx->mlp, p, MlpBatch{nullptr, (uint32_t)n, 0, 1}, ctx->stream));
   if (loss) {
     HIP_TRY(ctx, hipMemcpyAsync(loss, ctx->dMlpLoss, sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
@@ -1072,9 +1123,25 @@ int gpugi_cache_config(gpugi_ctx* ctx, float spread_c, int extra_bounces, uint32
   return GPUGI_OK;
 }
 
+int gpugi_cache_set_query(gpugi_ctx* ctx, int query) {
+  if (!ctx) return Fail(ctx, GPUGI_ERR_ARG, "gpugi_cache_set_query: null context");
+  if (query != GPUGI_CACHE_SPREAD && query != GPUGI_CACHE_PRIMARY_INDIRECT)
+    return Fail(ctx, GPUGI_ERR_ARG, "gpugi_cache_set_query: unknown query rule");
+  ctx->cacheCfg.query = query;
+  return GPUGI_OK;
+}
+
 int gpugi_cache_stats(gpugi_ctx* ctx, uint32_t* train_samples, float* loss) {
   if (!ctx) return Fail(ctx, GPUGI_ERR_ARG, "gpugi_cache_stats: null context");
-  if (train_samples) *train_samples = ctx->cacheLastSamples;
+  if (train_samples) {
+    *train_samples = 0;
+    if (ctx->dCacheCount) {
+      uint32_t c = 0;
+      HIP_TRY(ctx, hipMemcpyAsync(&c, ctx->dCacheCount, sizeof(c), hipMemcpyDeviceToHost, ctx->stream));
+      HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+      *train_samples = c < ctx->cacheMaxSamples ? c : ctx->cacheMaxSamples;
+    }
+  }
   if (loss) {
     *loss = 0.0f;
     if (ctx->hasMlp) {

@@ -29,6 +29,23 @@ struct ClosestHit {
   gg::V3 geoNormal;  // cross(e1, e0), not normalized (TRINORMAL_OUTPUT)
 };
 
+// ClosestHit from a hit record {slot, t, b_y, b_z} of the wide walk or a
+// trace kernel (b_x = 1 - (b_z + b_y), the reference formula; the geometric
+// normal from the vertices with the reference formula cross(p0 - p2, p1 - p0)).
+__device__ __forceinline__ ClosestHit closest_from_record(const DrawParams& p, int slot, float t, float by, float bz) {
+  ClosestHit h;
+  h.hit = slot >= 0;
+  h.t = t;
+  if (h.hit) {
+    h.tri = p.ref.slots[slot];
+    h.bary = gg::v3(1.0f - (bz + by), by, bz);
+    gg::V3 p0 = load_pos(p.ref.positions, h.tri.x), p1 = load_pos(p.ref.positions, h.tri.y),
+           p2 = load_pos(p.ref.positions, h.tri.z);
+    h.geoNormal = gg::cross(p0 - p2, p1 - p0);
+  }
+  return h;
+}
+
 // With DrawParams::watertight the watertight triangle test replaces the
 // reference one in both traversals (the geometric normal is then rebuilt
 // from the vertices with the reference formula cross(p0 - p2, p1 - p0)).
@@ -42,15 +59,7 @@ __device__ __forceinline__ ClosestHit trace_closest(const TraceCtx& c, gg::V3 o,
                                                                                   c.stk, c.tid, c.spill)
                        : trace_wide_ray<false, kTileThreads, kWideLdsStack, false>(c.p->wide, o, d, gg::kRayMax,
                                                                                    c.stk, c.tid, c.spill);
-    h.hit = r.slot >= 0;
-    h.t = r.t;
-    if (h.hit) {
-      h.tri = c.p->ref.slots[r.slot];
-      h.bary = gg::v3(1.0f - (r.bz + r.by), r.by, r.bz);
-      gg::V3 p0 = load_pos(c.p->ref.positions, h.tri.x), p1 = load_pos(c.p->ref.positions, h.tri.y),
-             p2 = load_pos(c.p->ref.positions, h.tri.z);
-      h.geoNormal = gg::cross(p0 - p2, p1 - p0);
-    }
+    h = closest_from_record(*c.p, r.slot, r.t, r.by, r.bz);
   } else {
     float rayLength = gg::kRayMax;
     RefHit rh;
@@ -92,11 +101,22 @@ __device__ __forceinline__ bool trace_any(const TraceCtx& c, gg::V3 o, gg::V3 d,
   return trace_ref<true, false, false>(c.p->ref, o, d, rayLength, rh, nb, nt);
 }
 
-// EstimateDirectLight (lightsource.glsl:3-93) without importance break / cache.
-template <bool WIDE>
-__device__ __forceinline__ gg::V3 estimate_direct_light(const TraceCtx& c, gg::V3 pos, gg::V3 normal, int idx,
-                                                        gg::V3 viewDir, const MaterialData& m, bool sampleAreaLights) {
-  const gpugi_light_sample& ls = c.p->lights[idx];
+// The light-sample connection of EstimateDirectLight (lightsource.glsl:3-93,
+// without importance break / cache): the shadow ray to test and the radiance
+// it carries if unoccluded.  test == false: nothing to connect.
+struct NeeSample {
+  gg::V3 o, dir;
+  float len;
+  gg::V3 value;
+  bool test;
+};
+
+__device__ __forceinline__ NeeSample nee_sample(const DrawParams& p, gg::V3 pos, gg::V3 normal, int idx,
+                                                gg::V3 viewDir, const MaterialData& m, bool sampleAreaLights) {
+  NeeSample s;
+  s.test = false;
+  s.value = gg::v3s(0.0f);
+  const gpugi_light_sample& ls = p.lights[idx];
   const gg::V3 lpos = gg::v3(ls.position[0], ls.position[1], ls.position[2]);
   const gg::V3 lint = gg::v3(ls.intensity[0], ls.intensity[1], ls.intensity[2]);
   gg::V3 dir = lpos - pos;
@@ -104,29 +124,40 @@ __device__ __forceinline__ gg::V3 estimate_direct_light(const TraceCtx& c, gg::V
   const float lightDist = sqrtf(lightDistSq);
   dir = dir / lightDist;
   const float surfaceCos = gg::saturate(gg::dot(dir, normal));
-  if (surfaceCos <= 0.0f) return gg::v3s(0.0f);
+  if (surfaceCos <= 0.0f) return s;
   if (ls.normal_theta_cos > 1.5f) {
-    const gg::V3 o = gg::kRayHitEpsilon * dir + pos;
-    if (!trace_any<WIDE>(c, o, dir, lightDist - gg::kRayHitEpsilon * 2.0f)) {
-      const gg::V3 irradiance = (surfaceCos / lightDistSq) * lint;
-      float pdf;
-      const gg::V3 bsdf = bsdf_eye(viewDir, dir, m, normal, pdf);
-      return irradiance * bsdf;
-    }
+    s.o = gg::kRayHitEpsilon * dir + pos;
+    s.dir = dir;
+    s.len = lightDist - gg::kRayHitEpsilon * 2.0f;
+    const gg::V3 irradiance = (surfaceCos / lightDistSq) * lint;
+    float pdf;
+    const gg::V3 bsdf = bsdf_eye(viewDir, dir, m, normal, pdf);
+    s.value = irradiance * bsdf;
+    s.test = true;
   } else if (sampleAreaLights) {
     const gg::V3 lightNormal = gg::unpack_normal(ls.normal_phi, ls.normal_theta_cos);
     const float f = gg::saturate(gg::dot(lightNormal, -dir));
     if (f > 0.0f) {
-      const gg::V3 o = gg::kRayHitEpsilon * dir + pos;
-      if (!trace_any<WIDE>(c, o, dir, lightDist - gg::kRayHitEpsilon * 2.0f)) {
-        float pdf;
-        const gg::V3 intensity = lint * f;
-        const gg::V3 bsdf = bsdf_eye(viewDir, dir, m, normal, pdf);
-        const gg::V3 irradiance = (surfaceCos / lightDistSq) * intensity;
-        return irradiance * bsdf;
-      }
+      s.o = gg::kRayHitEpsilon * dir + pos;
+      s.dir = dir;
+      s.len = lightDist - gg::kRayHitEpsilon * 2.0f;
+      float pdf;
+      const gg::V3 intensity = lint * f;
+      const gg::V3 bsdf = bsdf_eye(viewDir, dir, m, normal, pdf);
+      const gg::V3 irradiance = (surfaceCos / lightDistSq) * intensity;
+      s.value = irradiance * bsdf;
+      s.test = true;
     }
   }
+  return s;
+}
+
+// EstimateDirectLight with the shadow ray traced here.
+template <bool WIDE>
+__device__ __forceinline__ gg::V3 estimate_direct_light(const TraceCtx& c, gg::V3 pos, gg::V3 normal, int idx,
+                                                        gg::V3 viewDir, const MaterialData& m, bool sampleAreaLights) {
+  const NeeSample s = nee_sample(*c.p, pos, normal, idx, viewDir, m, sampleAreaLights);
+  if (s.test && !trace_any<WIDE>(c, s.o, s.dir, s.len)) return s.value;
   return gg::v3s(0.0f);
 }
 

@@ -92,6 +92,7 @@ hipError_t BptWarmup(IntegratorState* st, const DrawParams& p, hipStream_t strea
 // Query / training records of the cached path tracer.  Feature vectors are the
 // 16-float raw cache inputs (mlp.h).
 struct CacheParams {
+  int query = GPUGI_CACHE_SPREAD;  // where paths query the cache (gpugi_cache_query)
   float spreadC = 0.01f;          // terminate when the path spread exceeds spreadC * a0
   int extraBounces = 2;           // training paths continue this many vertices past termination
   uint32_t trainStride = 32;      // one training path per trainStride pixels
@@ -113,7 +114,18 @@ inline uint32_t CacheTrainPaths(int width, int height, uint32_t stride) {
 }
 // p[0..n) = value (a kernel, so the call can be recorded into a graph).
 hipError_t LaunchFillU32(uint32_t* p, uint32_t value, size_t n, hipStream_t stream);
-// Traces the short paths (and the extended training paths) of one frame.
+// PRIMARY_INDIRECT wavefront passes over all W*H pixels (ray-indexed buffers
+// in a permuted pixel order): primary rays (2 float4 each)
+// and random states; shading of the closest-hit records into pixelColor, the
+// cache queries, shadow rays (in place of the primary rays) and their NEE
+// contributions; the NEE sum given the any-hit records of the shadow rays.
+hipError_t LaunchPrimaryRaygen(const DrawParams& p, float4* rays, uint32_t* seeds, hipStream_t stream);
+hipError_t LaunchPrimaryShade(const DrawParams& p, const CacheParams& cp, float4* rays, const int4* hits,
+                              const uint32_t* seeds, float4* nee, hipStream_t stream);
+hipError_t LaunchPrimaryFinish(const DrawParams& p, const CacheParams& cp, const int4* occluded, const float4* nee,
+                               hipStream_t stream);
+// Traces the short paths (and the extended training paths) of one frame; with
+// PRIMARY_INDIRECT only the training paths.
 hipError_t LaunchCachedPaths(const DrawParams& p, const CacheParams& cp, void* spill, hipStream_t stream);
 // accum += pixelColor + queryWeight * max(queryPred, 0).
 hipError_t LaunchCacheResolve(const DrawParams& p, const CacheParams& cp, hipStream_t stream);

@@ -64,16 +64,41 @@ struct MlpDev {
 // Tiles of 32 samples the gradient kernel processes.
 inline uint32_t MlpTiles(size_t n) { return (uint32_t)((n + 31) / 32); }
 
+// Samples of one training step.  With count == nullptr: samples [0, n).  With
+// a device counter: total = min(*count, n) samples are split into `steps`
+// equal chunks (the remainder is dropped) and the step uses chunk `step`, so
+// the host never waits for the count.
+struct MlpBatch {
+  const uint32_t* count;
+  uint32_t n;
+  uint32_t step;
+  uint32_t steps;
+};
+
+__host__ __device__ inline void MlpBatchRange(const MlpBatch& b, uint32_t* first, uint32_t* size) {
+  if (!b.count) {
+    *first = 0;
+    *size = b.n;
+    return;
+  }
+  const uint32_t c = *b.count;
+  const uint32_t total = c < b.n ? c : b.n;
+  *size = total / b.steps;
+  *first = b.step * *size;
+}
+
 hipError_t LaunchMlpEncode(const float* raw, uint16_t* enc, size_t n, hipStream_t stream);
 // radiance: n x 4 floats (rgb, 0).
 hipError_t LaunchMlpInfer(const MlpDev& m, const float* raw, float* radiance, size_t n, hipStream_t stream);
 // Gradient of the relative L2 loss mean_{s,c<3} (y - t)^2 / (y^2 + 0.01) (y treated as a
-// constant in the denominator) into m.grad, and the loss into *lossOut (device scalar).
-// targets: n x 4 floats.  Needs m.partialTiles >= MlpTiles(n).
-hipError_t LaunchMlpGradient(const MlpDev& m, const float* raw, const float* targets, size_t n, float* lossOut,
-                             hipStream_t stream);
-// Adam update of the master weights from m.grad, then the BF16 copies.
-hipError_t LaunchMlpAdam(const MlpDev& m, const AdamParams& p, hipStream_t stream);
+// constant in the denominator) over the samples of `batch` into m.grad, and the loss
+// into *lossOut (device scalar); both are 0 for an empty batch.  raw / targets:
+// n x 16 / n x 4 floats.  Needs m.partialTiles >= MlpTiles(batch.n / batch.steps).
+hipError_t LaunchMlpGradient(const MlpDev& m, const float* raw, const float* targets, const MlpBatch& batch,
+                             float* lossOut, hipStream_t stream);
+// Adam update of the master weights from m.grad, then the BF16 copies; skipped
+// when `batch` is empty.
+hipError_t LaunchMlpAdam(const MlpDev& m, const AdamParams& p, const MlpBatch& batch, hipStream_t stream);
 // Re-derives the BF16 copies from the master weights (after an upload).
 hipError_t LaunchMlpRefreshBf16(const MlpDev& m, hipStream_t stream);
 #endif

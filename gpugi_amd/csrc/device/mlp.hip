@@ -10,8 +10,9 @@
 // sample.  The backward pass uses the same instruction for
 //   dW_l = dY_l^T X_l      (K = the 32 samples of a tile; feature-major copies)
 //   dX_l = W_l^T dY_l      (A = the transposed weights WT_l[in][out32])
-// Gradients are written per 32-sample tile and summed in tile order, so a
-// training step is deterministic.
+// Gradients are written per 32-sample tile and summed in a fixed order (tile
+// classes t mod 4 in tile order, then the four classes), so a training step
+// is deterministic.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -68,8 +69,18 @@ constexpr int kSmpStride = 40;   // feature-major rows: 32 samples + 8
 
 constexpr int kInferBlock = 256;  // 4 waves x 16 samples per block step
 
+// Orders a wave's LDS accesses across lanes for the compiler (a wavefront-
+// scope fence emits no barrier instruction): LDS runs one wave's operations in
+// issue order, so a wave-private buffer needs nothing more.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
 // One hidden layer for the 16 samples of a wave: act rows (stride `stride`)
-// are read as K_IN inputs and overwritten with the 64 ReLU outputs.
+// are read as K_IN inputs and overwritten with the 64 ReLU outputs.  The rows
+// belong to this wave alone, so a wave-level sync orders the reads before the
+// overwrite.
 template <int K_IN>
 __device__ __forceinline__ void infer_hidden(const uint16_t* __restrict__ W, uint16_t* act, int stride, int lane) {
   const int s = lane & 15, q = lane >> 4;
@@ -82,7 +93,7 @@ __device__ __forceinline__ void infer_hidden(const uint16_t* __restrict__ W, uin
 #pragma unroll
     for (int ob = 0; ob < 4; ++ob) acc[ob] = mfma(ld8(W + (16 * ob + s) * K_IN + 32 * ks + 8 * q), b, acc[ob]);
   }
-  __syncthreads();  // every wave has read its rows before any row is overwritten
+  wave_lds_sync();
 #pragma unroll
   for (int ob = 0; ob < 4; ++ob) {
     uint2 pk;
@@ -90,7 +101,7 @@ __device__ __forceinline__ void infer_hidden(const uint16_t* __restrict__ W, uin
     pk.y = (uint32_t)f2bf(fmaxf(acc[ob][2], 0.0f)) | ((uint32_t)f2bf(fmaxf(acc[ob][3], 0.0f)) << 16);
     *reinterpret_cast<uint2*>(act + s * stride + 16 * ob + 4 * q) = pk;
   }
-  __syncthreads();
+  wave_lds_sync();
 }
 
 __global__ __launch_bounds__(kInferBlock) void k_mlp_infer(const uint16_t* __restrict__ wb,
@@ -100,13 +111,15 @@ __global__ __launch_bounds__(kInferBlock) void k_mlp_infer(const uint16_t* __res
   __shared__ __align__(16) uint16_t sAct[4 * 16 * kActStride];
   for (int i = threadIdx.x; i < kMlpParams / 8; i += kInferBlock)
     reinterpret_cast<uint4*>(sW)[i] = reinterpret_cast<const uint4*>(wb)[i];
+  __syncthreads();  // the weights are shared and read-only from here on
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int s = lane & 15, q = lane >> 4;
   uint16_t* act = sAct + wave * 16 * kActStride;
-  const uint32_t blockTiles = (n + 63) / 64;
-  // block-uniform loop: every wave runs the same number of steps
-  for (uint32_t bt = blockIdx.x; bt < blockTiles; bt += gridDim.x) {
-    const uint32_t idx = bt * 64 + wave * 16 + s;
+  const uint32_t groups = (n + 15) / 16;
+  const uint32_t wavesTotal = gridDim.x * (kInferBlock / 64);
+  // each wave walks its own sequence of 16-sample groups
+  for (uint32_t g = blockIdx.x * (kInferBlock / 64) + wave; g < groups; g += wavesTotal) {
+    const uint32_t idx = g * 16 + s;
     float f[kMlpRaw];
 #pragma unroll
     for (int i = 0; i < kMlpRaw; ++i) f[i] = 0.0f;
@@ -122,7 +135,7 @@ __global__ __launch_bounds__(kInferBlock) void k_mlp_infer(const uint16_t* __res
     }
     uint16_t* row = act + s * kActStride;
     encode_part(f, q, [&](int k, float x) { row[k] = f2bf(x); });
-    __syncthreads();
+    wave_lds_sync();  // the four lanes of a row wrote it; MFMA lanes read across rows
     infer_hidden<kMlpIn>(sW + kMlpWOff[0], act, kActStride, lane);
     infer_hidden<kMlpHidden>(sW + kMlpWOff[1], act, kActStride, lane);
     infer_hidden<kMlpHidden>(sW + kMlpWOff[2], act, kActStride, lane);
@@ -133,7 +146,7 @@ __global__ __launch_bounds__(kInferBlock) void k_mlp_infer(const uint16_t* __res
       y = mfma(ld8(sW + kMlpWOff[4] + s * kMlpHidden + 32 * ks + 8 * q), ld8(act + s * kActStride + 32 * ks + 8 * q), y);
     if (q == 0 && idx < n)
       reinterpret_cast<float4*>(out)[idx] = make_float4(y[0], y[1], y[2], 0.0f);
-    __syncthreads();  // rows are rewritten by the next step's encoding
+    wave_lds_sync();  // rows are rewritten by the next group's encoding
   }
 }
 
@@ -223,12 +236,17 @@ __device__ __forceinline__ void grad_dx(const uint16_t* __restrict__ WT, const u
 }
 
 __global__ __launch_bounds__(kGradBlock) void k_mlp_grad(const uint16_t* __restrict__ wb, const uint16_t* __restrict__ wtb,
-                                                         const float* __restrict__ raw, const float* __restrict__ targets,
-                                                         uint32_t n, float invCount, float* __restrict__ partials,
+                                                         const float* __restrict__ rawAll, const float* __restrict__ targetsAll,
+                                                         MlpBatch batch, float* __restrict__ partials,
                                                          float* __restrict__ lossPart) {
   __shared__ __align__(16) GradLds L;
   const int lane = threadIdx.x;
   const int s = lane & 15, q = lane >> 4;
+  uint32_t first, n;
+  MlpBatchRange(batch, &first, &n);
+  const float* raw = rawAll + (size_t)first * kMlpRaw;
+  const float* targets = targetsAll + (size_t)first * 4;
+  const float invCount = n ? (float)(1.0 / (3.0 * (double)n)) : 0.0f;
   const uint32_t tiles = (n + 31) / 32;
   for (uint32_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
     // encode both halves
@@ -317,18 +335,32 @@ __global__ __launch_bounds__(kGradBlock) void k_mlp_grad(const uint16_t* __restr
   }
 }
 
-__global__ void k_mlp_reduce(const float* __restrict__ partials, uint32_t tiles, const float* __restrict__ lossPart,
-                             float* __restrict__ grad, float* __restrict__ lossOut) {
-  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p < (uint32_t)kMlpParams) {
-    float g = 0.0f;
-    for (uint32_t t = 0; t < tiles; ++t) g += partials[(size_t)t * kMlpParams + p];
-    grad[p] = g;
-  }
-  if (p == 0 && lossOut) {
+// Sum of the per-tile partials: block = 4 waves x 64 parameters; wave c sums
+// tiles c, c + 4, c + 8, ... in order, then the four class sums are added in
+// class order.  Block 0 also sums the tile losses (lane l: tiles l, l + 64, ...,
+// then a fixed butterfly).
+constexpr int kReduceParams = 64;
+__global__ __launch_bounds__(256) void k_mlp_reduce(const float* __restrict__ partials, MlpBatch batch,
+                                                    const float* __restrict__ lossPart, float* __restrict__ grad,
+                                                    float* __restrict__ lossOut) {
+  __shared__ float cls[4][kReduceParams];
+  uint32_t first, n;
+  MlpBatchRange(batch, &first, &n);
+  const uint32_t tiles = (n + 31) / 32;
+  const int lane = threadIdx.x & 63, c = threadIdx.x >> 6;
+  const uint32_t p = blockIdx.x * kReduceParams + lane;
+  float g = 0.0f;
+  if (p < (uint32_t)kMlpParams)
+    for (uint32_t t = (uint32_t)c; t < tiles; t += 4) g += partials[(size_t)t * kMlpParams + p];
+  cls[c][lane] = g;
+  __syncthreads();
+  if (c == 0 && p < (uint32_t)kMlpParams) grad[p] = ((cls[0][lane] + cls[1][lane]) + cls[2][lane]) + cls[3][lane];
+  if (blockIdx.x == 0 && c == 0 && lossOut) {
     float l = 0.0f;
-    for (uint32_t t = 0; t < tiles; ++t) l += lossPart[t];
-    *lossOut = l;
+    for (uint32_t t = (uint32_t)lane; t < tiles; t += 64) l += lossPart[t];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) l += __shfl_xor(l, off);
+    if (lane == 0) *lossOut = l;
   }
 }
 
@@ -345,9 +377,12 @@ __device__ __forceinline__ int wt_index(int p) {
 
 __global__ void k_mlp_adam(float* __restrict__ w, float* __restrict__ m, float* __restrict__ v,
                            const float* __restrict__ grad, uint16_t* __restrict__ wb, uint16_t* __restrict__ wtb,
-                           float lr, float beta1, float beta2, float eps, float bc1, float bc2) {
+                           float lr, float beta1, float beta2, float eps, float bc1, float bc2, MlpBatch batch) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= kMlpParams) return;
+  uint32_t first, n;
+  MlpBatchRange(batch, &first, &n);
+  if (n == 0) return;  // no samples this step
   const float g = grad[p];
   const float mm = beta1 * m[p] + (1.0f - beta1) * g;
   const float vv = beta2 * v[p] + (1.0f - beta2) * (g * g);
@@ -401,31 +436,33 @@ hipError_t LaunchMlpEncode(const float* raw, uint16_t* enc, size_t n, hipStream_
 hipError_t LaunchMlpInfer(const MlpDev& m, const float* raw, float* radiance, size_t n, hipStream_t stream) {
   if (n == 0) return hipSuccess;
   if (n > 0xffffff00ull) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_mlp_infer, dim3(GridFor(n, 64, 2048)), dim3(kInferBlock), 0, stream, m.wb, raw, radiance,
+  hipLaunchKernelGGL(k_mlp_infer, dim3(GridFor(n, 64 * 8, 1024)), dim3(kInferBlock), 0, stream, m.wb, raw, radiance,
                      (uint32_t)n);
   return hipGetLastError();
 }
 
-hipError_t LaunchMlpGradient(const MlpDev& m, const float* raw, const float* targets, size_t n, float* lossOut,
-                             hipStream_t stream) {
-  if (n == 0 || n > 0xffffff00ull) return hipErrorInvalidValue;
-  const uint32_t tiles = MlpTiles(n);
-  if (tiles > m.partialTiles) return hipErrorInvalidValue;
-  const float invCount = (float)(1.0 / (3.0 * (double)n));
+hipError_t LaunchMlpGradient(const MlpDev& m, const float* raw, const float* targets, const MlpBatch& batch,
+                             float* lossOut, hipStream_t stream) {
+  if (batch.n == 0 || batch.n > 0xffffff00u || batch.steps == 0 || batch.step >= batch.steps)
+    return hipErrorInvalidValue;
+  if (!batch.count && (batch.step != 0 || batch.steps != 1)) return hipErrorInvalidValue;
+  // the most tiles the batch can hold (its count is read on the device)
+  const uint32_t tiles = MlpTiles(batch.n / batch.steps);
+  if (tiles == 0 || tiles > m.partialTiles) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_mlp_grad, dim3(GridFor(tiles, 1, 4096)), dim3(kGradBlock), 0, stream, m.wb, m.wtb, raw,
-                     targets, (uint32_t)n, invCount, m.partials, m.loss);
+                     targets, batch, m.partials, m.loss);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_mlp_reduce, dim3((kMlpParams + 255) / 256), dim3(256), 0, stream, m.partials, tiles, m.loss,
-                     m.grad, lossOut);
+  hipLaunchKernelGGL(k_mlp_reduce, dim3((kMlpParams + kReduceParams - 1) / kReduceParams), dim3(256), 0, stream,
+                     m.partials, batch, m.loss, m.grad, lossOut);
   return hipGetLastError();
 }
 
-hipError_t LaunchMlpAdam(const MlpDev& m, const AdamParams& p, hipStream_t stream) {
+hipError_t LaunchMlpAdam(const MlpDev& m, const AdamParams& p, const MlpBatch& batch, hipStream_t stream) {
   const double bc1 = 1.0 - std::pow((double)p.beta1, (double)p.step);
   const double bc2 = 1.0 - std::pow((double)p.beta2, (double)p.step);
   hipLaunchKernelGGL(k_mlp_adam, dim3((kMlpParams + 255) / 256), dim3(256), 0, stream, m.w, m.m, m.v, m.grad, m.wb,
-                     m.wtb, p.lr, p.beta1, p.beta2, p.eps, (float)bc1, (float)bc2);
+                     m.wtb, p.lr, p.beta1, p.beta2, p.eps, (float)bc1, (float)bc2, batch);
   return hipGetLastError();
 }
 

@@ -77,7 +77,10 @@ struct WideScene {
   uint32_t spillDepth;                // entries per thread in `spill`
   uint32_t numThreads;                // persistent threads the spill buffer is sized for
   uint32_t* errorFlag;                // set to nonzero if a stack would overflow
+  uint32_t* queue;                    // kWideQueueWords: the trace kernel's per-XCD batch counters
 };
+constexpr int kWideQueueStride = 32;  // counters 128 bytes apart
+constexpr int kWideQueueWords = 8 * kWideQueueStride;
 #endif
 
 }  // namespace gpugi

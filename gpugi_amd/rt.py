@@ -17,6 +17,7 @@ CLOSEST_HIT, ANY_HIT = 0, 1
 KERNEL_AUTO, KERNEL_REFERENCE, KERNEL_WIDE = 0, 1, 2
 PT, LT, BPT, PT_CACHE = 0, 1, 2, 3
 TRIANGLE_REFERENCE, TRIANGLE_WATERTIGHT = 0, 1
+CACHE_SPREAD, CACHE_PRIMARY_INDIRECT = 0, 1
 RAY_MAX = np.float32(3.402823466e38)
 
 RAY_DTYPE = np.dtype([("origin", "<f4", 3), ("tmax", "<f4"), ("dir", "<f4", 3), ("pad", "<u4")])
@@ -138,6 +139,7 @@ _SIGNATURES = {
     "gpugi_cache_config": [_P, ctypes.c_float, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int,
                            ctypes.c_float],
     "gpugi_cache_stats": [_P, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_float)],
+    "gpugi_cache_set_query": [_P, ctypes.c_int],
     "gpugi_mlp_init": [_P, ctypes.c_uint64],
     "gpugi_mlp_set_weights": [_P, _P, ctypes.c_size_t],
     "gpugi_mlp_get_weights": [_P, _P, ctypes.c_size_t],
@@ -500,6 +502,9 @@ class Context:
                      train_steps=4, learning_rate=1e-3):
         self._call("gpugi_cache_config", ctypes.c_float(spread_c), int(extra_bounces), int(train_stride),
                    int(max_train_samples), int(train_steps), ctypes.c_float(learning_rate))
+
+    def cache_set_query(self, query):
+        self._call("gpugi_cache_set_query", int(query))
 
     def cache_stats(self):
         n = ctypes.c_uint32()

@@ -380,6 +380,12 @@ int gpugi_mlp_gradient(gpugi_ctx* ctx, const float* features, const float* targe
 int gpugi_mlp_train_step(gpugi_ctx* ctx, const float* features, const float* targets, size_t n,
                          float learning_rate, float* loss, int on_device);
 
+/* Where cached paths query the cache (gpugi_cache_set_query). */
+enum gpugi_cache_query {
+  GPUGI_CACHE_SPREAD = 0,           /* footprint-spread termination, the cache predicts reflected radiance */
+  GPUGI_CACHE_PRIMARY_INDIRECT = 1  /* at the primary hit after its NEE, the cache predicts the bounce part */
+};
+
 /* Cached path tracer (GPUGI_PT_CACHE): a path terminates into the cache at
  * the first vertex from the second hit on whose footprint spread exceeds
  * spread_c times the primary one (default 0.01); one pixel in train_stride
@@ -389,6 +395,13 @@ int gpugi_mlp_train_step(gpugi_ctx* ctx, const float* features, const float* tar
  * learning_rate (default 1e-3) after every frame. */
 int gpugi_cache_config(gpugi_ctx* ctx, float spread_c, int extra_bounces, uint32_t train_stride,
                        uint32_t max_train_samples, int train_steps, float learning_rate);
+/* Query rule of the cached path tracer (default GPUGI_CACHE_SPREAD).  With
+ * GPUGI_CACHE_PRIMARY_INDIRECT every path stops at its primary hit after the
+ * hit's emission and NEE, and the cache supplies the radiance the hit
+ * reflects of light arriving by its sampled bounce (training paths do NEE at
+ * their tail vertex for the same reason).  Changing the rule changes what
+ * the network predicts: re-initialize or retrain it. */
+int gpugi_cache_set_query(gpugi_ctx* ctx, int query);
 /* Training samples and loss of the last cached frame. */
 int gpugi_cache_stats(gpugi_ctx* ctx, uint32_t* train_samples, float* loss);
 
