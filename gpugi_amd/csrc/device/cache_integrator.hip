@@ -188,15 +188,16 @@ __global__ __launch_bounds__(kTileThreads) void k_cached_paths(DrawParams p, Cac
 }
 
 // ---- PRIMARY_INDIRECT wavefront passes: one thread per ray r, every buffer but
-// the per-pixel ones indexed by r.  Ray r belongs to pixel
-//   pix(r) = (r mod Q) * 8 + r / Q,  Q = W*H / 8,
+// the per-pixel ones indexed by r.  Rays go in runs of 64 consecutive pixels
+// (coherent rays, coalesced pixel writes); run c of the ray array is pixel run
+//   (c mod Q) * 8 + c / Q,  Q = W*H / 512,
 // so each eighth of the ray array, which the trace kernel hands to one XCD,
-// samples the whole image (every 8th pixel) and the XCDs get equal work
-// (pix(r) = r when W*H is not a multiple of 8).
+// samples the whole image (every 8th run) and the XCDs get equal work
+// (identity when W*H is not a multiple of 512).
 __device__ __forceinline__ uint32_t wave_pixel(uint32_t r, uint32_t P) {
-  if (P & 7u) return r;
-  const uint32_t Q = P / 8u;
-  return (r % Q) * 8u + r / Q;
+  if (P & 511u) return r;
+  const uint32_t Q = P / 512u, c = r / 64u;
+  return ((c % Q) * 8u + c / Q) * 64u + (r % 64u);
 }
 
 // Primary rays {origin, tmax}{dir, 0} and each pixel's random state after them.

@@ -24,7 +24,7 @@ struct gpugi_ctx {
   // second stream (default priority) for work that overlaps the main stream's,
   // forked and joined with the two events; created on first use
   hipStream_t side = nullptr;
-  hipEvent_t evFork = nullptr, evJoin = nullptr;
+  hipEvent_t evFork = nullptr, evJoin = nullptr, evInfer = nullptr;
   bool timerStarted = false;
   std::string error;
   // reference layout
@@ -256,6 +256,7 @@ void gpugi_destroy(gpugi_ctx* ctx) {
   if (ctx->ev1) hipEventDestroy(ctx->ev1);
   if (ctx->evFork) hipEventDestroy(ctx->evFork);
   if (ctx->evJoin) hipEventDestroy(ctx->evJoin);
+  if (ctx->evInfer) hipEventDestroy(ctx->evInfer);
   if (ctx->side) hipStreamDestroy(ctx->side);
   if (ctx->stream) hipStreamDestroy(ctx->stream);
   delete ctx;
@@ -474,14 +475,18 @@ int DrawCached(gpugi_ctx* ctx, uint32_t frameSeed) {
   cp.trainOffset = frameSeed % cp.trainStride;
   if (int rc = SceneBounds(ctx, cp.bboxMin, cp.bboxInvExtent)) return rc;
 
-  // With PRIMARY_INDIRECT the training paths (few, latency-bound threads) run
-  // on the side stream beside the wavefront passes and join before the targets.
+  // With PRIMARY_INDIRECT the training paths (few, latency-bound threads) and
+  // the whole training chain run on the side stream beside the wavefront
+  // passes; the Adam steps wait until the frame's queries have been inferred
+  // (so every query sees the weights from before the frame, as with SPREAD),
+  // and the main stream joins the side stream at the end of the frame.
   hipStream_t trainStream = ctx->stream;
   if (atPrimary) {
     if (!ctx->side) {
       HIP_TRY(ctx, hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking));
       HIP_TRY(ctx, hipEventCreateWithFlags(&ctx->evFork, hipEventDisableTiming));
       HIP_TRY(ctx, hipEventCreateWithFlags(&ctx->evJoin, hipEventDisableTiming));
+      HIP_TRY(ctx, hipEventCreateWithFlags(&ctx->evInfer, hipEventDisableTiming));
     }
     HIP_TRY(ctx, hipEventRecord(ctx->evFork, ctx->stream));
     HIP_TRY(ctx, hipStreamWaitEvent(ctx->side, ctx->evFork, 0));
@@ -496,8 +501,11 @@ int DrawCached(gpugi_ctx* ctx, uint32_t frameSeed) {
     spill = ctx->integ.spill;
   }
   HIP_TRY(ctx, LaunchCachedPaths(p, cp, spill, trainStream));
+  HIP_TRY(ctx, LaunchMlpInfer(ctx->mlp, cp.tailFeat, tailPred, NT, trainStream));
+  HIP_TRY(ctx, LaunchExclusiveScanU32(cp.trainCount, offsets, NT, scanTemp, dTotal, trainStream));
+  HIP_TRY(ctx, LaunchCacheTargets(p, cp, offsets, dTotal, S, (uint32_t)std::max(ctx->cacheTrainSteps, 1), feat,
+                                  targets, trainStream));
   if (atPrimary) {
-    HIP_TRY(ctx, hipEventRecord(ctx->evJoin, ctx->side));
     // every pixel's primary hit and NEE as wavefront passes through the trace kernels
     WideScene ws;
     if (p.useWide)
@@ -518,12 +526,8 @@ int DrawCached(gpugi_ctx* ctx, uint32_t frameSeed) {
     HIP_TRY(ctx, LaunchPrimaryFinish(p, cp, wHits, wNee, ctx->stream));
   }
   HIP_TRY(ctx, LaunchMlpInfer(ctx->mlp, cp.queryFeat, queryPred, P, ctx->stream));
+  if (atPrimary) HIP_TRY(ctx, hipEventRecord(ctx->evInfer, ctx->stream));
   HIP_TRY(ctx, LaunchCacheResolve(p, cp, ctx->stream));
-  if (atPrimary) HIP_TRY(ctx, hipStreamWaitEvent(ctx->stream, ctx->evJoin, 0));
-  HIP_TRY(ctx, LaunchMlpInfer(ctx->mlp, cp.tailFeat, tailPred, NT, ctx->stream));
-  HIP_TRY(ctx, LaunchExclusiveScanU32(cp.trainCount, offsets, NT, scanTemp, dTotal, ctx->stream));
-  HIP_TRY(ctx, LaunchCacheTargets(p, cp, offsets, dTotal, S, (uint32_t)std::max(ctx->cacheTrainSteps, 1), feat,
-                                  targets, ctx->stream));
   // The sample count stays on the device: each step's kernels read it and
   // take their chunk of min(count, S) samples, so the host does not wait.
   ctx->dCacheCount = dTotal;
@@ -532,10 +536,15 @@ int DrawCached(gpugi_ctx* ctx, uint32_t frameSeed) {
     if (int rc = MlpEnsureTiles(ctx, MlpTiles(S / (uint32_t)steps))) return rc;
     for (int s = 0; s < steps; ++s) {
       const MlpBatch batch{dTotal, S, (uint32_t)s, (uint32_t)steps};
-      HIP_TRY(ctx, LaunchMlpGradient(ctx->mlp, feat, targets, batch, ctx->dMlpLoss, ctx->stream));
+      HIP_TRY(ctx, LaunchMlpGradient(ctx->mlp, feat, targets, batch, ctx->dMlpLoss, trainStream));
+      if (atPrimary && s == 0) HIP_TRY(ctx, hipStreamWaitEvent(ctx->side, ctx->evInfer, 0));
       AdamParams ap{ctx->cacheLr, 0.9f, 0.99f, 1e-8f, ++ctx->mlpStep};
-      HIP_TRY(ctx, LaunchMlpAdam(ctx->mlp, ap, batch, ctx->stream));
+      HIP_TRY(ctx, LaunchMlpAdam(ctx->mlp, ap, batch, trainStream));
     }
+  }
+  if (atPrimary) {
+    HIP_TRY(ctx, hipEventRecord(ctx->evJoin, ctx->side));
+    HIP_TRY(ctx, hipStreamWaitEvent(ctx->stream, ctx->evJoin, 0));
   }
   return GPUGI_OK;
 }

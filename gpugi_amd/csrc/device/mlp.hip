@@ -26,11 +26,33 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 namespace {
 
-// Round to nearest even (finite inputs).
-__device__ __forceinline__ uint16_t f2bf(float f) {
-  uint32_t u = __float_as_uint(f);
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return (uint16_t)(u >> 16);
+// Round to nearest even (v_cvt_pk_bf16_f32).
+__device__ __forceinline__ uint16_t f2bf(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
+__device__ __forceinline__ uint32_t pack_bf2(float lo, float hi) {
+  return (uint32_t)f2bf(lo) | ((uint32_t)f2bf(hi) << 16);
+}
+
+// sin / cos of 2^k pi p for k < 12: sinpi / cospi at k = 0, 3, 6, 9 and two
+// double-angle steps from each (sin 2t = 2 sin t cos t, cos 2t = (cos t - sin
+// t)(cos t + sin t)); for p in [0, 1] the error stays below 4e-7, far inside
+// the BF16 rounding of the features.
+__device__ __forceinline__ void encode_axis(float p, float* sn, float* cs) {
+#pragma unroll
+  for (int a = 0; a < kMlpFreqs; a += 3) {
+    const float x = ldexpf(p, a);  // exact
+    float s = sinpif(x), c = cospif(x);
+    sn[a] = s;
+    cs[a] = c;
+#pragma unroll
+    for (int j = 1; j < 3; ++j) {
+      const float s2 = (s + s) * c;
+      const float c2 = (c - s) * (c + s);
+      s = s2;
+      c = c2;
+      sn[a + j] = s;
+      cs[a + j] = c;
+    }
+  }
 }
 __device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
 
@@ -45,12 +67,12 @@ __device__ __forceinline__ f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c) {
 template <typename Dst>
 __device__ __forceinline__ void encode_part(const float* __restrict__ raw, int q, Dst dst) {
   if (q < 3) {
-    const float p = raw[q];
+    float sn[kMlpFreqs], cs[kMlpFreqs];
+    encode_axis(raw[q], sn, cs);
 #pragma unroll
     for (int k = 0; k < kMlpFreqs; ++k) {
-      const float x = ldexpf(p, k);  // exact
-      dst(q * kMlpFreqs + k, sinpif(x));
-      dst(36 + q * kMlpFreqs + k, cospif(x));
+      dst(q * kMlpFreqs + k, sn[k]);
+      dst(36 + q * kMlpFreqs + k, cs[k]);
     }
   } else {
 #pragma unroll
@@ -67,7 +89,7 @@ constexpr int kSmpStride = 40;   // feature-major rows: 32 samples + 8
 
 // ---------------------------------------------------------------- inference
 
-constexpr int kInferBlock = 256;  // 4 waves x 16 samples per block step
+constexpr int kInferBlock = 512;  // 8 waves sharing one LDS copy of the weights
 
 // Orders a wave's LDS accesses across lanes for the compiler (a wavefront-
 // scope fence emits no barrier instruction): LDS runs one wave's operations in
@@ -108,7 +130,7 @@ __global__ __launch_bounds__(kInferBlock) void k_mlp_infer(const uint16_t* __res
                                                            const float* __restrict__ raw, float* __restrict__ out,
                                                            uint32_t n) {
   __shared__ __align__(16) uint16_t sW[kMlpParams];
-  __shared__ __align__(16) uint16_t sAct[4 * 16 * kActStride];
+  __shared__ __align__(16) uint16_t sAct[(kInferBlock / 64) * 16 * kActStride];
   for (int i = threadIdx.x; i < kMlpParams / 8; i += kInferBlock)
     reinterpret_cast<uint4*>(sW)[i] = reinterpret_cast<const uint4*>(wb)[i];
   __syncthreads();  // the weights are shared and read-only from here on
@@ -133,8 +155,26 @@ __global__ __launch_bounds__(kInferBlock) void k_mlp_infer(const uint16_t* __res
         f[4 * i + 3] = v.w;
       }
     }
+    // the sample's row, 8-byte stores: lane q < 3 writes features [12q, 12q + 12)
+    // and [36 + 12q, 48 + 12q), lane q == 3 features [72, 96)
     uint16_t* row = act + s * kActStride;
-    encode_part(f, q, [&](int k, float x) { row[k] = f2bf(x); });
+    if (q < 3) {
+      float sn[kMlpFreqs], cs[kMlpFreqs];
+      encode_axis(f[q], sn, cs);
+      uint2* ds = reinterpret_cast<uint2*>(row + q * kMlpFreqs);
+      uint2* dc = reinterpret_cast<uint2*>(row + 36 + q * kMlpFreqs);
+#pragma unroll
+      for (int j = 0; j < kMlpFreqs / 4; ++j) {
+        ds[j] = make_uint2(pack_bf2(sn[4 * j], sn[4 * j + 1]), pack_bf2(sn[4 * j + 2], sn[4 * j + 3]));
+        dc[j] = make_uint2(pack_bf2(cs[4 * j], cs[4 * j + 1]), pack_bf2(cs[4 * j + 2], cs[4 * j + 3]));
+      }
+    } else {
+      uint2* d = reinterpret_cast<uint2*>(row + 72);
+      d[0] = make_uint2(pack_bf2(f[3], f[4]), pack_bf2(f[5], f[6]));
+      d[1] = make_uint2(pack_bf2(f[7], f[8]), pack_bf2(f[9], f[10]));
+      d[2] = make_uint2(pack_bf2(f[11], f[12]), 0u);
+      d[3] = d[4] = d[5] = make_uint2(0u, 0u);
+    }
     wave_lds_sync();  // the four lanes of a row wrote it; MFMA lanes read across rows
     infer_hidden<kMlpIn>(sW + kMlpWOff[0], act, kActStride, lane);
     infer_hidden<kMlpHidden>(sW + kMlpWOff[1], act, kActStride, lane);
@@ -436,7 +476,7 @@ hipError_t LaunchMlpEncode(const float* raw, uint16_t* enc, size_t n, hipStream_
 hipError_t LaunchMlpInfer(const MlpDev& m, const float* raw, float* radiance, size_t n, hipStream_t stream) {
   if (n == 0) return hipSuccess;
   if (n > 0xffffff00ull) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_mlp_infer, dim3(GridFor(n, 64 * 8, 1024)), dim3(kInferBlock), 0, stream, m.wb, raw, radiance,
+  hipLaunchKernelGGL(k_mlp_infer, dim3(GridFor(n, 16 * (kInferBlock / 64) * 8, 512)), dim3(kInferBlock), 0, stream, m.wb, raw, radiance,
                      (uint32_t)n);
   return hipGetLastError();
 }
