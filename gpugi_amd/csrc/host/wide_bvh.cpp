@@ -2,6 +2,8 @@
 #include "../device/wide_bvh.h"
 
 #include <algorithm>
+#include <array>
+#include <limits>
 #include <cmath>
 #include <cstring>
 
@@ -118,6 +120,110 @@ std::string BuildWideBVH(const gpugi_scene_desc& desc, uint32_t maxLeafTris, Wid
     return first;
   };
 
+  // SAH-optimal collapse of the BVH2 into 4-wide nodes (the dynamic program of
+  // Ylitie, Karras and Laine 2017, "Efficient incoherent ray traversal on
+  // GPUs through compressed wide BVHs").  dist[n][j] is the least expected
+  // cost of covering n's subtree with at most j children of one wide node;
+  // a child is a leaf (a reference leaf, or a subtree of at most maxLeafTris
+  // valid triangles) or a wide node of its own.  A wide node costs
+  // kNodeCost * area per visit, a leaf kTriCost * area * triangles; the ratio
+  // is the measured cost of a box pass (four slab tests, ordering, stack) to
+  // that of one triangle test.  The topology stays the reference's: only which
+  // BVH2 nodes become wide nodes or leaves is chosen.
+  constexpr float kNodeCost = 1.0f, kTriCost = 0.4f;
+  constexpr float kInf = std::numeric_limits<float>::infinity();
+  struct Dp {
+    float dist[5];
+    uint8_t pick[5];     // 0: no child (empty subtree), 1: n as one child, 2: n's children share j
+    bool leaf;           // n as one child: a leaf (else a wide node)
+  };
+  std::vector<Dp> dp(v.n);
+  // split[splitAt[i] + k][j]: children given to kid k when kids 0..k of node i
+  // share j (knapsack back-pointers)
+  std::vector<size_t> splitAt(v.n + 1, 0);
+  for (uint32_t i = 0; i < v.n; ++i) splitAt[i + 1] = splitAt[i] + kids[i].size();
+  std::vector<std::array<uint8_t, 5>> split(splitAt[v.n], std::array<uint8_t, 5>{0, 0, 0, 0, 0});
+  for (uint32_t n = v.n; n-- > 0;) {
+    Dp& d = dp[n];
+    if (count[n] == 0) {
+      for (int j = 0; j < 5; ++j) { d.dist[j] = 0.0f; d.pick[j] = 0; }
+      d.leaf = true;
+      continue;
+    }
+    const float area = v.surface(n);
+    const bool leafOk = v.isLeafNode(n) || count[n] <= maxLeafTris;
+    const float leafCost = leafOk ? kTriCost * area * (float)count[n] : kInf;
+    std::array<float, 5> comb = {0.0f, kInf, kInf, kInf, kInf};
+    if (!v.isLeafNode(n)) {
+      for (size_t k = 0; k < kids[n].size(); ++k) {
+        const Dp& c = dp[kids[n][k]];
+        std::array<float, 5> next = {kInf, kInf, kInf, kInf, kInf};
+        for (int j = 0; j < 5; ++j)
+          for (int a = 0; a <= j; ++a) {
+            const float cost = comb[j - a] + c.dist[a];
+            if (cost < next[j]) { next[j] = cost; split[splitAt[n] + k][j] = (uint8_t)a; }
+          }
+        comb = next;
+      }
+    } else {
+      comb = {kInf, kInf, kInf, kInf, kInf};
+    }
+    const float nodeCost = v.isLeafNode(n) ? kInf : kNodeCost * area + comb[4];
+    const float single = std::min(leafCost, nodeCost);
+    d.leaf = leafCost <= nodeCost;
+    d.dist[0] = kInf;
+    d.pick[0] = 0;
+    for (int j = 1; j < 5; ++j) {
+      if (single <= comb[j]) { d.dist[j] = single; d.pick[j] = 1; }
+      else { d.dist[j] = comb[j]; d.pick[j] = 2; }
+    }
+  }
+  // The children of the wide node for BVH2 node n: n's own children sharing 4.
+  struct Item {
+    uint32_t bvh2;
+    bool leaf;
+  };
+  std::vector<Item> items;
+  std::vector<std::pair<uint32_t, int>> work;  // (node, children it may use)
+  auto wideChildren = [&](uint32_t n) {
+    items.clear();
+    if (v.isLeafNode(n)) {  // a one-leaf scene
+      if (count[n] > 0) items.push_back({n, true});
+      return;
+    }
+    // knapsack back-tracking over n's kids, then each kid's own choice
+    work.clear();
+    int j = 4;
+    std::vector<std::pair<uint32_t, int>> alloc;
+    for (size_t k = kids[n].size(); k-- > 0;) {
+      const int a = split[splitAt[n] + k][j];
+      alloc.push_back({kids[n][k], a});
+      j -= a;
+    }
+    for (size_t k = alloc.size(); k-- > 0;) work.push_back(alloc[k]);
+    // expand in child order
+    for (size_t w = 0; w < work.size(); ++w) {
+      const uint32_t c = work[w].first;
+      const int jc = work[w].second;
+      const Dp& d = dp[c];
+      if (jc == 0 || d.pick[jc] == 0) continue;
+      if (d.pick[jc] == 1) {
+        items.push_back({c, d.leaf});
+        continue;
+      }
+      // c opened: its kids share jc, inserted in place to keep the reference's child order
+      std::vector<std::pair<uint32_t, int>> sub;
+      int jj = jc;
+      for (size_t k = kids[c].size(); k-- > 0;) {
+        const int a = split[splitAt[c] + k][jj];
+        sub.push_back({kids[c][k], a});
+        jj -= a;
+      }
+      std::reverse(sub.begin(), sub.end());
+      work.insert(work.begin() + (std::ptrdiff_t)w + 1, sub.begin(), sub.end());
+    }
+  };
+
   struct Pending {
     uint32_t bvh2;       // BVH2 node whose subtree this wide node covers
     uint32_t wideIndex;  // index of the wide node to fill
@@ -132,28 +238,8 @@ std::string BuildWideBVH(const gpugi_scene_desc& desc, uint32_t maxLeafTris, Wid
   while (!todo.empty()) {
     Pending p = todo.back();
     todo.pop_back();
-    std::vector<uint32_t> items;
-    if (v.isLeafNode(p.bvh2) || count[p.bvh2] <= maxLeafTris) items.push_back(p.bvh2);
-    else items = kids[p.bvh2];
-    // Open the largest-surface inner item while the node has room.
-    for (;;) {
-      int best = -1;
-      float bestSA = -1.0f;
-      for (size_t k = 0; k < items.size(); ++k) {
-        uint32_t it = items[k];
-        if (v.isLeafNode(it) || count[it] <= maxLeafTris) continue;
-        if (items.size() - 1 + kids[it].size() > 4) continue;
-        float sa = v.surface(it);
-        if (sa > bestSA) { bestSA = sa; best = (int)k; }
-      }
-      if (best < 0) break;
-      uint32_t it = items[(size_t)best];
-      items.erase(items.begin() + best);
-      items.insert(items.begin() + best, kids[it].begin(), kids[it].end());
-    }
-    // Subtrees without a valid triangle can never produce a hit: drop them.
-    items.erase(std::remove_if(items.begin(), items.end(), [&](uint32_t it) { return count[it] == 0; }),
-                items.end());
+    // Subtrees without a valid triangle never produce a hit and are left out.
+    wideChildren(p.bvh2);
     if (items.size() > 4) return "BVH node with more than 4 children cannot be packed";
     if (items.empty()) {  // a scene without any valid triangle
       WideNodeHost empty;
@@ -167,11 +253,11 @@ std::string BuildWideBVH(const gpugi_scene_desc& desc, uint32_t maxLeafTris, Wid
     // Quantization frame.
     float origin[3], top[3];
     for (int a = 0; a < 3; ++a) {
-      origin[a] = v.box(items[0])[a];
-      top[a] = v.box(items[0])[4 + a];
-      for (uint32_t it : items) {
-        origin[a] = std::min(origin[a], v.box(it)[a]);
-        top[a] = std::max(top[a], v.box(it)[4 + a]);
+      origin[a] = v.box(items[0].bvh2)[a];
+      top[a] = v.box(items[0].bvh2)[4 + a];
+      for (const Item& it : items) {
+        origin[a] = std::min(origin[a], v.box(it.bvh2)[a]);
+        top[a] = std::max(top[a], v.box(it.bvh2)[4 + a]);
       }
       node.origin[a] = origin[a];
     }
@@ -185,7 +271,7 @@ std::string BuildWideBVH(const gpugi_scene_desc& desc, uint32_t maxLeafTris, Wid
         const float scale = ScaleOf((uint32_t)b);
         bool ok = true;
         for (size_t k = 0; k < items.size() && ok; ++k) {
-          const float lo = v.box(items[k])[a], hi = v.box(items[k])[4 + a];
+          const float lo = v.box(items[k].bvh2)[a], hi = v.box(items[k].bvh2)[4 + a];
           double ql = std::floor(((double)lo - (double)origin[a]) / (double)scale);
           double qh = std::ceil(((double)hi - (double)origin[a]) / (double)scale);
           if (ql < 0) ql = 0;
@@ -204,8 +290,8 @@ std::string BuildWideBVH(const gpugi_scene_desc& desc, uint32_t maxLeafTris, Wid
     node.exps = biased[0] | (biased[1] << 8) | (biased[2] << 16) | ((uint32_t)items.size() << 24);
     uint32_t innerKids = 0;
     for (size_t k = 0; k < items.size(); ++k) {
-      uint32_t it = items[k];
-      if (v.isLeafNode(it) || count[it] <= maxLeafTris) {
+      const uint32_t it = items[k].bvh2;
+      if (items[k].leaf) {
         uint32_t c = count[it];
         uint32_t first = appendTris(it);
         if (first > kWideLeafFirstMask) return "too many triangles for the wide layout";
