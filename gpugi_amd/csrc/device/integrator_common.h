@@ -161,54 +161,84 @@ __device__ __forceinline__ gg::V3 estimate_direct_light(const TraceCtx& c, gg::V
   return gg::v3s(0.0f);
 }
 
-// pathtracer.comp:16-110 for pixel (gx, gy) (NUM_LIGHT_SAMPLES 1,
-// RUSSIAN_ROULETTE, black environment): adds vec4(color, 0) to the
-// accumulation buffer.  rays (optional) counts the rays this path traced.
-template <bool WIDE>
-__device__ __forceinline__ void pt_pixel(const DrawParams& p, TraceCtx c, int gx, int gy, unsigned long long* rays) {
-  const float lightSampleWeight = (float)p.numLights / 1.0f;
-  uint32_t seed = gg::init_random_seed(p.frameSeed, (uint32_t)(gx + gy * p.width));
-  gg::V3 color = gg::v3s(0.0f);
-  gg::V3 throughput = gg::v3s(1.0f);
-  bool didNEE = false;
-  uint32_t traced = 0;
-  if (rays) c.count = &traced;
-  const float j0 = gg::random01(seed);
-  const float j1 = gg::random01(seed);
+// One path of pathtracer.comp:16-110 (NUM_LIGHT_SAMPLES 1, RUSSIAN_ROULETTE,
+// black environment) as a state that advances one bounce at a time, so a
+// persistent kernel can give a finished lane a new pixel.
+struct PtPath {
+  gg::V3 color, throughput, o, d;
+  uint32_t seed;
+  int bounce;
+  bool didNEE;
+};
+
+// Camera ray of pixel (gx, gy) and the random state after it.
+__device__ __forceinline__ void pt_begin(const DrawParams& p, PtPath& s, int gx, int gy) {
+  s.seed = gg::init_random_seed(p.frameSeed, (uint32_t)(gx + gy * p.width));
+  s.color = gg::v3s(0.0f);
+  s.throughput = gg::v3s(1.0f);
+  s.didNEE = false;
+  s.bounce = 0;
+  const float j0 = gg::random01(s.seed);
+  const float j1 = gg::random01(s.seed);
   const gg::V3 U = gg::v3(p.camera.u[0], p.camera.u[1], p.camera.u[2]);
   const gg::V3 V = gg::v3(p.camera.v[0], p.camera.v[1], p.camera.v[2]);
   const gg::V3 W = gg::v3(p.camera.w[0], p.camera.w[1], p.camera.w[2]);
-  gg::V3 o = gg::v3(p.camera.position[0], p.camera.position[1], p.camera.position[2]);
-  gg::V3 d = gg::primary_direction(j0, j1, gx, gy, p.width, p.height, U, V, W);
-  for (int i = 0; i < p.maxPathLength; ++i) {
-    const ClosestHit h = trace_closest<WIDE>(c, o, d);
-    if (!h.hit) {
-      color = color + throughput * gg::v3s(0.0f);  // black environment map (renderersystem.cpp:39-47)
-      break;
-    }
-    o = o + h.t * d;
-    const gg::V3 n = shading_normal(p.ref.vinfo, h.tri, h.bary);
-    const MaterialData m = load_material(p.materials, h.tri.w);
-    if (!didNEE) color = color + throughput * m.emissivity;
-    didNEE = m.refl[3] < 500.0f;
-    const int sampleIndex = (int)(gg::xorshift32(seed) % p.numLights);
-    const gg::V3 radiance = estimate_direct_light<WIDE>(c, o, n, sampleIndex, d, m, didNEE);
-    color = color + radiance * throughput * lightSampleWeight;
-    gg::V3 tp = gg::v3s(1.0f);
-    float pdf;
-    bool diffuse;
-    d = sample_bsdf(d, m, seed, n, tp, false, pdf, diffuse);
-    o = o + d * gg::kRayHitEpsilon;
-    throughput = throughput * tp;
-    if (i >= p.rrStartDepth) {  // Russian roulette (pathtracer.comp:99-104) from rrStartDepth on
-      const float q = gg::saturate(gg::luminance(tp));
-      if (gg::random01(seed) >= q) break;
-      throughput = throughput / q;
-    }
+  s.o = gg::v3(p.camera.position[0], p.camera.position[1], p.camera.position[2]);
+  s.d = gg::primary_direction(j0, j1, gx, gy, p.width, p.height, U, V, W);
+}
+
+// One bounce of the path (the loop body of pathtracer.comp).  Returns true
+// when the path has ended; s.color is then final.
+template <bool WIDE>
+__device__ __forceinline__ bool pt_bounce(const DrawParams& p, const TraceCtx& c, PtPath& s) {
+  if (s.bounce >= p.maxPathLength) return true;
+  const float lightSampleWeight = (float)p.numLights / 1.0f;
+  const ClosestHit h = trace_closest<WIDE>(c, s.o, s.d);
+  if (!h.hit) {
+    s.color = s.color + s.throughput * gg::v3s(0.0f);  // black environment map (renderersystem.cpp:39-47)
+    return true;
   }
+  s.o = s.o + h.t * s.d;
+  const gg::V3 n = shading_normal(p.ref.vinfo, h.tri, h.bary);
+  const MaterialData m = load_material(p.materials, h.tri.w);
+  if (!s.didNEE) s.color = s.color + s.throughput * m.emissivity;
+  s.didNEE = m.refl[3] < 500.0f;
+  const int sampleIndex = (int)(gg::xorshift32(s.seed) % p.numLights);
+  const gg::V3 radiance = estimate_direct_light<WIDE>(c, s.o, n, sampleIndex, s.d, m, s.didNEE);
+  s.color = s.color + radiance * s.throughput * lightSampleWeight;
+  gg::V3 tp = gg::v3s(1.0f);
+  float pdf;
+  bool diffuse;
+  s.d = sample_bsdf(s.d, m, s.seed, n, tp, false, pdf, diffuse);
+  s.o = s.o + s.d * gg::kRayHitEpsilon;
+  s.throughput = s.throughput * tp;
+  if (s.bounce >= p.rrStartDepth) {  // Russian roulette (pathtracer.comp:99-104) from rrStartDepth on
+    const float q = gg::saturate(gg::luminance(tp));
+    if (gg::random01(s.seed) >= q) return true;
+    s.throughput = s.throughput / q;
+  }
+  ++s.bounce;
+  return s.bounce >= p.maxPathLength;
+}
+
+// Adds vec4(color, 0) to the pixel's accumulation entry.
+__device__ __forceinline__ void pt_finish(const DrawParams& p, const PtPath& s, int gx, int gy) {
   float4* px = &p.accum[(size_t)gy * p.width + gx];
-  float4 v = *px;
-  *px = make_float4(color.x + v.x, color.y + v.y, color.z + v.z, 0.0f + v.w);
+  const float4 v = *px;
+  *px = make_float4(s.color.x + v.x, s.color.y + v.y, s.color.z + v.z, 0.0f + v.w);
+}
+
+// The whole path of pixel (gx, gy) on the calling lane.  rays (optional)
+// counts the rays this path traced.
+template <bool WIDE>
+__device__ __forceinline__ void pt_pixel(const DrawParams& p, TraceCtx c, int gx, int gy, unsigned long long* rays) {
+  uint32_t traced = 0;
+  if (rays) c.count = &traced;
+  PtPath s;
+  pt_begin(p, s, gx, gy);
+  while (!pt_bounce<WIDE>(p, c, s)) {
+  }
+  pt_finish(p, s, gx, gy);
   if (rays) {
     // one atomic per wave
     uint32_t sum = traced;

@@ -63,6 +63,8 @@ struct IntegratorState {
   void* pathCounts = nullptr;       // per light path: stored vertices; then exclusive offsets
   size_t pathCountsBytes = 0;
   uint32_t* lastTotal = nullptr;    // device NumLightCacheEntries of the last BPT draw
+  uint32_t* queue = nullptr;        // the persistent path tracer's tile counter
+  uint32_t ptBlocks[2] = {0, 0};    // its resident grid (reference / wide traversal)
 };
 
 // Grows a device buffer to at least `bytes` (contents are not preserved).
