@@ -21,6 +21,70 @@ struct TraceCtx {
   uint32_t* count = nullptr;  // when set, incremented per ray query
 };
 
+// Work items [0, numItems) for a persistent one-wave block, with regeneration:
+// the wave takes chunks of 64 consecutive items from *queue (zeroed before the
+// launch) and gives each lane whose item has finished the next item of its
+// chunk.  begin(item) starts an item on the calling lane, step() advances it
+// and returns true when it has finished, finish() then completes it.  The
+// order in which items run does not change what any item computes.
+template <class Begin, class Step, class Finish>
+__device__ __forceinline__ void persistent_items(uint32_t numItems, uint32_t* queue, Begin begin, Step step,
+                                                 Finish finish) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t below = (1ull << lane) - 1ull;
+  const uint32_t chunks = (numItems + 63u) / 64u;
+  // wave-uniform: the chunk being handed out and how many of its items are gone
+  uint32_t cur = 0, pos = 64;
+  bool exhausted = false, live = false;
+  for (;;) {
+    const uint64_t idle = ~__ballot(live);  // the block is one full wave
+    const uint32_t nidle = (uint32_t)__popcll(idle);
+    if (!exhausted && nidle > 0u) {
+      const uint32_t avail = 64u - pos;
+      uint32_t next = chunks;
+      if (avail < nidle) {
+        uint32_t v = 0;
+        if (lane == 0) v = atomicAdd(queue, 1u);
+        v = (uint32_t)__shfl((int)v, 0);
+        if (v < chunks) next = v;
+      }
+      if (!live) {
+        const uint32_t rank = (uint32_t)__popcll(idle & below);
+        uint32_t item = numItems;
+        if (rank < avail) item = cur * 64u + pos + rank;
+        else if (next < chunks) item = next * 64u + (rank - avail);
+        if (item < numItems) {
+          begin(item);
+          live = true;
+        }
+      }
+      if (avail >= nidle) {
+        pos += nidle;
+      } else if (next < chunks) {
+        cur = next;
+        pos = nidle - avail;
+      } else {
+        exhausted = true;
+      }
+    }
+    if (__ballot(live) == 0) {
+      if (exhausted) break;
+      continue;
+    }
+    if (live && step()) {
+      finish();
+      live = false;
+    }
+  }
+}
+
+// Pixel of item `item` when items run over 8x8 tiles in raster order.
+__device__ __forceinline__ void tile_pixel(uint32_t item, int width, int& gx, int& gy) {
+  const uint32_t tilesX = (uint32_t)width / kTile, tile = item / kTileThreads, k = item % kTileThreads;
+  gx = (int)((tile % tilesX) * kTile + k % kTile);
+  gy = (int)((tile / tilesX) * kTile + k / kTile);
+}
+
 struct ClosestHit {
   bool hit;
   float t;

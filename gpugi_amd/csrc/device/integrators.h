@@ -1,5 +1,6 @@
 // Integrator entry points (device side of gpugi_draw).
 #pragma once
+#include <map>
 #include <hip/hip_runtime.h>
 
 #include <string>
@@ -63,9 +64,16 @@ struct IntegratorState {
   void* pathCounts = nullptr;       // per light path: stored vertices; then exclusive offsets
   size_t pathCountsBytes = 0;
   uint32_t* lastTotal = nullptr;    // device NumLightCacheEntries of the last BPT draw
-  uint32_t* queue = nullptr;        // the persistent path tracer's tile counter
-  uint32_t ptBlocks[2] = {0, 0};    // its resident grid (reference / wide traversal)
+  uint32_t* queue = nullptr;        // work counter of the persistent integrator kernels
+  std::map<const void*, uint32_t> residentBlocks;  // per kernel: resident one-wave blocks on the device
 };
+
+// Zeroes st->queue on `stream` and sizes the grid of one-wave blocks for a
+// persistent_items kernel: the blocks resident on the device, at most one per
+// 64 items (resident), or one per 64 items, so that blocks rather than lanes
+// take new work (the light paths, whose splats into the image prefer it).
+hipError_t PersistentGrid(IntegratorState* st, const void* kernel, size_t items, uint32_t* blocks,
+                          hipStream_t stream, bool resident = true);
 
 // Grows a device buffer to at least `bytes` (contents are not preserved).
 inline hipError_t EnsureBuffer(void** buf, size_t* cap, size_t bytes) {

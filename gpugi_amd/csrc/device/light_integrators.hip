@@ -98,16 +98,18 @@ __device__ __forceinline__ void store_entry(LightCacheEntryDev* dst, const Light
 }
 
 // lighttracer.comp:49-193.  MODE selects LT, BPT light pass or BPT warm-up.
-template <bool WIDE, int MODE>
-__global__ __launch_bounds__(kLightBlock) void k_lighttracer(DrawParams p, int raysPerSample,
-                                                             LightCacheEntryDev* __restrict__ cache,
-                                                             LightCacheEntryDev* __restrict__ staging,
-                                                             uint32_t* __restrict__ pathCounts) {
-  __shared__ uint2 stk[WIDE ? kWideLdsStack * kLightBlock : 1];
-  const int tid = threadIdx.x;
-  const uint32_t gid = blockIdx.x * kLightBlock + tid;
-  TraceCtx c{&p, stk, tid, WIDE ? p.wide.spill + (size_t)gid * p.wide.spillDepth : nullptr};
-  uint32_t seed = gg::init_random_seed(p.frameSeed, gid);
+// One light path as a state that advances one bounce at a time.
+struct LtPath {
+  gg::V3 tp, o, d;
+  uint32_t seed, gid, stored;
+  int bounce;
+};
+
+template <int MODE>
+__device__ __forceinline__ void lt_begin(const DrawParams& p, LtPath& s, uint32_t gid, int raysPerSample,
+                                         LightCacheEntryDev* __restrict__ cache) {
+  s.gid = gid;
+  s.seed = gg::init_random_seed(p.frameSeed, gid);
   const int lsIndex = (int)(gid / (uint32_t)raysPerSample);
   const gpugi_light_sample& ls = p.lights[lsIndex];
   const gg::V3 lpos = gg::v3(ls.position[0], ls.position[1], ls.position[2]);
@@ -118,55 +120,82 @@ __global__ __launch_bounds__(kLightBlock) void k_lighttracer(DrawParams p, int r
     store_entry(&cache[gid], make_entry(flux, -1, lpos, ls.normal_phi, ls.normal_theta_cos, 0.0f, 0.0f,
                                         gg::v3s(0.0f), 0));
   }
-  gg::V3 tp = lint * (gg::kPI / (float)raysPerSample);
-  gg::V3 o = lpos;
-  gg::V3 d;
+  s.tp = lint * (gg::kPI / (float)raysPerSample);
+  s.o = lpos;
   if (ls.normal_theta_cos > 1.5f) {
-    const float r0 = gg::random01(seed);
-    const float r1 = gg::random01(seed);
-    d = gg::sample_direction(r0, r1);
-    tp = tp * 4.0f;
+    const float r0 = gg::random01(s.seed);
+    const float r1 = gg::random01(s.seed);
+    s.d = gg::sample_direction(r0, r1);
+    s.tp = s.tp * 4.0f;
   } else {
     const gg::V3 n = gg::unpack_normal(ls.normal_phi, ls.normal_theta_cos);
     gg::V3 U, V;
     gg::create_onb(n, U, V);
-    const float r0 = gg::random01(seed);
-    const float r1 = gg::random01(seed);
-    d = gg::sample_hemisphere_cosine(r0, r1, U, V, n);
+    const float r0 = gg::random01(s.seed);
+    const float r1 = gg::random01(s.seed);
+    s.d = gg::sample_hemisphere_cosine(r0, r1, U, V, n);
   }
-  uint32_t stored = 0;
-  for (int i = 0; i < p.maxPathLength; ++i) {
-    const ClosestHit h = trace_closest<WIDE>(c, o, d);
-    if (!h.hit) break;
-    o = o + h.t * d;
-    const gg::V3 sN = shading_normal(p.ref.vinfo, h.tri, h.bary);
-    const float2 uv = hit_texcoord(p.ref.vinfo, h.tri, h.bary);
-    const MaterialData m = load_material(p.materials, h.tri.w);
-    if (MODE != kModeLT && m.refl[3] < 100.0f) {
-      if (MODE == kModeBptLight) {
-        float n0, n1;
-        gg::pack_normal(sN, n0, n1);
-        store_entry(&staging[(size_t)gid * p.maxPathLength + stored],
-                    make_entry(tp, h.tri.w, o, n0, n1, uv.x, uv.y, d, i + 1));
-      }
-      ++stored;
+  s.stored = 0;
+  s.bounce = 0;
+}
+
+// One bounce; true when the path has ended.
+template <bool WIDE, int MODE>
+__device__ __forceinline__ bool lt_bounce(const DrawParams& p, const TraceCtx& c, LtPath& s,
+                                          LightCacheEntryDev* __restrict__ staging) {
+  const int i = s.bounce;
+  if (i >= p.maxPathLength) return true;
+  const ClosestHit h = trace_closest<WIDE>(c, s.o, s.d);
+  if (!h.hit) return true;
+  s.o = s.o + h.t * s.d;
+  const gg::V3 sN = shading_normal(p.ref.vinfo, h.tri, h.bary);
+  const float2 uv = hit_texcoord(p.ref.vinfo, h.tri, h.bary);
+  const MaterialData m = load_material(p.materials, h.tri.w);
+  if (MODE != kModeLT && m.refl[3] < 100.0f) {
+    if (MODE == kModeBptLight) {
+      float n0, n1;
+      gg::pack_normal(sN, n0, n1);
+      store_entry(&staging[(size_t)s.gid * p.maxPathLength + s.stored],
+                  make_entry(s.tp, h.tri.w, s.o, n0, n1, uv.x, uv.y, s.d, i + 1));
     }
-    if (MODE != kModeBptWarmup) project_to_screen<WIDE>(c, o, d, h.geoNormal, sN, m, tp, MODE == kModeBptLight);
-    gg::V3 t1 = gg::v3s(1.0f);
-    float pdf;
-    bool diffuse;
-    const gg::V3 nd = sample_bsdf(d, m, seed, sN, t1, true, pdf, diffuse);
-    t1 = t1 * adjoint_correction(d, nd, h.geoNormal, sN);
-    tp = tp * t1;
-    if (i >= p.rrStartDepth) {  // Russian roulette (lighttracer.comp:183-188) from rrStartDepth on
-      const float q = gg::saturate(gg::luminance(t1));
-      if (gg::random01(seed) >= q) break;
-      tp = tp / q;
-    }
-    d = nd;
-    o = o + d * gg::kRayHitEpsilon;
+    ++s.stored;
   }
-  if (MODE != kModeLT) pathCounts[gid] = stored;
+  if (MODE != kModeBptWarmup) project_to_screen<WIDE>(c, s.o, s.d, h.geoNormal, sN, m, s.tp, MODE == kModeBptLight);
+  gg::V3 t1 = gg::v3s(1.0f);
+  float pdf;
+  bool diffuse;
+  const gg::V3 nd = sample_bsdf(s.d, m, s.seed, sN, t1, true, pdf, diffuse);
+  t1 = t1 * adjoint_correction(s.d, nd, h.geoNormal, sN);
+  s.tp = s.tp * t1;
+  if (i >= p.rrStartDepth) {  // Russian roulette (lighttracer.comp:183-188) from rrStartDepth on
+    const float q = gg::saturate(gg::luminance(t1));
+    if (gg::random01(s.seed) >= q) return true;
+    s.tp = s.tp / q;
+  }
+  s.d = nd;
+  s.o = s.o + s.d * gg::kRayHitEpsilon;
+  ++s.bounce;
+  return s.bounce >= p.maxPathLength;
+}
+
+// All numPaths light paths with path regeneration (persistent_items).
+template <bool WIDE, int MODE>
+__global__ __launch_bounds__(kLightBlock) void k_lighttracer(DrawParams p, int raysPerSample, uint32_t numPaths,
+                                                             LightCacheEntryDev* __restrict__ cache,
+                                                             LightCacheEntryDev* __restrict__ staging,
+                                                             uint32_t* __restrict__ pathCounts,
+                                                             uint32_t* __restrict__ queue) {
+  __shared__ uint2 stk[WIDE ? kWideLdsStack * kLightBlock : 1];
+  const int tid = threadIdx.x;
+  const uint32_t linear = blockIdx.x * kLightBlock + tid;
+  TraceCtx c{&p, stk, tid, WIDE ? p.wide.spill + (size_t)linear * p.wide.spillDepth : nullptr};
+  LtPath s;
+  persistent_items(
+      numPaths, queue, [&](uint32_t gid) { lt_begin<MODE>(p, s, gid, raysPerSample, cache); },
+      [&]() { return lt_bounce<WIDE, MODE>(p, c, s, staging); },
+      [&]() {
+        if (MODE != kModeLT) pathCounts[s.gid] = s.stored;
+      });
 }
 
 // showlightcaches.comp:9-65 for exactly N light samples.
@@ -231,128 +260,161 @@ __global__ __launch_bounds__(256) void k_bpt_scatter(const LightCacheEntryDev* _
   }
 }
 
-// pathtracer_bidir.comp:16-251 without CHAINED_MIS.
-template <bool WIDE>
-__global__ __launch_bounds__(kTileThreads) void k_bpt_eye(DrawParams p, const LightCacheEntryDev* __restrict__ cache,
-                                                          const uint32_t* __restrict__ numEntriesPtr) {
-  __shared__ uint2 stk[WIDE ? kWideLdsStack * kTileThreads : 1];
-  const int tid = threadIdx.x;
-  const int gx = blockIdx.x * kTile + (tid % kTile);
-  const int gy = blockIdx.y * kTile + (tid / kTile);
-  const uint32_t linear = (blockIdx.y * gridDim.x + blockIdx.x) * kTileThreads + tid;
-  TraceCtx c{&p, stk, tid, WIDE ? p.wide.spill + (size_t)linear * p.wide.spillDepth : nullptr};
-  uint32_t seed = gg::init_random_seed(p.frameSeed, (uint32_t)(gx + gy * p.width));
-  const float j0 = gg::random01(seed);
-  const float j1 = gg::random01(seed);
+// pathtracer_bidir.comp:16-251 without CHAINED_MIS: the eye path of one pixel
+// as a state that advances one bounce at a time (for path regeneration).
+struct BptEyePath {
+  gg::V3 color, tp, o, d;
+  uint32_t seed;
+  int bounce;
+  float lastConnectionProbability, lastCosAbs, mis;
+};
+
+__device__ __forceinline__ void bpt_eye_begin(const DrawParams& p, BptEyePath& s, int gx, int gy) {
+  s.seed = gg::init_random_seed(p.frameSeed, (uint32_t)(gx + gy * p.width));
+  const float j0 = gg::random01(s.seed);
+  const float j1 = gg::random01(s.seed);
   const gg::V3 U = gg::v3(p.camera.u[0], p.camera.u[1], p.camera.u[2]);
   const gg::V3 V = gg::v3(p.camera.v[0], p.camera.v[1], p.camera.v[2]);
   const gg::V3 W = gg::v3(p.camera.w[0], p.camera.w[1], p.camera.w[2]);
-  gg::V3 o = gg::v3(p.camera.position[0], p.camera.position[1], p.camera.position[2]);
-  gg::V3 d = gg::primary_direction(j0, j1, gx, gy, p.width, p.height, U, V, W);
-  gg::V3 color = gg::v3s(0.0f);
-  gg::V3 tp = gg::v3s(1.0f);
+  s.o = gg::v3(p.camera.position[0], p.camera.position[1], p.camera.position[2]);
+  s.d = gg::primary_direction(j0, j1, gx, gy, p.width, p.height, U, V, W);
+  s.color = gg::v3s(0.0f);
+  s.tp = gg::v3s(1.0f);
+  s.lastConnectionProbability = 1.0f;  // InitialMIS
+  s.lastCosAbs = 1.0f;
+  s.mis = 0.0f;
+  s.bounce = 0;
+}
+
+// One bounce; true when the path has ended.  actualN: light-cache entries in use.
+template <bool WIDE>
+__device__ __forceinline__ bool bpt_eye_bounce(const DrawParams& p, const TraceCtx& c, BptEyePath& s,
+                                               const LightCacheEntryDev* __restrict__ cache, uint32_t actualN) {
+  const int i = s.bounce;
+  if (i >= p.maxPathLength) return true;
+  const float lightSampleWeight = (float)actualN;
+  const ClosestHit h = trace_closest<WIDE>(c, s.o, s.d);
+  if (!h.hit) return true;
+  const float rayHit = h.t;
+  s.o = s.o + rayHit * s.d;
+  const gg::V3 n = shading_normal(p.ref.vinfo, h.tri, h.bary);
+  const MaterialData m = load_material(p.materials, h.tri.w);
+  const float incidentSurfaceCosSat = gg::saturate(-gg::dot(n, s.d));
+  const float connectionLightToEyePath = distance_mis(rayHit, s.lastCosAbs) * mis_heuristic(incidentSurfaceCosSat / gg::kPI);
+  s.mis = (i == 0) ? 1.0f
+                   : (s.lastConnectionProbability /
+                      fmaxf(s.lastConnectionProbability + connectionLightToEyePath, gg::kDivisorEpsilon));
+  s.color = s.color + s.mis * s.tp * m.emissivity;
+  // Connection to one random light-cache vertex.
+  const uint32_t idx = gg::xorshift32(s.seed) % actualN;
+  const LightCacheEntryDev& e = cache[idx];
+  const int pathLength = e.pathLength + i + 1;
+  if (pathLength <= p.maxPathLength) {
+    const gg::V3 epos = gg::v3(e.position[0], e.position[1], e.position[2]);
+    gg::V3 ld = epos - s.o;
+    const float lightDistSq = gg::dot(ld, ld);
+    const float lightDist = sqrtf(lightDistSq);
+    ld = ld / fmaxf(lightDist, gg::kDivisorEpsilon);
+    const gg::V3 lo = gg::kRayHitEpsilon * ld + s.o;
+    if (!trace_any<WIDE>(c, lo, ld, lightDist - gg::kRayHitEpsilon * 2.0f)) {
+      const int lmat = e.materialIndex;
+      const gg::V3 flux = gg::v3(e.flux[0], e.flux[1], e.flux[2]);
+      gg::V3 lnormal = gg::v3(0.0f, 1.0f, 0.0f);
+      float cosLight = 0.0f;
+      gg::V3 lintensity;
+      const float cosCamera = fabsf(gg::dot(ld, n));
+      if (e.normal1 < 1.5f) {
+        lnormal = gg::unpack_normal_raw(e.normal0, e.normal1);
+        cosLight = fabsf(gg::dot(ld, lnormal));
+      }
+      float pdf;
+      const gg::V3 bsdf = bsdf_eye(s.d, ld, m, n, pdf);
+      float eyeToLight = mis_heuristic(pdf);
+      if (i > 0) {
+        const float distMIS = distance_mis(rayHit, s.lastCosAbs);
+        eyeToLight = eyeToLight * distMIS;
+        s.mis = s.lastConnectionProbability / fmaxf(s.lastConnectionProbability + eyeToLight, gg::kDivisorEpsilon);
+        if (m.refl[3] >= 100.0f) s.mis = 1.0f;
+      } else {
+        s.mis = eyeToLight / fmaxf(s.lastConnectionProbability + eyeToLight, gg::kDivisorEpsilon);
+      }
+      float misLight = 1.0f;
+      if (e.normal1 < 1.5f) {
+        float lightToEye;
+        if (lmat != -1) {
+          const MaterialData lm = load_material(p.materials, lmat);
+          const gg::V3 incidentDir = gg::unpack_normal(e.incidentPacked[0], e.incidentPacked[1]);
+          const gg::V3 lbsdf = bsdf_adjoint(ld, -incidentDir, lm, lnormal, pdf);
+          lintensity = flux * lbsdf * cosLight;
+          lightToEye = mis_heuristic(pdf) * distance_mis(lightDist, cosCamera);
+        } else {
+          lintensity = flux * (cosLight * gg::kPI_2);
+          lightToEye = mis_heuristic(cosLight / gg::kPI) * distance_mis(lightDist, cosCamera);
+        }
+        misLight = lightToEye / fmaxf(lightToEye + eyeToLight, gg::kDivisorEpsilon);
+      } else {
+        lintensity = flux * gg::kPI_2;
+        misLight = 1.0f;
+      }
+      s.mis = s.mis * misLight;
+      const gg::V3 irradiance = (s.mis * lightSampleWeight * cosCamera / lightDistSq) * lintensity;
+      s.color = s.color + irradiance * s.tp * bsdf;
+    }
+  }
+  float samplePDF;
+  gg::V3 t1 = gg::v3s(1.0f);
+  bool diffuse;
+  s.d = sample_bsdf(s.d, m, s.seed, n, t1, false, samplePDF, diffuse);
+  s.o = s.o + s.d * gg::kRayHitEpsilon;
+  s.tp = s.tp * t1;
+  if (i >= p.rrStartDepth) {  // Russian roulette from rrStartDepth on
+    const float q = gg::saturate(gg::luminance(t1));
+    if (gg::random01(s.seed) >= q) return true;
+    s.tp = s.tp / q;
+  }
+  float reverse = mis_heuristic(samplePDF);
+  if (i == 0) {
+    s.mis = reverse / fmaxf(reverse + s.lastConnectionProbability, gg::kDivisorEpsilon);
+  } else {
+    const float distMIS = distance_mis(rayHit, s.lastCosAbs);
+    reverse = reverse * distMIS;
+    s.mis = s.lastConnectionProbability / fmaxf(s.lastConnectionProbability + distMIS * reverse, gg::kDivisorEpsilon);
+    if (m.refl[3] >= 100.0f) s.mis = 1.0f;
+  }
+  s.tp = s.tp * s.mis;
+  s.lastConnectionProbability = reverse;
+  s.lastCosAbs = fabsf(gg::dot(s.d, n));
+  ++s.bounce;
+  return s.bounce >= p.maxPathLength;
+}
+
+// Eye paths of all pixels of the 8x8 tiles with path regeneration
+// (persistent_items); each pixel is written by one lane.
+template <bool WIDE>
+__global__ __launch_bounds__(kTileThreads) void k_bpt_eye(DrawParams p, const LightCacheEntryDev* __restrict__ cache,
+                                                          const uint32_t* __restrict__ numEntriesPtr,
+                                                          uint32_t* __restrict__ queue) {
+  __shared__ uint2 stk[WIDE ? kWideLdsStack * kTileThreads : 1];
+  const int tid = threadIdx.x;
+  const uint32_t linear = blockIdx.x * kTileThreads + tid;
+  TraceCtx c{&p, stk, tid, WIDE ? p.wide.spill + (size_t)linear * p.wide.spillDepth : nullptr};
   const int numEntries = (int)*numEntriesPtr;
   const uint32_t actualN = (uint32_t)(numEntries < p.bptCapacity ? numEntries : p.bptCapacity);
-  const float lightSampleWeight = (float)actualN;
-  float lastConnectionProbability = 1.0f;  // InitialMIS
-  float lastCosAbs = 1.0f;
-  float mis = 0.0f;
-  for (int i = 0; i < p.maxPathLength; ++i) {
-    const ClosestHit h = trace_closest<WIDE>(c, o, d);
-    if (!h.hit) break;
-    const float rayHit = h.t;
-    o = o + rayHit * d;
-    const gg::V3 n = shading_normal(p.ref.vinfo, h.tri, h.bary);
-    const MaterialData m = load_material(p.materials, h.tri.w);
-    const float incidentSurfaceCosSat = gg::saturate(-gg::dot(n, d));
-    const float connectionLightToEyePath = distance_mis(rayHit, lastCosAbs) * mis_heuristic(incidentSurfaceCosSat / gg::kPI);
-    mis = (i == 0) ? 1.0f
-                   : (lastConnectionProbability /
-                      fmaxf(lastConnectionProbability + connectionLightToEyePath, gg::kDivisorEpsilon));
-    color = color + mis * tp * m.emissivity;
-    // Connection to one random light-cache vertex.
-    const uint32_t idx = gg::xorshift32(seed) % actualN;
-    const LightCacheEntryDev& e = cache[idx];
-    const int pathLength = e.pathLength + i + 1;
-    if (pathLength <= p.maxPathLength) {
-      const gg::V3 epos = gg::v3(e.position[0], e.position[1], e.position[2]);
-      gg::V3 ld = epos - o;
-      const float lightDistSq = gg::dot(ld, ld);
-      const float lightDist = sqrtf(lightDistSq);
-      ld = ld / fmaxf(lightDist, gg::kDivisorEpsilon);
-      const gg::V3 lo = gg::kRayHitEpsilon * ld + o;
-      if (!trace_any<WIDE>(c, lo, ld, lightDist - gg::kRayHitEpsilon * 2.0f)) {
-        const int lmat = e.materialIndex;
-        const gg::V3 flux = gg::v3(e.flux[0], e.flux[1], e.flux[2]);
-        gg::V3 lnormal = gg::v3(0.0f, 1.0f, 0.0f);
-        float cosLight = 0.0f;
-        gg::V3 lintensity;
-        const float cosCamera = fabsf(gg::dot(ld, n));
-        if (e.normal1 < 1.5f) {
-          lnormal = gg::unpack_normal_raw(e.normal0, e.normal1);
-          cosLight = fabsf(gg::dot(ld, lnormal));
-        }
-        float pdf;
-        const gg::V3 bsdf = bsdf_eye(d, ld, m, n, pdf);
-        float eyeToLight = mis_heuristic(pdf);
-        if (i > 0) {
-          const float distMIS = distance_mis(rayHit, lastCosAbs);
-          eyeToLight = eyeToLight * distMIS;
-          mis = lastConnectionProbability / fmaxf(lastConnectionProbability + eyeToLight, gg::kDivisorEpsilon);
-          if (m.refl[3] >= 100.0f) mis = 1.0f;
-        } else {
-          mis = eyeToLight / fmaxf(lastConnectionProbability + eyeToLight, gg::kDivisorEpsilon);
-        }
-        float misLight = 1.0f;
-        if (e.normal1 < 1.5f) {
-          float lightToEye;
-          if (lmat != -1) {
-            const MaterialData lm = load_material(p.materials, lmat);
-            const gg::V3 incidentDir = gg::unpack_normal(e.incidentPacked[0], e.incidentPacked[1]);
-            const gg::V3 lbsdf = bsdf_adjoint(ld, -incidentDir, lm, lnormal, pdf);
-            lintensity = flux * lbsdf * cosLight;
-            lightToEye = mis_heuristic(pdf) * distance_mis(lightDist, cosCamera);
-          } else {
-            lintensity = flux * (cosLight * gg::kPI_2);
-            lightToEye = mis_heuristic(cosLight / gg::kPI) * distance_mis(lightDist, cosCamera);
-          }
-          misLight = lightToEye / fmaxf(lightToEye + eyeToLight, gg::kDivisorEpsilon);
-        } else {
-          lintensity = flux * gg::kPI_2;
-          misLight = 1.0f;
-        }
-        mis = mis * misLight;
-        const gg::V3 irradiance = (mis * lightSampleWeight * cosCamera / lightDistSq) * lintensity;
-        color = color + irradiance * tp * bsdf;
-      }
-    }
-    float samplePDF;
-    gg::V3 t1 = gg::v3s(1.0f);
-    bool diffuse;
-    d = sample_bsdf(d, m, seed, n, t1, false, samplePDF, diffuse);
-    o = o + d * gg::kRayHitEpsilon;
-    tp = tp * t1;
-    if (i >= p.rrStartDepth) {  // Russian roulette from rrStartDepth on
-      const float q = gg::saturate(gg::luminance(t1));
-      if (gg::random01(seed) >= q) break;
-      tp = tp / q;
-    }
-    float reverse = mis_heuristic(samplePDF);
-    if (i == 0) {
-      mis = reverse / fmaxf(reverse + lastConnectionProbability, gg::kDivisorEpsilon);
-    } else {
-      const float distMIS = distance_mis(rayHit, lastCosAbs);
-      reverse = reverse * distMIS;
-      mis = lastConnectionProbability / fmaxf(lastConnectionProbability + distMIS * reverse, gg::kDivisorEpsilon);
-      if (m.refl[3] >= 100.0f) mis = 1.0f;
-    }
-    tp = tp * mis;
-    lastConnectionProbability = reverse;
-    lastCosAbs = fabsf(gg::dot(d, n));
-  }
-  float4* px = &p.accum[(size_t)gy * p.width + gx];
-  const float4 v = *px;
-  *px = make_float4(color.x + v.x, color.y + v.y, color.z + v.z, 1.0f + v.w);
+  const uint32_t pixels = ((uint32_t)p.width / kTile) * ((uint32_t)p.height / kTile) * kTileThreads;
+  int gx = 0, gy = 0;
+  BptEyePath s;
+  persistent_items(
+      pixels, queue,
+      [&](uint32_t item) {
+        tile_pixel(item, p.width, gx, gy);
+        bpt_eye_begin(p, s, gx, gy);
+      },
+      [&]() { return bpt_eye_bounce<WIDE>(p, c, s, cache, actualN); },
+      [&]() {
+        float4* px = &p.accum[(size_t)gy * p.width + gx];
+        const float4 v = *px;
+        *px = make_float4(s.color.x + v.x, s.color.y + v.y, s.color.z + v.z, 1.0f + v.w);
+      });
 }
 
 // ---------------------------------------------------------------------------
@@ -364,14 +426,25 @@ static hipError_t PrepareSpill(IntegratorState* st, DrawParams* p, size_t thread
   return e;
 }
 
+// The light paths of whole 64-path work groups (the reference dispatches
+// numPaths / 64 groups), one block per group.
 template <int MODE>
-static void LaunchLight(const DrawParams& p, int R, uint32_t numPaths, LightCacheEntryDev* cache,
-                        LightCacheEntryDev* staging, uint32_t* counts, hipStream_t stream) {
-  dim3 grid(numPaths / kLightBlock);
+static hipError_t LaunchLight(IntegratorState* st, const DrawParams& p, int R, uint32_t numPaths,
+                              LightCacheEntryDev* cache, LightCacheEntryDev* staging, uint32_t* counts,
+                              hipStream_t stream) {
+  const uint32_t paths = numPaths / kLightBlock * kLightBlock;
+  if (paths == 0) return hipSuccess;
+  const void* fn = p.useWide ? reinterpret_cast<const void*>(&k_lighttracer<true, MODE>)
+                             : reinterpret_cast<const void*>(&k_lighttracer<false, MODE>);
+  uint32_t blocks = 0;
+  if (hipError_t e = PersistentGrid(st, fn, paths, &blocks, stream, false)) return e;
   if (p.useWide)
-    hipLaunchKernelGGL((k_lighttracer<true, MODE>), grid, dim3(kLightBlock), 0, stream, p, R, cache, staging, counts);
+    hipLaunchKernelGGL((k_lighttracer<true, MODE>), dim3(blocks), dim3(kLightBlock), 0, stream, p, R, paths, cache,
+                       staging, counts, st->queue);
   else
-    hipLaunchKernelGGL((k_lighttracer<false, MODE>), grid, dim3(kLightBlock), 0, stream, p, R, cache, staging, counts);
+    hipLaunchKernelGGL((k_lighttracer<false, MODE>), dim3(blocks), dim3(kLightBlock), 0, stream, p, R, paths, cache,
+                       staging, counts, st->queue);
+  return hipGetLastError();
 }
 
 hipError_t DrawLT(IntegratorState* st, const DrawParams& p0, hipStream_t stream) {
@@ -380,7 +453,8 @@ hipError_t DrawLT(IntegratorState* st, const DrawParams& p0, hipStream_t stream)
   const uint32_t numPaths = p.numLights * (uint32_t)R;
   hipError_t e = PrepareSpill(st, &p, std::max<size_t>(numPaths, p.numLights));
   if (e != hipSuccess) return e;
-  LaunchLight<kModeLT>(p, R, numPaths, nullptr, nullptr, nullptr, stream);
+  e = LaunchLight<kModeLT>(st, p, R, numPaths, nullptr, nullptr, nullptr, stream);
+  if (e != hipSuccess) return e;
   // LightPathtracer::Draw then projects the light samples themselves (lightpathtracer.cpp:50).
   dim3 grid((p.numLights + kLightBlock - 1) / kLightBlock);
   if (p.useWide) hipLaunchKernelGGL(k_showlightcaches<true>, grid, dim3(kLightBlock), 0, stream, p);
@@ -401,7 +475,8 @@ hipError_t BptWarmup(IntegratorState* st, const DrawParams& p0, hipStream_t stre
   if (e != hipSuccess) return e;
   uint32_t* counts = (uint32_t*)st->pathCounts;
   uint32_t* total = (uint32_t*)((char*)st->scratch + scanTemp);
-  LaunchLight<kModeBptWarmup>(p, R, numPaths, nullptr, nullptr, counts, stream);
+  e = LaunchLight<kModeBptWarmup>(st, p, R, numPaths, nullptr, nullptr, counts, stream);
+  if (e != hipSuccess) return e;
   e = LaunchExclusiveScanU32(counts, counts, numPaths, st->scratch, total, stream);
   if (e != hipSuccess) return e;
   uint32_t stored = 0;
@@ -438,7 +513,8 @@ hipError_t DrawBPT(IntegratorState* st, const DrawParams& p0, hipStream_t stream
   uint32_t* counts = (uint32_t*)st->pathCounts;
   uint32_t* offsets = counts + numPaths;
   uint32_t* total = (uint32_t*)((char*)st->scratch + scanTemp);
-  LaunchLight<kModeBptLight>(p, R, numPaths, cache, staging, counts, stream);
+  e = LaunchLight<kModeBptLight>(st, p, R, numPaths, cache, staging, counts, stream);
+  if (e != hipSuccess) return e;
   e = LaunchExclusiveScanU32(counts, offsets, numPaths, st->scratch, total, stream);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_bpt_scatter, dim3((numPaths + 255) / 256), dim3(256), 0, stream, staging, counts, offsets,
@@ -446,10 +522,16 @@ hipError_t DrawBPT(IntegratorState* st, const DrawParams& p0, hipStream_t stream
   // NumLightCacheEntries = R*N + appended (resetcounter.comp:9 plus the appends)
   AddConstantU32(total, numPaths, stream);
   st->lastTotal = total;
-  dim3 grid(p.width / kTile, p.height / kTile);
-  if (grid.x && grid.y) {
-    if (p.useWide) hipLaunchKernelGGL(k_bpt_eye<true>, grid, dim3(kTileThreads), 0, stream, p, cache, total);
-    else hipLaunchKernelGGL(k_bpt_eye<false>, grid, dim3(kTileThreads), 0, stream, p, cache, total);
+  const size_t pixels = (size_t)(p.width / kTile) * (p.height / kTile) * kTileThreads;
+  if (pixels) {
+    const void* fn = p.useWide ? reinterpret_cast<const void*>(&k_bpt_eye<true>)
+                               : reinterpret_cast<const void*>(&k_bpt_eye<false>);
+    uint32_t blocks = 0;
+    if (hipError_t e2 = PersistentGrid(st, fn, pixels, &blocks, stream)) return e2;
+    if (p.useWide)
+      hipLaunchKernelGGL(k_bpt_eye<true>, dim3(blocks), dim3(kTileThreads), 0, stream, p, cache, total, st->queue);
+    else
+      hipLaunchKernelGGL(k_bpt_eye<false>, dim3(blocks), dim3(kTileThreads), 0, stream, p, cache, total, st->queue);
   }
   return hipGetLastError();
 }
