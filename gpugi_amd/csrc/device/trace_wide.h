@@ -73,9 +73,12 @@ struct WideState {
   WtRay wr;  // watertight mode only
 };
 
+// 1/d by v_rcp_f32 (within 1 ulp): a reciprocal with relative error
+// below 2^-23 scales every slab distance of a ray by the same factor (1 + δ),
+// which the far-distance and hit-distance widening of 1 + 2^-20 covers.
 __device__ __forceinline__ float safe_inv(float d) {
   const float tiny = 1e-20f;
-  return 1.0f / (fabsf(d) > tiny ? d : copysignf(tiny, d));
+  return __builtin_amdgcn_rcpf(fabsf(d) > tiny ? d : copysignf(tiny, d));
 }
 
 __device__ __forceinline__ float byte_f(uint32_t w, int k) { return (float)((w >> (8 * k)) & 0xffu); }
@@ -115,12 +118,14 @@ __device__ __forceinline__ bool wide_visit(const WideScene& s, WideState& st, co
   // Slab distances straight from the quantized bytes: along axis a,
   //   t(q) = (origin_a + q * s_a - o_a) / d_a ~ q * A_a + B_a,
   //   A_a = s_a * inv_a (exact: s_a is a power of two),  B_a = (origin_a - o_a) * inv_a.
-  // B_a carries at most ~3 roundings of |B_a|, so near distances use B_a - e_a and
-  // far ones B_a + e_a with e_a = |B_a| * 2^-21; the byte word giving the near
-  // plane (lo or hi) is picked per axis from the sign of inv_a.  The host checks
-  // in extended precision that origin + q * s encloses every child box, so with
-  // the far distance widened by 1 + 2^-20 a box that holds the closest hit is
-  // never rejected (the same argument as Ize 2013, "Robust BVH ray traversal").
+  // With inv_a = (1/d_a)(1 + δ), |δ| <= 2^-23 (safe_inv), every exact distance of
+  // the ray is scaled by 1 + δ; beyond that B_a carries two roundings, so near
+  // distances use B_a - e_a and far ones B_a + e_a with e_a = |B_a| * 2^-21; the
+  // byte word giving the near plane (lo or hi) is picked per axis from the sign of
+  // inv_a.  The host checks in extended precision that origin + q * s encloses
+  // every child box, so with the far distance and the hit distance widened by
+  // 1 + 2^-20 (> 1 + 2^-23 + the fma's rounding) a box that holds the closest hit
+  // is never rejected (the same argument as Ize 2013, "Robust BVH ray traversal").
   const float Ax = sx * st.ix, Ay = sy * st.iy, Az = sz * st.iz;
   const float Bx = (ox - o.x) * st.ix, By = (oy - o.y) * st.iy, Bz = (oz - o.z) * st.iz;
   const float ex = fabsf(Bx) * 4.76837158e-7f, ey = fabsf(By) * 4.76837158e-7f, ez = fabsf(Bz) * 4.76837158e-7f;
