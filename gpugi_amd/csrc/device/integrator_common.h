@@ -22,25 +22,25 @@ struct TraceCtx {
 };
 
 // Work items [0, numItems) for a persistent one-wave block, with regeneration:
-// the wave takes chunks of 64 consecutive items from *queue (zeroed before the
-// launch) and gives each lane whose item has finished the next item of its
-// chunk.  begin(item) starts an item on the calling lane, step() advances it
-// and returns true when it has finished, finish() then completes it.  The
-// order in which items run does not change what any item computes.
+// the wave takes chunks of `chunk` consecutive items from *queue (zeroed
+// before the launch) and gives each lane whose item has finished the next item
+// of its chunk.  begin(item) starts an item on the calling lane, step()
+// advances it and returns true when it has finished, finish() then completes
+// it.  The order in which items run does not change what any item computes.
 template <class Begin, class Step, class Finish>
 __device__ __forceinline__ void persistent_items(uint32_t numItems, uint32_t* queue, Begin begin, Step step,
-                                                 Finish finish) {
+                                                 Finish finish, uint32_t chunk = 64) {
   const int lane = threadIdx.x & 63;
   const uint64_t below = (1ull << lane) - 1ull;
-  const uint32_t chunks = (numItems + 63u) / 64u;
+  const uint32_t chunks = (numItems + chunk - 1u) / chunk;
   // wave-uniform: the chunk being handed out and how many of its items are gone
-  uint32_t cur = 0, pos = 64;
+  uint32_t cur = 0, pos = chunk;
   bool exhausted = false, live = false;
   for (;;) {
     const uint64_t idle = ~__ballot(live);  // the block is one full wave
     const uint32_t nidle = (uint32_t)__popcll(idle);
     if (!exhausted && nidle > 0u) {
-      const uint32_t avail = 64u - pos;
+      const uint32_t avail = chunk - pos;
       uint32_t next = chunks;
       if (avail < nidle) {
         uint32_t v = 0;
@@ -51,8 +51,8 @@ __device__ __forceinline__ void persistent_items(uint32_t numItems, uint32_t* qu
       if (!live) {
         const uint32_t rank = (uint32_t)__popcll(idle & below);
         uint32_t item = numItems;
-        if (rank < avail) item = cur * 64u + pos + rank;
-        else if (next < chunks) item = next * 64u + (rank - avail);
+        if (rank < avail) item = cur * chunk + pos + rank;
+        else if (next < chunks) item = next * chunk + (rank - avail);
         if (item < numItems) {
           begin(item);
           live = true;
@@ -236,8 +236,8 @@ struct PtPath {
 };
 
 // Camera ray of pixel (gx, gy) and the random state after it.
-__device__ __forceinline__ void pt_begin(const DrawParams& p, PtPath& s, int gx, int gy) {
-  s.seed = gg::init_random_seed(p.frameSeed, (uint32_t)(gx + gy * p.width));
+__device__ __forceinline__ void pt_begin(const DrawParams& p, PtPath& s, int gx, int gy, uint32_t frameSeed) {
+  s.seed = gg::init_random_seed(frameSeed, (uint32_t)(gx + gy * p.width));
   s.color = gg::v3s(0.0f);
   s.throughput = gg::v3s(1.0f);
   s.didNEE = false;
@@ -299,7 +299,7 @@ __device__ __forceinline__ void pt_pixel(const DrawParams& p, TraceCtx c, int gx
   uint32_t traced = 0;
   if (rays) c.count = &traced;
   PtPath s;
-  pt_begin(p, s, gx, gy);
+  pt_begin(p, s, gx, gy, p.frameSeed);
   while (!pt_bounce<WIDE>(p, c, s)) {
   }
   pt_finish(p, s, gx, gy);

@@ -31,7 +31,7 @@ __global__ __launch_bounds__(kTileThreads) void k_pathtracer(DrawParams p, uint3
       pixels, queue,
       [&](uint32_t item) {
         tile_pixel(item, p.width, gx, gy);
-        pt_begin(p, s, gx, gy);
+        pt_begin(p, s, gx, gy, p.frameSeed);
       },
       [&]() { return pt_bounce<WIDE>(p, c, s); }, [&]() { pt_finish(p, s, gx, gy); });
 }
