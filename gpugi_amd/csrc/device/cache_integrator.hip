@@ -62,16 +62,19 @@ __device__ __forceinline__ void zero_features(float* dst) {
   for (int k = 0; k < 4; ++k) d4[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
-// SPREAD: an 8x8 tile of pixels per block.  PRIMARY_INDIRECT: a 1-D grid, one
-// thread per training path (the wavefront passes render the pixels).
-template <bool WIDE>
+// TRAIN == false: the pixels' paths up to their query vertex (SPREAD; an 8x8
+// tile per block), writing pixelColor and the queries.  TRAIN == true: the
+// training paths (a 1-D grid, one thread per path), writing only the training
+// records; a training path is its pixel's path continued, so both passes
+// compute the same prefix.
+template <bool WIDE, bool TRAIN>
 __global__ __launch_bounds__(kTileThreads) void k_cached_paths(DrawParams p, CacheParams cp) {
   __shared__ uint2 stk[WIDE ? kWideLdsStack * kTileThreads : 1];
   const int tid = threadIdx.x;
   const bool atPrimary = cp.query == GPUGI_CACHE_PRIMARY_INDIRECT;
   int gx, gy;
   uint32_t linear, pix;
-  if (atPrimary) {
+  if (TRAIN) {
     linear = blockIdx.x * kTileThreads + tid;
     pix = linear * cp.trainStride + cp.trainOffset;
     if (pix >= (uint32_t)p.width * (uint32_t)p.height) return;
@@ -84,7 +87,7 @@ __global__ __launch_bounds__(kTileThreads) void k_cached_paths(DrawParams p, Cac
     pix = (uint32_t)gy * (uint32_t)p.width + (uint32_t)gx;
   }
   TraceCtx c{&p, stk, tid, WIDE ? p.wide.spill + (size_t)linear * p.wide.spillDepth : nullptr};
-  const bool isTrain = pix % cp.trainStride == cp.trainOffset;
+  const bool isTrain = TRAIN;
   const uint32_t trainIdx = pix / cp.trainStride;
   float* tv = cp.trainVert + (size_t)trainIdx * p.maxPathLength * kTrainVertFloats;
 
@@ -115,8 +118,10 @@ __global__ __launch_bounds__(kTileThreads) void k_cached_paths(DrawParams p, Cac
     const gg::V3 te = throughput * (didNEE ? gg::v3s(0.0f) : m.emissivity);
     if (!atPrimary && !term && i >= 1 && spread > cp.spreadC * a0) {
       color = color + te;
-      write_features(cp.queryFeat + (size_t)pix * kMlpRaw, cp, o, -d, n, m);
-      reinterpret_cast<float4*>(cp.queryWeight)[pix] = make_float4(throughput.x, throughput.y, throughput.z, 1.0f);
+      if (!TRAIN) {
+        write_features(cp.queryFeat + (size_t)pix * kMlpRaw, cp, o, -d, n, m);
+        reinterpret_cast<float4*>(cp.queryWeight)[pix] = make_float4(throughput.x, throughput.y, throughput.z, 1.0f);
+      }
       term = true;
       termDepth = i;
       if (!isTrain) break;
@@ -171,14 +176,13 @@ __global__ __launch_bounds__(kTileThreads) void k_cached_paths(DrawParams p, Cac
       throughput = throughput / q;
     }
   }
-  if (!atPrimary) {  // with PRIMARY_INDIRECT the wavefront passes write the pixel and its query
+  if (!TRAIN) {
     reinterpret_cast<float4*>(cp.pixelColor)[pix] = make_float4(color.x, color.y, color.z, 0.0f);
     if (!term) {
       reinterpret_cast<float4*>(cp.queryWeight)[pix] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
       zero_features(cp.queryFeat + (size_t)pix * kMlpRaw);
     }
-  }
-  if (isTrain) {
+  } else {
     cp.trainCount[trainIdx] = nTrain;
     float4* tp4 = reinterpret_cast<float4*>(cp.trainPath + (size_t)trainIdx * 8);
     tp4[0] = make_float4(cacc.x, cacc.y, cacc.z, 0.0f);
@@ -362,19 +366,19 @@ hipError_t LaunchFillU32(uint32_t* p, uint32_t value, size_t n, hipStream_t stre
   return hipGetLastError();
 }
 
-hipError_t LaunchCachedPaths(const DrawParams& p, const CacheParams& cp, void* spill, hipStream_t stream) {
+hipError_t LaunchCachedPaths(const DrawParams& p, const CacheParams& cp, void* spill, bool train,
+                             hipStream_t stream) {
   dim3 grid(p.width / kTile, p.height / kTile);
   if (grid.x == 0 || grid.y == 0) return hipSuccess;
   if (cp.trainStride == 0 || cp.trainOffset >= cp.trainStride) return hipErrorInvalidValue;
-  if (cp.query == GPUGI_CACHE_PRIMARY_INDIRECT)  // one thread per training path
-    grid = dim3((CacheTrainPaths(p.width, p.height, cp.trainStride) + kTileThreads - 1) / kTileThreads);
-  if (p.useWide) {
-    DrawParams q = p;
-    q.wide.spill = (uint2*)spill;
-    hipLaunchKernelGGL(k_cached_paths<true>, grid, dim3(kTileThreads), 0, stream, q, cp);
-  } else {
-    hipLaunchKernelGGL(k_cached_paths<false>, grid, dim3(kTileThreads), 0, stream, p, cp);
-  }
+  if (!train && cp.query == GPUGI_CACHE_PRIMARY_INDIRECT) return hipErrorInvalidValue;  // wavefront passes instead
+  if (train) grid = dim3((CacheTrainPaths(p.width, p.height, cp.trainStride) + kTileThreads - 1) / kTileThreads);
+  DrawParams q = p;
+  if (p.useWide) q.wide.spill = (uint2*)spill;
+  if (p.useWide && train) hipLaunchKernelGGL((k_cached_paths<true, true>), grid, dim3(kTileThreads), 0, stream, q, cp);
+  else if (p.useWide) hipLaunchKernelGGL((k_cached_paths<true, false>), grid, dim3(kTileThreads), 0, stream, q, cp);
+  else if (train) hipLaunchKernelGGL((k_cached_paths<false, true>), grid, dim3(kTileThreads), 0, stream, q, cp);
+  else hipLaunchKernelGGL((k_cached_paths<false, false>), grid, dim3(kTileThreads), 0, stream, q, cp);
   return hipGetLastError();
 }
 

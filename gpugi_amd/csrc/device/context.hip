@@ -436,11 +436,14 @@ int DrawCached(gpugi_ctx* ctx, uint32_t frameSeed) {
   const uint32_t S = ctx->cacheMaxSamples;
   const bool atPrimary = cp.query == GPUGI_CACHE_PRIMARY_INDIRECT;
   const size_t PW = atPrimary ? P : 0;  // wavefront buffers: rays, hit records, random states, NEE
-  size_t off[18], total = 0;
-  const size_t sizes[18] = {P * 16, P * 64, P * 16, P * 16, (size_t)NT * L * 96, (size_t)NT * 4, (size_t)NT * 4,
+  // the training pass's own traversal spill (it runs beside the pixel pass)
+  const size_t trainSpillBytes =
+      p.useWide ? ((size_t)NT + 64) / 64 * 64 * p.wide.spillDepth * sizeof(uint2) : 8;  // 64-thread blocks
+  size_t off[19], total = 0;
+  const size_t sizes[19] = {P * 16, P * 64, P * 16, P * 16, (size_t)NT * L * 96, (size_t)NT * 4, (size_t)NT * 4,
                             (size_t)NT * 32, (size_t)NT * 64, (size_t)NT * 16, ExclusiveScanTempBytes(NT), 4,
-                            (size_t)S * 64, (size_t)S * 16, PW * 32, PW * 16, PW * 4, PW * 16};
-  for (int i = 0; i < 18; ++i) {
+                            (size_t)S * 64, (size_t)S * 16, PW * 32, PW * 16, PW * 4, PW * 16, trainSpillBytes};
+  for (int i = 0; i < 19; ++i) {
     off[i] = total;
     total += Align256(sizes[i]);
   }
@@ -472,35 +475,36 @@ int DrawCached(gpugi_ctx* ctx, uint32_t frameSeed) {
   int4* wHits = (int4*)(b + off[15]);
   uint32_t* wSeeds = (uint32_t*)(b + off[16]);
   float4* wNee = (float4*)(b + off[17]);
+  void* trainSpill = b + off[18];
   cp.trainOffset = frameSeed % cp.trainStride;
   if (int rc = SceneBounds(ctx, cp.bboxMin, cp.bboxInvExtent)) return rc;
 
-  // With PRIMARY_INDIRECT the training paths (few, latency-bound threads) and
-  // the whole training chain run on the side stream beside the wavefront
-  // passes; the Adam steps wait until the frame's queries have been inferred
-  // (so every query sees the weights from before the frame, as with SPREAD),
-  // and the main stream joins the side stream at the end of the frame.
-  hipStream_t trainStream = ctx->stream;
-  if (atPrimary) {
-    if (!ctx->side) {
-      HIP_TRY(ctx, hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking));
-      HIP_TRY(ctx, hipEventCreateWithFlags(&ctx->evFork, hipEventDisableTiming));
-      HIP_TRY(ctx, hipEventCreateWithFlags(&ctx->evJoin, hipEventDisableTiming));
-      HIP_TRY(ctx, hipEventCreateWithFlags(&ctx->evInfer, hipEventDisableTiming));
-    }
-    HIP_TRY(ctx, hipEventRecord(ctx->evFork, ctx->stream));
-    HIP_TRY(ctx, hipStreamWaitEvent(ctx->side, ctx->evFork, 0));
-    trainStream = ctx->side;
+  // The training paths (few, latency-bound threads) and the whole training
+  // chain run on the side stream beside the pixels' passes; the Adam steps
+  // wait until the frame's queries have been inferred (so every query sees the
+  // weights from before the frame), and the main stream joins the side stream
+  // at the end of the frame.
+  if (!ctx->side) {
+    HIP_TRY(ctx, hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking));
+    HIP_TRY(ctx, hipEventCreateWithFlags(&ctx->evFork, hipEventDisableTiming));
+    HIP_TRY(ctx, hipEventCreateWithFlags(&ctx->evJoin, hipEventDisableTiming));
+    HIP_TRY(ctx, hipEventCreateWithFlags(&ctx->evInfer, hipEventDisableTiming));
   }
+  HIP_TRY(ctx, hipEventRecord(ctx->evFork, ctx->stream));
+  HIP_TRY(ctx, hipStreamWaitEvent(ctx->side, ctx->evFork, 0));
+  const hipStream_t trainStream = ctx->side;
   HIP_TRY(ctx, LaunchFillU32(cp.trainCount, 0u, NT, trainStream));
-  void* spill = nullptr;
-  if (p.useWide) {
-    const size_t threads = (size_t)(ctx->width / 8) * (ctx->height / 8) * 64;
-    HIP_TRY(ctx, EnsureBuffer(&ctx->integ.spill, &ctx->integ.spillBytes,
-                              std::max<size_t>(8, threads * p.wide.spillDepth * sizeof(uint2))));
-    spill = ctx->integ.spill;
+  HIP_TRY(ctx, LaunchCachedPaths(p, cp, trainSpill, true, trainStream));
+  if (!atPrimary) {
+    void* spill = nullptr;
+    if (p.useWide) {
+      const size_t threads = (size_t)(ctx->width / 8) * (ctx->height / 8) * 64;
+      HIP_TRY(ctx, EnsureBuffer(&ctx->integ.spill, &ctx->integ.spillBytes,
+                                std::max<size_t>(8, threads * p.wide.spillDepth * sizeof(uint2))));
+      spill = ctx->integ.spill;
+    }
+    HIP_TRY(ctx, LaunchCachedPaths(p, cp, spill, false, ctx->stream));
   }
-  HIP_TRY(ctx, LaunchCachedPaths(p, cp, spill, trainStream));
   HIP_TRY(ctx, LaunchMlpInfer(ctx->mlp, cp.tailFeat, tailPred, NT, trainStream));
   HIP_TRY(ctx, LaunchExclusiveScanU32(cp.trainCount, offsets, NT, scanTemp, dTotal, trainStream));
   HIP_TRY(ctx, LaunchCacheTargets(p, cp, offsets, dTotal, S, (uint32_t)std::max(ctx->cacheTrainSteps, 1), feat,
@@ -526,7 +530,7 @@ int DrawCached(gpugi_ctx* ctx, uint32_t frameSeed) {
     HIP_TRY(ctx, LaunchPrimaryFinish(p, cp, wHits, wNee, ctx->stream));
   }
   HIP_TRY(ctx, LaunchMlpInfer(ctx->mlp, cp.queryFeat, queryPred, P, ctx->stream));
-  if (atPrimary) HIP_TRY(ctx, hipEventRecord(ctx->evInfer, ctx->stream));
+  HIP_TRY(ctx, hipEventRecord(ctx->evInfer, ctx->stream));
   HIP_TRY(ctx, LaunchCacheResolve(p, cp, ctx->stream));
   // The sample count stays on the device: each step's kernels read it and
   // take their chunk of min(count, S) samples, so the host does not wait.
@@ -537,15 +541,13 @@ int DrawCached(gpugi_ctx* ctx, uint32_t frameSeed) {
     for (int s = 0; s < steps; ++s) {
       const MlpBatch batch{dTotal, S, (uint32_t)s, (uint32_t)steps};
       HIP_TRY(ctx, LaunchMlpGradient(ctx->mlp, feat, targets, batch, ctx->dMlpLoss, trainStream));
-      if (atPrimary && s == 0) HIP_TRY(ctx, hipStreamWaitEvent(ctx->side, ctx->evInfer, 0));
+      if (s == 0) HIP_TRY(ctx, hipStreamWaitEvent(ctx->side, ctx->evInfer, 0));
       AdamParams ap{ctx->cacheLr, 0.9f, 0.99f, 1e-8f, ++ctx->mlpStep};
       HIP_TRY(ctx, LaunchMlpAdam(ctx->mlp, ap, batch, trainStream));
     }
   }
-  if (atPrimary) {
-    HIP_TRY(ctx, hipEventRecord(ctx->evJoin, ctx->side));
-    HIP_TRY(ctx, hipStreamWaitEvent(ctx->stream, ctx->evJoin, 0));
-  }
+  HIP_TRY(ctx, hipEventRecord(ctx->evJoin, ctx->side));
+  HIP_TRY(ctx, hipStreamWaitEvent(ctx->stream, ctx->evJoin, 0));
   return GPUGI_OK;
 }
 

@@ -134,9 +134,11 @@ hipError_t LaunchPrimaryShade(const DrawParams& p, const CacheParams& cp, float4
                               const uint32_t* seeds, float4* nee, hipStream_t stream);
 hipError_t LaunchPrimaryFinish(const DrawParams& p, const CacheParams& cp, const int4* occluded, const float4* nee,
                                hipStream_t stream);
-// Traces the short paths (and the extended training paths) of one frame; with
-// PRIMARY_INDIRECT only the training paths.
-hipError_t LaunchCachedPaths(const DrawParams& p, const CacheParams& cp, void* spill, hipStream_t stream);
+// train == false: the pixels' paths up to their query vertex (SPREAD only;
+// spill for W/8 * H/8 * 64 threads).  train == true: the training paths (spill
+// for CacheTrainPaths rounded up to 64 threads).
+hipError_t LaunchCachedPaths(const DrawParams& p, const CacheParams& cp, void* spill, bool train,
+                             hipStream_t stream);
 // accum += pixelColor + queryWeight * max(queryPred, 0).
 hipError_t LaunchCacheResolve(const DrawParams& p, const CacheParams& cp, hipStream_t stream);
 // Dense training samples (features n x 16, targets n x 4), n = min(*total, maxSamples):
