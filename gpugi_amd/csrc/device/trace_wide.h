@@ -1,20 +1,20 @@
 // Ordered traversal of the 4-wide quantized BVH (wide_bvh.h), one ray per lane.
 //
-// A ray's state holds the next inner node to visit, at most one pending leaf
+// A ray's state holds the next inner node to visit, up to two pending leaves
 // and a stack of (child word, entry distance) pairs.  Work comes in two kinds
 // of wave-wide passes:
 //   box pass   every lane with a node visits it: four child box tests, the
 //              nearest hit inner child becomes the next node, the nearest hit
-//              leaf becomes the pending leaf if that slot is free, the other
-//              hit children are pushed farthest first;
-//   leaf pass  every lane with a pending leaf tests its triangles.
-// A lane whose pending slot is taken keeps traversing (speculative traversal,
+//              leaf takes a free pending slot, the other hit children are
+//              pushed farthest first;
+//   leaf pass  every lane with pending leaves tests their triangles.
+// A lane with a pending leaf keeps traversing (speculative traversal,
 // Aila and Laine 2009, "Understanding the efficiency of ray traversal on
 // GPUs"), so a wave runs a leaf pass only once enough lanes wait for one and
 // does not pay a divergent triangle loop on every node visit.  The caller
 // chooses the pass from two ballots (wide_choose_leaf_pass).  Lanes left
 // without a node pop: entries beyond the current hit are dropped, a leaf
-// entry fills the free pending slot, an inner entry becomes the next node.
+// entry fills a free pending slot, an inner entry becomes the next node.
 // The result does not depend on the order of the work: the closest hit is the
 // minimum distance, ties resolved to the smaller reference slot index (the
 // triangle the reference's preorder walk meets first).
@@ -68,6 +68,7 @@ struct WideState {
   float bu, bv;
   uint32_t node;  // next inner node to visit, kWideNoNode if none
   uint32_t leaf;  // pending leaf child word (leaf flag set), 0 if none
+  uint32_t leaf2; // second pending leaf (filled after leaf), 0 if none
   int sp;
   bool found;
   WtRay wr;  // watertight mode only
@@ -95,6 +96,7 @@ __device__ __forceinline__ void wide_init(WideState& st, gg::V3 o, gg::V3 d, flo
   st.bu = st.bv = 0.0f;
   st.node = 0;
   st.leaf = 0;
+  st.leaf2 = 0;
   st.sp = 0;
   st.found = false;
   if (WT) st.wr = wt_setup(d);
@@ -174,17 +176,20 @@ __device__ __forceinline__ bool wide_visit(const WideScene& s, WideState& st, co
   const bool h0 = t0 != INFINITY, h1 = t1 != INFINITY, h2 = t2 != INFINITY, h3 = t3 != INFINITY;
   const bool f0 = (int32_t)c0 < 0, f1 = (int32_t)c1 < 0, f2 = (int32_t)c2 < 0, f3 = (int32_t)c3 < 0;
   const bool in0 = h0 && !f0, in1 = h1 && !f1, in2 = h2 && !f2, in3 = h3 && !f3;
-  const bool freeSlot = st.leaf == 0u;
+  const bool freeSlot = st.leaf2 == 0u;
   const bool lf0 = freeSlot && h0 && f0, lf1 = freeSlot && h1 && f1, lf2 = freeSlot && h2 && f2,
              lf3 = freeSlot && h3 && f3;
   uint32_t node = in3 ? c3 : kWideNoNode;
   node = in2 ? c2 : node;
   node = in1 ? c1 : node;
   node = in0 ? c0 : node;
-  uint32_t leaf = lf3 ? c3 : st.leaf;
-  leaf = lf2 ? c2 : leaf;
-  leaf = lf1 ? c1 : leaf;
-  leaf = lf0 ? c0 : leaf;
+  uint32_t claimed = lf3 ? c3 : 0u;
+  claimed = lf2 ? c2 : claimed;
+  claimed = lf1 ? c1 : claimed;
+  claimed = lf0 ? c0 : claimed;
+  const bool firstFree = st.leaf == 0u;
+  const uint32_t leaf = firstFree ? claimed : st.leaf;
+  const uint32_t leaf2 = (firstFree || claimed == 0u) ? st.leaf2 : claimed;
   const bool p0 = h0 && !in0 && !lf0;
   const bool p1 = h1 && !(in1 && !in0) && !(lf1 && !lf0);
   const bool p2 = h2 && !(in2 && !in0 && !in1) && !(lf2 && !lf0 && !lf1);
@@ -220,6 +225,7 @@ __device__ __forceinline__ bool wide_visit(const WideScene& s, WideState& st, co
   }
   st.node = node;
   st.leaf = leaf;
+  st.leaf2 = leaf2;
   st.sp = sp;
   if (overflow) {
     atomicOr(s.errorFlag, 1u);
@@ -229,15 +235,19 @@ __device__ __forceinline__ bool wide_visit(const WideScene& s, WideState& st, co
   return true;
 }
 
-// Leaf pass for a lane with a pending leaf: tests its triangles and frees the slot.
+// Leaf pass for a lane with pending leaves: tests their triangles (first slot,
+// then second) and frees both slots.
 template <bool ANY, bool WT>
 __device__ __forceinline__ void wide_leaf(const WideScene& s, WideState& st) {
   const gg::V3 o = st.o, d = st.d;
-  const uint32_t first = st.leaf & kWideLeafFirstMask;
-  const uint32_t cnt = ((st.leaf >> 27) & 0xfu) + 1u;
+  const uint32_t wa = st.leaf, wb = st.leaf2;
+  const uint32_t cntA = ((wa >> 27) & 0xfu) + 1u;
+  const uint32_t cnt = cntA + (wb != 0u ? ((wb >> 27) & 0xfu) + 1u : 0u);
   st.leaf = 0;
+  st.leaf2 = 0;
   for (uint32_t m = 0; m < cnt; ++m) {
-    const size_t ti = 3 * (size_t)(first + m);
+    const uint32_t tri = m < cntA ? (wa & kWideLeafFirstMask) + m : (wb & kWideLeafFirstMask) + (m - cntA);
+    const size_t ti = 3 * (size_t)tri;
     const float4 a = s.tris[ti + 0];
     const float4 b = s.tris[ti + 1];
     const float4 c = s.tris[ti + 2];
@@ -277,13 +287,13 @@ __device__ __forceinline__ void wide_leaf(const WideScene& s, WideState& st) {
 }
 
 // For a lane without a next node: pops entries still in front of the closest
-// hit until one yields a node, or a leaf meets a taken pending slot (the entry
-// stays), or the stack is empty.
+// hit until one yields a node, or a leaf finds both pending slots taken (the
+// entry stays), or the stack is empty.
 template <int B, int S>
 __device__ __forceinline__ void wide_pop(WideState& st, const WideStack& stk) {
   const float tb = st.tbest * 1.00000095f;  // entry distances are lower bounds with this slack
   int sp = st.sp;
-  uint32_t leaf = st.leaf;
+  uint32_t leaf = st.leaf, leaf2 = st.leaf2;
   uint32_t node = kWideNoNode;
   while (sp > 0) {
     const int top = sp - 1;
@@ -303,8 +313,9 @@ __device__ __forceinline__ void wide_pop(WideState& st, const WideStack& stk) {
       continue;
     }
     if (nd & kWideLeafFlag) {
-      if (leaf != 0u) break;
-      leaf = nd;
+      if (leaf2 != 0u) break;
+      if (leaf == 0u) leaf = nd;
+      else leaf2 = nd;
       sp = top;
       continue;
     }
@@ -314,6 +325,7 @@ __device__ __forceinline__ void wide_pop(WideState& st, const WideStack& stk) {
   }
   st.sp = sp;
   st.leaf = leaf;
+  st.leaf2 = leaf2;
   st.node = node;
 }
 
