@@ -170,30 +170,32 @@ __device__ __forceinline__ bool wide_visit(const WideScene& s, WideState& st, co
     GPUGI_CSWAP(t1, c1, t2, c2);
 #undef GPUGI_CSWAP
   }
-  // The nearest hit inner child is visited next, the nearest hit leaf fills a
-  // free pending slot, every other hit child is pushed (p_k).  The conditions
-  // are lane masks; each claim is a chain of selects from the farthest child in.
-  const bool h0 = t0 != INFINITY, h1 = t1 != INFINITY, h2 = t2 != INFINITY, h3 = t3 != INFINITY;
-  const bool f0 = (int32_t)c0 < 0, f1 = (int32_t)c1 < 0, f2 = (int32_t)c2 < 0, f3 = (int32_t)c3 < 0;
-  const bool in0 = h0 && !f0, in1 = h1 && !f1, in2 = h2 && !f2, in3 = h3 && !f3;
-  const bool freeSlot = st.leaf2 == 0u;
-  const bool lf0 = freeSlot && h0 && f0, lf1 = freeSlot && h1 && f1, lf2 = freeSlot && h2 && f2,
-             lf3 = freeSlot && h3 && f3;
-  uint32_t node = in3 ? c3 : kWideNoNode;
-  node = in2 ? c2 : node;
-  node = in1 ? c1 : node;
-  node = in0 ? c0 : node;
-  uint32_t claimed = lf3 ? c3 : 0u;
-  claimed = lf2 ? c2 : claimed;
-  claimed = lf1 ? c1 : claimed;
-  claimed = lf0 ? c0 : claimed;
+  // The nearest hit inner child is visited next, the nearest hit leaves fill the
+  // free pending slots, every other hit child is pushed (p_k).  Ranks count the
+  // hit inner and hit leaf children nearer than child k; the claims are selects.
+  const float tk[4] = {t0, t1, t2, t3};
+  const uint32_t ck[4] = {c0, c1, c2, c3};
+  const int freeSlots = (st.leaf == 0u ? 1 : 0) + (st.leaf2 == 0u ? 1 : 0);
+  uint32_t node = kWideNoNode, la = 0u, lb = 0u;
+  int nInner = 0, nLeaf = 0;
+  bool pk[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const bool hit = tk[k] != INFINITY;
+    const bool isLeaf = (int32_t)ck[k] < 0;
+    const bool takeNode = hit && !isLeaf && nInner == 0;
+    const bool takeLeaf = hit && isLeaf && nLeaf < freeSlots;
+    node = takeNode ? ck[k] : node;
+    la = (takeLeaf && nLeaf == 0) ? ck[k] : la;
+    lb = (takeLeaf && nLeaf == 1) ? ck[k] : lb;
+    nInner += (hit && !isLeaf) ? 1 : 0;
+    nLeaf += (hit && isLeaf) ? 1 : 0;
+    pk[k] = hit && !takeNode && !takeLeaf;
+  }
   const bool firstFree = st.leaf == 0u;
-  const uint32_t leaf = firstFree ? claimed : st.leaf;
-  const uint32_t leaf2 = (firstFree || claimed == 0u) ? st.leaf2 : claimed;
-  const bool p0 = h0 && !in0 && !lf0;
-  const bool p1 = h1 && !(in1 && !in0) && !(lf1 && !lf0);
-  const bool p2 = h2 && !(in2 && !in0 && !in1) && !(lf2 && !lf0 && !lf1);
-  const bool p3 = h3 && !(in3 && !in0 && !in1 && !in2) && !(lf3 && !lf0 && !lf1 && !lf2);
+  const uint32_t leaf = firstFree ? la : st.leaf;
+  const uint32_t leaf2 = firstFree ? lb : (st.leaf2 == 0u ? la : st.leaf2);
+  const bool p0 = pk[0], p1 = pk[1], p2 = pk[2], p3 = pk[3];
   bool overflow = false;
   int sp = st.sp;
   if (sp + 4 <= S) {  // every push stays in LDS: one store and one increment each
