@@ -60,6 +60,10 @@ struct WideStack {
   uint32_t spillStride;
 };
 
+// Pending-leaf slots per lane, filled in order (slot j is taken only while
+// slots 0..j-1 are).
+constexpr int kWidePending = 2;
+
 struct WideState {
   gg::V3 o, d;
   float ix, iy, iz;
@@ -67,8 +71,7 @@ struct WideState {
   int bestSlot;
   float bu, bv;
   uint32_t node;  // next inner node to visit, kWideNoNode if none
-  uint32_t leaf;  // pending leaf child word (leaf flag set), 0 if none
-  uint32_t leaf2; // second pending leaf (filled after leaf), 0 if none
+  uint32_t leaf[kWidePending];  // pending leaf child words (leaf flag set), 0 if free
   int sp;
   bool found;
   WtRay wr;  // watertight mode only
@@ -95,8 +98,8 @@ __device__ __forceinline__ void wide_init(WideState& st, gg::V3 o, gg::V3 d, flo
   st.bestSlot = -1;
   st.bu = st.bv = 0.0f;
   st.node = 0;
-  st.leaf = 0;
-  st.leaf2 = 0;
+#pragma unroll
+  for (int j = 0; j < kWidePending; ++j) st.leaf[j] = 0;
   st.sp = 0;
   st.found = false;
   if (WT) st.wr = wt_setup(d);
@@ -175,8 +178,14 @@ __device__ __forceinline__ bool wide_visit(const WideScene& s, WideState& st, co
   // hit inner and hit leaf children nearer than child k; the claims are selects.
   const float tk[4] = {t0, t1, t2, t3};
   const uint32_t ck[4] = {c0, c1, c2, c3};
-  const int freeSlots = (st.leaf == 0u ? 1 : 0) + (st.leaf2 == 0u ? 1 : 0);
-  uint32_t node = kWideNoNode, la = 0u, lb = 0u;
+  int used = 0;
+#pragma unroll
+  for (int j = 0; j < kWidePending; ++j) used += st.leaf[j] != 0u ? 1 : 0;
+  const int freeSlots = kWidePending - used;
+  uint32_t node = kWideNoNode;
+  uint32_t claimed[kWidePending];
+#pragma unroll
+  for (int j = 0; j < kWidePending; ++j) claimed[j] = 0u;
   int nInner = 0, nLeaf = 0;
   bool pk[4];
 #pragma unroll
@@ -186,15 +195,21 @@ __device__ __forceinline__ bool wide_visit(const WideScene& s, WideState& st, co
     const bool takeNode = hit && !isLeaf && nInner == 0;
     const bool takeLeaf = hit && isLeaf && nLeaf < freeSlots;
     node = takeNode ? ck[k] : node;
-    la = (takeLeaf && nLeaf == 0) ? ck[k] : la;
-    lb = (takeLeaf && nLeaf == 1) ? ck[k] : lb;
+#pragma unroll
+    for (int j = 0; j < kWidePending; ++j) claimed[j] = (takeLeaf && nLeaf == j) ? ck[k] : claimed[j];
     nInner += (hit && !isLeaf) ? 1 : 0;
     nLeaf += (hit && isLeaf) ? 1 : 0;
     pk[k] = hit && !takeNode && !takeLeaf;
   }
-  const bool firstFree = st.leaf == 0u;
-  const uint32_t leaf = firstFree ? la : st.leaf;
-  const uint32_t leaf2 = firstFree ? lb : (st.leaf2 == 0u ? la : st.leaf2);
+  // free slot j (j >= used) takes claimed[j - used]
+  uint32_t leaf[kWidePending];
+#pragma unroll
+  for (int j = 0; j < kWidePending; ++j) {
+    uint32_t w = st.leaf[j];
+#pragma unroll
+    for (int u = 0; u <= j; ++u) w = (used == u) ? claimed[j - u] : w;
+    leaf[j] = w;
+  }
   const bool p0 = pk[0], p1 = pk[1], p2 = pk[2], p3 = pk[3];
   bool overflow = false;
   int sp = st.sp;
@@ -226,8 +241,8 @@ __device__ __forceinline__ bool wide_visit(const WideScene& s, WideState& st, co
 #undef GPUGI_PUSH
   }
   st.node = node;
-  st.leaf = leaf;
-  st.leaf2 = leaf2;
+#pragma unroll
+  for (int j = 0; j < kWidePending; ++j) st.leaf[j] = leaf[j];
   st.sp = sp;
   if (overflow) {
     atomicOr(s.errorFlag, 1u);
@@ -237,18 +252,25 @@ __device__ __forceinline__ bool wide_visit(const WideScene& s, WideState& st, co
   return true;
 }
 
-// Leaf pass for a lane with pending leaves: tests their triangles (first slot,
-// then second) and frees both slots.
+// Leaf pass for a lane with pending leaves: tests their triangles (one loop
+// over the slots in order) and frees every slot.
 template <bool ANY, bool WT>
 __device__ __forceinline__ void wide_leaf(const WideScene& s, WideState& st) {
   const gg::V3 o = st.o, d = st.d;
-  const uint32_t wa = st.leaf, wb = st.leaf2;
-  const uint32_t cntA = ((wa >> 27) & 0xfu) + 1u;
-  const uint32_t cnt = cntA + (wb != 0u ? ((wb >> 27) & 0xfu) + 1u : 0u);
-  st.leaf = 0;
-  st.leaf2 = 0;
+  uint32_t first[kWidePending], end[kWidePending];  // end[j]: triangles of slots 0..j
+  uint32_t cnt = 0;
+#pragma unroll
+  for (int j = 0; j < kWidePending; ++j) {
+    const uint32_t w = st.leaf[j];
+    first[j] = (w & kWideLeafFirstMask) - cnt;  // triangle of loop index m is first[j] + m
+    cnt += w != 0u ? ((w >> 27) & 0xfu) + 1u : 0u;
+    end[j] = cnt;
+    st.leaf[j] = 0;
+  }
   for (uint32_t m = 0; m < cnt; ++m) {
-    const uint32_t tri = m < cntA ? (wa & kWideLeafFirstMask) + m : (wb & kWideLeafFirstMask) + (m - cntA);
+    uint32_t tri = first[kWidePending - 1] + m;
+#pragma unroll
+    for (int j = kWidePending - 2; j >= 0; --j) tri = m < end[j] ? first[j] + m : tri;
     const size_t ti = 3 * (size_t)tri;
     const float4 a = s.tris[ti + 0];
     const float4 b = s.tris[ti + 1];
@@ -295,7 +317,9 @@ template <int B, int S>
 __device__ __forceinline__ void wide_pop(WideState& st, const WideStack& stk) {
   const float tb = st.tbest * 1.00000095f;  // entry distances are lower bounds with this slack
   int sp = st.sp;
-  uint32_t leaf = st.leaf, leaf2 = st.leaf2;
+  uint32_t leaf[kWidePending];
+#pragma unroll
+  for (int j = 0; j < kWidePending; ++j) leaf[j] = st.leaf[j];
   uint32_t node = kWideNoNode;
   while (sp > 0) {
     const int top = sp - 1;
@@ -315,9 +339,14 @@ __device__ __forceinline__ void wide_pop(WideState& st, const WideStack& stk) {
       continue;
     }
     if (nd & kWideLeafFlag) {
-      if (leaf2 != 0u) break;
-      if (leaf == 0u) leaf = nd;
-      else leaf2 = nd;
+      if (leaf[kWidePending - 1] != 0u) break;
+      bool placed = false;
+#pragma unroll
+      for (int j = 0; j < kWidePending; ++j) {
+        const bool take = !placed && leaf[j] == 0u;
+        leaf[j] = take ? nd : leaf[j];
+        placed = placed || take;
+      }
       sp = top;
       continue;
     }
@@ -326,8 +355,8 @@ __device__ __forceinline__ void wide_pop(WideState& st, const WideStack& stk) {
     break;
   }
   st.sp = sp;
-  st.leaf = leaf;
-  st.leaf2 = leaf2;
+#pragma unroll
+  for (int j = 0; j < kWidePending; ++j) st.leaf[j] = leaf[j];
   st.node = node;
 }
 
@@ -343,7 +372,7 @@ __device__ __forceinline__ bool wide_choose_leaf_pass(uint64_t working, const Wi
 template <bool ANY, int B, int S, bool WT>
 __device__ __forceinline__ bool wide_pass(const WideScene& s, WideState& st, bool leafPass, const WideStack& stk) {
   if (leafPass) {
-    if (st.leaf == 0u) return false;  // still traversing; unaffected by this pass
+    if (st.leaf[0] == 0u) return false;  // still traversing; unaffected by this pass
     wide_leaf<ANY, WT>(s, st);
     if (ANY && st.found) return true;
   } else {
@@ -352,7 +381,7 @@ __device__ __forceinline__ bool wide_pass(const WideScene& s, WideState& st, boo
   }
   if (st.node == kWideNoNode) {
     wide_pop<B, S>(st, stk);
-    return st.node == kWideNoNode && st.leaf == 0u;
+    return st.node == kWideNoNode && st.leaf[0] == 0u;
   }
   return false;
 }
