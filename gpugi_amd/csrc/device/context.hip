@@ -84,6 +84,10 @@ struct gpugi_ctx {
   float* dMlpLoss = nullptr;  // loss of the last gradient / training step
   void* dMlpIo = nullptr;     // host-pointer staging of features / targets / outputs
   size_t mlpIoCap = 0;
+  // reference image of the MSE check (RGB32F) and the reduction's block sums
+  void* dRefImage = nullptr;
+  int refW = 0, refH = 0;
+  void* dMseTemp = nullptr;
   // scratch
   void* dScratch = nullptr;
   size_t scratchCap = 0;
@@ -244,7 +248,7 @@ void gpugi_destroy(gpugi_ctx* ctx) {
   if (ctx->stream) hipStreamSynchronize(ctx->stream);
   for (void** p : {&ctx->dNodes, &ctx->dSlots, &ctx->dPos, &ctx->dVinfo, &ctx->dMats, &ctx->dWideNodes,
                    &ctx->dWideTris, &ctx->dSpill, &ctx->dCounters, &ctx->dWideQueue, &ctx->dAccum, &ctx->dLights,
-                   &ctx->dScratch})
+                   &ctx->dScratch, &ctx->dRefImage, &ctx->dMseTemp})
     FreeDev(*p);
   FreeIntegratorState(&ctx->integ);
   for (void** p : {(void**)&ctx->mlp.w, (void**)&ctx->mlp.m, (void**)&ctx->mlp.v, (void**)&ctx->mlp.wb,
@@ -653,6 +657,37 @@ int gpugi_readback(gpugi_ctx* ctx, float* rgba, size_t bytes, uint32_t* iteratio
   HIP_TRY(ctx, hipMemcpyAsync(rgba, ctx->dAccum, need, hipMemcpyDeviceToHost, ctx->stream));
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   if (iterations) *iterations = ctx->iterations;
+  return GPUGI_OK;
+}
+
+int gpugi_set_reference_image(gpugi_ctx* ctx, const float* rgb, int width, int height) {
+  if (!ctx || !rgb) return Fail(ctx, GPUGI_ERR_ARG, "gpugi_set_reference_image: null argument");
+  if (width <= 0 || height <= 0) return Fail(ctx, GPUGI_ERR_ARG, "gpugi_set_reference_image: empty image");
+  if (SetDev(ctx)) return GPUGI_ERR_HIP;
+  const size_t bytes = (size_t)width * height * 12;
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));  // an MSE computation may still read the old image
+  FreeDev(ctx->dRefImage);
+  ctx->refW = ctx->refH = 0;
+  HIP_TRY(ctx, hipMalloc(&ctx->dRefImage, bytes));
+  if (!ctx->dMseTemp) HIP_TRY(ctx, hipMalloc(&ctx->dMseTemp, ImageMseTempBytes() + 256));
+  HIP_TRY(ctx, hipMemcpy(ctx->dRefImage, rgb, bytes, hipMemcpyHostToDevice));
+  ctx->refW = width;
+  ctx->refH = height;
+  return GPUGI_OK;
+}
+
+int gpugi_compute_mse(gpugi_ctx* ctx, uint32_t divisor, float* mse) {
+  if (!ctx || !mse) return Fail(ctx, GPUGI_ERR_ARG, "gpugi_compute_mse: null argument");
+  *mse = INFINITY;  // no reference image: ComputeCurrentMSE returns +inf
+  if (!ctx->dRefImage) return GPUGI_OK;
+  if (!ctx->dAccum) return Fail(ctx, GPUGI_ERR_STATE, "gpugi_compute_mse: no screen set");
+  if (SetDev(ctx)) return GPUGI_ERR_HIP;
+  float* dOut = reinterpret_cast<float*>(static_cast<char*>(ctx->dMseTemp) + ImageMseTempBytes());
+  HIP_TRY(ctx, LaunchImageMse(static_cast<const float4*>(ctx->dAccum), ctx->width, ctx->height, divisor,
+                              static_cast<const float*>(ctx->dRefImage), ctx->refW, ctx->refH, ctx->dMseTemp, dOut,
+                              ctx->stream));
+  HIP_TRY(ctx, hipMemcpyAsync(mse, dOut, sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   return GPUGI_OK;
 }
 

@@ -36,4 +36,11 @@ hipError_t LaunchRaygenShadow(const gpugi_ray* prim, const gpugi_hit* hits, cons
                               const uint32_t* count, const float* lights, uint32_t numLights,
                               gpugi_ray* rays, size_t n, hipStream_t stream);
 
+// TextureMSE::ComputeCurrentMSE (texturemse.cpp:39-66): *out = mean over the
+// refW x refH reference pixels (RGB32F rows) of dot(d, d), d = accum / divisor
+// - reference, the accumulation buffer sampled nearest at each pixel centre.
+// temp holds ImageMseTempBytes() bytes.
+size_t ImageMseTempBytes();
+hipError_t LaunchImageMse(const float4* accum, int width, int height, uint32_t divisor, const float* ref,
+                          int refW, int refH, void* temp, float* out, hipStream_t stream);
 }  // namespace gpugi

@@ -84,6 +84,9 @@ const char* kHelp =
     "waitIterations = n                    draw n iterations\n"
     "waitSeconds = s                       draw until s seconds have passed\n"
     "screenshot / namedscreenshot = file   write the image (sum / iterations) as PFM\n"
+    "referenceImage = file.pfm             reference image of the MSE checks\n"
+    "mseCheckInterval = n                  compute and print the MSE every n-th iteration (0: never)\n"
+    "waitMSE = x                           draw until the MSE is below x (needs the two above)\n"
     "tic / toc                             reset / print the stopwatch\n"
     "help                                  this text\n"
     "exit                                  stop processing\n";
@@ -141,12 +144,22 @@ class App {
     } else if (c.name == "waitIterations") {
       NeedArgs(c, 1);
       const int n = AsInt(c.args[0]);
-      for (int i = 0; i < n; ++i) System().Draw();
+      for (int i = 0; i < n; ++i) Draw();
     } else if (c.name == "waitSeconds") {
       NeedArgs(c, 1);
       const double s = AsFloat(c.args[0]);
       const auto t0 = std::chrono::steady_clock::now();
-      while (std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() < s) System().Draw();
+      while (std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() < s) Draw();
+    } else if (c.name == "referenceImage") {
+      NeedArgs(c, 1);
+      Mse().LoadReferenceFromPFM(c.args[0]);
+      if (!Mse().HasValidReferenceImage()) std::cerr << "Error reading hdr image from " << c.args[0] << "\n";
+    } else if (c.name == "mseCheckInterval") {
+      NeedArgs(c, 1);
+      m_mseCheckInterval = AsInt(c.args[0]);
+    } else if (c.name == "waitMSE") {
+      NeedArgs(c, 1);
+      WaitMse(AsFloat(c.args[0]));
     } else if (c.name == "screenshot" || c.name == "namedscreenshot") {
       SaveImage(c.name == "namedscreenshot" && !c.args.empty() ? c.args[0] : std::string());
     } else {
@@ -159,6 +172,57 @@ class App {
   gpugi::RendererSystem& System() {
     if (!m_system) m_system.reset(new gpugi::RendererSystem(m_device));
     return *m_system;
+  }
+
+  gpugi::TextureMSE& Mse() {
+    if (!m_mse) m_mse.reset(new gpugi::TextureMSE(System()));
+    return *m_mse;
+  }
+
+  // Application::Draw (application.cpp:328-344): one iteration, then the MSE check.
+  void Draw() {
+    System().Draw();
+    MseChecks();
+  }
+
+  // Application::MSEChecks (application.cpp:346-376); returns the MSE when it was computed.
+  bool MseChecks(float* mseOut = nullptr) {
+    ++m_iterationSinceLastMseCheck;
+    if (m_mseCheckInterval <= 0 || m_iterationSinceLastMseCheck < (unsigned)m_mseCheckInterval) return false;
+    m_iterationSinceLastMseCheck = 0;
+    const float mse = Mse().ComputeCurrentMSE((unsigned)System().GetIterationCount());
+    std::cout << "Iteration " << System().GetIterationCount() << " MSE: " << mse << "\n";
+    if (mseOut) *mseOut = mse;
+    return true;
+  }
+
+  // The waitMSE listener (application.cpp:180-188) and the script pause it
+  // starts: without a window loop the iterations are drawn here until a check
+  // finds the MSE below the goal or NaN (or at once for a goal <= 0).
+  void WaitMse(float goal) {
+    if (m_mseCheckInterval <= 0) {
+      std::cerr << "Can't activate waitMSE as long the mseCheckInterval is 0!\n";
+      return;
+    }
+    if (!Mse().HasValidReferenceImage()) {
+      std::cerr << "Can't activate waitMSE as long as there is no reference image loaded!\n";
+      return;
+    }
+    if (!System().GetActiveRenderer()) throw std::invalid_argument("waitMSE: no renderer");
+    for (;;) {
+      System().Draw();
+      float mse = 0.0f;
+      if (!MseChecks(&mse)) continue;
+      if (!(goal > 0.0f)) return;
+      if (mse != mse) {
+        std::cerr << "MSE is NaN. Resuming script.\n";
+        return;
+      }
+      if (goal > mse) {
+        std::cout << "MSE now below " << goal << "! Resuming script\n";
+        return;
+      }
+    }
   }
 
   void LoadScene(const std::string& name) {
@@ -213,6 +277,9 @@ class App {
 
   int m_device;
   std::unique_ptr<gpugi::RendererSystem> m_system;
+  std::unique_ptr<gpugi::TextureMSE> m_mse;
+  int m_mseCheckInterval = 0;
+  unsigned m_iterationSinceLastMseCheck = 0;
   std::shared_ptr<gpugi::Scene> m_scene;
   gpugi::Camera m_camera;
   std::chrono::steady_clock::time_point m_tic = std::chrono::steady_clock::now();

@@ -3,6 +3,8 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <cstring>
+#include <fstream>
 #include <limits>
 
 namespace gpugi {
@@ -308,6 +310,54 @@ void RendererSystem::WritePfm(const std::string& filename) const {
     ok = std::fwrite(row.data(), sizeof(float), row.size(), f) == row.size();
   }
   if (std::fclose(f) != 0 || !ok) throw Error(GPUGI_ERR_IO, "WritePfm: write failed for " + filename);
+}
+
+// ---------------------------------------------------------------- MSE check
+
+bool LoadPfm(const std::string& filename, std::vector<float>* rgb, int* width, int* height) {
+  std::ifstream f(filename, std::ios::binary);
+  if (!f) return false;
+  char magic[3] = {0, 0, 0};
+  f.read(magic, 3);
+  if (!f || magic[0] != 'P' || magic[1] != 'F' || (magic[2] != '\n' && magic[2] != ' ' && magic[2] != '\r'))
+    return false;
+  long w = 0, h = 0;
+  double scale = 0.0;
+  f >> w >> h >> scale;
+  if (!f || w <= 0 || h <= 0 || scale == 0.0 || w > (1L << 20) || h > (1L << 20)) return false;
+  f.get();  // the single whitespace character that ends the header
+  std::vector<float> data((size_t)w * h * 3);
+  f.read(reinterpret_cast<char*>(data.data()), (std::streamsize)(data.size() * sizeof(float)));
+  if (!f) return false;
+  if (scale > 0.0) {  // big-endian file
+    for (float& v : data) {
+      uint32_t u;
+      std::memcpy(&u, &v, 4);
+      u = (u >> 24) | ((u >> 8) & 0xff00u) | ((u << 8) & 0xff0000u) | (u << 24);
+      std::memcpy(&v, &u, 4);
+    }
+  }
+  *rgb = std::move(data);
+  *width = (int)w;
+  *height = (int)h;
+  return true;
+}
+
+void TextureMSE::LoadReferenceFromPFM(const std::string& filename) {
+  std::vector<float> rgb;
+  int w = 0, h = 0;
+  if (!LoadPfm(filename, &rgb, &w, &h)) return;
+  gpugi_ctx* ctx = m_rs.GetContext();
+  Check(gpugi_set_reference_image(ctx, rgb.data(), w, h), ctx, "gpugi_set_reference_image");
+  m_hasReference = true;
+}
+
+float TextureMSE::ComputeCurrentMSE(unsigned int divisor) {
+  if (!m_hasReference) return std::numeric_limits<float>::infinity();
+  gpugi_ctx* ctx = m_rs.GetContext();
+  float mse = 0.0f;
+  Check(gpugi_compute_mse(ctx, divisor, &mse), ctx, "gpugi_compute_mse");
+  return mse;
 }
 
 }  // namespace gpugi

@@ -248,4 +248,27 @@ class RendererSystem {
   int m_traversalKernel = GPUGI_KERNEL_AUTO;
 };
 
+// LoadPfm (hdrimage.cpp:35-63): "PF", width and height, the scale line, then
+// width*height RGB float triples in file order (a positive scale marks
+// big-endian floats, which are swapped).  Returns false, leaving the outputs
+// untouched, when the file cannot be read or is not an RGB PFM image.
+bool LoadPfm(const std::string& filename, std::vector<float>* rgb, int* width, int* height);
+
+// TextureMSE (imageut/texturemse.hpp): mean squared error of a RendererSystem's
+// backbuffer against a reference image, computed on its device.
+class TextureMSE {
+ public:
+  explicit TextureMSE(RendererSystem& rs) : m_rs(rs) {}
+  bool HasValidReferenceImage() const { return m_hasReference; }
+  // texturemse.cpp:26-37: a file that cannot be loaded changes nothing.
+  void LoadReferenceFromPFM(const std::string& filename);
+  // texturemse.cpp:39-66: mean over the reference's pixels of
+  // |backbuffer / divisor - reference|^2; +inf without a reference image.
+  float ComputeCurrentMSE(unsigned int divisor);
+
+ private:
+  RendererSystem& m_rs;
+  bool m_hasReference = false;
+};
+
 }  // namespace gpugi

@@ -81,6 +81,8 @@ _SIGNATURES = {
     "gpugi_bpt_read_light_cache": [_P, _P, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32)],
     "gpugi_reset_accum": [_P],
     "gpugi_readback": [_P, _P, ctypes.c_size_t, ctypes.POINTER(ctypes.c_uint32)],
+    "gpugi_set_reference_image": [_P, _P, ctypes.c_int, ctypes.c_int],
+    "gpugi_compute_mse": [_P, ctypes.c_uint32, ctypes.POINTER(ctypes.c_float)],
     "gpugi_trace": [_P, ctypes.c_int, ctypes.c_int, _P, _P, ctypes.c_size_t, ctypes.c_int],
     "gpugi_trace_counts": [_P, ctypes.c_int, _P, _P, _P, ctypes.c_size_t, ctypes.c_int],
     "gpugi_set_wide_config": [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int],
@@ -430,6 +432,19 @@ class Context:
         it = ctypes.c_uint32()
         self._call("gpugi_readback", _ptr(out), ctypes.c_size_t(out.nbytes), ctypes.byref(it))
         return out, it.value
+
+    def set_reference_image(self, rgb):
+        """TextureMSE::LoadReferenceFromPFM's upload: rgb is (h, w, 3) float32 in PFM row order."""
+        rgb = np.ascontiguousarray(rgb, dtype=np.float32)
+        if rgb.ndim != 3 or rgb.shape[2] != 3:
+            raise ValueError("reference image must be (h, w, 3)")
+        self._call("gpugi_set_reference_image", _ptr(rgb), int(rgb.shape[1]), int(rgb.shape[0]))
+
+    def compute_mse(self, divisor):
+        """TextureMSE::ComputeCurrentMSE(backbuffer, divisor); +inf without a reference image."""
+        v = ctypes.c_float()
+        self._call("gpugi_compute_mse", int(divisor), ctypes.byref(v))
+        return v.value
 
     def trace(self, rays, query=CLOSEST_HIT, kernel=KERNEL_AUTO):
         rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)

@@ -207,6 +207,20 @@ int gpugi_reset_accum(gpugi_ctx* ctx);
  * the last reset. */
 int gpugi_readback(gpugi_ctx* ctx, float* rgba, size_t bytes, uint32_t* iterations);
 
+/* Reference image of the MSE check (TextureMSE::LoadReferenceFromPFM,
+ * texturemse.cpp:26-37): width*height RGB32F pixels in the row order LoadPfm
+ * returns (hdrimage.cpp:35-63).  Replaces any earlier reference image; its
+ * size may differ from the screen's. */
+int gpugi_set_reference_image(gpugi_ctx* ctx, const float* rgb, int width, int height);
+/* TextureMSE::ComputeCurrentMSE(backbuffer, divisor) (texturemse.cpp:39-66,
+ * shader/utils/sqerror.frag): the mean over the reference image's pixels of
+ * dot(d, d), d = accum.rgb / divisor - reference.rgb, with the accumulation
+ * buffer sampled nearest at each reference pixel's centre.  The mean is taken
+ * in double over all pixels (the reference averages a float mip chain).
+ * *mse = +inf when no reference image is set; divisor 0 gives inf/NaN as the
+ * shader's division does.  Blocks until the value is on the host. */
+int gpugi_compute_mse(gpugi_ctx* ctx, uint32_t divisor, float* mse);
+
 /* Ray queries for parity and the traversal microbenchmark (the GLSL
  * TraceRay/TraceRayAnyHit functions, traceray.glsl:18-39).  rays and hits
  * are host pointers when rays_on_device == 0 (the call then blocks until the

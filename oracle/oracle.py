@@ -217,6 +217,29 @@ def bpt_draw(scene, lights, cam, w, h, frame_seed, max_path_length, capacity, ac
     return accum, cache, int(n)
 
 
+def image_mse(accum, divisor, ref):
+    """TextureMSE::ComputeCurrentMSE (imageut/texturemse.cpp:39-66) with the
+    per-pixel error of shader/utils/sqerror.frag:13-17.
+
+    accum: (h, w, 4) float32 accumulation buffer; ref: (rh, rw, 3) float32.
+    Every reference pixel samples the render nearest at its centre (texcoord
+    (x + 0.5) / rw, the sampler the reference documents, texturemse.hpp:14-16),
+    diff = render.rgb / divisor - ref.rgb in float32, error = dot(diff, diff)
+    summed in the GLSL order x, y, z; the result is the mean over the reference's
+    pixels (taken in float64; the reference averages a float mip chain).
+    """
+    accum = np.asarray(accum, dtype=np.float32)
+    ref = np.asarray(ref, dtype=np.float32)
+    h, w = accum.shape[:2]
+    rh, rw = ref.shape[:2]
+    xs = ((2 * np.arange(rw, dtype=np.int64) + 1) * w) // (2 * rw)
+    ys = ((2 * np.arange(rh, dtype=np.int64) + 1) * h) // (2 * rh)
+    c = accum[ys[:, None], xs[None, :], :3]
+    d = c / np.float32(divisor) - ref
+    e = (d[..., 0] * d[..., 0] + d[..., 1] * d[..., 1]) + d[..., 2] * d[..., 2]
+    return np.float32(e.astype(np.float64).sum() / e.size)
+
+
 # ---- reference RNG built from the reference's own utilities/random.cpp ----
 _ref = None
 

@@ -27,7 +27,8 @@ def test_help_lists_every_command():
     r = run("help")
     assert r.returncode == 0
     for name in ("sceneFilename", "cameraPos", "cameraLookAt", "cameraFOV", "resolution", "pathLength",
-                 "renderer", "traversal", "addPointLight", "waitIterations", "namedscreenshot", "exit"):
+                 "renderer", "traversal", "addPointLight", "waitIterations", "namedscreenshot",
+                 "referenceImage", "mseCheckInterval", "waitMSE", "exit"):
         assert name in r.stdout
 
 
@@ -45,6 +46,8 @@ def test_comments_blank_lines_and_exit_stop_processing():
     ("resolution = 64", "expects 2 argument"),
     ("bvhType = 1", "only axis-aligned boxes"),
     ("cameraFOV = abc", "not a number"),
+    ("mseCheckInterval = x", "not an integer"),
+    ("waitMSE = 1, 2", "expects 1 argument"),
 ])
 def test_argument_errors(line, msg):
     r = run(line)
@@ -60,3 +63,10 @@ def test_missing_script_file():
 def test_point_light_needs_a_scene():
     r = run("addPointLight = 0, 1, 0, 1, 1, 1")
     assert r.returncode == 2 and "no scene loaded" in r.stderr
+
+
+def test_wait_mse_needs_a_check_interval():
+    # application.cpp:181-182: logged, and the script goes on (no context is created)
+    r = run("waitMSE = 0.5", "help")
+    assert r.returncode == 0
+    assert "mseCheckInterval is 0" in r.stderr and "sceneFilename" in r.stdout
