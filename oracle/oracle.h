@@ -80,6 +80,14 @@ void or_raygen_diffuse(const or_scene* s, const float* prim, const int32_t* prim
 void or_raygen_shadow(const float* prim, const int32_t* prim_hits, size_t n_prim, const float* lights,
                       uint32_t n_lights, float* rays, size_t first, size_t n);
 
+/* BSDF probes (__SampleBSDF / __BSDF, materials.glsl:118-391) for a 20-float
+ * material record: n samples as {dir3, throughput3, pdf, diffuse} and n
+ * evaluations at the given excident directions as {bsdf3, pdf}. */
+void or_bsdf_samples(const float* mat20, const float* inc, const float* N, uint32_t seed, int adjoint, size_t n,
+                     float* out);
+void or_bsdf_evals(const float* mat20, const float* inc, const float* N, int adjoint, const float* exc, size_t n,
+                   float* out);
+
 /* Integrators: one draw adds into accum (width x height x 4 floats). */
 void or_pathtracer(const or_scene* s, const float* lights, uint32_t num_lights, const or_camera* c, int width,
                    int height, uint32_t frame_seed, int max_path_length, float* accum, int row_begin, int row_end);

@@ -70,6 +70,10 @@ def lib():
         l.or_raygen_diffuse.restype = None
         l.or_raygen_shadow.argtypes = [P, P, ctypes.c_size_t, P, ctypes.c_uint32, P, ctypes.c_size_t, ctypes.c_size_t]
         l.or_raygen_shadow.restype = None
+        l.or_bsdf_samples.argtypes = [P, P, P, ctypes.c_uint32, ctypes.c_int, ctypes.c_size_t, P]
+        l.or_bsdf_samples.restype = None
+        l.or_bsdf_evals.argtypes = [P, P, P, ctypes.c_int, P, ctypes.c_size_t, P]
+        l.or_bsdf_evals.restype = None
         l.or_pathtracer.argtypes = [P, P, ctypes.c_uint32, P, ctypes.c_int, ctypes.c_int, ctypes.c_uint32,
                                     ctypes.c_int, P, ctypes.c_int, ctypes.c_int]
         l.or_pathtracer.restype = None
@@ -171,6 +175,35 @@ def raygen_shadow(prim, prim_hits, lights, n, first=0):
     l = np.ascontiguousarray(lights, dtype=np.float32).reshape(-1, 3)
     lib().or_raygen_shadow(_p(p), _p(h), len(p), _p(l), len(l), _p(out), first, n)
     return out
+
+
+def _mat20(material):
+    m = np.ascontiguousarray(material, dtype=np.float32).reshape(-1)
+    assert m.size == 20
+    return m
+
+
+def bsdf_samples(material, inc, normal, seed, n, adjoint=False):
+    """n draws of __SampleBSDF (materials.glsl:118-277): returns (dirs n x 3,
+    throughput n x 3, pdf n, diffuse-branch flag n)."""
+    out = np.zeros((n, 8), np.float32)
+    m = _mat20(material)
+    i = np.ascontiguousarray(inc, np.float32)
+    nn = np.ascontiguousarray(normal, np.float32)
+    lib().or_bsdf_samples(_p(m), _p(i), _p(nn), seed, int(adjoint), n, _p(out))
+    return out[:, 0:3], out[:, 3:6], out[:, 6], out[:, 7] > 0.5
+
+
+def bsdf_evals(material, inc, normal, exc, adjoint=False):
+    """__BSDF (materials.glsl:306-391) at the excident directions exc (n x 3):
+    returns (bsdf n x 3, pdf n)."""
+    e = np.ascontiguousarray(exc, np.float32).reshape(-1, 3)
+    out = np.zeros((len(e), 4), np.float32)
+    m = _mat20(material)
+    i = np.ascontiguousarray(inc, np.float32)
+    nn = np.ascontiguousarray(normal, np.float32)
+    lib().or_bsdf_evals(_p(m), _p(i), _p(nn), int(adjoint), _p(e), len(e), _p(out))
+    return out[:, 0:3], out[:, 3]
 
 
 def pathtrace(scene, lights, cam, w, h, frame_seed, max_path_length, accum=None, rows=None):

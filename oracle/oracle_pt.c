@@ -218,3 +218,53 @@ void or_pathtracer(const or_scene* s, const float* lights, uint32_t numLights, c
     }
   }
 }
+
+/* Batched probes of the two BSDF routines for the consistency tests: mat20 is
+ * one 20-float material record (the scene layout), N the shading normal.
+ * or_bsdf_samples: n draws of or_sample_bsdf for the incident direction `inc`
+ * from the stream seed, writing direction (3), throughput (3), pdf and the
+ * diffuse flag as 8 floats per draw.  or_bsdf_evals: or_bsdf for n excident
+ * directions (3 floats each), writing the BSDF (3) and pdf as 4 floats. */
+void or_bsdf_samples(const float* mat20, const float* inc, const float* N, uint32_t seed, int adjoint, size_t n,
+                     float* out) {
+  or_material m;
+  m.diffuse = v3(mat20[0], mat20[1], mat20[2]);
+  m.opacity = v3(mat20[3], mat20[4], mat20[5]);
+  m.refl[0] = mat20[6]; m.refl[1] = mat20[7]; m.refl[2] = mat20[8]; m.refl[3] = mat20[9];
+  m.emissivity = v3(mat20[10], mat20[11], mat20[12]);
+  m.fresnel0 = v3(mat20[13], mat20[14], mat20[15]);
+  m.rIndex = mat20[16];
+  m.fresnel1 = v3(mat20[17], mat20[18], mat20[19]);
+  const vec3 vi = v3(inc[0], inc[1], inc[2]), vn = v3(N[0], N[1], N[2]);
+  for (size_t i = 0; i < n; ++i) {
+    vec3 tp = v3(1, 1, 1);
+    float pdf;
+    int dif;
+    const vec3 d = or_sample_bsdf(vi, &m, &seed, vn, &tp, adjoint, &pdf, &dif);
+    float* o = out + i * 8;
+    o[0] = d.x; o[1] = d.y; o[2] = d.z;
+    o[3] = tp.x; o[4] = tp.y; o[5] = tp.z;
+    o[6] = pdf;
+    o[7] = (float)dif;
+  }
+}
+
+void or_bsdf_evals(const float* mat20, const float* inc, const float* N, int adjoint, const float* exc, size_t n,
+                   float* out) {
+  or_material m;
+  m.diffuse = v3(mat20[0], mat20[1], mat20[2]);
+  m.opacity = v3(mat20[3], mat20[4], mat20[5]);
+  m.refl[0] = mat20[6]; m.refl[1] = mat20[7]; m.refl[2] = mat20[8]; m.refl[3] = mat20[9];
+  m.emissivity = v3(mat20[10], mat20[11], mat20[12]);
+  m.fresnel0 = v3(mat20[13], mat20[14], mat20[15]);
+  m.rIndex = mat20[16];
+  m.fresnel1 = v3(mat20[17], mat20[18], mat20[19]);
+  const vec3 vi = v3(inc[0], inc[1], inc[2]), vn = v3(N[0], N[1], N[2]);
+  for (size_t i = 0; i < n; ++i) {
+    float pdf;
+    const vec3 f = or_bsdf(vi, v3(exc[3 * i], exc[3 * i + 1], exc[3 * i + 2]), &m, vn, &pdf, adjoint, 0);
+    float* o = out + i * 4;
+    o[0] = f.x; o[1] = f.y; o[2] = f.z;
+    o[3] = pdf;
+  }
+}

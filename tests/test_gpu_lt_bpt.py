@@ -127,3 +127,30 @@ def test_wide_traversal_integrators_match_oracle_statistically(rt, oracle, ctx, 
     img, _ = ctx.readback(W, H)
     assert close(img, acc).mean() > 0.98
     assert abs(img[..., :3].sum() / acc[..., :3].sum() - 1) < 2e-3
+
+
+@pytest.mark.parametrize("variant", ["glossy", "refractive"])
+def test_lt_specular_branches_match_oracle(rt, oracle, ctx, variant):
+    # adjoint sampling and evaluation of the reflect / refract lobes
+    from test_oracle_integrators import furnace_scene
+    params = {"glossy": dict(specular=(0.3, 0.3, 0.3), roughness=0.4),
+              "refractive": dict(specular=(0.1, 0.1, 0.1), roughness=0.2, opacity=0.4)}[variant]
+    hs = furnace_scene(rt, **params)
+    cam = rt.camera([0.1, 0.2, 0.3], [0.5, -0.3, -1.0], [0, 1, 0], 60, W, H)
+    ctx.load_scene(hs)
+    ctx.set_screen(W, H)
+    ctx.set_camera(cam)
+    ctx.set_max_path_length(L)
+    ctx.set_integrator_kernel(rt.KERNEL_REFERENCE)
+    sc = oracle.Scene(hs.arrays())
+    sampler = rt.LightSampler(hs, 123)
+    acc = np.zeros((H, W, 4), np.float32)
+    print("case: LT specular branch %s" % variant, flush=True)
+    ls = sampler.generate(256)
+    ctx.set_light_samples(ls)
+    ctx.draw(rt.LT, rt.wanghash(0))
+    oracle.lighttrace(sc, ls, cam, W, H, rt.wanghash(0), L, acc)
+    img, n = ctx.readback(W, H)
+    assert n == 1 and acc[..., :3].mean() > 0
+    ok = close(img, acc)
+    assert ok.all(), np.argwhere(~ok)[:5]

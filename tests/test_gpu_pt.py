@@ -55,3 +55,21 @@ def test_pt_wide_traversal_matches_oracle(rt, oracle, ctx):
     same = (img.view(np.uint32) == acc.view(np.uint32)).all(-1)
     assert same.mean() > 0.99
     assert abs(img[..., :3].mean() - acc[..., :3].mean()) <= 1e-3 * acc[..., :3].mean()
+
+
+@pytest.mark.parametrize("variant", ["mirror", "glossy", "refractive"])
+def test_pt_specular_branches_bitexact(rt, oracle, ctx, variant):
+    # the furnace cube with a mirror (roughness 0, no NEE), a Phong-glossy
+    # (exponent 1 / 0.4^2, with NEE) and a partly transparent material: the
+    # reflect and refract branches of the BSDF on the GPU against the oracle
+    from test_oracle_integrators import furnace_scene
+    params = {"mirror": dict(specular=(0.5, 0.5, 0.5), roughness=0.0),
+              "glossy": dict(specular=(0.3, 0.3, 0.3), roughness=0.4),
+              "refractive": dict(specular=(0.1, 0.1, 0.1), roughness=0.2, opacity=0.4)}[variant]
+    hs = furnace_scene(rt, **params)
+    cam = rt.camera([0.1, 0.2, 0.3], [0.5, -0.3, -1.0], [0, 1, 0], 60, 32, 32)
+    print("case: specular branch %s" % variant, flush=True)
+    img, acc = run_pt(rt, oracle, ctx, hs, cam, 32, 32, 2, rt.KERNEL_REFERENCE)
+    assert img[..., :3].mean() > 0.5
+    diff = np.nonzero(img.view(np.uint32) != acc.view(np.uint32))
+    assert len(diff[0]) == 0, (diff[0][:5], diff[1][:5])

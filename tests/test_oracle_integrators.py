@@ -15,7 +15,7 @@ import pytest
 from gpugi_amd import workloads
 
 
-def furnace_scene(rt):
+def furnace_scene(rt, **material):
     P = np.array([[x, y, z] for x in (-1, 1) for y in (-1, 1) for z in (-1, 1)], np.float32)
     faces = [(0, 1, 3, 2), (4, 6, 7, 5), (0, 4, 5, 1), (2, 3, 7, 6), (0, 2, 6, 4), (1, 5, 7, 3)]
     pos, nrm, idx = [], [], []
@@ -29,7 +29,9 @@ def furnace_scene(rt):
             nrm.append(n)
         # wound so that cross(v0 - v1, v0 - v2) points into the cube: the emission side
         idx += [[b, b + 2, b + 1], [b, b + 3, b + 2]]
-    m = rt.default_material("Furnace", albedo=(0.5, 0.5, 0.5), specular=(0, 0, 0), emissivity=(1.0, 2.0, 3.0))
+    args = dict(albedo=(0.5, 0.5, 0.5), specular=(0, 0, 0), emissivity=(1.0, 2.0, 3.0))
+    args.update(material)
+    m = rt.default_material("Furnace", **args)
     return rt.HostScene.build(np.array(pos), np.array(idx), normals=np.array(nrm), materials=[m])
 
 
@@ -47,6 +49,28 @@ def test_pt_white_furnace(rt, oracle, path_length):
     got = acc[..., :3].reshape(-1, 3).mean(0) / n
     want = np.array([1.0, 2.0, 3.0]) * sum(a ** k for k in range(path_length + 1))
     assert np.allclose(got, want, rtol=0.03), (got, want)
+
+
+@pytest.mark.parametrize("path_length", [1, 3])
+def test_pt_mirror_furnace(rt, oracle, path_length):
+    # Roughness 0 gives the exponent 1e20 >= 500: no next-event estimation
+    # (pathtracer.comp:81), so emission is counted at every vertex the path
+    # reaches; each bounce keeps rho = s + (1 - s) * a on average (mirror lobe
+    # with probability s, diffuse otherwise), and the last of the L vertices
+    # cannot see the emission of a next one: E * (1 + rho + ... + rho^(L-1)).
+    hs = furnace_scene(rt, specular=(0.5, 0.5, 0.5), roughness=0.0)
+    sc = oracle.Scene(hs.arrays())
+    a = float(np.float32(128) / np.float32(255))
+    rho = 0.5 + 0.5 * a
+    cam = rt.camera([0.1, 0.2, 0.3], [0.5, -0.3, -1.0], [0, 1, 0], 60, 16, 16)
+    sampler = rt.LightSampler(hs, 123)
+    acc = np.zeros((16, 16, 4), np.float32)
+    n = 200
+    for it in range(n):
+        oracle.pathtrace(sc, sampler.generate(128), cam, 16, 16, rt.wanghash(it), path_length, acc)
+    got = acc[..., :3].reshape(-1, 3).mean(0) / n
+    want = np.array([1.0, 2.0, 3.0]) * sum(rho ** k for k in range(path_length))
+    assert np.allclose(got, want, rtol=0.02), (got, want)
 
 
 def emitter_mask(rt, oracle, sc, cam, w, h):
