@@ -107,7 +107,12 @@ __global__ __launch_bounds__(kTileThreads) void k_cached_paths(DrawParams p, Cac
   gg::V3 d = gg::primary_direction(j0, j1, gx, gy, p.width, p.height, U, V, W);
   for (int i = 0; i < p.maxPathLength; ++i) {
     const ClosestHit h = trace_closest<WIDE>(c, o, d);
-    if (!h.hit) break;  // black environment
+    if (!h.hit) {  // the environment, as the path tracer adds it (pathtracer.comp:56-58)
+      const gg::V3 e = throughput * (p.env ? env_radiance(p.env, p.envSize, d) : gg::v3s(0.0f));
+      cacc = cacc + e;
+      if (!term) color = color + e;
+      break;
+    }
     o = o + h.t * d;
     const gg::V3 n = shading_normal(p.ref.vinfo, h.tri, h.bary);
     const MaterialData m = load_material(p.materials, h.tri.w);
@@ -260,6 +265,8 @@ __global__ void k_pi_shade(DrawParams p, CacheParams cp, float4* __restrict__ ra
     write_features(cp.queryFeat + (size_t)pix * kMlpRaw, cp, o, -d, n, m);
     reinterpret_cast<float4*>(cp.queryWeight)[pix] = make_float4(throughput.x, throughput.y, throughput.z, 1.0f);
   } else {
+    const gg::V3 d = gg::v3(r1.x, r1.y, r1.z);
+    color = color + throughput * (p.env ? env_radiance(p.env, p.envSize, d) : gg::v3s(0.0f));
     zero_features(cp.queryFeat + (size_t)pix * kMlpRaw);
     reinterpret_cast<float4*>(cp.queryWeight)[pix] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   }

@@ -2,6 +2,7 @@
 // device buffers and the microbenchmark ray generators.
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstring>
 #include <new>
 #include <set>
@@ -84,6 +85,9 @@ struct gpugi_ctx {
   float* dMlpLoss = nullptr;  // loss of the last gradient / training step
   void* dMlpIo = nullptr;     // host-pointer staging of features / targets / outputs
   size_t mlpIoCap = 0;
+  // environment cube map (linear RGB float4 texels, faces x-, x+, y-, y+, z-, z+)
+  void* dEnv = nullptr;
+  int envSize = 0;
   // reference image of the MSE check (RGB32F) and the reduction's block sums
   void* dRefImage = nullptr;
   int refW = 0, refH = 0;
@@ -248,7 +252,7 @@ void gpugi_destroy(gpugi_ctx* ctx) {
   if (ctx->stream) hipStreamSynchronize(ctx->stream);
   for (void** p : {&ctx->dNodes, &ctx->dSlots, &ctx->dPos, &ctx->dVinfo, &ctx->dMats, &ctx->dWideNodes,
                    &ctx->dWideTris, &ctx->dSpill, &ctx->dCounters, &ctx->dWideQueue, &ctx->dAccum, &ctx->dLights,
-                   &ctx->dScratch, &ctx->dRefImage, &ctx->dMseTemp})
+                   &ctx->dScratch, &ctx->dRefImage, &ctx->dMseTemp, &ctx->dEnv})
     FreeDev(*p);
   FreeIntegratorState(&ctx->integ);
   for (void** p : {(void**)&ctx->mlp.w, (void**)&ctx->mlp.m, (void**)&ctx->mlp.v, (void**)&ctx->mlp.wb,
@@ -596,6 +600,8 @@ static DrawParams MakeDrawParams(gpugi_ctx* ctx, uint32_t frame_seed) {
   p.watertight = Watertight(ctx);
   p.counters = (uint32_t*)ctx->dCounters;
   p.bptCapacity = ctx->bptCapacity;
+  p.env = (const float4*)ctx->dEnv;
+  p.envSize = ctx->envSize;
   return p;
 }
 
@@ -657,6 +663,33 @@ int gpugi_readback(gpugi_ctx* ctx, float* rgba, size_t bytes, uint32_t* iteratio
   HIP_TRY(ctx, hipMemcpyAsync(rgba, ctx->dAccum, need, hipMemcpyDeviceToHost, ctx->stream));
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   if (iterations) *iterations = ctx->iterations;
+  return GPUGI_OK;
+}
+
+int gpugi_set_environment_map(gpugi_ctx* ctx, int size, const uint8_t* faces) {
+  if (!ctx) return Fail(ctx, GPUGI_ERR_ARG, "gpugi_set_environment_map: null context");
+  if (size < 0 || size > 16384 || (size > 0 && !faces))
+    return Fail(ctx, GPUGI_ERR_ARG, "gpugi_set_environment_map: bad size or null faces");
+  if (SetDev(ctx)) return GPUGI_ERR_HIP;
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));  // draws in flight may still read the old map
+  if (ctx->side) HIP_TRY(ctx, hipStreamSynchronize(ctx->side));
+  FreeDev(ctx->dEnv);
+  ctx->envSize = 0;
+  if (size == 0) return GPUGI_OK;  // the black default map (renderersystem.cpp:39-47)
+  // SRGB8 texels are decoded to linear RGB before filtering, as GL does for
+  // sRGB textures: the sRGB transfer function evaluated in double per byte.
+  float lut[256];
+  for (int i = 0; i < 256; ++i) {
+    const double c = i / 255.0;
+    lut[i] = (float)(c <= 0.04045 ? c / 12.92 : std::pow((c + 0.055) / 1.055, 2.4));
+  }
+  const size_t n = (size_t)6 * size * size;
+  std::vector<float4> texels(n);
+  for (size_t i = 0; i < n; ++i)
+    texels[i] = make_float4(lut[faces[3 * i]], lut[faces[3 * i + 1]], lut[faces[3 * i + 2]], 1.0f);
+  HIP_TRY(ctx, hipMalloc(&ctx->dEnv, n * sizeof(float4)));
+  HIP_TRY(ctx, hipMemcpy(ctx->dEnv, texels.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+  ctx->envSize = size;
   return GPUGI_OK;
 }
 

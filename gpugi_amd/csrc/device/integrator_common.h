@@ -259,7 +259,8 @@ __device__ __forceinline__ bool pt_bounce(const DrawParams& p, const TraceCtx& c
   const float lightSampleWeight = (float)p.numLights / 1.0f;
   const ClosestHit h = trace_closest<WIDE>(c, s.o, s.d);
   if (!h.hit) {
-    s.color = s.color + s.throughput * gg::v3s(0.0f);  // black environment map (renderersystem.cpp:39-47)
+    // environment on a miss (pathtracer.comp:56-58); black by default (renderersystem.cpp:39-47)
+    s.color = s.color + s.throughput * (p.env ? env_radiance(p.env, p.envSize, s.d) : gg::v3s(0.0f));
     return true;
   }
   s.o = s.o + h.t * s.d;

@@ -25,6 +25,8 @@ struct DrawParams {
   int maxPathLength = 16;
   int rrStartDepth = 0;                   // Russian roulette from this bounce on (0: every bounce, as the reference)
   bool watertight = false;                // watertight triangle test instead of IntersectTriangle
+  const float4* env = nullptr;            // environment cube map: 6 faces x envSize^2 linear RGB texels, or none (black)
+  int envSize = 0;
   uint32_t* counters = nullptr;
   int bptCapacity = -1;                   // LightCacheCapacity (bidirectionalpathtracer.cpp:110-112)
 };

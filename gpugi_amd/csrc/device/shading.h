@@ -35,6 +35,49 @@ __device__ __forceinline__ MaterialData load_material(const gpugi_material* __re
   return d;
 }
 
+// textureLod(EnvironmentMap, direction, 0) of pathtracer.comp:57.  Face and
+// (s, t) follow the OpenGL cube-map selection table (major axis; ties go to x,
+// then y); faces are stored x-, x+, y-, y+, z-, z+, each size x size linear RGB
+// texels, row j holding t = (j + 0.5) / size.  Bilinear filtering with the
+// texel coordinates clamped to the face.  Directions that select no texel
+// (zero or NaN components) give black.
+__device__ __forceinline__ gg::V3 env_radiance(const float4* __restrict__ tex, int size, gg::V3 d) {
+  const float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+  int face;
+  float sc, tc, ma;
+  if (ax >= ay && ax >= az) {
+    ma = ax;
+    face = d.x >= 0.0f ? 1 : 0;
+    sc = d.x >= 0.0f ? -d.z : d.z;
+    tc = -d.y;
+  } else if (ay >= az) {
+    ma = ay;
+    face = d.y >= 0.0f ? 3 : 2;
+    sc = d.x;
+    tc = d.y >= 0.0f ? d.z : -d.z;
+  } else {
+    ma = az;
+    face = d.z >= 0.0f ? 5 : 4;
+    sc = d.z >= 0.0f ? d.x : -d.x;
+    tc = -d.y;
+  }
+  const float s = (sc / ma + 1.0f) * 0.5f;
+  const float t = (tc / ma + 1.0f) * 0.5f;
+  if (!(ma > 0.0f) || !(s >= 0.0f && s <= 1.0f && t >= 0.0f && t <= 1.0f)) return gg::v3s(0.0f);
+  const float u = s * (float)size - 0.5f, v = t * (float)size - 0.5f;
+  const float fu = floorf(u), fv = floorf(v);
+  const float a = u - fu, b = v - fv;
+  const int i0 = (int)fu, j0 = (int)fv;
+  const int x0 = max(i0, 0), x1 = min(i0 + 1, size - 1), y0 = max(j0, 0), y1 = min(j0 + 1, size - 1);
+  const float4* f = tex + (size_t)face * size * size;
+  const float4 t00 = f[(size_t)y0 * size + x0], t10 = f[(size_t)y0 * size + x1];
+  const float4 t01 = f[(size_t)y1 * size + x0], t11 = f[(size_t)y1 * size + x1];
+  const float ia = 1.0f - a, ib = 1.0f - b;
+  return gg::v3((t00.x * ia + t10.x * a) * ib + (t01.x * ia + t11.x * a) * b,
+                (t00.y * ia + t10.y * a) * ib + (t01.y * ia + t11.y * a) * b,
+                (t00.z * ia + t10.z * a) * ib + (t01.z * ia + t11.z * a) * b);
+}
+
 // AvgProbability / Avg (materials.glsl:33-41).
 __device__ __forceinline__ float avg_probability(gg::V3 c) {
   return (c.x + c.y + c.z + gg::kDivisorEpsilon) / (3.0f + gg::kDivisorEpsilon);

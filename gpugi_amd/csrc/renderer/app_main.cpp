@@ -84,6 +84,7 @@ const char* kHelp =
     "waitIterations = n                    draw n iterations\n"
     "waitSeconds = s                       draw until s seconds have passed\n"
     "screenshot / namedscreenshot = file   write the image (sum / iterations) as PFM\n"
+    "envMap = size, x-, x+, y-, y+, z-, z+  environment cube map from six size x size binary PPM faces\n"
     "referenceImage = file.pfm             reference image of the MSE checks\n"
     "mseCheckInterval = n                  compute and print the MSE every n-th iteration (0: never)\n"
     "waitMSE = x                           draw until the MSE is below x (needs the two above)\n"
@@ -150,6 +151,10 @@ class App {
       const double s = AsFloat(c.args[0]);
       const auto t0 = std::chrono::steady_clock::now();
       while (std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() < s) Draw();
+    } else if (c.name == "envMap") {
+      NeedArgs(c, 7);
+      System().SetEnvironmentMap(AsInt(c.args[0]), c.args[1], c.args[2], c.args[3], c.args[4], c.args[5],
+                                 c.args[6]);
     } else if (c.name == "referenceImage") {
       NeedArgs(c, 1);
       Mse().LoadReferenceFromPFM(c.args[0]);

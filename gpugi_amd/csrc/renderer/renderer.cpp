@@ -2,7 +2,9 @@
 #include "renderer.hpp"
 
 #include <algorithm>
+#include <cctype>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <limits>
@@ -310,6 +312,55 @@ void RendererSystem::WritePfm(const std::string& filename) const {
     ok = std::fwrite(row.data(), sizeof(float), row.size(), f) == row.size();
   }
   if (std::fclose(f) != 0 || !ok) throw Error(GPUGI_ERR_IO, "WritePfm: write failed for " + filename);
+}
+
+// ---------------------------------------------------------------- environment map
+
+namespace {
+
+// Binary PPM (P6): magic, width, height, maxval (whitespace separated, '#'
+// comments allowed), one whitespace character, then width * height RGB bytes.
+bool LoadPpmRgb8(const std::string& filename, int size, uint8_t* out) {
+  std::ifstream f(filename, std::ios::binary);
+  if (!f) return false;
+  auto token = [&f](std::string* t) {
+    t->clear();
+    int c;
+    while ((c = f.get()) != EOF) {
+      if (c == '#') {
+        while ((c = f.get()) != EOF && c != '\n') {
+        }
+      } else if (!std::isspace(c)) {
+        break;
+      }
+    }
+    while (c != EOF && !std::isspace(c)) {
+      t->push_back((char)c);
+      c = f.get();
+    }
+    return !t->empty();
+  };
+  std::string magic, w, h, maxval;
+  if (!token(&magic) || magic != "P6" || !token(&w) || !token(&h) || !token(&maxval)) return false;
+  if (std::atoi(w.c_str()) != size || std::atoi(h.c_str()) != size || std::atoi(maxval.c_str()) != 255) return false;
+  f.read(reinterpret_cast<char*>(out), (std::streamsize)size * size * 3);
+  return (bool)f;
+}
+
+}  // namespace
+
+void RendererSystem::SetEnvironmentMap(int size, const std::string& xneg, const std::string& xpos,
+                                       const std::string& yneg, const std::string& ypos, const std::string& zneg,
+                                       const std::string& zpos) {
+  if (size <= 0) throw Error(GPUGI_ERR_ARG, "SetEnvironmentMap: size must be positive");
+  const std::string* files[6] = {&xneg, &xpos, &yneg, &ypos, &zneg, &zpos};
+  const size_t faceBytes = (size_t)size * size * 3;
+  std::vector<uint8_t> faces(6 * faceBytes);
+  for (int k = 0; k < 6; ++k)
+    if (!LoadPpmRgb8(*files[k], size, faces.data() + k * faceBytes))
+      throw Error(GPUGI_ERR_IO, "SetEnvironmentMap: cannot read a " + std::to_string(size) + "x" +
+                                    std::to_string(size) + " binary PPM from " + *files[k]);
+  Check(gpugi_set_environment_map(m_ctx, size, faces.data()), m_ctx, "gpugi_set_environment_map");
 }
 
 // ---------------------------------------------------------------- MSE check

@@ -217,6 +217,14 @@ class RendererSystem {
   void SetTraversalKernel(int kernel) { m_traversalKernel = kernel; }
   int GetTraversalKernel() const { return m_traversalKernel; }
 
+  // RendererSystem::SetEnvironmentMap (renderersystem.cpp:173-186): six
+  // size x size faces in the order x-, x+, y-, y+, z-, z+, here binary PPM
+  // (P6, maxval 255) files holding sRGB RGB8 texels, the first file row being
+  // face row t = 0.5 / size.  Throws Error(GPUGI_ERR_IO) for a file that cannot
+  // be read or has another size.
+  void SetEnvironmentMap(int size, const std::string& xneg, const std::string& xpos, const std::string& yneg,
+                         const std::string& ypos, const std::string& zneg, const std::string& zpos);
+
   int GetWidth() const { return m_width; }
   int GetHeight() const { return m_height; }
   uint32_t GetFrameSeed() const { return m_frameSeed; }

@@ -82,6 +82,7 @@ _SIGNATURES = {
     "gpugi_reset_accum": [_P],
     "gpugi_readback": [_P, _P, ctypes.c_size_t, ctypes.POINTER(ctypes.c_uint32)],
     "gpugi_set_reference_image": [_P, _P, ctypes.c_int, ctypes.c_int],
+    "gpugi_set_environment_map": [_P, ctypes.c_int, _P],
     "gpugi_compute_mse": [_P, ctypes.c_uint32, ctypes.POINTER(ctypes.c_float)],
     "gpugi_trace": [_P, ctypes.c_int, ctypes.c_int, _P, _P, ctypes.c_size_t, ctypes.c_int],
     "gpugi_trace_counts": [_P, ctypes.c_int, _P, _P, _P, ctypes.c_size_t, ctypes.c_int],
@@ -432,6 +433,17 @@ class Context:
         it = ctypes.c_uint32()
         self._call("gpugi_readback", _ptr(out), ctypes.c_size_t(out.nbytes), ctypes.byref(it))
         return out, it.value
+
+    def set_environment_map(self, faces):
+        """RendererSystem::SetEnvironmentMap: faces (6, size, size, 3) uint8 sRGB in the
+        order x-, x+, y-, y+, z-, z+, or None for the black default."""
+        if faces is None:
+            self._call("gpugi_set_environment_map", 0, None)
+            return
+        f = np.ascontiguousarray(faces, dtype=np.uint8)
+        if f.ndim != 4 or f.shape[0] != 6 or f.shape[1] != f.shape[2] or f.shape[3] != 3:
+            raise ValueError("environment faces must be (6, size, size, 3)")
+        self._call("gpugi_set_environment_map", int(f.shape[1]), _ptr(f))
 
     def set_reference_image(self, rgb):
         """TextureMSE::LoadReferenceFromPFM's upload: rgb is (h, w, 3) float32 in PFM row order."""

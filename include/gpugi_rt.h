@@ -207,6 +207,19 @@ int gpugi_reset_accum(gpugi_ctx* ctx);
  * the last reset. */
 int gpugi_readback(gpugi_ctx* ctx, float* rgba, size_t bytes, uint32_t* iterations);
 
+/* RendererSystem::SetEnvironmentMap (renderersystem.cpp:173-186): six square
+ * faces of size x size RGB8 texels, sRGB-encoded like the reference's SRGB8
+ * cube map, in the order x-, x+, y-, y+, z-, z+ of the envMap script command
+ * (application.cpp:151-155); texel (i, j) of a face is faces[((face * size +
+ * j) * size + i) * 3 + c], the texel centred at (s, t) = ((i + 0.5) / size,
+ * (j + 0.5) / size) of the OpenGL cube-map face coordinates.  The path tracers
+ * add throughput * environment(direction) when a ray leaves the scene
+ * (pathtracer.comp:56-58), with the face chosen by the major axis of the
+ * direction and bilinear filtering clamped to the face.  size 0 (faces NULL)
+ * restores the black default (renderersystem.cpp:39-47).  Does not reset the
+ * accumulation, as the reference does not. */
+int gpugi_set_environment_map(gpugi_ctx* ctx, int size, const uint8_t* faces);
+
 /* Reference image of the MSE check (TextureMSE::LoadReferenceFromPFM,
  * texturemse.cpp:26-37): width*height RGB32F pixels in the row order LoadPfm
  * returns (hdrimage.cpp:35-63).  Replaces any earlier reference image; its

@@ -88,6 +88,13 @@ void or_bsdf_samples(const float* mat20, const float* inc, const float* N, uint3
 void or_bsdf_evals(const float* mat20, const float* inc, const float* N, int adjoint, const float* exc, size_t n,
                    float* out);
 
+/* Environment cube map used by or_pathtracer on a miss (NULL / 0: black):
+ * texels as 6 faces (x-, x+, y-, y+, z-, z+) of size x size linear RGB, 4
+ * floats each; the array must outlive the draws. */
+void or_set_environment(const float* texels, int size);
+/* The cube-map lookup for n directions (3 floats each) into out (3 floats each). */
+void or_env_lookups(const float* tex, int size, const float* dirs, size_t n, float* out);
+
 /* Integrators: one draw adds into accum (width x height x 4 floats). */
 void or_pathtracer(const or_scene* s, const float* lights, uint32_t num_lights, const or_camera* c, int width,
                    int height, uint32_t frame_seed, int max_path_length, float* accum, int row_begin, int row_end);

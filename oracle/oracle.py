@@ -74,6 +74,8 @@ def lib():
         l.or_bsdf_samples.restype = None
         l.or_bsdf_evals.argtypes = [P, P, P, ctypes.c_int, P, ctypes.c_size_t, P]
         l.or_bsdf_evals.restype = None
+        l.or_set_environment.argtypes, l.or_set_environment.restype = [P, ctypes.c_int], None
+        l.or_env_lookups.argtypes, l.or_env_lookups.restype = [P, ctypes.c_int, P, ctypes.c_size_t, P], None
         l.or_pathtracer.argtypes = [P, P, ctypes.c_uint32, P, ctypes.c_int, ctypes.c_int, ctypes.c_uint32,
                                     ctypes.c_int, P, ctypes.c_int, ctypes.c_int]
         l.or_pathtracer.restype = None
@@ -204,6 +206,43 @@ def bsdf_evals(material, inc, normal, exc, adjoint=False):
     nn = np.ascontiguousarray(normal, np.float32)
     lib().or_bsdf_evals(_p(m), _p(i), _p(nn), int(adjoint), _p(e), len(e), _p(out))
     return out[:, 0:3], out[:, 3]
+
+
+_env_keep = None
+
+
+def set_environment(texels):
+    """The path tracer's environment map: texels (6, size, size, 4) float32
+    linear RGB (faces x-, x+, y-, y+, z-, z+), or None for the black default."""
+    global _env_keep
+    if texels is None:
+        lib().or_set_environment(None, 0)
+        _env_keep = None
+        return
+    t = np.ascontiguousarray(texels, dtype=np.float32)
+    assert t.ndim == 4 and t.shape[0] == 6 and t.shape[1] == t.shape[2] and t.shape[3] == 4
+    _env_keep = t
+    lib().or_set_environment(_p(t), t.shape[1])
+
+
+def env_lookups(texels, dirs):
+    """The cube-map lookup of the path tracer's miss for directions dirs (n x 3)."""
+    t = np.ascontiguousarray(texels, dtype=np.float32)
+    d = np.ascontiguousarray(dirs, dtype=np.float32).reshape(-1, 3)
+    out = np.zeros((len(d), 3), np.float32)
+    lib().or_env_lookups(_p(t), t.shape[1], _p(d), len(d), _p(out))
+    return out
+
+
+def srgb_decode(faces):
+    """RGB8 sRGB faces (6, size, size, 3) -> linear float texels (6, size, size, 4),
+    the sRGB transfer function in double rounded to float per byte value."""
+    c = np.arange(256, dtype=np.float64) / 255.0
+    lut = np.where(c <= 0.04045, c / 12.92, ((c + 0.055) / 1.055) ** 2.4).astype(np.float32)
+    f = np.asarray(faces, dtype=np.uint8)
+    out = np.ones(f.shape[:3] + (4,), np.float32)
+    out[..., :3] = lut[f]
+    return out
 
 
 def pathtrace(scene, lights, cam, w, h, frame_seed, max_path_length, accum=None, rows=None):

@@ -126,6 +126,55 @@ vec3 or_shading_normal(const or_scene* s, const int32_t* tri, vec3 b) {
   return vnormalize(n);
 }
 
+/* Environment cube map of the path tracer (pathtracer.comp:56-58,
+ * renderersystem.cpp:39-47, 173-186): NULL means the black default.  Texels are
+ * linear RGB, 4 floats each, faces x-, x+, y-, y+, z-, z+. */
+static const float* g_env = NULL;
+static int g_envSize = 0;
+
+void or_set_environment(const float* texels, int size) {
+  g_env = size > 0 ? texels : NULL;
+  g_envSize = size;
+}
+
+/* The OpenGL cube-map selection (major axis, ties to x then y; per-face
+ * (sc, tc, ma) of the specification's table), s = (sc / |ma| + 1) / 2,
+ * t = (tc / |ma| + 1) / 2, then bilinear filtering over texel centres with the
+ * coordinates clamped to the face. */
+vec3 or_env_lookup(const float* tex, int size, vec3 d) {
+  float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z), sc, tc, ma;
+  int face;
+  if (ax >= ay && ax >= az) {
+    ma = ax;
+    if (d.x >= 0.0f) { face = 1; sc = -d.z; } else { face = 0; sc = d.z; }
+    tc = -d.y;
+  } else if (ay >= az) {
+    ma = ay;
+    sc = d.x;
+    if (d.y >= 0.0f) { face = 3; tc = d.z; } else { face = 2; tc = -d.z; }
+  } else {
+    ma = az;
+    if (d.z >= 0.0f) { face = 5; sc = d.x; } else { face = 4; sc = -d.x; }
+    tc = -d.y;
+  }
+  float s = (sc / ma + 1.0f) * 0.5f, t = (tc / ma + 1.0f) * 0.5f;
+  if (!(ma > 0.0f) || !(s >= 0.0f && s <= 1.0f && t >= 0.0f && t <= 1.0f)) return v3(0, 0, 0);
+  float u = s * (float)size - 0.5f, v = t * (float)size - 0.5f;
+  float fu = floorf(u), fv = floorf(v);
+  float a = u - fu, b = v - fv, ia = 1.0f - a, ib = 1.0f - b;
+  int i0 = (int)fu, j0 = (int)fv;
+  int x0 = i0 < 0 ? 0 : i0, x1 = i0 + 1 > size - 1 ? size - 1 : i0 + 1;
+  int y0 = j0 < 0 ? 0 : j0, y1 = j0 + 1 > size - 1 ? size - 1 : j0 + 1;
+  const float* f = tex + (size_t)face * size * size * 4;
+  const float* t00 = f + ((size_t)y0 * size + x0) * 4;
+  const float* t10 = f + ((size_t)y0 * size + x1) * 4;
+  const float* t01 = f + ((size_t)y1 * size + x0) * 4;
+  const float* t11 = f + ((size_t)y1 * size + x1) * 4;
+  float r[3];
+  for (int c = 0; c < 3; ++c) r[c] = (t00[c] * ia + t10[c] * a) * ib + (t01[c] * ia + t11[c] * a) * b;
+  return v3(r[0], r[1], r[2]);
+}
+
 int or_any_hit(const or_scene* s, vec3 o, vec3 d, float len) {
   int32_t slot;
   vec3 b;
@@ -189,7 +238,7 @@ void or_pathtracer(const or_scene* s, const float* lights, uint32_t numLights, c
         vec3 b, gn;
         or_trace_one(s, 0, o, d, &t, &slot, &b, &gn, tri, NULL, NULL);
         if (t == 3.402823466e+38f) {
-          color = vadd(color, vmul(thr, v3(0, 0, 0)));
+          color = vadd(color, vmul(thr, g_env ? or_env_lookup(g_env, g_envSize, d) : v3(0, 0, 0)));
           break;
         }
         o = vadd(o, vscale(t, d));
@@ -266,5 +315,14 @@ void or_bsdf_evals(const float* mat20, const float* inc, const float* N, int adj
     float* o = out + i * 4;
     o[0] = f.x; o[1] = f.y; o[2] = f.z;
     o[3] = pdf;
+  }
+}
+
+void or_env_lookups(const float* tex, int size, const float* dirs, size_t n, float* out) {
+  for (size_t i = 0; i < n; ++i) {
+    vec3 r = or_env_lookup(tex, size, v3(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]));
+    out[3 * i] = r.x;
+    out[3 * i + 1] = r.y;
+    out[3 * i + 2] = r.z;
   }
 }

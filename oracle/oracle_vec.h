@@ -103,6 +103,7 @@ or_material or_mat_of(const or_scene* s, int id);
 vec3 or_sample_bsdf(vec3 inc, const or_material* m, uint32_t* seed, vec3 N, vec3* tp, int adjoint, float* pdf,
                     int* diffuseOut);
 vec3 or_bsdf(vec3 inc, vec3 exc, const or_material* m, vec3 N, float* pdf, int adjoint, int specularOnly);
+vec3 or_env_lookup(const float* tex, int size, vec3 d);
 vec3 or_shading_normal(const or_scene* s, const int32_t* tri, vec3 b);
 int or_any_hit(const or_scene* s, vec3 o, vec3 d, float len);
 

@@ -28,7 +28,7 @@ def test_help_lists_every_command():
     assert r.returncode == 0
     for name in ("sceneFilename", "cameraPos", "cameraLookAt", "cameraFOV", "resolution", "pathLength",
                  "renderer", "traversal", "addPointLight", "waitIterations", "namedscreenshot",
-                 "referenceImage", "mseCheckInterval", "waitMSE", "exit"):
+                 "referenceImage", "mseCheckInterval", "waitMSE", "envMap", "exit"):
         assert name in r.stdout
 
 
@@ -48,6 +48,7 @@ def test_comments_blank_lines_and_exit_stop_processing():
     ("cameraFOV = abc", "not a number"),
     ("mseCheckInterval = x", "not an integer"),
     ("waitMSE = 1, 2", "expects 1 argument"),
+    ("envMap = 4, a, b", "expects 7 argument"),
 ])
 def test_argument_errors(line, msg):
     r = run(line)
