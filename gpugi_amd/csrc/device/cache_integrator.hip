@@ -115,7 +115,7 @@ __global__ __launch_bounds__(kTileThreads) void k_cached_paths(DrawParams p, Cac
     }
     o = o + h.t * d;
     const gg::V3 n = shading_normal(p.ref.vinfo, h.tri, h.bary);
-    const MaterialData m = load_material(p.materials, h.tri.w);
+    const MaterialData m = load_material_hit(p.materials, p.tex, p.ref.vinfo, h.tri, h.bary);
     const float cosIn = fmaxf(fabsf(gg::dot(n, d)), 1e-4f);
     const float dist2 = h.t * h.t;
     if (i == 0) a0 = sqrtf(dist2 / (4.0f * gg::kPI * cosIn));
@@ -250,7 +250,7 @@ __global__ void k_pi_shade(DrawParams p, CacheParams cp, float4* __restrict__ ra
     const gg::V3 d = gg::v3(r1.x, r1.y, r1.z);
     const gg::V3 o = gg::v3(r0.x, r0.y, r0.z) + h.t * d;
     const gg::V3 n = shading_normal(p.ref.vinfo, h.tri, h.bary);
-    const MaterialData m = load_material(p.materials, h.tri.w);
+    const MaterialData m = load_material_hit(p.materials, p.tex, p.ref.vinfo, h.tri, h.bary);
     color = color + throughput * m.emissivity;  // the path's first vertex follows the camera, no NEE before it
     const bool didNEE = m.refl[3] < 500.0f;
     uint32_t seed = seeds[r];

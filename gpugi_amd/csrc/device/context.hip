@@ -85,6 +85,14 @@ struct gpugi_ctx {
   float* dMlpLoss = nullptr;  // loss of the last gradient / training step
   void* dMlpIo = nullptr;     // host-pointer staging of features / targets / outputs
   size_t mlpIoCap = 0;
+  // material textures: device texels per texture, their descriptors, and per
+  // material the texture of each slot (-1: the record's constant)
+  std::vector<void*> texBufs;
+  std::vector<TextureDesc> texDescs;
+  std::vector<int4> matTex;
+  void* dTexDesc = nullptr;
+  void* dMatTex = nullptr;
+  bool anyTextureBound = false;
   // environment cube map (linear RGB float4 texels, faces x-, x+, y-, y+, z-, z+)
   void* dEnv = nullptr;
   int envSize = 0;
@@ -255,6 +263,9 @@ void gpugi_destroy(gpugi_ctx* ctx) {
                    &ctx->dScratch, &ctx->dRefImage, &ctx->dMseTemp, &ctx->dEnv})
     FreeDev(*p);
   FreeIntegratorState(&ctx->integ);
+  for (void* p : ctx->texBufs) hipFree(p);
+  FreeDev(ctx->dTexDesc);
+  FreeDev(ctx->dMatTex);
   for (void** p : {(void**)&ctx->mlp.w, (void**)&ctx->mlp.m, (void**)&ctx->mlp.v, (void**)&ctx->mlp.wb,
                    (void**)&ctx->mlp.wtb, (void**)&ctx->mlp.partials, (void**)&ctx->mlp.grad,
                    (void**)&ctx->mlp.loss, (void**)&ctx->dMlpLoss, &ctx->dMlpIo, &ctx->dCache})
@@ -302,9 +313,18 @@ int gpugi_load_scene(gpugi_ctx* ctx, const gpugi_scene_desc* d) {
   }
   if (SetDev(ctx)) return GPUGI_ERR_HIP;
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  if (ctx->side) HIP_TRY(ctx, hipStreamSynchronize(ctx->side));
   for (void** p : {&ctx->dNodes, &ctx->dSlots, &ctx->dPos, &ctx->dVinfo, &ctx->dMats, &ctx->dWideNodes,
                    &ctx->dWideTris, &ctx->dSpill})
     FreeDev(*p);
+  // textures belong to the scene's materials
+  for (void* p : ctx->texBufs) hipFree(p);
+  ctx->texBufs.clear();
+  ctx->texDescs.clear();
+  ctx->matTex.assign(d->num_materials, make_int4(-1, -1, -1, -1));
+  ctx->anyTextureBound = false;
+  FreeDev(ctx->dTexDesc);
+  FreeDev(ctx->dMatTex);
   ctx->spillBytes = 0;
   ctx->hasScene = ctx->hasWide = false;
   const size_t bNodes = (size_t)nn * 32, bSlots = (size_t)nl * 2 * 16, bPos = (size_t)nv * 12 + 4,
@@ -600,6 +620,10 @@ static DrawParams MakeDrawParams(gpugi_ctx* ctx, uint32_t frame_seed) {
   p.watertight = Watertight(ctx);
   p.counters = (uint32_t*)ctx->dCounters;
   p.bptCapacity = ctx->bptCapacity;
+  if (ctx->anyTextureBound) {
+    p.tex.matTex = (const int4*)ctx->dMatTex;
+    p.tex.desc = (const TextureDesc*)ctx->dTexDesc;
+  }
   p.env = (const float4*)ctx->dEnv;
   p.envSize = ctx->envSize;
   return p;
@@ -666,6 +690,85 @@ int gpugi_readback(gpugi_ctx* ctx, float* rgba, size_t bytes, uint32_t* iteratio
   return GPUGI_OK;
 }
 
+// Re-uploads the texture descriptors and the material bindings.
+static int UploadTextureTables(gpugi_ctx* ctx) {
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  if (ctx->side) HIP_TRY(ctx, hipStreamSynchronize(ctx->side));
+  FreeDev(ctx->dTexDesc);
+  FreeDev(ctx->dMatTex);
+  ctx->anyTextureBound = false;
+  for (const int4& t : ctx->matTex)
+    if (t.x >= 0 || t.y >= 0 || t.z >= 0 || t.w >= 0) ctx->anyTextureBound = true;
+  if (!ctx->anyTextureBound) return GPUGI_OK;
+  HIP_TRY(ctx, hipMalloc(&ctx->dTexDesc, ctx->texDescs.size() * sizeof(TextureDesc)));
+  HIP_TRY(ctx, hipMemcpy(ctx->dTexDesc, ctx->texDescs.data(), ctx->texDescs.size() * sizeof(TextureDesc),
+                         hipMemcpyHostToDevice));
+  HIP_TRY(ctx, hipMalloc(&ctx->dMatTex, ctx->matTex.size() * sizeof(int4)));
+  HIP_TRY(ctx, hipMemcpy(ctx->dMatTex, ctx->matTex.data(), ctx->matTex.size() * sizeof(int4), hipMemcpyHostToDevice));
+  return GPUGI_OK;
+}
+
+// The sRGB transfer function per byte value, evaluated in double.
+static void SrgbTable(float lut[256]) {
+  for (int i = 0; i < 256; ++i) {
+    const double c = i / 255.0;
+    lut[i] = (float)(c <= 0.04045 ? c / 12.92 : std::pow((c + 0.055) / 1.055, 2.4));
+  }
+}
+
+int gpugi_add_texture(gpugi_ctx* ctx, int width, int height, int format, const void* texels, int* id) {
+  if (!ctx || !texels || !id) return Fail(ctx, GPUGI_ERR_ARG, "gpugi_add_texture: null argument");
+  if (!ctx->hasScene) return Fail(ctx, GPUGI_ERR_STATE, "gpugi_add_texture: load a scene first");
+  if (width <= 0 || height <= 0 || width > 32768 || height > 32768)
+    return Fail(ctx, GPUGI_ERR_ARG, "gpugi_add_texture: bad size");
+  if (format != GPUGI_TEXTURE_RGBA8_SRGB && format != GPUGI_TEXTURE_RGBA32F)
+    return Fail(ctx, GPUGI_ERR_ARG, "gpugi_add_texture: unknown format");
+  if (SetDev(ctx)) return GPUGI_ERR_HIP;
+  const size_t n = (size_t)width * height;
+  std::vector<float4> lin;
+  try {
+    lin.resize(n);
+  } catch (const std::bad_alloc&) {
+    return Fail(ctx, GPUGI_ERR_NOMEM, "gpugi_add_texture: host staging");
+  }
+  if (format == GPUGI_TEXTURE_RGBA8_SRGB) {
+    // SRGB8_ALPHA8: colour channels decoded before filtering, alpha linear
+    float lut[256];
+    SrgbTable(lut);
+    const uint8_t* b = static_cast<const uint8_t*>(texels);
+    for (size_t i = 0; i < n; ++i)
+      lin[i] = make_float4(lut[b[4 * i]], lut[b[4 * i + 1]], lut[b[4 * i + 2]], (float)b[4 * i + 3] / 255.0f);
+  } else {
+    std::memcpy(lin.data(), texels, n * sizeof(float4));
+  }
+  void* buf = nullptr;
+  HIP_TRY(ctx, hipMalloc(&buf, n * sizeof(float4)));
+  const hipError_t e = hipMemcpy(buf, lin.data(), n * sizeof(float4), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    hipFree(buf);
+    return Fail(ctx, GPUGI_ERR_HIP, std::string("gpugi_add_texture: ") + hipGetErrorString(e));
+  }
+  ctx->texBufs.push_back(buf);
+  ctx->texDescs.push_back(TextureDesc{static_cast<const float4*>(buf), width, height});
+  *id = (int)ctx->texDescs.size() - 1;
+  return UploadTextureTables(ctx);
+}
+
+int gpugi_set_material_texture(gpugi_ctx* ctx, uint32_t material, int slot, int texture) {
+  if (!ctx) return Fail(ctx, GPUGI_ERR_ARG, "gpugi_set_material_texture: null context");
+  if (!ctx->hasScene) return Fail(ctx, GPUGI_ERR_STATE, "gpugi_set_material_texture: load a scene first");
+  if (material >= ctx->matTex.size()) return Fail(ctx, GPUGI_ERR_ARG, "gpugi_set_material_texture: bad material");
+  if (slot < GPUGI_TEX_DIFFUSE || slot > GPUGI_TEX_EMISSIVITY)
+    return Fail(ctx, GPUGI_ERR_ARG, "gpugi_set_material_texture: bad slot");
+  if (texture < -1 || texture >= (int)ctx->texDescs.size())
+    return Fail(ctx, GPUGI_ERR_ARG, "gpugi_set_material_texture: unknown texture");
+  if (SetDev(ctx)) return GPUGI_ERR_HIP;
+  int4& t = ctx->matTex[material];
+  (slot == GPUGI_TEX_DIFFUSE ? t.x : slot == GPUGI_TEX_OPACITY ? t.y : slot == GPUGI_TEX_REFLECTIVENESS ? t.z : t.w) =
+      texture;
+  return UploadTextureTables(ctx);
+}
+
 int gpugi_set_environment_map(gpugi_ctx* ctx, int size, const uint8_t* faces) {
   if (!ctx) return Fail(ctx, GPUGI_ERR_ARG, "gpugi_set_environment_map: null context");
   if (size < 0 || size > 16384 || (size > 0 && !faces))
@@ -679,10 +782,7 @@ int gpugi_set_environment_map(gpugi_ctx* ctx, int size, const uint8_t* faces) {
   // SRGB8 texels are decoded to linear RGB before filtering, as GL does for
   // sRGB textures: the sRGB transfer function evaluated in double per byte.
   float lut[256];
-  for (int i = 0; i < 256; ++i) {
-    const double c = i / 255.0;
-    lut[i] = (float)(c <= 0.04045 ? c / 12.92 : std::pow((c + 0.055) / 1.055, 2.4));
-  }
+  SrgbTable(lut);
   const size_t n = (size_t)6 * size * size;
   std::vector<float4> texels(n);
   for (size_t i = 0; i < n; ++i)

@@ -265,7 +265,7 @@ __device__ __forceinline__ bool pt_bounce(const DrawParams& p, const TraceCtx& c
   }
   s.o = s.o + h.t * s.d;
   const gg::V3 n = shading_normal(p.ref.vinfo, h.tri, h.bary);
-  const MaterialData m = load_material(p.materials, h.tri.w);
+  const MaterialData m = load_material_hit(p.materials, p.tex, p.ref.vinfo, h.tri, h.bary);
   if (!s.didNEE) s.color = s.color + s.throughput * m.emissivity;
   s.didNEE = m.refl[3] < 500.0f;
   const int sampleIndex = (int)(gg::xorshift32(s.seed) % p.numLights);

@@ -6,6 +6,7 @@
 #include <string>
 
 #include "gpugi_rt.h"
+#include "shading.h"
 #include "trace_ref.h"
 #include "wide_bvh.h"
 
@@ -25,6 +26,7 @@ struct DrawParams {
   int maxPathLength = 16;
   int rrStartDepth = 0;                   // Russian roulette from this bounce on (0: every bounce, as the reference)
   bool watertight = false;                // watertight triangle test instead of IntersectTriangle
+  MaterialTextures tex;                   // material textures (none by default)
   const float4* env = nullptr;            // environment cube map: 6 faces x envSize^2 linear RGB texels, or none (black)
   int envSize = 0;
   uint32_t* counters = nullptr;

@@ -24,6 +24,12 @@ namespace gpugi {
 constexpr int kLightBlock = 64;  // local size of lighttracer.comp:48
 static_assert(kLightBlock == kTileThreads, "the wide traversal stack is laid out for kTileThreads lanes");
 
+// Float add into the accumulation buffer (device memory): a global atomic, so
+// the address needs no test for the LDS or scratch aperture a flat atomic does.
+__device__ __forceinline__ void splat_add(float* p, float v) {
+  (void)__builtin_amdgcn_global_atomic_fadd_f32((__attribute__((address_space(1))) float*)p, v);
+}
+
 __device__ __forceinline__ float mis_heuristic(float p) { return fminf(1e18f, p); }     // lightcache.glsl:61-73
 __device__ __forceinline__ float distance_mis(float d, float c) { return mis_heuristic(d * d); }  // :84-89
 
@@ -68,9 +74,9 @@ __device__ __forceinline__ void project_to_screen(const TraceCtx& c, gg::V3 rayO
   // Splats outside the image are dropped, as image atomics outside the image are.
   if (ix < 0 || ix >= p.width || iy < 0 || iy >= p.height) return;
   float* px = reinterpret_cast<float*>(&p.accum[(size_t)iy * p.width + ix]);
-  atomicAdd(px + 0, color.x);
-  atomicAdd(px + 1, color.y);
-  atomicAdd(px + 2, color.z);
+  splat_add(px + 0, color.x);
+  splat_add(px + 1, color.y);
+  splat_add(px + 2, color.z);
 }
 
 enum LightMode { kModeLT = 0, kModeBptLight = 1, kModeBptWarmup = 2 };
@@ -150,7 +156,7 @@ __device__ __forceinline__ bool lt_bounce(const DrawParams& p, const TraceCtx& c
   s.o = s.o + h.t * s.d;
   const gg::V3 sN = shading_normal(p.ref.vinfo, h.tri, h.bary);
   const float2 uv = hit_texcoord(p.ref.vinfo, h.tri, h.bary);
-  const MaterialData m = load_material(p.materials, h.tri.w);
+  const MaterialData m = load_material_uv(p.materials, p.tex, h.tri.w, uv);
   if (MODE != kModeLT && m.refl[3] < 100.0f) {
     if (MODE == kModeBptLight) {
       float n0, n1;
@@ -236,9 +242,9 @@ __global__ __launch_bounds__(kLightBlock) void k_showlightcaches(DrawParams p) {
   // The reference does an unsynchronized load/store here (showlightcaches.comp:54-55);
   // atomics give the intended sum.
   float* px = reinterpret_cast<float*>(&p.accum[(size_t)iy * p.width + ix]);
-  atomicAdd(px + 0, color.x);
-  atomicAdd(px + 1, color.y);
-  atomicAdd(px + 2, color.z);
+  splat_add(px + 0, color.x);
+  splat_add(px + 1, color.y);
+  splat_add(px + 2, color.z);
 }
 
 // Scatter of the staged BPT vertices behind the initial ones.
@@ -298,7 +304,7 @@ __device__ __forceinline__ bool bpt_eye_bounce(const DrawParams& p, const TraceC
   const float rayHit = h.t;
   s.o = s.o + rayHit * s.d;
   const gg::V3 n = shading_normal(p.ref.vinfo, h.tri, h.bary);
-  const MaterialData m = load_material(p.materials, h.tri.w);
+  const MaterialData m = load_material_hit(p.materials, p.tex, p.ref.vinfo, h.tri, h.bary);
   const float incidentSurfaceCosSat = gg::saturate(-gg::dot(n, s.d));
   const float connectionLightToEyePath = distance_mis(rayHit, s.lastCosAbs) * mis_heuristic(incidentSurfaceCosSat / gg::kPI);
   s.mis = (i == 0) ? 1.0f
@@ -342,7 +348,8 @@ __device__ __forceinline__ bool bpt_eye_bounce(const DrawParams& p, const TraceC
       if (e.normal1 < 1.5f) {
         float lightToEye;
         if (lmat != -1) {
-          const MaterialData lm = load_material(p.materials, lmat);
+          // the light vertex's material at its stored texture coordinate (lightcache.glsl:1-16)
+          const MaterialData lm = load_material_uv(p.materials, p.tex, lmat, make_float2(e.texcoord[0], e.texcoord[1]));
           const gg::V3 incidentDir = gg::unpack_normal(e.incidentPacked[0], e.incidentPacked[1]);
           const gg::V3 lbsdf = bsdf_adjoint(ld, -incidentDir, lm, lnormal, pdf);
           lintensity = flux * lbsdf * cosLight;

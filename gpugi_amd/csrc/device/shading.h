@@ -21,6 +21,45 @@ struct MaterialData {
   gg::V3 fresnel1;
 };
 
+// Material textures (the bindless handles of the reference's Material,
+// globaluniforms.glsl:23-27): per material the texture of its diffuse,
+// opacity, reflectiveness and emissivity slots (-1: the constant of the
+// material record); per texture its linear RGBA float texels (row-major, row
+// 0 at t = 0.5 / height) and size.  matTex == nullptr: no textures at all.
+struct TextureDesc {
+  const float4* texels;
+  int width, height;
+};
+struct MaterialTextures {
+  const int4* __restrict__ matTex = nullptr;
+  const TextureDesc* __restrict__ desc = nullptr;
+};
+
+// textureLod(sampler, uv, 0) with the scene's sampler (scene.cpp:16-20: LINEAR
+// magnification, REPEAT): bilinear over texel centres, coordinates wrapped.
+__device__ __forceinline__ float4 sample_texture(const MaterialTextures& tx, int id, float2 uv) {
+  const TextureDesc d = tx.desc[id];
+  const float u = uv.x * (float)d.width - 0.5f, v = uv.y * (float)d.height - 0.5f;
+  const float fu = floorf(u), fv = floorf(v);
+  const float a = u - fu, b = v - fv;
+  if (!(a >= 0.0f && a <= 1.0f && b >= 0.0f && b <= 1.0f)) return make_float4(0.0f, 0.0f, 0.0f, 0.0f);  // NaN / inf uv
+  // REPEAT: fu, fv are integers (fmodf is exact on them); the clamp only
+  // guards the index against what rounding cannot produce
+  float wx = fmodf(fu, (float)d.width), wy = fmodf(fv, (float)d.height);
+  wx = wx < 0.0f ? wx + (float)d.width : wx;
+  wy = wy < 0.0f ? wy + (float)d.height : wy;
+  const int x0 = min(max((int)wx, 0), d.width - 1), y0 = min(max((int)wy, 0), d.height - 1);
+  const int x1 = x0 + 1 == d.width ? 0 : x0 + 1, y1 = y0 + 1 == d.height ? 0 : y0 + 1;
+  const float4* t = d.texels;
+  const float4 t00 = t[(size_t)y0 * d.width + x0], t10 = t[(size_t)y0 * d.width + x1];
+  const float4 t01 = t[(size_t)y1 * d.width + x0], t11 = t[(size_t)y1 * d.width + x1];
+  const float ia = 1.0f - a, ib = 1.0f - b;
+  return make_float4((t00.x * ia + t10.x * a) * ib + (t01.x * ia + t11.x * a) * b,
+                     (t00.y * ia + t10.y * a) * ib + (t01.y * ia + t11.y * a) * b,
+                     (t00.z * ia + t10.z * a) * ib + (t01.z * ia + t11.z * a) * b,
+                     (t00.w * ia + t10.w * a) * ib + (t01.w * ia + t11.w * a) * b);
+}
+
 __device__ __forceinline__ MaterialData load_material(const gpugi_material* __restrict__ m, int id) {
   const gpugi_material& s = m[id];
   MaterialData d;
@@ -222,6 +261,48 @@ __device__ __forceinline__ gg::V3 shading_normal(const float4* __restrict__ vinf
 __device__ __forceinline__ float2 hit_texcoord(const float4* __restrict__ vinfo, int4 tri, gg::V3 b) {
   const float4 a = vinfo[tri.x], c = vinfo[tri.y], e = vinfo[tri.z];
   return make_float2(a.z * b.x + c.z * b.y + e.z * b.z, a.w * b.x + c.w * b.y + e.w * b.z);
+}
+
+// SampleMaterialData (materials.glsl:19-30) at texture coordinate uv for a
+// scene with textures: the material record's constants, with each textured
+// slot sampled instead.  Kept out of line: only textured scenes call it.
+__device__ __noinline__ MaterialData load_material_textured(const gpugi_material* __restrict__ m,
+                                                            MaterialTextures tx, int id, float2 uv) {
+  MaterialData d = load_material(m, id);
+  const int4 t = tx.matTex[id];
+  if (t.x >= 0) {
+    const float4 c = sample_texture(tx, t.x, uv);
+    d.diffuse = gg::v3(c.x, c.y, c.z);
+  }
+  if (t.y >= 0) {
+    const float4 c = sample_texture(tx, t.y, uv);
+    d.opacity = gg::v3(c.x, c.y, c.z);
+  }
+  if (t.z >= 0) {
+    const float4 c = sample_texture(tx, t.z, uv);
+    d.refl[0] = c.x; d.refl[1] = c.y; d.refl[2] = c.z; d.refl[3] = c.w;
+  }
+  if (t.w >= 0) {
+    const float4 c = sample_texture(tx, t.w, uv);
+    d.emissivity = gg::v3(c.x, c.y, c.z);
+  }
+  return d;
+}
+
+// SampleMaterialData (materials.glsl:19-30) at texture coordinate uv.
+__device__ __forceinline__ MaterialData load_material_uv(const gpugi_material* __restrict__ m,
+                                                         const MaterialTextures& tx, int id, float2 uv) {
+  if (!tx.matTex) return load_material(m, id);
+  return load_material_textured(m, tx, id, uv);
+}
+
+// The same at a hit: the texture coordinate of GetTriangleHitInfo
+// (scenedata.glsl:94-105), computed only when the scene has textures.
+__device__ __forceinline__ MaterialData load_material_hit(const gpugi_material* __restrict__ m,
+                                                          const MaterialTextures& tx,
+                                                          const float4* __restrict__ vinfo, int4 tri, gg::V3 b) {
+  if (!tx.matTex) return load_material(m, tri.w);
+  return load_material_uv(m, tx, tri.w, hit_texcoord(vinfo, tri, b));
 }
 
 }  // namespace gpugi

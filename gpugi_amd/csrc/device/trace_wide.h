@@ -53,8 +53,18 @@ struct WideResult {
 // (spillStride = number of threads sharing the spill buffer when it is laid out
 // [depth][thread], so a wave's spills at one depth are contiguous; 1 for a
 // per-thread slice).
+// The LDS part is typed as LDS memory, so its accesses stay ds_* instructions
+// and are never merged with the spill's global accesses into flat ones.
+// (The host pass of the compiler sees the device code too and has no LDS.)
+#if defined(__HIP_DEVICE_COMPILE__)
+typedef __attribute__((address_space(3))) uint2 LdsEntry;
+#else
+typedef uint2 LdsEntry;
+#endif
+__device__ __forceinline__ LdsEntry* as_lds(uint2* p) { return (LdsEntry*)p; }
+
 struct WideStack {
-  uint2* lds;
+  LdsEntry* lds;
   int tid;
   uint2* spill;
   uint32_t spillStride;
@@ -214,7 +224,7 @@ __device__ __forceinline__ bool wide_visit(const WideScene& s, WideState& st, co
   bool overflow = false;
   int sp = st.sp;
   if (sp + 4 <= S) {  // every push stays in LDS: one store and one increment each
-    uint2* e = stk.lds + sp * B + stk.tid;
+    LdsEntry* e = stk.lds + sp * B + stk.tid;
     int pushed = 0;
     if (p3) { e[pushed * B] = make_uint2(c3, __float_as_uint(t3)); ++pushed; }
     if (p2) { e[pushed * B] = make_uint2(c2, __float_as_uint(t2)); ++pushed; }
@@ -392,7 +402,7 @@ __device__ __forceinline__ bool wide_pass(const WideScene& s, WideState& st, boo
 template <bool ANY, int B, int S, bool WT = false>
 __device__ __forceinline__ WideResult trace_wide_ray(const WideScene& s, gg::V3 o, gg::V3 d, float tmax,
                                                      uint2* lds, int tid, uint2* spill) {
-  const WideStack stk{lds, tid, spill, 1u};
+  const WideStack stk{as_lds(lds), tid, spill, 1u};
   WideState st;
   wide_init<WT>(st, o, d, tmax);
   bool live = true;

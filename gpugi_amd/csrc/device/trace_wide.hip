@@ -37,7 +37,7 @@ __global__ __launch_bounds__(B) void k_trace_wide(WideScene s, const float4* __r
   const int lane = tid & 63;
   const uint32_t gtid = blockIdx.x * B + tid;
   // spill laid out [depth][thread] over the whole grid (s.numThreads == gridDim.x * B)
-  const WideStack stk{lds, tid, s.spill + gtid, gridDim.x * (uint32_t)B};
+  const WideStack stk{as_lds(lds), tid, s.spill + gtid, gridDim.x * (uint32_t)B};
 
   const uint32_t xcd = blockIdx.x % kNumXcd;
   const uint32_t batches = (n + 63u) / 64u;

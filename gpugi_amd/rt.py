@@ -18,6 +18,8 @@ KERNEL_AUTO, KERNEL_REFERENCE, KERNEL_WIDE = 0, 1, 2
 PT, LT, BPT, PT_CACHE = 0, 1, 2, 3
 TRIANGLE_REFERENCE, TRIANGLE_WATERTIGHT = 0, 1
 CACHE_SPREAD, CACHE_PRIMARY_INDIRECT = 0, 1
+TEXTURE_RGBA8_SRGB, TEXTURE_RGBA32F = 0, 1
+TEX_DIFFUSE, TEX_OPACITY, TEX_REFLECTIVENESS, TEX_EMISSIVITY = 0, 1, 2, 3
 RAY_MAX = np.float32(3.402823466e38)
 
 RAY_DTYPE = np.dtype([("origin", "<f4", 3), ("tmax", "<f4"), ("dir", "<f4", 3), ("pad", "<u4")])
@@ -83,6 +85,8 @@ _SIGNATURES = {
     "gpugi_readback": [_P, _P, ctypes.c_size_t, ctypes.POINTER(ctypes.c_uint32)],
     "gpugi_set_reference_image": [_P, _P, ctypes.c_int, ctypes.c_int],
     "gpugi_set_environment_map": [_P, ctypes.c_int, _P],
+    "gpugi_add_texture": [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P, ctypes.POINTER(ctypes.c_int)],
+    "gpugi_set_material_texture": [_P, ctypes.c_uint32, ctypes.c_int, ctypes.c_int],
     "gpugi_compute_mse": [_P, ctypes.c_uint32, ctypes.POINTER(ctypes.c_float)],
     "gpugi_trace": [_P, ctypes.c_int, ctypes.c_int, _P, _P, ctypes.c_size_t, ctypes.c_int],
     "gpugi_trace_counts": [_P, ctypes.c_int, _P, _P, _P, ctypes.c_size_t, ctypes.c_int],
@@ -433,6 +437,22 @@ class Context:
         it = ctypes.c_uint32()
         self._call("gpugi_readback", _ptr(out), ctypes.c_size_t(out.nbytes), ctypes.byref(it))
         return out, it.value
+
+    def add_texture(self, texels):
+        """Scene::GetBindlessHandle: texels (h, w, 4) uint8 (sRGB colour, linear alpha)
+        or float32 (linear).  Returns the texture index."""
+        t = np.ascontiguousarray(texels)
+        if t.ndim != 3 or t.shape[2] != 4:
+            raise ValueError("texture must be (h, w, 4)")
+        fmt = TEXTURE_RGBA8_SRGB if t.dtype == np.uint8 else TEXTURE_RGBA32F
+        if fmt == TEXTURE_RGBA32F:
+            t = np.ascontiguousarray(t, dtype=np.float32)
+        out = ctypes.c_int()
+        self._call("gpugi_add_texture", int(t.shape[1]), int(t.shape[0]), fmt, _ptr(t), ctypes.byref(out))
+        return out.value
+
+    def set_material_texture(self, material, slot, texture):
+        self._call("gpugi_set_material_texture", int(material), int(slot), int(texture))
 
     def set_environment_map(self, faces):
         """RendererSystem::SetEnvironmentMap: faces (6, size, size, 3) uint8 sRGB in the

@@ -207,6 +207,27 @@ int gpugi_reset_accum(gpugi_ctx* ctx);
  * the last reset. */
 int gpugi_readback(gpugi_ctx* ctx, float* rgba, size_t bytes, uint32_t* iterations);
 
+/* Material textures (Scene::GetBindlessHandle and the texture branches of
+ * Scene::LoadMaterial, scene.cpp:161-178, 189-203).  A texture is width x
+ * height texels, row 0 at t = 0.5 / height: GPUGI_TEXTURE_RGBA8_SRGB (4 bytes
+ * per texel, colour sRGB-encoded, alpha linear, as an 8-bit texture the
+ * reference loads) or GPUGI_TEXTURE_RGBA32F (16 bytes, linear).  Sampling is
+ * textureLod(uv, 0) with the scene's sampler (scene.cpp:16-20): bilinear,
+ * REPEAT.  Textures and bindings belong to the loaded scene: gpugi_load_scene
+ * drops them.  gpugi_add_texture stores a copy and returns its index in *id;
+ * gpugi_set_material_texture makes slot `slot` of material `material` sample
+ * texture `texture` at the hit's texture coordinate (-1: back to the constant
+ * of the material record). */
+enum gpugi_texture_format { GPUGI_TEXTURE_RGBA8_SRGB = 0, GPUGI_TEXTURE_RGBA32F = 1 };
+enum gpugi_material_slot {
+  GPUGI_TEX_DIFFUSE = 0,        /* MaterialData.Diffuse = .rgb */
+  GPUGI_TEX_OPACITY = 1,        /* MaterialData.Opacity = .rgb */
+  GPUGI_TEX_REFLECTIVENESS = 2, /* MaterialData.Reflectiveness = .rgba (.a: Phong exponent) */
+  GPUGI_TEX_EMISSIVITY = 3      /* MaterialData.Emissivity = .rgb */
+};
+int gpugi_add_texture(gpugi_ctx* ctx, int width, int height, int format, const void* texels, int* id);
+int gpugi_set_material_texture(gpugi_ctx* ctx, uint32_t material, int slot, int texture);
+
 /* RendererSystem::SetEnvironmentMap (renderersystem.cpp:173-186): six square
  * faces of size x size RGB8 texels, sRGB-encoded like the reference's SRGB8
  * cube map, in the order x-, x+, y-, y+, z-, z+ of the envMap script command

@@ -92,6 +92,13 @@ void or_bsdf_evals(const float* mat20, const float* inc, const float* N, int adj
  * texels as 6 faces (x-, x+, y-, y+, z-, z+) of size x size linear RGB, 4
  * floats each; the array must outlive the draws. */
 void or_set_environment(const float* texels, int size);
+/* Material textures for every integrator (NULL: none): matTex holds 4 texture
+ * indices per material (diffuse, opacity, reflectiveness, emissivity; -1 for
+ * the record's constant), texels[i] the linear RGBA floats of texture i and
+ * sizes[2i], sizes[2i+1] its width and height; the arrays must outlive the draws. */
+void or_set_textures(const int32_t* matTex, uint32_t numMaterials, const float* const* texels, const int32_t* sizes);
+/* The texture lookup of texture id (set by or_set_textures) at n coordinates (u, v) into out (4 floats each). */
+void or_texture_samples(int id, const float* uvs, size_t n, float* out);
 /* The cube-map lookup for n directions (3 floats each) into out (3 floats each). */
 void or_env_lookups(const float* tex, int size, const float* dirs, size_t n, float* out);
 

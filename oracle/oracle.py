@@ -75,6 +75,8 @@ def lib():
         l.or_bsdf_evals.argtypes = [P, P, P, ctypes.c_int, P, ctypes.c_size_t, P]
         l.or_bsdf_evals.restype = None
         l.or_set_environment.argtypes, l.or_set_environment.restype = [P, ctypes.c_int], None
+        l.or_set_textures.argtypes, l.or_set_textures.restype = [P, ctypes.c_uint32, P, P], None
+        l.or_texture_samples.argtypes, l.or_texture_samples.restype = [ctypes.c_int, P, ctypes.c_size_t, P], None
         l.or_env_lookups.argtypes, l.or_env_lookups.restype = [P, ctypes.c_int, P, ctypes.c_size_t, P], None
         l.or_pathtracer.argtypes = [P, P, ctypes.c_uint32, P, ctypes.c_int, ctypes.c_int, ctypes.c_uint32,
                                     ctypes.c_int, P, ctypes.c_int, ctypes.c_int]
@@ -223,6 +225,34 @@ def set_environment(texels):
     assert t.ndim == 4 and t.shape[0] == 6 and t.shape[1] == t.shape[2] and t.shape[3] == 4
     _env_keep = t
     lib().or_set_environment(_p(t), t.shape[1])
+
+
+_tex_keep = None
+
+
+def set_textures(mat_tex, textures):
+    """Material textures of every integrator: mat_tex (materials, 4) texture
+    indices (diffuse, opacity, reflectiveness, emissivity; -1: constant) and
+    textures, a list of (h, w, 4) float32 linear RGBA arrays; None clears."""
+    global _tex_keep
+    if mat_tex is None:
+        lib().or_set_textures(None, 0, None, None)
+        _tex_keep = None
+        return
+    mt = np.ascontiguousarray(mat_tex, dtype=np.int32).reshape(-1, 4)
+    texs = [np.ascontiguousarray(t, dtype=np.float32) for t in textures]
+    ptrs = (ctypes.c_void_p * max(1, len(texs)))(*[t.ctypes.data for t in texs])
+    sizes = np.array([[t.shape[1], t.shape[0]] for t in texs] or [[0, 0]], np.int32)
+    _tex_keep = (mt, texs, ptrs, sizes)
+    lib().or_set_textures(_p(mt), len(mt), ctypes.cast(ptrs, ctypes.c_void_p), _p(sizes))
+
+
+def texture_samples(tex_id, uvs):
+    """The material-texture lookup of texture tex_id (of the last set_textures) at uvs (n x 2)."""
+    uv = np.ascontiguousarray(uvs, dtype=np.float32).reshape(-1, 2)
+    out = np.zeros((len(uv), 4), np.float32)
+    lib().or_texture_samples(int(tex_id), _p(uv), len(uv), _p(out))
+    return out
 
 
 def env_lookups(texels, dirs):

@@ -122,7 +122,7 @@ static int light_path(const or_scene* s, const float* lights, const or_camera* c
     const float* vc = s->vertex_info + (size_t)tri[2] * 4;
     float u = va[2] * b.x + vb[2] * b.y + vc[2] * b.z;
     float v = va[3] * b.x + vb[3] * b.y + vc[3] * b.z;
-    or_material m = or_mat_of(s, tri[3]);
+    or_material m = or_mat_at(s, tri[3], u, v);
     if (mode != 0 && m.refl[3] < 100.0f) {
       if (mode == 1) {
         float n0, n1;
@@ -248,7 +248,9 @@ void or_bpt_eye(const or_scene* s, const or_camera* c, int W, int H, uint32_t fr
         float rayHit = t;
         o = vadd(o, vscale(rayHit, d));
         vec3 n = or_shading_normal(s, tri, b);
-        or_material m = or_mat_of(s, tri[3]);
+        float tu, tv;
+        or_hit_texcoord(s, tri, b, &tu, &tv);
+        or_material m = or_mat_at(s, tri[3], tu, tv);
         float inCos = fsat(-vdot(n, d));
         float l2e = dist_mis(rayHit, lastCos) * mis_h(inCos / OR_PI);
         mis = (i == 0) ? 1.0f : (lastP / fmaxf(lastP + l2e, 1e-5f));
@@ -290,7 +292,7 @@ void or_bpt_eye(const or_scene* s, const or_camera* c, int W, int H, uint32_t fr
             if (e[10] < 1.5f) {
               float lightToEye;
               if (emat != -1) {
-                or_material lm = or_mat_of(s, emat);
+                or_material lm = or_mat_at(s, emat, e[8], e[9]);
                 vec3 inc = or_unpack_normal(e[12], e[13]);
                 vec3 lb = adjoint_bsdf(ld, vneg(inc), &lm, lnormal, &pdf);
                 lint = vscalev(vmul(flux, lb), cosLight);

@@ -21,6 +21,71 @@ or_material or_mat_of(const or_scene* s, int id) {
   return r;
 }
 
+/* Material textures (the bindless handles of Material, globaluniforms.glsl:
+ * 23-27): per material the texture of its diffuse, opacity, reflectiveness and
+ * emissivity slots (-1: the record's constant); per texture its linear RGBA
+ * texels and size.  NULL: no textures. */
+static const int32_t* g_matTex = NULL;
+static uint32_t g_numMatTex = 0;
+static const float* const* g_texels = NULL;
+static const int32_t* g_texSize = NULL;
+
+void or_set_textures(const int32_t* matTex, uint32_t numMaterials, const float* const* texels, const int32_t* sizes) {
+  g_matTex = matTex;
+  g_numMatTex = numMaterials;
+  g_texels = texels;
+  g_texSize = sizes;
+}
+
+/* textureLod(uv, 0) with the scene's sampler (scene.cpp:16-20): bilinear over
+ * texel centres, REPEAT; a NaN or infinite coordinate gives 0. */
+static void tex_sample(int id, float uu, float vv, float out[4]) {
+  const int w = g_texSize[2 * id], h = g_texSize[2 * id + 1];
+  float u = uu * (float)w - 0.5f, v = vv * (float)h - 0.5f;
+  float fu = floorf(u), fv = floorf(v);
+  float a = u - fu, b = v - fv;
+  if (!(a >= 0.0f && a <= 1.0f && b >= 0.0f && b <= 1.0f)) {
+    out[0] = out[1] = out[2] = out[3] = 0.0f;
+    return;
+  }
+  float wx = fmodf(fu, (float)w), wy = fmodf(fv, (float)h);
+  if (wx < 0.0f) wx += (float)w;
+  if (wy < 0.0f) wy += (float)h;
+  int x0 = (int)wx, y0 = (int)wy;
+  x0 = x0 < 0 ? 0 : (x0 > w - 1 ? w - 1 : x0);
+  y0 = y0 < 0 ? 0 : (y0 > h - 1 ? h - 1 : y0);
+  int x1 = x0 + 1 == w ? 0 : x0 + 1, y1 = y0 + 1 == h ? 0 : y0 + 1;
+  const float* t = g_texels[id];
+  const float* t00 = t + ((size_t)y0 * w + x0) * 4;
+  const float* t10 = t + ((size_t)y0 * w + x1) * 4;
+  const float* t01 = t + ((size_t)y1 * w + x0) * 4;
+  const float* t11 = t + ((size_t)y1 * w + x1) * 4;
+  float ia = 1.0f - a, ib = 1.0f - b;
+  for (int c = 0; c < 4; ++c) out[c] = (t00[c] * ia + t10[c] * a) * ib + (t01[c] * ia + t11[c] * a) * b;
+}
+
+/* SampleMaterialData (materials.glsl:19-30) at texture coordinate (u, v). */
+or_material or_mat_at(const or_scene* s, int id, float u, float v) {
+  or_material r = or_mat_of(s, id);
+  if (!g_matTex || (uint32_t)id >= g_numMatTex) return r;
+  const int32_t* t = g_matTex + (size_t)id * 4;
+  float c[4];
+  if (t[0] >= 0) { tex_sample(t[0], u, v, c); r.diffuse = v3(c[0], c[1], c[2]); }
+  if (t[1] >= 0) { tex_sample(t[1], u, v, c); r.opacity = v3(c[0], c[1], c[2]); }
+  if (t[2] >= 0) { tex_sample(t[2], u, v, c); r.refl[0] = c[0]; r.refl[1] = c[1]; r.refl[2] = c[2]; r.refl[3] = c[3]; }
+  if (t[3] >= 0) { tex_sample(t[3], u, v, c); r.emissivity = v3(c[0], c[1], c[2]); }
+  return r;
+}
+
+/* The texture coordinate of GetTriangleHitInfo (scenedata.glsl:94-105). */
+void or_hit_texcoord(const or_scene* s, const int32_t* tri, vec3 b, float* u, float* v) {
+  const float* va = s->vertex_info + (size_t)tri[0] * 4;
+  const float* vb = s->vertex_info + (size_t)tri[1] * 4;
+  const float* vc = s->vertex_info + (size_t)tri[2] * 4;
+  *u = va[2] * b.x + vb[2] * b.y + vc[2] * b.z;
+  *v = va[3] * b.x + vb[3] * b.y + vc[3] * b.z;
+}
+
 /* AvgProbability / Avg (materials.glsl:33-41) */
 static float avg_p(vec3 c) { return (c.x + c.y + c.z + 1e-5f) / (3.0f + 1e-5f); }
 static float avg(vec3 c) { return (c.x + c.y + c.z) / 3.0f; }
@@ -243,7 +308,9 @@ void or_pathtracer(const or_scene* s, const float* lights, uint32_t numLights, c
         }
         o = vadd(o, vscale(t, d));
         vec3 n = or_shading_normal(s, tri, b);
-        or_material m = or_mat_of(s, tri[3]);
+        float tu, tv;
+        or_hit_texcoord(s, tri, b, &tu, &tv);
+        or_material m = or_mat_at(s, tri[3], tu, tv);
         if (!didNEE) color = vadd(color, vmul(thr, m.emissivity));
         didNEE = m.refl[3] < 500.0f;
         int idx = (int)(or_xorshift32(&seed) % numLights);
@@ -325,4 +392,8 @@ void or_env_lookups(const float* tex, int size, const float* dirs, size_t n, flo
     out[3 * i + 1] = r.y;
     out[3 * i + 2] = r.z;
   }
+}
+
+void or_texture_samples(int id, const float* uvs, size_t n, float* out) {
+  for (size_t i = 0; i < n; ++i) tex_sample(id, uvs[2 * i], uvs[2 * i + 1], out + 4 * i);
 }

@@ -104,6 +104,8 @@ vec3 or_sample_bsdf(vec3 inc, const or_material* m, uint32_t* seed, vec3 N, vec3
                     int* diffuseOut);
 vec3 or_bsdf(vec3 inc, vec3 exc, const or_material* m, vec3 N, float* pdf, int adjoint, int specularOnly);
 vec3 or_env_lookup(const float* tex, int size, vec3 d);
+or_material or_mat_at(const or_scene* s, int id, float u, float v);
+void or_hit_texcoord(const or_scene* s, const int32_t* tri, vec3 b, float* u, float* v);
 vec3 or_shading_normal(const or_scene* s, const int32_t* tri, vec3 b);
 int or_any_hit(const or_scene* s, vec3 o, vec3 d, float len);
 
