@@ -189,6 +189,47 @@ int gpugi_hostscene_info(const gpugi_hostscene* scene, uint32_t* num_triangles, 
   return GPUGI_OK;
 }
 
+int gpugi_hostscene_num_textures(const gpugi_hostscene* scene, uint32_t* n) {
+  if (!scene || !n) {
+    SetGlobalError("gpugi_hostscene_num_textures: null argument");
+    return GPUGI_ERR_ARG;
+  }
+  *n = (uint32_t)scene->scene.textures.size();
+  return GPUGI_OK;
+}
+
+int gpugi_hostscene_texture(const gpugi_hostscene* scene, uint32_t i, int* width, int* height, int* format,
+                            const void** texels) {
+  if (!scene || !width || !height || !format || !texels) {
+    SetGlobalError("gpugi_hostscene_texture: null argument");
+    return GPUGI_ERR_ARG;
+  }
+  if (i >= scene->scene.textures.size()) {
+    SetGlobalError("gpugi_hostscene_texture: index out of range");
+    return GPUGI_ERR_ARG;
+  }
+  const gpugi::HostTexture& t = scene->scene.textures[i];
+  *width = t.width;
+  *height = t.height;
+  *format = t.format;
+  *texels = t.data.data();
+  return GPUGI_OK;
+}
+
+int gpugi_hostscene_material_textures(const gpugi_hostscene* scene, uint32_t material, int32_t slots[4]) {
+  if (!scene || !slots) {
+    SetGlobalError("gpugi_hostscene_material_textures: null argument");
+    return GPUGI_ERR_ARG;
+  }
+  if (material >= scene->scene.materials.size()) {
+    SetGlobalError("gpugi_hostscene_material_textures: material out of range");
+    return GPUGI_ERR_ARG;
+  }
+  for (int k = 0; k < 4; ++k)
+    slots[k] = material < scene->scene.materialTextures.size() ? scene->scene.materialTextures[material][k] : -1;
+  return GPUGI_OK;
+}
+
 int gpugi_hostscene_add_point_light(gpugi_hostscene* scene, const float position[3], const float intensity[3]) {
   GPUGI_GUARD_BEGIN
   if (!scene || !position || !intensity) return GPUGI_ERR_ARG;

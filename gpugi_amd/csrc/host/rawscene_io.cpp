@@ -183,6 +183,11 @@ std::string LoadRawScene(const std::string& path, const std::string& jsonPath, H
   *status = GPUGI_ERR_FORMAT;
   std::ifstream f(path, std::ios::binary);
   if (!f) { *status = GPUGI_ERR_IO; return "cannot open " + path; }
+  // texture files are relative to the scene file's directory (Scene::m_sourceDirectory)
+  const size_t slash = path.find_last_of('/');
+  const std::string sceneDir = slash == std::string::npos ? std::string() : path.substr(0, slash + 1);
+  out->textures.clear();
+  out->materialTextures.clear();
   std::vector<std::string> materialNames;
   bool haveVertices = false, haveTriangles = false, haveHierarchy = false, haveBoxes = false;
   for (;;) {
@@ -265,14 +270,42 @@ std::string LoadRawScene(const std::string& path, const std::string& jsonPath, H
           out->materials.push_back(DefaultMaterialParams(m.first.c_str()));
         }
         gpugi_material_params& p = out->materials[idx];
+        // Scene::LoadMaterial (scene.cpp:155-178): albedo, opacity, reflectiveness
+        // and emissivity are either values or texture files (a string, relative
+        // to the scene's directory), each file loaded once (GetBindlessHandle).
+        static const char* kSlotKeys[4] = {"albedo", "opacity", "reflectiveness", "emissivity"};
+        for (int slot = 0; slot < 4; ++slot) {
+          const JValue* v = m.second.Get(kSlotKeys[slot]);
+          if (!v || v->kind != JValue::kString) continue;
+          std::string rel = v->str;
+          for (char& ch : rel)
+            if (ch == '\\') ch = '/';  // the reference's scene files use Windows separators
+          const std::string file = sceneDir + rel;
+          int tex = -1;
+          for (size_t t = 0; t < out->textures.size(); ++t)
+            if (out->textures[t].file == file) tex = (int)t;
+          if (tex < 0) {
+            HostTexture ht;
+            const std::string terr = LoadTextureFile(file, &ht);
+            if (!terr.empty()) { *status = GPUGI_ERR_IO; return terr; }
+            ht.name = v->str;
+            out->textures.push_back(std::move(ht));
+            tex = (int)out->textures.size() - 1;
+          }
+          if (out->materialTextures.size() < out->materials.size())
+            out->materialTextures.resize(out->materials.size(), {-1, -1, -1, -1});
+          out->materialTextures[idx][slot] = tex;
+        }
         ReadFloats(m.second.Get("albedo"), p.albedo, 3);
         ReadFloats(m.second.Get("specularColor"), p.specular_color, 3);
         ReadFloats(m.second.Get("roughness"), &p.roughness, 1);
         ReadFloats(m.second.Get("opacity"), &p.opacity, 1);
         ReadFloats(m.second.Get("emissivity"), p.emissivity, 3);
         ReadFloats(m.second.Get("refractionIndex"), &p.refraction_index, 1);
+        if (ReadFloats(m.second.Get("reflectiveness"), p.reflectiveness, 4)) p.has_reflectiveness = 1;
       }
     }
+    if (!out->materialTextures.empty()) out->materialTextures.resize(out->materials.size(), {-1, -1, -1, -1});
     const JValue* lights = root.Get("lights");
     if (lights && lights->kind == JValue::kObject) {
       for (const auto& l : lights->members) {
@@ -329,13 +362,28 @@ std::string SaveRawScene(const HostScene& s, const std::string& path, const std:
   j << "{\n\t\"materials\": {\n";
   for (size_t i = 0; i < s.materials.size(); ++i) {
     const auto& m = s.materials[i];
+    // a textured slot is written as its texture file, as it was read
+    auto slot = [&](int k, const std::string& value) {
+      if (i < s.materialTextures.size() && s.materialTextures[i][k] >= 0) {
+        std::string name = s.textures[s.materialTextures[i][k]].name;
+        std::string esc;
+        for (char ch : name) {
+          if (ch == '\\' || ch == '"') esc.push_back('\\');
+          esc.push_back(ch);
+        }
+        return "\"" + esc + "\"";
+      }
+      return value;
+    };
     j << "\t\t\"" << m.name << "\": {\n"
       << "\t\t\t\"type\": \"legacy\",\n"
-      << "\t\t\t\"albedo\": " << vec(m.albedo, 3) << ",\n"
+      << "\t\t\t\"albedo\": " << slot(0, vec(m.albedo, 3)) << ",\n"
       << "\t\t\t\"specularColor\": " << vec(m.specular_color, 3) << ",\n"
       << "\t\t\t\"roughness\": [" << FmtFloat(m.roughness) << "],\n"
-      << "\t\t\t\"opacity\": [" << FmtFloat(m.opacity) << "],\n"
-      << "\t\t\t\"emissivity\": " << vec(m.emissivity, 3) << ",\n"
+      << "\t\t\t\"opacity\": " << slot(1, "[" + FmtFloat(m.opacity) + "]") << ",\n";
+    if (m.has_reflectiveness || (i < s.materialTextures.size() && s.materialTextures[i][2] >= 0))
+      j << "\t\t\t\"reflectiveness\": " << slot(2, vec(m.reflectiveness, 4)) << ",\n";
+    j << "\t\t\t\"emissivity\": " << slot(3, vec(m.emissivity, 3)) << ",\n"
       << "\t\t\t\"refractionIndex\": [" << FmtFloat(m.refraction_index) << "]\n"
       << "\t\t}" << (i + 1 < s.materials.size() ? "," : "") << "\n";
   }

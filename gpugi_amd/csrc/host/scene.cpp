@@ -26,6 +26,8 @@ gpugi_material_params DefaultMaterialParams(const char* name) {
   p.roughness = 1.0f;
   p.opacity = 1.0f;
   p.refraction_index = 1.5f;
+  for (int i = 0; i < 4; ++i) p.reflectiveness[i] = 0.0f;
+  p.has_reflectiveness = 0;
   return p;
 }
 
@@ -45,6 +47,8 @@ gpugi_material ResolveMaterial(const gpugi_material_params& p) {
     m.fresnel1[i] = 1.0f - p.specular_color[i];
   }
   m.reflectiveness[3] = exponent;
+  if (p.has_reflectiveness)  // an RGBA32F texture of the given value
+    for (int i = 0; i < 4; ++i) m.reflectiveness[i] = p.reflectiveness[i];
   m.refraction_index_avg = p.refraction_index;
   return m;
 }

@@ -4,6 +4,7 @@
 #pragma once
 
 #include <cstdint>
+#include <array>
 #include <string>
 #include <vector>
 
@@ -44,6 +45,18 @@ struct LightTriangle {  // Scene::LightTriangle (scene.hpp:95-100)
 
 inline bool IsValidSlot(const RawTriangle& t) { return t.vertices[0] != t.vertices[1]; }
 
+// A material texture as loaded from a file (GPUGI_TEXTURE_* format, rows
+// bottom first: row 0 is texture coordinate t = 0.5 / height).
+struct HostTexture {
+  int width = 0, height = 0;
+  int format = GPUGI_TEXTURE_RGBA8_SRGB;
+  std::vector<uint8_t> data;  // 4 bytes (RGBA8) or 16 bytes (RGBA32F) per texel
+  std::string file;  // path it was read from
+  std::string name;  // the material json's string (relative to the scene's directory)
+};
+// Reads a TGA, binary PPM or PFM file.  Returns an error message or "".
+std::string LoadTextureFile(const std::string& path, HostTexture* out);
+
 class HostScene {
  public:
   std::vector<RawVertex> vertices;
@@ -53,6 +66,11 @@ class HostScene {
   std::vector<RawBox> boxes;
   std::vector<gpugi_material_params> materials;
   std::vector<PointLight> pointLights;
+  // Textures the materials reference (Scene::GetBindlessHandle, scene.cpp:189-203)
+  // and per material the texture of each slot (diffuse, opacity, reflectiveness,
+  // emissivity; -1: constant).  Empty when no material has a texture.
+  std::vector<HostTexture> textures;
+  std::vector<std::array<int32_t, 4>> materialTextures;
 
   // Derived data, rebuilt by Finalize().
   std::vector<float> gpuHierarchy;      // 8 floats per node

@@ -238,6 +238,24 @@ void RendererSystem::SetScene(std::shared_ptr<Scene> scene) {
   gpugi_scene_desc desc;
   Check(gpugi_hostscene_desc(scene->Handle(), &desc), nullptr, "gpugi_hostscene_desc");
   Check(gpugi_load_scene(m_ctx, &desc), m_ctx, "gpugi_load_scene");
+  // the material textures the scene's json names (Scene::LoadMaterial, scene.cpp:161-178)
+  uint32_t numTextures = 0;
+  Check(gpugi_hostscene_num_textures(scene->Handle(), &numTextures), nullptr, "gpugi_hostscene_num_textures");
+  if (numTextures > 0) {
+    std::vector<int> ids(numTextures);
+    for (uint32_t t = 0; t < numTextures; ++t) {
+      int w = 0, h = 0, format = 0;
+      const void* texels = nullptr;
+      Check(gpugi_hostscene_texture(scene->Handle(), t, &w, &h, &format, &texels), nullptr, "gpugi_hostscene_texture");
+      Check(gpugi_add_texture(m_ctx, w, h, format, texels, &ids[t]), m_ctx, "gpugi_add_texture");
+    }
+    for (uint32_t m = 0; m < desc.num_materials; ++m) {
+      int32_t slots[4];
+      Check(gpugi_hostscene_material_textures(scene->Handle(), m, slots), nullptr, "gpugi_hostscene_material_textures");
+      for (int k = 0; k < 4; ++k)
+        if (slots[k] >= 0) Check(gpugi_set_material_texture(m_ctx, m, k, ids[slots[k]]), m_ctx, "gpugi_set_material_texture");
+    }
+  }
   m_scene = scene;
   m_lightSampler.SetScene(m_scene);
   PerIterationBufferUpdate(false);

@@ -54,7 +54,8 @@ class Camera(ctypes.Structure):
 class MaterialParams(ctypes.Structure):
     _fields_ = [("name", ctypes.c_char * 32), ("albedo", ctypes.c_float * 3), ("specular_color", ctypes.c_float * 3),
                 ("roughness", ctypes.c_float), ("opacity", ctypes.c_float), ("emissivity", ctypes.c_float * 3),
-                ("refraction_index", ctypes.c_float)]
+                ("refraction_index", ctypes.c_float), ("reflectiveness", ctypes.c_float * 4),
+                ("has_reflectiveness", ctypes.c_int32)]
 
 
 class GpugiError(RuntimeError):
@@ -115,6 +116,10 @@ _SIGNATURES = {
     "gpugi_hostscene_load": [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(ctypes.c_void_p)],
     "gpugi_hostscene_save": [_P, ctypes.c_char_p, ctypes.c_char_p],
     "gpugi_hostscene_desc": [_P, ctypes.POINTER(SceneDesc)],
+    "gpugi_hostscene_num_textures": [_P, ctypes.POINTER(ctypes.c_uint32)],
+    "gpugi_hostscene_texture": [_P, ctypes.c_uint32, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                                ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_void_p)],
+    "gpugi_hostscene_material_textures": [_P, ctypes.c_uint32, _P],
     "gpugi_hostscene_info": [_P, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32),
                              ctypes.POINTER(ctypes.c_float * 3), ctypes.POINTER(ctypes.c_float * 3)],
     "gpugi_hostscene_add_point_light": [_P, ctypes.POINTER(ctypes.c_float * 3), ctypes.POINTER(ctypes.c_float * 3)],
@@ -286,6 +291,31 @@ class HostScene:
             "triangles_per_leaf": d.triangles_per_leaf,
         }
 
+    def textures(self):
+        """The material textures the scene's json references: a list of (h, w, 4)
+        arrays (uint8 sRGB or float32 linear), row 0 at t = 0.5 / h."""
+        n = ctypes.c_uint32()
+        _check("gpugi_hostscene_num_textures", lib().gpugi_hostscene_num_textures(self._h, ctypes.byref(n)))
+        out = []
+        for i in range(n.value):
+            w, h, f, p = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_void_p()
+            _check("gpugi_hostscene_texture", lib().gpugi_hostscene_texture(
+                self._h, i, ctypes.byref(w), ctypes.byref(h), ctypes.byref(f), ctypes.byref(p)))
+            dtype, size = (np.uint8, 4) if f.value == TEXTURE_RGBA8_SRGB else (np.float32, 16)
+            buf = (ctypes.c_char * (w.value * h.value * size)).from_address(p.value)
+            out.append(np.frombuffer(buf, dtype=dtype).copy().reshape(h.value, w.value, 4))
+        return out
+
+    def material_textures(self):
+        """(materials, 4) texture indices of the diffuse, opacity, reflectiveness and
+        emissivity slots (-1: the material's constant)."""
+        nm = self.desc().num_materials
+        out = np.full((nm, 4), -1, np.int32)
+        for m in range(nm):
+            _check("gpugi_hostscene_material_textures",
+                   lib().gpugi_hostscene_material_textures(self._h, m, _ptr(out[m])))
+        return out
+
     def info(self):
         nt, dep = ctypes.c_uint32(), ctypes.c_uint32()
         lo, hi = (ctypes.c_float * 3)(), (ctypes.c_float * 3)()
@@ -386,9 +416,17 @@ class Context:
         _check(name, getattr(lib(), name)(self._h, *args), self._h)
 
     def load_scene(self, scene):
+        """gpugi_load_scene, then the scene's material textures (if any) uploaded and bound."""
         d = scene.desc()
         self._call("gpugi_load_scene", ctypes.byref(d))
         self._scene = scene
+        texs = scene.textures()
+        if texs:
+            ids = [self.add_texture(t) for t in texs]
+            for m, slots in enumerate(scene.material_textures()):
+                for k, t in enumerate(slots):
+                    if t >= 0:
+                        self.set_material_texture(m, k, ids[t])
 
     def set_camera(self, cam):
         self._call("gpugi_set_camera", ctypes.byref(cam))

@@ -334,7 +334,24 @@ typedef struct gpugi_material_params {
   float opacity;
   float emissivity[3];
   float refraction_index;
+  /* "reflectiveness" constant (Vec4, scene.cpp:170-172) replacing the default
+   * (1, 1, 1, 1 / (roughness^2 + 1e-20)) when has_reflectiveness != 0 */
+  float reflectiveness[4];
+  int32_t has_reflectiveness;
 } gpugi_material_params;
+
+/* Textures a loaded scene's materials reference: the material json's string
+ * values for albedo, opacity, reflectiveness and emissivity name texture files
+ * relative to the scene file's directory (Scene::LoadMaterial and
+ * GetBindlessHandle, scene.cpp:155-203), read here as TGA, binary PPM or PFM
+ * (rows bottom first, t = 0.5 / height for row 0).  A caller uploads them with
+ * gpugi_add_texture (texels, format and size from gpugi_hostscene_texture,
+ * valid while the scene lives) and binds them with gpugi_set_material_texture
+ * (slots from gpugi_hostscene_material_textures, -1 for a constant slot). */
+int gpugi_hostscene_num_textures(const gpugi_hostscene* scene, uint32_t* n);
+int gpugi_hostscene_texture(const gpugi_hostscene* scene, uint32_t i, int* width, int* height, int* format,
+                            const void** texels);
+int gpugi_hostscene_material_textures(const gpugi_hostscene* scene, uint32_t material, int32_t slots[4]);
 
 /* Builds a scene from a triangle soup with the SAH sweep builder (restating
  * bvhmake/buildmethods/sweep.cpp:14-274 with leaf size triangles_per_leaf).

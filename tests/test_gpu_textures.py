@@ -163,3 +163,61 @@ def test_textures_are_dropped_with_the_scene(rt, tctx, scene):
     ctx.draw(rt.PT, 5)
     b, _ = ctx.readback(W, H)
     assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+def test_json_textures_through_load_scene_and_script(rt, oracle, tctx, scene, tmp_path):
+    """A scene whose material json names a TGA albedo texture: Context.load_scene
+    uploads and binds it (PT bit for bit against the oracle given the decoded
+    texels), and gpugi_render renders the same image from the files."""
+    import json
+    import os
+    import subprocess
+    from test_host_textures import tga
+    scene.save(str(tmp_path / "cube.rawscene"), str(tmp_path / "cube.json"))
+    rgb = np.random.default_rng(9).integers(0, 256, (6, 10, 3), dtype=np.uint8)
+    tga(tmp_path / "albedo.tga", rgb)
+    with open(tmp_path / "cube.json") as f:
+        j = json.load(f)
+    j["materials"]["Tex"]["albedo"] = "albedo.tga"
+    with open(tmp_path / "cube.json", "w") as f:
+        json.dump(j, f)
+    hs = rt.HostScene.load(str(tmp_path / "cube.rawscene"), str(tmp_path / "cube.json"))
+    ctx = tctx
+    ctx.load_scene(hs)
+    ctx.set_screen(W, H)
+    cam = rt.camera([0.1, 0.2, 0.3], [0.5, -0.3, -1.0], [0, 1, 0], 60, W, H)
+    ctx.set_camera(cam)
+    ctx.set_max_path_length(L)
+    ctx.set_integrator_kernel(rt.KERNEL_REFERENCE)
+    sampler = rt.LightSampler(hs, 123)
+    sc = oracle.Scene(hs.arrays())
+    acc = np.zeros((H, W, 4), np.float32)
+    print("case: json texture", flush=True)
+    oracle.set_textures(hs.material_textures(), [to_linear(t) if t.dtype == np.uint8 else t for t in hs.textures()])
+    try:
+        ls = sampler.generate(128)
+        ctx.set_light_samples(ls)
+        ctx.draw(rt.PT, rt.wanghash(0))
+        oracle.pathtrace(sc, ls, cam, W, H, rt.wanghash(0), L, acc)
+    finally:
+        oracle.set_textures(None, None)
+    img, _ = ctx.readback(W, H)
+    assert np.array_equal(img.view(np.uint32), acc.view(np.uint32))
+    # the script front end: same scene files, same first iteration
+    app = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "gpugi_amd", "bin", "gpugi_render")
+    out = str(tmp_path / "s.pfm")
+    lines = ["sceneFilename = " + str(tmp_path / "cube.rawscene"), "cameraPos = 0.1, 0.2, 0.3",
+             "cameraLookAt = 0.6, -0.1, -0.7", "cameraFOV = 60", "resolution = %d, %d" % (W, H),
+             "pathLength = %d" % L, "traversal = 1", "renderer = 0", "waitIterations = 1", "namedscreenshot = " + out,
+             "exit"]
+    args = [app]
+    for l in lines:
+        args += ["-e", l]
+    r = subprocess.run(args, capture_output=True, text=True, timeout=100)
+    assert r.returncode == 0, r.stdout + r.stderr
+    with open(out, "rb") as f:
+        f.readline(), f.readline(), f.readline()
+        simg = np.frombuffer(f.read(), dtype="<f4").reshape(H, W, 3)
+    assert simg.mean() > 0.1
+    plain = rt.HostScene.load(str(tmp_path / "cube.rawscene"), None)
+    assert plain.textures() == []
