@@ -560,6 +560,14 @@ def main():
         ctx.timer_begin()
         ctx.trace_device(R.incoherent, R.hits, n, rt.CLOSEST_HIT, rt.KERNEL_REFERENCE)
         extra["reference_walk_incoherent_grays"] = n / (ctx.timer_end() * 1e-3) / 1e9
+        # the same walk over oriented boxes (bvhType 1), then back to axis-aligned boxes
+        ctx.set_bvh_type(rt.BVH_OBOX)
+        ctx.load_scene(hs)
+        ctx.timer_begin()
+        ctx.trace_device(R.incoherent, R.hits, n, rt.CLOSEST_HIT, rt.KERNEL_REFERENCE)
+        extra["reference_walk_obox_incoherent_grays"] = n / (ctx.timer_end() * 1e-3) / 1e9
+        ctx.set_bvh_type(rt.BVH_AABOX)
+        ctx.load_scene(hs)  # the wide configuration is the context's and survives the reload
         # host buffers through the same entry point: upload of the rays, the launch and the hit download
         rays_np = R.incoherent.download(np.float32, n * 8).view(rt.RAY_DTYPE)
         ctx.trace(rays_np[: 1 << 20], rt.CLOSEST_HIT, kernel)

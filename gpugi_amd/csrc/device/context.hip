@@ -64,6 +64,8 @@ struct gpugi_ctx {
   int maxPathLength = 16;  // default pathLength (application.cpp:123)
   int rrStartDepth = 0;
   int triangleTest = GPUGI_TRIANGLE_REFERENCE;
+  int bvhType = GPUGI_BVH_AABOX;       // applies from the next scene load
+  bool sceneObox = false;              // dNodes holds TreeNode<OBox> (3 texels per node)
   int integratorKernel = GPUGI_KERNEL_AUTO;
   int bptCapacity = -1;
   int bptAverageLength = 0;
@@ -134,6 +136,7 @@ RefScene MakeRef(const gpugi_ctx* c) {
   s.slots = (const int4*)c->dSlots;
   s.positions = (const float*)c->dPos;
   s.vinfo = (const float4*)c->dVinfo;
+  s.obox = c->sceneObox ? 1 : 0;
   return s;
 }
 
@@ -327,14 +330,24 @@ int gpugi_load_scene(gpugi_ctx* ctx, const gpugi_scene_desc* d) {
   FreeDev(ctx->dMatTex);
   ctx->spillBytes = 0;
   ctx->hasScene = ctx->hasWide = false;
-  const size_t bNodes = (size_t)nn * 32, bSlots = (size_t)nl * 2 * 16, bPos = (size_t)nv * 12 + 4,
+  std::vector<float> oboxNodes;
+  if (ctx->bvhType == GPUGI_BVH_OBOX) {
+    try {
+      oboxNodes = FitOBoxHierarchy(d->hierarchy, nn, d->triangles, nl, d->positions, nv);
+    } catch (const std::bad_alloc&) {
+      return Fail(ctx, GPUGI_ERR_NOMEM, "gpugi_load_scene: oriented-box fit");
+    }
+  }
+  ctx->sceneObox = ctx->bvhType == GPUGI_BVH_OBOX;
+  const size_t bNodes = (size_t)nn * (ctx->sceneObox ? 48 : 32), bSlots = (size_t)nl * 2 * 16, bPos = (size_t)nv * 12 + 4,
                bInfo = (size_t)nv * 16, bMats = (size_t)d->num_materials * sizeof(gpugi_material);
   HIP_TRY(ctx, hipMalloc(&ctx->dNodes, bNodes));
   HIP_TRY(ctx, hipMalloc(&ctx->dSlots, bSlots));
   HIP_TRY(ctx, hipMalloc(&ctx->dPos, bPos));
   HIP_TRY(ctx, hipMalloc(&ctx->dVinfo, bInfo));
   HIP_TRY(ctx, hipMalloc(&ctx->dMats, bMats));
-  HIP_TRY(ctx, hipMemcpy(ctx->dNodes, d->hierarchy, bNodes, hipMemcpyHostToDevice));
+  HIP_TRY(ctx, hipMemcpy(ctx->dNodes, ctx->sceneObox ? oboxNodes.data() : d->hierarchy, bNodes,
+                         hipMemcpyHostToDevice));
   HIP_TRY(ctx, hipMemcpy(ctx->dSlots, d->triangles, bSlots, hipMemcpyHostToDevice));
   HIP_TRY(ctx, hipMemcpy(ctx->dPos, d->positions, (size_t)nv * 12, hipMemcpyHostToDevice));
   HIP_TRY(ctx, hipMemcpy(ctx->dVinfo, d->vertex_info, bInfo, hipMemcpyHostToDevice));
@@ -1335,6 +1348,14 @@ int gpugi_set_rr_start_depth(gpugi_ctx* ctx, int depth) {
   if (!ctx) return Fail(ctx, GPUGI_ERR_ARG, "gpugi_set_rr_start_depth: null context");
   if (depth < 0 || depth > 64) return Fail(ctx, GPUGI_ERR_ARG, "gpugi_set_rr_start_depth: depth in [0, 64]");
   ctx->rrStartDepth = depth;
+  return GPUGI_OK;
+}
+
+int gpugi_set_bvh_type(gpugi_ctx* ctx, int bvh_type) {
+  if (!ctx) return Fail(ctx, GPUGI_ERR_ARG, "gpugi_set_bvh_type: null context");
+  if (bvh_type != GPUGI_BVH_AABOX && bvh_type != GPUGI_BVH_OBOX)
+    return Fail(ctx, GPUGI_ERR_ARG, "gpugi_set_bvh_type: unknown type");
+  ctx->bvhType = bvh_type;
   return GPUGI_OK;
 }
 

@@ -16,6 +16,7 @@ struct RefScene {
   const int4* __restrict__ slots;       // TriangleBuffer, TBO 1
   const float* __restrict__ positions;  // VertexPositionBuffer RGB32F, TBO 2
   const float4* __restrict__ vinfo;     // VertexInfoBuffer, TBO 3
+  int obox = 0;                         // nodes are TreeNode<OBox>: 3 texels each (OBOX_BVH)
 };
 
 struct RefHit {
@@ -44,6 +45,23 @@ __device__ __forceinline__ bool intersect_box(gg::V3 o, gg::V3 inv, gg::V3 bmin,
   ty = fminf(tmax.x, tmax.z);
   lastHit = fminf(tx, ty);
   return firstHit <= lastHit;
+}
+
+// rotate (intersectiontests.glsl:82-88): x rotated by the quaternion q.
+__device__ __forceinline__ gg::V3 rotate_q(gg::V3 x, float4 q) {
+  const gg::V3 t = gg::cross(gg::v3(q.x, q.y, q.z), x);
+  return x + 2.0f * gg::v3(q.w * t.x + q.y * t.z - q.z * t.y, q.w * t.y + q.z * t.x - q.x * t.z,
+                           q.w * t.z + q.x * t.y - q.y * t.x);
+}
+
+// FetchIntersectOBoxNode (intersectiontests.glsl:114-128) after the fetch: the
+// ray in the box's frame against the centred box.
+__device__ __forceinline__ bool intersect_obox(gg::V3 o, gg::V3 d, float4 a, float4 b, float4 q, float& firstHit,
+                                               float& lastHit) {
+  const gg::V3 lo = rotate_q(o - gg::v3(a.x, a.y, a.z), q);
+  const gg::V3 ld = rotate_q(d, q);
+  const gg::V3 half = gg::v3(b.x, b.y, b.z);
+  return intersect_box(lo, gg::v3(1.0f / ld.x, 1.0f / ld.y, 1.0f / ld.z), -half, half, firstHit, lastHit);
 }
 
 // IntersectTriangle (intersectiontests.glsl:63-79), the OptiX-style branchless
@@ -140,13 +158,21 @@ __device__ __forceinline__ bool trace_ref(const RefScene& s, gg::V3 o, gg::V3 d,
   do {
     if (!nextIsLeafNode) {
       if (COUNT) ++nbox;
-      float4 a = s.nodes[2 * currentNodeIndex];
-      float4 b = s.nodes[2 * currentNodeIndex + 1];
+      float newHit, exitDist;
+      bool boxHit;
+      float4 a, b;
+      if (s.obox) {  // OBOX_BVH (traceray.glsl:68-69)
+        a = s.nodes[3 * currentNodeIndex];
+        b = s.nodes[3 * currentNodeIndex + 1];
+        boxHit = intersect_obox(o, d, a, b, s.nodes[3 * currentNodeIndex + 2], newHit, exitDist);
+      } else {
+        a = s.nodes[2 * currentNodeIndex];
+        b = s.nodes[2 * currentNodeIndex + 1];
+        boxHit = intersect_box(o, inv, gg::v3(a.x, a.y, a.z), gg::v3(b.x, b.y, b.z), newHit, exitDist);
+      }
       uint32_t childCode = __float_as_uint(a.w);
       int escape = __float_as_int(b.w);
-      float newHit, exitDist;
-      if (intersect_box(o, inv, gg::v3(a.x, a.y, a.z), gg::v3(b.x, b.y, b.z), newHit, exitDist) &&
-          newHit <= rayLength) {
+      if (boxHit && newHit <= rayLength) {
         lastNodeIndex = currentNodeIndex;
         currentNodeIndex = (int)(childCode & 0x7fffffffu);
         nextIsLeafNode = (uint32_t)currentNodeIndex != childCode;

@@ -189,6 +189,22 @@ int gpugi_hostscene_info(const gpugi_hostscene* scene, uint32_t* num_triangles, 
   return GPUGI_OK;
 }
 
+int gpugi_hostscene_obox_hierarchy(const gpugi_hostscene* scene, float* out, size_t num_floats) {
+  GPUGI_GUARD_BEGIN
+  if (!scene || !out) return GPUGI_ERR_ARG;
+  gpugi_scene_desc d;
+  scene->scene.Describe(&d);
+  if (num_floats < (size_t)d.num_nodes * 12) {
+    SetGlobalError("gpugi_hostscene_obox_hierarchy: need num_nodes * 12 floats");
+    return GPUGI_ERR_ARG;
+  }
+  const std::vector<float> h =
+      gpugi::FitOBoxHierarchy(d.hierarchy, d.num_nodes, d.triangles, d.num_leaves, d.positions, d.num_vertices);
+  std::memcpy(out, h.data(), h.size() * sizeof(float));
+  return GPUGI_OK;
+  GPUGI_GUARD_END
+}
+
 int gpugi_hostscene_num_textures(const gpugi_hostscene* scene, uint32_t* n) {
   if (!scene || !n) {
     SetGlobalError("gpugi_hostscene_num_textures: null argument");

@@ -54,6 +54,12 @@ struct HostTexture {
   std::string file;  // path it was read from
   std::string name;  // the material json's string (relative to the scene's directory)
 };
+// The oriented-box form of a reference-layout hierarchy (8 floats per node):
+// 12 floats per node {centre, firstChild}{half sides, escape}{rotationInv
+// quaternion x, y, z, w} (TreeNode<OBox>, scene.hpp:39-49).
+std::vector<float> FitOBoxHierarchy(const float* hier8, uint32_t numNodes, const int32_t* slots, uint32_t numLeaves,
+                                    const float* positions, uint32_t numVertices);
+
 // Reads a TGA, binary PPM or PFM file.  Returns an error message or "".
 std::string LoadTextureFile(const std::string& path, HostTexture* out);
 

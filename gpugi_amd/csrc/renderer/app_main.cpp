@@ -79,7 +79,7 @@ const char* kHelp =
     "pathLength = n                                                       maximum path length (default 16)\n"
     "renderer = 0 | 1 | 2                  0: Pathtracer, 1: LightPathtracer, 2: BidirectionalPathtracer\n"
     "traversal = 0 | 1 | 2                 integrator traversal: auto, reference escape walk, wide BVH\n"
-    "bvhType = 0                           axis-aligned boxes (oriented boxes are not supported)\n"
+    "bvhType = 0 | 1                       0: axis-aligned, 1: oriented boxes for the next scene load\n"
     "addPointLight = x, y, z, r, g, b      add a point light to the scene\n"
     "waitIterations = n                    draw n iterations\n"
     "waitSeconds = s                       draw until s seconds have passed\n"
@@ -108,7 +108,9 @@ class App {
       std::cout << std::chrono::duration<double>(std::chrono::steady_clock::now() - m_tic).count() << "s\n";
     } else if (c.name == "bvhType") {
       NeedArgs(c, 1);
-      if (AsInt(c.args[0]) != 0) throw std::invalid_argument("bvhType: only axis-aligned boxes (0) are supported");
+      const int t = AsInt(c.args[0]);
+      if (t != 0 && t != 1) throw std::invalid_argument("bvhType: 0 (axis-aligned boxes) or 1 (oriented boxes)");
+      System().SetBvhType(t);
     } else if (c.name == "sceneFilename") {
       NeedArgs(c, 1);
       LoadScene(c.args[0]);

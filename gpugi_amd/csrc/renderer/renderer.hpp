@@ -216,6 +216,9 @@ class RendererSystem {
   // Traversal the integrators use (GPUGI_KERNEL_*); applied on the next SetScene of a renderer.
   void SetTraversalKernel(int kernel) { m_traversalKernel = kernel; }
   int GetTraversalKernel() const { return m_traversalKernel; }
+  // GlobalConfig "bvhType" (application.cpp:135): GPUGI_BVH_AABOX or GPUGI_BVH_OBOX,
+  // used by the next SetScene, as the reference reads it when a scene loads.
+  void SetBvhType(int bvhType) { Check(gpugi_set_bvh_type(m_ctx, bvhType), m_ctx, "gpugi_set_bvh_type"); }
 
   // RendererSystem::SetEnvironmentMap (renderersystem.cpp:173-186): six
   // size x size faces in the order x-, x+, y-, y+, z-, z+, here binary PPM

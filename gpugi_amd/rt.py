@@ -19,6 +19,7 @@ PT, LT, BPT, PT_CACHE = 0, 1, 2, 3
 TRIANGLE_REFERENCE, TRIANGLE_WATERTIGHT = 0, 1
 CACHE_SPREAD, CACHE_PRIMARY_INDIRECT = 0, 1
 TEXTURE_RGBA8_SRGB, TEXTURE_RGBA32F = 0, 1
+BVH_AABOX, BVH_OBOX = 0, 1
 TEX_DIFFUSE, TEX_OPACITY, TEX_REFLECTIVENESS, TEX_EMISSIVITY = 0, 1, 2, 3
 RAY_MAX = np.float32(3.402823466e38)
 
@@ -117,6 +118,8 @@ _SIGNATURES = {
     "gpugi_hostscene_save": [_P, ctypes.c_char_p, ctypes.c_char_p],
     "gpugi_hostscene_desc": [_P, ctypes.POINTER(SceneDesc)],
     "gpugi_hostscene_num_textures": [_P, ctypes.POINTER(ctypes.c_uint32)],
+    "gpugi_hostscene_obox_hierarchy": [_P, _P, ctypes.c_size_t],
+    "gpugi_set_bvh_type": [_P, ctypes.c_int],
     "gpugi_hostscene_texture": [_P, ctypes.c_uint32, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                 ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_void_p)],
     "gpugi_hostscene_material_textures": [_P, ctypes.c_uint32, _P],
@@ -290,6 +293,13 @@ class HostScene:
             "materials": arr(d.materials, d.num_materials, MATERIAL_DTYPE),
             "triangles_per_leaf": d.triangles_per_leaf,
         }
+
+    def obox_hierarchy(self):
+        """The TreeNode<OBox> records (num_nodes x 12 float32) a GPUGI_BVH_OBOX load builds."""
+        n = self.desc().num_nodes
+        out = np.zeros((n, 12), np.float32)
+        _check("gpugi_hostscene_obox_hierarchy", lib().gpugi_hostscene_obox_hierarchy(self._h, _ptr(out), out.size))
+        return out
 
     def textures(self):
         """The material textures the scene's json references: a list of (h, w, 4)
@@ -491,6 +501,10 @@ class Context:
 
     def set_material_texture(self, material, slot, texture):
         self._call("gpugi_set_material_texture", int(material), int(slot), int(texture))
+
+    def set_bvh_type(self, bvh_type):
+        """bvhType (BVH_AABOX / BVH_OBOX); applies from the next load_scene."""
+        self._call("gpugi_set_bvh_type", int(bvh_type))
 
     def set_environment_map(self, faces):
         """RendererSystem::SetEnvironmentMap: faces (6, size, size, 3) uint8 sRGB in the

@@ -52,6 +52,13 @@ enum gpugi_kernel {
   GPUGI_KERNEL_WIDE = 2       /* ordered walk on the re-packed quantized 4-wide BVH */
 };
 
+/* Bounding volumes of the reference-layout hierarchy (the bvhType parameter,
+ * application.cpp:135: Types3D BOX or OBOX). */
+enum gpugi_bvh_type {
+  GPUGI_BVH_AABOX = 0, /* TreeNode<Box>, FetchIntersectBoxNode (intersectiontests.glsl:100-111) */
+  GPUGI_BVH_OBOX = 1   /* TreeNode<OBox>, FetchIntersectOBoxNode (intersectiontests.glsl:114-128) */
+};
+
 /* Which ray/triangle test traversal uses. */
 enum gpugi_triangle_test {
   GPUGI_TRIANGLE_REFERENCE = 0,  /* IntersectTriangle (intersectiontests.glsl:63-79): not watertight */
@@ -171,6 +178,14 @@ int gpugi_set_max_path_length(gpugi_ctx* ctx, int max_path_length);
  * as raw vertices; barycentrics keep the reference meaning (bary_y, bary_z
  * weight the second and third vertex). */
 int gpugi_set_triangle_test(gpugi_ctx* ctx, int mode);
+/* bvhType (application.cpp:135): like the reference it must be set before the
+ * scene is loaded and applies from the next gpugi_load_scene.  With
+ * GPUGI_BVH_OBOX the load fits an oriented box to every node of the given
+ * hierarchy (the reference reads them from its scene library, not in the
+ * tree; see gpugi_hostscene_obox_hierarchy) and the reference walk
+ * (GPUGI_KERNEL_REFERENCE, in gpugi_trace and the integrators) tests those;
+ * the wide kernel keeps its own axis-aligned 4-wide tree. */
+int gpugi_set_bvh_type(gpugi_ctx* ctx, int bvh_type);
 /* Bounce from which Russian roulette runs (default 0: every bounce, as
  * pathtracer.comp:99-104 and lighttracer.comp:183-188 do; 3 gives the
  * "roulette after depth 3" variant, which has no reference counterpart). */
@@ -348,6 +363,13 @@ typedef struct gpugi_material_params {
  * gpugi_add_texture (texels, format and size from gpugi_hostscene_texture,
  * valid while the scene lives) and binds them with gpugi_set_material_texture
  * (slots from gpugi_hostscene_material_textures, -1 for a constant slot). */
+/* The oriented-box hierarchy gpugi_load_scene builds under GPUGI_BVH_OBOX for
+ * this scene: num_nodes x 12 floats {centre.xyz, firstChild}{halfSides.xyz,
+ * escape}{rotationInv quaternion xyzw} (TreeNode<OBox>, scene.hpp:39-49).  Each
+ * node's box holds its subtree's triangles: principal or world axes, whichever
+ * gives the smaller surface, exact under the float quaternion, widened by
+ * 4e-6 x the scene's largest coordinate for the walk's float arithmetic. */
+int gpugi_hostscene_obox_hierarchy(const gpugi_hostscene* scene, float* out, size_t num_floats);
 int gpugi_hostscene_num_textures(const gpugi_hostscene* scene, uint32_t* n);
 int gpugi_hostscene_texture(const gpugi_hostscene* scene, uint32_t i, int* width, int* height, int* format,
                             const void** texels);

@@ -31,6 +31,7 @@ typedef struct {
   uint32_t num_vertices;
   const float* materials;   /* num_materials x 20 (gpugi_material) */
   uint32_t num_materials;
+  const float* obox;        /* NULL, or num_nodes x 12 (TreeNode<OBox>): the walk tests these boxes */
 } or_scene;
 
 typedef struct {

@@ -26,7 +26,7 @@ class _Scene(ctypes.Structure):
                 ("triangles", ctypes.c_void_p), ("num_leaves", ctypes.c_uint32),
                 ("positions", ctypes.c_void_p), ("vertex_info", ctypes.c_void_p),
                 ("num_vertices", ctypes.c_uint32), ("materials", ctypes.c_void_p),
-                ("num_materials", ctypes.c_uint32)]
+                ("num_materials", ctypes.c_uint32), ("obox", ctypes.c_void_p)]
 
 
 class _Camera(ctypes.Structure):
@@ -109,9 +109,13 @@ class Scene:
         mats = self.a["materials"]
         self.mat_floats = np.ascontiguousarray(mats.view(np.float32).reshape(len(mats), -1))
         assert self.mat_floats.shape[1] == 20
+        obox = self.a.get("obox")  # optional TreeNode<OBox> records (num_nodes x 12 float32)
+        if obox is not None:
+            assert obox.dtype == np.float32 and obox.shape == (len(self.a["hierarchy"]), 12)
         self.c = _Scene(_p(self.a["hierarchy"]), len(self.a["hierarchy"]), _p(self.a["triangles"]),
                         len(self.a["triangles"]) // 2, _p(self.a["positions"]), _p(self.a["vertex_info"]),
-                        len(self.a["positions"]), _p(self.mat_floats), len(mats))
+                        len(self.a["positions"]), _p(self.mat_floats), len(mats),
+                        _p(obox) if obox is not None else None)
 
     @property
     def ptr(self):

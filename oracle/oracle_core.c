@@ -162,6 +162,22 @@ static int or_box(vec3 o, vec3 inv, vec3 mn, vec3 mx, float* first, float* last)
   return *first <= *last;
 }
 
+/* rotate (intersectiontests.glsl:82-88) */
+static vec3 or_rotate(vec3 x, const float* q) {
+  vec3 t = vcross(v3(q[0], q[1], q[2]), x);
+  vec3 r = v3(q[3] * t.x + q[1] * t.z - q[2] * t.y, q[3] * t.y + q[2] * t.x - q[0] * t.z,
+              q[3] * t.z + q[0] * t.y - q[1] * t.x);
+  return vadd(x, vscale(2.0f, r));
+}
+
+/* FetchIntersectOBoxNode (intersectiontests.glsl:114-128) for node record h (12 floats) */
+static int or_obox(vec3 o, vec3 d, const float* h, float* first, float* last) {
+  vec3 lo = or_rotate(vsub(o, v3(h[0], h[1], h[2])), h + 8);
+  vec3 ld = or_rotate(d, h + 8);
+  vec3 half = v3(h[4], h[5], h[6]);
+  return or_box(lo, v3(1.0f / ld.x, 1.0f / ld.y, 1.0f / ld.z), vscale(-1.0f, half), half, first, last);
+}
+
 /* IntersectTriangle (intersectiontests.glsl:63-79) */
 int or_triangle(vec3 o, vec3 d, vec3 p0, vec3 p1, vec3 p2, float* t, vec3* b, vec3* n) {
   vec3 e0 = vsub(p1, p0);
@@ -247,12 +263,14 @@ static int trace_one(const or_scene* s, int any, int wt, vec3 o, vec3 d, float* 
   *slot = -1;
   do {
     if (!nextIsLeaf) {
-      const float* h = s->hierarchy + (size_t)node * 8;
+      const float* h = s->obox ? s->obox + (size_t)node * 12 : s->hierarchy + (size_t)node * 8;
       uint32_t childCode = bits_of(h[3]);
       int escape = (int)bits_of(h[7]);
       float newHit, exitDist;
       if (nbox) ++*nbox;
-      if (or_box(o, inv, v3(h[0], h[1], h[2]), v3(h[4], h[5], h[6]), &newHit, &exitDist) && newHit <= *rayLength) {
+      int boxHit = s->obox ? or_obox(o, d, h, &newHit, &exitDist)
+                           : or_box(o, inv, v3(h[0], h[1], h[2]), v3(h[4], h[5], h[6]), &newHit, &exitDist);
+      if (boxHit && newHit <= *rayLength) {
         node = (int)(childCode & 0x7fffffffu);
         nextIsLeaf = (uint32_t)node != childCode;
         if (nextIsLeaf) {

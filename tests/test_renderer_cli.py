@@ -44,7 +44,7 @@ def test_comments_blank_lines_and_exit_stop_processing():
     ("renderer = x", "not an integer"),
     ("cameraPos = 1, 2", "expects 3 argument"),
     ("resolution = 64", "expects 2 argument"),
-    ("bvhType = 1", "only axis-aligned boxes"),
+    ("bvhType = 2", "0 (axis-aligned boxes) or 1 (oriented boxes)"),
     ("cameraFOV = abc", "not a number"),
     ("mseCheckInterval = x", "not an integer"),
     ("waitMSE = 1, 2", "expects 1 argument"),
