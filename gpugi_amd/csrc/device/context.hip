@@ -873,6 +873,42 @@ int gpugi_trace(gpugi_ctx* ctx, int query, int kernel, const gpugi_ray* rays, gp
   return GPUGI_OK;
 }
 
+int gpugi_trace_importance_break(gpugi_ctx* ctx, int query, const gpugi_ray* rays, gpugi_hit* hits,
+                                 gpugi_node_hit* nodes, size_t n, const float* importance, uint32_t num_nodes,
+                                 float threshold, float ray_threshold) {
+  if (!ctx || (n > 0 && (!rays || !hits)) || !importance)
+    return Fail(ctx, GPUGI_ERR_ARG, "gpugi_trace_importance_break: null argument");
+  if (query != GPUGI_CLOSEST_HIT && query != GPUGI_ANY_HIT)
+    return Fail(ctx, GPUGI_ERR_ARG, "gpugi_trace_importance_break: bad query");
+  if (!ctx->hasScene) return Fail(ctx, GPUGI_ERR_STATE, "gpugi_trace_importance_break: no scene loaded");
+  if (num_nodes != ctx->numNodes)
+    return Fail(ctx, GPUGI_ERR_ARG, "gpugi_trace_importance_break: need one importance pair per node");
+  if (n > 0xffffff00ull) return Fail(ctx, GPUGI_ERR_ARG, "gpugi_trace_importance_break: at most 2^32 - 256 rays");
+  if (n == 0) return GPUGI_OK;
+  if (SetDev(ctx)) return GPUGI_ERR_HIP;
+  const size_t bRays = n * sizeof(gpugi_ray), bHits = n * sizeof(gpugi_hit), bNodes = n * sizeof(gpugi_node_hit);
+  const size_t bImp = (size_t)num_nodes * 8;
+  if (int rc = EnsureScratch(ctx, bRays + bHits + bNodes + bImp)) return rc;
+  char* base = (char*)ctx->dScratch;
+  gpugi_ray* dr = (gpugi_ray*)base;
+  gpugi_hit* dh = (gpugi_hit*)(base + bRays);
+  int4* dn = (int4*)(base + bRays + bHits);
+  float2* di = (float2*)(base + bRays + bHits + bNodes);
+  HIP_TRY(ctx, hipMemcpyAsync(dr, rays, bRays, hipMemcpyHostToDevice, ctx->stream));
+  HIP_TRY(ctx, hipMemcpyAsync(di, importance, bImp, hipMemcpyHostToDevice, ctx->stream));
+  ImportanceBreak ib;
+  ib.importance = di;
+  ib.threshold = threshold;
+  ib.rayThreshold = ray_threshold;
+  HIP_TRY(ctx, LaunchTraceRefImportance(MakeRef(ctx), query, dr, dh, nodes ? dn : nullptr, n, ib, Watertight(ctx),
+                                        ctx->stream));
+  HIP_TRY(ctx, hipMemcpyAsync(hits, dh, bHits, hipMemcpyDeviceToHost, ctx->stream));
+  if (nodes && query == GPUGI_CLOSEST_HIT)
+    HIP_TRY(ctx, hipMemcpyAsync(nodes, dn, bNodes, hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return GPUGI_OK;
+}
+
 int gpugi_set_wide_config(gpugi_ctx* ctx, int max_leaf_triangles, int lds_stack_entries, int refill_lanes,
                           int wait_lanes) {
   if (!ctx) return Fail(ctx, GPUGI_ERR_ARG, "gpugi_set_wide_config: null context");

@@ -11,6 +11,11 @@ namespace gpugi {
 // watertight: the watertight triangle test in place of the reference one.
 hipError_t LaunchTraceRef(const RefScene& s, int query, const gpugi_ray* rays, gpugi_hit* hits, size_t n,
                           bool watertight, hipStream_t stream);
+// The TRACERAY_IMPORTANCE_BREAK walk (ib.importance: one float2 per node);
+// nodes (optional, closest hit) receive {node, importance bits, size^2 bits, 0}.
+hipError_t LaunchTraceRefImportance(const RefScene& s, int query, const gpugi_ray* rays, gpugi_hit* hits,
+                                    int4* nodes, size_t n, const ImportanceBreak& ib, bool watertight,
+                                    hipStream_t stream);
 hipError_t LaunchTraceRefCounts(const RefScene& s, int query, const gpugi_ray* rays, uint32_t* nbox,
                                 uint32_t* ntri, size_t n, hipStream_t stream);
 // Wide kernel with an LDS stack of `stack` entries per lane (one of kWideStackVariants).

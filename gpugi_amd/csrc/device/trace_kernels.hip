@@ -28,6 +28,29 @@ __global__ __launch_bounds__(256) void k_trace_ref(RefScene s, const float4* __r
   }
 }
 
+// TRACERAY_IMPORTANCE_BREAK walk; nodes (optional) get {hit node, node
+// importance bits, node size^2 bits, 0} per ray.
+template <bool ANY, bool WT>
+__global__ __launch_bounds__(256) void k_trace_ref_ib(RefScene s, const float4* __restrict__ rays,
+                                                      int4* __restrict__ hits, int4* __restrict__ nodes, uint32_t n,
+                                                      ImportanceBreak ib) {
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i >= n) return;
+  const float4 r0 = rays[2 * (size_t)i], r1 = rays[2 * (size_t)i + 1];
+  gg::V3 o = gg::v3(r0.x, r0.y, r0.z), d = gg::v3(r1.x, r1.y, r1.z);
+  float rayLength = r0.w;
+  RefHit h;
+  uint32_t nb = 0, nt = 0;
+  const bool any = trace_ref<ANY, false, WT>(s, o, d, rayLength, h, nb, nt, ib);
+  if (ANY) {
+    hits[i] = make_int4(any ? 1 : 0, 0, 0, 0);
+  } else {
+    hits[i] = make_int4(h.slot, __float_as_int(rayLength), __float_as_int(h.slot >= 0 ? h.bary.y : 0.0f),
+                        __float_as_int(h.slot >= 0 ? h.bary.z : 0.0f));
+    if (nodes) nodes[i] = make_int4(h.node, __float_as_int(h.nodeImportance), __float_as_int(h.nodeSizeSq), 0);
+  }
+}
+
 template <bool ANY>
 __global__ __launch_bounds__(256) void k_trace_ref_counts(RefScene s, const float4* __restrict__ rays,
                                                           uint32_t* __restrict__ nbox,
@@ -42,6 +65,24 @@ __global__ __launch_bounds__(256) void k_trace_ref_counts(RefScene s, const floa
   trace_ref<ANY, true>(s, o, d, rayLength, h, nb, nt);
   nbox[i] = nb;
   ntri[i] = nt;
+}
+
+hipError_t LaunchTraceRefImportance(const RefScene& s, int query, const gpugi_ray* rays, gpugi_hit* hits,
+                                    int4* nodes, size_t n, const ImportanceBreak& ib, bool watertight,
+                                    hipStream_t stream) {
+  if (n == 0) return hipSuccess;
+  dim3 grid((unsigned)((n + 255) / 256));
+  const float4* r = (const float4*)rays;
+  int4* h = (int4*)hits;
+  const uint32_t m = (uint32_t)n;
+  if (query == GPUGI_ANY_HIT) {
+    if (watertight) hipLaunchKernelGGL((k_trace_ref_ib<true, true>), grid, dim3(256), 0, stream, s, r, h, nodes, m, ib);
+    else hipLaunchKernelGGL((k_trace_ref_ib<true, false>), grid, dim3(256), 0, stream, s, r, h, nodes, m, ib);
+  } else {
+    if (watertight) hipLaunchKernelGGL((k_trace_ref_ib<false, true>), grid, dim3(256), 0, stream, s, r, h, nodes, m, ib);
+    else hipLaunchKernelGGL((k_trace_ref_ib<false, false>), grid, dim3(256), 0, stream, s, r, h, nodes, m, ib);
+  }
+  return hipGetLastError();
 }
 
 hipError_t LaunchTraceRef(const RefScene& s, int query, const gpugi_ray* rays, gpugi_hit* hits, size_t n,

@@ -19,6 +19,19 @@ struct RefScene {
   int obox = 0;                         // nodes are TreeNode<OBox>: 3 texels each (OBOX_BVH)
 };
 
+// TRACERAY_IMPORTANCE_BREAK (traceray.glsl:48-112): per-node importance {x, y}
+// (HierarchyImportanceBuffer, RG32F), the IMPORTANCE_THRESHOLD define and the
+// _importanceThreshold argument.  A hit node with (x < threshold and newHit > 0)
+// or y <= rayThreshold * newHit stops the descent: its subtree is skipped, and
+// the closest-hit walk records a hit at a pseudo-random distance inside the
+// node's box interval, the any-hit walk returns true when that interval ends
+// before the ray does.  importance == nullptr: the plain walk.
+struct ImportanceBreak {
+  const float2* __restrict__ importance = nullptr;
+  float threshold = 0.0f;
+  float rayThreshold = 0.0f;
+};
+
 struct RefHit {
   float t;           // rayLength
   int slot;          // _hitIndex.y
@@ -26,6 +39,8 @@ struct RefHit {
   gg::V3 bary;       // barycentricCoord
   gg::V3 triNormal;  // TRINORMAL_OUTPUT (not normalized)
   int4 tri;          // outTriangle
+  float nodeImportance = 0.0f;  // _nodeImportance (importance break)
+  float nodeSizeSq = 0.0f;      // _nodeSizeSq
 };
 
 __device__ __forceinline__ gg::V3 load_pos(const float* __restrict__ p, int v) {
@@ -147,7 +162,9 @@ __device__ __forceinline__ bool intersect_triangle_wt(gg::V3 o, const WtRay& r, 
 // triangle test in place of IntersectTriangle (triNormal is then not set).
 template <bool ANY, bool COUNT, bool WT = false>
 __device__ __forceinline__ bool trace_ref(const RefScene& s, gg::V3 o, gg::V3 d, float& rayLength,
-                                          RefHit& out, uint32_t& nbox, uint32_t& ntri) {
+                                          RefHit& out, uint32_t& nbox, uint32_t& ntri,
+                                          const ImportanceBreak& ib = ImportanceBreak{}) {
+  float lastNodeImportance = 0.0f, lastNodeSizeSq = 0.0f;
   const WtRay wr = WT ? wt_setup(d) : WtRay{};
   int currentNodeIndex = 0;
   int currentLeafIndex = 0;
@@ -172,13 +189,46 @@ __device__ __forceinline__ bool trace_ref(const RefScene& s, gg::V3 o, gg::V3 d,
       }
       uint32_t childCode = __float_as_uint(a.w);
       int escape = __float_as_int(b.w);
+      bool broke = false;
       if (boxHit && newHit <= rayLength) {
         lastNodeIndex = currentNodeIndex;
-        currentNodeIndex = (int)(childCode & 0x7fffffffu);
-        nextIsLeafNode = (uint32_t)currentNodeIndex != childCode;
-        if (nextIsLeafNode) {
-          currentLeafIndex = 2 * currentNodeIndex;
-          currentNodeIndex = escape;
+        if (ib.importance) {
+          const float2 importance = ib.importance[currentNodeIndex];
+          // nodeSizeSq of FetchIntersect(O)BoxNode (intersectiontests.glsl:110,126)
+          float sizeSq;
+          if (s.obox) {
+            const gg::V3 h = gg::v3(b.x, b.y, b.z);
+            sizeSq = gg::dot(h, h) * 4.0f;
+          } else {
+            const gg::V3 e = gg::v3(b.x, b.y, b.z) - gg::v3(a.x, a.y, a.z);
+            sizeSq = gg::dot(e, e);
+          }
+          if (!ANY) {
+            lastNodeImportance = importance.x;
+            lastNodeSizeSq = sizeSq;
+          }
+          if ((importance.x < ib.threshold && newHit > 0.0f) || importance.y <= ib.rayThreshold * newHit) {
+            broke = true;
+            if (ANY) {
+              if (exitDist <= rayLength) return true;
+            } else {
+              out.nodeImportance = importance.x;
+              out.nodeSizeSq = lastNodeSizeSq;
+              out.node = currentNodeIndex;
+              rayLength = newHit + (exitDist - newHit) *
+                                       ((float)(gg::wanghash(__float_as_uint(exitDist)) % 8388593u) / 8388593.0f);
+              out.slot = -1;
+            }
+            currentNodeIndex = escape;
+          }
+        }
+        if (!broke) {
+          currentNodeIndex = (int)(childCode & 0x7fffffffu);
+          nextIsLeafNode = (uint32_t)currentNodeIndex != childCode;
+          if (nextIsLeafNode) {
+            currentLeafIndex = 2 * currentNodeIndex;
+            currentNodeIndex = escape;
+          }
         }
       } else {
         currentNodeIndex = escape;
@@ -205,6 +255,8 @@ __device__ __forceinline__ bool trace_ref(const RefScene& s, gg::V3 o, gg::V3 d,
           out.triNormal = n;
           out.node = lastNodeIndex;
           out.slot = currentLeafIndex;
+          out.nodeImportance = lastNodeImportance;
+          out.nodeSizeSq = lastNodeSizeSq;
         }
         ++currentLeafIndex;
         nextIsLeafNode = (currentLeafIndex % 2) != 0;

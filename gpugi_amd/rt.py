@@ -25,6 +25,7 @@ RAY_MAX = np.float32(3.402823466e38)
 
 RAY_DTYPE = np.dtype([("origin", "<f4", 3), ("tmax", "<f4"), ("dir", "<f4", 3), ("pad", "<u4")])
 HIT_DTYPE = np.dtype([("slot", "<i4"), ("t", "<f4"), ("bary_y", "<f4"), ("bary_z", "<f4")])
+NODE_HIT_DTYPE = np.dtype([("node", "<i4"), ("node_importance", "<f4"), ("node_size_sq", "<f4"), ("pad", "<i4")])
 LIGHT_DTYPE = np.dtype([("position", "<f4", 3), ("normal_phi", "<f4"), ("intensity", "<f4", 3),
                         ("normal_theta_cos", "<f4")])
 LIGHT_CACHE_DTYPE = np.dtype([("flux", "<f4", 3), ("material", "<i4"), ("position", "<f4", 3), ("normal0", "<f4"),
@@ -120,6 +121,8 @@ _SIGNATURES = {
     "gpugi_hostscene_num_textures": [_P, ctypes.POINTER(ctypes.c_uint32)],
     "gpugi_hostscene_obox_hierarchy": [_P, _P, ctypes.c_size_t],
     "gpugi_set_bvh_type": [_P, ctypes.c_int],
+    "gpugi_trace_importance_break": [_P, ctypes.c_int, _P, _P, _P, ctypes.c_size_t, _P, ctypes.c_uint32,
+                                     ctypes.c_float, ctypes.c_float],
     "gpugi_hostscene_texture": [_P, ctypes.c_uint32, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                 ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_void_p)],
     "gpugi_hostscene_material_textures": [_P, ctypes.c_uint32, _P],
@@ -535,6 +538,18 @@ class Context:
         hits = np.zeros(len(rays), dtype=HIT_DTYPE)
         self._call("gpugi_trace", int(query), int(kernel), _ptr(rays), _ptr(hits), ctypes.c_size_t(len(rays)), 0)
         return hits
+
+    def trace_importance_break(self, rays, importance, threshold, ray_threshold, query=CLOSEST_HIT):
+        """TraceRay with TRACERAY_IMPORTANCE_BREAK: importance (num_nodes, 2) float32.
+        Returns (hits, nodes) for closest hit (nodes: NODE_HIT_DTYPE), hits for any hit."""
+        rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
+        imp = np.ascontiguousarray(importance, dtype=np.float32).reshape(-1, 2)
+        hits = np.zeros(len(rays), dtype=HIT_DTYPE)
+        nodes = np.zeros(len(rays), dtype=NODE_HIT_DTYPE)
+        self._call("gpugi_trace_importance_break", int(query), _ptr(rays), _ptr(hits), _ptr(nodes),
+                   ctypes.c_size_t(len(rays)), _ptr(imp), len(imp), ctypes.c_float(threshold),
+                   ctypes.c_float(ray_threshold))
+        return hits if query == ANY_HIT else (hits, nodes)
 
     def trace_device(self, rays_buf, hits_buf, n, query=CLOSEST_HIT, kernel=KERNEL_AUTO):
         self._call("gpugi_trace", int(query), int(kernel), ctypes.c_void_p(rays_buf.ptr),

@@ -270,6 +270,30 @@ int gpugi_set_reference_image(gpugi_ctx* ctx, const float* rgb, int width, int h
  * shader's division does.  Blocks until the value is on the host. */
 int gpugi_compute_mse(gpugi_ctx* ctx, uint32_t divisor, float* mse);
 
+/* Per-ray extra output of the importance-break walk: _hitIndex.x,
+ * _nodeImportance and _nodeSizeSq of TraceRay (traceray.glsl:25-31). */
+typedef struct gpugi_node_hit {
+  int32_t node;            /* last hit inner node (the node the walk stopped in after a break), -1: none */
+  float node_importance;   /* importance.x of that node */
+  float node_size_sq;      /* |box diagonal|^2 of that node */
+  int32_t pad;
+} gpugi_node_hit;
+
+/* TraceRay / TraceRayAnyHit compiled with TRACERAY_IMPORTANCE_BREAK
+ * (traceray.glsl:48-112), the escape walk of GPUGI_KERNEL_REFERENCE (either
+ * bvh type).  importance: num_nodes pairs {x, y} (HierarchyImportanceBuffer);
+ * threshold: the IMPORTANCE_THRESHOLD define; ray_threshold: the
+ * _importanceThreshold argument.  When a node's box is hit at newHit (within
+ * the ray) and x < threshold with newHit > 0, or y <= ray_threshold * newHit,
+ * its subtree is skipped: a closest-hit query then ends there with slot -1 and
+ * t = newHit + (exit - newHit) * (wanghash(bits(exit)) % 8388593) / 8388593
+ * unless a closer triangle follows, an any-hit query reports a hit if the box
+ * interval ends within the ray.  Host arrays; nodes (closest hit only) may be NULL; blocks until
+ * the results are copied back. */
+int gpugi_trace_importance_break(gpugi_ctx* ctx, int query, const gpugi_ray* rays, gpugi_hit* hits,
+                                 gpugi_node_hit* nodes, size_t n, const float* importance, uint32_t num_nodes,
+                                 float threshold, float ray_threshold);
+
 /* Ray queries for parity and the traversal microbenchmark (the GLSL
  * TraceRay/TraceRayAnyHit functions, traceray.glsl:18-39).  rays and hits
  * are host pointers when rays_on_device == 0 (the call then blocks until the

@@ -93,6 +93,11 @@ void or_bsdf_evals(const float* mat20, const float* inc, const float* N, int adj
  * texels as 6 faces (x-, x+, y-, y+, z-, z+) of size x size linear RGB, 4
  * floats each; the array must outlive the draws. */
 void or_set_environment(const float* texels, int size);
+/* TraceRay / TraceRayAnyHit with TRACERAY_IMPORTANCE_BREAK (traceray.glsl:48-112):
+ * importance holds {x, y} per node; nodes (closest hit, optional) get
+ * {node, importance.x bits, size^2 bits, 0} per ray. */
+void or_trace_importance(const or_scene* s, int any, const float* rays, int32_t* hits, int32_t* nodes, size_t n,
+                         const float* importance, float threshold, float rayThreshold);
 /* Material textures for every integrator (NULL: none): matTex holds 4 texture
  * indices per material (diffuse, opacity, reflectiveness, emissivity; -1 for
  * the record's constant), texels[i] the linear RGBA floats of texture i and

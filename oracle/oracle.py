@@ -75,6 +75,9 @@ def lib():
         l.or_bsdf_evals.argtypes = [P, P, P, ctypes.c_int, P, ctypes.c_size_t, P]
         l.or_bsdf_evals.restype = None
         l.or_set_environment.argtypes, l.or_set_environment.restype = [P, ctypes.c_int], None
+        l.or_trace_importance.argtypes = [P, ctypes.c_int, P, P, P, ctypes.c_size_t, P, ctypes.c_float,
+                                          ctypes.c_float]
+        l.or_trace_importance.restype = None
         l.or_set_textures.argtypes, l.or_set_textures.restype = [P, ctypes.c_uint32, P, P], None
         l.or_texture_samples.argtypes, l.or_texture_samples.restype = [ctypes.c_int, P, ctypes.c_size_t, P], None
         l.or_env_lookups.argtypes, l.or_env_lookups.restype = [P, ctypes.c_int, P, ctypes.c_size_t, P], None
@@ -143,6 +146,20 @@ def trace(scene, rays, any_hit=False, counts=False, watertight=False):
     lib().or_trace_mode(scene.ptr, int(any_hit), int(watertight), _p(r), _p(hits), _p(nb) if counts else None,
                         _p(nt) if counts else None, n)
     return (hits, nb, nt) if counts else hits
+
+
+def trace_importance(scene, rays, importance, threshold, ray_threshold, any_hit=False):
+    """The TRACERAY_IMPORTANCE_BREAK walk: returns (hits n x 4, nodes n x 4 {node,
+    importance bits, size^2 bits, 0}) for closest hit, hits for any hit."""
+    r = _rays_f32(rays)
+    n = len(r)
+    hits = np.zeros((n, 4), dtype=np.int32)
+    nodes = np.zeros((n, 4), dtype=np.int32)
+    imp = np.ascontiguousarray(importance, dtype=np.float32).reshape(-1, 2)
+    assert len(imp) == scene.c.num_nodes
+    lib().or_trace_importance(scene.ptr, int(any_hit), _p(r), _p(hits), _p(nodes), n, _p(imp), float(threshold),
+                              float(ray_threshold))
+    return hits if any_hit else (hits, nodes)
 
 
 def bruteforce(scene, rays, any_hit=False, watertight=False):

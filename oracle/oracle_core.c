@@ -254,9 +254,21 @@ static int tri_test(int wt, vec3 o, vec3 d, const or_wtray* r, vec3 p0, vec3 p1,
 
 /* TraceRay / TraceRayAnyHit, TRIANGLES_PER_LEAF = 2 (traceray.glsl:41-175),
  * with the reference triangle test (wt = 0) or the watertight one. */
-static int trace_one(const or_scene* s, int any, int wt, vec3 o, vec3 d, float* rayLength, int32_t* slot,
-                     vec3* bary, vec3* triNormal, int32_t tri[4], uint32_t* nbox, uint32_t* ntri) {
+/* TRACERAY_IMPORTANCE_BREAK parameters (traceray.glsl:48-112): one {x, y}
+ * pair per node, the IMPORTANCE_THRESHOLD define, the _importanceThreshold
+ * argument. */
+typedef struct {
+  const float* importance;
+  float threshold, rayThreshold;
+} or_imp;
+
+/* nodeOut (closest hit, optional): {last hit node, its importance.x, its size^2} */
+static int trace_one_ib(const or_scene* s, int any, int wt, vec3 o, vec3 d, float* rayLength, int32_t* slot,
+                        vec3* bary, vec3* triNormal, int32_t tri[4], uint32_t* nbox, uint32_t* ntri,
+                        const or_imp* ib, float* nodeOut) {
   const or_wtray wr = or_wt_setup(d);
+  int lastNode = 0, hitNode = -1;
+  float lastImp = 0.0f, lastSize = 0.0f, hitImp = 0.0f, hitSize = 0.0f;
   int node = 0, leaf = 0;
   int nextIsLeaf = 0;
   vec3 inv = v3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
@@ -271,11 +283,43 @@ static int trace_one(const or_scene* s, int any, int wt, vec3 o, vec3 d, float* 
       int boxHit = s->obox ? or_obox(o, d, h, &newHit, &exitDist)
                            : or_box(o, inv, v3(h[0], h[1], h[2]), v3(h[4], h[5], h[6]), &newHit, &exitDist);
       if (boxHit && newHit <= *rayLength) {
-        node = (int)(childCode & 0x7fffffffu);
-        nextIsLeaf = (uint32_t)node != childCode;
-        if (nextIsLeaf) {
-          leaf = 2 * node;
-          node = escape;
+        int broke = 0;
+        lastNode = node;
+        if (ib) {
+          const float ix = ib->importance[2 * (size_t)node], iy = ib->importance[2 * (size_t)node + 1];
+          float sizeSq;
+          if (s->obox) {
+            vec3 hh = v3(h[4], h[5], h[6]);
+            sizeSq = vdot(hh, hh) * 4.0f;
+          } else {
+            vec3 e = vsub(v3(h[4], h[5], h[6]), v3(h[0], h[1], h[2]));
+            sizeSq = vdot(e, e);
+          }
+          if (!any) {
+            lastImp = ix;
+            lastSize = sizeSq;
+          }
+          if ((ix < ib->threshold && newHit > 0.0f) || iy <= ib->rayThreshold * newHit) {
+            broke = 1;
+            if (any) {
+              if (exitDist <= *rayLength) return 1;
+            } else {
+              hitImp = ix;
+              hitSize = lastSize;
+              hitNode = node;
+              *rayLength = newHit + (exitDist - newHit) * ((float)(or_wanghash(bits_of(exitDist)) % 8388593u) / 8388593.0f);
+              *slot = -1;
+            }
+            node = escape;
+          }
+        }
+        if (!broke) {
+          node = (int)(childCode & 0x7fffffffu);
+          nextIsLeaf = (uint32_t)node != childCode;
+          if (nextIsLeaf) {
+            leaf = 2 * node;
+            node = escape;
+          }
         }
       } else {
         node = escape;
@@ -294,6 +338,9 @@ static int trace_one(const or_scene* s, int any, int wt, vec3 o, vec3 d, float* 
           if (any) return 1;
           *rayLength = hit;
           *slot = leaf;
+          hitNode = lastNode;
+          hitImp = lastImp;
+          hitSize = lastSize;
           *bary = b;
           if (triNormal) *triNormal = n;
           if (tri) memcpy(tri, t, 16);
@@ -303,7 +350,46 @@ static int trace_one(const or_scene* s, int any, int wt, vec3 o, vec3 d, float* 
       }
     }
   } while (node != 0 || nextIsLeaf);
+  if (nodeOut) {
+    memcpy(&nodeOut[0], &hitNode, 4);
+    nodeOut[1] = hitImp;
+    nodeOut[2] = hitSize;
+  }
   return 0;
+}
+
+static int trace_one(const or_scene* s, int any, int wt, vec3 o, vec3 d, float* rayLength, int32_t* slot,
+                     vec3* bary, vec3* triNormal, int32_t tri[4], uint32_t* nbox, uint32_t* ntri) {
+  return trace_one_ib(s, any, wt, o, d, rayLength, slot, bary, triNormal, tri, nbox, ntri, NULL, NULL);
+}
+
+void or_trace_importance(const or_scene* s, int any, const float* rays, int32_t* hits, int32_t* nodes, size_t n,
+                         const float* importance, float threshold, float rayThreshold) {
+  or_imp ib = {importance, threshold, rayThreshold};
+  for (size_t i = 0; i < n; ++i) {
+    const float* r = rays + i * 8;
+    vec3 o = v3(r[0], r[1], r[2]), d = v3(r[4], r[5], r[6]);
+    float len = r[3];
+    int32_t slot;
+    vec3 b = v3(0, 0, 0);
+    float info[3];
+    int a = trace_one_ib(s, any, 0, o, d, &len, &slot, &b, NULL, NULL, NULL, NULL, &ib, info);
+    int32_t* h = hits + i * 4;
+    if (any) {
+      h[0] = a; h[1] = h[2] = h[3] = 0;
+    } else {
+      h[0] = slot;
+      h[1] = (int32_t)bits_of(len);
+      h[2] = (int32_t)bits_of(slot >= 0 ? b.y : 0.0f);
+      h[3] = (int32_t)bits_of(slot >= 0 ? b.z : 0.0f);
+      if (nodes) {
+        memcpy(&nodes[4 * i], &info[0], 4);
+        nodes[4 * i + 1] = (int32_t)bits_of(info[1]);
+        nodes[4 * i + 2] = (int32_t)bits_of(info[2]);
+        nodes[4 * i + 3] = 0;
+      }
+    }
+  }
 }
 
 int or_trace_one(const or_scene* s, int any, vec3 o, vec3 d, float* rayLength, int32_t* slot, vec3* bary,
