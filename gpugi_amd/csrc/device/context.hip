@@ -25,6 +25,10 @@ struct gpugi_ctx {
   // second stream (default priority) for work that overlaps the main stream's,
   // forked and joined with the two events; created on first use
   hipStream_t side = nullptr;
+  // host-buffer traces: a copy stream and per-slot events of the chunk pipeline
+  hipStream_t copy = nullptr;
+  hipEvent_t evIn[2] = {nullptr, nullptr}, evTraced[2] = {nullptr, nullptr}, evStart = nullptr;
+  std::set<void*> hostAllocs;  // pinned host memory from gpugi_host_alloc
   hipEvent_t evFork = nullptr, evJoin = nullptr, evInfer = nullptr;
   bool timerStarted = false;
   std::string error;
@@ -279,6 +283,11 @@ void gpugi_destroy(gpugi_ctx* ctx) {
   if (ctx->evFork) hipEventDestroy(ctx->evFork);
   if (ctx->evJoin) hipEventDestroy(ctx->evJoin);
   if (ctx->evInfer) hipEventDestroy(ctx->evInfer);
+  if (ctx->copy) hipStreamSynchronize(ctx->copy);
+  for (hipEvent_t e : {ctx->evIn[0], ctx->evIn[1], ctx->evTraced[0], ctx->evTraced[1], ctx->evStart})
+    if (e) hipEventDestroy(e);
+  if (ctx->copy) hipStreamDestroy(ctx->copy);
+  for (void* p : ctx->hostAllocs) hipHostFree(p);
   if (ctx->side) hipStreamDestroy(ctx->side);
   if (ctx->stream) hipStreamDestroy(ctx->stream);
   delete ctx;
@@ -837,6 +846,109 @@ int gpugi_compute_mse(gpugi_ctx* ctx, uint32_t divisor, float* mse) {
   return GPUGI_OK;
 }
 
+// Pinned host-buffer trace in chunks through two device slots: the copy stream
+// brings chunk k + 1 in and takes chunk k - 1 out while the main stream
+// traces chunk k, so the copies overlap the traversal (the caller checked
+// that both host buffers are pinned).
+static int TraceHostChunked(gpugi_ctx* ctx, int query, bool wide, const gpugi_ray* rays, gpugi_hit* hits, size_t n) {
+  constexpr size_t kChunk = (size_t)1 << 22;
+  const size_t slotBytes = kChunk * (sizeof(gpugi_ray) + sizeof(gpugi_hit));
+  if (int rc = EnsureScratch(ctx, 2 * slotBytes)) return rc;
+  if (!ctx->copy) {
+    HIP_TRY(ctx, hipStreamCreateWithFlags(&ctx->copy, hipStreamNonBlocking));
+    for (int s = 0; s < 2; ++s) {
+      HIP_TRY(ctx, hipEventCreateWithFlags(&ctx->evIn[s], hipEventDisableTiming));
+      HIP_TRY(ctx, hipEventCreateWithFlags(&ctx->evTraced[s], hipEventDisableTiming));
+    }
+    HIP_TRY(ctx, hipEventCreateWithFlags(&ctx->evStart, hipEventDisableTiming));
+  }
+  WideScene ws;
+  if (wide)
+    if (int rc = PrepareTraceWide(ctx, ctx->wideStack, &ws)) return rc;
+  // the slots are free once the main stream's earlier work is done
+  HIP_TRY(ctx, hipEventRecord(ctx->evStart, ctx->stream));
+  HIP_TRY(ctx, hipStreamWaitEvent(ctx->copy, ctx->evStart, 0));
+  const size_t numChunks = (n + kChunk - 1) / kChunk;
+  auto slotRays = [&](int s) { return (gpugi_ray*)((char*)ctx->dScratch + s * slotBytes); };
+  auto slotHits = [&](int s) { return (gpugi_hit*)((char*)ctx->dScratch + s * slotBytes + kChunk * sizeof(gpugi_ray)); };
+  auto copyIn = [&](size_t k) -> int {
+    const int s = (int)(k & 1);
+    const size_t first = k * kChunk, m = std::min(kChunk, n - first);
+    HIP_TRY(ctx, hipMemcpyAsync(slotRays(s), rays + first, m * sizeof(gpugi_ray), hipMemcpyHostToDevice, ctx->copy));
+    HIP_TRY(ctx, hipEventRecord(ctx->evIn[s], ctx->copy));
+    return GPUGI_OK;
+  };
+  auto copyOut = [&](size_t k) -> int {
+    const int s = (int)(k & 1);
+    const size_t first = k * kChunk, m = std::min(kChunk, n - first);
+    HIP_TRY(ctx, hipStreamWaitEvent(ctx->copy, ctx->evTraced[s], 0));
+    HIP_TRY(ctx, hipMemcpyAsync(hits + first, slotHits(s), m * sizeof(gpugi_hit), hipMemcpyDeviceToHost, ctx->copy));
+    return GPUGI_OK;
+  };
+  // copy stream: in(0) in(1) out(0) in(2) out(1) in(3) ...; slot k & 1 is
+  // reused by in(k) only after out(k - 2), which waited for trace(k - 2)
+  if (int rc = copyIn(0)) return rc;
+  for (size_t k = 0; k < numChunks; ++k) {
+    if (k + 1 < numChunks)
+      if (int rc = copyIn(k + 1)) return rc;
+    const int s = (int)(k & 1);
+    const size_t m = std::min(kChunk, n - k * kChunk);
+    HIP_TRY(ctx, hipStreamWaitEvent(ctx->stream, ctx->evIn[s], 0));
+    if (wide)
+      HIP_TRY(ctx, LaunchTraceWide(ws, ctx->wideStack, query, slotRays(s), slotHits(s), m, ctx->wideRefill,
+                                   ctx->wideWait, Watertight(ctx), ctx->stream));
+    else
+      HIP_TRY(ctx, LaunchTraceRef(MakeRef(ctx), query, slotRays(s), slotHits(s), m, Watertight(ctx), ctx->stream));
+    HIP_TRY(ctx, hipEventRecord(ctx->evTraced[s], ctx->stream));
+    if (int rc = copyOut(k)) return rc;
+  }
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->copy));
+  uint32_t flag = 0;
+  HIP_TRY(ctx, hipMemcpyAsync(&flag, (uint32_t*)ctx->dCounters + 1, 4, hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  if (flag) return Fail(ctx, GPUGI_ERR_STATE, "gpugi_trace: traversal stack overflow (bug in the stack bound)");
+  return GPUGI_OK;
+}
+
+// Whether [p, p + bytes) is pinned host memory (gpugi_host_alloc or any other
+// page-locked allocation the runtime knows): both ends must be.
+static bool IsPinned(const void* p, size_t bytes) {
+  const char* ends[2] = {static_cast<const char*>(p), static_cast<const char*>(p) + bytes - 1};
+  for (const char* q : ends) {
+    hipPointerAttribute_t a;
+    if (hipPointerGetAttributes(&a, q) != hipSuccess) {
+      (void)hipGetLastError();  // pageable memory: the query fails
+      return false;
+    }
+    if (a.type != hipMemoryTypeHost) return false;
+  }
+  return true;
+}
+
+int gpugi_host_alloc(gpugi_ctx* ctx, size_t bytes, void** host_ptr) {
+  if (!ctx || !host_ptr || bytes == 0) return Fail(ctx, GPUGI_ERR_ARG, "gpugi_host_alloc: invalid argument");
+  if (SetDev(ctx)) return GPUGI_ERR_HIP;
+  void* p = nullptr;
+  hipError_t e = hipHostMalloc(&p, bytes, hipHostMallocDefault);
+  if (e != hipSuccess) return Fail(ctx, GPUGI_ERR_NOMEM, std::string("gpugi_host_alloc: ") + hipGetErrorString(e));
+  ctx->hostAllocs.insert(p);
+  *host_ptr = p;
+  return GPUGI_OK;
+}
+
+int gpugi_host_free(gpugi_ctx* ctx, void* host_ptr) {
+  if (!ctx) return Fail(ctx, GPUGI_ERR_ARG, "gpugi_host_free: null context");
+  auto it = ctx->hostAllocs.find(host_ptr);
+  if (it == ctx->hostAllocs.end())
+    return Fail(ctx, GPUGI_ERR_ARG, "gpugi_host_free: pointer not allocated by gpugi_host_alloc");
+  if (SetDev(ctx)) return GPUGI_ERR_HIP;
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  if (ctx->copy) HIP_TRY(ctx, hipStreamSynchronize(ctx->copy));
+  HIP_TRY(ctx, hipHostFree(host_ptr));
+  ctx->hostAllocs.erase(it);
+  return GPUGI_OK;
+}
+
 int gpugi_trace(gpugi_ctx* ctx, int query, int kernel, const gpugi_ray* rays, gpugi_hit* hits, size_t n,
                 int rays_on_device) {
   if (!ctx || (n > 0 && (!rays || !hits))) return Fail(ctx, GPUGI_ERR_ARG, "gpugi_trace: null argument");
@@ -847,6 +959,11 @@ int gpugi_trace(gpugi_ctx* ctx, int query, int kernel, const gpugi_ray* rays, gp
   if (n == 0) return GPUGI_OK;
   if (SetDev(ctx)) return GPUGI_ERR_HIP;
   const bool wide = kernel == GPUGI_KERNEL_WIDE || (kernel == GPUGI_KERNEL_AUTO && ctx->hasWide);
+  // pinned host buffers: chunked copies that overlap the traversal; pageable
+  // ones go through one staged copy each way, which the runtime does faster
+  if (!rays_on_device && n > ((size_t)1 << 22) && IsPinned(rays, n * sizeof(gpugi_ray)) &&
+      IsPinned(hits, n * sizeof(gpugi_hit)))
+    return TraceHostChunked(ctx, query, wide, rays, hits, n);
   const gpugi_ray* dr = rays;
   gpugi_hit* dh = hits;
   if (!rays_on_device) {

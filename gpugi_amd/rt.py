@@ -121,6 +121,8 @@ _SIGNATURES = {
     "gpugi_hostscene_num_textures": [_P, ctypes.POINTER(ctypes.c_uint32)],
     "gpugi_hostscene_obox_hierarchy": [_P, _P, ctypes.c_size_t],
     "gpugi_set_bvh_type": [_P, ctypes.c_int],
+    "gpugi_host_alloc": [_P, ctypes.c_size_t, ctypes.POINTER(ctypes.c_void_p)],
+    "gpugi_host_free": [_P, _P],
     "gpugi_trace_importance_break": [_P, ctypes.c_int, _P, _P, _P, ctypes.c_size_t, _P, ctypes.c_uint32,
                                      ctypes.c_float, ctypes.c_float],
     "gpugi_hostscene_texture": [_P, ctypes.c_uint32, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
@@ -533,11 +535,25 @@ class Context:
         self._call("gpugi_compute_mse", int(divisor), ctypes.byref(v))
         return v.value
 
-    def trace(self, rays, query=CLOSEST_HIT, kernel=KERNEL_AUTO):
+    def trace(self, rays, query=CLOSEST_HIT, kernel=KERNEL_AUTO, out=None):
+        """Host-buffer trace; out (optional, HIT_DTYPE, e.g. from host_empty) receives the hits."""
         rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
-        hits = np.zeros(len(rays), dtype=HIT_DTYPE)
+        hits = np.zeros(len(rays), dtype=HIT_DTYPE) if out is None else out
+        assert hits.dtype == HIT_DTYPE and len(hits) >= len(rays) and hits.flags.c_contiguous
         self._call("gpugi_trace", int(query), int(kernel), _ptr(rays), _ptr(hits), ctypes.c_size_t(len(rays)), 0)
         return hits
+
+    def host_empty(self, count, dtype):
+        """An uninitialised numpy array of `count` elements in pinned host memory
+        (gpugi_host_alloc); release it with host_free before the context goes."""
+        dtype = np.dtype(dtype)
+        p = ctypes.c_void_p()
+        self._call("gpugi_host_alloc", ctypes.c_size_t(max(1, count * dtype.itemsize)), ctypes.byref(p))
+        buf = (ctypes.c_char * (count * dtype.itemsize)).from_address(p.value)
+        return np.frombuffer(buf, dtype=dtype, count=count)
+
+    def host_free(self, arr):
+        self._call("gpugi_host_free", ctypes.c_void_p(arr.ctypes.data))
 
     def trace_importance_break(self, rays, importance, threshold, ray_threshold, query=CLOSEST_HIT):
         """TraceRay with TRACERAY_IMPORTANCE_BREAK: importance (num_nodes, 2) float32.

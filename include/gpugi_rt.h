@@ -294,10 +294,18 @@ int gpugi_trace_importance_break(gpugi_ctx* ctx, int query, const gpugi_ray* ray
                                  gpugi_node_hit* nodes, size_t n, const float* importance, uint32_t num_nodes,
                                  float threshold, float ray_threshold);
 
+/* Pinned host memory (hipHostMalloc) owned by the context: host ray and hit
+ * buffers in it let gpugi_trace overlap its copies with the traversal.
+ * gpugi_destroy frees what is left. */
+int gpugi_host_alloc(gpugi_ctx* ctx, size_t bytes, void** host_ptr);
+int gpugi_host_free(gpugi_ctx* ctx, void* host_ptr);
+
 /* Ray queries for parity and the traversal microbenchmark (the GLSL
  * TraceRay/TraceRayAnyHit functions, traceray.glsl:18-39).  rays and hits
  * are host pointers when rays_on_device == 0 (the call then blocks until the
- * hits are copied back), device pointers otherwise (asynchronous). */
+ * hits are copied back; above 2^22 rays in pinned memory they move in chunks
+ * on a second stream while earlier chunks are traced), device pointers
+ * otherwise (asynchronous). */
 int gpugi_trace(gpugi_ctx* ctx, int query, int kernel, const gpugi_ray* rays, gpugi_hit* hits,
                 size_t n, int rays_on_device);
 /* Configuration of the wide traversal: triangles per merged leaf (1..16; the
