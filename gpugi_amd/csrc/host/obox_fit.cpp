@@ -8,10 +8,12 @@
 // vertices exactly under the float quaternion the walk rotates with, then
 // widened by a small margin for the walk's float arithmetic.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <string>
+#include <new>
 #include <thread>
 #include <vector>
 
@@ -161,71 +163,76 @@ std::vector<float> FitOBoxHierarchy(const float* hier8, uint32_t numNodes, const
   // nodes are independent: interleaved over up to 16 threads
   const unsigned hw = std::thread::hardware_concurrency();
   const unsigned numThreads = numNodes < 4096 ? 1u : std::max(1u, std::min(16u, hw));
+  std::atomic<bool> failed{false};
   auto work = [&](unsigned tIdx) {
-    std::vector<double> pts;
-    std::vector<uint32_t> stack;
-    for (uint32_t n = tIdx; n < numNodes; n += numThreads) {
-      // the subtree's valid triangle vertices: a node's children are its
-      // firstChild and that child's escape (the left child escapes to its sibling)
-      pts.clear();
-      stack.assign(1, n);
-      while (!stack.empty()) {
-        const uint32_t m = stack.back();
-        stack.pop_back();
-        const uint32_t fc = child(m);
-        if (fc & 0x80000000u) {
-          const uint32_t leaf = fc & 0x7fffffffu;
-          if (leaf >= numLeaves) continue;
-          for (int s = 0; s < 2; ++s) {
-            const int32_t* t = slots + ((size_t)leaf * 2 + s) * 4;
-            if (t[0] == t[1]) break;
-            for (int k = 0; k < 3; ++k)
-              for (int a = 0; a < 3; ++a) pts.push_back(positions[(size_t)t[k] * 3 + a]);
+    try {
+      std::vector<double> pts;
+      std::vector<uint32_t> stack;
+      for (uint32_t n = tIdx; n < numNodes; n += numThreads) {
+        // the subtree's valid triangle vertices: a node's children are its
+        // firstChild and that child's escape (the left child escapes to its sibling)
+        pts.clear();
+        stack.assign(1, n);
+        while (!stack.empty()) {
+          const uint32_t m = stack.back();
+          stack.pop_back();
+          const uint32_t fc = child(m);
+          if (fc & 0x80000000u) {
+            const uint32_t leaf = fc & 0x7fffffffu;
+            if (leaf >= numLeaves) continue;
+            for (int s = 0; s < 2; ++s) {
+              const int32_t* t = slots + ((size_t)leaf * 2 + s) * 4;
+              if (t[0] == t[1]) break;
+              for (int k = 0; k < 3; ++k)
+                for (int a = 0; a < 3; ++a) pts.push_back(positions[(size_t)t[k] * 3 + a]);
+            }
+          } else if (fc < numNodes) {
+            stack.push_back(fc);
+            const uint32_t second = escape(fc);
+            if (second != 0 && second < numNodes && second != escape(m)) stack.push_back(second);
           }
-        } else if (fc < numNodes) {
-          stack.push_back(fc);
-          const uint32_t second = escape(fc);
-          if (second != 0 && second < numNodes && second != escape(m)) stack.push_back(second);
         }
+        float* o = out.data() + (size_t)n * 12;
+        std::memcpy(o + 3, hier8 + (size_t)n * 8 + 3, 4);  // firstChild
+        std::memcpy(o + 7, hier8 + (size_t)n * 8 + 7, 4);  // escape
+        if (pts.empty()) {  // an empty subtree: a point box far from nothing (never entered: the walk
+          o[11] = 1.0f;     // still tests it, and a zero-size box at the origin holds no triangle)
+          continue;
+        }
+        // candidate orientations: world axes and principal axes
+        Box best = BoxFor(Q{0, 0, 0, 1}, pts);
+        const size_t np = pts.size() / 3;
+        if (np >= 4) {
+          double mean[3] = {0, 0, 0};
+          for (size_t i = 0; i < pts.size(); i += 3)
+            for (int k = 0; k < 3; ++k) mean[k] += pts[i + k];
+          for (int k = 0; k < 3; ++k) mean[k] /= (double)np;
+          double cov[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+          for (size_t i = 0; i < pts.size(); i += 3)
+            for (int a = 0; a < 3; ++a)
+              for (int b2 = 0; b2 < 3; ++b2) cov[a][b2] += (pts[i + a] - mean[a]) * (pts[i + b2] - mean[b2]);
+          double ev[3][3];
+          Eigenvectors(cov, ev);
+          // a right-handed frame: rows are the local axes (R maps world to local)
+          const double cr[3] = {ev[0][1] * ev[1][2] - ev[0][2] * ev[1][1], ev[0][2] * ev[1][0] - ev[0][0] * ev[1][2],
+                                ev[0][0] * ev[1][1] - ev[0][1] * ev[1][0]};
+          if (cr[0] * ev[2][0] + cr[1] * ev[2][1] + cr[2] * ev[2][2] < 0.0)
+            for (int k = 0; k < 3; ++k) ev[2][k] = -ev[2][k];
+          const Box pca = BoxFor(FromMatrix(ev), pts);
+          if (pca.area < best.area) best = pca;
+        }
+        for (int k = 0; k < 3; ++k) o[k] = best.center[k];
+        for (int k = 0; k < 3; ++k) {
+          // widened by the margin and rounded up to float
+          const double h = best.half[k] * (1.0 + 1e-6) + margin;
+          float hf = (float)h;
+          if ((double)hf < h) hf = std::nextafter(hf, INFINITY);
+          o[4 + k] = hf;
+        }
+        for (int k = 0; k < 4; ++k) o[8 + k] = best.q[k];
       }
-      float* o = out.data() + (size_t)n * 12;
-      std::memcpy(o + 3, hier8 + (size_t)n * 8 + 3, 4);  // firstChild
-      std::memcpy(o + 7, hier8 + (size_t)n * 8 + 7, 4);  // escape
-      if (pts.empty()) {  // an empty subtree: a point box far from nothing (never entered: the walk
-        o[11] = 1.0f;     // still tests it, and a zero-size box at the origin holds no triangle)
-        continue;
-      }
-      // candidate orientations: world axes and principal axes
-      Box best = BoxFor(Q{0, 0, 0, 1}, pts);
-      const size_t np = pts.size() / 3;
-      if (np >= 4) {
-        double mean[3] = {0, 0, 0};
-        for (size_t i = 0; i < pts.size(); i += 3)
-          for (int k = 0; k < 3; ++k) mean[k] += pts[i + k];
-        for (int k = 0; k < 3; ++k) mean[k] /= (double)np;
-        double cov[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-        for (size_t i = 0; i < pts.size(); i += 3)
-          for (int a = 0; a < 3; ++a)
-            for (int b2 = 0; b2 < 3; ++b2) cov[a][b2] += (pts[i + a] - mean[a]) * (pts[i + b2] - mean[b2]);
-        double ev[3][3];
-        Eigenvectors(cov, ev);
-        // a right-handed frame: rows are the local axes (R maps world to local)
-        const double cr[3] = {ev[0][1] * ev[1][2] - ev[0][2] * ev[1][1], ev[0][2] * ev[1][0] - ev[0][0] * ev[1][2],
-                              ev[0][0] * ev[1][1] - ev[0][1] * ev[1][0]};
-        if (cr[0] * ev[2][0] + cr[1] * ev[2][1] + cr[2] * ev[2][2] < 0.0)
-          for (int k = 0; k < 3; ++k) ev[2][k] = -ev[2][k];
-        const Box pca = BoxFor(FromMatrix(ev), pts);
-        if (pca.area < best.area) best = pca;
-      }
-      for (int k = 0; k < 3; ++k) o[k] = best.center[k];
-      for (int k = 0; k < 3; ++k) {
-        // widened by the margin and rounded up to float
-        const double h = best.half[k] * (1.0 + 1e-6) + margin;
-        float hf = (float)h;
-        if ((double)hf < h) hf = std::nextafter(hf, INFINITY);
-        o[4 + k] = hf;
-      }
-      for (int k = 0; k < 4; ++k) o[8 + k] = best.q[k];
+    } catch (const std::bad_alloc&) {  // reported on the calling thread
+      failed = true;
     }
   };
   if (numThreads == 1) {
@@ -235,6 +242,7 @@ std::vector<float> FitOBoxHierarchy(const float* hier8, uint32_t numNodes, const
     for (unsigned t = 0; t < numThreads; ++t) threads.emplace_back(work, t);
     for (auto& t : threads) t.join();
   }
+  if (failed) throw std::bad_alloc();
   return out;
 }
 
