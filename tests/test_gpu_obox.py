@@ -86,3 +86,32 @@ def test_aabox_returns_with_the_next_load(rt, oracle, octx):
     octx.load_scene(hs)
     aa = hits_int(octx.trace(r, rt.CLOSEST_HIT, rt.KERNEL_REFERENCE))
     assert np.array_equal(aa, oracle.trace(sc, rays))
+
+
+def test_bvh_type_script_command(rt, oracle, tmp_path):
+    """gpugi_render's bvhType = 1 before sceneFilename renders with the oriented
+    boxes: the PT image equals the oracle's over the same records."""
+    import os
+    import subprocess
+    app = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "gpugi_amd", "bin", "gpugi_render")
+    out = str(tmp_path / "obox.pfm")
+    w = 32
+    lines = ["bvhType = 1", "sceneFilename = cornell", "cameraPos = 0, 1, 3.7", "cameraLookAt = 0, 1, 0",
+             "cameraFOV = 40", "resolution = %d, %d" % (w, w), "pathLength = 5", "traversal = 1", "renderer = 0",
+             "waitIterations = 1", "namedscreenshot = " + out, "exit"]
+    args = [app]
+    for l in lines:
+        args += ["-e", l]
+    print("gpugi_render bvhType 1", flush=True)
+    r = subprocess.run(args, capture_output=True, text=True, timeout=100)
+    assert r.returncode == 0, r.stdout + r.stderr
+    with open(out, "rb") as f:
+        f.readline(), f.readline(), f.readline()
+        img = np.frombuffer(f.read(), dtype="<f4").reshape(w, w, 3)
+    hs = rt.HostScene.cornell()
+    sco = oracle.Scene(dict(hs.arrays(), obox=hs.obox_hierarchy()))
+    ls = rt.LightSampler(hs, 123)
+    # RendererSystem: the Pathtracer's constructor draws light-sample batch 1 with FrameSeed WangHash(0)
+    acc = oracle.pathtrace(sco, ls.generate(128), workloads.cornell_camera(w), w, w, rt.wanghash(0), 5)
+    assert img.mean() > 0.05
+    assert np.array_equal(img, np.ascontiguousarray(acc[..., :3]))
