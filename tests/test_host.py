@@ -265,3 +265,26 @@ def test_errors_without_context(rt):
     assert lib.gpugi_hostscene_cornell(None) == -1
     assert lib.gpugi_camera_compute(None, None, None, 40.0, 1, 1, None) == -1
     assert b"invalid" in lib.gpugi_last_error(None)
+
+
+def test_next_row_entry_points_reject_null_arguments(rt):
+    """The texture, environment, MSE, bvh-type, importance-break and pinned-memory
+    entry points fail with GPUGI_ERR_ARG (-1) on a null context, before any device call."""
+    lib = rt.lib()
+    null = None
+    assert lib.gpugi_add_texture(null, 1, 1, 0, null, null) == -1
+    assert lib.gpugi_set_material_texture(null, 0, 0, 0) == -1
+    assert lib.gpugi_set_environment_map(null, 0, null) == -1
+    assert lib.gpugi_set_reference_image(null, null, 1, 1) == -1
+    assert lib.gpugi_compute_mse(null, 1, null) == -1
+    assert lib.gpugi_set_bvh_type(null, 1) == -1
+    assert lib.gpugi_trace_importance_break(null, 0, null, null, null, 0, null, 0, 0.0, 0.0) == -1
+    assert lib.gpugi_host_alloc(null, 16, null) == -1
+    assert lib.gpugi_host_free(null, null) == -1
+    assert lib.gpugi_hostscene_obox_hierarchy(null, null, 0) == -1
+    assert lib.gpugi_hostscene_num_textures(null, null) == -1
+    hs = rt.HostScene.cornell()
+    assert hs.textures() == [] and (hs.material_textures() == -1).all()
+    ob = hs.obox_hierarchy()
+    short = np.zeros(ob.size - 1, np.float32)
+    assert lib.gpugi_hostscene_obox_hierarchy(hs._h, short.ctypes.data, short.size) == -1
