@@ -14,6 +14,7 @@
 #include "launch.h"
 #include "mlp.h"
 #include "scan.h"
+#include "wavefront_pt.h"
 
 using namespace gpugi;
 
@@ -71,6 +72,10 @@ struct gpugi_ctx {
   int bvhType = GPUGI_BVH_AABOX;       // applies from the next scene load
   bool sceneObox = false;              // dNodes holds TreeNode<OBox> (3 texels per node)
   int integratorKernel = GPUGI_KERNEL_AUTO;
+  int ptSchedule = GPUGI_PT_MEGAKERNEL;
+  void* dWave = nullptr;         // wavefront path state and ray queues
+  size_t waveCap = 0;
+  uint32_t* hWaveCount = nullptr;  // pinned: the next queue's length
   int bptCapacity = -1;
   int bptAverageLength = 0;
   IntegratorState integ;
@@ -267,7 +272,7 @@ void gpugi_destroy(gpugi_ctx* ctx) {
   if (ctx->stream) hipStreamSynchronize(ctx->stream);
   for (void** p : {&ctx->dNodes, &ctx->dSlots, &ctx->dPos, &ctx->dVinfo, &ctx->dMats, &ctx->dWideNodes,
                    &ctx->dWideTris, &ctx->dSpill, &ctx->dCounters, &ctx->dWideQueue, &ctx->dAccum, &ctx->dLights,
-                   &ctx->dScratch, &ctx->dRefImage, &ctx->dMseTemp, &ctx->dEnv})
+                   &ctx->dScratch, &ctx->dRefImage, &ctx->dMseTemp, &ctx->dEnv, &ctx->dWave})
     FreeDev(*p);
   FreeIntegratorState(&ctx->integ);
   for (void* p : ctx->texBufs) hipFree(p);
@@ -288,6 +293,7 @@ void gpugi_destroy(gpugi_ctx* ctx) {
     if (e) hipEventDestroy(e);
   if (ctx->copy) hipStreamDestroy(ctx->copy);
   for (void* p : ctx->hostAllocs) hipHostFree(p);
+  if (ctx->hWaveCount) hipHostFree(ctx->hWaveCount);
   if (ctx->side) hipStreamDestroy(ctx->side);
   if (ctx->stream) hipStreamDestroy(ctx->stream);
   delete ctx;
@@ -457,6 +463,13 @@ int gpugi_set_integrator_kernel(gpugi_ctx* ctx, int kernel) {
   return GPUGI_OK;
 }
 
+int gpugi_set_pt_schedule(gpugi_ctx* ctx, int schedule) {
+  if (!ctx || (schedule != GPUGI_PT_MEGAKERNEL && schedule != GPUGI_PT_WAVEFRONT))
+    return Fail(ctx, GPUGI_ERR_ARG, "gpugi_set_pt_schedule: bad schedule");
+  ctx->ptSchedule = schedule;
+  return GPUGI_OK;
+}
+
 namespace {
 
 size_t Align256(size_t b) { return (b + 255) & ~(size_t)255; }
@@ -601,6 +614,29 @@ int DrawCached(gpugi_ctx* ctx, uint32_t frameSeed) {
   return GPUGI_OK;
 }
 
+// GPUGI_PT as per-bounce passes over device ray queues (wavefront_pt.hip).
+int DrawWavefront(gpugi_ctx* ctx, const DrawParams& p) {
+  const size_t pixels = (size_t)ctx->width * ctx->height;
+  const size_t bytes = WavefrontBytes(pixels);
+  if (ctx->waveCap < bytes) {
+    FreeDev(ctx->dWave);
+    ctx->waveCap = 0;
+    HIP_TRY(ctx, hipMalloc(&ctx->dWave, bytes));
+    ctx->waveCap = bytes;
+  }
+  if (!ctx->hWaveCount) HIP_TRY(ctx, hipHostMalloc((void**)&ctx->hWaveCount, 4, hipHostMallocDefault));
+  WavefrontTrace tr;
+  tr.wide = p.useWide;
+  if (tr.wide)
+    if (int rc = PrepareTraceWide(ctx, ctx->wideStack, &tr.ws)) return rc;
+  tr.stack = ctx->wideStack;
+  tr.refill = ctx->wideRefill;
+  tr.wait = ctx->wideWait;
+  tr.hostCount = ctx->hWaveCount;
+  HIP_TRY(ctx, DrawPTWavefront(p, tr, CarveWavefront(ctx->dWave, pixels), ctx->stream));
+  return GPUGI_OK;
+}
+
 }  // namespace
 
 int gpugi_draw(gpugi_ctx* ctx, int integrator, uint32_t frame_seed) {
@@ -616,6 +652,11 @@ int gpugi_draw(gpugi_ctx* ctx, int integrator, uint32_t frame_seed) {
     return GPUGI_OK;
   }
   DrawParams p = MakeDrawParams(ctx, frame_seed);
+  if (integrator == GPUGI_PT && ctx->ptSchedule == GPUGI_PT_WAVEFRONT) {
+    if (int rc = DrawWavefront(ctx, p)) return rc;
+    ++ctx->iterations;
+    return GPUGI_OK;
+  }
   std::string err;
   hipError_t e = DrawIntegrator(&ctx->integ, integrator, p, ctx->stream, &err);
   if (e != hipSuccess) return Fail(ctx, GPUGI_ERR_HIP, "gpugi_draw: " + err + hipGetErrorString(e));

@@ -1,0 +1,42 @@
+// Wavefront schedule of the path tracer (device buffers and the per-bounce driver).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "integrators.h"
+#include "wide_bvh.h"
+
+namespace gpugi {
+
+// Per-pixel path state and per-queue-slot buffers, all in one allocation.
+struct WavefrontBuffers {
+  uint32_t* counters;  // [0], [1]: lengths of the two path queues
+  float4* color;       // per pixel
+  float4* throughput;  // per pixel
+  uint32_t* seed;      // per pixel
+  uint32_t* flags;     // per pixel: bounce << 1 | didNEE
+  uint32_t* queue[2];  // pixel indices of the live paths
+  float4* rays[2];     // their rays (gpugi_ray layout), in queue order
+  int4* hits;          // closest-hit records of the current queue
+  float4* shadowRays;  // NEE shadow rays of the current queue
+  int4* occluded;      // their any-hit records
+  float4* neeValue;    // NEE value, w: 1 shadow-tested term, 0.5 zero term, 0 none
+  float4* neeWeight;   // throughput and light-sample weight of the term
+};
+
+// How the passes trace: the wide kernel (with its launch parameters) or the
+// reference walk; hostCount: pinned or pageable 4 bytes for the queue length.
+struct WavefrontTrace {
+  bool wide = false;
+  WideScene ws;
+  int stack = 10, refill = 8, wait = 16;
+  uint32_t* hostCount = nullptr;
+};
+
+size_t WavefrontBytes(size_t numPixels);
+WavefrontBuffers CarveWavefront(void* base, size_t numPixels);
+// One PT draw (the reference triangle test only).  Synchronizes the stream once
+// per bounce to read the next queue's length.
+hipError_t DrawPTWavefront(const DrawParams& p, const WavefrontTrace& tr, const WavefrontBuffers& b,
+                           hipStream_t stream);
+
+}  // namespace gpugi

@@ -1,0 +1,213 @@
+// Wavefront path tracer: the paths of pathtracer.comp:16-110 advanced one
+// bounce at a time over the whole image, with the rays of every bounce traced
+// by the trace kernels (closest hit, then the NEE shadow rays as any hit) on
+// compact device ray queues.  Each path consumes its random numbers and adds
+// its terms in the order of the one-kernel path tracer (emission, then NEE,
+// then the next vertex), so both schedules give the same image bit for bit.
+//
+// Per bounce, for the queue of live paths (pixel indices):
+//   trace closest hit of rays[i]                 -> hits[i]
+//   k_wf_shade:  emission, NEE record i (value, throughput * weight, shadow
+//                ray, or a zero-length ray), BSDF sample, roulette; a
+//                path that goes on is appended to the next queue with its ray
+//   trace any hit of the shadow rays             -> occluded[i]
+//   k_wf_resolve: color += (visible ? value : 0) * throughput * weight
+#include "integrator_common.h"
+#include "launch.h"
+#include "wavefront_pt.h"
+
+namespace gpugi {
+
+namespace {
+
+constexpr int kWfBlock = 256;
+
+__global__ __launch_bounds__(kWfBlock) void k_wf_raygen(DrawParams p, WavefrontBuffers b, uint32_t tilesW,
+                                                        uint32_t numPaths) {
+  const uint32_t i = blockIdx.x * kWfBlock + threadIdx.x;
+  if (i >= numPaths) return;
+  // the reference dispatches W/8 x H/8 tiles: only those pixels are traced
+  const int gx = (int)(i % (tilesW * kTile)), gy = (int)(i / (tilesW * kTile));
+  const uint32_t pix = (uint32_t)gy * (uint32_t)p.width + (uint32_t)gx;
+  PtPath s;
+  pt_begin(p, s, gx, gy, p.frameSeed);
+  b.color[pix] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  b.throughput[pix] = make_float4(s.throughput.x, s.throughput.y, s.throughput.z, 0.0f);
+  b.seed[pix] = s.seed;
+  b.flags[pix] = 0u;  // bounce 0, no NEE before
+  b.queue[0][i] = pix;
+  b.rays[0][2 * (size_t)i] = make_float4(s.o.x, s.o.y, s.o.z, gg::kRayMax);
+  b.rays[0][2 * (size_t)i + 1] = make_float4(s.d.x, s.d.y, s.d.z, 0.0f);
+}
+
+// One bounce of pt_bounce for queue entry i (cur: the queue being shaded).
+__global__ __launch_bounds__(kWfBlock) void k_wf_shade(DrawParams p, WavefrontBuffers b, int cur, uint32_t count) {
+  const uint32_t i = blockIdx.x * kWfBlock + threadIdx.x;
+  if (i >= count) return;
+  const uint32_t pix = b.queue[cur][i];
+  const float4 r0 = b.rays[cur][2 * (size_t)i], r1 = b.rays[cur][2 * (size_t)i + 1];
+  gg::V3 o = gg::v3(r0.x, r0.y, r0.z), d = gg::v3(r1.x, r1.y, r1.z);
+  const int4 rec = b.hits[i];
+  const ClosestHit h = closest_from_record(p, rec.x, __int_as_float(rec.y), __int_as_float(rec.z),
+                                           __int_as_float(rec.w));
+  const float4 c4 = b.color[pix], t4 = b.throughput[pix];
+  gg::V3 color = gg::v3(c4.x, c4.y, c4.z), throughput = gg::v3(t4.x, t4.y, t4.z);
+  uint32_t seed = b.seed[pix];
+  const uint32_t flags = b.flags[pix];
+  const int bounce = (int)(flags >> 1);
+  bool didNEE = (flags & 1u) != 0;
+  // tmax 0 where there is nothing to connect (its result is not read)
+  b.shadowRays[2 * (size_t)i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  b.shadowRays[2 * (size_t)i + 1] = make_float4(0.0f, 0.0f, 1.0f, 0.0f);
+  if (!h.hit) {
+    // environment on a miss (pathtracer.comp:56-58); the path ends, no NEE term
+    color = color + throughput * (p.env ? env_radiance(p.env, p.envSize, d) : gg::v3s(0.0f));
+    b.color[pix] = make_float4(color.x, color.y, color.z, 0.0f);
+    b.neeValue[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    return;
+  }
+  const float lightSampleWeight = (float)p.numLights / 1.0f;
+  o = o + h.t * d;
+  const gg::V3 n = shading_normal(p.ref.vinfo, h.tri, h.bary);
+  const MaterialData m = load_material_hit(p.materials, p.tex, p.ref.vinfo, h.tri, h.bary);
+  if (!didNEE) color = color + throughput * m.emissivity;
+  didNEE = m.refl[3] < 500.0f;
+  const int sampleIndex = (int)(gg::xorshift32(seed) % p.numLights);
+  const NeeSample ns = nee_sample(p, o, n, sampleIndex, d, m, didNEE);
+  if (ns.test) {
+    b.shadowRays[2 * (size_t)i] = make_float4(ns.o.x, ns.o.y, ns.o.z, ns.len);
+    b.shadowRays[2 * (size_t)i + 1] = make_float4(ns.dir.x, ns.dir.y, ns.dir.z, 0.0f);
+  }
+  // w = 1: a term is added; value.xyz counts only when the shadow ray is not blocked
+  b.neeValue[i] = make_float4(ns.value.x, ns.value.y, ns.value.z, ns.test ? 1.0f : 0.5f);
+  b.neeWeight[i] = make_float4(throughput.x, throughput.y, throughput.z, lightSampleWeight);
+  b.color[pix] = make_float4(color.x, color.y, color.z, 0.0f);
+  gg::V3 tp = gg::v3s(1.0f);
+  float pdf;
+  bool diffuse;
+  d = sample_bsdf(d, m, seed, n, tp, false, pdf, diffuse);
+  o = o + d * gg::kRayHitEpsilon;
+  throughput = throughput * tp;
+  bool alive = true;
+  if (bounce >= p.rrStartDepth) {  // Russian roulette (pathtracer.comp:99-104) from rrStartDepth on
+    const float q = gg::saturate(gg::luminance(tp));
+    if (gg::random01(seed) >= q) {
+      alive = false;
+    } else {
+      throughput = throughput / q;
+    }
+  }
+  b.seed[pix] = seed;
+  b.throughput[pix] = make_float4(throughput.x, throughput.y, throughput.z, 0.0f);
+  b.flags[pix] = ((uint32_t)(bounce + 1) << 1) | (didNEE ? 1u : 0u);
+  if (alive && bounce + 1 < p.maxPathLength) {
+    const uint32_t k = atomicAdd(&b.counters[1 - cur], 1u);
+    b.queue[1 - cur][k] = pix;
+    b.rays[1 - cur][2 * (size_t)k] = make_float4(o.x, o.y, o.z, gg::kRayMax);
+    b.rays[1 - cur][2 * (size_t)k + 1] = make_float4(d.x, d.y, d.z, 0.0f);
+  }
+}
+
+// color += radiance * throughput * weight, radiance = 0 when the shadow ray was
+// blocked or there was nothing to connect to (as estimate_direct_light returns).
+__global__ __launch_bounds__(kWfBlock) void k_wf_resolve(WavefrontBuffers b, int cur, uint32_t count) {
+  const uint32_t i = blockIdx.x * kWfBlock + threadIdx.x;
+  if (i >= count) return;
+  const float4 v = b.neeValue[i];
+  if (v.w == 0.0f) return;  // a miss: no NEE term
+  const bool visible = v.w == 1.0f && b.occluded[i].x == 0;
+  const gg::V3 radiance = visible ? gg::v3(v.x, v.y, v.z) : gg::v3s(0.0f);
+  const float4 w = b.neeWeight[i];
+  const uint32_t pix = b.queue[cur][i];
+  const float4 c4 = b.color[pix];
+  const gg::V3 color = gg::v3(c4.x, c4.y, c4.z) + radiance * gg::v3(w.x, w.y, w.z) * w.w;
+  b.color[pix] = make_float4(color.x, color.y, color.z, 0.0f);
+}
+
+__global__ __launch_bounds__(kWfBlock) void k_wf_finish(DrawParams p, WavefrontBuffers b, uint32_t tilesW,
+                                                        uint32_t numPaths) {
+  const uint32_t i = blockIdx.x * kWfBlock + threadIdx.x;
+  if (i >= numPaths) return;
+  const int gx = (int)(i % (tilesW * kTile)), gy = (int)(i / (tilesW * kTile));
+  const uint32_t pix = (uint32_t)gy * (uint32_t)p.width + (uint32_t)gx;
+  const float4 c = b.color[pix];
+  float4* px = &p.accum[pix];
+  const float4 a = *px;
+  *px = make_float4(c.x + a.x, c.y + a.y, c.z + a.z, 0.0f + a.w);  // vec4(color, 0) + imageLoad
+}
+
+__global__ void k_wf_zero(uint32_t* c) { c[threadIdx.x] = 0u; }
+
+dim3 Grid(uint32_t n) { return dim3((n + kWfBlock - 1) / kWfBlock); }
+
+}  // namespace
+
+size_t WavefrontBytes(size_t numPixels) {
+  // color, throughput (16 B), seed, flags (4 B) per pixel; per queue slot: two
+  // queues and two ray arrays, hits, shadow rays, occlusion, NEE value and weight
+  return numPixels * (16 + 16 + 4 + 4) + numPixels * (4 * 2 + 32 * 2 + 16 + 32 + 16 + 16 + 16) +
+         16 * 256;  // the counters and each buffer's rounding to 256 bytes
+}
+
+WavefrontBuffers CarveWavefront(void* base, size_t numPixels) {
+  char* q = static_cast<char*>(base);
+  auto take = [&](size_t bytes) {
+    char* r = q;
+    q += (bytes + 255) & ~(size_t)255;
+    return r;
+  };
+  WavefrontBuffers b;
+  b.counters = (uint32_t*)take(256);
+  b.color = (float4*)take(numPixels * 16);
+  b.throughput = (float4*)take(numPixels * 16);
+  b.seed = (uint32_t*)take(numPixels * 4);
+  b.flags = (uint32_t*)take(numPixels * 4);
+  for (int k = 0; k < 2; ++k) {
+    b.queue[k] = (uint32_t*)take(numPixels * 4);
+    b.rays[k] = (float4*)take(numPixels * 32);
+  }
+  b.hits = (int4*)take(numPixels * 16);
+  b.shadowRays = (float4*)take(numPixels * 32);
+  b.occluded = (int4*)take(numPixels * 16);
+  b.neeValue = (float4*)take(numPixels * 16);
+  b.neeWeight = (float4*)take(numPixels * 16);
+  return b;
+}
+
+hipError_t DrawPTWavefront(const DrawParams& p, const WavefrontTrace& tr, const WavefrontBuffers& b,
+                           hipStream_t stream) {
+  const uint32_t tilesW = (uint32_t)(p.width / kTile), tilesH = (uint32_t)(p.height / kTile);
+  const uint32_t numPaths = tilesW * tilesH * kTileThreads;
+  if (numPaths == 0 || p.maxPathLength <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_wf_zero, dim3(1), dim3(2), 0, stream, b.counters);
+  hipLaunchKernelGGL(k_wf_raygen, Grid(numPaths), dim3(kWfBlock), 0, stream, p, b, tilesW, numPaths);
+  uint32_t count = numPaths;
+  int cur = 0;
+  for (int bounce = 0; bounce < p.maxPathLength && count > 0; ++bounce) {
+    const gpugi_ray* rays = (const gpugi_ray*)b.rays[cur];
+    hipError_t e = tr.wide ? LaunchTraceWide(tr.ws, tr.stack, GPUGI_CLOSEST_HIT, rays, (gpugi_hit*)b.hits, count,
+                                             tr.refill, tr.wait, p.watertight, stream)
+                           : LaunchTraceRef(p.ref, GPUGI_CLOSEST_HIT, rays, (gpugi_hit*)b.hits, count, p.watertight,
+                                            stream);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_wf_zero, dim3(1), dim3(1), 0, stream, b.counters + (1 - cur));
+    hipLaunchKernelGGL(k_wf_shade, Grid(count), dim3(kWfBlock), 0, stream, p, b, cur, count);
+    const gpugi_ray* shadow = (const gpugi_ray*)b.shadowRays;
+    e = tr.wide ? LaunchTraceWide(tr.ws, tr.stack, GPUGI_ANY_HIT, shadow, (gpugi_hit*)b.occluded, count, tr.refill,
+                                  tr.wait, p.watertight, stream)
+                : LaunchTraceRef(p.ref, GPUGI_ANY_HIT, shadow, (gpugi_hit*)b.occluded, count, p.watertight, stream);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_wf_resolve, Grid(count), dim3(kWfBlock), 0, stream, b, cur, count);
+    // the next queue's length sizes the next launches
+    e = hipMemcpyAsync(tr.hostCount, b.counters + (1 - cur), 4, hipMemcpyDeviceToHost, stream);
+    if (e != hipSuccess) return e;
+    e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) return e;
+    count = *tr.hostCount;
+    cur = 1 - cur;
+  }
+  hipLaunchKernelGGL(k_wf_finish, Grid(numPaths), dim3(kWfBlock), 0, stream, p, b, tilesW, numPaths);
+  return hipGetLastError();
+}
+
+}  // namespace gpugi

@@ -15,6 +15,7 @@ LIB_PATH = os.path.join(_HERE, "lib", "libgpugi_rt.so")
 GPUGI_OK = 0
 CLOSEST_HIT, ANY_HIT = 0, 1
 KERNEL_AUTO, KERNEL_REFERENCE, KERNEL_WIDE = 0, 1, 2
+PT_MEGAKERNEL, PT_WAVEFRONT = 0, 1
 PT, LT, BPT, PT_CACHE = 0, 1, 2, 3
 TRIANGLE_REFERENCE, TRIANGLE_WATERTIGHT = 0, 1
 CACHE_SPREAD, CACHE_PRIMARY_INDIRECT = 0, 1
@@ -78,6 +79,7 @@ _SIGNATURES = {
     "gpugi_set_light_samples": [_P, _P, ctypes.c_uint32],
     "gpugi_set_max_path_length": [_P, ctypes.c_int],
     "gpugi_set_integrator_kernel": [_P, ctypes.c_int],
+    "gpugi_set_pt_schedule": [_P, ctypes.c_int],
     "gpugi_set_rr_start_depth": [_P, ctypes.c_int],
     "gpugi_set_triangle_test": [_P, ctypes.c_int],
     "gpugi_draw": [_P, ctypes.c_int, ctypes.c_uint32],
@@ -458,6 +460,10 @@ class Context:
 
     def set_integrator_kernel(self, k):
         self._call("gpugi_set_integrator_kernel", int(k))
+
+    def set_pt_schedule(self, schedule):
+        """PT_MEGAKERNEL (default) or PT_WAVEFRONT: how draw(PT) schedules its paths."""
+        self._call("gpugi_set_pt_schedule", int(schedule))
 
     def set_rr_start_depth(self, depth):
         self._call("gpugi_set_rr_start_depth", int(depth))

@@ -52,6 +52,9 @@ enum gpugi_kernel {
   GPUGI_KERNEL_WIDE = 2       /* ordered walk on the re-packed quantized 4-wide BVH */
 };
 
+/* Path schedules of gpugi_set_pt_schedule. */
+enum gpugi_pt_schedule { GPUGI_PT_MEGAKERNEL = 0, GPUGI_PT_WAVEFRONT = 1 };
+
 /* Bounding volumes of the reference-layout hierarchy (the bvhType parameter,
  * application.cpp:135: Types3D BOX or OBOX). */
 enum gpugi_bvh_type {
@@ -194,6 +197,15 @@ int gpugi_set_rr_start_depth(gpugi_ctx* ctx, int depth);
  * walk) or GPUGI_KERNEL_WIDE / GPUGI_KERNEL_AUTO (4-wide quantized BVH).
  * Replaces the AABOX_BVH define of the shader compile (pathtracer.cpp:55-64). */
 int gpugi_set_integrator_kernel(gpugi_ctx* ctx, int kernel);
+/* How gpugi_draw(GPUGI_PT) schedules the paths: GPUGI_PT_MEGAKERNEL (default;
+ * one lane carries a path through all its bounces, as the reference's
+ * pathtracer.comp dispatch does) or GPUGI_PT_WAVEFRONT (one pass per bounce
+ * over the live paths: closest-hit and shadow rays traced by the gpugi_trace
+ * kernels on compact device queues; the host waits once per bounce for the
+ * queue length).  Both give the same image; the wavefront schedule with the
+ * reference walk and GPUGI_TRIANGLE_WATERTIGHT rebuilds b_x as 1 - (b_y + b_z)
+ * from the hit record.  No counterpart in the reference. */
+int gpugi_set_pt_schedule(gpugi_ctx* ctx, int schedule);
 /* One iteration of an integrator that adds into the accumulation buffer
  * (Renderer::Draw, renderer.hpp:42; FrameSeed UBO, globaluniforms.glsl:17-21). */
 int gpugi_draw(gpugi_ctx* ctx, int integrator, uint32_t frame_seed);

@@ -1,0 +1,130 @@
+"""Wavefront schedule of the path tracer (gpugi_set_pt_schedule(GPUGI_PT_WAVEFRONT)):
+per-bounce passes over device ray queues, the rays traced by the gpugi_trace
+kernels.  With the reference walk the image equals the oracle's
+pathtracer.comp bit for bit (Cornell box, a truncated 8x8 dispatch grid, the
+mirror / glossy / refractive branches, a heightfield with many misses);
+with the wide walk it equals the one-kernel schedule's wide image on all but
+tie pixels; roulette from depth 3, path length 1 and the clutter room equal
+the one-kernel schedule bit for bit (the clutter room's wide image too)."""
+import numpy as np
+import pytest
+
+from gpugi_amd import workloads
+from test_gpu_pt import run_pt
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def wctx(rt):
+    c = rt.Context(0)  # its own context: the schedule stays out of other modules' tests
+    c.set_pt_schedule(rt.PT_WAVEFRONT)
+    yield c
+    c.close()
+
+
+def bitexact(img, acc):
+    diff = np.nonzero(img.view(np.uint32) != acc.view(np.uint32))
+    assert len(diff[0]) == 0, (diff[0][:5], diff[1][:5])
+
+
+def test_wavefront_cornell_bitexact(rt, oracle, wctx):
+    hs = rt.HostScene.cornell()
+    print("case: wavefront cornell 64x64", flush=True)
+    img, acc = run_pt(rt, oracle, wctx, hs, workloads.cornell_camera(64), 64, 64, 3, rt.KERNEL_REFERENCE)
+    assert img.mean() > 0.05
+    bitexact(img, acc)
+
+
+def test_wavefront_truncated_grid(rt, oracle, wctx):
+    hs = rt.HostScene.cornell()
+    print("case: wavefront cornell 60x44", flush=True)
+    img, acc = run_pt(rt, oracle, wctx, hs, workloads.cornell_camera(60), 60, 44, 1, rt.KERNEL_REFERENCE)
+    assert (img[40:, :, :3] == 0).all() and (img[:, 56:, :3] == 0).all()
+    bitexact(img, acc)
+
+
+@pytest.mark.parametrize("variant", ["mirror", "glossy", "refractive"])
+def test_wavefront_specular_branches(rt, oracle, wctx, variant):
+    from test_oracle_integrators import furnace_scene
+    params = {"mirror": dict(specular=(0.5, 0.5, 0.5), roughness=0.0),
+              "glossy": dict(specular=(0.3, 0.3, 0.3), roughness=0.4),
+              "refractive": dict(specular=(0.1, 0.1, 0.1), roughness=0.2, opacity=0.4)}[variant]
+    hs = furnace_scene(rt, **params)
+    cam = rt.camera([0.1, 0.2, 0.3], [0.5, -0.3, -1.0], [0, 1, 0], 60, 32, 32)
+    print("case: wavefront specular %s" % variant, flush=True)
+    img, acc = run_pt(rt, oracle, wctx, hs, cam, 32, 32, 2, rt.KERNEL_REFERENCE)
+    assert img[..., :3].mean() > 0.5
+    bitexact(img, acc)
+
+
+def test_wavefront_heightfield_misses(rt, oracle, wctx):
+    hs = rt.HostScene.heightfield(24, workloads.MASTER_SEED)
+    cam = rt.camera([0.5, 1.2, -0.6], [0.5, 0.1, 0.5], [0, 1, 0], 70, 48, 40)
+    print("case: wavefront heightfield 48x40", flush=True)
+    img, acc = run_pt(rt, oracle, wctx, hs, cam, 48, 40, 2, rt.KERNEL_REFERENCE, path_length=6)
+    bitexact(img, acc)
+
+
+def draw_both(rt, ctx_a, ctx_b, hs, cam, w, h, kernel, path_length, rr_depth=0, iters=2):
+    imgs = []
+    for c in (ctx_a, ctx_b):
+        c.load_scene(hs)
+        c.set_screen(w, h)
+        c.set_camera(cam)
+        c.set_max_path_length(path_length)
+        c.set_rr_start_depth(rr_depth)
+        c.set_integrator_kernel(kernel)
+        sampler = rt.LightSampler(hs, 7)
+        for it in range(iters):
+            c.set_light_samples(sampler.generate(128))
+            c.draw(rt.PT, rt.wanghash(it + 11))
+        imgs.append(c.readback(w, h)[0])
+        c.set_rr_start_depth(0)
+    return imgs
+
+
+def test_wavefront_wide_matches_megakernel(rt, ctx, wctx):
+    hs = rt.HostScene.cornell()
+    print("case: wavefront wide cornell", flush=True)
+    a, b = draw_both(rt, ctx, wctx, hs, workloads.cornell_camera(64), 64, 64, rt.KERNEL_WIDE, 5)
+    same = (a.view(np.uint32) == b.view(np.uint32)).all(-1)
+    assert same.mean() > 0.99
+    assert abs(a[..., :3].mean() - b[..., :3].mean()) <= 1e-3 * a[..., :3].mean()
+    ctx.set_integrator_kernel(rt.KERNEL_AUTO)
+
+
+@pytest.mark.parametrize("path_length,rr_depth", [(1, 0), (12, 3)])
+def test_wavefront_depth_settings_match_megakernel(rt, ctx, wctx, path_length, rr_depth):
+    hs = rt.HostScene.cornell()
+    print("case: wavefront path length %d rr depth %d" % (path_length, rr_depth), flush=True)
+    a, b = draw_both(rt, ctx, wctx, hs, workloads.cornell_camera(32), 32, 32, rt.KERNEL_REFERENCE, path_length,
+                     rr_depth)
+    assert a[..., :3].mean() > 0.01
+    bitexact(b, a)
+    ctx.set_integrator_kernel(rt.KERNEL_AUTO)
+
+
+def test_schedule_rejects_unknown_value(rt, wctx):
+    with pytest.raises(Exception):
+        wctx.set_pt_schedule(7)
+
+
+@pytest.mark.parametrize("kernel", ["reference", "wide"])
+def test_wavefront_clutter_matches_megakernel(rt, ctx, wctx, kernel):
+    """The clutter room (its emissive panels, path length 8, roulette from bounce 3)."""
+    hs = rt.HostScene.clutter(300, workloads.MASTER_SEED)
+    w, h = 192, 112
+    k = rt.KERNEL_REFERENCE if kernel == "reference" else rt.KERNEL_WIDE
+    print("case: wavefront clutter %s" % kernel, flush=True)
+    a, b = draw_both(rt, ctx, wctx, hs, workloads.clutter_camera(w, h), w, h, k, workloads.CLUTTER_PATH_LENGTH,
+                     workloads.CLUTTER_RR_START)
+    print("means", a[..., :3].mean(), b[..., :3].mean(), flush=True)
+    assert a[..., :3].mean() > 1e-3
+    if kernel == "reference":
+        bitexact(b, a)
+    else:
+        same = (a.view(np.uint32) == b.view(np.uint32)).all(-1)
+        assert same.mean() > 0.99
+        assert abs(a[..., :3].mean() - b[..., :3].mean()) <= 1e-3 * a[..., :3].mean()
+    ctx.set_integrator_kernel(rt.KERNEL_AUTO)
