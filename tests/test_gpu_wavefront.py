@@ -5,7 +5,8 @@ pathtracer.comp bit for bit (Cornell box, a truncated 8x8 dispatch grid, the
 mirror / glossy / refractive branches, a heightfield with many misses);
 with the wide walk it equals the one-kernel schedule's wide image on all but
 tie pixels; roulette from depth 3, path length 1 and the clutter room equal
-the one-kernel schedule bit for bit (the clutter room's wide image too)."""
+the one-kernel schedule bit for bit (the clutter room's wide image too). The environment map on misses, point
+lights and textured materials also equal the one-kernel schedule."""
 import numpy as np
 import pytest
 
@@ -128,3 +129,64 @@ def test_wavefront_clutter_matches_megakernel(rt, ctx, wctx, kernel):
         assert same.mean() > 0.99
         assert abs(a[..., :3].mean() - b[..., :3].mean()) <= 1e-3 * a[..., :3].mean()
     ctx.set_integrator_kernel(rt.KERNEL_AUTO)
+
+
+def both_schedules(rt, c, hs, cam, w, h, path_length):
+    """The same frames under the one-kernel and the wavefront schedule on context c."""
+    imgs = []
+    for sched in (rt.PT_MEGAKERNEL, rt.PT_WAVEFRONT):
+        c.set_pt_schedule(sched)
+        c.set_screen(w, h)
+        c.set_camera(cam)
+        c.set_max_path_length(path_length)
+        c.reset_accum()
+        sampler = rt.LightSampler(hs, 5)
+        for k in range(2):
+            c.set_light_samples(sampler.generate(128))
+            c.draw(rt.PT, rt.wanghash(k + 3))
+        imgs.append(c.readback(w, h)[0])
+    c.set_pt_schedule(rt.PT_MEGAKERNEL)
+    return imgs
+
+
+@pytest.mark.parametrize("kernel", ["reference", "wide"])
+def test_wavefront_environment_and_point_lights(rt, kernel):
+    """Misses that add the environment map, NEE toward point lights."""
+    hs = rt.HostScene.heightfield(16, 77)
+    for p in workloads.heightfield_lights():
+        hs.add_point_light(p, (0.5, 0.5, 0.5))
+    faces = np.random.default_rng(3).integers(0, 256, size=(6, 8, 8, 3), dtype=np.uint8)
+    c = rt.Context(0)
+    try:
+        c.load_scene(hs)
+        c.set_integrator_kernel(rt.KERNEL_REFERENCE if kernel == "reference" else rt.KERNEL_WIDE)
+        c.set_environment_map(faces)
+        print("case: wavefront environment %s" % kernel, flush=True)
+        a, b = both_schedules(rt, c, hs, workloads.heightfield_camera(96, 64), 96, 64, 4)
+    finally:
+        c.close()
+    assert a[..., :3].mean() > 0.01
+    if kernel == "reference":
+        bitexact(b, a)
+    else:
+        assert (a.view(np.uint32) == b.view(np.uint32)).all(-1).mean() > 0.99
+
+
+def test_wavefront_textures(rt):
+    """Textured diffuse / reflectiveness / emissivity slots read at the hit's texcoord."""
+    from test_gpu_textures import textured_cube, textures
+    hs = textured_cube(rt)
+    d8, refl, emis = textures()
+    c = rt.Context(0)
+    try:
+        c.load_scene(hs)
+        c.set_integrator_kernel(rt.KERNEL_REFERENCE)
+        for slot, tex in ((rt.TEX_DIFFUSE, d8), (rt.TEX_REFLECTIVENESS, refl), (rt.TEX_EMISSIVITY, emis)):
+            c.set_material_texture(0, slot, c.add_texture(tex))
+        print("case: wavefront textures", flush=True)
+        cam = rt.camera([0.1, 0.2, 0.3], [0.5, -0.3, -1.0], [0, 1, 0], 60, 32, 32)
+        a, b = both_schedules(rt, c, hs, cam, 32, 32, 4)
+    finally:
+        c.close()
+    assert a[..., :3].mean() > 0.1
+    bitexact(b, a)
