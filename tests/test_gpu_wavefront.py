@@ -5,8 +5,8 @@ pathtracer.comp bit for bit (Cornell box, a truncated 8x8 dispatch grid, the
 mirror / glossy / refractive branches, a heightfield with many misses);
 with the wide walk it equals the one-kernel schedule's wide image on all but
 tie pixels; roulette from depth 3, path length 1 and the clutter room equal
-the one-kernel schedule bit for bit (the clutter room's wide image too). The environment map on misses, point
-lights and textured materials also equal the one-kernel schedule."""
+the one-kernel schedule bit for bit (the clutter room's wide image too), and
+so do the environment map on misses, point lights and textured materials."""
 import numpy as np
 import pytest
 
