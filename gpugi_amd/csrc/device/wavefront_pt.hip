@@ -22,12 +22,21 @@ namespace {
 
 constexpr int kWfBlock = 256;
 
+// Pixel of path i: 8x8 tiles in row order, row-major inside a tile.
+__device__ __forceinline__ void tile_pixel(uint32_t i, uint32_t tilesW, int& gx, int& gy) {
+  const uint32_t tile = i / kTileThreads, k = i % kTileThreads;
+  gx = (int)((tile % tilesW) * kTile + k % kTile);
+  gy = (int)((tile / tilesW) * kTile + k / kTile);
+}
+
 __global__ __launch_bounds__(kWfBlock) void k_wf_raygen(DrawParams p, WavefrontBuffers b, uint32_t tilesW,
                                                         uint32_t numPaths) {
   const uint32_t i = blockIdx.x * kWfBlock + threadIdx.x;
   if (i >= numPaths) return;
-  // the reference dispatches W/8 x H/8 tiles: only those pixels are traced
-  const int gx = (int)(i % (tilesW * kTile)), gy = (int)(i / (tilesW * kTile));
+  // the reference dispatches W/8 x H/8 tiles: only those pixels are traced.
+  // Queue order is tile by tile, so 64 neighbouring pixels share a wave.
+  int gx, gy;
+  tile_pixel(i, tilesW, gx, gy);
   const uint32_t pix = (uint32_t)gy * (uint32_t)p.width + (uint32_t)gx;
   PtPath s;
   pt_begin(p, s, gx, gy, p.frameSeed);
@@ -128,7 +137,8 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_finish(DrawParams p, WavefrontB
                                                         uint32_t numPaths) {
   const uint32_t i = blockIdx.x * kWfBlock + threadIdx.x;
   if (i >= numPaths) return;
-  const int gx = (int)(i % (tilesW * kTile)), gy = (int)(i / (tilesW * kTile));
+  int gx, gy;
+  tile_pixel(i, tilesW, gx, gy);
   const uint32_t pix = (uint32_t)gy * (uint32_t)p.width + (uint32_t)gx;
   const float4 c = b.color[pix];
   float4* px = &p.accum[pix];
