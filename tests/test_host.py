@@ -268,8 +268,8 @@ def test_errors_without_context(rt):
 
 
 def test_next_row_entry_points_reject_null_arguments(rt):
-    """The texture, environment, MSE, bvh-type, importance-break and pinned-memory
-    entry points fail with GPUGI_ERR_ARG (-1) on a null context, before any device call."""
+    """The texture, environment, MSE, bvh-type, importance-break, pinned-memory and
+    PT-schedule entry points fail with GPUGI_ERR_ARG (-1) on a null context, before any device call."""
     lib = rt.lib()
     null = None
     assert lib.gpugi_add_texture(null, 1, 1, 0, null, null) == -1
@@ -283,6 +283,7 @@ def test_next_row_entry_points_reject_null_arguments(rt):
     assert lib.gpugi_host_free(null, null) == -1
     assert lib.gpugi_hostscene_obox_hierarchy(null, null, 0) == -1
     assert lib.gpugi_hostscene_num_textures(null, null) == -1
+    assert lib.gpugi_set_pt_schedule(null, 1) == -1
     hs = rt.HostScene.cornell()
     assert hs.textures() == [] and (hs.material_textures() == -1).all()
     ob = hs.obox_hierarchy()
