@@ -108,6 +108,18 @@ void or_texture_samples(int id, const float* uvs, size_t n, float* out);
 /* The cube-map lookup for n directions (3 floats each) into out (3 floats each). */
 void or_env_lookups(const float* tex, int size, const float* dirs, size_t n, float* out);
 
+/* Options of every integrator below (global state; defaults -1 and 0):
+ * specificPathLength >= 0 keeps only paths of that many bounces (the reference's
+ * SHOW_SPECIFIC_PATHLENGTH define, renderer.hpp:6-9, 1 = direct lighting);
+ * rrStartDepth is the first bounce at which Russian roulette runs (0, every
+ * bounce, is the reference; a later start has no reference counterpart). */
+void or_set_path_options(int specificPathLength, int rrStartDepth);
+extern int or_spl, or_rr_start;
+/* Pixel that ProjectToScreen (screenreproject.glsl:55-96) splats world point p
+ * into for the camera: returns 0 and (*ix, *iy) when it lies in front of the
+ * camera inside the screen, -1 otherwise. */
+int or_screen_pixel(const or_camera* c, int width, int height, const float* p, int* ix, int* iy);
+
 /* Integrators: one draw adds into accum (width x height x 4 floats). */
 void or_pathtracer(const or_scene* s, const float* lights, uint32_t num_lights, const or_camera* c, int width,
                    int height, uint32_t frame_seed, int max_path_length, float* accum, int row_begin, int row_end);

@@ -3,9 +3,11 @@
 Imported by tests/, by __graft_entry__.smoke() and by bench.py's cpu_baseline
 leg as the checker; the product package never imports it.
 """
+import contextlib
 import ctypes
 import os
 import subprocess
+import tempfile
 
 import numpy as np
 
@@ -42,62 +44,108 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB):
             build(reference=False)
-        l = ctypes.CDLL(LIB)
-        P = ctypes.c_void_p
-        l.or_wanghash.argtypes, l.or_wanghash.restype = [ctypes.c_uint32], ctypes.c_uint32
-        l.or_xorshift32.argtypes, l.or_xorshift32.restype = [ctypes.POINTER(ctypes.c_uint32)], ctypes.c_uint32
-        l.or_random01.argtypes, l.or_random01.restype = [ctypes.POINTER(ctypes.c_uint32)], ctypes.c_float
-        l.or_xorshift64star.argtypes, l.or_xorshift64star.restype = [ctypes.c_uint64], ctypes.c_uint64
-        l.or_xorshiftf.argtypes, l.or_xorshiftf.restype = [ctypes.POINTER(ctypes.c_uint64)], ctypes.c_float
-        for f in ("or_sinf", "or_cosf", "or_log2f", "or_exp2f", "or_atanf"):
-            getattr(l, f).argtypes, getattr(l, f).restype = [ctypes.c_float], ctypes.c_float
-        for f in ("or_powf", "or_atan2f"):
-            getattr(l, f).argtypes, getattr(l, f).restype = [ctypes.c_float, ctypes.c_float], ctypes.c_float
-        l.or_trace.argtypes = [P, ctypes.c_int, P, P, P, P, ctypes.c_size_t]
-        l.or_trace.restype = None
-        l.or_bruteforce.argtypes = [P, ctypes.c_int, P, P, ctypes.c_size_t]
-        l.or_bruteforce.restype = None
-        l.or_trace_mode.argtypes = [P, ctypes.c_int, ctypes.c_int, P, P, P, P, ctypes.c_size_t]
-        l.or_trace_mode.restype = None
-        l.or_bruteforce_mode.argtypes = [P, ctypes.c_int, ctypes.c_int, P, P, ctypes.c_size_t]
-        l.or_bruteforce_mode.restype = None
-        l.or_classify.argtypes = [P, P, ctypes.c_int32, ctypes.c_int32, ctypes.c_double]
-        l.or_classify.restype = ctypes.c_int
-        l.or_raygen_primary.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_uint32, P, ctypes.c_size_t,
-                                        ctypes.c_size_t]
-        l.or_raygen_primary.restype = None
-        l.or_raygen_diffuse.argtypes = [P, P, P, ctypes.c_size_t, ctypes.c_uint32, P, ctypes.c_size_t, ctypes.c_size_t]
-        l.or_raygen_diffuse.restype = None
-        l.or_raygen_shadow.argtypes = [P, P, ctypes.c_size_t, P, ctypes.c_uint32, P, ctypes.c_size_t, ctypes.c_size_t]
-        l.or_raygen_shadow.restype = None
-        l.or_bsdf_samples.argtypes = [P, P, P, ctypes.c_uint32, ctypes.c_int, ctypes.c_size_t, P]
-        l.or_bsdf_samples.restype = None
-        l.or_bsdf_evals.argtypes = [P, P, P, ctypes.c_int, P, ctypes.c_size_t, P]
-        l.or_bsdf_evals.restype = None
-        l.or_set_environment.argtypes, l.or_set_environment.restype = [P, ctypes.c_int], None
-        l.or_trace_importance.argtypes = [P, ctypes.c_int, P, P, P, ctypes.c_size_t, P, ctypes.c_float,
-                                          ctypes.c_float]
-        l.or_trace_importance.restype = None
-        l.or_set_textures.argtypes, l.or_set_textures.restype = [P, ctypes.c_uint32, P, P], None
-        l.or_texture_samples.argtypes, l.or_texture_samples.restype = [ctypes.c_int, P, ctypes.c_size_t, P], None
-        l.or_env_lookups.argtypes, l.or_env_lookups.restype = [P, ctypes.c_int, P, ctypes.c_size_t, P], None
-        l.or_pathtracer.argtypes = [P, P, ctypes.c_uint32, P, ctypes.c_int, ctypes.c_int, ctypes.c_uint32,
-                                    ctypes.c_int, P, ctypes.c_int, ctypes.c_int]
-        l.or_pathtracer.restype = None
-        l.or_lighttracer.argtypes = [P, P, ctypes.c_uint32, P, ctypes.c_int, ctypes.c_int, ctypes.c_uint32,
-                                     ctypes.c_int, P]
-        l.or_lighttracer.restype = None
-        l.or_bpt_warmup.argtypes = [P, P, ctypes.c_uint32, P, ctypes.c_int, ctypes.c_int, ctypes.c_uint32,
-                                    ctypes.c_int, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)]
-        l.or_bpt_warmup.restype = None
-        l.or_bpt_light_pass.argtypes = [P, P, ctypes.c_uint32, P, ctypes.c_int, ctypes.c_int, ctypes.c_uint32,
-                                        ctypes.c_int, ctypes.c_int, P, P]
-        l.or_bpt_light_pass.restype = ctypes.c_uint32
-        l.or_bpt_eye.argtypes = [P, P, ctypes.c_int, ctypes.c_int, ctypes.c_uint32, ctypes.c_int, ctypes.c_int, P,
-                                 ctypes.c_uint32, P, ctypes.c_int, ctypes.c_int]
-        l.or_bpt_eye.restype = None
-        _lib = l
+        _lib = _bind(ctypes.CDLL(LIB))
     return _lib
+
+
+@contextlib.contextmanager
+def planted(mistake):
+    """Runs the body against a separate oracle build with a planted mistake
+    (YFLIP, XSHIFT, NO_LIGHT_DIST_MIS or NO_HIT_DIST_MIS, the OR_PLANT_* macros
+    of oracle_lt.c), so a test can show that its check rejects that mistake."""
+    global _lib
+    with tempfile.TemporaryDirectory() as d:
+        out = os.path.join(d, "liboracle_%s.so" % mistake.lower())
+        subprocess.run(["make", "-s", "-C", HERE, "planted", "PLANT=" + mistake, "OUT=" + out], check=True)
+        saved = lib()
+        _lib = _bind(ctypes.CDLL(out))
+        try:
+            yield _lib
+        finally:
+            _lib = saved
+
+
+def _bind(l):
+    """Declares the argument and result types of every oracle entry point on library l."""
+    P = ctypes.c_void_p
+    l.or_wanghash.argtypes, l.or_wanghash.restype = [ctypes.c_uint32], ctypes.c_uint32
+    l.or_xorshift32.argtypes, l.or_xorshift32.restype = [ctypes.POINTER(ctypes.c_uint32)], ctypes.c_uint32
+    l.or_random01.argtypes, l.or_random01.restype = [ctypes.POINTER(ctypes.c_uint32)], ctypes.c_float
+    l.or_xorshift64star.argtypes, l.or_xorshift64star.restype = [ctypes.c_uint64], ctypes.c_uint64
+    l.or_xorshiftf.argtypes, l.or_xorshiftf.restype = [ctypes.POINTER(ctypes.c_uint64)], ctypes.c_float
+    for f in ("or_sinf", "or_cosf", "or_log2f", "or_exp2f", "or_atanf"):
+        getattr(l, f).argtypes, getattr(l, f).restype = [ctypes.c_float], ctypes.c_float
+    for f in ("or_powf", "or_atan2f"):
+        getattr(l, f).argtypes, getattr(l, f).restype = [ctypes.c_float, ctypes.c_float], ctypes.c_float
+    l.or_trace.argtypes = [P, ctypes.c_int, P, P, P, P, ctypes.c_size_t]
+    l.or_trace.restype = None
+    l.or_bruteforce.argtypes = [P, ctypes.c_int, P, P, ctypes.c_size_t]
+    l.or_bruteforce.restype = None
+    l.or_trace_mode.argtypes = [P, ctypes.c_int, ctypes.c_int, P, P, P, P, ctypes.c_size_t]
+    l.or_trace_mode.restype = None
+    l.or_bruteforce_mode.argtypes = [P, ctypes.c_int, ctypes.c_int, P, P, ctypes.c_size_t]
+    l.or_bruteforce_mode.restype = None
+    l.or_classify.argtypes = [P, P, ctypes.c_int32, ctypes.c_int32, ctypes.c_double]
+    l.or_classify.restype = ctypes.c_int
+    l.or_raygen_primary.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_uint32, P, ctypes.c_size_t,
+                                    ctypes.c_size_t]
+    l.or_raygen_primary.restype = None
+    l.or_raygen_diffuse.argtypes = [P, P, P, ctypes.c_size_t, ctypes.c_uint32, P, ctypes.c_size_t, ctypes.c_size_t]
+    l.or_raygen_diffuse.restype = None
+    l.or_raygen_shadow.argtypes = [P, P, ctypes.c_size_t, P, ctypes.c_uint32, P, ctypes.c_size_t, ctypes.c_size_t]
+    l.or_raygen_shadow.restype = None
+    l.or_bsdf_samples.argtypes = [P, P, P, ctypes.c_uint32, ctypes.c_int, ctypes.c_size_t, P]
+    l.or_bsdf_samples.restype = None
+    l.or_bsdf_evals.argtypes = [P, P, P, ctypes.c_int, P, ctypes.c_size_t, P]
+    l.or_bsdf_evals.restype = None
+    l.or_set_environment.argtypes, l.or_set_environment.restype = [P, ctypes.c_int], None
+    l.or_trace_importance.argtypes = [P, ctypes.c_int, P, P, P, ctypes.c_size_t, P, ctypes.c_float,
+                                      ctypes.c_float]
+    l.or_trace_importance.restype = None
+    l.or_set_textures.argtypes, l.or_set_textures.restype = [P, ctypes.c_uint32, P, P], None
+    l.or_texture_samples.argtypes, l.or_texture_samples.restype = [ctypes.c_int, P, ctypes.c_size_t, P], None
+    l.or_env_lookups.argtypes, l.or_env_lookups.restype = [P, ctypes.c_int, P, ctypes.c_size_t, P], None
+    l.or_pathtracer.argtypes = [P, P, ctypes.c_uint32, P, ctypes.c_int, ctypes.c_int, ctypes.c_uint32,
+                                ctypes.c_int, P, ctypes.c_int, ctypes.c_int]
+    l.or_pathtracer.restype = None
+    l.or_lighttracer.argtypes = [P, P, ctypes.c_uint32, P, ctypes.c_int, ctypes.c_int, ctypes.c_uint32,
+                                 ctypes.c_int, P]
+    l.or_lighttracer.restype = None
+    l.or_bpt_warmup.argtypes = [P, P, ctypes.c_uint32, P, ctypes.c_int, ctypes.c_int, ctypes.c_uint32,
+                                ctypes.c_int, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)]
+    l.or_bpt_warmup.restype = None
+    l.or_bpt_light_pass.argtypes = [P, P, ctypes.c_uint32, P, ctypes.c_int, ctypes.c_int, ctypes.c_uint32,
+                                    ctypes.c_int, ctypes.c_int, P, P]
+    l.or_bpt_light_pass.restype = ctypes.c_uint32
+    l.or_bpt_eye.argtypes = [P, P, ctypes.c_int, ctypes.c_int, ctypes.c_uint32, ctypes.c_int, ctypes.c_int, P,
+                             ctypes.c_uint32, P, ctypes.c_int, ctypes.c_int]
+    l.or_bpt_eye.restype = None
+    l.or_set_path_options.argtypes, l.or_set_path_options.restype = [ctypes.c_int, ctypes.c_int], None
+    l.or_screen_pixel.argtypes = [P, ctypes.c_int, ctypes.c_int, P, ctypes.POINTER(ctypes.c_int),
+                                  ctypes.POINTER(ctypes.c_int)]
+    l.or_screen_pixel.restype = ctypes.c_int
+    return l
+
+
+@contextlib.contextmanager
+def path_options(specific_path_length=-1, rr_start_depth=0):
+    """Every integrator in the body keeps only paths of specific_path_length
+    bounces (>= 0; the reference's SHOW_SPECIFIC_PATHLENGTH) and runs Russian
+    roulette from bounce rr_start_depth on (0: every bounce, the reference)."""
+    lib().or_set_path_options(int(specific_path_length), int(rr_start_depth))
+    try:
+        yield
+    finally:
+        lib().or_set_path_options(-1, 0)
+
+
+def screen_pixel(cam, w, h, point):
+    """The pixel ProjectToScreen splats a world point into, or None outside the screen."""
+    c = camera(cam)
+    p = np.ascontiguousarray(point, dtype=np.float32).reshape(3)
+    ix, iy = ctypes.c_int(), ctypes.c_int()
+    r = lib().or_screen_pixel(ctypes.byref(c), w, h, _p(p), ctypes.byref(ix), ctypes.byref(iy))
+    return None if r != 0 else (ix.value, iy.value)
 
 
 def _p(a):

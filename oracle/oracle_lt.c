@@ -42,25 +42,54 @@ static void cam_vectors(const or_camera* c, vec3* U, vec3* V, vec3* W, vec3* P) 
   *P = v3(c->position[0], c->position[1], c->position[2]);
 }
 
-/* ProjectToScreen, box filter, no depth-buffer test, no chained MIS (screenreproject.glsl:55-227) */
-static void project_to_screen(const or_scene* s, const or_camera* c, int Wd, int Ht, float* accum, vec3 rayO,
-                              vec3 rayD, vec3 gN, vec3 sN, const or_material* m, vec3 tp, int computeMIS) {
+/* The screen position of ProjectToScreen (screenreproject.glsl:71-95):
+ * crd = camera - p, the projection onto U/V over the view distance, the
+ * half-open screen window, and the pixel by round-to-nearest.  Returns 0 and
+ * the pixel when p is in front of the camera and inside the window.
+ *
+ * The OR_PLANT_* macros plant a known mistake into a separate build of the
+ * oracle (oracle.planted() in oracle.py); tests use those builds to show that
+ * their checks reject such a mistake.  The normal build defines none. */
+static int screen_pixel(const or_camera* c, int Wd, int Ht, vec3 p, vec3* crdOut, float* camViewDotRayOut,
+                        int* ix, int* iy) {
   vec3 U, V, W, P;
   cam_vectors(c, &U, &V, &W, &P);
-  vec3 crd = vsub(P, rayO);
+  vec3 crd = vsub(P, p);
   float camViewDotRay = -vdot(W, crd);
-  if (!(camViewDotRay > 0.0f)) return;
+  if (!(camViewDotRay > 0.0f)) return -1;
   float sx = -vdot(U, crd) / camViewDotRay;
   float sy = -vdot(V, crd) / camViewDotRay;
   sx = sx / vdot(U, U);
   sy = sy / vdot(V, V);
-  if (!(sx >= -1.0f && sx < 1.0f && sy > -1.0f && sy <= 1.0f)) return;
+#ifdef OR_PLANT_YFLIP
+  sy = -sy;
+#endif
+  if (!(sx >= -1.0f && sx < 1.0f && sy > -1.0f && sy <= 1.0f)) return -1;
+  *ix = (int)roundf((sx * 0.5f + 0.5f) * (float)Wd);
+  *iy = (int)roundf((sy * 0.5f + 0.5f) * (float)Ht);
+#ifdef OR_PLANT_XSHIFT
+  *ix += 1;
+#endif
+  if (crdOut) *crdOut = crd;
+  if (camViewDotRayOut) *camViewDotRayOut = camViewDotRay;
+  return 0;
+}
+
+int or_screen_pixel(const or_camera* c, int width, int height, const float* p, int* ix, int* iy) {
+  return screen_pixel(c, width, height, v3(p[0], p[1], p[2]), NULL, NULL, ix, iy);
+}
+
+/* ProjectToScreen, box filter, no depth-buffer test, no chained MIS (screenreproject.glsl:55-227) */
+static void project_to_screen(const or_scene* s, const or_camera* c, int Wd, int Ht, float* accum, vec3 rayO,
+                              vec3 rayD, vec3 gN, vec3 sN, const or_material* m, vec3 tp, int computeMIS) {
+  vec3 crd;
+  float camViewDotRay;
+  int ix, iy;
+  if (screen_pixel(c, Wd, Ht, rayO, &crd, &camViewDotRay, &ix, &iy) != 0) return;
   float distSq = vdot(crd, crd);
   float dist = sqrtf(distSq);
   crd = vdiv(crd, dist);
   vec3 co = vadd(vscalev(crd, 0.0001f), rayO);
-  int ix = (int)roundf((sx * 0.5f + 0.5f) * (float)Wd);
-  int iy = (int)roundf((sy * 0.5f + 0.5f) * (float)Ht);
   if (or_any_hit(s, co, crd, dist)) return;
   float cosAtCamera = fsat(camViewDotRay / dist);
   float f2i = corrected_out_dot_n(rayD, crd, gN, sN) / (cosAtCamera * cosAtCamera * cosAtCamera * c->pixel_area * distSq);
@@ -131,16 +160,20 @@ static int light_path(const or_scene* s, const float* lights, const or_camera* c
       }
       ++stored;
     }
-    if (mode != 2) project_to_screen(s, c, Wd, Ht, accum, o, d, gN, sN, &m, tp, mode == 1);
+    /* the camera connection closes a path of i+1 bounces (lighttracer.comp:133-157) */
+    if (mode != 2 && (or_spl < 0 || i + 1 == or_spl))
+      project_to_screen(s, c, Wd, Ht, accum, o, d, gN, sN, &m, tp, mode == 1);
     vec3 t1 = v3(1, 1, 1);
     float pdf;
     int dif;
     vec3 nd = or_sample_bsdf(d, &m, &seed, sN, &t1, 1, &pdf, &dif);
     t1 = vscalev(t1, adjoint_correction(d, nd, gN, sN));
     tp = vmul(tp, t1);
-    float q = fsat(or_luminance(t1));
-    if (or_random01(&seed) >= q) break;
-    tp = vdivv(tp, v3(q, q, q));
+    if (i >= or_rr_start) { /* Russian roulette (lighttracer.comp:183-188) from or_rr_start on */
+      float q = fsat(or_luminance(t1));
+      if (or_random01(&seed) >= q) break;
+      tp = vdivv(tp, v3(q, q, q));
+    }
     d = nd;
     o = vadd(o, vscalev(d, 0.0001f));
   }
@@ -253,15 +286,22 @@ void or_bpt_eye(const or_scene* s, const or_camera* c, int W, int H, uint32_t fr
         or_material m = or_mat_at(s, tri[3], tu, tv);
         float inCos = fsat(-vdot(n, d));
         float l2e = dist_mis(rayHit, lastCos) * mis_h(inCos / OR_PI);
-        mis = (i == 0) ? 1.0f : (lastP / fmaxf(lastP + l2e, 1e-5f));
-        color = vadd(color, vmul(vscale(mis, tp), m.emissivity));
+#ifdef OR_PLANT_NO_HIT_DIST_MIS
+        l2e = mis_h(inCos / OR_PI);
+#endif
+        /* SHOW_SPECIFIC_PATHLENGTH (pathtracer_bidir.comp:75-92): emission at vertex i */
+        if (or_spl < 0 || i == or_spl) {
+          mis = (i == 0) ? 1.0f : (lastP / fmaxf(lastP + l2e, 1e-5f));
+          color = vadd(color, vmul(vscale(mis, tp), m.emissivity));
+        }
         uint32_t idx = or_xorshift32(&seed) % actualN;
         const float* e = cache + (size_t)idx * 16;
         int32_t emat, elen;
         memcpy(&emat, &e[3], 4);
         memcpy(&elen, &e[11], 4);
         int pathLength = elen + i + 1;
-        if (pathLength <= maxLen) {
+        /* a connection closes a path of elen + i + 1 bounces (pathtracer_bidir.comp:101-104) */
+        if (pathLength <= maxLen && (or_spl < 0 || pathLength == or_spl)) {
           vec3 ld = vsub(v3(e[4], e[5], e[6]), o);
           float distSq = vdot(ld, ld);
           float dist = sqrtf(distSq);
@@ -301,6 +341,9 @@ void or_bpt_eye(const or_scene* s, const or_camera* c, int W, int H, uint32_t fr
                 lint = vscalev(flux, cosLight * OR_PI_2);
                 lightToEye = mis_h(cosLight / OR_PI) * dist_mis(dist, cosCamera);
               }
+#ifdef OR_PLANT_NO_LIGHT_DIST_MIS
+              lightToEye = lightToEye / dist_mis(dist, cosCamera);
+#endif
               misLight = lightToEye / fmaxf(lightToEye + e2l, 1e-5f);
             } else {
               lint = vscalev(flux, OR_PI_2);
@@ -317,9 +360,11 @@ void or_bpt_eye(const or_scene* s, const or_camera* c, int W, int H, uint32_t fr
         d = or_sample_bsdf(d, &m, &seed, n, &t1, 0, &spdf, &dif);
         o = vadd(o, vscalev(d, 0.0001f));
         tp = vmul(tp, t1);
-        float q = fsat(or_luminance(t1));
-        if (or_random01(&seed) >= q) break;
-        tp = vdivv(tp, v3(q, q, q));
+        if (i >= or_rr_start) { /* Russian roulette (pathtracer_bidir.comp:213-218) from or_rr_start on */
+          float q = fsat(or_luminance(t1));
+          if (or_random01(&seed) >= q) break;
+          tp = vdivv(tp, v3(q, q, q));
+        }
         float rev = mis_h(spdf);
         if (i == 0) {
           mis = rev / fmaxf(rev + lastP, 1e-5f);

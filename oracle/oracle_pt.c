@@ -194,6 +194,15 @@ vec3 or_shading_normal(const or_scene* s, const int32_t* tri, vec3 b) {
 /* Environment cube map of the path tracer (pathtracer.comp:56-58,
  * renderersystem.cpp:39-47, 173-186): NULL means the black default.  Texels are
  * linear RGB, 4 floats each, faces x-, x+, y-, y+, z-, z+. */
+/* Path options shared by every integrator of the oracle (or_set_path_options). */
+int or_spl = -1;
+int or_rr_start = 0;
+
+void or_set_path_options(int specificPathLength, int rrStartDepth) {
+  or_spl = specificPathLength < 0 ? -1 : specificPathLength;
+  or_rr_start = rrStartDepth < 0 ? 0 : rrStartDepth;
+}
+
 static const float* g_env = NULL;
 static int g_envSize = 0;
 
@@ -311,20 +320,27 @@ void or_pathtracer(const or_scene* s, const float* lights, uint32_t numLights, c
         float tu, tv;
         or_hit_texcoord(s, tri, b, &tu, &tv);
         or_material m = or_mat_at(s, tri[3], tu, tv);
-        if (!didNEE) color = vadd(color, vmul(thr, m.emissivity));
+        /* SHOW_SPECIFIC_PATHLENGTH (pathtracer.comp:74-84): emission at vertex i
+         * closes a path of length i, the light sample of vertex i one of length i+1 */
+        if (or_spl < 0 || i == or_spl)
+          if (!didNEE) color = vadd(color, vmul(thr, m.emissivity));
         didNEE = m.refl[3] < 500.0f;
-        int idx = (int)(or_xorshift32(&seed) % numLights);
-        vec3 rad = or_direct_light(s, lights, o, n, idx, d, &m, didNEE);
-        color = vadd(color, vscalev(vmul(rad, thr), lsw));
+        if (or_spl < 0 || i + 1 == or_spl) {
+          int idx = (int)(or_xorshift32(&seed) % numLights);
+          vec3 rad = or_direct_light(s, lights, o, n, idx, d, &m, didNEE);
+          color = vadd(color, vscalev(vmul(rad, thr), lsw));
+        }
         vec3 tp = v3(1, 1, 1);
         float pdf;
         int dif;
         d = or_sample_bsdf(d, &m, &seed, n, &tp, 0, &pdf, &dif);
         o = vadd(o, vscalev(d, 0.0001f));
         thr = vmul(thr, tp);
-        float q = fsat(or_luminance(tp));
-        if (or_random01(&seed) >= q) break;
-        thr = vdivv(thr, v3(q, q, q));
+        if (i >= or_rr_start) { /* Russian roulette (pathtracer.comp:99-104) from or_rr_start on */
+          float q = fsat(or_luminance(tp));
+          if (or_random01(&seed) >= q) break;
+          thr = vdivv(thr, v3(q, q, q));
+        }
       }
       float* px = accum + ((size_t)y * W + x) * 4;
       px[0] = color.x + px[0];

@@ -3,8 +3,8 @@
 - PT white furnace: a closed cube whose walls all emit E with albedo a renders
   E * (1 + a + ... + a^L) for path length L (NEE at every vertex, emission at
   the first hit only, Russian roulette unbiased).
-- LT and BPT agree with PT on the Cornell room away from directly visible
-  emitters and the half-footprint border row (the reference's light-sample
+- LT and BPT agree with PT (image means within 2.5 %) on the Cornell room away
+  from directly visible emitters and the half-footprint border row (the reference's light-sample
   splat divides by 2*pi and rounds pixel positions, showlightcaches.comp:61,
   screenreproject.glsl:288).
 - BPT warm-up counters: NumLightCacheEntries - R*N equals LightPathLengthSum.
@@ -74,7 +74,7 @@ def test_pt_mirror_furnace(rt, oracle, path_length):
 
 
 def emitter_mask(rt, oracle, sc, cam, w, h):
-    """Pixels whose centre ray sees an emitter, dilated by one pixel, plus the border."""
+    """Pixels whose centre ray sees an emitter, dilated by one pixel in every direction, plus the border."""
     ys, xs = np.mgrid[0:h, 0:w]
     U, V, W = (np.array(getattr(cam, k), np.float64) for k in ("u", "v", "w"))
     sx = (xs / w) * 2 - 1
@@ -92,11 +92,14 @@ def emitter_mask(rt, oracle, sc, cam, w, h):
     ok = hits[:, 0] >= 0
     lit[ok] = em[tri[hits[ok, 0], 3]]
     lit = lit.reshape(h, w)
+    # dilated over the 8-neighbourhood: a pixel diagonal to an emitter pixel can
+    # still see part of the emitter, which only the path tracer renders
     dil = lit.copy()
     dil[1:] |= lit[:-1]
     dil[:-1] |= lit[1:]
-    dil[:, 1:] |= lit[:, :-1]
-    dil[:, :-1] |= lit[:, 1:]
+    row = dil.copy()
+    dil[:, 1:] |= row[:, :-1]
+    dil[:, :-1] |= row[:, 1:]
     dil[0, :] = dil[-1, :] = True
     dil[:, 0] = dil[:, -1] = True
     return dil
@@ -128,8 +131,8 @@ def test_lt_and_bpt_agree_with_pt(rt, oracle):
             else:
                 acc, _, _ = oracle.bpt_draw(sc, ls, cam, w, h, rt.wanghash(it), L, cap, acc)
         imgs[name] = acc[..., :3].sum(-1)[mask].mean() / n
-    assert abs(imgs["lt"] / imgs["pt"] - 1) < 0.04, imgs
-    assert abs(imgs["bpt"] / imgs["pt"] - 1) < 0.04, imgs
+    assert abs(imgs["lt"] / imgs["pt"] - 1) < 0.025, imgs
+    assert abs(imgs["bpt"] / imgs["pt"] - 1) < 0.025, imgs
 
 
 def test_bpt_warmup_and_cache_structure(rt, oracle):
