@@ -68,6 +68,7 @@ struct gpugi_ctx {
   uint32_t numLights = 0, lightCap = 0;
   int maxPathLength = 16;  // default pathLength (application.cpp:123)
   int rrStartDepth = 0;
+  int specificPathLength = -1;  // SHOW_SPECIFIC_PATHLENGTH; -1: every path
   int triangleTest = GPUGI_TRIANGLE_REFERENCE;
   int bvhType = GPUGI_BVH_AABOX;       // applies from the next scene load
   bool sceneObox = false;              // dNodes holds TreeNode<OBox> (3 texels per node)
@@ -647,6 +648,8 @@ int gpugi_draw(gpugi_ctx* ctx, int integrator, uint32_t frame_seed) {
     return Fail(ctx, GPUGI_ERR_ARG, "gpugi_draw: unknown integrator");
   if (SetDev(ctx)) return GPUGI_ERR_HIP;
   if (integrator == GPUGI_PT_CACHE) {
+    if (ctx->specificPathLength >= 0)
+      return Fail(ctx, GPUGI_ERR_STATE, "gpugi_draw: the cached path tracer does not isolate path lengths");
     if (int rc = DrawCached(ctx, frame_seed)) return rc;
     ++ctx->iterations;
     return GPUGI_OK;
@@ -680,6 +683,7 @@ static DrawParams MakeDrawParams(gpugi_ctx* ctx, uint32_t frame_seed) {
   p.frameSeed = frame_seed;
   p.maxPathLength = ctx->maxPathLength;
   p.rrStartDepth = ctx->rrStartDepth;
+  p.specificPathLength = ctx->specificPathLength;
   p.watertight = Watertight(ctx);
   p.counters = (uint32_t*)ctx->dCounters;
   p.bptCapacity = ctx->bptCapacity;
@@ -999,6 +1003,8 @@ int gpugi_trace(gpugi_ctx* ctx, int query, int kernel, const gpugi_ray* rays, gp
   if (n > 0xffffff00ull) return Fail(ctx, GPUGI_ERR_ARG, "gpugi_trace: at most 2^32 - 256 rays per call");
   if (n == 0) return GPUGI_OK;
   if (SetDev(ctx)) return GPUGI_ERR_HIP;
+  // the stack-overflow flag reports this call only (a fill kernel, so the call can be captured)
+  HIP_TRY(ctx, LaunchFillU32((uint32_t*)ctx->dCounters + 1, 0u, 1, ctx->stream));
   const bool wide = kernel == GPUGI_KERNEL_WIDE || (kernel == GPUGI_KERNEL_AUTO && ctx->hasWide);
   // pinned host buffers: chunked copies that overlap the traversal; pageable
   // ones go through one staged copy each way, which the runtime does faster
@@ -1542,6 +1548,14 @@ int gpugi_set_rr_start_depth(gpugi_ctx* ctx, int depth) {
   if (!ctx) return Fail(ctx, GPUGI_ERR_ARG, "gpugi_set_rr_start_depth: null context");
   if (depth < 0 || depth > 64) return Fail(ctx, GPUGI_ERR_ARG, "gpugi_set_rr_start_depth: depth in [0, 64]");
   ctx->rrStartDepth = depth;
+  return GPUGI_OK;
+}
+
+int gpugi_set_specific_path_length(gpugi_ctx* ctx, int length) {
+  if (!ctx) return Fail(ctx, GPUGI_ERR_ARG, "gpugi_set_specific_path_length: null context");
+  if (length < -1 || length > 64)
+    return Fail(ctx, GPUGI_ERR_ARG, "gpugi_set_specific_path_length: length in [-1, 64]");
+  ctx->specificPathLength = length;
   return GPUGI_OK;
 }
 

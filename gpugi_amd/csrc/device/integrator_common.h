@@ -266,11 +266,16 @@ __device__ __forceinline__ bool pt_bounce(const DrawParams& p, const TraceCtx& c
   s.o = s.o + h.t * s.d;
   const gg::V3 n = shading_normal(p.ref.vinfo, h.tri, h.bary);
   const MaterialData m = load_material_hit(p.materials, p.tex, p.ref.vinfo, h.tri, h.bary);
-  if (!s.didNEE) s.color = s.color + s.throughput * m.emissivity;
+  // SHOW_SPECIFIC_PATHLENGTH (pathtracer.comp:74-84): the emission at bounce i
+  // closes a path of length i, the light sample one of length i + 1
+  const int spl = p.specificPathLength;
+  if ((spl < 0 || s.bounce == spl) && !s.didNEE) s.color = s.color + s.throughput * m.emissivity;
   s.didNEE = m.refl[3] < 500.0f;
-  const int sampleIndex = (int)(gg::xorshift32(s.seed) % p.numLights);
-  const gg::V3 radiance = estimate_direct_light<WIDE>(c, s.o, n, sampleIndex, s.d, m, s.didNEE);
-  s.color = s.color + radiance * s.throughput * lightSampleWeight;
+  if (spl < 0 || s.bounce + 1 == spl) {
+    const int sampleIndex = (int)(gg::xorshift32(s.seed) % p.numLights);
+    const gg::V3 radiance = estimate_direct_light<WIDE>(c, s.o, n, sampleIndex, s.d, m, s.didNEE);
+    s.color = s.color + radiance * s.throughput * lightSampleWeight;
+  }
   gg::V3 tp = gg::v3s(1.0f);
   float pdf;
   bool diffuse;

@@ -25,6 +25,7 @@ struct DrawParams {
   uint32_t frameSeed = 0;
   int maxPathLength = 16;
   int rrStartDepth = 0;                   // Russian roulette from this bounce on (0: every bounce, as the reference)
+  int specificPathLength = -1;            // >= 0: only paths of this many bounces (SHOW_SPECIFIC_PATHLENGTH)
   bool watertight = false;                // watertight triangle test instead of IntersectTriangle
   MaterialTextures tex;                   // material textures (none by default)
   const float4* env = nullptr;            // environment cube map: 6 faces x envSize^2 linear RGB texels, or none (black)

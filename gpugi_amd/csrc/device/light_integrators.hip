@@ -166,7 +166,9 @@ __device__ __forceinline__ bool lt_bounce(const DrawParams& p, const TraceCtx& c
     }
     ++s.stored;
   }
-  if (MODE != kModeBptWarmup) project_to_screen<WIDE>(c, s.o, s.d, h.geoNormal, sN, m, s.tp, MODE == kModeBptLight);
+  // the camera connection closes a path of i + 1 bounces (lighttracer.comp:133-157)
+  if (MODE != kModeBptWarmup && (p.specificPathLength < 0 || i + 1 == p.specificPathLength))
+    project_to_screen<WIDE>(c, s.o, s.d, h.geoNormal, sN, m, s.tp, MODE == kModeBptLight);
   gg::V3 t1 = gg::v3s(1.0f);
   float pdf;
   bool diffuse;
@@ -307,15 +309,19 @@ __device__ __forceinline__ bool bpt_eye_bounce(const DrawParams& p, const TraceC
   const MaterialData m = load_material_hit(p.materials, p.tex, p.ref.vinfo, h.tri, h.bary);
   const float incidentSurfaceCosSat = gg::saturate(-gg::dot(n, s.d));
   const float connectionLightToEyePath = distance_mis(rayHit, s.lastCosAbs) * mis_heuristic(incidentSurfaceCosSat / gg::kPI);
-  s.mis = (i == 0) ? 1.0f
-                   : (s.lastConnectionProbability /
-                      fmaxf(s.lastConnectionProbability + connectionLightToEyePath, gg::kDivisorEpsilon));
-  s.color = s.color + s.mis * s.tp * m.emissivity;
+  const int spl = p.specificPathLength;
+  if (spl < 0 || i == spl) {  // SHOW_SPECIFIC_PATHLENGTH (pathtracer_bidir.comp:75-92)
+    s.mis = (i == 0) ? 1.0f
+                     : (s.lastConnectionProbability /
+                        fmaxf(s.lastConnectionProbability + connectionLightToEyePath, gg::kDivisorEpsilon));
+    s.color = s.color + s.mis * s.tp * m.emissivity;
+  }
   // Connection to one random light-cache vertex.
   const uint32_t idx = gg::xorshift32(s.seed) % actualN;
   const LightCacheEntryDev& e = cache[idx];
   const int pathLength = e.pathLength + i + 1;
-  if (pathLength <= p.maxPathLength) {
+  // a connection closes a path of pathLength bounces (pathtracer_bidir.comp:101-108)
+  if (pathLength <= p.maxPathLength && (spl < 0 || pathLength == spl)) {
     const gg::V3 epos = gg::v3(e.position[0], e.position[1], e.position[2]);
     gg::V3 ld = epos - s.o;
     const float lightDistSq = gg::dot(ld, ld);

@@ -79,17 +79,23 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_shade(DrawParams p, WavefrontBu
   o = o + h.t * d;
   const gg::V3 n = shading_normal(p.ref.vinfo, h.tri, h.bary);
   const MaterialData m = load_material_hit(p.materials, p.tex, p.ref.vinfo, h.tri, h.bary);
-  if (!didNEE) color = color + throughput * m.emissivity;
+  // SHOW_SPECIFIC_PATHLENGTH (pathtracer.comp:74-84)
+  const int spl = p.specificPathLength;
+  if ((spl < 0 || bounce == spl) && !didNEE) color = color + throughput * m.emissivity;
   didNEE = m.refl[3] < 500.0f;
-  const int sampleIndex = (int)(gg::xorshift32(seed) % p.numLights);
-  const NeeSample ns = nee_sample(p, o, n, sampleIndex, d, m, didNEE);
-  if (ns.test) {
-    b.shadowRays[2 * (size_t)i] = make_float4(ns.o.x, ns.o.y, ns.o.z, ns.len);
-    b.shadowRays[2 * (size_t)i + 1] = make_float4(ns.dir.x, ns.dir.y, ns.dir.z, 0.0f);
+  if (spl < 0 || bounce + 1 == spl) {
+    const int sampleIndex = (int)(gg::xorshift32(seed) % p.numLights);
+    const NeeSample ns = nee_sample(p, o, n, sampleIndex, d, m, didNEE);
+    if (ns.test) {
+      b.shadowRays[2 * (size_t)i] = make_float4(ns.o.x, ns.o.y, ns.o.z, ns.len);
+      b.shadowRays[2 * (size_t)i + 1] = make_float4(ns.dir.x, ns.dir.y, ns.dir.z, 0.0f);
+    }
+    // w = 1: a term is added; value.xyz counts only when the shadow ray is not blocked
+    b.neeValue[i] = make_float4(ns.value.x, ns.value.y, ns.value.z, ns.test ? 1.0f : 0.5f);
+    b.neeWeight[i] = make_float4(throughput.x, throughput.y, throughput.z, lightSampleWeight);
+  } else {
+    b.neeValue[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);  // no light sample at this bounce
   }
-  // w = 1: a term is added; value.xyz counts only when the shadow ray is not blocked
-  b.neeValue[i] = make_float4(ns.value.x, ns.value.y, ns.value.z, ns.test ? 1.0f : 0.5f);
-  b.neeWeight[i] = make_float4(throughput.x, throughput.y, throughput.z, lightSampleWeight);
   b.color[pix] = make_float4(color.x, color.y, color.z, 0.0f);
   gg::V3 tp = gg::v3s(1.0f);
   float pdf;

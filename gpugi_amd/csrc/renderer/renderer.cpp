@@ -373,12 +373,16 @@ void RendererSystem::SetEnvironmentMap(int size, const std::string& xneg, const 
   if (size <= 0) throw Error(GPUGI_ERR_ARG, "SetEnvironmentMap: size must be positive");
   const std::string* files[6] = {&xneg, &xpos, &yneg, &ypos, &zneg, &zpos};
   const size_t faceBytes = (size_t)size * size * 3;
-  std::vector<uint8_t> faces(6 * faceBytes);
+  auto env = std::make_shared<EnvironmentMap>();
+  env->size = size;
+  env->faces.resize(6 * faceBytes);
   for (int k = 0; k < 6; ++k)
-    if (!LoadPpmRgb8(*files[k], size, faces.data() + k * faceBytes))
+    if (!LoadPpmRgb8(*files[k], size, env->faces.data() + k * faceBytes))
       throw Error(GPUGI_ERR_IO, "SetEnvironmentMap: cannot read a " + std::to_string(size) + "x" +
                                     std::to_string(size) + " binary PPM from " + *files[k]);
-  Check(gpugi_set_environment_map(m_ctx, size, faces.data()), m_ctx, "gpugi_set_environment_map");
+  Check(gpugi_set_environment_map(m_ctx, size, env->faces.data()), m_ctx, "gpugi_set_environment_map");
+  m_envMap = env;
+  if (m_activeRenderer) m_activeRenderer->SetEnvironmentMap(m_envMap);
 }
 
 // ---------------------------------------------------------------- MSE check

@@ -109,6 +109,15 @@ class LightSampler {
 
 class RendererSystem;
 
+// The cube map RendererSystem::SetEnvironmentMap loaded (the reference's
+// gl::TextureCubemap, renderersystem.cpp:173-186): six size x size sRGB RGB8
+// faces in the order x-, x+, y-, y+, z-, z+.  The context holds the decoded
+// copy every integrator samples on a miss.
+struct EnvironmentMap {
+  int size = 0;
+  std::vector<uint8_t> faces;
+};
+
 // renderer/renderer.hpp:25-50.
 class Renderer {
  public:
@@ -117,6 +126,8 @@ class Renderer {
   virtual void SetScene(std::shared_ptr<Scene> /*scene*/) {}
   virtual void SetCamera(const Camera& /*camera*/) {}
   virtual void SetScreenSize(int width, int height) = 0;
+  // renderer.hpp:40; called with the map RendererSystem holds (null: none).
+  virtual void SetEnvironmentMap(std::shared_ptr<const EnvironmentMap> /*envMap*/) {}
   virtual void Draw() = 0;
   const RendererSystem& GetRendererSystem() const { return m_rendererSystem; }
 
@@ -193,6 +204,7 @@ class RendererSystem {
     if (m_hasBackbuffer) m_activeRenderer->SetScreenSize(m_width, m_height);
     m_activeRenderer->SetCamera(m_camera);
     if (m_scene) m_activeRenderer->SetScene(m_scene);
+    m_activeRenderer->SetEnvironmentMap(m_envMap);
   }
   Renderer* GetActiveRenderer() { return m_activeRenderer.get(); }
 
@@ -247,6 +259,7 @@ class RendererSystem {
   Camera m_camera;
   LightSampler m_lightSampler;
   std::unique_ptr<Renderer> m_activeRenderer;
+  std::shared_ptr<const EnvironmentMap> m_envMap;
   std::vector<gpugi_light_sample> m_lightSamples;
   unsigned long m_iterationCount = 0;
   uint64_t m_renderTimeMs = 0;

@@ -81,6 +81,7 @@ _SIGNATURES = {
     "gpugi_set_integrator_kernel": [_P, ctypes.c_int],
     "gpugi_set_pt_schedule": [_P, ctypes.c_int],
     "gpugi_set_rr_start_depth": [_P, ctypes.c_int],
+    "gpugi_set_specific_path_length": [_P, ctypes.c_int],
     "gpugi_set_triangle_test": [_P, ctypes.c_int],
     "gpugi_draw": [_P, ctypes.c_int, ctypes.c_uint32],
     "gpugi_bpt_warmup": [_P, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)],
@@ -467,6 +468,10 @@ class Context:
 
     def set_rr_start_depth(self, depth):
         self._call("gpugi_set_rr_start_depth", int(depth))
+
+    def set_specific_path_length(self, length):
+        """Only paths of `length` bounces in PT/LT/BPT draws; -1 keeps every path."""
+        self._call("gpugi_set_specific_path_length", int(length))
 
     def set_triangle_test(self, mode):
         self._call("gpugi_set_triangle_test", int(mode))

@@ -193,6 +193,14 @@ int gpugi_set_bvh_type(gpugi_ctx* ctx, int bvh_type);
  * pathtracer.comp:99-104 and lighttracer.comp:183-188 do; 3 gives the
  * "roulette after depth 3" variant, which has no reference counterpart). */
 int gpugi_set_rr_start_depth(gpugi_ctx* ctx, int depth);
+/* Keeps only the paths of `length` bounces (1 = direct lighting) in
+ * GPUGI_PT, GPUGI_LT and GPUGI_BPT draws; -1 (default) keeps every path.
+ * Replaces the SHOW_SPECIFIC_PATHLENGTH define (renderer.hpp:6-9) that the
+ * reference compiles into pathtracer.comp:74-84, lighttracer.comp:133-157 and
+ * pathtracer_bidir.comp:75-108; as there, the light tracer's splat of the
+ * light samples themselves (showlightcaches.comp) is kept.  GPUGI_PT_CACHE
+ * draws fail with GPUGI_ERR_STATE while it is set. */
+int gpugi_set_specific_path_length(gpugi_ctx* ctx, int length);
 /* Traversal the integrators use: GPUGI_KERNEL_REFERENCE (bit-exact escape
  * walk) or GPUGI_KERNEL_WIDE / GPUGI_KERNEL_AUTO (4-wide quantized BVH).
  * Replaces the AABOX_BVH define of the shader compile (pathtracer.cpp:55-64). */
