@@ -5,12 +5,21 @@ BVH with 2-triangle leaves) and 2^24 incoherent rays (cosine-distributed from
 the hits of 1920 x 1080 primary rays), resident in HBM before timing.  A step
 is one closest-hit launch over all 2^24 rays.
 
-roofline: algorithmic bytes per ray B = 32 (ray) + 16 (hit) + 32 * N_box +
-52 * N_tri, where N_box / N_tri are the reference escape walk's box and
-triangle tests for that ray (counted on the device by the reference-walk
-counting kernel, which the tests pin to the CPU oracle).  achieved = sum(B) /
-average launch time (HIP events on the library stream), peak 8 TB/s.
-cpu_baseline: the CPU oracle's escape walk (single thread) on a bounded
+roofline: the HBM bytes one launch of the dominant kernel moves (FETCH_SIZE +
+WRITE_SIZE from the committed rocprofv3 --pmc summary, each scaled by what it
+reads for a 1 GiB copy) over this run's average launch time (HIP events on
+the library stream), against the 8 TB/s HBM peak.  The traversal is bound by
+VALU issue and latency rather than HBM; roofline.valu gives its VALU
+wave-instructions per CU cycle over those of the calibration kernel
+(gpugi_probe_valu) and the active lanes per VALU instruction, both from the
+committed counter summary.  extra.alg_bytes_*: the reference walk's
+algorithmic bytes per ray, B = 32 (ray) + 16 (hit) + 32 * N_box + 52 * N_tri
+(N_box / N_tri: the escape walk's box and triangle tests of that ray, counted
+on the device by the counting kernel the tests pin to the CPU oracle), over
+the launch time: the wide kernel visits far fewer nodes than that walk, and
+caches serve the rest, so that rate exceeds the HBM peak.
+cpu_baseline: the CPU oracle's escape walk on every host core the run may
+use (one thread per core, OMP_NUM_THREADS or the CPU count) on a bounded
 sample of the same rays.
 """
 import argparse
@@ -61,6 +70,8 @@ def parse():
     p.add_argument("--clutter-incoherent-only", action="store_true", help="only the incoherent trace (profiling)")
     p.add_argument("--calibrate", action="store_true",
                    help="append 3 device-to-device copies of 1 GiB (counter calibration for rocprofv3 --pmc)")
+    p.add_argument("--valu-probe", action="store_true",
+                   help="append 3 launches of the VALU calibration kernel (gpugi_probe_valu) for rocprofv3 --pmc")
     return p.parse_args()
 
 
@@ -79,6 +90,7 @@ def traffic_from_profile(name="trace_wide_heightfield_pmc.json"):
 
 PROFILE_WORKLOAD = "heightfield 1024x1024 cells (2,097,152 triangles), 2^24 incoherent rays, closest hit"
 DOMINANT_KERNEL = "void gpugi::k_trace_wide<false"
+PROBE_KERNEL = "gpugi::k_probe_valu"
 PROFILE_COMMANDS = {
     "kt": "cd /tmp && export TMPDIR=/tmp && timeout -k 10 300 rocprofv3 --kernel-trace --stats -d <out>/prof_kt "
           "-o kt --output-format csv -- python3 <repo>/bench.py > <out>/prof_kt.log",
@@ -168,15 +180,19 @@ def _has_counter(directory, counter):
 def summarize_counters(name, *dirs):
     """Writes profiles/<name>.json: per-launch averages of every counter that rocprofv3 --pmc collected for
     the dominant kernel in the given output directories, plus derived ratios when their inputs are there."""
-    per = {}
-    for d in dirs:
-        rows = _csv_rows(d, "_counter_collection.csv")
-        acc = {}
-        for r in rows:
-            if r["Kernel_Name"].startswith(DOMINANT_KERNEL):
-                acc.setdefault(r["Counter_Name"], []).append(float(r["Counter_Value"]))
-        for k, v in acc.items():
-            per[k] = sum(v) / len(v)
+    def per_kernel(prefix):
+        per = {}
+        for d in dirs:
+            acc = {}
+            for r in _csv_rows(d, "_counter_collection.csv"):
+                if r["Kernel_Name"].startswith(prefix):
+                    acc.setdefault(r["Counter_Name"], []).append(float(r["Counter_Value"]))
+            for k, v in acc.items():
+                per[k] = sum(v) / len(v)
+        return per
+
+    per = per_kernel(DOMINANT_KERNEL)
+    probe = per_kernel(PROBE_KERNEL)
     rays = float(1 << 24)
     derived = {}
     if "TCC_HIT_sum" in per and "TCC_MISS_sum" in per:
@@ -185,28 +201,43 @@ def summarize_counters(name, *dirs):
         derived["valu_wave_instructions_per_ray"] = per["SQ_INSTS_VALU"] / rays
     if "SQ_INSTS_VMEM_RD" in per:
         derived["vmem_read_wave_instructions_per_ray"] = per["SQ_INSTS_VMEM_RD"] / rays
+
+    def per_cu_cycle(c):
+        # GRBM_GUI_ACTIVE counts the launch's GPU-busy cycles summed over the 8 XCDs;
+        # 256 CUs: VALU wave-instructions issued per CU per cycle
+        return c["SQ_INSTS_VALU"] / (c["GRBM_GUI_ACTIVE"] / 8.0 * 256.0)
+
+    if "SQ_INSTS_VALU" in per and "GRBM_GUI_ACTIVE" in per:
+        derived["valu_wave_insts_per_cu_cycle"] = per_cu_cycle(per)
+    if "SQ_INSTS_VALU" in probe and "GRBM_GUI_ACTIVE" in probe:
+        derived["probe_valu_wave_insts_per_cu_cycle"] = per_cu_cycle(probe)
+        if "valu_wave_insts_per_cu_cycle" in derived:
+            derived["valu_issue_frac_of_probe"] = (derived["valu_wave_insts_per_cu_cycle"] /
+                                                   derived["probe_valu_wave_insts_per_cu_cycle"])
     if "SQ_THREAD_CYCLES_VALU" in per and "SQ_ACTIVE_INST_VALU" in per:
         derived["valu_thread_cycles_per_active_inst"] = per["SQ_THREAD_CYCLES_VALU"] / per["SQ_ACTIVE_INST_VALU"]
-    if "SQ_INSTS_VALU" in per and "GRBM_GUI_ACTIVE" in per:
-        # GRBM_GUI_ACTIVE is summed over the 8 XCDs; each of the 1024 SIMDs needs 4 cycles per
-        # wave64 FP32 VALU instruction, so this is the fraction of SIMD cycles spent issuing VALU
-        derived["valu_issue_fraction"] = 4.0 * per["SQ_INSTS_VALU"] / (per["GRBM_GUI_ACTIVE"] / 8.0 * 1024.0)
+        if "SQ_THREAD_CYCLES_VALU" in probe and "SQ_ACTIVE_INST_VALU" in probe:
+            # the calibration kernel runs all 64 lanes of every instruction
+            full = probe["SQ_THREAD_CYCLES_VALU"] / probe["SQ_ACTIVE_INST_VALU"]
+            derived["probe_thread_cycles_per_active_inst"] = full
+            derived["valu_active_lanes"] = 64.0 * derived["valu_thread_cycles_per_active_inst"] / full
     clutter = "clutter" in name
     workload = ("clutter room, 4000 icospheres (5,120,016 triangles), 2^24 incoherent rays, closest hit"
                 if clutter else PROFILE_WORKLOAD)
-    run = "--clutter --clutter-incoherent-only --steps 3 --calibrate" if clutter else "--steps 2 --warmup 1 --no-cpu-baseline"
+    run = ("--clutter --clutter-incoherent-only --steps 3 --calibrate --valu-probe" if clutter
+           else "--steps 2 --warmup 1 --no-cpu-baseline --calibrate --valu-probe")
     if "FETCH_SIZE" in per and "WRITE_SIZE" in per:
         gib_kib = float(1 << 20)
         fc = _pmc_per_launch([d for d in dirs if _has_counter(d, "FETCH_SIZE")][0], "FETCH_SIZE")[1]
         wc = _pmc_per_launch([d for d in dirs if _has_counter(d, "WRITE_SIZE")][0], "WRITE_SIZE")[1]
         derived["calibration"] = {"fetch_factor": gib_kib / fc, "write_factor": gib_kib / wc}
         derived["hbm_bytes_per_launch"] = (per["FETCH_SIZE"] * gib_kib / fc + per["WRITE_SIZE"] * gib_kib / wc) * 1024.0
-    out = {"kernel_prefix": DOMINANT_KERNEL, "workload": workload,
+    out = {"kernel_prefix": DOMINANT_KERNEL, "probe_kernel_prefix": PROBE_KERNEL, "workload": workload,
            "commands": ["cd /tmp && export TMPDIR=/tmp && timeout -s KILL 120 rocprofv3 --pmc <counters of one pass> "
                         "-d <out>/<pass> -o <pass> --output-format csv -- python3 <repo>/bench.py " + run,
                         "python3 <repo>/bench.py --summarize-counters %s %s" % (name, " ".join(
                             "<out>/" + os.path.basename(os.path.normpath(d)) for d in dirs))],
-           "per_launch": per, "derived": derived}
+           "per_launch": per, "probe_per_launch": probe, "derived": derived}
     with open(os.path.join(ROOT, "profiles", name + ".json"), "w") as f:
         json.dump(out, f, indent=1)
         f.write("\n")
@@ -249,7 +280,18 @@ def batch_benchmark(frames):
     }
 
 
-def clutter_benchmark(steps, iterations, incoherent_only=False, calibrate=False):
+def run_valu_probe(ctx):
+    """Three launches of the VALU calibration kernel (256 CUs x 32 blocks of 256 lanes,
+    64 x 2048 FMAs per lane): the counter run reads its VALU issue rate."""
+    blocks, iters = 256 * 32, 2048
+    out = ctx.alloc(blocks * 256 * 4)
+    probe = [ctx.probe_valu(out, blocks, iters) for _ in range(3)]
+    out.free()
+    return {"blocks": blocks, "iters": iters, "ms": [round(x, 4) for x in probe],
+            "fma_tflops": round(blocks * 256 * iters * 64 * 2 / (min(probe) * 1e-3) / 1e12, 2)}
+
+
+def clutter_benchmark(steps, iterations, incoherent_only=False, calibrate=False, valu_probe=False):
     """The 5,120,016-triangle clutter room: incoherent closest-hit Grays/s with its
     roofline (its BVH and triangles exceed the 256 MB MALL), then PT, LT and BPT
     iterations at 1920 x 1080, path length 8, roulette from bounce 3.
@@ -285,14 +327,20 @@ def clutter_benchmark(steps, iterations, incoherent_only=False, calibrate=False)
     hits = R.hits.download(np.int32, n * 4).reshape(n, 4)
     R.free()
     ms_inc = trace["incoherent"]["ms"]
-    roof = {"bound": "hbm", "achieved": round(alg_bytes / (ms_inc * 1e-3) / 1e9, 2), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "alg_bytes_per_ray": alg_bytes / n, "mean_box_tests": float(nbox.mean()),
+    # measured HBM bytes of one launch (committed counter summary) over this run's launch time; the
+    # reference walk's algorithmic bytes ride along under their own names
+    alg_gbs = alg_bytes / (ms_inc * 1e-3) / 1e9
+    roof = {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None, "traffic": None,
+            "alg_bytes_per_ray": alg_bytes / n, "alg_bytes_gbs": round(alg_gbs, 2),
+            "alg_bytes_over_hbm_peak": round(alg_gbs / HBM_PEAK_GBS, 4), "mean_box_tests": float(nbox.mean()),
             "mean_tri_tests": float(ntri.mean())}
-    roof["frac"] = round(roof["achieved"] / HBM_PEAK_GBS, 4)
     traffic = traffic_from_profile("trace_wide_clutter_pmc.json")
     if traffic:
-        roof["traffic"] = traffic
-        roof["hbm_gbs_measured"] = round(traffic / (ms_inc * 1e-3) / 1e9, 2)
+        hbm_gbs = traffic / (ms_inc * 1e-3) / 1e9
+        roof.update(traffic=traffic, achieved=round(hbm_gbs, 2), frac=round(hbm_gbs / HBM_PEAK_GBS, 4))
+    valu = valu_from_profile("trace_wide_clutter_pmc.json")
+    if valu:
+        roof["valu"] = valu
     if incoherent_only:
         if calibrate:  # 3 device-to-device copies of 1 GiB (counter calibration)
             a, b = ctx.alloc(1 << 30), ctx.alloc(1 << 30)
@@ -301,9 +349,10 @@ def clutter_benchmark(steps, iterations, incoherent_only=False, calibrate=False)
             ctx.sync()
             a.free()
             b.free()
+        probe = run_valu_probe(ctx) if valu_probe else None
         ctx.close()
         return {"metric": "incoherent closest-hit Grays/s (clutter room)", "value": trace["incoherent"]["grays"],
-                "unit": "Grays/s", "trace": trace, "roofline": roof}
+                "unit": "Grays/s", "trace": trace, "roofline": roof, "valu_probe": probe}
 
     ctx.set_screen(W, H)
     ctx.set_camera(workloads.clutter_camera(W, H))
@@ -452,18 +501,77 @@ def mlp_benchmark(ref_spp, max_frames, warmup=32):
     }
 
 
+def host_threads():
+    """Host cores this process may use: OMP_NUM_THREADS when set (the share a shared
+    host grants the run), else the CPUs in the affinity mask."""
+    try:
+        return max(1, int(os.environ["OMP_NUM_THREADS"]))
+    except (KeyError, ValueError):
+        return max(1, len(os.sched_getaffinity(0)))
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(scene_arrays, rays_host, seconds):
+    """The CPU oracle's escape walk on `host_threads()` threads (its C calls release
+    the GIL), each taking 4096-ray chunks of the sample in order until `seconds` pass."""
+    import concurrent.futures
+    import threading
     from oracle import oracle
     sc = oracle.Scene(scene_arrays)
-    done, t0 = 0, time.perf_counter()
+    threads = host_threads()
     chunk = 4096
-    while time.perf_counter() - t0 < seconds and done < len(rays_host):
-        oracle.trace(sc, rays_host[done:done + chunk])
-        done += chunk
+    lock = threading.Lock()
+    state = {"next": 0, "done": 0}
+    t0 = time.perf_counter()
+
+    def worker():
+        while time.perf_counter() - t0 < seconds:
+            with lock:
+                first = state["next"]
+                if first >= len(rays_host):
+                    return
+                state["next"] = first + chunk
+            part = rays_host[first:first + chunk]
+            oracle.trace(sc, part)
+            with lock:
+                state["done"] += len(part)
+
+    with concurrent.futures.ThreadPoolExecutor(max_workers=threads) as ex:
+        for f in [ex.submit(worker) for _ in range(threads)]:
+            f.result()
     dt = time.perf_counter() - t0
-    return {"value": done / dt / 1e9, "unit": "Grays/s", "cores": 1, "kind": "port",
+    done = state["done"]
+    return {"value": done / dt / 1e9, "unit": "Grays/s", "cores": threads, "kind": "port",
             "sample": "%d incoherent heightfield rays (the first of the 2^24 set), CPU oracle escape walk, "
-                      "1 thread, %.1f s" % (done, dt)}
+                      "%d threads, %.1f s; host: %s, %d CPUs visible" % (done, threads, dt, cpu_model(),
+                                                                     os.cpu_count() or 0)}
+
+
+def valu_from_profile(name="trace_wide_heightfield_utilization.json"):
+    """VALU issue fraction (trace kernel over the calibration kernel, wave-instructions
+    per CU cycle) and active lanes per VALU instruction, from a committed counter summary."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    try:
+        with open(path) as f:
+            d = json.load(f)["derived"]
+    except (OSError, KeyError, ValueError):
+        return None
+    out = {k: d[k] for k in ("valu_issue_frac_of_probe", "valu_wave_insts_per_cu_cycle",
+                             "probe_valu_wave_insts_per_cu_cycle", "valu_active_lanes",
+                             "valu_wave_instructions_per_ray", "tcc_hit_rate") if k in d}
+    return out or None
 
 
 def main():
@@ -486,7 +594,7 @@ def main():
         return
     if args.clutter:
         print(json.dumps(clutter_benchmark(args.steps, args.clutter_iterations, args.clutter_incoherent_only,
-                                           args.calibrate)))
+                                           args.calibrate, args.valu_probe)))
         return
     from gpugi_amd import rt, workloads
 
@@ -597,10 +705,21 @@ def main():
         ctx.sync()
         a.free()
         b.free()
+    if args.valu_probe:
+        extra["valu_probe"] = run_valu_probe(ctx)
+    # the reference walk's algorithmic bytes over the launch time (the contract's per-ray figure)
+    extra["alg_bytes_gbs"] = round(achieved, 2)
+    extra["alg_bytes_over_hbm_peak"] = round(achieved / HBM_PEAK_GBS, 4)
     traffic = traffic_from_profile()
+    roofline = {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None,
+                "traffic": traffic}
     if traffic:
         # HBM traffic the counters measured for one launch, over this run's launch time
-        extra["hbm_gbs_measured"] = round(traffic / (ms * 1e-3) / 1e9, 2)
+        hbm_gbs = traffic / (ms * 1e-3) / 1e9
+        roofline.update(achieved=round(hbm_gbs, 2), frac=round(hbm_gbs / HBM_PEAK_GBS, 4))
+    valu = valu_from_profile()
+    if valu:
+        roofline["valu"] = valu
     result = {
         "metric": "incoherent closest-hit Grays/s",
         "value": round(grays, 4),
@@ -617,8 +736,7 @@ def main():
         "config": {"workload": "heightfield 1024x1024 cells (2,097,152 triangles), 2^24 incoherent "
                                "cosine rays from 1920x1080 primary hits, closest hit",
                    "rays": n, "kernel": args.kernel},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
+        "roofline": roofline,
         "extra": extra,
     }
     if not args.no_cpu_baseline:

@@ -69,6 +69,7 @@ struct gpugi_ctx {
   int maxPathLength = 16;  // default pathLength (application.cpp:123)
   int rrStartDepth = 0;
   int specificPathLength = -1;  // SHOW_SPECIFIC_PATHLENGTH; -1: every path
+  hipGraphExec_t drawGraph = nullptr;  // the draw gpugi_capture_draw recorded (null: none)
   int triangleTest = GPUGI_TRIANGLE_REFERENCE;
   int bvhType = GPUGI_BVH_AABOX;       // applies from the next scene load
   bool sceneObox = false;              // dNodes holds TreeNode<OBox> (3 texels per node)
@@ -76,7 +77,6 @@ struct gpugi_ctx {
   int ptSchedule = GPUGI_PT_MEGAKERNEL;
   void* dWave = nullptr;         // wavefront path state and ray queues
   size_t waveCap = 0;
-  uint32_t* hWaveCount = nullptr;  // pinned: the next queue's length
   int bptCapacity = -1;
   int bptAverageLength = 0;
   IntegratorState integ;
@@ -139,6 +139,15 @@ void FreeDev(void*& p) {
 }
 
 bool Watertight(const gpugi_ctx* c) { return c->triangleTest == GPUGI_TRIANGLE_WATERTIGHT; }
+
+// Any call that changes what a recorded draw reads (buffers, their sizes, the
+// values its kernels were given) drops the recording.
+void DropGraph(gpugi_ctx* c) {
+  if (c && c->drawGraph) {
+    (void)hipGraphExecDestroy(c->drawGraph);
+    c->drawGraph = nullptr;
+  }
+}
 
 RefScene MakeRef(const gpugi_ctx* c) {
   RefScene s;
@@ -269,6 +278,8 @@ int gpugi_create(int device, gpugi_ctx** out) {
 void gpugi_destroy(gpugi_ctx* ctx) {
   if (!ctx) return;
   hipSetDevice(ctx->device);
+  if (ctx->stream) hipStreamSynchronize(ctx->stream);
+  DropGraph(ctx);
   if (ctx->side) hipStreamSynchronize(ctx->side);
   if (ctx->stream) hipStreamSynchronize(ctx->stream);
   for (void** p : {&ctx->dNodes, &ctx->dSlots, &ctx->dPos, &ctx->dVinfo, &ctx->dMats, &ctx->dWideNodes,
@@ -294,7 +305,6 @@ void gpugi_destroy(gpugi_ctx* ctx) {
     if (e) hipEventDestroy(e);
   if (ctx->copy) hipStreamDestroy(ctx->copy);
   for (void* p : ctx->hostAllocs) hipHostFree(p);
-  if (ctx->hWaveCount) hipHostFree(ctx->hWaveCount);
   if (ctx->side) hipStreamDestroy(ctx->side);
   if (ctx->stream) hipStreamDestroy(ctx->stream);
   delete ctx;
@@ -303,6 +313,7 @@ void gpugi_destroy(gpugi_ctx* ctx) {
 const char* gpugi_last_error(const gpugi_ctx* ctx) { return ctx ? ctx->error.c_str() : GlobalError(); }
 
 int gpugi_load_scene(gpugi_ctx* ctx, const gpugi_scene_desc* d) {
+  DropGraph(ctx);
   if (!ctx || !d) return Fail(ctx, GPUGI_ERR_ARG, "gpugi_load_scene: null argument");
   if (!d->hierarchy || !d->triangles || !d->positions || !d->vertex_info || d->num_nodes == 0 ||
       d->num_leaves == 0 || d->num_vertices == 0)
@@ -404,6 +415,7 @@ int gpugi_load_scene(gpugi_ctx* ctx, const gpugi_scene_desc* d) {
 }
 
 int gpugi_set_camera(gpugi_ctx* ctx, const gpugi_camera* camera) {
+  DropGraph(ctx);
   if (!ctx || !camera) return Fail(ctx, GPUGI_ERR_ARG, "gpugi_set_camera: null argument");
   ctx->camera = *camera;
   ctx->hasCamera = true;
@@ -416,6 +428,7 @@ int gpugi_set_camera(gpugi_ctx* ctx, const gpugi_camera* camera) {
 }
 
 int gpugi_set_screen(gpugi_ctx* ctx, int width, int height) {
+  DropGraph(ctx);
   if (!ctx || width <= 0 || height <= 0 || (size_t)width * height > (size_t)1 << 28)
     return Fail(ctx, GPUGI_ERR_ARG, "gpugi_set_screen: invalid size");
   if (SetDev(ctx)) return GPUGI_ERR_HIP;
@@ -432,6 +445,7 @@ int gpugi_set_screen(gpugi_ctx* ctx, int width, int height) {
 }
 
 int gpugi_set_light_samples(gpugi_ctx* ctx, const gpugi_light_sample* samples, uint32_t n) {
+  DropGraph(ctx);
   if (!ctx || !samples || n == 0) return Fail(ctx, GPUGI_ERR_ARG, "gpugi_set_light_samples: need n > 0 samples");
   if (SetDev(ctx)) return GPUGI_ERR_HIP;
   if (n > ctx->lightCap) {
@@ -450,6 +464,7 @@ int gpugi_set_light_samples(gpugi_ctx* ctx, const gpugi_light_sample* samples, u
 }
 
 int gpugi_set_max_path_length(gpugi_ctx* ctx, int max_path_length) {
+  DropGraph(ctx);
   if (!ctx || max_path_length < 1 || max_path_length > 64)
     return Fail(ctx, GPUGI_ERR_ARG, "gpugi_set_max_path_length: must be in [1, 64]");
   ctx->maxPathLength = max_path_length;
@@ -458,6 +473,7 @@ int gpugi_set_max_path_length(gpugi_ctx* ctx, int max_path_length) {
 }
 
 int gpugi_set_integrator_kernel(gpugi_ctx* ctx, int kernel) {
+  DropGraph(ctx);
   if (!ctx || kernel < GPUGI_KERNEL_AUTO || kernel > GPUGI_KERNEL_WIDE)
     return Fail(ctx, GPUGI_ERR_ARG, "gpugi_set_integrator_kernel: bad kernel");
   ctx->integratorKernel = kernel;
@@ -465,6 +481,7 @@ int gpugi_set_integrator_kernel(gpugi_ctx* ctx, int kernel) {
 }
 
 int gpugi_set_pt_schedule(gpugi_ctx* ctx, int schedule) {
+  DropGraph(ctx);
   if (!ctx || (schedule != GPUGI_PT_MEGAKERNEL && schedule != GPUGI_PT_WAVEFRONT))
     return Fail(ctx, GPUGI_ERR_ARG, "gpugi_set_pt_schedule: bad schedule");
   ctx->ptSchedule = schedule;
@@ -615,8 +632,9 @@ int DrawCached(gpugi_ctx* ctx, uint32_t frameSeed) {
   return GPUGI_OK;
 }
 
-// GPUGI_PT as per-bounce passes over device ray queues (wavefront_pt.hip).
-int DrawWavefront(gpugi_ctx* ctx, const DrawParams& p) {
+// Buffers and launch parameters of the wavefront path tracer (allocates what
+// is missing, so a capture can follow without allocating).
+int PrepareWavefront(gpugi_ctx* ctx, const DrawParams& p, WavefrontTrace* tr) {
   const size_t pixels = (size_t)ctx->width * ctx->height;
   const size_t bytes = WavefrontBytes(pixels);
   if (ctx->waveCap < bytes) {
@@ -625,16 +643,21 @@ int DrawWavefront(gpugi_ctx* ctx, const DrawParams& p) {
     HIP_TRY(ctx, hipMalloc(&ctx->dWave, bytes));
     ctx->waveCap = bytes;
   }
-  if (!ctx->hWaveCount) HIP_TRY(ctx, hipHostMalloc((void**)&ctx->hWaveCount, 4, hipHostMallocDefault));
+  tr->wide = p.useWide;
+  if (tr->wide)
+    if (int rc = PrepareTraceWide(ctx, ctx->wideStack, &tr->ws)) return rc;
+  tr->stack = ctx->wideStack;
+  tr->refill = ctx->wideRefill;
+  tr->wait = ctx->wideWait;
+  tr->passBlocks = (uint32_t)ctx->numCUs * 8u;  // 8 resident blocks of 256 lanes per CU
+  return GPUGI_OK;
+}
+
+// GPUGI_PT as per-bounce passes over device ray queues (wavefront_pt.hip).
+int DrawWavefront(gpugi_ctx* ctx, const DrawParams& p) {
   WavefrontTrace tr;
-  tr.wide = p.useWide;
-  if (tr.wide)
-    if (int rc = PrepareTraceWide(ctx, ctx->wideStack, &tr.ws)) return rc;
-  tr.stack = ctx->wideStack;
-  tr.refill = ctx->wideRefill;
-  tr.wait = ctx->wideWait;
-  tr.hostCount = ctx->hWaveCount;
-  HIP_TRY(ctx, DrawPTWavefront(p, tr, CarveWavefront(ctx->dWave, pixels), ctx->stream));
+  if (int rc = PrepareWavefront(ctx, p, &tr)) return rc;
+  HIP_TRY(ctx, DrawPTWavefront(p, tr, CarveWavefront(ctx->dWave, (size_t)ctx->width * ctx->height), ctx->stream));
   return GPUGI_OK;
 }
 
@@ -694,6 +717,45 @@ static DrawParams MakeDrawParams(gpugi_ctx* ctx, uint32_t frame_seed) {
   p.env = (const float4*)ctx->dEnv;
   p.envSize = ctx->envSize;
   return p;
+}
+
+int gpugi_capture_draw(gpugi_ctx* ctx, int integrator, uint32_t frame_seed) {
+  DropGraph(ctx);
+  if (!ctx) return Fail(ctx, GPUGI_ERR_ARG, "gpugi_capture_draw: null context");
+  if (!ctx->hasScene || !ctx->dAccum || !ctx->hasCamera || ctx->numLights == 0)
+    return Fail(ctx, GPUGI_ERR_STATE, "gpugi_capture_draw: scene, screen, camera and light samples must be set first");
+  if (integrator != GPUGI_PT || ctx->ptSchedule != GPUGI_PT_WAVEFRONT)
+    return Fail(ctx, GPUGI_ERR_UNSUPPORTED, "gpugi_capture_draw: only GPUGI_PT with the wavefront schedule is recorded");
+  if (SetDev(ctx)) return GPUGI_ERR_HIP;
+  const DrawParams p = MakeDrawParams(ctx, frame_seed);
+  WavefrontTrace tr;
+  if (int rc = PrepareWavefront(ctx, p, &tr)) return rc;  // every buffer exists before the capture
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  HIP_TRY(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+  const hipError_t e = DrawPTWavefront(p, tr, CarveWavefront(ctx->dWave, (size_t)ctx->width * ctx->height),
+                                       ctx->stream);
+  hipGraph_t graph = nullptr;
+  const hipError_t ee = hipStreamEndCapture(ctx->stream, &graph);
+  if (e != hipSuccess || ee != hipSuccess) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return Fail(ctx, GPUGI_ERR_HIP, std::string("gpugi_capture_draw: ") + hipGetErrorString(e != hipSuccess ? e : ee));
+  }
+  const hipError_t ei = hipGraphInstantiate(&ctx->drawGraph, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (ei != hipSuccess) {
+    ctx->drawGraph = nullptr;
+    return Fail(ctx, GPUGI_ERR_HIP, std::string("gpugi_capture_draw: ") + hipGetErrorString(ei));
+  }
+  return GPUGI_OK;
+}
+
+int gpugi_replay_draw(gpugi_ctx* ctx) {
+  if (!ctx) return Fail(ctx, GPUGI_ERR_ARG, "gpugi_replay_draw: null context");
+  if (!ctx->drawGraph) return Fail(ctx, GPUGI_ERR_STATE, "gpugi_replay_draw: no recorded draw (or a setter dropped it)");
+  if (SetDev(ctx)) return GPUGI_ERR_HIP;
+  HIP_TRY(ctx, hipGraphLaunch(ctx->drawGraph, ctx->stream));
+  ++ctx->iterations;
+  return GPUGI_OK;
 }
 
 int gpugi_bpt_warmup(gpugi_ctx* ctx, uint32_t frame_seed, uint64_t* num_light_cache_entries,
@@ -784,6 +846,7 @@ static void SrgbTable(float lut[256]) {
 }
 
 int gpugi_add_texture(gpugi_ctx* ctx, int width, int height, int format, const void* texels, int* id) {
+  DropGraph(ctx);
   if (!ctx || !texels || !id) return Fail(ctx, GPUGI_ERR_ARG, "gpugi_add_texture: null argument");
   if (!ctx->hasScene) return Fail(ctx, GPUGI_ERR_STATE, "gpugi_add_texture: load a scene first");
   if (width <= 0 || height <= 0 || width > 32768 || height > 32768)
@@ -822,6 +885,7 @@ int gpugi_add_texture(gpugi_ctx* ctx, int width, int height, int format, const v
 }
 
 int gpugi_set_material_texture(gpugi_ctx* ctx, uint32_t material, int slot, int texture) {
+  DropGraph(ctx);
   if (!ctx) return Fail(ctx, GPUGI_ERR_ARG, "gpugi_set_material_texture: null context");
   if (!ctx->hasScene) return Fail(ctx, GPUGI_ERR_STATE, "gpugi_set_material_texture: load a scene first");
   if (material >= ctx->matTex.size()) return Fail(ctx, GPUGI_ERR_ARG, "gpugi_set_material_texture: bad material");
@@ -837,6 +901,7 @@ int gpugi_set_material_texture(gpugi_ctx* ctx, uint32_t material, int slot, int 
 }
 
 int gpugi_set_environment_map(gpugi_ctx* ctx, int size, const uint8_t* faces) {
+  DropGraph(ctx);
   if (!ctx) return Fail(ctx, GPUGI_ERR_ARG, "gpugi_set_environment_map: null context");
   if (size < 0 || size > 16384 || (size > 0 && !faces))
     return Fail(ctx, GPUGI_ERR_ARG, "gpugi_set_environment_map: bad size or null faces");
@@ -1075,6 +1140,7 @@ int gpugi_trace_importance_break(gpugi_ctx* ctx, int query, const gpugi_ray* ray
 
 int gpugi_set_wide_config(gpugi_ctx* ctx, int max_leaf_triangles, int lds_stack_entries, int refill_lanes,
                           int wait_lanes) {
+  DropGraph(ctx);
   if (!ctx) return Fail(ctx, GPUGI_ERR_ARG, "gpugi_set_wide_config: null context");
   bool stackOk = false;
   for (int v : kWideStackVariants) stackOk |= v == lds_stack_entries;
@@ -1108,6 +1174,7 @@ int gpugi_get_wide_config(gpugi_ctx* ctx, int* max_leaf_triangles, int* lds_stac
 
 int gpugi_autotune_trace(gpugi_ctx* ctx, int query, const gpugi_ray* rays_dev, gpugi_hit* hits_dev, size_t n,
                          int repeats, float* best_ms) {
+  DropGraph(ctx);
   if (!ctx || !rays_dev || !hits_dev || n == 0 || repeats < 1)
     return Fail(ctx, GPUGI_ERR_ARG, "gpugi_autotune_trace: invalid argument");
   if (!ctx->hasScene) return Fail(ctx, GPUGI_ERR_STATE, "gpugi_autotune_trace: no scene loaded");
@@ -1183,6 +1250,18 @@ int gpugi_trace_counts(gpugi_ctx* ctx, int query, const gpugi_ray* rays, uint32_
     HIP_TRY(ctx, hipMemcpyAsync(tri_tests, dt, n * 4, hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   }
+  return GPUGI_OK;
+}
+
+int gpugi_probe_valu(gpugi_ctx* ctx, uint32_t blocks, uint32_t iters, float* out_device, float* milliseconds) {
+  if (!ctx || !out_device || !milliseconds || blocks == 0)
+    return Fail(ctx, GPUGI_ERR_ARG, "gpugi_probe_valu: invalid argument");
+  if (SetDev(ctx)) return GPUGI_ERR_HIP;
+  HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+  HIP_TRY(ctx, LaunchProbeValu(out_device, blocks, iters, ctx->stream));
+  HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+  HIP_TRY(ctx, hipEventSynchronize(ctx->ev1));
+  HIP_TRY(ctx, hipEventElapsedTime(milliseconds, ctx->ev0, ctx->ev1));
   return GPUGI_OK;
 }
 
@@ -1545,6 +1624,7 @@ int gpugi_cache_stats(gpugi_ctx* ctx, uint32_t* train_samples, float* loss) {
 }
 
 int gpugi_set_rr_start_depth(gpugi_ctx* ctx, int depth) {
+  DropGraph(ctx);
   if (!ctx) return Fail(ctx, GPUGI_ERR_ARG, "gpugi_set_rr_start_depth: null context");
   if (depth < 0 || depth > 64) return Fail(ctx, GPUGI_ERR_ARG, "gpugi_set_rr_start_depth: depth in [0, 64]");
   ctx->rrStartDepth = depth;
@@ -1552,6 +1632,7 @@ int gpugi_set_rr_start_depth(gpugi_ctx* ctx, int depth) {
 }
 
 int gpugi_set_specific_path_length(gpugi_ctx* ctx, int length) {
+  DropGraph(ctx);
   if (!ctx) return Fail(ctx, GPUGI_ERR_ARG, "gpugi_set_specific_path_length: null context");
   if (length < -1 || length > 64)
     return Fail(ctx, GPUGI_ERR_ARG, "gpugi_set_specific_path_length: length in [-1, 64]");
@@ -1568,6 +1649,7 @@ int gpugi_set_bvh_type(gpugi_ctx* ctx, int bvh_type) {
 }
 
 int gpugi_set_triangle_test(gpugi_ctx* ctx, int mode) {
+  DropGraph(ctx);
   if (!ctx) return Fail(ctx, GPUGI_ERR_ARG, "gpugi_set_triangle_test: null context");
   if (mode != GPUGI_TRIANGLE_REFERENCE && mode != GPUGI_TRIANGLE_WATERTIGHT)
     return Fail(ctx, GPUGI_ERR_ARG, "gpugi_set_triangle_test: unknown mode");

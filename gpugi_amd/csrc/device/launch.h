@@ -9,8 +9,9 @@
 namespace gpugi {
 
 // watertight: the watertight triangle test in place of the reference one.
+// nDevice (optional): device word holding the ray count, at most n (the grid is sized for n).
 hipError_t LaunchTraceRef(const RefScene& s, int query, const gpugi_ray* rays, gpugi_hit* hits, size_t n,
-                          bool watertight, hipStream_t stream);
+                          bool watertight, hipStream_t stream, const uint32_t* nDevice = nullptr);
 // The TRACERAY_IMPORTANCE_BREAK walk (ib.importance: one float2 per node);
 // nodes (optional, closest hit) receive {node, importance bits, size^2 bits, 0}.
 hipError_t LaunchTraceRefImportance(const RefScene& s, int query, const gpugi_ray* rays, gpugi_hit* hits,
@@ -22,12 +23,18 @@ hipError_t LaunchTraceRefCounts(const RefScene& s, int query, const gpugi_ray* r
 // refill: a wave reloads idle lanes once at least this many (1..64) are idle.
 // wait: a wave runs a leaf pass once at least this many (1..64) lanes wait for one.
 // watertight: the watertight triangle test (needs the raw-vertex triangle layout).
+// nDevice (optional): device word holding the ray count, at most n, read by the kernel.
 hipError_t LaunchTraceWide(const WideScene& s, int stack, int query, const gpugi_ray* rays, gpugi_hit* hits,
-                           size_t n, int refill, int wait, bool watertight, hipStream_t stream);
+                           size_t n, int refill, int wait, bool watertight, hipStream_t stream,
+                           const uint32_t* nDevice = nullptr);
 // Resident blocks per CU of the wide kernel variant (sizes the persistent grid).
 hipError_t WideBlocksPerCU(int stack, int* blocks);
 constexpr int kNumWideStackVariants = 6;
 extern const int kWideStackVariants[kNumWideStackVariants];
+
+// VALU calibration kernel: blocks x 256 lanes, each 64 * iters FP32 FMAs;
+// out receives one float per lane.
+hipError_t LaunchProbeValu(float* out, uint32_t blocks, uint32_t iters, hipStream_t stream);
 
 hipError_t LaunchRaygenPrimary(const gpugi_camera& cam, int width, int height, uint32_t seed,
                                gpugi_ray* rays, size_t n, hipStream_t stream);

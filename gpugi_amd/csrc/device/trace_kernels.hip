@@ -11,9 +11,10 @@ namespace gpugi {
 // Reference layout: one thread per ray.
 template <bool ANY, bool WT>
 __global__ __launch_bounds__(256) void k_trace_ref(RefScene s, const float4* __restrict__ rays,
-                                                   int4* __restrict__ hits, uint32_t n) {
+                                                   int4* __restrict__ hits, uint32_t n,
+                                                   const uint32_t* __restrict__ nDev) {
   const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-  if (i >= n) return;
+  if (i >= (nDev ? min(*nDev, n) : n)) return;  // nDev: the ray count as the device holds it
   const float4 r0 = rays[2 * (size_t)i], r1 = rays[2 * (size_t)i + 1];
   gg::V3 o = gg::v3(r0.x, r0.y, r0.z), d = gg::v3(r1.x, r1.y, r1.z);
   float rayLength = r0.w;
@@ -86,18 +87,18 @@ hipError_t LaunchTraceRefImportance(const RefScene& s, int query, const gpugi_ra
 }
 
 hipError_t LaunchTraceRef(const RefScene& s, int query, const gpugi_ray* rays, gpugi_hit* hits, size_t n,
-                          bool watertight, hipStream_t stream) {
+                          bool watertight, hipStream_t stream, const uint32_t* nDevice) {
   if (n == 0) return hipSuccess;
   dim3 grid((unsigned)((n + 255) / 256));
   const float4* r = (const float4*)rays;
   int4* h = (int4*)hits;
   const uint32_t m = (uint32_t)n;
   if (query == GPUGI_ANY_HIT) {
-    if (watertight) hipLaunchKernelGGL((k_trace_ref<true, true>), grid, dim3(256), 0, stream, s, r, h, m);
-    else hipLaunchKernelGGL((k_trace_ref<true, false>), grid, dim3(256), 0, stream, s, r, h, m);
+    if (watertight) hipLaunchKernelGGL((k_trace_ref<true, true>), grid, dim3(256), 0, stream, s, r, h, m, nDevice);
+    else hipLaunchKernelGGL((k_trace_ref<true, false>), grid, dim3(256), 0, stream, s, r, h, m, nDevice);
   } else {
-    if (watertight) hipLaunchKernelGGL((k_trace_ref<false, true>), grid, dim3(256), 0, stream, s, r, h, m);
-    else hipLaunchKernelGGL((k_trace_ref<false, false>), grid, dim3(256), 0, stream, s, r, h, m);
+    if (watertight) hipLaunchKernelGGL((k_trace_ref<false, true>), grid, dim3(256), 0, stream, s, r, h, m, nDevice);
+    else hipLaunchKernelGGL((k_trace_ref<false, false>), grid, dim3(256), 0, stream, s, r, h, m, nDevice);
   }
   return hipGetLastError();
 }

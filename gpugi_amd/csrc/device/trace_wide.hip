@@ -31,8 +31,10 @@ __global__ void k_wide_queue_reset(uint32_t* q) {
 
 template <bool ANY, int B, int S, bool WT>
 __global__ __launch_bounds__(B) void k_trace_wide(WideScene s, const float4* __restrict__ rays,
-                                                  int4* __restrict__ hits, uint32_t n, int refill, int wait) {
+                                                  int4* __restrict__ hits, uint32_t n, int refill, int wait,
+                                                  const uint32_t* __restrict__ nDev) {
   __shared__ uint2 lds[S * B];
+  if (nDev) n = min(*nDev, n);  // the ray count as the device holds it (a queue length)
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const uint32_t gtid = blockIdx.x * B + tid;
@@ -125,21 +127,21 @@ const Variant* FindVariant(int stack) {
 
 template <int S, bool WT>
 void LaunchS(const WideScene& s, int query, const gpugi_ray* rays, gpugi_hit* hits, size_t n, int refill,
-             int wait, hipStream_t stream) {
+             int wait, hipStream_t stream, const uint32_t* nDev) {
   dim3 grid(s.numThreads / kWideBlock);
   if (query == GPUGI_ANY_HIT)
     hipLaunchKernelGGL((k_trace_wide<true, kWideBlock, S, WT>), grid, dim3(kWideBlock), 0, stream, s,
-                       (const float4*)rays, (int4*)hits, (uint32_t)n, refill, wait);
+                       (const float4*)rays, (int4*)hits, (uint32_t)n, refill, wait, nDev);
   else
     hipLaunchKernelGGL((k_trace_wide<false, kWideBlock, S, WT>), grid, dim3(kWideBlock), 0, stream, s,
-                       (const float4*)rays, (int4*)hits, (uint32_t)n, refill, wait);
+                       (const float4*)rays, (int4*)hits, (uint32_t)n, refill, wait, nDev);
 }
 
 template <int S>
 void LaunchS(const WideScene& s, int query, const gpugi_ray* rays, gpugi_hit* hits, size_t n, int refill,
-             int wait, bool watertight, hipStream_t stream) {
-  if (watertight) LaunchS<S, true>(s, query, rays, hits, n, refill, wait, stream);
-  else LaunchS<S, false>(s, query, rays, hits, n, refill, wait, stream);
+             int wait, bool watertight, hipStream_t stream, const uint32_t* nDev) {
+  if (watertight) LaunchS<S, true>(s, query, rays, hits, n, refill, wait, stream, nDev);
+  else LaunchS<S, false>(s, query, rays, hits, n, refill, wait, stream, nDev);
 }
 
 }  // namespace
@@ -159,7 +161,8 @@ hipError_t WideBlocksPerCU(int stack, int* blocks) {
 }
 
 hipError_t LaunchTraceWide(const WideScene& s, int stack, int query, const gpugi_ray* rays, gpugi_hit* hits,
-                           size_t n, int refill, int wait, bool watertight, hipStream_t stream) {
+                           size_t n, int refill, int wait, bool watertight, hipStream_t stream,
+                           const uint32_t* nDevice) {
   if (n == 0) return hipSuccess;
   if (n > 0xffffffffull - 64) return hipErrorInvalidValue;
   // the static schedule needs a whole number of blocks per XCD
@@ -168,12 +171,12 @@ hipError_t LaunchTraceWide(const WideScene& s, int stack, int query, const gpugi
   if (!s.queue) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_wide_queue_reset, dim3(1), dim3(kWideQueueWords), 0, stream, s.queue);
   switch (stack) {
-    case 4: LaunchS<4>(s, query, rays, hits, n, refill, wait, watertight, stream); break;
-    case 8: LaunchS<8>(s, query, rays, hits, n, refill, wait, watertight, stream); break;
-    case 10: LaunchS<10>(s, query, rays, hits, n, refill, wait, watertight, stream); break;
-    case 12: LaunchS<12>(s, query, rays, hits, n, refill, wait, watertight, stream); break;
-    case 16: LaunchS<16>(s, query, rays, hits, n, refill, wait, watertight, stream); break;
-    case 24: LaunchS<24>(s, query, rays, hits, n, refill, wait, watertight, stream); break;
+    case 4: LaunchS<4>(s, query, rays, hits, n, refill, wait, watertight, stream, nDevice); break;
+    case 8: LaunchS<8>(s, query, rays, hits, n, refill, wait, watertight, stream, nDevice); break;
+    case 10: LaunchS<10>(s, query, rays, hits, n, refill, wait, watertight, stream, nDevice); break;
+    case 12: LaunchS<12>(s, query, rays, hits, n, refill, wait, watertight, stream, nDevice); break;
+    case 16: LaunchS<16>(s, query, rays, hits, n, refill, wait, watertight, stream, nDevice); break;
+    case 24: LaunchS<24>(s, query, rays, hits, n, refill, wait, watertight, stream, nDevice); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

@@ -24,18 +24,19 @@ struct WavefrontBuffers {
 };
 
 // How the passes trace: the wide kernel (with its launch parameters) or the
-// reference walk; hostCount: pinned or pageable 4 bytes for the queue length.
+// reference walk; passBlocks: the grid of the shade and resolve passes.
 struct WavefrontTrace {
   bool wide = false;
   WideScene ws;
   int stack = 10, refill = 8, wait = 16;
-  uint32_t* hostCount = nullptr;
+  uint32_t passBlocks = 0;
 };
 
 size_t WavefrontBytes(size_t numPixels);
 WavefrontBuffers CarveWavefront(void* base, size_t numPixels);
-// One PT draw (the reference triangle test only).  Synchronizes the stream once
-// per bounce to read the next queue's length.
+// One PT draw: maxPathLength bounces of passes enqueued on `stream`, each
+// reading its queue length on the device (no host synchronization, so the
+// draw can be captured into a graph).
 hipError_t DrawPTWavefront(const DrawParams& p, const WavefrontTrace& tr, const WavefrontBuffers& b,
                            hipStream_t stream);
 

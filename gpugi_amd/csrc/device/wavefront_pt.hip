@@ -12,6 +12,10 @@
 //                path that goes on is appended to the next queue with its ray
 //   trace any hit of the shadow rays             -> occluded[i]
 //   k_wf_resolve: color += (visible ? value : 0) * throughput * weight
+// Every pass reads the queue length from device memory: the host enqueues
+// maxPathLength bounces without waiting, so a frame can be captured in a
+// graph; passes over an empty queue return at once.  The shade and resolve
+// passes run a fixed grid of resident blocks that strides over the queue.
 #include "integrator_common.h"
 #include "launch.h"
 #include "wavefront_pt.h"
@@ -50,10 +54,11 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_raygen(DrawParams p, WavefrontB
 }
 
 // One bounce of pt_bounce for queue entry i (cur: the queue being shaded).
-__global__ __launch_bounds__(kWfBlock) void k_wf_shade(DrawParams p, WavefrontBuffers b, int cur, uint32_t count) {
-  const uint32_t i = blockIdx.x * kWfBlock + threadIdx.x;
-  if (i >= count) return;
+// Returns whether the path goes on; then (o, d) is its next ray.
+__device__ __forceinline__ bool wf_shade_one(const DrawParams& p, const WavefrontBuffers& b, int cur, uint32_t i,
+                                             uint32_t& pixOut, gg::V3& oOut, gg::V3& dOut) {
   const uint32_t pix = b.queue[cur][i];
+  pixOut = pix;
   const float4 r0 = b.rays[cur][2 * (size_t)i], r1 = b.rays[cur][2 * (size_t)i + 1];
   gg::V3 o = gg::v3(r0.x, r0.y, r0.z), d = gg::v3(r1.x, r1.y, r1.z);
   const int4 rec = b.hits[i];
@@ -73,7 +78,7 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_shade(DrawParams p, WavefrontBu
     color = color + throughput * (p.env ? env_radiance(p.env, p.envSize, d) : gg::v3s(0.0f));
     b.color[pix] = make_float4(color.x, color.y, color.z, 0.0f);
     b.neeValue[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    return;
+    return false;
   }
   const float lightSampleWeight = (float)p.numLights / 1.0f;
   o = o + h.t * d;
@@ -115,28 +120,53 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_shade(DrawParams p, WavefrontBu
   b.seed[pix] = seed;
   b.throughput[pix] = make_float4(throughput.x, throughput.y, throughput.z, 0.0f);
   b.flags[pix] = ((uint32_t)(bounce + 1) << 1) | (didNEE ? 1u : 0u);
-  if (alive && bounce + 1 < p.maxPathLength) {
-    const uint32_t k = atomicAdd(&b.counters[1 - cur], 1u);
-    b.queue[1 - cur][k] = pix;
-    b.rays[1 - cur][2 * (size_t)k] = make_float4(o.x, o.y, o.z, gg::kRayMax);
-    b.rays[1 - cur][2 * (size_t)k + 1] = make_float4(d.x, d.y, d.z, 0.0f);
+  oOut = o;
+  dOut = d;
+  return alive && bounce + 1 < p.maxPathLength;
+}
+
+// The shade pass over the queue's device length; paths that go on are appended
+// to the next queue, one atomic per wave and in lane order within the wave.
+__global__ __launch_bounds__(kWfBlock) void k_wf_shade(DrawParams p, WavefrontBuffers b, int cur) {
+  const uint32_t count = b.counters[cur];
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint64_t below = (1ull << lane) - 1ull;
+  // every lane of a wave runs the same trips: i - lane is uniform per wave
+  for (uint32_t base = blockIdx.x * kWfBlock + (threadIdx.x & ~63u); base < count; base += gridDim.x * kWfBlock) {
+    const uint32_t i = base + lane;
+    uint32_t pix = 0;
+    gg::V3 o = gg::v3s(0.0f), d = gg::v3s(0.0f);
+    const bool push = i < count && wf_shade_one(p, b, cur, i, pix, o, d);
+    const uint64_t m = __ballot(push);
+    if (m == 0) continue;
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    uint32_t first = 0;
+    if ((int)lane == leader) first = atomicAdd(&b.counters[1 - cur], (uint32_t)__popcll(m));
+    first = (uint32_t)__shfl((int)first, leader);
+    if (push) {
+      const uint32_t k = first + (uint32_t)__popcll(m & below);
+      b.queue[1 - cur][k] = pix;
+      b.rays[1 - cur][2 * (size_t)k] = make_float4(o.x, o.y, o.z, gg::kRayMax);
+      b.rays[1 - cur][2 * (size_t)k + 1] = make_float4(d.x, d.y, d.z, 0.0f);
+    }
   }
 }
 
 // color += radiance * throughput * weight, radiance = 0 when the shadow ray was
 // blocked or there was nothing to connect to (as estimate_direct_light returns).
-__global__ __launch_bounds__(kWfBlock) void k_wf_resolve(WavefrontBuffers b, int cur, uint32_t count) {
-  const uint32_t i = blockIdx.x * kWfBlock + threadIdx.x;
-  if (i >= count) return;
-  const float4 v = b.neeValue[i];
-  if (v.w == 0.0f) return;  // a miss: no NEE term
-  const bool visible = v.w == 1.0f && b.occluded[i].x == 0;
-  const gg::V3 radiance = visible ? gg::v3(v.x, v.y, v.z) : gg::v3s(0.0f);
-  const float4 w = b.neeWeight[i];
-  const uint32_t pix = b.queue[cur][i];
-  const float4 c4 = b.color[pix];
-  const gg::V3 color = gg::v3(c4.x, c4.y, c4.z) + radiance * gg::v3(w.x, w.y, w.z) * w.w;
-  b.color[pix] = make_float4(color.x, color.y, color.z, 0.0f);
+__global__ __launch_bounds__(kWfBlock) void k_wf_resolve(WavefrontBuffers b, int cur) {
+  const uint32_t count = b.counters[cur];
+  for (uint32_t i = blockIdx.x * kWfBlock + threadIdx.x; i < count; i += gridDim.x * kWfBlock) {
+    const float4 v = b.neeValue[i];
+    if (v.w == 0.0f) continue;  // a miss: no NEE term
+    const bool visible = v.w == 1.0f && b.occluded[i].x == 0;
+    const gg::V3 radiance = visible ? gg::v3(v.x, v.y, v.z) : gg::v3s(0.0f);
+    const float4 w = b.neeWeight[i];
+    const uint32_t pix = b.queue[cur][i];
+    const float4 c4 = b.color[pix];
+    const gg::V3 color = gg::v3(c4.x, c4.y, c4.z) + radiance * gg::v3(w.x, w.y, w.z) * w.w;
+    b.color[pix] = make_float4(color.x, color.y, color.z, 0.0f);
+  }
 }
 
 __global__ __launch_bounds__(kWfBlock) void k_wf_finish(DrawParams p, WavefrontBuffers b, uint32_t tilesW,
@@ -152,7 +182,12 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_finish(DrawParams p, WavefrontB
   *px = make_float4(c.x + a.x, c.y + a.y, c.z + a.z, 0.0f + a.w);  // vec4(color, 0) + imageLoad
 }
 
-__global__ void k_wf_zero(uint32_t* c) { c[threadIdx.x] = 0u; }
+// counters[0] = the first queue's length, counters[1] = 0
+__global__ void k_wf_init_counts(uint32_t* c, uint32_t first) {
+  if (threadIdx.x < 2) c[threadIdx.x] = threadIdx.x == 0 ? first : 0u;
+}
+
+__global__ void k_wf_zero(uint32_t* c) { c[0] = 0u; }
 
 dim3 Grid(uint32_t n) { return dim3((n + kWfBlock - 1) / kWfBlock); }
 
@@ -195,31 +230,29 @@ hipError_t DrawPTWavefront(const DrawParams& p, const WavefrontTrace& tr, const 
   const uint32_t tilesW = (uint32_t)(p.width / kTile), tilesH = (uint32_t)(p.height / kTile);
   const uint32_t numPaths = tilesW * tilesH * kTileThreads;
   if (numPaths == 0 || p.maxPathLength <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_wf_zero, dim3(1), dim3(2), 0, stream, b.counters);
+  if (tr.passBlocks == 0) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_wf_raygen, Grid(numPaths), dim3(kWfBlock), 0, stream, p, b, tilesW, numPaths);
-  uint32_t count = numPaths;
+  hipLaunchKernelGGL(k_wf_init_counts, dim3(1), dim3(64), 0, stream, b.counters, numPaths);
+  // resident blocks of the shade / resolve passes, at most one per 256 paths
+  const dim3 passGrid(std::min<uint32_t>(tr.passBlocks, (numPaths + kWfBlock - 1) / kWfBlock));
   int cur = 0;
-  for (int bounce = 0; bounce < p.maxPathLength && count > 0; ++bounce) {
+  for (int bounce = 0; bounce < p.maxPathLength; ++bounce) {
+    const uint32_t* count = b.counters + cur;  // this bounce's queue length, on the device
     const gpugi_ray* rays = (const gpugi_ray*)b.rays[cur];
-    hipError_t e = tr.wide ? LaunchTraceWide(tr.ws, tr.stack, GPUGI_CLOSEST_HIT, rays, (gpugi_hit*)b.hits, count,
-                                             tr.refill, tr.wait, p.watertight, stream)
-                           : LaunchTraceRef(p.ref, GPUGI_CLOSEST_HIT, rays, (gpugi_hit*)b.hits, count, p.watertight,
-                                            stream);
+    hipError_t e = tr.wide ? LaunchTraceWide(tr.ws, tr.stack, GPUGI_CLOSEST_HIT, rays, (gpugi_hit*)b.hits, numPaths,
+                                             tr.refill, tr.wait, p.watertight, stream, count)
+                           : LaunchTraceRef(p.ref, GPUGI_CLOSEST_HIT, rays, (gpugi_hit*)b.hits, numPaths,
+                                            p.watertight, stream, count);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_wf_zero, dim3(1), dim3(1), 0, stream, b.counters + (1 - cur));
-    hipLaunchKernelGGL(k_wf_shade, Grid(count), dim3(kWfBlock), 0, stream, p, b, cur, count);
+    hipLaunchKernelGGL(k_wf_shade, passGrid, dim3(kWfBlock), 0, stream, p, b, cur);
     const gpugi_ray* shadow = (const gpugi_ray*)b.shadowRays;
-    e = tr.wide ? LaunchTraceWide(tr.ws, tr.stack, GPUGI_ANY_HIT, shadow, (gpugi_hit*)b.occluded, count, tr.refill,
-                                  tr.wait, p.watertight, stream)
-                : LaunchTraceRef(p.ref, GPUGI_ANY_HIT, shadow, (gpugi_hit*)b.occluded, count, p.watertight, stream);
+    e = tr.wide ? LaunchTraceWide(tr.ws, tr.stack, GPUGI_ANY_HIT, shadow, (gpugi_hit*)b.occluded, numPaths,
+                                  tr.refill, tr.wait, p.watertight, stream, count)
+                : LaunchTraceRef(p.ref, GPUGI_ANY_HIT, shadow, (gpugi_hit*)b.occluded, numPaths, p.watertight,
+                                 stream, count);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_wf_resolve, Grid(count), dim3(kWfBlock), 0, stream, b, cur, count);
-    // the next queue's length sizes the next launches
-    e = hipMemcpyAsync(tr.hostCount, b.counters + (1 - cur), 4, hipMemcpyDeviceToHost, stream);
-    if (e != hipSuccess) return e;
-    e = hipStreamSynchronize(stream);
-    if (e != hipSuccess) return e;
-    count = *tr.hostCount;
+    hipLaunchKernelGGL(k_wf_resolve, passGrid, dim3(kWfBlock), 0, stream, b, cur);
     cur = 1 - cur;
   }
   hipLaunchKernelGGL(k_wf_finish, Grid(numPaths), dim3(kWfBlock), 0, stream, p, b, tilesW, numPaths);

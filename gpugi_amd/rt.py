@@ -81,6 +81,8 @@ _SIGNATURES = {
     "gpugi_set_integrator_kernel": [_P, ctypes.c_int],
     "gpugi_set_pt_schedule": [_P, ctypes.c_int],
     "gpugi_set_rr_start_depth": [_P, ctypes.c_int],
+    "gpugi_capture_draw": [_P, ctypes.c_int, ctypes.c_uint32],
+    "gpugi_replay_draw": [_P],
     "gpugi_set_specific_path_length": [_P, ctypes.c_int],
     "gpugi_set_triangle_test": [_P, ctypes.c_int],
     "gpugi_draw": [_P, ctypes.c_int, ctypes.c_uint32],
@@ -95,6 +97,7 @@ _SIGNATURES = {
     "gpugi_set_material_texture": [_P, ctypes.c_uint32, ctypes.c_int, ctypes.c_int],
     "gpugi_compute_mse": [_P, ctypes.c_uint32, ctypes.POINTER(ctypes.c_float)],
     "gpugi_trace": [_P, ctypes.c_int, ctypes.c_int, _P, _P, ctypes.c_size_t, ctypes.c_int],
+    "gpugi_probe_valu": [_P, ctypes.c_uint32, ctypes.c_uint32, _P, ctypes.POINTER(ctypes.c_float)],
     "gpugi_trace_counts": [_P, ctypes.c_int, _P, _P, _P, ctypes.c_size_t, ctypes.c_int],
     "gpugi_set_wide_config": [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int],
     "gpugi_get_wide_config": [_P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
@@ -469,6 +472,13 @@ class Context:
     def set_rr_start_depth(self, depth):
         self._call("gpugi_set_rr_start_depth", int(depth))
 
+    def capture_draw(self, integrator, frame_seed):
+        """Records one draw into a HIP graph (wavefront PT only); replay_draw runs it."""
+        self._call("gpugi_capture_draw", int(integrator), ctypes.c_uint32(frame_seed))
+
+    def replay_draw(self):
+        self._call("gpugi_replay_draw")
+
     def set_specific_path_length(self, length):
         """Only paths of `length` bounces in PT/LT/BPT draws; -1 keeps every path."""
         self._call("gpugi_set_specific_path_length", int(length))
@@ -581,6 +591,13 @@ class Context:
     def trace_device(self, rays_buf, hits_buf, n, query=CLOSEST_HIT, kernel=KERNEL_AUTO):
         self._call("gpugi_trace", int(query), int(kernel), ctypes.c_void_p(rays_buf.ptr),
                    ctypes.c_void_p(hits_buf.ptr), ctypes.c_size_t(n), 1)
+
+    def probe_valu(self, out_buf, blocks, iters):
+        """Runs the VALU calibration kernel (blocks x 256 lanes, 64 * iters FMAs each); returns ms."""
+        ms = ctypes.c_float()
+        self._call("gpugi_probe_valu", ctypes.c_uint32(blocks), ctypes.c_uint32(iters), ctypes.c_void_p(out_buf.ptr),
+                   ctypes.byref(ms))
+        return ms.value
 
     def set_wide_config(self, max_leaf_triangles, lds_stack_entries, refill_lanes=8, wait_lanes=16):
         self._call("gpugi_set_wide_config", int(max_leaf_triangles), int(lds_stack_entries), int(refill_lanes),

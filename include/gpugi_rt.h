@@ -209,14 +209,34 @@ int gpugi_set_integrator_kernel(gpugi_ctx* ctx, int kernel);
  * one lane carries a path through all its bounces, as the reference's
  * pathtracer.comp dispatch does) or GPUGI_PT_WAVEFRONT (one pass per bounce
  * over the live paths: closest-hit and shadow rays traced by the gpugi_trace
- * kernels on compact device queues; the host waits once per bounce for the
- * queue length).  Both give the same image; the wavefront schedule with the
+ * kernels on compact device queues whose lengths every pass reads on the
+ * device, so the host never waits; see gpugi_capture_draw).  Both give the
+ * same image with the reference walk; the wavefront schedule with the
  * reference walk and GPUGI_TRIANGLE_WATERTIGHT rebuilds b_x as 1 - (b_y + b_z)
  * from the hit record.  No counterpart in the reference. */
 int gpugi_set_pt_schedule(gpugi_ctx* ctx, int schedule);
+/* Measurement helper, no counterpart in the reference: launches the VALU
+ * calibration kernel (blocks x 256 lanes, eight independent FP32 FMA chains
+ * of 8 * iters each per lane, results into out_device, one float per lane) on
+ * the context's stream and returns its duration in milliseconds.  Under
+ * rocprofv3 its SQ_INSTS_VALU / GRBM_GUI_ACTIVE give the device's VALU issue
+ * rate, the roofline of the traversal kernels. */
+int gpugi_probe_valu(gpugi_ctx* ctx, uint32_t blocks, uint32_t iters, float* out_device, float* milliseconds);
 /* One iteration of an integrator that adds into the accumulation buffer
  * (Renderer::Draw, renderer.hpp:42; FrameSeed UBO, globaluniforms.glsl:17-21). */
 int gpugi_draw(gpugi_ctx* ctx, int integrator, uint32_t frame_seed);
+/* Records one gpugi_draw into a HIP graph (stream capture) without running
+ * it; gpugi_replay_draw launches the recording on the context's stream, each
+ * replay adding that frame (same frame seed) to the accumulation buffer and
+ * counting one iteration, with no host work per pass.  Only GPUGI_PT with the
+ * wavefront schedule is recorded (GPUGI_ERR_UNSUPPORTED otherwise): its passes
+ * read every queue length on the device.  Buffers are allocated before the
+ * capture.  Any setter that changes the scene, screen, camera, light samples,
+ * textures, environment or traversal settings drops the recording
+ * (gpugi_reset_accum keeps it).  No counterpart in the reference, whose draws
+ * are single dispatches. */
+int gpugi_capture_draw(gpugi_ctx* ctx, int integrator, uint32_t frame_seed);
+int gpugi_replay_draw(gpugi_ctx* ctx);
 /* One warm-up light pass of the bidirectional path tracer with the current
  * light samples (lighttracer.comp with SAVE_LIGHT_CACHE_WARMUP): returns the
  * NumLightCacheEntries and LightPathLengthSum counters the reference reads

@@ -6,7 +6,8 @@ mirror / glossy / refractive branches, a heightfield with many misses);
 with the wide walk it equals the one-kernel schedule's wide image on all but
 tie pixels; roulette from depth 3, path length 1 and the clutter room equal
 the one-kernel schedule bit for bit (the clutter room's wide image too), and
-so do the environment map on misses, point lights and textured materials."""
+so do the environment map on misses, point lights and textured materials.
+A frame recorded into a HIP graph replays to the eager frame bit for bit."""
 import numpy as np
 import pytest
 
@@ -190,3 +191,52 @@ def test_wavefront_textures(rt):
         c.close()
     assert a[..., :3].mean() > 0.1
     bitexact(b, a)
+
+
+@pytest.mark.parametrize("kernel", ["reference", "wide"])
+def test_wavefront_frame_graph_replay(rt, oracle, wctx, kernel):
+    """One wavefront frame recorded into a HIP graph (every pass reads its queue
+    length on the device) replays to the eager frame bit for bit, replay after
+    replay with eager draws of other frames in between; a setter drops it."""
+    print("case: wavefront graph replay %s" % kernel, flush=True)
+    hs = rt.HostScene.cornell()
+    w = h = 64
+    cam = workloads.cornell_camera(w)
+    wctx.load_scene(hs)
+    wctx.set_screen(w, h)
+    wctx.set_camera(cam)
+    wctx.set_max_path_length(5)
+    wctx.set_integrator_kernel(rt.KERNEL_REFERENCE if kernel == "reference" else rt.KERNEL_WIDE)
+    ls = rt.LightSampler(hs, 123).generate(128)
+    wctx.set_light_samples(ls)
+    seed = rt.wanghash(7)
+    wctx.reset_accum()
+    wctx.draw(rt.PT, seed)
+    eager, _ = wctx.readback(w, h)
+    if kernel == "reference":
+        sc = oracle.Scene(hs.arrays())
+        bitexact(eager, oracle.pathtrace(sc, ls, cam, w, h, seed, 5))
+    wctx.capture_draw(rt.PT, seed)
+    for k in range(3):
+        wctx.reset_accum()
+        wctx.draw(rt.PT, rt.wanghash(100 + k))  # eager work between the replays
+        wctx.sync()
+        wctx.reset_accum()
+        wctx.replay_draw()
+        img, n = wctx.readback(w, h)
+        assert n == 1
+        bitexact(img, eager)
+    # two replays add the frame twice
+    wctx.reset_accum()
+    wctx.replay_draw()
+    wctx.replay_draw()
+    img, n = wctx.readback(w, h)
+    assert n == 2
+    bitexact(img, eager + eager)
+    wctx.set_camera(cam)
+    with pytest.raises(rt.GpugiError):
+        wctx.replay_draw()
+    wctx.set_pt_schedule(rt.PT_MEGAKERNEL)
+    with pytest.raises(rt.GpugiError):
+        wctx.capture_draw(rt.PT, seed)
+    wctx.set_pt_schedule(rt.PT_WAVEFRONT)
