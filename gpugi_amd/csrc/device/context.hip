@@ -69,6 +69,7 @@ struct gpugi_ctx {
   int maxPathLength = 16;  // default pathLength (application.cpp:123)
   int rrStartDepth = 0;
   int specificPathLength = -1;  // SHOW_SPECIFIC_PATHLENGTH; -1: every path
+  void* noAllocScratch = nullptr;  // device address of gpugi_trace_noalloc's scratch array
   hipGraphExec_t drawGraph = nullptr;  // the draw gpugi_capture_draw recorded (null: none)
   int triangleTest = GPUGI_TRIANGLE_REFERENCE;
   int bvhType = GPUGI_BVH_AABOX;       // applies from the next scene load
@@ -270,6 +271,11 @@ int gpugi_create(int device, gpugi_ctx** out) {
       hipMalloc(&ctx->dWideQueue, kWideQueueWords * sizeof(uint32_t)) != hipSuccess) {
     gpugi_destroy(ctx);
     return Fail(nullptr, GPUGI_ERR_HIP, "gpugi_create: HIP stream/event/counter setup failed");
+  }
+  // the device address of gpugi_trace_noalloc's scratch array, once per device
+  if (NoAllocScratchAddress(device, &ctx->noAllocScratch) != hipSuccess) {
+    gpugi_destroy(ctx);
+    return Fail(nullptr, GPUGI_ERR_HIP, "gpugi_create: scratch symbol lookup failed");
   }
   *out = ctx;
   return GPUGI_OK;
@@ -1099,6 +1105,46 @@ int gpugi_trace(gpugi_ctx* ctx, int query, int kernel, const gpugi_ray* rays, gp
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     if (flag) return Fail(ctx, GPUGI_ERR_STATE, "gpugi_trace: traversal stack overflow (bug in the stack bound)");
   }
+  return GPUGI_OK;
+}
+
+int gpugi_trace_noalloc(gpugi_ctx* ctx, int query, int kernel, const gpugi_ray* rays, gpugi_hit* hits, size_t n) {
+  if (!ctx || (n > 0 && (!rays || !hits))) return Fail(ctx, GPUGI_ERR_ARG, "gpugi_trace_noalloc: null argument");
+  if (query != GPUGI_CLOSEST_HIT && query != GPUGI_ANY_HIT)
+    return Fail(ctx, GPUGI_ERR_ARG, "gpugi_trace_noalloc: bad query");
+  if (kernel < GPUGI_KERNEL_AUTO || kernel > GPUGI_KERNEL_WIDE)
+    return Fail(ctx, GPUGI_ERR_ARG, "gpugi_trace_noalloc: bad kernel");
+  if (!ctx->hasScene) return Fail(ctx, GPUGI_ERR_STATE, "gpugi_trace_noalloc: no scene loaded");
+  if (n > 0xffffff00ull) return Fail(ctx, GPUGI_ERR_ARG, "gpugi_trace_noalloc: at most 2^32 - 256 rays per call");
+  if (n == 0) return GPUGI_OK;
+  if (!ctx->noAllocScratch) return Fail(ctx, GPUGI_ERR_STATE, "gpugi_trace_noalloc: scratch not set up");
+  if (SetDev(ctx)) return GPUGI_ERR_HIP;
+  HIP_TRY(ctx, LaunchFillU32((uint32_t*)ctx->dCounters + 1, 0u, 1, ctx->stream));
+  const bool wide = kernel == GPUGI_KERNEL_WIDE || (kernel == GPUGI_KERNEL_AUTO && ctx->hasWide);
+  if (!wide) {  // the escape walk keeps no stack: nothing but the caller's buffers
+    HIP_TRY(ctx, LaunchTraceRef(MakeRef(ctx), query, rays, hits, n, Watertight(ctx), ctx->stream));
+    return GPUGI_OK;
+  }
+  if (!ctx->hasWide) return Fail(ctx, GPUGI_ERR_STATE, "gpugi_trace_noalloc: no wide hierarchy");
+  int blocksPerCU = 0;
+  HIP_TRY(ctx, WideBlocksPerCU(ctx->wideStack, &blocksPerCU));
+  if (blocksPerCU < 1) blocksPerCU = 1;
+  WideScene s = MakeWide(ctx, ctx->wideStack);
+  // the resident grid (whole blocks per XCD), shrunk until its spill stacks fit the array
+  uint32_t blocks = std::max<uint32_t>(8, (uint32_t)(ctx->numCUs * blocksPerCU) / 8 * 8);
+  const size_t perBlock = (size_t)s.spillDepth * kWideBlock * sizeof(uint2);
+  const size_t avail = GPUGI_NOALLOC_SCRATCH_BYTES - kNoAllocQueueBytes;
+  if (perBlock > 0) blocks = (uint32_t)std::min<size_t>(blocks, avail / perBlock / 8 * 8);
+  if (blocks < 8)
+    return Fail(ctx, GPUGI_ERR_ARG, "gpugi_trace_noalloc: the scene's traversal stack needs " +
+                                        std::to_string(8 * perBlock + kNoAllocQueueBytes) +
+                                        " bytes of scratch for the smallest grid, more than the " +
+                                        std::to_string(GPUGI_NOALLOC_SCRATCH_BYTES) + " the array holds");
+  s.numThreads = blocks * (uint32_t)kWideBlock;
+  s.queue = static_cast<uint32_t*>(ctx->noAllocScratch);
+  s.spill = reinterpret_cast<uint2*>(static_cast<char*>(ctx->noAllocScratch) + kNoAllocQueueBytes);
+  HIP_TRY(ctx, LaunchTraceWide(s, ctx->wideStack, query, rays, hits, n, ctx->wideRefill, ctx->wideWait,
+                               Watertight(ctx), ctx->stream));
   return GPUGI_OK;
 }
 

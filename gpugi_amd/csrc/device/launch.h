@@ -32,6 +32,13 @@ hipError_t WideBlocksPerCU(int stack, int* blocks);
 constexpr int kNumWideStackVariants = 6;
 extern const int kWideStackVariants[kNumWideStackVariants];
 
+// The allocation-free trace's scratch (trace_noalloc.hip): GPUGI_NOALLOC_SCRATCH_BYTES
+// per device, the first kNoAllocQueueBytes for the wide kernel's batch counters,
+// the rest for its spill stacks.  *out = the array's device address on `device`
+// (looked up once per device with that device current).
+constexpr size_t kNoAllocQueueBytes = 1024;
+hipError_t NoAllocScratchAddress(int device, void** out);
+
 // VALU calibration kernel: blocks x 256 lanes, each 64 * iters FP32 FMAs;
 // out receives one float per lane.
 hipError_t LaunchProbeValu(float* out, uint32_t blocks, uint32_t iters, hipStream_t stream);

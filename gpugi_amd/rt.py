@@ -98,6 +98,7 @@ _SIGNATURES = {
     "gpugi_compute_mse": [_P, ctypes.c_uint32, ctypes.POINTER(ctypes.c_float)],
     "gpugi_trace": [_P, ctypes.c_int, ctypes.c_int, _P, _P, ctypes.c_size_t, ctypes.c_int],
     "gpugi_probe_valu": [_P, ctypes.c_uint32, ctypes.c_uint32, _P, ctypes.POINTER(ctypes.c_float)],
+    "gpugi_trace_noalloc": [_P, ctypes.c_int, ctypes.c_int, _P, _P, ctypes.c_size_t],
     "gpugi_trace_counts": [_P, ctypes.c_int, _P, _P, _P, ctypes.c_size_t, ctypes.c_int],
     "gpugi_set_wide_config": [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int],
     "gpugi_get_wide_config": [_P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
@@ -598,6 +599,11 @@ class Context:
         self._call("gpugi_probe_valu", ctypes.c_uint32(blocks), ctypes.c_uint32(iters), ctypes.c_void_p(out_buf.ptr),
                    ctypes.byref(ms))
         return ms.value
+
+    def trace_noalloc(self, rays_buf, hits_buf, n, query=CLOSEST_HIT, kernel=KERNEL_AUTO):
+        """gpugi_trace_noalloc: device-resident trace that allocates nothing (asynchronous)."""
+        self._call("gpugi_trace_noalloc", int(query), int(kernel), ctypes.c_void_p(rays_buf.ptr),
+                   ctypes.c_void_p(hits_buf.ptr), ctypes.c_size_t(n))
 
     def set_wide_config(self, max_leaf_triangles, lds_stack_entries, refill_lanes=8, wait_lanes=16):
         self._call("gpugi_set_wide_config", int(max_leaf_triangles), int(lds_stack_entries), int(refill_lanes),

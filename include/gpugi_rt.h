@@ -222,6 +222,22 @@ int gpugi_set_pt_schedule(gpugi_ctx* ctx, int schedule);
  * rocprofv3 its SQ_INSTS_VALU / GRBM_GUI_ACTIVE give the device's VALU issue
  * rate, the roofline of the traversal kernels. */
 int gpugi_probe_valu(gpugi_ctx* ctx, uint32_t blocks, uint32_t iters, float* out_device, float* milliseconds);
+/* Bytes of the per-device scratch array of gpugi_trace_noalloc (64 MiB). */
+#define GPUGI_NOALLOC_SCRATCH_BYTES (64u << 20)
+/* gpugi_trace for device-resident rays and hits that allocates no memory: no
+ * hipMalloc, hipFree or allocating library call on any path, so it can run in
+ * code that must not allocate (under HIP graph capture, in serving loops, in
+ * harnesses that forbid allocation).  No counterpart in the reference; it is
+ * for those callers.  The wide kernel's batch counters and spill stacks live
+ * in a file-scope device array of GPUGI_NOALLOC_SCRATCH_BYTES that each GPU
+ * holds until the process ends; its grid shrinks to the spill that fits, and
+ * a scene whose traversal stack needs more than the array holds even for the
+ * smallest grid fails with GPUGI_ERR_ARG.  The call is asynchronous on the
+ * context's stream and reads nothing back (gpugi_sync waits for it).  Every
+ * call on one GPU shares the array: calls that overlap on one GPU race, so
+ * order them (one stream).  rays: n x 32 B and hits: n x 16 B device
+ * pointers; query and kernel as gpugi_trace. */
+int gpugi_trace_noalloc(gpugi_ctx* ctx, int query, int kernel, const gpugi_ray* rays, gpugi_hit* hits, size_t n);
 /* One iteration of an integrator that adds into the accumulation buffer
  * (Renderer::Draw, renderer.hpp:42; FrameSeed UBO, globaluniforms.glsl:17-21). */
 int gpugi_draw(gpugi_ctx* ctx, int integrator, uint32_t frame_seed);
