@@ -14,6 +14,7 @@
 #include "launch.h"
 #include "mlp.h"
 #include "scan.h"
+#include "hierarchy.h"
 #include "wavefront_pt.h"
 
 using namespace gpugi;
@@ -71,6 +72,16 @@ struct gpugi_ctx {
   int specificPathLength = -1;  // SHOW_SPECIFIC_PATHLENGTH; -1: every path
   void* noAllocScratch = nullptr;  // device address of gpugi_trace_noalloc's scratch array
   hipGraphExec_t drawGraph = nullptr;  // the draw gpugi_capture_draw recorded (null: none)
+  // HierarchyImportance passes: the tree levels, per node/slot importance {x, y},
+  // per node subtree importance and packed material (12 words)
+  bool hierTreeReady = false;
+  HierTree hierTree;
+  void* dHierParent = nullptr;
+  void* dHierLevel = nullptr;
+  void* dHierImp = nullptr;
+  void* dHierSubtree = nullptr;
+  void* dHierMat = nullptr;
+  bool hasHierImportance = false, hasHierMaterials = false;
   int triangleTest = GPUGI_TRIANGLE_REFERENCE;
   int bvhType = GPUGI_BVH_AABOX;       // applies from the next scene load
   bool sceneObox = false;              // dNodes holds TreeNode<OBox> (3 texels per node)
@@ -290,7 +301,8 @@ void gpugi_destroy(gpugi_ctx* ctx) {
   if (ctx->stream) hipStreamSynchronize(ctx->stream);
   for (void** p : {&ctx->dNodes, &ctx->dSlots, &ctx->dPos, &ctx->dVinfo, &ctx->dMats, &ctx->dWideNodes,
                    &ctx->dWideTris, &ctx->dSpill, &ctx->dCounters, &ctx->dWideQueue, &ctx->dAccum, &ctx->dLights,
-                   &ctx->dScratch, &ctx->dRefImage, &ctx->dMseTemp, &ctx->dEnv, &ctx->dWave})
+                   &ctx->dScratch, &ctx->dRefImage, &ctx->dMseTemp, &ctx->dEnv, &ctx->dWave, &ctx->dHierParent,
+                   &ctx->dHierLevel, &ctx->dHierImp, &ctx->dHierSubtree, &ctx->dHierMat})
     FreeDev(*p);
   FreeIntegratorState(&ctx->integ);
   for (void* p : ctx->texBufs) hipFree(p);
@@ -390,6 +402,10 @@ int gpugi_load_scene(gpugi_ctx* ctx, const gpugi_scene_desc* d) {
   ctx->numVertices = nv;
   ctx->numMaterials = d->num_materials;
   ctx->hasScene = true;
+  // the hierarchy passes' buffers belong to the previous tree
+  for (void** p : {&ctx->dHierParent, &ctx->dHierLevel, &ctx->dHierImp, &ctx->dHierSubtree, &ctx->dHierMat})
+    FreeDev(*p);
+  ctx->hierTreeReady = ctx->hasHierImportance = ctx->hasHierMaterials = false;
   try {
     ctx->hHier.assign(d->hierarchy, d->hierarchy + (size_t)nn * 8);
     ctx->hTris.assign(d->triangles, d->triangles + (size_t)nl * 8);
@@ -1148,29 +1164,118 @@ int gpugi_trace_noalloc(gpugi_ctx* ctx, int query, int kernel, const gpugi_ray* 
   return GPUGI_OK;
 }
 
+namespace {
+
+// Parent pointers, the nodes grouped by tree level and the importance,
+// subtree and material buffers of the current scene (made on first use).
+int EnsureHierarchy(gpugi_ctx* ctx) {
+  if (ctx->hierTreeReady) return GPUGI_OK;
+  std::vector<int32_t> parent;
+  std::vector<uint32_t> order, start;
+  if (!BuildHierTree(ctx->hHier.data(), ctx->numNodes, &parent, &order, &start))
+    return Fail(ctx, GPUGI_ERR_FORMAT, "hierarchy: the nodes are not in preorder");
+  const size_t nn = ctx->numNodes;
+  for (void** p : {&ctx->dHierParent, &ctx->dHierLevel, &ctx->dHierImp, &ctx->dHierSubtree, &ctx->dHierMat})
+    FreeDev(*p);
+  HIP_TRY(ctx, hipMalloc(&ctx->dHierParent, nn * 4));
+  HIP_TRY(ctx, hipMalloc(&ctx->dHierLevel, nn * 4));
+  HIP_TRY(ctx, hipMalloc(&ctx->dHierImp, (nn + ctx->numSlots) * 8));
+  HIP_TRY(ctx, hipMalloc(&ctx->dHierSubtree, nn * 4));
+  HIP_TRY(ctx, hipMalloc(&ctx->dHierMat, nn * 48));
+  HIP_TRY(ctx, hipMemcpy(ctx->dHierParent, parent.data(), nn * 4, hipMemcpyHostToDevice));
+  HIP_TRY(ctx, hipMemcpy(ctx->dHierLevel, order.data(), nn * 4, hipMemcpyHostToDevice));
+  ctx->hierTree.numNodes = ctx->numNodes;
+  ctx->hierTree.parent = static_cast<const int32_t*>(ctx->dHierParent);
+  ctx->hierTree.levelOrder = static_cast<const uint32_t*>(ctx->dHierLevel);
+  ctx->hierTree.levelStart = std::move(start);
+  ctx->hierTreeReady = true;
+  return GPUGI_OK;
+}
+
+}  // namespace
+
+int gpugi_hierarchy_update_importance(gpugi_ctx* ctx, int iterations) {
+  if (!ctx) return Fail(ctx, GPUGI_ERR_ARG, "gpugi_hierarchy_update_importance: null context");
+  if (iterations < 1 || iterations > 4096)
+    return Fail(ctx, GPUGI_ERR_ARG, "gpugi_hierarchy_update_importance: iterations in [1, 4096]");
+  if (!ctx->hasScene || !ctx->dAccum || !ctx->hasCamera || ctx->numLights == 0)
+    return Fail(ctx, GPUGI_ERR_STATE,
+                "gpugi_hierarchy_update_importance: scene, screen, camera and light samples must be set first");
+  if (SetDev(ctx)) return GPUGI_ERR_HIP;
+  if (int rc = EnsureHierarchy(ctx)) return rc;
+  ctx->hasHierImportance = false;
+  const DrawParams p = MakeDrawParams(ctx, 0);
+  float2* imp = static_cast<float2*>(ctx->dHierImp);
+  HIP_TRY(ctx, hipMemsetAsync(imp, 0, ((size_t)ctx->numNodes + ctx->numSlots) * 8, ctx->stream));
+  for (int i = 1; i <= iterations; ++i)  // NumImportanceIterations = i (hierarchyimportance.cpp:93-103)
+    HIP_TRY(ctx, LaunchHierAcquire(p, i, imp, ctx->numNodes, ctx->stream));
+  HIP_TRY(ctx, LaunchHierPropagate(p.ref, ctx->hierTree, iterations, imp, static_cast<float*>(ctx->dHierSubtree),
+                                   ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->hasHierImportance = true;
+  return GPUGI_OK;
+}
+
+int gpugi_hierarchy_compute_materials(gpugi_ctx* ctx) {
+  if (!ctx) return Fail(ctx, GPUGI_ERR_ARG, "gpugi_hierarchy_compute_materials: null context");
+  if (!ctx->hasScene) return Fail(ctx, GPUGI_ERR_STATE, "gpugi_hierarchy_compute_materials: no scene loaded");
+  if (SetDev(ctx)) return GPUGI_ERR_HIP;
+  if (int rc = EnsureHierarchy(ctx)) return rc;
+  DrawParams p = MakeDrawParams(ctx, 0);
+  HIP_TRY(ctx, LaunchHierMaterials(p, ctx->hierTree, static_cast<uint32_t*>(ctx->dHierMat), ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->hasHierMaterials = true;
+  return GPUGI_OK;
+}
+
+int gpugi_hierarchy_read(gpugi_ctx* ctx, float* importance, size_t importance_floats, float* subtree,
+                         size_t subtree_floats, uint32_t* materials, size_t material_words) {
+  if (!ctx) return Fail(ctx, GPUGI_ERR_ARG, "gpugi_hierarchy_read: null context");
+  const size_t nn = ctx->numNodes;
+  if ((importance && importance_floats < (nn + ctx->numSlots) * 2) || (subtree && subtree_floats < nn) ||
+      (materials && material_words < nn * 12))
+    return Fail(ctx, GPUGI_ERR_ARG, "gpugi_hierarchy_read: buffer too small");
+  if ((importance || subtree) && !ctx->hasHierImportance)
+    return Fail(ctx, GPUGI_ERR_STATE, "gpugi_hierarchy_read: no importance computed for this scene");
+  if (materials && !ctx->hasHierMaterials)
+    return Fail(ctx, GPUGI_ERR_STATE, "gpugi_hierarchy_read: no materials computed for this scene");
+  if (SetDev(ctx)) return GPUGI_ERR_HIP;
+  if (importance)
+    HIP_TRY(ctx, hipMemcpyAsync(importance, ctx->dHierImp, (nn + ctx->numSlots) * 8, hipMemcpyDeviceToHost,
+                                ctx->stream));
+  if (subtree) HIP_TRY(ctx, hipMemcpyAsync(subtree, ctx->dHierSubtree, nn * 4, hipMemcpyDeviceToHost, ctx->stream));
+  if (materials) HIP_TRY(ctx, hipMemcpyAsync(materials, ctx->dHierMat, nn * 48, hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return GPUGI_OK;
+}
+
 int gpugi_trace_importance_break(gpugi_ctx* ctx, int query, const gpugi_ray* rays, gpugi_hit* hits,
                                  gpugi_node_hit* nodes, size_t n, const float* importance, uint32_t num_nodes,
                                  float threshold, float ray_threshold) {
-  if (!ctx || (n > 0 && (!rays || !hits)) || !importance)
+  if (!ctx || (n > 0 && (!rays || !hits)))
     return Fail(ctx, GPUGI_ERR_ARG, "gpugi_trace_importance_break: null argument");
+  if (!importance && !ctx->hasHierImportance)
+    return Fail(ctx, GPUGI_ERR_STATE,
+                "gpugi_trace_importance_break: no importance given and none computed (gpugi_hierarchy_update_importance)");
   if (query != GPUGI_CLOSEST_HIT && query != GPUGI_ANY_HIT)
     return Fail(ctx, GPUGI_ERR_ARG, "gpugi_trace_importance_break: bad query");
   if (!ctx->hasScene) return Fail(ctx, GPUGI_ERR_STATE, "gpugi_trace_importance_break: no scene loaded");
-  if (num_nodes != ctx->numNodes)
+  if (importance && num_nodes != ctx->numNodes)
     return Fail(ctx, GPUGI_ERR_ARG, "gpugi_trace_importance_break: need one importance pair per node");
   if (n > 0xffffff00ull) return Fail(ctx, GPUGI_ERR_ARG, "gpugi_trace_importance_break: at most 2^32 - 256 rays");
   if (n == 0) return GPUGI_OK;
   if (SetDev(ctx)) return GPUGI_ERR_HIP;
   const size_t bRays = n * sizeof(gpugi_ray), bHits = n * sizeof(gpugi_hit), bNodes = n * sizeof(gpugi_node_hit);
-  const size_t bImp = (size_t)num_nodes * 8;
+  const size_t bImp = importance ? (size_t)num_nodes * 8 : 0;
   if (int rc = EnsureScratch(ctx, bRays + bHits + bNodes + bImp)) return rc;
   char* base = (char*)ctx->dScratch;
   gpugi_ray* dr = (gpugi_ray*)base;
   gpugi_hit* dh = (gpugi_hit*)(base + bRays);
   int4* dn = (int4*)(base + bRays + bHits);
-  float2* di = (float2*)(base + bRays + bHits + bNodes);
+  // the caller's importance, or the one gpugi_hierarchy_update_importance left on the device
+  float2* di = importance ? (float2*)(base + bRays + bHits + bNodes) : static_cast<float2*>(ctx->dHierImp);
   HIP_TRY(ctx, hipMemcpyAsync(dr, rays, bRays, hipMemcpyHostToDevice, ctx->stream));
-  HIP_TRY(ctx, hipMemcpyAsync(di, importance, bImp, hipMemcpyHostToDevice, ctx->stream));
+  if (importance) HIP_TRY(ctx, hipMemcpyAsync(di, importance, bImp, hipMemcpyHostToDevice, ctx->stream));
   ImportanceBreak ib;
   ib.importance = di;
   ib.threshold = threshold;

@@ -82,6 +82,9 @@ _SIGNATURES = {
     "gpugi_set_pt_schedule": [_P, ctypes.c_int],
     "gpugi_set_rr_start_depth": [_P, ctypes.c_int],
     "gpugi_capture_draw": [_P, ctypes.c_int, ctypes.c_uint32],
+    "gpugi_hierarchy_update_importance": [_P, ctypes.c_int],
+    "gpugi_hierarchy_compute_materials": [_P],
+    "gpugi_hierarchy_read": [_P, _P, ctypes.c_size_t, _P, ctypes.c_size_t, _P, ctypes.c_size_t],
     "gpugi_replay_draw": [_P],
     "gpugi_set_specific_path_length": [_P, ctypes.c_int],
     "gpugi_set_triangle_test": [_P, ctypes.c_int],
@@ -578,16 +581,45 @@ class Context:
         self._call("gpugi_host_free", ctypes.c_void_p(arr.ctypes.data))
 
     def trace_importance_break(self, rays, importance, threshold, ray_threshold, query=CLOSEST_HIT):
-        """TraceRay with TRACERAY_IMPORTANCE_BREAK: importance (num_nodes, 2) float32.
+        """TraceRay with TRACERAY_IMPORTANCE_BREAK: importance (num_nodes, 2) float32, or None
+        for the importance hierarchy_update_importance left on the device.
         Returns (hits, nodes) for closest hit (nodes: NODE_HIT_DTYPE), hits for any hit."""
         rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
-        imp = np.ascontiguousarray(importance, dtype=np.float32).reshape(-1, 2)
+        imp = None if importance is None else np.ascontiguousarray(importance, dtype=np.float32).reshape(-1, 2)
         hits = np.zeros(len(rays), dtype=HIT_DTYPE)
         nodes = np.zeros(len(rays), dtype=NODE_HIT_DTYPE)
         self._call("gpugi_trace_importance_break", int(query), _ptr(rays), _ptr(hits), _ptr(nodes),
-                   ctypes.c_size_t(len(rays)), _ptr(imp), len(imp), ctypes.c_float(threshold),
-                   ctypes.c_float(ray_threshold))
+                   ctypes.c_size_t(len(rays)), _ptr(imp), 0 if imp is None else len(imp),
+                   ctypes.c_float(threshold), ctypes.c_float(ray_threshold))
         return hits if query == ANY_HIT else (hits, nodes)
+
+    def hierarchy_update_importance(self, iterations=10):
+        """HierarchyImportance's acquisition passes and propagation (result stays on the device)."""
+        self._call("gpugi_hierarchy_update_importance", int(iterations))
+
+    def hierarchy_compute_materials(self):
+        self._call("gpugi_hierarchy_compute_materials")
+
+    def _hier_sizes(self):
+        d = self._scene.desc()
+        return d.num_nodes, d.num_leaves * d.triangles_per_leaf
+
+    def hierarchy_read(self):
+        """(importance ((nodes + slots), 2) float32, subtree importance (nodes,) float32)."""
+        nn, ns = self._hier_sizes()
+        imp = np.zeros((nn + ns, 2), np.float32)
+        sub = np.zeros(nn, np.float32)
+        self._call("gpugi_hierarchy_read", _ptr(imp), ctypes.c_size_t(imp.size), _ptr(sub), ctypes.c_size_t(nn),
+                   None, ctypes.c_size_t(0))
+        return imp, sub
+
+    def hierarchy_read_materials(self):
+        """(nodes, 12) uint32: six pairs of packed halves per node."""
+        nn, _ = self._hier_sizes()
+        mats = np.zeros((nn, 12), np.uint32)
+        self._call("gpugi_hierarchy_read", None, ctypes.c_size_t(0), None, ctypes.c_size_t(0), _ptr(mats),
+                   ctypes.c_size_t(mats.size))
+        return mats
 
     def trace_device(self, rays_buf, hits_buf, n, query=CLOSEST_HIT, kernel=KERNEL_AUTO):
         self._call("gpugi_trace", int(query), int(kernel), ctypes.c_void_p(rays_buf.ptr),

@@ -345,7 +345,38 @@ typedef struct gpugi_node_hit {
  * t = newHit + (exit - newHit) * (wanghash(bits(exit)) % 8388593) / 8388593
  * unless a closer triangle follows, an any-hit query reports a hit if the box
  * interval ends within the ray.  Host arrays; nodes (closest hit only) may be NULL; blocks until
- * the results are copied back. */
+ * the results are copied back.  importance == NULL uses the importance the last
+ * gpugi_hierarchy_update_importance computed on the device (num_nodes is then
+ * not read); GPUGI_ERR_STATE if there is none for the current scene. */
+/* The importance update of HierarchyImportance::Draw on its first iteration
+ * (hierarchyimportance.cpp:84-105, 120-152): `iterations` acquisition passes
+ * (acquisition2.comp: one path of up to 8 vertices per pixel of the W/8*8 x
+ * H/8*8 grid with the current camera and light samples, iteration i seeded
+ * by wanghash(i); each ray adds the path's importance, halved at every
+ * vertex, to every node box it enters and triangle it hits) then the
+ * propagation (hierarchypropagation_init.comp, hierarchypropagation_nodes.comp):
+ * per node x = rays through it per iteration, subtree importance = what its
+ * triangles (or children) received, y = (x / subtree)^2 * a third of the box
+ * surface.  The reference uses 10 iterations.  Runs the reference walk and
+ * triangle test; blocks until done.  The result stays on the device for
+ * gpugi_trace_importance_break and gpugi_hierarchy_read.  The GPU sums with
+ * float atomics: the values are exact sums of powers of two up to 2^16 rays
+ * per node, beyond which the order of the additions shows in the last bits. */
+int gpugi_hierarchy_update_importance(gpugi_ctx* ctx, int iterations);
+/* HierarchyImportance::ComputeHierarchyMaterials (hierarchyimportance.cpp:154-178):
+ * per leaf node the area-weighted material of its triangles sampled at their
+ * centroids (sampleleafmaterials.comp), pulled up the tree weighted by the
+ * children's areas (pullmaterials.comp); each node keeps six pairs of packed
+ * halves (hierarchymaterialbuffer.glsl: reflectiveness xy|zw, opacity xy|z
+ * area, diffuse, emissivity, fresnel0 xy|z refraction index, fresnel1).  The
+ * oriented-box layout reads each child's escape from its own record (the
+ * reference's pull pass assumes two texels per node there). */
+int gpugi_hierarchy_compute_materials(gpugi_ctx* ctx);
+/* Copies the hierarchy passes' results to host arrays (any may be NULL):
+ * importance (num_nodes + 2 * num_leaves) pairs {x, y} (nodes, then triangle
+ * slots), subtree num_nodes floats, materials num_nodes x 12 words. */
+int gpugi_hierarchy_read(gpugi_ctx* ctx, float* importance, size_t importance_floats, float* subtree,
+                         size_t subtree_floats, uint32_t* materials, size_t material_words);
 int gpugi_trace_importance_break(gpugi_ctx* ctx, int query, const gpugi_ray* rays, gpugi_hit* hits,
                                  gpugi_node_hit* nodes, size_t n, const float* importance, uint32_t num_nodes,
                                  float threshold, float ray_threshold);

@@ -140,4 +140,18 @@ void or_bpt_eye(const or_scene* s, const or_camera* c, int width, int height, ui
                 int max_path_length, int capacity, const float* cache, uint32_t num_entries, float* accum,
                 int row_begin, int row_end);
 
+/* HierarchyImportance passes (hierarchyimportance.cpp:84-178, the shader/hierarchy passes).
+ * importance: (num_nodes + 2 * num_leaves) x {x, y}; or_count_hits_rays adds
+ * w[i] for ray i (rays: n x 8 floats {o, length, d, pad}); or_slot_hits: per ray
+ * and valid slot 1 if IntersectTriangle accepts it at t <= length; parents: num_nodes. */
+void or_count_hits_rays(const or_scene* s, const float* rays, const float* w, size_t n, float* importance);
+void or_slot_hits(const or_scene* s, const float* rays, size_t n, uint8_t* out);
+void or_hier_parents(const or_scene* s, int32_t* parent);
+void or_hier_acquire(const or_scene* s, const float* lights, uint32_t num_lights, const or_camera* c, int width,
+                     int height, int iteration, float* importance);
+void or_hier_propagate(const or_scene* s, int iterations, float* importance, float* subtree);
+void or_hier_materials(const or_scene* s, uint32_t* out);
+uint16_t or_half_bits(float f);
+float or_half_value(uint16_t h);
+
 #endif

@@ -121,6 +121,15 @@ def _bind(l):
                              ctypes.c_uint32, P, ctypes.c_int, ctypes.c_int]
     l.or_bpt_eye.restype = None
     l.or_set_path_options.argtypes, l.or_set_path_options.restype = [ctypes.c_int, ctypes.c_int], None
+    l.or_count_hits_rays.argtypes, l.or_count_hits_rays.restype = [P, P, P, ctypes.c_size_t, P], None
+    l.or_slot_hits.argtypes, l.or_slot_hits.restype = [P, P, ctypes.c_size_t, P], None
+    l.or_hier_parents.argtypes, l.or_hier_parents.restype = [P, P], None
+    l.or_hier_acquire.argtypes = [P, P, ctypes.c_uint32, P, ctypes.c_int, ctypes.c_int, ctypes.c_int, P]
+    l.or_hier_acquire.restype = None
+    l.or_hier_propagate.argtypes, l.or_hier_propagate.restype = [P, ctypes.c_int, P, P], None
+    l.or_hier_materials.argtypes, l.or_hier_materials.restype = [P, P], None
+    l.or_half_bits.argtypes, l.or_half_bits.restype = [ctypes.c_float], ctypes.c_uint16
+    l.or_half_value.argtypes, l.or_half_value.restype = [ctypes.c_uint16], ctypes.c_float
     l.or_screen_pixel.argtypes = [P, ctypes.c_int, ctypes.c_int, P, ctypes.POINTER(ctypes.c_int),
                                   ctypes.POINTER(ctypes.c_int)]
     l.or_screen_pixel.restype = ctypes.c_int
@@ -386,6 +395,65 @@ def bpt_draw(scene, lights, cam, w, h, frame_seed, max_path_length, capacity, ac
     lib().or_bpt_eye(scene.ptr, ctypes.byref(c), w, h, frame_seed, max_path_length, capacity, _p(cache), n,
                      _p(accum), 0, h)
     return accum, cache, int(n)
+
+
+def count_hits(scene, rays, w):
+    """TraceRayCountHits of every ray (n x 8 {o, length, d, pad}) with weight w[i]:
+    returns the importance array ((nodes + slots) x 2, only .x written)."""
+    r = _rays_f32(rays)
+    w = np.ascontiguousarray(w, dtype=np.float32).reshape(-1)
+    assert len(w) == len(r)
+    out = np.zeros((scene.c.num_nodes + 2 * scene.c.num_leaves, 2), np.float32)
+    lib().or_count_hits_rays(scene.ptr, _p(r), _p(w), len(r), _p(out))
+    return out
+
+
+def slot_hits(scene, rays):
+    """(n, slots) bool: IntersectTriangle accepts the valid slot at t <= the ray's length."""
+    r = _rays_f32(rays)
+    out = np.zeros((len(r), 2 * scene.c.num_leaves), np.uint8)
+    lib().or_slot_hits(scene.ptr, _p(r), len(r), _p(out))
+    return out.astype(bool)
+
+
+def hier_parents(scene):
+    out = np.zeros(scene.c.num_nodes, np.int32)
+    lib().or_hier_parents(scene.ptr, _p(out))
+    return out
+
+
+def hier_acquire(scene, lights, cam, w, h, iteration, importance):
+    """One acquisition2.comp pass (iteration is NumImportanceIterations, 1-based)
+    adding into importance ((nodes + slots) x 2 float32, in place)."""
+    assert importance.dtype == np.float32 and importance.flags.c_contiguous
+    assert importance.shape == (scene.c.num_nodes + 2 * scene.c.num_leaves, 2)
+    lt = _lights(lights)
+    c = camera(cam)
+    lib().or_hier_acquire(scene.ptr, _p(lt), len(lt), ctypes.byref(c), w, h, iteration, _p(importance))
+    return importance
+
+
+def hier_propagate(scene, iterations, importance):
+    """The propagation passes: returns (importance {x / iterations, y}, subtree importance)."""
+    imp = np.ascontiguousarray(importance, dtype=np.float32).copy()
+    sub = np.zeros(scene.c.num_nodes, np.float32)
+    lib().or_hier_propagate(scene.ptr, int(iterations), _p(imp), _p(sub))
+    return imp, sub
+
+
+def hier_materials(scene):
+    """The node materials: (nodes, 12) uint32, six pairs of packed halves per node."""
+    out = np.zeros((scene.c.num_nodes, 12), np.uint32)
+    lib().or_hier_materials(scene.ptr, _p(out))
+    return out
+
+
+def half_bits(x):
+    return int(lib().or_half_bits(float(x)))
+
+
+def half_value(bits):
+    return float(lib().or_half_value(int(bits)))
 
 
 def image_mse(accum, divisor, ref):

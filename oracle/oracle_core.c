@@ -392,6 +392,76 @@ void or_trace_importance(const or_scene* s, int any, const float* rays, int32_t*
   }
 }
 
+/* TraceRayCountHits (hierarchy/counthits.glsl:1-71): the escape walk that adds
+ * w to importance[2 * node] for every node whose box the ray enters at most
+ * rayLength ahead, and to importance[2 * (num_nodes + slot)] for every valid
+ * triangle slot it hits at most rayLength ahead. */
+void or_count_hits(const or_scene* s, vec3 o, vec3 d, float rayLength, float w, float* importance) {
+  int node = 0, leaf = 0, nextIsLeaf = 0;
+  const vec3 inv = v3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+  do {
+    if (!nextIsLeaf) {
+      const float* h = s->obox ? s->obox + (size_t)node * 12 : s->hierarchy + (size_t)node * 8;
+      const uint32_t childCode = bits_of(h[3]);
+      const int escape = (int)bits_of(h[7]);
+      float newHit, exitDist;
+      const int boxHit = s->obox ? or_obox(o, d, h, &newHit, &exitDist)
+                                 : or_box(o, inv, v3(h[0], h[1], h[2]), v3(h[4], h[5], h[6]), &newHit, &exitDist);
+      if (boxHit && newHit <= rayLength) {
+        importance[2 * (size_t)node] += w;
+        const int child = (int)(childCode & 0x7fffffffu);
+        nextIsLeaf = (uint32_t)child != childCode;
+        if (nextIsLeaf) {
+          leaf = 2 * child;
+          node = escape;
+        } else {
+          node = child;
+        }
+      } else {
+        node = escape;
+      }
+    }
+    if (nextIsLeaf) {
+      const int32_t* t = s->triangles + (size_t)leaf * 4;
+      if (t[0] == t[1]) {
+        nextIsLeaf = 0;
+      } else {
+        float newHit;
+        vec3 b, n;
+        if (or_triangle(o, d, pos_of(s, t[0]), pos_of(s, t[1]), pos_of(s, t[2]), &newHit, &b, &n) &&
+            newHit <= rayLength)
+          importance[2 * ((size_t)s->num_nodes + (size_t)leaf)] += w;
+        ++leaf;
+        nextIsLeaf = (leaf % 2) != 0;
+      }
+    }
+  } while (node != 0 || nextIsLeaf);
+}
+
+void or_count_hits_rays(const or_scene* s, const float* rays, const float* w, size_t n, float* importance) {
+  for (size_t i = 0; i < n; ++i) {
+    const float* r = rays + i * 8;
+    or_count_hits(s, v3(r[0], r[1], r[2]), v3(r[4], r[5], r[6]), r[3], w[i], importance);
+  }
+}
+
+/* Brute force over every slot (checker of the count walk): IntersectTriangle
+ * accepts the slot at t <= the ray's length. */
+void or_slot_hits(const or_scene* s, const float* rays, size_t n, uint8_t* out) {
+  const size_t slots = 2 * (size_t)s->num_leaves;
+  for (size_t i = 0; i < n; ++i) {
+    const float* r = rays + i * 8;
+    const vec3 o = v3(r[0], r[1], r[2]), d = v3(r[4], r[5], r[6]);
+    for (size_t k = 0; k < slots; ++k) {
+      const int32_t* t = s->triangles + k * 4;
+      float th;
+      vec3 b, nn;
+      out[i * slots + k] = t[0] != t[1] && or_triangle(o, d, pos_of(s, t[0]), pos_of(s, t[1]), pos_of(s, t[2]), &th,
+                                                        &b, &nn) && th <= r[3];
+    }
+  }
+}
+
 int or_trace_one(const or_scene* s, int any, vec3 o, vec3 d, float* rayLength, int32_t* slot, vec3* bary,
                  vec3* triNormal, int32_t tri[4], uint32_t* nbox, uint32_t* ntri) {
   return trace_one(s, any, 0, o, d, rayLength, slot, bary, triNormal, tri, nbox, ntri);

@@ -116,6 +116,7 @@ typedef struct {
 } or_wtray;
 or_wtray or_wt_setup(vec3 d);
 int or_triangle_wt(vec3 o, const or_wtray* r, vec3 p0, vec3 p1, vec3 p2, float* t, vec3* b);
+void or_count_hits(const or_scene* s, vec3 o, vec3 d, float rayLength, float w, float* importance);
 int or_trace_one(const or_scene* s, int any, vec3 o, vec3 d, float* rayLength, int32_t* slot, vec3* bary,
                  vec3* triNormal, int32_t tri[4], uint32_t* nbox, uint32_t* ntri);
 
