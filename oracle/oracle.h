@@ -135,6 +135,18 @@ void or_bpt_warmup(const or_scene* s, const float* lights, uint32_t num_lights, 
 uint32_t or_bpt_light_pass(const or_scene* s, const float* lights, uint32_t num_lights, const or_camera* c,
                            int width, int height, uint32_t frame_seed, int max_path_length, int capacity,
                            float* accum, float* cache);
+/* Parts of the LT draw and the BPT light pass for splitting light paths over
+ * threads: paths [first, last) into accum; the light-sample splat; BPT paths
+ * [first, last) with initial entries into cache and bounce vertices staged per
+ * path (staged: (last - first) x max_path_length x 16 floats, counts per path). */
+void or_lighttracer_paths(const or_scene* s, const float* lights, uint32_t num_lights, const or_camera* c, int width,
+                          int height, uint32_t frame_seed, int max_path_length, uint32_t first, uint32_t last,
+                          float* accum);
+void or_showlightcaches(const or_scene* s, const float* lights, uint32_t num_lights, const or_camera* c, int width,
+                        int height, float* accum);
+void or_bpt_light_paths(const or_scene* s, const float* lights, uint32_t num_lights, const or_camera* c, int width,
+                        int height, uint32_t frame_seed, int max_path_length, int capacity, uint32_t first,
+                        uint32_t last, float* accum, float* cache, float* staged, uint32_t* counts);
 /* BPT eye pass over rows [row_begin, row_end). */
 void or_bpt_eye(const or_scene* s, const or_camera* c, int width, int height, uint32_t frame_seed,
                 int max_path_length, int capacity, const float* cache, uint32_t num_entries, float* accum,

@@ -122,6 +122,14 @@ def _bind(l):
     l.or_bpt_eye.restype = None
     l.or_set_path_options.argtypes, l.or_set_path_options.restype = [ctypes.c_int, ctypes.c_int], None
     l.or_count_hits_rays.argtypes, l.or_count_hits_rays.restype = [P, P, P, ctypes.c_size_t, P], None
+    l.or_lighttracer_paths.argtypes = [P, P, ctypes.c_uint32, P, ctypes.c_int, ctypes.c_int, ctypes.c_uint32,
+                                       ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32, P]
+    l.or_lighttracer_paths.restype = None
+    l.or_showlightcaches.argtypes = [P, P, ctypes.c_uint32, P, ctypes.c_int, ctypes.c_int, P]
+    l.or_showlightcaches.restype = None
+    l.or_bpt_light_paths.argtypes = [P, P, ctypes.c_uint32, P, ctypes.c_int, ctypes.c_int, ctypes.c_uint32,
+                                     ctypes.c_int, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32, P, P, P, P]
+    l.or_bpt_light_paths.restype = None
     l.or_slot_hits.argtypes, l.or_slot_hits.restype = [P, P, ctypes.c_size_t, P], None
     l.or_hier_parents.argtypes, l.or_hier_parents.restype = [P, P], None
     l.or_hier_acquire.argtypes = [P, P, ctypes.c_uint32, P, ctypes.c_int, ctypes.c_int, ctypes.c_int, P]
@@ -373,6 +381,79 @@ def lighttrace(scene, lights, cam, w, h, frame_seed, max_path_length, accum=None
     c = camera(cam)
     lib().or_lighttracer(scene.ptr, _p(lt), len(lt), ctypes.byref(c), w, h, frame_seed, max_path_length, _p(accum))
     return accum
+
+
+def _split(n, workers):
+    step = max(1, (n + workers - 1) // workers)
+    return [(a, min(n, a + step)) for a in range(0, n, step)]
+
+
+def _rays_per_sample(w, h, num_lights):
+    r = (w * h // num_lights // 64) * 64  # NumRaysPerLightSample (lightpathtracer.cpp:29-31)
+    return max(64, r)
+
+
+def lighttrace_parallel(scene, lights, cam, w, h, frame_seed, max_path_length, workers):
+    """The light tracer draw with its light paths split over `workers` threads, each
+    into its own buffer (float64 sum of the parts; the splat order differs from a
+    sequential draw only in float rounding)."""
+    import concurrent.futures
+    lt = _lights(lights)
+    c = camera(cam)
+    n = len(lt) * _rays_per_sample(w, h, len(lt))
+
+    def part(rng):
+        acc = np.zeros((h, w, 4), np.float32)
+        lib().or_lighttracer_paths(scene.ptr, _p(lt), len(lt), ctypes.byref(c), w, h, frame_seed, max_path_length,
+                                   rng[0], rng[1], _p(acc))
+        return acc.astype(np.float64)
+
+    with concurrent.futures.ThreadPoolExecutor(max_workers=workers) as ex:
+        total = sum(ex.map(part, _split(n, workers)))
+    splat = np.zeros((h, w, 4), np.float32)
+    lib().or_showlightcaches(scene.ptr, _p(lt), len(lt), ctypes.byref(c), w, h, _p(splat))
+    return total + splat
+
+
+def bpt_draw_parallel(scene, lights, cam, w, h, frame_seed, max_path_length, capacity, workers):
+    """One BPT draw (light pass, then eye pass) with the light paths and the eye
+    rows split over `workers` threads.  The cache equals the sequential draw's
+    bit for bit (bounce vertices appended in path order); the image equals it up
+    to the order of the light splats.  Returns (accum float64, cache, NumLightCacheEntries)."""
+    import concurrent.futures
+    lt = _lights(lights)
+    c = camera(cam)
+    R = _rays_per_sample(w, h, len(lt))
+    n = len(lt) * R
+    cache = np.zeros((capacity, 16), np.float32)
+
+    def part(rng):
+        acc = np.zeros((h, w, 4), np.float32)
+        staged = np.zeros((rng[1] - rng[0], max_path_length, 16), np.float32)
+        counts = np.zeros(rng[1] - rng[0], np.uint32)
+        lib().or_bpt_light_paths(scene.ptr, _p(lt), len(lt), ctypes.byref(c), w, h, frame_seed, max_path_length,
+                                 capacity, rng[0], rng[1], _p(acc), _p(cache), _p(staged), _p(counts))
+        return acc.astype(np.float64), staged, counts
+
+    with concurrent.futures.ThreadPoolExecutor(max_workers=workers) as ex:
+        parts = list(ex.map(part, _split(n, workers)))
+    accum = sum(p[0] for p in parts)
+    nxt = n
+    for _, staged, counts in parts:
+        for j, k in enumerate(counts):
+            for b in range(int(k)):
+                if nxt < capacity:
+                    cache[nxt] = staged[j, b]
+                nxt += 1
+    eye = np.zeros((h, w, 4), np.float32)
+
+    def rows(rng):
+        lib().or_bpt_eye(scene.ptr, ctypes.byref(c), w, h, frame_seed, max_path_length, capacity, _p(cache), nxt,
+                         _p(eye), rng[0], rng[1])
+
+    with concurrent.futures.ThreadPoolExecutor(max_workers=workers) as ex:
+        list(ex.map(rows, _split(h, workers)))
+    return accum + eye, cache, nxt
 
 
 def bpt_warmup(scene, lights, cam, w, h, frame_seed, max_path_length):

@@ -256,6 +256,44 @@ uint32_t or_bpt_light_pass(const or_scene* s, const float* lights, uint32_t numL
   return next;
 }
 
+/* Parts of the draws above, so a caller can split the light paths over
+ * threads (each into its own accumulation buffer): the light paths
+ * [first, last) of a light tracer draw, the light-sample splat of a light
+ * tracer draw, and the light paths [first, last) of a BPT light pass with
+ * their initial cache entries (at [first, last), below capacity) and their
+ * stored bounce vertices staged per path (staged: (last - first) x maxLen
+ * entries of 16 floats, counts: stored vertices per path). */
+void or_lighttracer_paths(const or_scene* s, const float* lights, uint32_t numLights, const or_camera* c, int W,
+                          int H, uint32_t frameSeed, int maxLen, uint32_t first, uint32_t last, float* accum) {
+  const int R = rays_per_sample(W, H, numLights);
+  const uint32_t n = numLights * (uint32_t)R;
+  if (last > n) last = n;
+  for (uint32_t gid = first; gid < last; ++gid) light_path(s, lights, c, W, H, frameSeed, maxLen, R, gid, 0, accum, NULL);
+}
+
+void or_showlightcaches(const or_scene* s, const float* lights, uint32_t numLights, const or_camera* c, int W,
+                        int H, float* accum) {
+  for (uint32_t i = 0; i < numLights; ++i) show_light_sample(s, lights, i, c, W, H, accum);
+}
+
+void or_bpt_light_paths(const or_scene* s, const float* lights, uint32_t numLights, const or_camera* c, int W,
+                        int H, uint32_t frameSeed, int maxLen, int capacity, uint32_t first, uint32_t last,
+                        float* accum, float* cache, float* staged, uint32_t* counts) {
+  const int R = rays_per_sample(W, H, numLights);
+  const uint32_t n = numLights * (uint32_t)R;
+  if (last > n) last = n;
+  for (uint32_t gid = first; gid < last; ++gid) {
+    const float* ls = lights + (size_t)(gid / (uint32_t)R) * 8;
+    if ((int)gid < capacity) {
+      vec3 flux = vdiv(vdiv(v3(ls[4], ls[5], ls[6]), OR_PI_2), (float)R);
+      put_entry(cache + (size_t)gid * 16, flux, -1, v3(ls[0], ls[1], ls[2]), ls[3], ls[7], 0.0f, 0.0f,
+                v3(0, 0, 0), 0);
+    }
+    counts[gid - first] = (uint32_t)light_path(s, lights, c, W, H, frameSeed, maxLen, R, gid, 1, accum,
+                                               staged + (size_t)(gid - first) * (size_t)maxLen * 16);
+  }
+}
+
 /* pathtracer_bidir.comp:16-251, no chained MIS; rows [r0, r1) */
 void or_bpt_eye(const or_scene* s, const or_camera* c, int W, int H, uint32_t frameSeed, int maxLen, int capacity,
                 const float* cache, uint32_t numEntries, float* accum, int row_begin, int row_end) {

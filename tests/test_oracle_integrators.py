@@ -161,3 +161,23 @@ def test_bpt_warmup_and_cache_structure(rt, oracle):
     acc2, cache2, n2 = oracle.bpt_draw(sc, ls, cam, w, h, 7, 5, 128 * R + 10)
     assert n2 == entries
     assert np.array_equal(cache2[:128 * R + 10].view(np.uint32), cache[:128 * R + 10].view(np.uint32))
+
+
+def test_parallel_lt_and_bpt_equal_the_sequential_draws(rt, oracle):
+    """The thread-split light tracer and BPT draws (used on the large scenes)
+    give the sequential draws' cache bit for bit and their images up to the
+    order of the light splats."""
+    hs = rt.HostScene.cornell()
+    sc = oracle.Scene(hs.arrays())
+    w = h = 32
+    cam = workloads.cornell_camera(w)
+    ls = rt.LightSampler(hs, 5).generate(128)
+    seq = oracle.lighttrace(sc, ls, cam, w, h, 17, 5)
+    par = oracle.lighttrace_parallel(sc, ls, cam, w, h, 17, 5, workers=4)
+    assert np.allclose(par, seq, rtol=1e-5, atol=1e-7) and seq[..., :3].max() > 0
+    entries, _ = oracle.bpt_warmup(sc, ls, cam, w, h, 17, 5)
+    cap = entries - 50  # some vertices past the capacity
+    acc, cache, n = oracle.bpt_draw(sc, ls, cam, w, h, 17, 5, cap)
+    pacc, pcache, pn = oracle.bpt_draw_parallel(sc, ls, cam, w, h, 17, 5, cap, workers=4)
+    assert pn == n and np.array_equal(pcache.view(np.uint32), cache.view(np.uint32))
+    assert np.allclose(pacc, acc, rtol=1e-5, atol=1e-7)
