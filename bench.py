@@ -54,6 +54,8 @@ def parse():
                    help="write the profiles/ summaries from rocprofv3 output directories and exit")
     p.add_argument("--summarize-counters", nargs="+", metavar="NAME_THEN_DIRS",
                    help="write profiles/<NAME>.json from rocprofv3 --pmc output directories and exit")
+    p.add_argument("--summarize-mlp", nargs="+", metavar="KT_DIR_THEN_PMC_DIRS",
+                   help="write profiles/mlp_kernels.json from a kernel trace and --pmc runs of --mlp-kernels and exit")
     p.add_argument("--sweep", action="store_true",
                    help="print Grays/s of every wide-kernel configuration on the three ray sets and exit")
     p.add_argument("--mlp", action="store_true",
@@ -61,6 +63,10 @@ def parse():
     p.add_argument("--mlp-ref-spp", type=int, default=4096, help="samples per pixel of the plain reference render")
     p.add_argument("--mlp-frames", type=int, default=256, help="largest frame count of the two progressive renders")
     p.add_argument("--mlp-warmup", type=int, default=32, help="cached frames drawn before accumulation starts")
+    p.add_argument("--mlp-kernels", action="store_true",
+                   help="time the cache's training step (16,384 samples) and inference (1920x1080 queries) "
+                        "on device-resident samples and report the BF16 matrix-core roofline")
+    p.add_argument("--mlp-kernel-steps", type=int, default=50, help="timed training steps / inference launches")
     p.add_argument("--batch", action="store_true",
                    help="batched rendering of 16 scenes: per-scene launches vs persistent launch vs hipGraph replay")
     p.add_argument("--batch-frames", type=int, default=10, help="frames timed per batched mode")
@@ -257,6 +263,15 @@ def batch_benchmark(frames):
     b.finalize()
     setup_s = time.perf_counter() - t0
     tuned = b.autotune(2)
+    # the persistent wide frame at each LDS stack depth, the rest as tuned
+    by_depth = {}
+    for depth in (16, 24, 32):
+        b.config(rt.KERNEL_WIDE, tuned["tiles_per_grab"], tuned["grid_multiplier"], depth)
+        b.reset(0)
+        b.run(1, 1)
+        b.reset(0)
+        by_depth[depth] = round(b.run(1, frames) / frames, 3)
+    b.config(tuned["kernel"], tuned["tiles_per_grab"], tuned["grid_multiplier"], tuned["stack_depth"])
     out = {}
     for name, mode in (("separate", 0), ("persistent", 1), ("graph", 2)):
         b.reset(0)
@@ -271,7 +286,7 @@ def batch_benchmark(frames):
         "metric": "batched path tracing Grays/s (16 scenes, one persistent launch)",
         "value": out["graph"]["grays"], "unit": "Grays/s", "higher_is_better": True,
         "speedup_vs_separate": round(out["separate"]["ms_per_frame"] / out["graph"]["ms_per_frame"], 3),
-        "modes": out, "autotuned": tuned, "frames": frames,
+        "modes": out, "autotuned": tuned, "wide_ms_per_frame_by_stack_depth": by_depth, "frames": frames,
         "config": {"workload": "16 heightfields, triangles log-uniform in [10k, 2M], square 256..1024 px, "
                                "4 point lights, path length %d" % workloads.BATCH_PATH_LENGTH,
                    "triangles": [int(s[0].info()["num_triangles"]) for s in scenes],
@@ -501,6 +516,149 @@ def mlp_benchmark(ref_spp, max_frames, warmup=32):
     }
 
 
+BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 matrix-core peak
+# matrix-core FLOPs per sample as the kernels issue them (outputs padded to 16 in the
+# forward pass and to 32 in the backward dX of the output layer):
+# forward 2 * 19456; dW 2 * 19456; dX of layers 4..1: 2 * (64 * 32 + 3 * 64 * 64)
+MLP_FWD_FLOP = 2 * 19456
+MLP_TRAIN_FLOP = MLP_FWD_FLOP + 2 * 19456 + 2 * (64 * 32 + 3 * 64 * 64)
+
+
+def mlp_kernel_benchmark(steps):
+    """The cache's training step (gradient, its reduction and Adam over 16,384
+    samples, the per-step batch of the cached path tracer) and inference over
+    1920 x 1080 queries, on device-resident random features; HIP events over
+    `steps` launches of each.  Matrix-core roofline from the FLOPs the kernels
+    issue per sample."""
+    import ctypes
+    from gpugi_amd import rt
+    rng = np.random.default_rng(5)
+
+    def features(n):
+        f = np.zeros((n, 16), np.float32)
+        f[:, 0:3] = rng.random((n, 3))
+        for a in (3, 6):
+            d = rng.normal(size=(n, 3))
+            f[:, a:a + 3] = d / np.linalg.norm(d, axis=1, keepdims=True)
+        f[:, 9:12] = rng.random((n, 3))
+        f[:, 12] = rng.random(n)
+        return f
+
+    n_train, n_infer = 16384, 1920 * 1080
+    ctx = rt.Context(0)
+    ctx.mlp_init(1)
+    ft, fi = features(n_train), features(n_infer)
+    tt = np.zeros((n_train, 4), np.float32)
+    tt[:, :3] = 0.5 + 0.4 * np.sin(2 * np.pi * ft[:, :3])
+    bft, btt, bfi, bout = ctx.alloc(ft.nbytes), ctx.alloc(tt.nbytes), ctx.alloc(fi.nbytes), ctx.alloc(n_infer * 16)
+    bft.upload(ft)
+    btt.upload(tt)
+    bfi.upload(fi)
+    vp = ctypes.c_void_p
+
+    def train():
+        ctx._call("gpugi_mlp_train_step", vp(bft.ptr), vp(btt.ptr), ctypes.c_size_t(n_train), ctypes.c_float(1e-3),
+                  None, 1)
+
+    def infer():
+        ctx._call("gpugi_mlp_infer", vp(bfi.ptr), vp(bout.ptr), ctypes.c_size_t(n_infer), 1)
+
+    out = {}
+    for name, fn, n, flop in (("train_step", train, n_train, MLP_TRAIN_FLOP), ("inference", infer, n_infer,
+                                                                                 MLP_FWD_FLOP)):
+        for _ in range(3):
+            fn()
+        ctx.sync()
+        ctx.timer_begin()
+        for _ in range(steps):
+            fn()
+        ms = ctx.timer_end() / steps
+        tflops = n * flop / (ms * 1e-3) / 1e12
+        out[name] = {"samples": n, "ms": round(ms, 4), "flop_per_sample": flop, "tflops": round(tflops, 2),
+                     "frac_bf16_peak": round(tflops / BF16_PEAK_TFLOPS, 4)}
+    ctx.close()
+    return {"metric": "radiance-cache MLP kernels, BF16 matrix cores", "unit": "TFLOP/s",
+            "value": out["inference"]["tflops"], "higher_is_better": True, "peak_tflops": BF16_PEAK_TFLOPS,
+            "steps": steps, "kernels": out,
+            "config": {"workload": "96-64x4-3 MLP; training step = gradient + reduction + Adam over 16,384 "
+                                   "samples; inference over 1920x1080 queries; random features on the device"}}
+
+
+MLP_KERNELS = ("k_mlp_grad", "k_mlp_reduce", "k_mlp_infer")
+
+
+def summarize_mlp(kt_dir, *pmc_dirs):
+    """Writes profiles/mlp_kernels.json: per-kernel average durations (kernel trace) and per-launch
+    counters (--pmc passes) of the MLP kernels in a --mlp-kernels run, with the matrix-core roofline:
+    FLOPs per launch from the model (checked against SQ_INSTS_VALU_MFMA_MOPS_BF16 x 512) over the
+    average duration, against the BF16 dense peak, and the MFMA busy fraction of all 1024 SIMDs."""
+    def short(name):
+        for k in MLP_KERNELS:
+            for tail in ("(", "<"):
+                if "::" + k + tail in name or name.startswith(k + tail):
+                    return k
+        return None
+
+    dur = {}
+    for r in _csv_rows(kt_dir, "_kernel_stats.csv"):
+        k = short(r["Name"])
+        if k:
+            dur[k] = {"calls": int(r["Calls"]), "avg_us": float(r["AverageNs"]) / 1e3,
+                      "min_us": float(r["MinNs"]) / 1e3, "max_us": float(r["MaxNs"]) / 1e3}
+    counters = {}
+    for d in pmc_dirs:
+        acc = {}
+        for r in _csv_rows(d, "_counter_collection.csv"):
+            k = short(r["Kernel_Name"])
+            if k:
+                acc.setdefault(k, {}).setdefault(r["Counter_Name"], []).append(float(r["Counter_Value"]))
+        for k, cs in acc.items():
+            for c, v in cs.items():
+                counters.setdefault(k, {})[c] = sum(v) / len(v)
+    model = {"k_mlp_grad": 16384 * (MLP_TRAIN_FLOP), "k_mlp_infer": 1920 * 1080 * MLP_FWD_FLOP}
+    kernels = {}
+    for k in MLP_KERNELS:
+        e = {"duration": dur.get(k), "per_launch": counters.get(k, {})}
+        c = e["per_launch"]
+        if k in model and e["duration"]:
+            flop = model[k]
+            tflops = flop / (e["duration"]["avg_us"] * 1e-6) / 1e12
+            e["roofline"] = {"bound": "mfma", "flop_per_launch": flop, "achieved_tflops": round(tflops, 2),
+                             "peak_tflops": BF16_PEAK_TFLOPS, "frac": round(tflops / BF16_PEAK_TFLOPS, 4)}
+            if "SQ_INSTS_VALU_MFMA_MOPS_BF16" in c:
+                e["roofline"]["counted_flop_per_launch"] = c["SQ_INSTS_VALU_MFMA_MOPS_BF16"] * 512.0
+        if "SQ_VALU_MFMA_BUSY_CYCLES" in c and "GRBM_GUI_ACTIVE" in c:
+            # GRBM_GUI_ACTIVE sums the launch's busy cycles over the 8 XCDs; 1024 SIMDs
+            e["mfma_busy_frac"] = c["SQ_VALU_MFMA_BUSY_CYCLES"] / (c["GRBM_GUI_ACTIVE"] / 8.0 * 1024.0)
+        if "SQ_LDS_BANK_CONFLICT" in c and "SQ_LDS_IDX_ACTIVE" in c and c["SQ_LDS_IDX_ACTIVE"] > 0:
+            e["lds_bank_conflict_frac"] = c["SQ_LDS_BANK_CONFLICT"] / c["SQ_LDS_IDX_ACTIVE"]
+        if "SQ_WAVES" in c and c["SQ_WAVES"] > 0:
+            for key in ("SQ_INSTS_MFMA", "SQ_INSTS_LDS", "SQ_INSTS_VALU", "SQ_INSTS_SALU", "SQ_INSTS_VMEM_RD",
+                        "SQ_INSTS_VMEM_WR"):
+                if key in c:
+                    e.setdefault("per_wave", {})[key] = c[key] / c["SQ_WAVES"]
+            if "SQ_WAVE_CYCLES" in c:
+                e.setdefault("per_wave", {})["SQ_WAVE_CYCLES"] = c["SQ_WAVE_CYCLES"] / c["SQ_WAVES"]
+            if "SQ_WAIT_INST_LDS" in c:
+                e.setdefault("per_wave", {})["SQ_WAIT_INST_LDS"] = c["SQ_WAIT_INST_LDS"] / c["SQ_WAVES"]
+            if "SQ_WAIT_INST_ANY" in c:
+                e.setdefault("per_wave", {})["SQ_WAIT_INST_ANY"] = c["SQ_WAIT_INST_ANY"] / c["SQ_WAVES"]
+        kernels[k] = e
+    out = {"workload": "bench.py --mlp-kernels: training step over 16,384 samples (k_mlp_grad, k_mlp_reduce, "
+                       "k_mlp_adam), inference over 1920x1080 queries (k_mlp_infer)",
+           "commands": ["cd /tmp && export TMPDIR=/tmp && timeout -k 10 300 rocprofv3 --kernel-trace --stats -d <out>/kt "
+                        "-o kt --output-format csv -- python3 <repo>/bench.py --mlp-kernels --mlp-kernel-steps 20",
+                        "cd /tmp && export TMPDIR=/tmp && timeout -s KILL 120 rocprofv3 --pmc <counters of one pass> "
+                        "-d <out>/<pass> -o <pass> --output-format csv -- python3 <repo>/bench.py --mlp-kernels "
+                        "--mlp-kernel-steps 5",
+                        "python3 <repo>/bench.py --summarize-mlp <out>/kt <out>/<pass> ..."],
+           "kernels": kernels}
+    with open(os.path.join(ROOT, "profiles", "mlp_kernels.json"), "w") as f:
+        json.dump(out, f, indent=1)
+        f.write("\n")
+    print(json.dumps(out))
+
+
 def host_threads():
     """Host cores this process may use: OMP_NUM_THREADS when set (the share a shared
     host grants the run), else the CPUs in the affinity mask."""
@@ -582,10 +740,16 @@ def main():
     if args.summarize_counters:
         summarize_counters(*args.summarize_counters)
         return
+    if args.summarize_mlp:
+        summarize_mlp(*args.summarize_mlp)
+        return
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; --gpus %d is not supported (ray sets and pixels would "
               "shard across GPUs with no communication during traversal)" % args.gpus, file=sys.stderr)
         sys.exit(2)
+    if args.mlp_kernels:
+        print(json.dumps(mlp_kernel_benchmark(args.mlp_kernel_steps)))
+        return
     if args.mlp:
         print(json.dumps(mlp_benchmark(args.mlp_ref_spp, args.mlp_frames, args.mlp_warmup)))
         return

@@ -59,7 +59,11 @@ __global__ void k_fill_f4(float4* p, size_t n) {
   if (i < n) p[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
-template <bool WIDE>
+// LDS stack depths of the wide walk the batch kernels are built for; the
+// spill slices are sized for the smallest.
+constexpr int kBatchStacks[3] = {16, 24, 32};
+
+template <bool WIDE, int S>
 __device__ __forceinline__ void render_tile(const BatchSlot& sl, uint32_t local, uint32_t seed, uint2* stk,
                                             uint2* spill, BatchCounters* c) {
   DrawParams p = sl.p;
@@ -68,15 +72,15 @@ __device__ __forceinline__ void render_tile(const BatchSlot& sl, uint32_t local,
   const int gx = (int)(local % sl.tilesX) * kTile + (tid % kTile);
   const int gy = (int)(local / sl.tilesX) * kTile + (tid / kTile);
   TraceCtx ctx{&p, stk, tid, spill};
-  pt_pixel<WIDE>(p, ctx, gx, gy, &c->rays);
+  pt_pixel<WIDE, S>(p, ctx, gx, gy, &c->rays);
 }
 
-template <bool WIDE>
+template <bool WIDE, int S>
 __global__ __launch_bounds__(kTileThreads) void k_batch_persistent(const BatchSlot* __restrict__ slots,
                                                                    uint32_t numScenes, uint32_t totalTiles,
                                                                    BatchCounters* c, uint2* spillBase,
                                                                    uint32_t spillDepth, uint32_t grab) {
-  __shared__ uint2 stk[WIDE ? kWideLdsStack * kTileThreads : 1];
+  __shared__ uint2 stk[WIDE ? S * kTileThreads : 1];
   __shared__ uint32_t sBase;
   const int tid = threadIdx.x;
   uint2* spill = WIDE ? spillBase + ((size_t)blockIdx.x * kTileThreads + tid) * spillDepth : nullptr;
@@ -91,18 +95,36 @@ __global__ __launch_bounds__(kTileThreads) void k_batch_persistent(const BatchSl
     for (uint32_t t = base; t < end; ++t) {
       uint32_t s = 0;
       while (s + 1 < numScenes && slots[s + 1].tileBase <= t) ++s;
-      render_tile<WIDE>(slots[s], t - slots[s].tileBase, seed, stk, spill, c);
+      render_tile<WIDE, S>(slots[s], t - slots[s].tileBase, seed, stk, spill, c);
     }
   }
 }
 
 // One scene, one tile per workgroup (the "separate launches" baseline).
-template <bool WIDE>
+template <bool WIDE, int S>
 __global__ __launch_bounds__(kTileThreads) void k_batch_scene(const BatchSlot* __restrict__ slot, BatchCounters* c,
                                                               uint2* spillBase, uint32_t spillDepth) {
-  __shared__ uint2 stk[WIDE ? kWideLdsStack * kTileThreads : 1];
+  __shared__ uint2 stk[WIDE ? S * kTileThreads : 1];
   uint2* spill = WIDE ? spillBase + ((size_t)blockIdx.x * kTileThreads + threadIdx.x) * spillDepth : nullptr;
-  render_tile<WIDE>(*slot, blockIdx.x, c->seed, stk, spill, c);
+  render_tile<WIDE, S>(*slot, blockIdx.x, c->seed, stk, spill, c);
+}
+
+// The launch of one frame's tiles for one traversal and stack depth.
+template <bool WIDE, int S>
+void LaunchTiles(bool persistent, dim3 persistentGrid, const std::vector<BatchSlot>& hSlots, BatchSlot* dSlots,
+                 uint32_t totalTiles, BatchCounters* c, uint2* spill, uint32_t spillDepth, uint32_t grab,
+                 hipStream_t stream) {
+  if (persistent) {
+    hipLaunchKernelGGL((k_batch_persistent<WIDE, S>), persistentGrid, dim3(kTileThreads), 0, stream, dSlots,
+                       (uint32_t)hSlots.size(), totalTiles, c, spill, spillDepth, grab);
+    return;
+  }
+  for (size_t i = 0; i < hSlots.size(); ++i) {
+    const dim3 grid(hSlots[i].tilesX * hSlots[i].tilesY);
+    if (grid.x == 0) continue;
+    hipLaunchKernelGGL((k_batch_scene<WIDE, S>), grid, dim3(kTileThreads), 0, stream, dSlots + i, c, spill,
+                       spillDepth);
+  }
 }
 
 }  // namespace
@@ -141,7 +163,9 @@ struct gpugi_batch {
   int kernel = GPUGI_KERNEL_WIDE;
   int grab = 2;
   int gridMul = 2;
-  int residentBlocks = 0;
+  int stackDepth = 16;     // one of kBatchStacks
+  int residentRef = 1;     // resident blocks per CU of the reference-walk kernel
+  int residentWide[3] = {1, 1, 1};  // ... of the wide kernel per kBatchStacks entry
   hipGraph_t graph = nullptr;
   hipGraphExec_t graphExec = nullptr;
 };
@@ -168,8 +192,15 @@ void DropGraph(gpugi_batch* b) {
   b->graph = nullptr;
 }
 
+int StackIndex(int depth) {
+  for (int i = 0; i < 3; ++i)
+    if (kBatchStacks[i] == depth) return i;
+  return -1;
+}
+
 int PersistentGrid(gpugi_batch* b) {
-  return std::max(1, b->residentBlocks * b->numCUs * b->gridMul / 4);
+  const int resident = b->kernel == GPUGI_KERNEL_WIDE ? b->residentWide[StackIndex(b->stackDepth)] : b->residentRef;
+  return std::max(1, resident * b->numCUs * b->gridMul / 4);
 }
 
 // Spill slices for every thread any launch of the current configuration uses.
@@ -188,29 +219,15 @@ int EnsureSpill(gpugi_batch* b) {
 
 int LaunchFrame(gpugi_batch* b, bool persistent) {
   hipLaunchKernelGGL(k_batch_begin, dim3(1), dim3(1), 0, b->stream, b->dCounters);
-  const bool wide = b->kernel == GPUGI_KERNEL_WIDE;
-  if (persistent) {
-    const dim3 grid(PersistentGrid(b));
-    if (wide)
-      hipLaunchKernelGGL(k_batch_persistent<true>, grid, dim3(kTileThreads), 0, b->stream, b->dSlots,
-                         (uint32_t)b->hSlots.size(), b->totalTiles, b->dCounters, b->dSpill, b->spillDepth,
-                         (uint32_t)b->grab);
-    else
-      hipLaunchKernelGGL(k_batch_persistent<false>, grid, dim3(kTileThreads), 0, b->stream, b->dSlots,
-                         (uint32_t)b->hSlots.size(), b->totalTiles, b->dCounters, b->dSpill, b->spillDepth,
-                         (uint32_t)b->grab);
-  } else {
-    for (size_t i = 0; i < b->hSlots.size(); ++i) {
-      const dim3 grid(b->hSlots[i].tilesX * b->hSlots[i].tilesY);
-      if (grid.x == 0) continue;
-      if (wide)
-        hipLaunchKernelGGL(k_batch_scene<true>, grid, dim3(kTileThreads), 0, b->stream, b->dSlots + i, b->dCounters,
-                           b->dSpill, b->spillDepth);
-      else
-        hipLaunchKernelGGL(k_batch_scene<false>, grid, dim3(kTileThreads), 0, b->stream, b->dSlots + i,
-                           b->dCounters, b->dSpill, b->spillDepth);
-    }
-  }
+  const dim3 grid(PersistentGrid(b));
+  auto launch = [&](auto fn) {
+    fn(persistent, grid, b->hSlots, b->dSlots, b->totalTiles, b->dCounters, b->dSpill, b->spillDepth,
+       (uint32_t)b->grab, b->stream);
+  };
+  if (b->kernel != GPUGI_KERNEL_WIDE) launch(LaunchTiles<false, 16>);
+  else if (b->stackDepth == 16) launch(LaunchTiles<true, 16>);
+  else if (b->stackDepth == 24) launch(LaunchTiles<true, 24>);
+  else launch(LaunchTiles<true, 32>);
   BHIP(b, hipGetLastError());
   return GPUGI_OK;
 }
@@ -332,7 +349,7 @@ int gpugi_batch_finalize(gpugi_batch* b) {
   uint32_t tiles = 0;
   b->spillDepth = 0;
   for (const BatchSceneHost& s : b->scenes)
-    b->spillDepth = std::max(b->spillDepth, s.wide.maxStack > (uint32_t)kWideLdsStack ? s.wide.maxStack - kWideLdsStack : 0u);
+    b->spillDepth = std::max(b->spillDepth, s.wide.maxStack > (uint32_t)kBatchStacks[0] ? s.wide.maxStack - kBatchStacks[0] : 0u);
   for (const BatchSceneHost& s : b->scenes) {
     const void* src[8] = {s.hier.data(), s.tris.data(), s.pos.data(), s.vinfo.data(), s.mats.data(), s.lights.data(),
                           s.wide.nodes.data(), s.wide.tris.data()};
@@ -378,22 +395,31 @@ int gpugi_batch_finalize(gpugi_batch* b) {
   for (size_t i = 0; i < b->hSlots.size(); ++i) b->hSlots[i].p.accum = b->dAccum + b->accumOffset[i];
   BHIP(b, hipMalloc((void**)&b->dSlots, b->hSlots.size() * sizeof(BatchSlot)));
   BHIP(b, hipMemcpy(b->dSlots, b->hSlots.data(), b->hSlots.size() * sizeof(BatchSlot), hipMemcpyHostToDevice));
-  int r0 = 0, r1 = 0;
-  BHIP(b, hipOccupancyMaxActiveBlocksPerMultiprocessor(&r0, (const void*)&k_batch_persistent<true>, kTileThreads, 0));
-  BHIP(b, hipOccupancyMaxActiveBlocksPerMultiprocessor(&r1, (const void*)&k_batch_persistent<false>, kTileThreads, 0));
-  b->residentBlocks = std::max(1, std::min(r0, r1));
+  const void* wideKernels[3] = {(const void*)&k_batch_persistent<true, 16>, (const void*)&k_batch_persistent<true, 24>,
+                                (const void*)&k_batch_persistent<true, 32>};
+  for (int i = 0; i < 3; ++i) {
+    int r = 0;
+    BHIP(b, hipOccupancyMaxActiveBlocksPerMultiprocessor(&r, wideKernels[i], kTileThreads, 0));
+    b->residentWide[i] = std::max(1, r);
+  }
+  int r = 0;
+  BHIP(b, hipOccupancyMaxActiveBlocksPerMultiprocessor(&r, (const void*)&k_batch_persistent<false, 16>, kTileThreads, 0));
+  b->residentRef = std::max(1, r);
   b->finalized = true;
   return gpugi_batch_reset(b, 0);
 }
 
-int gpugi_batch_config(gpugi_batch* b, int kernel, int tiles_per_grab, int grid_multiplier) {
+int gpugi_batch_config(gpugi_batch* b, int kernel, int tiles_per_grab, int grid_multiplier, int stack_depth) {
   if (int rc = Ready(b, "gpugi_batch_config")) return rc;
   if ((kernel != GPUGI_KERNEL_WIDE && kernel != GPUGI_KERNEL_REFERENCE) || tiles_per_grab < 1 ||
-      tiles_per_grab > 64 || grid_multiplier < 1 || grid_multiplier > 16)
-    return BFail(b, GPUGI_ERR_ARG, "gpugi_batch_config: kernel REFERENCE or WIDE, grab in [1, 64], multiplier in [1, 16]");
+      tiles_per_grab > 64 || grid_multiplier < 1 || grid_multiplier > 16 || StackIndex(stack_depth) < 0)
+    return BFail(b, GPUGI_ERR_ARG,
+                 "gpugi_batch_config: kernel REFERENCE or WIDE, grab in [1, 64], multiplier in [1, 16], "
+                 "stack depth 16, 24 or 32");
   b->kernel = kernel;
   b->grab = tiles_per_grab;
   b->gridMul = grid_multiplier;
+  b->stackDepth = stack_depth;
   DropGraph(b);
   return EnsureSpill(b);
 }
@@ -480,29 +506,34 @@ int gpugi_batch_rays(gpugi_batch* b, uint64_t* rays) {
 }
 
 int gpugi_batch_autotune(gpugi_batch* b, int frames, int* kernel, int* tiles_per_grab, int* grid_multiplier,
-                         float* best_ms) {
+                         int* stack_depth, float* best_ms) {
   if (int rc = Ready(b, "gpugi_batch_autotune")) return rc;
   if (frames < 1) return BFail(b, GPUGI_ERR_ARG, "gpugi_batch_autotune: frames >= 1");
   float best = 1e30f;
-  int bk = b->kernel, bg = b->grab, bm = b->gridMul;
+  int bk = b->kernel, bg = b->grab, bm = b->gridMul, bs = b->stackDepth;
   for (int k : {GPUGI_KERNEL_REFERENCE, GPUGI_KERNEL_WIDE})
-    for (int g : {1, 2, 4, 8})
-      for (int m : {1, 2, 4}) {
-        if (int rc = gpugi_batch_config(b, k, g, m)) return rc;
-        float ms = 0.0f;
-        if (int rc = gpugi_batch_run(b, 1, 1, &ms)) return rc;  // warm-up
-        if (int rc = gpugi_batch_run(b, 1, frames, &ms)) return rc;
-        if (ms < best) {
-          best = ms;
-          bk = k;
-          bg = g;
-          bm = m;
+    for (int st : kBatchStacks) {
+      if (k == GPUGI_KERNEL_REFERENCE && st != kBatchStacks[0]) continue;  // the escape walk keeps no stack
+      for (int g : {1, 2, 4, 8})
+        for (int m : {1, 2, 4}) {
+          if (int rc = gpugi_batch_config(b, k, g, m, st)) return rc;
+          float ms = 0.0f;
+          if (int rc = gpugi_batch_run(b, 1, 1, &ms)) return rc;  // warm-up
+          if (int rc = gpugi_batch_run(b, 1, frames, &ms)) return rc;
+          if (ms < best) {
+            best = ms;
+            bk = k;
+            bg = g;
+            bm = m;
+            bs = st;
+          }
         }
-      }
-  if (int rc = gpugi_batch_config(b, bk, bg, bm)) return rc;
+    }
+  if (int rc = gpugi_batch_config(b, bk, bg, bm, bs)) return rc;
   if (kernel) *kernel = bk;
   if (tiles_per_grab) *tiles_per_grab = bg;
   if (grid_multiplier) *grid_multiplier = bm;
+  if (stack_depth) *stack_depth = bs;
   if (best_ms) *best_ms = best / frames;
   return GPUGI_OK;
 }

@@ -309,7 +309,7 @@ void gpugi_destroy(gpugi_ctx* ctx) {
   FreeDev(ctx->dTexDesc);
   FreeDev(ctx->dMatTex);
   for (void** p : {(void**)&ctx->mlp.w, (void**)&ctx->mlp.m, (void**)&ctx->mlp.v, (void**)&ctx->mlp.wb,
-                   (void**)&ctx->mlp.wtb, (void**)&ctx->mlp.partials, (void**)&ctx->mlp.grad,
+                   (void**)&ctx->mlp.wtb, (void**)&ctx->mlp.wbi, (void**)&ctx->mlp.partials, (void**)&ctx->mlp.grad,
                    (void**)&ctx->mlp.loss, (void**)&ctx->dMlpLoss, &ctx->dMlpIo, &ctx->dCache})
     FreeDev(*p);
   for (void* p : ctx->userAllocs) hipFree(p);
@@ -643,10 +643,11 @@ int DrawCached(gpugi_ctx* ctx, uint32_t frameSeed) {
     if (int rc = MlpEnsureTiles(ctx, MlpTiles(S / (uint32_t)steps))) return rc;
     for (int s = 0; s < steps; ++s) {
       const MlpBatch batch{dTotal, S, (uint32_t)s, (uint32_t)steps};
-      HIP_TRY(ctx, LaunchMlpGradient(ctx->mlp, feat, targets, batch, ctx->dMlpLoss, trainStream));
+      HIP_TRY(ctx, LaunchMlpGradientTiles(ctx->mlp, feat, targets, batch, trainStream));
+      // the first weight update waits for the frame's query inference, which reads the weights
       if (s == 0) HIP_TRY(ctx, hipStreamWaitEvent(ctx->side, ctx->evInfer, 0));
       AdamParams ap{ctx->cacheLr, 0.9f, 0.99f, 1e-8f, ++ctx->mlpStep};
-      HIP_TRY(ctx, LaunchMlpAdam(ctx->mlp, ap, batch, trainStream));
+      HIP_TRY(ctx, LaunchMlpReduce(ctx->mlp, batch, ctx->dMlpLoss, &ap, trainStream));
     }
   }
   HIP_TRY(ctx, hipEventRecord(ctx->evJoin, ctx->side));
@@ -1620,6 +1621,8 @@ int gpugi_mlp_init(gpugi_ctx* ctx, uint64_t seed) {
     HIP_TRY(ctx, hipMalloc((void**)&ctx->mlp.wb, kMlpParams * sizeof(uint16_t)));
     HIP_TRY(ctx, hipMalloc((void**)&ctx->mlp.wtb, kMlpWTSize * sizeof(uint16_t)));
     HIP_TRY(ctx, hipMemset(ctx->mlp.wtb, 0, kMlpWTSize * sizeof(uint16_t)));
+    HIP_TRY(ctx, hipMalloc((void**)&ctx->mlp.wbi, kMlpInferSize * sizeof(uint16_t)));
+    HIP_TRY(ctx, hipMemset(ctx->mlp.wbi, 0, kMlpInferSize * sizeof(uint16_t)));
     HIP_TRY(ctx, hipMalloc((void**)&ctx->dMlpLoss, 256));
     ctx->hasMlp = true;
   }
@@ -1683,8 +1686,10 @@ int gpugi_mlp_infer(gpugi_ctx* ctx, const float* features, float* radiance, size
 namespace {
 
 // Stages features/targets for host pointers and runs the gradient kernels.
+// The gradient of n samples into ctx->mlp.grad and its loss; with `adam`
+// the Adam step follows in the same reduction launch.
 int MlpGradientCommon(gpugi_ctx* ctx, const char* fn, const float* features, const float* targets, size_t n,
-                      int on_device) {
+                      int on_device, const AdamParams* adam = nullptr) {
   if (n == 0 || !features || !targets) return Fail(ctx, GPUGI_ERR_ARG, std::string(fn) + ": need samples");
   if (int rc = MlpEnsureTiles(ctx, MlpTiles(n))) return rc;
   const float* df = features;
@@ -1697,7 +1702,9 @@ int MlpGradientCommon(gpugi_ctx* ctx, const char* fn, const float* features, con
     HIP_TRY(ctx, hipMemcpyAsync((void*)df, features, fb, hipMemcpyHostToDevice, ctx->stream));
     HIP_TRY(ctx, hipMemcpyAsync((void*)dt, targets, tb, hipMemcpyHostToDevice, ctx->stream));
   }
-  HIP_TRY(ctx, LaunchMlpGradient(ctx->mlp, df, dt, MlpBatch{nullptr, (uint32_t)n, 0, 1}, ctx->dMlpLoss, ctx->stream));
+  const MlpBatch batch{nullptr, (uint32_t)n, 0, 1};
+  HIP_TRY(ctx, LaunchMlpGradientTiles(ctx->mlp, df, dt, batch, ctx->stream));
+  HIP_TRY(ctx, LaunchMlpReduce(ctx->mlp, batch, ctx->dMlpLoss, adam, ctx->stream));
   return GPUGI_OK;
 }
 
@@ -1720,9 +1727,9 @@ int gpugi_mlp_train_step(gpugi_ctx* ctx, const float* features, const float* tar
                          float learning_rate, float* loss, int on_device) {
   if (int rc = MlpCheck(ctx, "gpugi_mlp_train_step", n)) return rc;
   if (!(learning_rate > 0.0f)) return Fail(ctx, GPUGI_ERR_ARG, "gpugi_mlp_train_step: learning rate must be > 0");
-  if (int rc = MlpGradientCommon(ctx, "gpugi_mlp_train_step", features, targets, n, on_device)) return rc;
-  AdamParams p{learning_rate, 0.9f, 0.99f, 1e-8f, ++ctx->mlpStep};
-  HIP_TRY(ctx, LaunchMlpAdam(ctx->mlp, p, MlpBatch{nullptr, (uint32_t)n, 0, 1}, ctx->stream));
+  AdamParams p{learning_rate, 0.9f, 0.99f, 1e-8f, ctx->mlpStep + 1};
+  if (int rc = MlpGradientCommon(ctx, "gpugi_mlp_train_step", features, targets, n, on_device, &p)) return rc;
+  ++ctx->mlpStep;
   if (loss) {
     HIP_TRY(ctx, hipMemcpyAsync(loss, ctx->dMlpLoss, sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));

@@ -113,16 +113,18 @@ __device__ __forceinline__ ClosestHit closest_from_record(const DrawParams& p, i
 // With DrawParams::watertight the watertight triangle test replaces the
 // reference one in both traversals (the geometric normal is then rebuilt
 // from the vertices with the reference formula cross(p0 - p2, p1 - p0)).
-template <bool WIDE>
+// S: LDS stack entries per lane of the wide walk (the block's stack array
+// holds S x kTileThreads entries).
+template <bool WIDE, int S = kWideLdsStack>
 __device__ __forceinline__ ClosestHit trace_closest(const TraceCtx& c, gg::V3 o, gg::V3 d) {
   ClosestHit h;
   if (c.count) ++*c.count;
   if (WIDE) {
     WideResult r = c.p->watertight
-                       ? trace_wide_ray<false, kTileThreads, kWideLdsStack, true>(c.p->wide, o, d, gg::kRayMax,
-                                                                                  c.stk, c.tid, c.spill)
-                       : trace_wide_ray<false, kTileThreads, kWideLdsStack, false>(c.p->wide, o, d, gg::kRayMax,
-                                                                                   c.stk, c.tid, c.spill);
+                       ? trace_wide_ray<false, kTileThreads, S, true>(c.p->wide, o, d, gg::kRayMax, c.stk, c.tid,
+                                                                      c.spill)
+                       : trace_wide_ray<false, kTileThreads, S, false>(c.p->wide, o, d, gg::kRayMax, c.stk, c.tid,
+                                                                       c.spill);
     h = closest_from_record(*c.p, r.slot, r.t, r.by, r.bz);
   } else {
     float rayLength = gg::kRayMax;
@@ -147,15 +149,13 @@ __device__ __forceinline__ ClosestHit trace_closest(const TraceCtx& c, gg::V3 o,
   return h;
 }
 
-template <bool WIDE>
+template <bool WIDE, int S = kWideLdsStack>
 __device__ __forceinline__ bool trace_any(const TraceCtx& c, gg::V3 o, gg::V3 d, float len) {
   if (c.count) ++*c.count;
   if (WIDE) {
     WideResult r = c.p->watertight
-                       ? trace_wide_ray<true, kTileThreads, kWideLdsStack, true>(c.p->wide, o, d, len, c.stk,
-                                                                                 c.tid, c.spill)
-                       : trace_wide_ray<true, kTileThreads, kWideLdsStack, false>(c.p->wide, o, d, len, c.stk,
-                                                                                  c.tid, c.spill);
+                       ? trace_wide_ray<true, kTileThreads, S, true>(c.p->wide, o, d, len, c.stk, c.tid, c.spill)
+                       : trace_wide_ray<true, kTileThreads, S, false>(c.p->wide, o, d, len, c.stk, c.tid, c.spill);
     return r.any;
   }
   float rayLength = len;
@@ -217,11 +217,11 @@ __device__ __forceinline__ NeeSample nee_sample(const DrawParams& p, gg::V3 pos,
 }
 
 // EstimateDirectLight with the shadow ray traced here.
-template <bool WIDE>
+template <bool WIDE, int S = kWideLdsStack>
 __device__ __forceinline__ gg::V3 estimate_direct_light(const TraceCtx& c, gg::V3 pos, gg::V3 normal, int idx,
                                                         gg::V3 viewDir, const MaterialData& m, bool sampleAreaLights) {
   const NeeSample s = nee_sample(*c.p, pos, normal, idx, viewDir, m, sampleAreaLights);
-  if (s.test && !trace_any<WIDE>(c, s.o, s.dir, s.len)) return s.value;
+  if (s.test && !trace_any<WIDE, S>(c, s.o, s.dir, s.len)) return s.value;
   return gg::v3s(0.0f);
 }
 
@@ -253,11 +253,11 @@ __device__ __forceinline__ void pt_begin(const DrawParams& p, PtPath& s, int gx,
 
 // One bounce of the path (the loop body of pathtracer.comp).  Returns true
 // when the path has ended; s.color is then final.
-template <bool WIDE>
+template <bool WIDE, int S = kWideLdsStack>
 __device__ __forceinline__ bool pt_bounce(const DrawParams& p, const TraceCtx& c, PtPath& s) {
   if (s.bounce >= p.maxPathLength) return true;
   const float lightSampleWeight = (float)p.numLights / 1.0f;
-  const ClosestHit h = trace_closest<WIDE>(c, s.o, s.d);
+  const ClosestHit h = trace_closest<WIDE, S>(c, s.o, s.d);
   if (!h.hit) {
     // environment on a miss (pathtracer.comp:56-58); black by default (renderersystem.cpp:39-47)
     s.color = s.color + s.throughput * (p.env ? env_radiance(p.env, p.envSize, s.d) : gg::v3s(0.0f));
@@ -273,7 +273,7 @@ __device__ __forceinline__ bool pt_bounce(const DrawParams& p, const TraceCtx& c
   s.didNEE = m.refl[3] < 500.0f;
   if (spl < 0 || s.bounce + 1 == spl) {
     const int sampleIndex = (int)(gg::xorshift32(s.seed) % p.numLights);
-    const gg::V3 radiance = estimate_direct_light<WIDE>(c, s.o, n, sampleIndex, s.d, m, s.didNEE);
+    const gg::V3 radiance = estimate_direct_light<WIDE, S>(c, s.o, n, sampleIndex, s.d, m, s.didNEE);
     s.color = s.color + radiance * s.throughput * lightSampleWeight;
   }
   gg::V3 tp = gg::v3s(1.0f);
@@ -300,13 +300,13 @@ __device__ __forceinline__ void pt_finish(const DrawParams& p, const PtPath& s, 
 
 // The whole path of pixel (gx, gy) on the calling lane.  rays (optional)
 // counts the rays this path traced.
-template <bool WIDE>
+template <bool WIDE, int S = kWideLdsStack>
 __device__ __forceinline__ void pt_pixel(const DrawParams& p, TraceCtx c, int gx, int gy, unsigned long long* rays) {
   uint32_t traced = 0;
   if (rays) c.count = &traced;
   PtPath s;
   pt_begin(p, s, gx, gy, p.frameSeed);
-  while (!pt_bounce<WIDE>(p, c, s)) {
+  while (!pt_bounce<WIDE, S>(p, c, s)) {
   }
   pt_finish(p, s, gx, gy);
   if (rays) {

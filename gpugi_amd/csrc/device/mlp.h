@@ -41,6 +41,12 @@ constexpr int kMlpParams = 19456;
 // WT_l is [in][32 * ceil(out / 32)]
 constexpr int kMlpWTOff[kMlpLayers + 1] = {0, 6144, 10240, 14336, 18432, 20480};
 constexpr int kMlpWTSize = 20480;
+// The inference copy: W_l rows padded to K_l + 8 elements (so the 16 rows an
+// A-operand read touches start 4 LDS banks apart) with the columns permuted
+// to the order in which the inference kernel holds a layer's inputs in
+// registers (see mlp.hip); per-layer offsets in elements.
+constexpr int kMlpInferOff[kMlpLayers + 1] = {0, 6656, 11264, 15872, 20480, 21632};
+constexpr int kMlpInferSize = 21632;
 
 struct AdamParams {
   float lr, beta1, beta2, eps;
@@ -55,6 +61,7 @@ struct MlpDev {
   float* v;          // Adam second moments
   uint16_t* wb;      // kMlpParams BF16 weights [out][in]
   uint16_t* wtb;     // kMlpWTSize BF16 transposed weights [in][out32]
+  uint16_t* wbi;     // kMlpInferSize BF16 inference copy (padded rows, permuted columns)
   float* partials;   // per-tile gradient partials (grad kernel)
   uint32_t partialTiles;
   float* grad;       // kMlpParams reduced gradient
@@ -94,11 +101,16 @@ hipError_t LaunchMlpInfer(const MlpDev& m, const float* raw, float* radiance, si
 // constant in the denominator) over the samples of `batch` into m.grad, and the loss
 // into *lossOut (device scalar); both are 0 for an empty batch.  raw / targets:
 // n x 16 / n x 4 floats.  Needs m.partialTiles >= MlpTiles(batch.n / batch.steps).
+// Two launches: the per-tile partial gradients, then their fixed-order sum.
 hipError_t LaunchMlpGradient(const MlpDev& m, const float* raw, const float* targets, const MlpBatch& batch,
                              float* lossOut, hipStream_t stream);
-// Adam update of the master weights from m.grad, then the BF16 copies; skipped
-// when `batch` is empty.
-hipError_t LaunchMlpAdam(const MlpDev& m, const AdamParams& p, const MlpBatch& batch, hipStream_t stream);
+// The same two launches separately; with `adam` the sum also applies the Adam
+// step to the master weights and refreshes their BF16 copies (skipped for an
+// empty batch), so a caller can order other work between the two.
+hipError_t LaunchMlpGradientTiles(const MlpDev& m, const float* raw, const float* targets, const MlpBatch& batch,
+                                  hipStream_t stream);
+hipError_t LaunchMlpReduce(const MlpDev& m, const MlpBatch& batch, float* lossOut, const AdamParams* adam,
+                           hipStream_t stream);
 // Re-derives the BF16 copies from the master weights (after an upload).
 hipError_t LaunchMlpRefreshBf16(const MlpDev& m, hipStream_t stream);
 #endif

@@ -10,9 +10,10 @@
 // sample.  The backward pass uses the same instruction for
 //   dW_l = dY_l^T X_l      (K = the 32 samples of a tile; feature-major copies)
 //   dX_l = W_l^T dY_l      (A = the transposed weights WT_l[in][out32])
-// Gradients are written per 32-sample tile and summed in a fixed order (tile
-// classes t mod 4 in tile order, then the four classes), so a training step
-// is deterministic.
+// Gradients are written per 32-sample tile (four waves per tile) and summed in
+// a fixed order (tile classes t mod 16 in tile order, then a pairwise tree over
+// the classes), so a training step is deterministic; the Adam step runs in the
+// same launch as that sum.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -28,8 +29,18 @@ namespace {
 
 // Round to nearest even (v_cvt_pk_bf16_f32).
 __device__ __forceinline__ uint16_t f2bf(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
+// Two values rounded in one v_cvt_pk_bf16_f32, lo in the low half.
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ uint32_t pack_bf2(float lo, float hi) {
-  return (uint32_t)f2bf(lo) | ((uint32_t)f2bf(hi) << 16);
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2){lo, hi}, bf16x2));
+}
+// ReLU of two packed BF16 values: a BF16 bit pattern read as int16 is
+// negative exactly when the value is negative, so max(x, 0) as int16 (one
+// v_pk_max_i16) equals rounding max(v, 0) for every value but NaN.
+typedef short s16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t relu_bf2(uint32_t x) {
+  return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(__builtin_bit_cast(s16x2, x), (s16x2){0, 0}));
 }
 
 // sin / cos of 2^k pi p for k < 12: sinpi / cospi at k = 0, 3, 6, 9 and two
@@ -88,111 +99,172 @@ constexpr int kHidStride = 72;   // 64 + 8
 constexpr int kSmpStride = 40;   // feature-major rows: 32 samples + 8
 
 // ---------------------------------------------------------------- inference
+//
+// A wave keeps a layer's inputs in registers as the B operand: lane (s, q)
+// holds, for each K-step ks, the 8 inputs in "slots" 32*ks + 8*q + j, j < 8.
+// The output of a hidden layer lands in the D layout, lane (s, q) holding
+// neuron 16*ob + 4*q + v of output block ob; packing D[2*ks'] and
+// D[2*ks' + 1] gives exactly that lane's B operand of K-step ks' of the next
+// layer, if the next layer's weight column for slot p is the one of neuron
+//   16*ob + 4*q + v  where p = 32*(ob >> 1) + 8*q + 4*(ob & 1) + v.
+// The encoding fills layer 0's slots in the order the lane computes the
+// features (slot 32*(t/8) + 8*q + t%8 holds feature t of lane q, see
+// encode_b).  The inference copy of the weights (MlpDev::wbi) stores every
+// layer with its columns in slot order, so hidden activations never leave
+// the registers.  The A fragments (the weights) do not depend on the samples:
+// each wave loads all 38 of them from the block's LDS copy once (152 VGPRs)
+// and keeps them for its whole sequence of samples, so the sample loop reads
+// no LDS; two groups per step give each weight fragment two independent
+// accumulation chains.
+
+// Slot (physical column) of logical input i of layer l.
+__host__ __device__ constexpr int infer_col(int l, int i) {
+  if (l == 0) {
+    const int q = i < 72 ? (i % 36) / 12 : 3;
+    const int t = i < 36 ? i % 12 : (i < 72 ? 12 + (i - 36) % 12 : i - 72);
+    return 32 * (t / 8) + 8 * q + t % 8;
+  }
+  const int ob = i / 16, q = (i % 16) / 4, v = i % 4;
+  return 32 * (ob >> 1) + 8 * q + 4 * (ob & 1) + v;
+}
+
+constexpr bool infer_cols_are_permutations() {
+  for (int l = 0; l < 2; ++l) {
+    const int k = l == 0 ? kMlpIn : kMlpHidden;
+    bool seen[kMlpIn] = {};
+    for (int i = 0; i < k; ++i) {
+      const int c = infer_col(l, i);
+      if (c < 0 || c >= k || seen[c]) return false;
+      seen[c] = true;
+    }
+  }
+  return true;
+}
+static_assert(infer_cols_are_permutations(), "every layer's inputs map one to one onto its slots");
+static_assert(kMlpInferOff[1] == 64 * (kMlpIn + 8) && kMlpInferOff[2] - kMlpInferOff[1] == 64 * (kMlpHidden + 8) &&
+                  kMlpInferOff[5] - kMlpInferOff[4] == kMlpOut * (kMlpHidden + 8),
+              "inference copy layout");
+
+// Index into the inference copy of parameter p = W_l[o][i].
+__device__ __forceinline__ int wi_index(int p) {
+  int l = 0;
+  while (p >= kMlpWOff[l + 1]) ++l;
+  const int r = p - kMlpWOff[l];
+  const int in = kMlpLayerIn[l];
+  return kMlpInferOff[l] + (r / in) * (in + 8) + infer_col(l, r % in);
+}
 
 constexpr int kInferBlock = 512;  // 8 waves sharing one LDS copy of the weights
+constexpr int kInferGroups = 2;   // 16-sample groups per wave step
 
-// Orders a wave's LDS accesses across lanes for the compiler (a wavefront-
-// scope fence emits no barrier instruction): LDS runs one wave's operations in
-// issue order, so a wave-private buffer needs nothing more.
-__device__ __forceinline__ void wave_lds_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-  __builtin_amdgcn_wave_barrier();
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+// Layer 0's B operands of one sample (lane q's 24 features, slot order):
+// q < 3 sin then cos of position axis q at the 12 frequencies; q == 3
+// direction, normal, albedo, exponent feature, zeros.
+__device__ __forceinline__ void encode_b(const float* __restrict__ raw, uint32_t idx, uint32_t n, int q, bf16x8* b) {
+  uint32_t w[12];
+  if (q < 3) {
+    const float p = idx < n ? raw[(size_t)idx * kMlpRaw + q] : 0.0f;
+    float sn[kMlpFreqs], cs[kMlpFreqs];
+    encode_axis(p, sn, cs);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      w[k] = pack_bf2(sn[2 * k], sn[2 * k + 1]);
+      w[6 + k] = pack_bf2(cs[2 * k], cs[2 * k + 1]);
+    }
+  } else {
+    float4 v[4];  // features 0..15
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      v[i] = idx < n ? reinterpret_cast<const float4*>(raw)[(size_t)idx * 4 + i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    w[0] = pack_bf2(v[0].w, v[1].x);  // features 3, 4
+    w[1] = pack_bf2(v[1].y, v[1].z);  // 5, 6
+    w[2] = pack_bf2(v[1].w, v[2].x);  // 7, 8
+    w[3] = pack_bf2(v[2].y, v[2].z);  // 9, 10
+    w[4] = pack_bf2(v[2].w, v[3].x);  // 11, 12
+#pragma unroll
+    for (int k = 5; k < 12; ++k) w[k] = 0u;
+  }
+#pragma unroll
+  for (int ks = 0; ks < 3; ++ks)
+    b[ks] = __builtin_bit_cast(bf16x8, (u32x4){w[4 * ks], w[4 * ks + 1], w[4 * ks + 2], w[4 * ks + 3]});
 }
 
-// One hidden layer for the 16 samples of a wave: act rows (stride `stride`)
-// are read as K_IN inputs and overwritten with the 64 ReLU outputs.  The rows
-// belong to this wave alone, so a wave-level sync orders the reads before the
-// overwrite.
+// One layer with 64 outputs for kInferGroups groups: K_IN / 32 K-steps of
+// B operands in, ReLU'd BF16 B operands of the next layer out.
 template <int K_IN>
-__device__ __forceinline__ void infer_hidden(const uint16_t* __restrict__ W, uint16_t* act, int stride, int lane) {
-  const int s = lane & 15, q = lane >> 4;
-  f32x4 acc[4];
+__device__ __forceinline__ void infer_layer(const uint16_t* __restrict__ W, const bf16x8 (&x)[kInferGroups][K_IN / 32],
+                                            bf16x8 (&h)[kInferGroups][2], int s, int q) {
+  f32x4 acc[kInferGroups][4];
 #pragma unroll
-  for (int ob = 0; ob < 4; ++ob) acc[ob] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  for (int g = 0; g < kInferGroups; ++g)
 #pragma unroll
-  for (int ks = 0; ks < K_IN / 32; ++ks) {
-    const bf16x8 b = ld8(act + s * stride + 32 * ks + 8 * q);
+    for (int ob = 0; ob < 4; ++ob) acc[g][ob] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-    for (int ob = 0; ob < 4; ++ob) acc[ob] = mfma(ld8(W + (16 * ob + s) * K_IN + 32 * ks + 8 * q), b, acc[ob]);
-  }
-  wave_lds_sync();
+  for (int ks = 0; ks < K_IN / 32; ++ks)
 #pragma unroll
-  for (int ob = 0; ob < 4; ++ob) {
-    uint2 pk;
-    pk.x = (uint32_t)f2bf(fmaxf(acc[ob][0], 0.0f)) | ((uint32_t)f2bf(fmaxf(acc[ob][1], 0.0f)) << 16);
-    pk.y = (uint32_t)f2bf(fmaxf(acc[ob][2], 0.0f)) | ((uint32_t)f2bf(fmaxf(acc[ob][3], 0.0f)) << 16);
-    *reinterpret_cast<uint2*>(act + s * stride + 16 * ob + 4 * q) = pk;
-  }
-  wave_lds_sync();
+    for (int ob = 0; ob < 4; ++ob) {
+      const bf16x8 a = ld8(W + (16 * ob + s) * (K_IN + 8) + 32 * ks + 8 * q);
+#pragma unroll
+      for (int g = 0; g < kInferGroups; ++g) acc[g][ob] = mfma(a, x[g][ks], acc[g][ob]);
+    }
+#pragma unroll
+  for (int g = 0; g < kInferGroups; ++g)
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks) {
+      const f32x4& lo = acc[g][2 * ks];
+      const f32x4& hi = acc[g][2 * ks + 1];
+      h[g][ks] = __builtin_bit_cast(bf16x8, (u32x4){relu_bf2(pack_bf2(lo[0], lo[1])), relu_bf2(pack_bf2(lo[2], lo[3])),
+                                                    relu_bf2(pack_bf2(hi[0], hi[1])), relu_bf2(pack_bf2(hi[2], hi[3]))});
+    }
 }
 
-__global__ __launch_bounds__(kInferBlock) void k_mlp_infer(const uint16_t* __restrict__ wb,
+__global__ __launch_bounds__(kInferBlock) void k_mlp_infer(const uint16_t* __restrict__ wbi,
                                                            const float* __restrict__ raw, float* __restrict__ out,
                                                            uint32_t n) {
-  __shared__ __align__(16) uint16_t sW[kMlpParams];
-  __shared__ __align__(16) uint16_t sAct[(kInferBlock / 64) * 16 * kActStride];
-  for (int i = threadIdx.x; i < kMlpParams / 8; i += kInferBlock)
-    reinterpret_cast<uint4*>(sW)[i] = reinterpret_cast<const uint4*>(wb)[i];
+  __shared__ __align__(16) uint16_t sW[kMlpInferSize];
+  for (int i = threadIdx.x; i < kMlpInferSize / 8; i += kInferBlock)
+    reinterpret_cast<uint4*>(sW)[i] = reinterpret_cast<const uint4*>(wbi)[i];
   __syncthreads();  // the weights are shared and read-only from here on
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int s = lane & 15, q = lane >> 4;
-  uint16_t* act = sAct + wave * 16 * kActStride;
-  const uint32_t groups = (n + 15) / 16;
+  const uint32_t steps = (n + 16 * kInferGroups - 1) / (16 * kInferGroups);
   const uint32_t wavesTotal = gridDim.x * (kInferBlock / 64);
-  // each wave walks its own sequence of 16-sample groups
-  for (uint32_t g = blockIdx.x * (kInferBlock / 64) + wave; g < groups; g += wavesTotal) {
-    const uint32_t idx = g * 16 + s;
-    float f[kMlpRaw];
+  // each wave walks its own sequence of kInferGroups x 16 samples
+  for (uint32_t u = blockIdx.x * (kInferBlock / 64) + wave; u < steps; u += wavesTotal) {
+    bf16x8 x0[kInferGroups][3];
 #pragma unroll
-    for (int i = 0; i < kMlpRaw; ++i) f[i] = 0.0f;
-    if (idx < n) {
+    for (int g = 0; g < kInferGroups; ++g) encode_b(raw, (u * kInferGroups + g) * 16 + s, n, q, x0[g]);
+    bf16x8 h[kInferGroups][2], h2[kInferGroups][2];
+    infer_layer<kMlpIn>(sW + kMlpInferOff[0], x0, h, s, q);
+    infer_layer<kMlpHidden>(sW + kMlpInferOff[1], h, h2, s, q);
+    infer_layer<kMlpHidden>(sW + kMlpInferOff[2], h2, h, s, q);
+    infer_layer<kMlpHidden>(sW + kMlpInferOff[3], h, h2, s, q);
+    f32x4 y[kInferGroups];
 #pragma unroll
-      for (int i = 0; i < kMlpRaw / 4; ++i) {
-        const float4 v = reinterpret_cast<const float4*>(raw)[(size_t)idx * 4 + i];
-        f[4 * i + 0] = v.x;
-        f[4 * i + 1] = v.y;
-        f[4 * i + 2] = v.z;
-        f[4 * i + 3] = v.w;
-      }
+    for (int g = 0; g < kInferGroups; ++g) y[g] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks) {
+      const bf16x8 a = ld8(sW + kMlpInferOff[4] + s * (kMlpHidden + 8) + 32 * ks + 8 * q);
+#pragma unroll
+      for (int g = 0; g < kInferGroups; ++g) y[g] = mfma(a, h2[g][ks], y[g]);
     }
-    // the sample's row, 8-byte stores: lane q < 3 writes features [12q, 12q + 12)
-    // and [36 + 12q, 48 + 12q), lane q == 3 features [72, 96)
-    uint16_t* row = act + s * kActStride;
-    if (q < 3) {
-      float sn[kMlpFreqs], cs[kMlpFreqs];
-      encode_axis(f[q], sn, cs);
-      uint2* ds = reinterpret_cast<uint2*>(row + q * kMlpFreqs);
-      uint2* dc = reinterpret_cast<uint2*>(row + 36 + q * kMlpFreqs);
 #pragma unroll
-      for (int j = 0; j < kMlpFreqs / 4; ++j) {
-        ds[j] = make_uint2(pack_bf2(sn[4 * j], sn[4 * j + 1]), pack_bf2(sn[4 * j + 2], sn[4 * j + 3]));
-        dc[j] = make_uint2(pack_bf2(cs[4 * j], cs[4 * j + 1]), pack_bf2(cs[4 * j + 2], cs[4 * j + 3]));
-      }
-    } else {
-      uint2* d = reinterpret_cast<uint2*>(row + 72);
-      d[0] = make_uint2(pack_bf2(f[3], f[4]), pack_bf2(f[5], f[6]));
-      d[1] = make_uint2(pack_bf2(f[7], f[8]), pack_bf2(f[9], f[10]));
-      d[2] = make_uint2(pack_bf2(f[11], f[12]), 0u);
-      d[3] = d[4] = d[5] = make_uint2(0u, 0u);
+    for (int g = 0; g < kInferGroups; ++g) {
+      const uint32_t idx = (u * kInferGroups + g) * 16 + s;
+      if (q == 0 && idx < n) reinterpret_cast<float4*>(out)[idx] = make_float4(y[g][0], y[g][1], y[g][2], 0.0f);
     }
-    wave_lds_sync();  // the four lanes of a row wrote it; MFMA lanes read across rows
-    infer_hidden<kMlpIn>(sW + kMlpWOff[0], act, kActStride, lane);
-    infer_hidden<kMlpHidden>(sW + kMlpWOff[1], act, kActStride, lane);
-    infer_hidden<kMlpHidden>(sW + kMlpWOff[2], act, kActStride, lane);
-    infer_hidden<kMlpHidden>(sW + kMlpWOff[3], act, kActStride, lane);
-    f32x4 y = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-    for (int ks = 0; ks < kMlpHidden / 32; ++ks)
-      y = mfma(ld8(sW + kMlpWOff[4] + s * kMlpHidden + 32 * ks + 8 * q), ld8(act + s * kActStride + 32 * ks + 8 * q), y);
-    if (q == 0 && idx < n)
-      reinterpret_cast<float4*>(out)[idx] = make_float4(y[0], y[1], y[2], 0.0f);
-    wave_lds_sync();  // rows are rewritten by the next group's encoding
   }
 }
 
 // ---------------------------------------------------------------- gradient
 
-constexpr int kGradBlock = 64;  // one wave, one 32-sample tile per step
+// Four waves share one 32-sample tile; every phase splits its 16 x 16 output
+// blocks over the waves (each block is computed by one wave with the same
+// instruction sequence whatever the split, so the split changes no bit).
+constexpr int kGradWaves = 4;
+constexpr int kGradBlock = 64 * kGradWaves;
 
 struct GradLds {
   uint16_t a0[32 * kActStride];            // encoded inputs, sample-major
@@ -201,87 +273,92 @@ struct GradLds {
   uint16_t t[4][kMlpHidden * kSmpStride];  // hidden outputs, feature-major
   uint16_t dys[2][32 * kHidStride];        // dY, sample-major (ping-pong)
   uint16_t dyt[2][kMlpHidden * kSmpStride];  // dY, feature-major
+  float loss[2];                           // the two halves' loss sums
 };
 
-// Forward of layer l for half h (samples 16h..16h+15) of the tile, K_IN inputs
-// from the sample-major `src` rows; ReLU outputs go to `dst` (sample-major)
-// and `dstT` (feature-major).
+// Forward of layer l for half h (samples 16h..16h+15) of the tile and output
+// blocks ob0, ob0 + 1, K_IN inputs from the sample-major `src` rows; ReLU
+// outputs go to `dst` (sample-major) and `dstT` (feature-major).
 template <int K_IN>
 __device__ __forceinline__ void grad_fwd_hidden(const uint16_t* __restrict__ W, const uint16_t* src, int srcStride,
-                                                uint16_t* dst, uint16_t* dstT, int h, int lane) {
+                                                uint16_t* dst, uint16_t* dstT, int h, int ob0, int lane) {
   const int s = lane & 15, q = lane >> 4;
   const int smp = 16 * h + s;
-  f32x4 acc[4];
+  f32x4 acc[2];
 #pragma unroll
-  for (int ob = 0; ob < 4; ++ob) acc[ob] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  for (int j = 0; j < 2; ++j) acc[j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
   for (int ks = 0; ks < K_IN / 32; ++ks) {
     const bf16x8 b = ld8(src + smp * srcStride + 32 * ks + 8 * q);
 #pragma unroll
-    for (int ob = 0; ob < 4; ++ob) acc[ob] = mfma(ld8(W + (16 * ob + s) * K_IN + 32 * ks + 8 * q), b, acc[ob]);
+    for (int j = 0; j < 2; ++j) acc[j] = mfma(ld8(W + (16 * (ob0 + j) + s) * K_IN + 32 * ks + 8 * q), b, acc[j]);
   }
 #pragma unroll
-  for (int ob = 0; ob < 4; ++ob) {
+  for (int j = 0; j < 2; ++j) {
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const int o = 16 * ob + 4 * q + v;
-      const uint16_t y = f2bf(fmaxf(acc[ob][v], 0.0f));
-      dst[smp * kHidStride + o] = y;
-      dstT[o * kSmpStride + smp] = y;
+    for (int v = 0; v < 4; v += 2) {
+      const int o = 16 * (ob0 + j) + 4 * q + v;  // even
+      const uint32_t y = relu_bf2(pack_bf2(acc[j][v], acc[j][v + 1]));
+      *reinterpret_cast<uint32_t*>(dst + smp * kHidStride + o) = y;
+      dstT[o * kSmpStride + smp] = (uint16_t)y;
+      dstT[(o + 1) * kSmpStride + smp] = (uint16_t)(y >> 16);
     }
   }
 }
 
-// dW_l (rows OUT_L, K_IN columns) += dY^T X over the tile's 32 samples.
+// dW_l (rows OUT_L, K_IN columns) += dY^T X over the tile's 32 samples: the
+// (ob, ib) blocks numbered ob * (K_IN / 16) + ib, block b on wave b mod 4.
 template <int OUT_L, int K_IN>
-__device__ __forceinline__ void grad_dw(const uint16_t* dyt, const uint16_t* xt, float* __restrict__ part, int lane) {
+__device__ __forceinline__ void grad_dw(const uint16_t* dyt, const uint16_t* xt, float* __restrict__ part, int wave,
+                                        int lane) {
   const int s = lane & 15, q = lane >> 4;
-#pragma unroll 1
-  for (int ob = 0; ob < OUT_L / 16; ++ob) {
-    const bf16x8 a = ld8(dyt + (16 * ob + s) * kSmpStride + 8 * q);
+  constexpr int kIb = K_IN / 16;
 #pragma unroll 2
-    for (int ib = 0; ib < K_IN / 16; ++ib) {
-      const f32x4 d = mfma(a, ld8(xt + (16 * ib + s) * kSmpStride + 8 * q), f32x4{0.0f, 0.0f, 0.0f, 0.0f});
+  for (int b = wave; b < (OUT_L / 16) * kIb; b += kGradWaves) {
+    const int ob = b / kIb, ib = b % kIb;
+    const f32x4 d = mfma(ld8(dyt + (16 * ob + s) * kSmpStride + 8 * q), ld8(xt + (16 * ib + s) * kSmpStride + 8 * q),
+                         f32x4{0.0f, 0.0f, 0.0f, 0.0f});
 #pragma unroll
-      for (int v = 0; v < 4; ++v) part[(16 * ob + 4 * q + v) * K_IN + 16 * ib + s] = d[v];
-    }
+    for (int v = 0; v < 4; ++v) part[(16 * ob + 4 * q + v) * K_IN + 16 * ib + s] = d[v];
   }
 }
 
 // dX_l = W_l^T dY_l (K = OUT32 outputs), gated by the ReLU of the layer input
-// x (sample-major, 64 features), into the next dY buffers.
+// x (sample-major, 64 features), into the next dY buffers: input block
+// ib = wave, both halves.
 template <int OUT32>
 __device__ __forceinline__ void grad_dx(const uint16_t* __restrict__ WT, const uint16_t* dys, const uint16_t* x,
-                                        uint16_t* ndys, uint16_t* ndyt, int lane) {
+                                        uint16_t* ndys, uint16_t* ndyt, int wave, int lane) {
   const int s = lane & 15, q = lane >> 4;
+  const int ib = wave;
 #pragma unroll 1
   for (int h = 0; h < 2; ++h) {
     const int smp = 16 * h + s;
-#pragma unroll 1
-    for (int ib = 0; ib < kMlpHidden / 16; ++ib) {
-      f32x4 acc = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    f32x4 acc = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-      for (int ks = 0; ks < OUT32 / 32; ++ks)
-        acc = mfma(ld8(WT + (16 * ib + s) * OUT32 + 32 * ks + 8 * q), ld8(dys + smp * kHidStride + 32 * ks + 8 * q), acc);
+    for (int ks = 0; ks < OUT32 / 32; ++ks)
+      acc = mfma(ld8(WT + (16 * ib + s) * OUT32 + 32 * ks + 8 * q), ld8(dys + smp * kHidStride + 32 * ks + 8 * q), acc);
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int i = 16 * ib + 4 * q + v;
-        const float g = bf2f(x[smp * kHidStride + i]) > 0.0f ? acc[v] : 0.0f;
-        const uint16_t gb = f2bf(g);
-        ndys[smp * kHidStride + i] = gb;
-        ndyt[i * kSmpStride + smp] = gb;
-      }
+    for (int v = 0; v < 4; ++v) {
+      const int i = 16 * ib + 4 * q + v;
+      const float g = bf2f(x[smp * kHidStride + i]) > 0.0f ? acc[v] : 0.0f;
+      const uint16_t gb = f2bf(g);
+      ndys[smp * kHidStride + i] = gb;
+      ndyt[i * kSmpStride + smp] = gb;
     }
   }
 }
+static_assert(kMlpHidden / 16 == kGradWaves, "grad_dx gives each wave one 16-input block");
 
 __global__ __launch_bounds__(kGradBlock) void k_mlp_grad(const uint16_t* __restrict__ wb, const uint16_t* __restrict__ wtb,
                                                          const float* __restrict__ rawAll, const float* __restrict__ targetsAll,
                                                          MlpBatch batch, float* __restrict__ partials,
                                                          float* __restrict__ lossPart) {
   __shared__ __align__(16) GradLds L;
-  const int lane = threadIdx.x;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int s = lane & 15, q = lane >> 4;
+  // the forward split: half h, output blocks ob0 and ob0 + 1
+  const int h = wave & 1, ob0 = 2 * (wave >> 1);
   uint32_t first, n;
   MlpBatchRange(batch, &first, &n);
   const float* raw = rawAll + (size_t)first * kMlpRaw;
@@ -289,9 +366,8 @@ __global__ __launch_bounds__(kGradBlock) void k_mlp_grad(const uint16_t* __restr
   const float invCount = n ? (float)(1.0 / (3.0 * (double)n)) : 0.0f;
   const uint32_t tiles = (n + 31) / 32;
   for (uint32_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
-    // encode both halves
-#pragma unroll 1
-    for (int h = 0; h < 2; ++h) {
+    // encode: waves 0 and 1, one half each
+    if (wave < 2) {
       const int smp = 16 * h + s;
       const uint32_t idx = tile * 32 + smp;
       float f[kMlpRaw];
@@ -315,22 +391,18 @@ __global__ __launch_bounds__(kGradBlock) void k_mlp_grad(const uint16_t* __restr
     }
     __syncthreads();
     // forward, keeping every layer's activations
-#pragma unroll 1
-    for (int h = 0; h < 2; ++h) grad_fwd_hidden<kMlpIn>(wb + kMlpWOff[0], L.a0, kActStride, L.a[0], L.t[0], h, lane);
+    grad_fwd_hidden<kMlpIn>(wb + kMlpWOff[0], L.a0, kActStride, L.a[0], L.t[0], h, ob0, lane);
     __syncthreads();
 #pragma unroll 1
     for (int l = 1; l < 4; ++l) {
-#pragma unroll 1
-      for (int h = 0; h < 2; ++h)
-        grad_fwd_hidden<kMlpHidden>(wb + kMlpWOff[l], L.a[l - 1], kHidStride, L.a[l], L.t[l], h, lane);
+      grad_fwd_hidden<kMlpHidden>(wb + kMlpWOff[l], L.a[l - 1], kHidStride, L.a[l], L.t[l], h, ob0, lane);
       __syncthreads();
     }
-    // output layer and the loss gradient dY4 (outputs 0..2; the rest stay 0)
-    float loss = 0.0f;
-#pragma unroll 1
-    for (int h = 0; h < 2; ++h) {
+    // output layer and the loss gradient dY4 (outputs 0..2; the rest stay 0): waves 0 and 1
+    if (wave < 2) {
       const int smp = 16 * h + s;
       const uint32_t idx = tile * 32 + smp;
+      float loss = 0.0f;
       f32x4 y = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
       for (int ks = 0; ks < kMlpHidden / 32; ++ks)
@@ -351,56 +423,27 @@ __global__ __launch_bounds__(kGradBlock) void k_mlp_grad(const uint16_t* __restr
         L.dys[0][smp * kHidStride + 16 + o] = 0;  // K padding to 32 outputs
         L.dyt[0][o * kSmpStride + smp] = gb;
       }
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) loss += __shfl_xor(loss, off);
+      if (lane == 0) L.loss[h] = loss;
     }
     __syncthreads();
     float* part = partials + (size_t)tile * kMlpParams;
     // layer 4: dW (16 x 64) and dX through WT_4 (64 x 32)
-    grad_dw<kMlpOut, kMlpHidden>(L.dyt[0], L.t[3], part + kMlpWOff[4], lane);
-    grad_dx<32>(wtb + kMlpWTOff[4], L.dys[0], L.a[3], L.dys[1], L.dyt[1], lane);
+    grad_dw<kMlpOut, kMlpHidden>(L.dyt[0], L.t[3], part + kMlpWOff[4], wave, lane);
+    grad_dx<32>(wtb + kMlpWTOff[4], L.dys[0], L.a[3], L.dys[1], L.dyt[1], wave, lane);
     __syncthreads();
     int cur = 1;
 #pragma unroll 1
     for (int l = 3; l >= 1; --l) {
-      grad_dw<kMlpHidden, kMlpHidden>(L.dyt[cur], L.t[l - 1], part + kMlpWOff[l], lane);
-      grad_dx<kMlpHidden>(wtb + kMlpWTOff[l], L.dys[cur], L.a[l - 1], L.dys[cur ^ 1], L.dyt[cur ^ 1], lane);
+      grad_dw<kMlpHidden, kMlpHidden>(L.dyt[cur], L.t[l - 1], part + kMlpWOff[l], wave, lane);
+      grad_dx<kMlpHidden>(wtb + kMlpWTOff[l], L.dys[cur], L.a[l - 1], L.dys[cur ^ 1], L.dyt[cur ^ 1], wave, lane);
       __syncthreads();
       cur ^= 1;
     }
-    grad_dw<kMlpHidden, kMlpIn>(L.dyt[cur], L.t0, part + kMlpWOff[0], lane);
-    // tile loss
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) loss += __shfl_xor(loss, off);
-    if (lane == 0) lossPart[tile] = loss;
+    grad_dw<kMlpHidden, kMlpIn>(L.dyt[cur], L.t0, part + kMlpWOff[0], wave, lane);
+    if (threadIdx.x == 0) lossPart[tile] = L.loss[0] + L.loss[1];
     __syncthreads();
-  }
-}
-
-// Sum of the per-tile partials: block = 4 waves x 64 parameters; wave c sums
-// tiles c, c + 4, c + 8, ... in order, then the four class sums are added in
-// class order.  Block 0 also sums the tile losses (lane l: tiles l, l + 64, ...,
-// then a fixed butterfly).
-constexpr int kReduceParams = 64;
-__global__ __launch_bounds__(256) void k_mlp_reduce(const float* __restrict__ partials, MlpBatch batch,
-                                                    const float* __restrict__ lossPart, float* __restrict__ grad,
-                                                    float* __restrict__ lossOut) {
-  __shared__ float cls[4][kReduceParams];
-  uint32_t first, n;
-  MlpBatchRange(batch, &first, &n);
-  const uint32_t tiles = (n + 31) / 32;
-  const int lane = threadIdx.x & 63, c = threadIdx.x >> 6;
-  const uint32_t p = blockIdx.x * kReduceParams + lane;
-  float g = 0.0f;
-  if (p < (uint32_t)kMlpParams)
-    for (uint32_t t = (uint32_t)c; t < tiles; t += 4) g += partials[(size_t)t * kMlpParams + p];
-  cls[c][lane] = g;
-  __syncthreads();
-  if (c == 0 && p < (uint32_t)kMlpParams) grad[p] = ((cls[0][lane] + cls[1][lane]) + cls[2][lane]) + cls[3][lane];
-  if (blockIdx.x == 0 && c == 0 && lossOut) {
-    float l = 0.0f;
-    for (uint32_t t = (uint32_t)lane; t < tiles; t += 64) l += lossPart[t];
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) l += __shfl_xor(l, off);
-    if (lane == 0) *lossOut = l;
   }
 }
 
@@ -415,34 +458,81 @@ __device__ __forceinline__ int wt_index(int p) {
   return kMlpWTOff[l] + i * out32 + o;
 }
 
-__global__ void k_mlp_adam(float* __restrict__ w, float* __restrict__ m, float* __restrict__ v,
-                           const float* __restrict__ grad, uint16_t* __restrict__ wb, uint16_t* __restrict__ wtb,
-                           float lr, float beta1, float beta2, float eps, float bc1, float bc2, MlpBatch batch) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= kMlpParams) return;
-  uint32_t first, n;
-  MlpBatchRange(batch, &first, &n);
-  if (n == 0) return;  // no samples this step
-  const float g = grad[p];
-  const float mm = beta1 * m[p] + (1.0f - beta1) * g;
-  const float vv = beta2 * v[p] + (1.0f - beta2) * (g * g);
-  m[p] = mm;
-  v[p] = vv;
-  const float mhat = mm / bc1;
-  const float vhat = vv / bc2;
-  const float ww = w[p] - lr * mhat / (sqrtf(vhat) + eps);
-  w[p] = ww;
+// Adam on one weight (bias corrections bc1 = 1 - beta1^t, bc2 = 1 - beta2^t
+// computed on the host), then its two BF16 copies.
+struct AdamArgs {
+  float lr, beta1, beta2, eps, bc1, bc2;
+};
+__device__ __forceinline__ void adam_update(const MlpDev& d, int p, float g, const AdamArgs& a) {
+  const float mm = a.beta1 * d.m[p] + (1.0f - a.beta1) * g;
+  const float vv = a.beta2 * d.v[p] + (1.0f - a.beta2) * (g * g);
+  d.m[p] = mm;
+  d.v[p] = vv;
+  const float mhat = mm / a.bc1;
+  const float vhat = vv / a.bc2;
+  const float ww = d.w[p] - a.lr * mhat / (sqrtf(vhat) + a.eps);
+  d.w[p] = ww;
   const uint16_t b = f2bf(ww);
-  wb[p] = b;
-  wtb[wt_index(p)] = b;
+  d.wb[p] = b;
+  d.wtb[wt_index(p)] = b;
+  d.wbi[wi_index(p)] = b;
 }
 
-__global__ void k_mlp_refresh(const float* __restrict__ w, uint16_t* __restrict__ wb, uint16_t* __restrict__ wtb) {
+// Sum of the per-tile partials: block = 16 waves x 64 parameters; wave c
+// sums tiles c, c + 16, c + 32, ... in tile order (the loads of an unrolled
+// run are independent of the sum, so eight are in flight per lane), then
+// wave 0 adds the sixteen class sums as a fixed pairwise tree.  The order
+// depends only on the tile count, so a step is deterministic.  With ADAM the
+// same lane then applies the Adam step to its weight (skipped for an empty
+// batch).  Block 0 also sums the tile losses (lane l: tiles l, l + 64, ...,
+// then a fixed butterfly).
+constexpr int kReduceParams = 64;
+constexpr int kReduceWaves = 16;
+template <bool ADAM>
+__global__ __launch_bounds__(64 * kReduceWaves) void k_mlp_reduce(MlpDev d, MlpBatch batch, float* __restrict__ lossOut,
+                                                                  AdamArgs adam) {
+  __shared__ float cls[kReduceWaves][kReduceParams];
+  uint32_t first, n;
+  MlpBatchRange(batch, &first, &n);
+  const uint32_t tiles = (n + 31) / 32;
+  const int lane = threadIdx.x & 63, c = threadIdx.x >> 6;
+  const uint32_t p = blockIdx.x * kReduceParams + lane;
+  const float* __restrict__ partials = d.partials;
+  float g = 0.0f;
+  if (p < (uint32_t)kMlpParams) {
+#pragma unroll 8
+    for (uint32_t t = (uint32_t)c; t < tiles; t += kReduceWaves) g += partials[(size_t)t * kMlpParams + p];
+  }
+  cls[c][lane] = g;
+  __syncthreads();
+  if (c == 0 && p < (uint32_t)kMlpParams) {
+    float v[kReduceWaves];
+#pragma unroll
+    for (int i = 0; i < kReduceWaves; ++i) v[i] = cls[i][lane];
+#pragma unroll
+    for (int w = 1; w < kReduceWaves; w *= 2)
+#pragma unroll
+      for (int i = 0; i < kReduceWaves; i += 2 * w) v[i] = v[i] + v[i + w];
+    d.grad[p] = v[0];
+    if (ADAM && n > 0) adam_update(d, (int)p, v[0], adam);
+  }
+  if (blockIdx.x == 0 && c == 0 && lossOut) {
+    float l = 0.0f;
+    for (uint32_t t = (uint32_t)lane; t < tiles; t += 64) l += d.loss[t];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) l += __shfl_xor(l, off);
+    if (lane == 0) *lossOut = l;
+  }
+}
+
+__global__ void k_mlp_refresh(const float* __restrict__ w, uint16_t* __restrict__ wb, uint16_t* __restrict__ wtb,
+                              uint16_t* __restrict__ wbi) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= kMlpParams) return;
   const uint16_t b = f2bf(w[p]);
   wb[p] = b;
   wtb[wt_index(p)] = b;
+  wbi[wi_index(p)] = b;
 }
 
 __global__ void k_mlp_encode(const float* __restrict__ raw, uint16_t* __restrict__ enc, uint32_t n) {
@@ -476,13 +566,17 @@ hipError_t LaunchMlpEncode(const float* raw, uint16_t* enc, size_t n, hipStream_
 hipError_t LaunchMlpInfer(const MlpDev& m, const float* raw, float* radiance, size_t n, hipStream_t stream) {
   if (n == 0) return hipSuccess;
   if (n > 0xffffff00ull) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_mlp_infer, dim3(GridFor(n, 16 * (kInferBlock / 64) * 8, 512)), dim3(kInferBlock), 0, stream, m.wb, raw, radiance,
-                     (uint32_t)n);
+  // at least 8 steps per wave, at most one block per CU (the kernel keeps the
+  // whole network in registers, two waves per SIMD)
+  int dev = 0, cus = 256;
+  if (hipGetDevice(&dev) == hipSuccess) hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  hipLaunchKernelGGL(k_mlp_infer, dim3(GridFor(n, 16 * kInferGroups * (kInferBlock / 64) * 8, cus)), dim3(kInferBlock), 0,
+                     stream, m.wbi, raw, radiance, (uint32_t)n);
   return hipGetLastError();
 }
 
-hipError_t LaunchMlpGradient(const MlpDev& m, const float* raw, const float* targets, const MlpBatch& batch,
-                             float* lossOut, hipStream_t stream) {
+hipError_t LaunchMlpGradientTiles(const MlpDev& m, const float* raw, const float* targets, const MlpBatch& batch,
+                                  hipStream_t stream) {
   if (batch.n == 0 || batch.n > 0xffffff00u || batch.steps == 0 || batch.step >= batch.steps)
     return hipErrorInvalidValue;
   if (!batch.count && (batch.step != 0 || batch.steps != 1)) return hipErrorInvalidValue;
@@ -491,23 +585,32 @@ hipError_t LaunchMlpGradient(const MlpDev& m, const float* raw, const float* tar
   if (tiles == 0 || tiles > m.partialTiles) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_mlp_grad, dim3(GridFor(tiles, 1, 4096)), dim3(kGradBlock), 0, stream, m.wb, m.wtb, raw,
                      targets, batch, m.partials, m.loss);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_mlp_reduce, dim3((kMlpParams + kReduceParams - 1) / kReduceParams), dim3(256), 0, stream,
-                     m.partials, batch, m.loss, m.grad, lossOut);
   return hipGetLastError();
 }
 
-hipError_t LaunchMlpAdam(const MlpDev& m, const AdamParams& p, const MlpBatch& batch, hipStream_t stream) {
-  const double bc1 = 1.0 - std::pow((double)p.beta1, (double)p.step);
-  const double bc2 = 1.0 - std::pow((double)p.beta2, (double)p.step);
-  hipLaunchKernelGGL(k_mlp_adam, dim3((kMlpParams + 255) / 256), dim3(256), 0, stream, m.w, m.m, m.v, m.grad, m.wb,
-                     m.wtb, p.lr, p.beta1, p.beta2, p.eps, (float)bc1, (float)bc2, batch);
+hipError_t LaunchMlpReduce(const MlpDev& m, const MlpBatch& batch, float* lossOut, const AdamParams* adam,
+                           hipStream_t stream) {
+  const dim3 grid((kMlpParams + kReduceParams - 1) / kReduceParams), block(64 * kReduceWaves);
+  if (!adam) {
+    hipLaunchKernelGGL(k_mlp_reduce<false>, grid, block, 0, stream, m, batch, lossOut, AdamArgs{});
+    return hipGetLastError();
+  }
+  const double bc1 = 1.0 - std::pow((double)adam->beta1, (double)adam->step);
+  const double bc2 = 1.0 - std::pow((double)adam->beta2, (double)adam->step);
+  const AdamArgs a{adam->lr, adam->beta1, adam->beta2, adam->eps, (float)bc1, (float)bc2};
+  hipLaunchKernelGGL(k_mlp_reduce<true>, grid, block, 0, stream, m, batch, lossOut, a);
   return hipGetLastError();
+}
+
+hipError_t LaunchMlpGradient(const MlpDev& m, const float* raw, const float* targets, const MlpBatch& batch,
+                             float* lossOut, hipStream_t stream) {
+  hipError_t e = LaunchMlpGradientTiles(m, raw, targets, batch, stream);
+  if (e != hipSuccess) return e;
+  return LaunchMlpReduce(m, batch, lossOut, nullptr, stream);
 }
 
 hipError_t LaunchMlpRefreshBf16(const MlpDev& m, hipStream_t stream) {
-  hipLaunchKernelGGL(k_mlp_refresh, dim3((kMlpParams + 255) / 256), dim3(256), 0, stream, m.w, m.wb, m.wtb);
+  hipLaunchKernelGGL(k_mlp_refresh, dim3((kMlpParams + 255) / 256), dim3(256), 0, stream, m.w, m.wb, m.wtb, m.wbi);
   return hipGetLastError();
 }
 

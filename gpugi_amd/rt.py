@@ -156,7 +156,7 @@ _SIGNATURES = {
     "gpugi_batch_add": [_P, ctypes.POINTER(SceneDesc), ctypes.POINTER(Camera), ctypes.c_int, ctypes.c_int, _P,
                         ctypes.c_uint32, ctypes.c_int],
     "gpugi_batch_finalize": [_P],
-    "gpugi_batch_config": [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int],
+    "gpugi_batch_config": [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int],
     "gpugi_batch_reset": [_P, ctypes.c_uint32],
     "gpugi_batch_draw": [_P, ctypes.c_int],
     "gpugi_batch_capture": [_P],
@@ -165,7 +165,8 @@ _SIGNATURES = {
     "gpugi_batch_readback": [_P, ctypes.c_int, _P, ctypes.c_size_t],
     "gpugi_batch_rays": [_P, ctypes.POINTER(ctypes.c_uint64)],
     "gpugi_batch_autotune": [_P, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
-                             ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_float)],
+                             ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                             ctypes.POINTER(ctypes.c_float)],
     "gpugi_cache_config": [_P, ctypes.c_float, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int,
                            ctypes.c_float],
     "gpugi_cache_stats": [_P, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_float)],
@@ -793,8 +794,8 @@ class Batch:
     def finalize(self):
         self._call("gpugi_batch_finalize")
 
-    def config(self, kernel=KERNEL_WIDE, tiles_per_grab=2, grid_multiplier=2):
-        self._call("gpugi_batch_config", int(kernel), int(tiles_per_grab), int(grid_multiplier))
+    def config(self, kernel=KERNEL_WIDE, tiles_per_grab=2, grid_multiplier=2, stack_depth=16):
+        self._call("gpugi_batch_config", int(kernel), int(tiles_per_grab), int(grid_multiplier), int(stack_depth))
 
     def reset(self, frame_index=0):
         self._call("gpugi_batch_reset", ctypes.c_uint32(frame_index))
@@ -825,10 +826,11 @@ class Batch:
         return r.value
 
     def autotune(self, frames=2):
-        k, g, m, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_float()
+        k, g, m, st, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_float()
         self._call("gpugi_batch_autotune", int(frames), ctypes.byref(k), ctypes.byref(g), ctypes.byref(m),
-                   ctypes.byref(ms))
-        return {"kernel": k.value, "tiles_per_grab": g.value, "grid_multiplier": m.value, "ms_per_frame": ms.value}
+                   ctypes.byref(st), ctypes.byref(ms))
+        return {"kernel": k.value, "tiles_per_grab": g.value, "grid_multiplier": m.value, "stack_depth": st.value,
+                "ms_per_frame": ms.value}
 
     def close(self):
         if self._h:

@@ -631,9 +631,11 @@ int gpugi_batch_add(gpugi_batch* batch, const gpugi_scene_desc* scene, const gpu
 /* Packs and uploads all scenes; then gpugi_batch_reset(batch, 0). */
 int gpugi_batch_finalize(gpugi_batch* batch);
 /* Traversal (GPUGI_KERNEL_REFERENCE: the BVH2 escape walk, GPUGI_KERNEL_WIDE:
- * the 4-wide BVH), tiles taken per queue access and the persistent grid in
- * quarters of full residency (1..16).  Defaults: WIDE, 2, 2. */
-int gpugi_batch_config(gpugi_batch* batch, int kernel, int tiles_per_grab, int grid_multiplier);
+ * the 4-wide BVH), tiles taken per queue access, the persistent grid in
+ * quarters of full residency (1..16) and the wide walk's LDS stack entries
+ * per lane (16, 24 or 32; deeper entries spill to global memory; the escape
+ * walk ignores it).  Defaults: WIDE, 2, 2, 16. */
+int gpugi_batch_config(gpugi_batch* batch, int kernel, int tiles_per_grab, int grid_multiplier, int stack_depth);
 /* Clears every accumulation buffer and the ray counter; the next frame is frame_index. */
 int gpugi_batch_reset(gpugi_batch* batch, uint32_t frame_index);
 /* One frame: persistent != 0 as one persistent launch, else one launch per scene. */
@@ -649,9 +651,10 @@ int gpugi_batch_readback(gpugi_batch* batch, int scene, float* rgba, size_t byte
 /* Ray queries (closest + any hit) traced since the last reset. */
 int gpugi_batch_rays(gpugi_batch* batch, uint64_t* rays);
 /* Times every (kernel, tiles per grab in {1,2,4,8}, grid multiplier in
- * {1,2,4}) on `frames` persistent frames and keeps the fastest. */
+ * {1,2,4}, and for the wide kernel stack depth in {16,24,32}) on `frames`
+ * persistent frames and keeps the fastest. */
 int gpugi_batch_autotune(gpugi_batch* batch, int frames, int* kernel, int* tiles_per_grab, int* grid_multiplier,
-                         float* best_ms);
+                         int* stack_depth, float* best_ms);
 
 /* WangHash (utilities/random.cpp:5-13): the per-iteration FrameSeed. */
 uint32_t gpugi_wanghash(uint32_t x);

@@ -63,6 +63,40 @@ def test_batch_matches_single_scene_renders(rt, ctx, kernel):
         b.close()
 
 
+@pytest.mark.parametrize("stack", [16, 24, 32])
+def test_batch_stack_depths_match_single_scene(rt, ctx, stack):
+    """The LDS stack depth of the wide walk (16, 24 or 32 entries per lane;
+    deeper entries spill to global memory) changes where the traversal stack
+    lives, not what it holds: every depth gives the single-scene image bit for
+    bit.  The 24-cell heightfield needs 19 entries (in LDS from depth 24 on),
+    the 30-sphere clutter room 36 (it spills at every depth)."""
+    scenes = small_scenes(rt)
+    hs = rt.HostScene.clutter(30, 5)
+    scenes.append((hs, workloads.clutter_camera(32, 32), 32, 32, rt.LightSampler(hs, 7).generate(128)))
+    ctx.load_scene(hs)
+    ctx.set_wide_config(2, 16, 8, 16)
+    assert ctx.wide_config()["max_stack"] > 32  # the spill path is exercised at every depth
+    b = rt.Batch(0)
+    try:
+        for hs, cam, w, h, lights in scenes:
+            b.add(hs, cam, w, h, lights, 5)
+        b.finalize()
+        b.config(rt.KERNEL_WIDE, 2, 2, stack)
+        print("case: batch stack depth %d" % stack, flush=True)
+        for _ in range(2):
+            b.draw(True)
+        got = [b.readback(i) for i in range(len(scenes))]
+        for i, (hs, cam, w, h, lights) in enumerate(scenes):
+            single = single_scene(rt, ctx, hs, cam, w, h, lights, 2, rt.KERNEL_WIDE)
+            assert single[..., :3].mean() > 0, i
+            assert np.array_equal(got[i].view(np.uint32), single.view(np.uint32)), i
+        with pytest.raises(rt.GpugiError):
+            b.config(rt.KERNEL_WIDE, 2, 2, 20)
+    finally:
+        ctx.set_wide_config(2, 10, 8, 16)
+        b.close()
+
+
 def test_graph_replay_matches_eager_frames(rt):
     scenes = small_scenes(rt)
     b = rt.Batch(0)
@@ -84,5 +118,18 @@ def test_graph_replay_matches_eager_frames(rt):
                 assert np.array_equal(got[i].view(np.uint32), eager[f][i].view(np.uint32)), (f, i)
         tuned = b.autotune(1)
         assert tuned["kernel"] in (rt.KERNEL_REFERENCE, rt.KERNEL_WIDE) and tuned["ms_per_frame"] > 0
+        assert tuned["tiles_per_grab"] in (1, 2, 4, 8) and tuned["grid_multiplier"] in (1, 2, 4)
+        assert tuned["stack_depth"] in (16, 24, 32)
+        # the tuned grab, grid and stack depth render what the defaults render with the tuned traversal
+        frames = []
+        for cfg in (None, (tuned["kernel"], 2, 2, 16)):
+            if cfg:
+                b.config(*cfg)
+            b.reset(0)
+            for f in range(3):
+                b.draw(True)
+            frames.append([b.readback(i) for i in range(len(scenes))])
+        for i in range(len(scenes)):
+            assert np.array_equal(frames[0][i].view(np.uint32), frames[1][i].view(np.uint32)), i
     finally:
         b.close()

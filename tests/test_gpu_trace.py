@@ -191,3 +191,39 @@ def test_wide_configurations(rt, oracle, ctx, leaf, stack, refill, wait):
     got_any = hits_int(ctx.trace(rays_struct(rt, shadow), rt.ANY_HIT, rt.KERNEL_WIDE))[:, 0]
     assert (got_any != want_any).sum() <= max(2, 1e-3 * len(shadow))
     ctx.set_wide_config(2, 10, 8, 16)
+
+
+@pytest.mark.parametrize("query", ["closest", "any"])
+def test_autotune_trace_keeps_answers(rt, oracle, ctx, query):
+    """gpugi_autotune_trace searches leaf size, LDS stack depth, refill and
+    wait thresholds on the given rays and keeps the fastest; the default
+    configuration (leaf 2, stack 10, refill 8, wait 16) is one of the timed
+    points, so the result is never slower than it on those rays.  After the
+    search the scene's wide BVH is rebuilt for the chosen leaf size and the
+    wide kernel must still give the oracle's answers."""
+    print("case: autotune", query, flush=True)
+    hs = rt.HostScene.heightfield(64, workloads.MASTER_SEED)
+    ctx.load_scene(hs)
+    sc, prim, ph, diff = scene_rays(rt, oracle, hs, workloads.heightfield_camera(), 1920, 1080, 32768)
+    rays = diff if query == "closest" else oracle.raygen_shadow(prim, ph, workloads.heightfield_lights(), 32768)
+    q = rt.CLOSEST_HIT if query == "closest" else rt.ANY_HIT
+    n = len(rays)
+    rb, hb = ctx.alloc(n * 32), ctx.alloc(n * 16)
+    try:
+        rb.upload(rays_struct(rt, rays))
+        best = ctx.autotune_trace(rb, hb, n, q, repeats=1)
+        assert 0 < best < 1e3
+        cfg = ctx.wide_config()
+        assert cfg["max_leaf_triangles"] in (1, 2, 4, 8)
+        assert cfg["lds_stack_entries"] in (4, 8, 10, 12, 16, 24)
+        assert cfg["refill_lanes"] in (8, 16, 32, 64)
+        assert cfg["wait_lanes"] in (1, 8, 16, 24, 32, 48)
+        ctx.trace_device(rb, hb, n, q, rt.KERNEL_WIDE)
+        got = hb.download(np.int32, n * 4).reshape(n, 4)
+        if q == rt.CLOSEST_HIT:
+            check_wide(oracle, sc, rays, oracle.trace(sc, rays), got, 1e-3)
+        else:
+            want = oracle.trace(sc, rays, any_hit=True)[:, 0]
+            assert (got[:, 0] != want).sum() <= max(2, 1e-3 * n)
+    finally:
+        ctx.set_wide_config(2, 10, 8, 16)
