@@ -120,8 +120,15 @@ constexpr int kSmpStride = 40;   // feature-major rows: 32 samples + 8
 // Slot (physical column) of logical input i of layer l.
 __host__ __device__ constexpr int infer_col(int l, int i) {
   if (l == 0) {
-    const int q = i < 72 ? (i % 36) / 12 : 3;
-    const int t = i < 36 ? i % 12 : (i < 72 ? 12 + (i - 36) % 12 : i - 72);
+    int q = 0, t = 0;
+    if (i < 72) {  // sin (i < 36) or cos of axis a at frequency k: pair P = 4a + k / 3
+      const int a = (i % 36) / 12, k = i % 12, P = 4 * a + k / 3;
+      q = P / 3;
+      t = 6 * (P % 3) + k % 3 + (i >= 36 ? 3 : 0);
+    } else {
+      q = (i - 72) / 6;
+      t = 18 + (i - 72) % 6;
+    }
     return 32 * (t / 8) + 8 * q + t % 8;
   }
   const int ob = i / 16, q = (i % 16) / 4, v = i % 4;
@@ -159,36 +166,45 @@ constexpr int kInferGroups = 2;   // 16-sample groups per wave step
 
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
-// Layer 0's B operands of one sample (lane q's 24 features, slot order):
-// q < 3 sin then cos of position axis q at the 12 frequencies; q == 3
-// direction, normal, albedo, exponent feature, zeros.
+// Layer 0's B operands of one sample (lane q's 24 features, slot order).
+// The 72 sin / cos features come from 12 (axis a, base frequency 3b) pairs
+// P = 4a + b, each a sinpi / cospi and two double-angle steps (the arithmetic
+// of encode_axis); lane q evaluates pairs 3q .. 3q + 2, pair m of the lane
+// filling slots 6m .. 6m + 2 (sin at frequencies 3b .. 3b + 2) and
+// 6m + 3 .. 6m + 5 (cos).  Slots 18..23 of lane q hold features 72 + 6q ..
+// 77 + 6q (direction, normal, albedo, exponent feature, zeros).
 __device__ __forceinline__ void encode_b(const float* __restrict__ raw, uint32_t idx, uint32_t n, int q, bf16x8* b) {
-  uint32_t w[12];
-  if (q < 3) {
-    const float p = idx < n ? raw[(size_t)idx * kMlpRaw + q] : 0.0f;
-    float sn[kMlpFreqs], cs[kMlpFreqs];
-    encode_axis(p, sn, cs);
+  float4 r[4];  // features 0..15
 #pragma unroll
-    for (int k = 0; k < 6; ++k) {
-      w[k] = pack_bf2(sn[2 * k], sn[2 * k + 1]);
-      w[6 + k] = pack_bf2(cs[2 * k], cs[2 * k + 1]);
+  for (int i = 0; i < 4; ++i)
+    r[i] = idx < n ? reinterpret_cast<const float4*>(raw)[(size_t)idx * 4 + i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  const float f[13] = {r[0].x, r[0].y, r[0].z, r[0].w, r[1].x, r[1].y, r[1].z,
+                       r[1].w, r[2].x, r[2].y, r[2].z, r[2].w, r[3].x};
+  float v[24];
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    const int P = 3 * q + m, a = P >> 2;
+    const float x = ldexpf(a == 0 ? f[0] : (a == 1 ? f[1] : f[2]), 3 * (P & 3));  // exact
+    float sn = sinpif(x), cs = cospif(x);
+    v[6 * m] = sn;
+    v[6 * m + 3] = cs;
+#pragma unroll
+    for (int j = 1; j < 3; ++j) {
+      const float s2 = (sn + sn) * cs;
+      const float c2 = (cs - sn) * (cs + sn);
+      sn = s2;
+      cs = c2;
+      v[6 * m + j] = sn;
+      v[6 * m + 3 + j] = cs;
     }
-  } else {
-    float4 v[4];  // features 0..15
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-      v[i] = idx < n ? reinterpret_cast<const float4*>(raw)[(size_t)idx * 4 + i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    w[0] = pack_bf2(v[0].w, v[1].x);  // features 3, 4
-    w[1] = pack_bf2(v[1].y, v[1].z);  // 5, 6
-    w[2] = pack_bf2(v[1].w, v[2].x);  // 7, 8
-    w[3] = pack_bf2(v[2].y, v[2].z);  // 9, 10
-    w[4] = pack_bf2(v[2].w, v[3].x);  // 11, 12
-#pragma unroll
-    for (int k = 5; k < 12; ++k) w[k] = 0u;
   }
 #pragma unroll
+  for (int k = 0; k < 6; ++k) v[18 + k] = q == 0 ? f[3 + k] : (q == 1 && k < 4 ? f[9 + k] : 0.0f);
+#pragma unroll
   for (int ks = 0; ks < 3; ++ks)
-    b[ks] = __builtin_bit_cast(bf16x8, (u32x4){w[4 * ks], w[4 * ks + 1], w[4 * ks + 2], w[4 * ks + 3]});
+    b[ks] = __builtin_bit_cast(bf16x8, (u32x4){pack_bf2(v[8 * ks], v[8 * ks + 1]), pack_bf2(v[8 * ks + 2], v[8 * ks + 3]),
+                                               pack_bf2(v[8 * ks + 4], v[8 * ks + 5]),
+                                               pack_bf2(v[8 * ks + 6], v[8 * ks + 7])});
 }
 
 // One layer with 64 outputs for kInferGroups groups: K_IN / 32 K-steps of
