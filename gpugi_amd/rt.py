@@ -6,6 +6,7 @@ there is no Python or CPU fallback for any entry point.
 """
 import ctypes
 import os
+import weakref
 
 import numpy as np
 
@@ -13,6 +14,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "libgpugi_rt.so")
 
 GPUGI_OK = 0
+GPUGI_ERR_ARG = -1  # gpugi_status in gpugi_rt.h
+GPUGI_ERR_STATE = -3
 CLOSEST_HIT, ANY_HIT = 0, 1
 KERNEL_AUTO, KERNEL_REFERENCE, KERNEL_WIDE = 0, 1, 2
 PT_MEGAKERNEL, PT_WAVEFRONT = 0, 1
@@ -399,6 +402,27 @@ class LightSampler:
             pass
 
 
+class _PinnedBlock:
+    """One gpugi_host_alloc allocation, freed once (by host_free or when its last array goes)."""
+
+    def __init__(self, ctx, nbytes):
+        p = ctypes.c_void_p()
+        ctx._call("gpugi_host_alloc", ctypes.c_size_t(nbytes), ctypes.byref(p))
+        self.ctx = ctx
+        self.ptr = p.value
+
+    def free(self):
+        if self.ptr and self.ctx._h:
+            self.ctx._call("gpugi_host_free", ctypes.c_void_p(self.ptr))
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
 class DeviceBuffer:
     def __init__(self, ctx, nbytes):
         self.ctx = ctx
@@ -437,6 +461,7 @@ class Context:
         _check("gpugi_create", lib().gpugi_create(device, ctypes.byref(h)))
         self._h = h
         self._scene = None
+        self._pinned = weakref.WeakSet()  # live host_empty allocations
 
     def _call(self, name, *args):
         _check(name, getattr(lib(), name)(self._h, *args), self._h)
@@ -571,15 +596,24 @@ class Context:
 
     def host_empty(self, count, dtype):
         """An uninitialised numpy array of `count` elements in pinned host memory
-        (gpugi_host_alloc); release it with host_free before the context goes."""
+        (gpugi_host_alloc).  The array owns the allocation: it is released when
+        the last view of it goes (or earlier by host_free), and the context
+        refuses to close while such an array is alive."""
         dtype = np.dtype(dtype)
-        p = ctypes.c_void_p()
-        self._call("gpugi_host_alloc", ctypes.c_size_t(max(1, count * dtype.itemsize)), ctypes.byref(p))
-        buf = (ctypes.c_char * (count * dtype.itemsize)).from_address(p.value)
+        block = _PinnedBlock(self, max(1, count * dtype.itemsize))
+        buf = (ctypes.c_char * (count * dtype.itemsize)).from_address(block.ptr)
+        buf._owner = block  # numpy keeps `buf` as the array's base, `buf` keeps the block
+        self._pinned.add(block)
         return np.frombuffer(buf, dtype=dtype, count=count)
 
     def host_free(self, arr):
-        self._call("gpugi_host_free", ctypes.c_void_p(arr.ctypes.data))
+        """Releases a host_empty array's pinned memory now; the array must not be used afterwards."""
+        base = arr
+        while base is not None and not hasattr(base, "_owner"):
+            base = getattr(base, "base", None) if not isinstance(base, memoryview) else base.obj
+        if base is None:
+            raise GpugiError("host_free", GPUGI_ERR_ARG, "not an array from host_empty")
+        base._owner.free()
 
     def trace_importance_break(self, rays, importance, threshold, ray_threshold, query=CLOSEST_HIT):
         """TraceRay with TRACERAY_IMPORTANCE_BREAK: importance (num_nodes, 2) float32, or None
@@ -752,6 +786,10 @@ class Context:
 
     def close(self):
         if self._h:
+            live = [blk for blk in self._pinned if blk.ptr]
+            if live:
+                raise GpugiError("close", GPUGI_ERR_STATE, "%d pinned host arrays from host_empty are still "
+                                 "alive; drop them or release them with host_free first" % len(live))
             lib().gpugi_destroy(self._h)
             self._h = None
 

@@ -41,3 +41,25 @@ def test_chunked_host_trace_equals_device_trace(rt, ctx, n):
     finally:
         dr.free()
         dh.free()
+
+
+def test_pinned_arrays_own_their_memory(rt):
+    """A host_empty array keeps its pinned allocation alive through its views
+    and releases it when the last one goes; the context refuses to close
+    while one is alive, and host_free releases it at once (also from a view)."""
+    c = rt.Context(0)
+    a = c.host_empty(1000, np.float32)
+    view = a[10:20]
+    del a
+    view[:] = 3.0  # the allocation outlives the array it was cut from
+    assert (view == 3.0).all()
+    with pytest.raises(rt.GpugiError):
+        c.close()
+    del view
+    b = c.host_empty(64, np.uint32)
+    c.host_free(b[4:8])
+    c.host_free(b)  # a second release is a no-op
+    with pytest.raises(rt.GpugiError):
+        c.host_free(np.zeros(4, np.float32))
+    del b
+    c.close()
