@@ -220,6 +220,22 @@ def summarize_counters(name, *dirs):
         if "valu_wave_insts_per_cu_cycle" in derived:
             derived["valu_issue_frac_of_probe"] = (derived["valu_wave_insts_per_cu_cycle"] /
                                                    derived["probe_valu_wave_insts_per_cu_cycle"])
+    # vector-memory pipeline: one TA, TD and TCP (L1) per CU, 256 CUs
+    cu_cycles = per["GRBM_GUI_ACTIVE"] / 8.0 * 256.0 if "GRBM_GUI_ACTIVE" in per else None
+    if cu_cycles:
+        for key, label in (("TA_TA_BUSY_sum", "ta_busy_frac"), ("TD_TD_BUSY_sum", "td_busy_frac"),
+                          ("TD_TC_STALL_sum", "td_stalled_by_l1_frac"),
+                          ("TCP_PENDING_STALL_CYCLES_sum", "l1_pending_stall_frac"),
+                          ("TCP_READ_TAGCONFLICT_STALL_CYCLES_sum", "l1_tag_conflict_stall_frac"),
+                          ("TCP_TOTAL_CACHE_ACCESSES_sum", "l1_accesses_per_cu_cycle")):
+            if key in per:
+                derived[label] = per[key] / cu_cycles
+    if "TCP_TOTAL_CACHE_ACCESSES_sum" in per and "TCP_TCC_READ_REQ_sum" in per:
+        derived["l1_hit_rate"] = 1.0 - per["TCP_TCC_READ_REQ_sum"] / per["TCP_TOTAL_CACHE_ACCESSES_sum"]
+    if "SQ_WAIT_INST_ANY" in per and "SQ_WAVE_CYCLES" in per:
+        derived["wave_wait_frac"] = per["SQ_WAIT_INST_ANY"] / per["SQ_WAVE_CYCLES"]
+    if "TCP_TOTAL_CACHE_ACCESSES_sum" in per and "SQ_INSTS_VMEM_RD" in per:
+        derived["l1_accesses_per_vmem_read_instruction"] = per["TCP_TOTAL_CACHE_ACCESSES_sum"] / per["SQ_INSTS_VMEM_RD"]
     if "SQ_THREAD_CYCLES_VALU" in per and "SQ_ACTIVE_INST_VALU" in per:
         derived["valu_thread_cycles_per_active_inst"] = per["SQ_THREAD_CYCLES_VALU"] / per["SQ_ACTIVE_INST_VALU"]
         if "SQ_THREAD_CYCLES_VALU" in probe and "SQ_ACTIVE_INST_VALU" in probe:
