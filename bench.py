@@ -54,6 +54,8 @@ def parse():
                    help="write the profiles/ summaries from rocprofv3 output directories and exit")
     p.add_argument("--summarize-counters", nargs="+", metavar="NAME_THEN_DIRS",
                    help="write profiles/<NAME>.json from rocprofv3 --pmc output directories and exit")
+    p.add_argument("--summarize-wavefront", metavar="KT_DIR",
+                   help="write profiles/wavefront_pt_clutter_kernel_trace.txt from a kernel trace of --clutter and exit")
     p.add_argument("--summarize-mlp", nargs="+", metavar="KT_DIR_THEN_PMC_DIRS",
                    help="write profiles/mlp_kernels.json from a kernel trace and --pmc runs of --mlp-kernels and exit")
     p.add_argument("--sweep", action="store_true",
@@ -600,6 +602,67 @@ def mlp_kernel_benchmark(steps):
                                    "samples; inference over 1920x1080 queries; random features on the device"}}
 
 
+def _short_kernel(name):
+    """Kernel name without namespaces, parameter list and return type."""
+    for ns in ("(anonymous namespace)::", "gpugi::"):
+        name = name.replace(ns, "")
+    name = name.split("(")[0]
+    if name.startswith("void "):
+        name = name[5:]
+    return name
+
+
+def summarize_wavefront(kt_dir):
+    """Writes profiles/wavefront_pt_clutter_kernel_trace.txt: per-kernel totals of a kernel trace of
+    `bench.py --clutter`, and the kernels of its last wavefront path-tracer draw bounce by bounce."""
+    stats = _csv_rows(kt_dir, "_kernel_stats.csv")
+    stats.sort(key=lambda r: -float(r["TotalDurationNs"]))
+    rows = _csv_rows(kt_dir, "_kernel_trace.csv")
+    rows.sort(key=lambda r: int(r["Start_Timestamp"]))
+    names = [_short_kernel(r["Kernel_Name"]) for r in rows]
+    raygen = [i for i, n in enumerate(names) if n == "k_wf_raygen"]
+    if not raygen:
+        raise SystemExit("bench.py --summarize-wavefront: no wavefront draw in the trace")
+    first = raygen[-1]
+    last = next(i for i in range(first, len(names)) if names[i] == "k_wf_finish")
+    dur = [(int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3 for r in rows]
+    lines = ["Wavefront path tracer, clutter room (5,120,016 triangles), 1920x1080, path length 8,",
+             "roulette from bounce 3, wide traversal; every pass reads its queue length on the device.",
+             "",
+             "Commands (repo root; the first after cd /tmp && export TMPDIR=/tmp):",
+             "  rocprofv3 --kernel-trace --stats -d <out>/wfprof -o run --output-format csv -- "
+             "python3 <repo>/bench.py --clutter --clutter-iterations 2 --steps 1",
+             "  python3 bench.py --summarize-wavefront <out>/wfprof",
+             "",
+             "Per-kernel totals over the run (k_trace_wide launches include the trace benchmark before the "
+             "integrators):",
+             "%-60s %6s %10s %10s" % ("kernel", "calls", "total ms", "avg us")]
+    for r in stats[:16]:
+        lines.append("%-60s %6d %10.3f %10.1f" % (_short_kernel(r["Name"])[:60], int(r["Calls"]),
+                                                 float(r["TotalDurationNs"]) / 1e6, float(r["AverageNs"]) / 1e3))
+    lines += ["", "The last wavefront draw, per bounce (kernel durations in us; late bounces launch over the",
+              "full queue capacity and their kernels exit once the device count is reached):"]
+    bounce = 0
+    for i in range(first, last):
+        if names[i] != "k_wf_shade":
+            continue
+        ch = max(j for j in range(first, i) if names[j].startswith("k_trace_wide<false") or
+                 names[j].startswith("k_trace_ref<false"))
+        sh = min((j for j in range(i, last) if names[j].startswith("k_trace_wide<true") or
+                  names[j].startswith("k_trace_ref<true")), default=None)
+        rs = min((j for j in range(i, last) if names[j] == "k_wf_resolve"), default=None)
+        lines.append("  bounce %d:  closest-hit trace %8.1f  shade %7.1f  shadow trace %7.1f  resolve %6.1f" % (
+            bounce, dur[ch], dur[i], dur[sh] if sh is not None else 0.0, dur[rs] if rs is not None else 0.0))
+        bounce += 1
+    span = (int(rows[last]["End_Timestamp"]) - int(rows[first]["Start_Timestamp"])) / 1e6
+    busy = sum(dur[first:last + 1]) / 1e3
+    lines.append("  draw: %.2f ms from the first to the last kernel, %.2f ms of kernel time" % (span, busy))
+    text = "\n".join(lines) + "\n"
+    with open(os.path.join(ROOT, "profiles", "wavefront_pt_clutter_kernel_trace.txt"), "w") as f:
+        f.write(text)
+    print(text)
+
+
 MLP_KERNELS = ("k_mlp_grad", "k_mlp_reduce", "k_mlp_infer")
 
 
@@ -758,6 +821,9 @@ def main():
         return
     if args.summarize_mlp:
         summarize_mlp(*args.summarize_mlp)
+        return
+    if args.summarize_wavefront:
+        summarize_wavefront(args.summarize_wavefront)
         return
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; --gpus %d is not supported (ray sets and pixels would "
