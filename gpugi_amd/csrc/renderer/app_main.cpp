@@ -7,7 +7,6 @@
 // Instead of drawing once per window frame, `waitIterations = n` draws n
 // iterations.  Usage:
 //   gpugi_render [--device N] [script.txt] [-e 'command'] ...
-#include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -82,13 +81,11 @@ const char* kHelp =
     "bvhType = 0 | 1                       0: axis-aligned, 1: oriented boxes for the next scene load\n"
     "addPointLight = x, y, z, r, g, b      add a point light to the scene\n"
     "waitIterations = n                    draw n iterations\n"
-    "waitSeconds = s                       draw until s seconds have passed\n"
     "screenshot / namedscreenshot = file   write the image (sum / iterations) as PFM\n"
     "envMap = size, x-, x+, y-, y+, z-, z+  environment cube map from six size x size binary PPM faces\n"
     "referenceImage = file.pfm             reference image of the MSE checks\n"
     "mseCheckInterval = n                  compute and print the MSE every n-th iteration (0: never)\n"
     "waitMSE = x                           draw until the MSE is below x (needs the two above)\n"
-    "tic / toc                             reset / print the stopwatch\n"
     "help                                  this text\n"
     "exit                                  stop processing\n";
 
@@ -102,10 +99,6 @@ class App {
       std::cout << kHelp;
     } else if (c.name == "exit") {
       return false;
-    } else if (c.name == "tic") {
-      m_tic = std::chrono::steady_clock::now();
-    } else if (c.name == "toc") {
-      std::cout << std::chrono::duration<double>(std::chrono::steady_clock::now() - m_tic).count() << "s\n";
     } else if (c.name == "bvhType") {
       NeedArgs(c, 1);
       const int t = AsInt(c.args[0]);
@@ -148,11 +141,6 @@ class App {
       NeedArgs(c, 1);
       const int n = AsInt(c.args[0]);
       for (int i = 0; i < n; ++i) Draw();
-    } else if (c.name == "waitSeconds") {
-      NeedArgs(c, 1);
-      const double s = AsFloat(c.args[0]);
-      const auto t0 = std::chrono::steady_clock::now();
-      while (std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() < s) Draw();
     } else if (c.name == "envMap") {
       NeedArgs(c, 7);
       System().SetEnvironmentMap(AsInt(c.args[0]), c.args[1], c.args[2], c.args[3], c.args[4], c.args[5],
@@ -289,7 +277,6 @@ class App {
   unsigned m_iterationSinceLastMseCheck = 0;
   std::shared_ptr<gpugi::Scene> m_scene;
   gpugi::Camera m_camera;
-  std::chrono::steady_clock::time_point m_tic = std::chrono::steady_clock::now();
 };
 
 }  // namespace
