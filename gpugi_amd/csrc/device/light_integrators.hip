@@ -186,13 +186,13 @@ __device__ __forceinline__ bool lt_bounce(const DrawParams& p, const TraceCtx& c
   return s.bounce >= p.maxPathLength;
 }
 
-// All numPaths light paths with path regeneration (persistent_items).
+// All numPaths light paths with path regeneration (persistent_items).  At
+// least 4 waves per SIMD: the BPT light pass otherwise takes 129 VGPRs and 3
+// waves, and the cap costs no scratch.
 template <bool WIDE, int MODE>
-__global__ __launch_bounds__(kLightBlock) void k_lighttracer(DrawParams p, int raysPerSample, uint32_t numPaths,
-                                                             LightCacheEntryDev* __restrict__ cache,
-                                                             LightCacheEntryDev* __restrict__ staging,
-                                                             uint32_t* __restrict__ pathCounts,
-                                                             uint32_t* __restrict__ queue) {
+__global__ __launch_bounds__(kLightBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_lighttracer(
+    DrawParams p, int raysPerSample, uint32_t numPaths, LightCacheEntryDev* __restrict__ cache,
+    LightCacheEntryDev* __restrict__ staging, uint32_t* __restrict__ pathCounts, uint32_t* __restrict__ queue) {
   __shared__ uint2 stk[WIDE ? kWideLdsStack * kLightBlock : 1];
   const int tid = threadIdx.x;
   const uint32_t linear = blockIdx.x * kLightBlock + tid;

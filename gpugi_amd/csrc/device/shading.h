@@ -263,27 +263,34 @@ __device__ __forceinline__ float2 hit_texcoord(const float4* __restrict__ vinfo,
   return make_float2(a.z * b.x + c.z * b.y + e.z * b.z, a.w * b.x + c.w * b.y + e.w * b.z);
 }
 
+// One texture sample of a textured material slot, kept out of line (only
+// scenes with textures call it); a float4 comes back in registers, so the
+// caller's MaterialData never has to live in memory.
+__device__ __noinline__ float4 sample_material_slot(MaterialTextures tx, int texture, float2 uv) {
+  return sample_texture(tx, texture, uv);
+}
+
 // SampleMaterialData (materials.glsl:19-30) at texture coordinate uv for a
 // scene with textures: the material record's constants, with each textured
-// slot sampled instead.  Kept out of line: only textured scenes call it.
-__device__ __noinline__ MaterialData load_material_textured(const gpugi_material* __restrict__ m,
-                                                            MaterialTextures tx, int id, float2 uv) {
+// slot sampled instead.
+__device__ __forceinline__ MaterialData load_material_textured(const gpugi_material* __restrict__ m,
+                                                               const MaterialTextures& tx, int id, float2 uv) {
   MaterialData d = load_material(m, id);
   const int4 t = tx.matTex[id];
   if (t.x >= 0) {
-    const float4 c = sample_texture(tx, t.x, uv);
+    const float4 c = sample_material_slot(tx, t.x, uv);
     d.diffuse = gg::v3(c.x, c.y, c.z);
   }
   if (t.y >= 0) {
-    const float4 c = sample_texture(tx, t.y, uv);
+    const float4 c = sample_material_slot(tx, t.y, uv);
     d.opacity = gg::v3(c.x, c.y, c.z);
   }
   if (t.z >= 0) {
-    const float4 c = sample_texture(tx, t.z, uv);
+    const float4 c = sample_material_slot(tx, t.z, uv);
     d.refl[0] = c.x; d.refl[1] = c.y; d.refl[2] = c.z; d.refl[3] = c.w;
   }
   if (t.w >= 0) {
-    const float4 c = sample_texture(tx, t.w, uv);
+    const float4 c = sample_material_slot(tx, t.w, uv);
     d.emissivity = gg::v3(c.x, c.y, c.z);
   }
   return d;
