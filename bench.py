@@ -54,6 +54,10 @@ def parse():
                    help="write the profiles/ summaries from rocprofv3 output directories and exit")
     p.add_argument("--summarize-counters", nargs="+", metavar="NAME_THEN_DIRS",
                    help="write profiles/<NAME>.json from rocprofv3 --pmc output directories and exit")
+    p.add_argument("--summarize-kernels", nargs=2, metavar=("NAME", "KT_DIR"),
+                   help="write profiles/NAME.txt: per-kernel totals of a kernel trace of bench.py with the "
+                        "arguments given by --profiled-args=..., and exit")
+    p.add_argument("--profiled-args", default="", help="the bench.py arguments of the traced run (--summarize-kernels)")
     p.add_argument("--summarize-wavefront", metavar="KT_DIR",
                    help="write profiles/wavefront_pt_clutter_kernel_trace.txt from a kernel trace of --clutter and exit")
     p.add_argument("--summarize-mlp", nargs="+", metavar="KT_DIR_THEN_PMC_DIRS",
@@ -69,6 +73,10 @@ def parse():
                    help="time the cache's training step (16,384 samples) and inference (1920x1080 queries) "
                         "on device-resident samples and report the BF16 matrix-core roofline")
     p.add_argument("--mlp-kernel-steps", type=int, default=50, help="timed training steps / inference launches")
+    p.add_argument("--hierarchy", action="store_true",
+                   help="HierarchyImportance passes on the 2,097,152-triangle heightfield at 1920x1080: "
+                        "importance acquisition + propagation, then node materials")
+    p.add_argument("--hierarchy-iterations", type=int, default=10, help="acquisition passes per update")
     p.add_argument("--batch", action="store_true",
                    help="batched rendering of 16 scenes: per-scene launches vs persistent launch vs hipGraph replay")
     p.add_argument("--batch-frames", type=int, default=10, help="frames timed per batched mode")
@@ -311,6 +319,47 @@ def batch_benchmark(frames):
                    "pixels": [int(s[2] * s[3]) for s in scenes]},
         "setup_s": round(setup_s, 2),
     }
+
+
+def hierarchy_benchmark(iterations, repeats=3):
+    """HierarchyImportance::Draw's importance update (acquisition2.comp x `iterations`, one path
+    of up to 8 vertices per pixel each, then the propagation passes) and
+    ComputeHierarchyMaterials on the heightfield; GPU time by HIP events, best of `repeats`."""
+    from gpugi_amd import rt, workloads
+    W, H = workloads.HF_WIDTH, workloads.HF_HEIGHT
+    hs = rt.HostScene.heightfield(workloads.HF_CELLS, workloads.MASTER_SEED)
+    for pos in workloads.heightfield_lights():
+        hs.add_point_light(pos, (0.5, 0.5, 0.5))
+    ctx = rt.Context(0)
+    ctx.load_scene(hs)
+    ctx.set_screen(W, H)
+    ctx.set_camera(workloads.heightfield_camera(W, H))
+    ctx.set_light_samples(rt.LightSampler(hs, 123).generate(128))
+    d = hs.desc()
+
+    def timed(fn):
+        fn()  # warm-up (allocates the buffers)
+        best = None
+        for _ in range(repeats):
+            ctx.sync()
+            ctx.timer_begin()
+            fn()
+            ms = ctx.timer_end()
+            best = ms if best is None else min(best, ms)
+        return best
+
+    upd = timed(lambda: ctx.hierarchy_update_importance(iterations))
+    mat = timed(ctx.hierarchy_compute_materials)
+    imp, sub = ctx.hierarchy_read()
+    ctx.close()
+    paths = iterations * (W // 8 * 8) * (H // 8 * 8)
+    return {"metric": "HierarchyImportance importance update, Mpaths/s", "value": round(paths / (upd * 1e3), 2),
+            "unit": "Mpaths/s", "higher_is_better": True,
+            "update_ms": round(upd, 3), "acquisition_iterations": iterations, "acquisition_paths": paths,
+            "materials_ms": round(mat, 3), "nodes": int(d.num_nodes), "triangle_slots": int(d.num_leaves * 2),
+            "root_importance": float(imp[0, 0]), "root_subtree": float(sub[0]),
+            "config": {"workload": "heightfield 1024x1024 cells (2,097,152 triangles), 4 point lights, 128 light "
+                                   "samples, 1920x1080, reference escape walk"}}
 
 
 def run_valu_probe(ctx):
@@ -612,6 +661,27 @@ def _short_kernel(name):
     return name
 
 
+def summarize_kernels(name, kt_dir, bench_args):
+    """Writes profiles/<name>.txt: every kernel of a `rocprofv3 --kernel-trace --stats` run of
+    `bench.py <bench_args>` with its calls, total and average duration."""
+    stats = _csv_rows(kt_dir, "_kernel_stats.csv")
+    stats.sort(key=lambda r: -float(r["TotalDurationNs"]))
+    lines = ["Kernel trace of bench.py %s on one MI355X." % bench_args, "",
+             "Commands (repo root; the first after cd /tmp && export TMPDIR=/tmp):",
+             "  rocprofv3 --kernel-trace --stats -d <out>/kt -o kt --output-format csv -- python3 <repo>/bench.py "
+             + bench_args,
+             "  python3 bench.py --summarize-kernels %s <out>/kt --profiled-args='%s'" % (name, bench_args), "",
+             "%-60s %6s %12s %12s %12s" % ("kernel", "calls", "total ms", "avg us", "max us")]
+    for r in stats:
+        lines.append("%-60s %6d %12.3f %12.1f %12.1f" % (_short_kernel(r["Name"])[:60], int(r["Calls"]),
+                                                         float(r["TotalDurationNs"]) / 1e6,
+                                                         float(r["AverageNs"]) / 1e3, float(r["MaxNs"]) / 1e3))
+    text = "\n".join(lines) + "\n"
+    with open(os.path.join(ROOT, "profiles", name + ".txt"), "w") as f:
+        f.write(text)
+    print(text)
+
+
 def summarize_wavefront(kt_dir):
     """Writes profiles/wavefront_pt_clutter_kernel_trace.txt: per-kernel totals of a kernel trace of
     `bench.py --clutter`, and the kernels of its last wavefront path-tracer draw bounce by bounce."""
@@ -825,6 +895,9 @@ def main():
     if args.summarize_wavefront:
         summarize_wavefront(args.summarize_wavefront)
         return
+    if args.summarize_kernels:
+        summarize_kernels(*args.summarize_kernels, args.profiled_args)
+        return
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; --gpus %d is not supported (ray sets and pixels would "
               "shard across GPUs with no communication during traversal)" % args.gpus, file=sys.stderr)
@@ -834,6 +907,9 @@ def main():
         return
     if args.mlp:
         print(json.dumps(mlp_benchmark(args.mlp_ref_spp, args.mlp_frames, args.mlp_warmup)))
+        return
+    if args.hierarchy:
+        print(json.dumps(hierarchy_benchmark(args.hierarchy_iterations)))
         return
     if args.batch:
         print(json.dumps(batch_benchmark(args.batch_frames)))

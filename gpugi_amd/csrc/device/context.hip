@@ -1206,12 +1206,14 @@ int gpugi_hierarchy_update_importance(gpugi_ctx* ctx, int iterations) {
   if (int rc = EnsureHierarchy(ctx)) return rc;
   ctx->hasHierImportance = false;
   const DrawParams p = MakeDrawParams(ctx, 0);
-  float2* imp = static_cast<float2*>(ctx->dHierImp);
-  HIP_TRY(ctx, hipMemsetAsync(imp, 0, ((size_t)ctx->numNodes + ctx->numSlots) * 8, ctx->stream));
+  const size_t entries = (size_t)ctx->numNodes + ctx->numSlots;
+  unsigned long long* acc = static_cast<unsigned long long*>(ctx->dHierImp);
+  HIP_TRY(ctx, hipMemsetAsync(acc, 0, entries * 8, ctx->stream));
   for (int i = 1; i <= iterations; ++i)  // NumImportanceIterations = i (hierarchyimportance.cpp:93-103)
-    HIP_TRY(ctx, LaunchHierAcquire(p, i, imp, ctx->numNodes, ctx->stream));
-  HIP_TRY(ctx, LaunchHierPropagate(p.ref, ctx->hierTree, iterations, imp, static_cast<float*>(ctx->dHierSubtree),
-                                   ctx->stream));
+    HIP_TRY(ctx, LaunchHierAcquire(p, i, acc, ctx->numNodes, ctx->stream));
+  HIP_TRY(ctx, LaunchHierToFloat(acc, entries, ctx->stream));
+  HIP_TRY(ctx, LaunchHierPropagate(p.ref, ctx->hierTree, iterations, static_cast<float2*>(ctx->dHierImp),
+                                   static_cast<float*>(ctx->dHierSubtree), ctx->stream));
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   ctx->hasHierImportance = true;
   return GPUGI_OK;

@@ -29,9 +29,12 @@ bool BuildHierTree(const float* hierarchy, uint32_t numNodes, std::vector<int32_
                    std::vector<uint32_t>* levelOrder, std::vector<uint32_t>* levelStart);
 
 // One acquisition pass (iteration: NumImportanceIterations, 1-based) adding into
-// importance (numNodes + triangle slots entries {x, y}).
-hipError_t LaunchHierAcquire(const DrawParams& p, int iteration, float2* importance, uint32_t numNodes,
+// importance (numNodes + triangle slots 64-bit fixed-point sums, zeroed before
+// the first pass).
+hipError_t LaunchHierAcquire(const DrawParams& p, int iteration, unsigned long long* importance, uint32_t numNodes,
                              hipStream_t stream);
+// The fixed-point sums of `entries` entries to FP32 {sum, 0} pairs, in place.
+hipError_t LaunchHierToFloat(unsigned long long* importance, size_t entries, hipStream_t stream);
 // Propagation over the tree: importance {x, y} per node (x divided by the
 // iteration count), subtree importance per node.
 hipError_t LaunchHierPropagate(const RefScene& s, const HierTree& tree, int iterations, float2* importance,

@@ -41,9 +41,32 @@ struct NodeView {
   __device__ __forceinline__ float4 t1(int n) const { return nodes[(obox ? 3 : 2) * n + 1]; }
 };
 
+// The acquisition sums each entry's importance in 64-bit fixed point, in
+// units of 2^-kImpFracBits (the smallest addend: the path importance halves at
+// each of the kHierPathLength vertices and once more for the light sample), so
+// every sum is exact and independent of the order of the additions.
+constexpr int kImpFracBits = 8;
+static_assert(kHierPathLength <= kImpFracBits, "every addend is a whole number of units");
+
+// acc[key] += w for every calling lane, the lanes that share the lowest active
+// lane's (key, w) combined into one atomic of w times their number: every ray
+// of a wave enters the top nodes, which would otherwise take one contended
+// atomic per ray.
+__device__ __forceinline__ void add_importance(unsigned long long* __restrict__ acc, uint32_t key, float w) {
+  const uint64_t active = __ballot(true);
+  const int leader = __builtin_ctzll(active);  // the calling lane is active, so active != 0
+  const uint32_t lk = (uint32_t)__shfl((int)key, leader);
+  const float lw = __shfl(w, leader);
+  const bool same = key == lk && w == lw;
+  const uint64_t group = __ballot(same);
+  const unsigned long long units = (unsigned long long)(w * (float)(1 << kImpFracBits));  // exact: w = 2^-k
+  if (!same) atomicAdd(&acc[key], units);
+  else if ((int)(threadIdx.x & 63) == leader) atomicAdd(&acc[key], units * (unsigned long long)__popcll(group));
+}
+
 // TraceRayCountHits (counthits.glsl:1-71) over the reference layout.
 __device__ void count_hits(const RefScene& s, gg::V3 o, gg::V3 d, float rayLength, float w,
-                           float2* __restrict__ imp, uint32_t numNodes) {
+                           unsigned long long* __restrict__ imp, uint32_t numNodes) {
   int node = 0, leaf = 0;
   bool nextIsLeaf = false;
   const gg::V3 inv = gg::v3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
@@ -64,7 +87,7 @@ __device__ void count_hits(const RefScene& s, gg::V3 o, gg::V3 d, float rayLengt
       const uint32_t childCode = __float_as_uint(a.w);
       const int escape = __float_as_int(b.w);
       if (boxHit && newHit <= rayLength) {
-        atomicAdd(&imp[node].x, w);
+        add_importance(imp, (uint32_t)node, w);
         const int child = (int)(childCode & 0x7fffffffu);
         nextIsLeaf = (uint32_t)child != childCode;
         if (nextIsLeaf) {
@@ -87,7 +110,7 @@ __device__ void count_hits(const RefScene& s, gg::V3 o, gg::V3 d, float rayLengt
         if (intersect_triangle(o, d, load_pos(s.positions, tri.x), load_pos(s.positions, tri.y),
                                load_pos(s.positions, tri.z), newHit, bary, n) &&
             newHit <= rayLength)
-          atomicAdd(&imp[numNodes + (uint32_t)leaf].x, w);
+          add_importance(imp, numNodes + (uint32_t)leaf, w);
         ++leaf;
         nextIsLeaf = (leaf % 2) != 0;
       }
@@ -96,7 +119,7 @@ __device__ void count_hits(const RefScene& s, gg::V3 o, gg::V3 d, float rayLengt
 }
 
 __global__ __launch_bounds__(kHierBlock) void k_hier_acquire(DrawParams p, uint32_t frameSeed,
-                                                             float2* __restrict__ imp, uint32_t numNodes,
+                                                             unsigned long long* __restrict__ imp, uint32_t numNodes,
                                                              uint32_t tilesW, uint32_t numPixels) {
   const uint32_t i = blockIdx.x * kHierBlock + threadIdx.x;
   if (i >= numPixels) return;
@@ -145,6 +168,16 @@ __global__ __launch_bounds__(kHierBlock) void k_hier_acquire(DrawParams p, uint3
     d = sample_bsdf(d, m, seed, n, tp, false, pdf, diffuse);
     o = o + d * gg::kRayHitEpsilon;
   }
+}
+
+// The fixed-point sums to the FP32 {sum, 0} pairs the propagation reads (in
+// place: each entry is 8 bytes either way).  The conversion is exact while a
+// sum stays below 2^24 units.
+__global__ __launch_bounds__(kHierBlock) void k_hier_to_float(unsigned long long* __restrict__ acc, size_t n) {
+  const size_t i = (size_t)blockIdx.x * kHierBlock + threadIdx.x;
+  if (i >= n) return;
+  const float x = (float)acc[i] * (1.0f / (float)(1 << kImpFracBits));
+  reinterpret_cast<float2*>(acc)[i] = make_float2(x, 0.0f);
 }
 
 // avgArea of hierarchypropagation_*.comp: a third of the box's surface
@@ -342,7 +375,7 @@ bool BuildHierTree(const float* hierarchy, uint32_t numNodes, std::vector<int32_
   return true;
 }
 
-hipError_t LaunchHierAcquire(const DrawParams& p, int iteration, float2* importance, uint32_t numNodes,
+hipError_t LaunchHierAcquire(const DrawParams& p, int iteration, unsigned long long* importance, uint32_t numNodes,
                              hipStream_t stream) {
   const uint32_t tilesW = (uint32_t)(p.width / kTile), tilesH = (uint32_t)(p.height / kTile);
   const uint32_t numPixels = tilesW * tilesH * kTileThreads;
@@ -350,6 +383,12 @@ hipError_t LaunchHierAcquire(const DrawParams& p, int iteration, float2* importa
   if (p.numLights == 0 || !p.lights) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_hier_acquire, Blocks(numPixels), dim3(kHierBlock), 0, stream, p,
                      gg::wanghash((uint32_t)iteration), importance, numNodes, tilesW, numPixels);
+  return hipGetLastError();
+}
+
+hipError_t LaunchHierToFloat(unsigned long long* importance, size_t entries, hipStream_t stream) {
+  if (entries == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_hier_to_float, Blocks(entries), dim3(kHierBlock), 0, stream, importance, entries);
   return hipGetLastError();
 }
 

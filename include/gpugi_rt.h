@@ -359,9 +359,12 @@ typedef struct gpugi_node_hit {
  * triangles (or children) received, y = (x / subtree)^2 * a third of the box
  * surface.  The reference uses 10 iterations.  Runs the reference walk and
  * triangle test; blocks until done.  The result stays on the device for
- * gpugi_trace_importance_break and gpugi_hierarchy_read.  The GPU sums with
- * float atomics: the values are exact sums of powers of two up to 2^16 rays
- * per node, beyond which the order of the additions shows in the last bits. */
+ * gpugi_trace_importance_break and gpugi_hierarchy_read.  The sums are exact
+ * and independent of the order of the additions (64-bit fixed point in units
+ * of 2^-8, the smallest addend), so they equal the reference's FP32 sums
+ * wherever those are exact, i.e. below 2^16 rays per node; beyond that an FP32
+ * sum drops addends (1 + 2^24 rounds to 2^24), so the most-visited nodes of a
+ * large scene get larger, exact values here. */
 int gpugi_hierarchy_update_importance(gpugi_ctx* ctx, int iterations);
 /* HierarchyImportance::ComputeHierarchyMaterials (hierarchyimportance.cpp:154-178):
  * per leaf node the area-weighted material of its triangles sampled at their

@@ -2,10 +2,11 @@
 
 gpugi_hierarchy_update_importance (acquisition2.comp + propagation) and
 gpugi_hierarchy_compute_materials (sampleleafmaterials.comp + pullmaterials.comp)
-must equal the oracle bit for bit: the acquisition adds powers of two, whose
-float sums are exact in any order while a node sees fewer than 2^16 rays, and
-the propagation and material passes are the same FP32 and half operations in
-the same order.  The importance then drives gpugi_trace_importance_break from
+must equal the oracle bit for bit: the acquisition sums powers of two in exact
+fixed point, which equals the oracle's FP32 sums while a node sees fewer than
+2^16 rays (those are exact too), and the propagation and material passes are
+the same FP32 and half operations in the same order.  Beyond that range the
+sums stay exact, so two updates give the same bits.  The importance then drives gpugi_trace_importance_break from
 the device and must give the oracle's importance-break walk.
 """
 import numpy as np
@@ -75,3 +76,28 @@ def test_hierarchy_state_errors(rt, ctx):
         ctx.hierarchy_read()  # nothing computed for this scene
     with pytest.raises(rt.GpugiError):
         ctx.hierarchy_update_importance(0)
+
+
+def test_importance_sums_are_exact_at_scale(rt, ctx):
+    """On a 256 x 256-cell heightfield at 512 x 512 pixels the root sees about
+    2^22 rays per update, far past the range where FP32 sums are exact: the
+    fixed-point sums still give the same bits twice, and every camera ray of
+    every iteration enters the root box with importance 1."""
+    hs = rt.HostScene.heightfield(256, workloads.MASTER_SEED)
+    for p in workloads.heightfield_lights():
+        hs.add_point_light(p, (0.5, 0.5, 0.5))
+    w = h = 512
+    ctx.set_bvh_type(rt.BVH_AABOX)
+    ctx.load_scene(hs)
+    ctx.set_screen(w, h)
+    ctx.set_camera(workloads.heightfield_camera(w, h))
+    ctx.set_light_samples(rt.LightSampler(hs, 5).generate(128))
+    print("case: hierarchy importance at scale", flush=True)
+    runs = []
+    for _ in range(2):
+        ctx.hierarchy_update_importance(10)
+        runs.append(ctx.hierarchy_read())
+    (imp1, sub1), (imp2, sub2) = runs
+    assert np.array_equal(imp1.view(np.uint32), imp2.view(np.uint32))
+    assert np.array_equal(sub1.view(np.uint32), sub2.view(np.uint32))
+    assert imp1[0, 0] >= w * h  # per iteration: every camera ray, importance 1
