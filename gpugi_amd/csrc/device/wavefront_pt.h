@@ -10,10 +10,8 @@ namespace gpugi {
 // Per-pixel path state and per-queue-slot buffers, all in one allocation.
 struct WavefrontBuffers {
   uint32_t* counters;  // [0], [1]: lengths of the two path queues
-  float4* color;       // per pixel
-  float4* throughput;  // per pixel
-  uint32_t* seed;      // per pixel
-  uint32_t* flags;     // per pixel: bounce << 1 | didNEE
+  float4* colorSeed;   // per pixel: color.xyz, w = the random state's bits
+  float4* tpFlags;     // per pixel: throughput.xyz, w = bits of bounce << 1 | didNEE
   uint32_t* queue[2];  // pixel indices of the live paths
   float4* rays[2];     // their rays (gpugi_ray layout), in queue order
   int4* hits;          // closest-hit records of the current queue

@@ -44,10 +44,8 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_raygen(DrawParams p, WavefrontB
   const uint32_t pix = (uint32_t)gy * (uint32_t)p.width + (uint32_t)gx;
   PtPath s;
   pt_begin(p, s, gx, gy, p.frameSeed);
-  b.color[pix] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-  b.throughput[pix] = make_float4(s.throughput.x, s.throughput.y, s.throughput.z, 0.0f);
-  b.seed[pix] = s.seed;
-  b.flags[pix] = 0u;  // bounce 0, no NEE before
+  b.colorSeed[pix] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(s.seed));
+  b.tpFlags[pix] = make_float4(s.throughput.x, s.throughput.y, s.throughput.z, __uint_as_float(0u));  // bounce 0
   b.queue[0][i] = pix;
   b.rays[0][2 * (size_t)i] = make_float4(s.o.x, s.o.y, s.o.z, gg::kRayMax);
   b.rays[0][2 * (size_t)i + 1] = make_float4(s.d.x, s.d.y, s.d.z, 0.0f);
@@ -64,19 +62,21 @@ __device__ __forceinline__ bool wf_shade_one(const DrawParams& p, const Wavefron
   const int4 rec = b.hits[i];
   const ClosestHit h = closest_from_record(p, rec.x, __int_as_float(rec.y), __int_as_float(rec.z),
                                            __int_as_float(rec.w));
-  const float4 c4 = b.color[pix], t4 = b.throughput[pix];
+  // the path's state: two 16-byte records (color and random state, throughput and flags)
+  const float4 c4 = b.colorSeed[pix], t4 = b.tpFlags[pix];
   gg::V3 color = gg::v3(c4.x, c4.y, c4.z), throughput = gg::v3(t4.x, t4.y, t4.z);
-  uint32_t seed = b.seed[pix];
-  const uint32_t flags = b.flags[pix];
+  uint32_t seed = __float_as_uint(c4.w);
+  const uint32_t flags = __float_as_uint(t4.w);
   const int bounce = (int)(flags >> 1);
   bool didNEE = (flags & 1u) != 0;
-  // tmax 0 where there is nothing to connect (its result is not read)
-  b.shadowRays[2 * (size_t)i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-  b.shadowRays[2 * (size_t)i + 1] = make_float4(0.0f, 0.0f, 1.0f, 0.0f);
+  // the shadow ray, written once: tmax 0 where there is nothing to connect (its result is not read)
+  float4 sr0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), sr1 = make_float4(0.0f, 0.0f, 1.0f, 0.0f);
   if (!h.hit) {
     // environment on a miss (pathtracer.comp:56-58); the path ends, no NEE term
     color = color + throughput * (p.env ? env_radiance(p.env, p.envSize, d) : gg::v3s(0.0f));
-    b.color[pix] = make_float4(color.x, color.y, color.z, 0.0f);
+    b.colorSeed[pix] = make_float4(color.x, color.y, color.z, c4.w);
+    b.shadowRays[2 * (size_t)i] = sr0;
+    b.shadowRays[2 * (size_t)i + 1] = sr1;
     b.neeValue[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     return false;
   }
@@ -92,8 +92,8 @@ __device__ __forceinline__ bool wf_shade_one(const DrawParams& p, const Wavefron
     const int sampleIndex = (int)(gg::xorshift32(seed) % p.numLights);
     const NeeSample ns = nee_sample(p, o, n, sampleIndex, d, m, didNEE);
     if (ns.test) {
-      b.shadowRays[2 * (size_t)i] = make_float4(ns.o.x, ns.o.y, ns.o.z, ns.len);
-      b.shadowRays[2 * (size_t)i + 1] = make_float4(ns.dir.x, ns.dir.y, ns.dir.z, 0.0f);
+      sr0 = make_float4(ns.o.x, ns.o.y, ns.o.z, ns.len);
+      sr1 = make_float4(ns.dir.x, ns.dir.y, ns.dir.z, 0.0f);
     }
     // w = 1: a term is added; value.xyz counts only when the shadow ray is not blocked
     b.neeValue[i] = make_float4(ns.value.x, ns.value.y, ns.value.z, ns.test ? 1.0f : 0.5f);
@@ -101,7 +101,8 @@ __device__ __forceinline__ bool wf_shade_one(const DrawParams& p, const Wavefron
   } else {
     b.neeValue[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);  // no light sample at this bounce
   }
-  b.color[pix] = make_float4(color.x, color.y, color.z, 0.0f);
+  b.shadowRays[2 * (size_t)i] = sr0;
+  b.shadowRays[2 * (size_t)i + 1] = sr1;
   gg::V3 tp = gg::v3s(1.0f);
   float pdf;
   bool diffuse;
@@ -117,9 +118,9 @@ __device__ __forceinline__ bool wf_shade_one(const DrawParams& p, const Wavefron
       throughput = throughput / q;
     }
   }
-  b.seed[pix] = seed;
-  b.throughput[pix] = make_float4(throughput.x, throughput.y, throughput.z, 0.0f);
-  b.flags[pix] = ((uint32_t)(bounce + 1) << 1) | (didNEE ? 1u : 0u);
+  b.colorSeed[pix] = make_float4(color.x, color.y, color.z, __uint_as_float(seed));
+  b.tpFlags[pix] = make_float4(throughput.x, throughput.y, throughput.z,
+                               __uint_as_float(((uint32_t)(bounce + 1) << 1) | (didNEE ? 1u : 0u)));
   oOut = o;
   dOut = d;
   return alive && bounce + 1 < p.maxPathLength;
@@ -163,9 +164,9 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_resolve(WavefrontBuffers b, int
     const gg::V3 radiance = visible ? gg::v3(v.x, v.y, v.z) : gg::v3s(0.0f);
     const float4 w = b.neeWeight[i];
     const uint32_t pix = b.queue[cur][i];
-    const float4 c4 = b.color[pix];
+    const float4 c4 = b.colorSeed[pix];
     const gg::V3 color = gg::v3(c4.x, c4.y, c4.z) + radiance * gg::v3(w.x, w.y, w.z) * w.w;
-    b.color[pix] = make_float4(color.x, color.y, color.z, 0.0f);
+    b.colorSeed[pix] = make_float4(color.x, color.y, color.z, c4.w);
   }
 }
 
@@ -176,7 +177,7 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_finish(DrawParams p, WavefrontB
   int gx, gy;
   tile_pixel(i, tilesW, gx, gy);
   const uint32_t pix = (uint32_t)gy * (uint32_t)p.width + (uint32_t)gx;
-  const float4 c = b.color[pix];
+  const float4 c = b.colorSeed[pix];
   float4* px = &p.accum[pix];
   const float4 a = *px;
   *px = make_float4(c.x + a.x, c.y + a.y, c.z + a.z, 0.0f + a.w);  // vec4(color, 0) + imageLoad
@@ -194,9 +195,9 @@ dim3 Grid(uint32_t n) { return dim3((n + kWfBlock - 1) / kWfBlock); }
 }  // namespace
 
 size_t WavefrontBytes(size_t numPixels) {
-  // color, throughput (16 B), seed, flags (4 B) per pixel; per queue slot: two
-  // queues and two ray arrays, hits, shadow rays, occlusion, NEE value and weight
-  return numPixels * (16 + 16 + 4 + 4) + numPixels * (4 * 2 + 32 * 2 + 16 + 32 + 16 + 16 + 16) +
+  // color and random state, throughput and flags (16 B each) per pixel; per queue
+  // slot: two queues and two ray arrays, hits, shadow rays, occlusion, NEE value and weight
+  return numPixels * (16 + 16) + numPixels * (4 * 2 + 32 * 2 + 16 + 32 + 16 + 16 + 16) +
          16 * 256;  // the counters and each buffer's rounding to 256 bytes
 }
 
@@ -209,10 +210,8 @@ WavefrontBuffers CarveWavefront(void* base, size_t numPixels) {
   };
   WavefrontBuffers b;
   b.counters = (uint32_t*)take(256);
-  b.color = (float4*)take(numPixels * 16);
-  b.throughput = (float4*)take(numPixels * 16);
-  b.seed = (uint32_t*)take(numPixels * 4);
-  b.flags = (uint32_t*)take(numPixels * 4);
+  b.colorSeed = (float4*)take(numPixels * 16);
+  b.tpFlags = (float4*)take(numPixels * 16);
   for (int k = 0; k < 2; ++k) {
     b.queue[k] = (uint32_t*)take(numPixels * 4);
     b.rays[k] = (float4*)take(numPixels * 32);
