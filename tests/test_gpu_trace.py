@@ -227,3 +227,52 @@ def test_autotune_trace_keeps_answers(rt, oracle, ctx, query):
             assert (got[:, 0] != want).sum() <= max(2, 1e-3 * n)
     finally:
         ctx.set_wide_config(2, 10, 8, 16)
+
+
+@pytest.mark.parametrize("scene", ["heightfield", "clutter"])
+def test_full_size_sets_wide_against_reference_walk(rt, oracle, ctx, scene):
+    """Every ray of the three 2^24-ray sets on the 2,097,152-triangle heightfield
+    and on the 5,120,016-triangle clutter room (its camera and lights at
+    1920x1080): the wide kernel's record equals the reference walk's (itself
+    bit-exact with the oracle) except on rays the FP64 classifier flags, at most
+    1e-3 of a set.  For any hit, a differing ray's closest hit (oracle walk)
+    must be flagged."""
+    if scene == "heightfield":
+        hs = rt.HostScene.heightfield(1024, workloads.MASTER_SEED)
+    else:
+        hs = rt.HostScene.clutter(workloads.CLUTTER_SPHERES, workloads.MASTER_SEED)
+    ctx.load_scene(hs)
+    sc = oracle.Scene(hs.arrays())
+    if scene == "heightfield":
+        R = workloads.HeightfieldRays(ctx)
+    else:
+        w, h = workloads.HF_WIDTH, workloads.HF_HEIGHT
+        R = workloads.SceneRays(ctx, workloads.clutter_camera(w, h), workloads.CLUTTER_LIGHTS, workloads.HF_RAYS,
+                                workloads.MASTER_SEED, w, h)
+    n = R.n
+    try:
+        for name, buf, query in (("incoherent", R.incoherent, rt.CLOSEST_HIT), ("coherent", R.primary, rt.CLOSEST_HIT),
+                                 ("shadow", R.shadow, rt.ANY_HIT)):
+            print("case: full-size %s %s set" % (scene, name), flush=True)
+            ctx.trace_device(buf, R.hits, n, query, rt.KERNEL_REFERENCE)
+            ref = R.hits.download(np.int32, n * 4).reshape(n, 4)
+            ctx.trace_device(buf, R.hits, n, query, rt.KERNEL_WIDE)
+            wide = R.hits.download(np.int32, n * 4).reshape(n, 4)
+            if query == rt.CLOSEST_HIT:
+                diff = np.nonzero((ref != wide).any(1))[0]
+            else:
+                diff = np.nonzero(ref[:, 0] != wide[:, 0])[0]
+            print("  %d of %d rays differ" % (len(diff), n), flush=True)
+            assert len(diff) <= 1e-3 * n
+            if len(diff) == 0:
+                continue
+            rays = buf.download(np.float32, n * 8).reshape(n, 8)[diff]
+            if query == rt.CLOSEST_HIT:
+                flags = oracle.classify(sc, rays, ref[diff, 0], wide[diff, 0])
+            else:
+                closest = oracle.trace(sc, rays)[:, 0]
+                flags = oracle.classify(sc, rays, closest, closest)
+            assert (flags != 0).all(), [(int(diff[i]), ref[diff[i]].tolist(), wide[diff[i]].tolist())
+                                        for i in np.nonzero(flags == 0)[0][:5]]
+    finally:
+        R.free()
