@@ -75,11 +75,11 @@ __device__ __forceinline__ void render_tile(const BatchSlot& sl, uint32_t local,
   pt_pixel<WIDE, S>(p, ctx, gx, gy, &c->rays);
 }
 
+// Five waves per SIMD, as the single-scene path tracer (no spill here).
 template <bool WIDE, int S>
-__global__ __launch_bounds__(kTileThreads) void k_batch_persistent(const BatchSlot* __restrict__ slots,
-                                                                   uint32_t numScenes, uint32_t totalTiles,
-                                                                   BatchCounters* c, uint2* spillBase,
-                                                                   uint32_t spillDepth, uint32_t grab) {
+__global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(5))) void k_batch_persistent(
+    const BatchSlot* __restrict__ slots, uint32_t numScenes, uint32_t totalTiles, BatchCounters* c, uint2* spillBase,
+    uint32_t spillDepth, uint32_t grab) {
   __shared__ uint2 stk[WIDE ? S * kTileThreads : 1];
   __shared__ uint32_t sBase;
   const int tid = threadIdx.x;
@@ -102,8 +102,8 @@ __global__ __launch_bounds__(kTileThreads) void k_batch_persistent(const BatchSl
 
 // One scene, one tile per workgroup (the "separate launches" baseline).
 template <bool WIDE, int S>
-__global__ __launch_bounds__(kTileThreads) void k_batch_scene(const BatchSlot* __restrict__ slot, BatchCounters* c,
-                                                              uint2* spillBase, uint32_t spillDepth) {
+__global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(5))) void k_batch_scene(
+    const BatchSlot* __restrict__ slot, BatchCounters* c, uint2* spillBase, uint32_t spillDepth) {
   __shared__ uint2 stk[WIDE ? S * kTileThreads : 1];
   uint2* spill = WIDE ? spillBase + ((size_t)blockIdx.x * kTileThreads + threadIdx.x) * spillDepth : nullptr;
   render_tile<WIDE, S>(*slot, blockIdx.x, c->seed, stk, spill, c);

@@ -67,8 +67,10 @@ __device__ __forceinline__ void zero_features(float* dst) {
 // training paths (a 1-D grid, one thread per path), writing only the training
 // records; a training path is its pixel's path continued, so both passes
 // compute the same prefix.
+// Five waves per SIMD, as the plain path tracer.
 template <bool WIDE, bool TRAIN>
-__global__ __launch_bounds__(kTileThreads) void k_cached_paths(DrawParams p, CacheParams cp) {
+__global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(5))) void k_cached_paths(DrawParams p,
+                                                                                                    CacheParams cp) {
   __shared__ uint2 stk[WIDE ? kWideLdsStack * kTileThreads : 1];
   const int tid = threadIdx.x;
   const bool atPrimary = cp.query == GPUGI_CACHE_PRIMARY_INDIRECT;

@@ -18,8 +18,11 @@ __global__ void k_queue_reset(uint32_t* q) {
   if (threadIdx.x == 0) q[0] = 0u;
 }
 
+// Five waves per SIMD (at most 96 VGPRs, a 16-byte spill in the wide variant)
+// instead of the four that 115 VGPRs allow: clutter room -4.5 % per iteration.
 template <bool WIDE>
-__global__ __launch_bounds__(kTileThreads) void k_pathtracer(DrawParams p, uint32_t* __restrict__ queue) {
+__global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(5))) void k_pathtracer(
+    DrawParams p, uint32_t* __restrict__ queue) {
   __shared__ uint2 stk[WIDE ? kWideLdsStack * kTileThreads : 1];
   const int tid = threadIdx.x;
   const uint32_t linear = blockIdx.x * kTileThreads + tid;
