@@ -118,9 +118,11 @@ __device__ void count_hits(const RefScene& s, gg::V3 o, gg::V3 d, float rayLengt
   } while (node != 0 || nextIsLeaf);
 }
 
-__global__ __launch_bounds__(kHierBlock) void k_hier_acquire(DrawParams p, uint32_t frameSeed,
-                                                             unsigned long long* __restrict__ imp, uint32_t numNodes,
-                                                             uint32_t tilesW, uint32_t numPixels) {
+// Eight waves per SIMD (64 VGPRs, a 16-byte spill) instead of six: the two
+// walks per ray wait on memory, and the extra waves cover it (-2 % per update).
+__global__ __launch_bounds__(kHierBlock) __attribute__((amdgpu_waves_per_eu(8))) void k_hier_acquire(
+    DrawParams p, uint32_t frameSeed, unsigned long long* __restrict__ imp, uint32_t numNodes, uint32_t tilesW,
+    uint32_t numPixels) {
   const uint32_t i = blockIdx.x * kHierBlock + threadIdx.x;
   if (i >= numPixels) return;
   // pixel of thread i: 8x8 tiles in row order (the 8x8 work groups of the dispatch)
