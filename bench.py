@@ -698,6 +698,8 @@ def summarize_wavefront(kt_dir):
     dur = [(int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3 for r in rows]
     lines = ["Wavefront path tracer, clutter room (5,120,016 triangles), 1920x1080, path length 8,",
              "roulette from bounce 3, wide traversal; every pass reads its queue length on the device.",
+             "Each bounce's shadow trace and resolve pass run on a second stream, beside the next",
+             "bounce's closest-hit trace, so kernel durations overlap (their sum exceeds the draw's span).",
              "",
              "Commands (repo root; the first after cd /tmp && export TMPDIR=/tmp):",
              "  rocprofv3 --kernel-trace --stats -d <out>/wfprof -o run --output-format csv -- "

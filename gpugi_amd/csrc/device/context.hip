@@ -32,6 +32,7 @@ struct gpugi_ctx {
   hipEvent_t evIn[2] = {nullptr, nullptr}, evTraced[2] = {nullptr, nullptr}, evStart = nullptr;
   std::set<void*> hostAllocs;  // pinned host memory from gpugi_host_alloc
   hipEvent_t evFork = nullptr, evJoin = nullptr, evInfer = nullptr;
+  hipEvent_t evShade = nullptr, evResolve = nullptr;  // wavefront draw: shade -> shadow trace -> next shade
   bool timerStarted = false;
   std::string error;
   // reference layout
@@ -89,6 +90,11 @@ struct gpugi_ctx {
   int ptSchedule = GPUGI_PT_MEGAKERNEL;
   void* dWave = nullptr;         // wavefront path state and ray queues
   size_t waveCap = 0;
+  // the wavefront shadow traces' own wide-kernel spill stacks and batch
+  // counters (they run beside the closest-hit traces on the side stream)
+  void* dWaveSpill = nullptr;
+  size_t waveSpillBytes = 0;
+  void* dWaveQueue = nullptr;
   int bptCapacity = -1;
   int bptAverageLength = 0;
   IntegratorState integ;
@@ -199,6 +205,15 @@ int SetDev(gpugi_ctx* ctx) {
   return GPUGI_OK;
 }
 
+// The side stream (default priority) and the events that fork work onto it
+// and join it back, created on first use.
+int EnsureSideStream(gpugi_ctx* ctx) {
+  if (!ctx->side) HIP_TRY(ctx, hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking));
+  for (hipEvent_t* e : {&ctx->evFork, &ctx->evJoin, &ctx->evInfer, &ctx->evShade, &ctx->evResolve})
+    if (!*e) HIP_TRY(ctx, hipEventCreateWithFlags(e, hipEventDisableTiming));
+  return GPUGI_OK;
+}
+
 // Persistent grid and spill buffer for the trace kernel variant `stack`.
 int PrepareTraceWide(gpugi_ctx* ctx, int stack, WideScene* out) {
   int blocksPerCU = 0;
@@ -301,8 +316,9 @@ void gpugi_destroy(gpugi_ctx* ctx) {
   if (ctx->stream) hipStreamSynchronize(ctx->stream);
   for (void** p : {&ctx->dNodes, &ctx->dSlots, &ctx->dPos, &ctx->dVinfo, &ctx->dMats, &ctx->dWideNodes,
                    &ctx->dWideTris, &ctx->dSpill, &ctx->dCounters, &ctx->dWideQueue, &ctx->dAccum, &ctx->dLights,
-                   &ctx->dScratch, &ctx->dRefImage, &ctx->dMseTemp, &ctx->dEnv, &ctx->dWave, &ctx->dHierParent,
-                   &ctx->dHierLevel, &ctx->dHierImp, &ctx->dHierSubtree, &ctx->dHierMat})
+                   &ctx->dScratch, &ctx->dRefImage, &ctx->dMseTemp, &ctx->dEnv, &ctx->dWave, &ctx->dWaveSpill,
+                   &ctx->dWaveQueue, &ctx->dHierParent, &ctx->dHierLevel, &ctx->dHierImp, &ctx->dHierSubtree,
+                   &ctx->dHierMat})
     FreeDev(*p);
   FreeIntegratorState(&ctx->integ);
   for (void* p : ctx->texBufs) hipFree(p);
@@ -318,6 +334,8 @@ void gpugi_destroy(gpugi_ctx* ctx) {
   if (ctx->evFork) hipEventDestroy(ctx->evFork);
   if (ctx->evJoin) hipEventDestroy(ctx->evJoin);
   if (ctx->evInfer) hipEventDestroy(ctx->evInfer);
+  if (ctx->evShade) hipEventDestroy(ctx->evShade);
+  if (ctx->evResolve) hipEventDestroy(ctx->evResolve);
   if (ctx->copy) hipStreamSynchronize(ctx->copy);
   for (hipEvent_t e : {ctx->evIn[0], ctx->evIn[1], ctx->evTraced[0], ctx->evTraced[1], ctx->evStart})
     if (e) hipEventDestroy(e);
@@ -363,7 +381,7 @@ int gpugi_load_scene(gpugi_ctx* ctx, const gpugi_scene_desc* d) {
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   if (ctx->side) HIP_TRY(ctx, hipStreamSynchronize(ctx->side));
   for (void** p : {&ctx->dNodes, &ctx->dSlots, &ctx->dPos, &ctx->dVinfo, &ctx->dMats, &ctx->dWideNodes,
-                   &ctx->dWideTris, &ctx->dSpill})
+                   &ctx->dWideTris, &ctx->dSpill, &ctx->dWaveSpill})
     FreeDev(*p);
   // textures belong to the scene's materials
   for (void* p : ctx->texBufs) hipFree(p);
@@ -373,7 +391,7 @@ int gpugi_load_scene(gpugi_ctx* ctx, const gpugi_scene_desc* d) {
   ctx->anyTextureBound = false;
   FreeDev(ctx->dTexDesc);
   FreeDev(ctx->dMatTex);
-  ctx->spillBytes = 0;
+  ctx->spillBytes = ctx->waveSpillBytes = 0;
   ctx->hasScene = ctx->hasWide = false;
   std::vector<float> oboxNodes;
   if (ctx->bvhType == GPUGI_BVH_OBOX) {
@@ -587,12 +605,7 @@ int DrawCached(gpugi_ctx* ctx, uint32_t frameSeed) {
   // wait until the frame's queries have been inferred (so every query sees the
   // weights from before the frame), and the main stream joins the side stream
   // at the end of the frame.
-  if (!ctx->side) {
-    HIP_TRY(ctx, hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking));
-    HIP_TRY(ctx, hipEventCreateWithFlags(&ctx->evFork, hipEventDisableTiming));
-    HIP_TRY(ctx, hipEventCreateWithFlags(&ctx->evJoin, hipEventDisableTiming));
-    HIP_TRY(ctx, hipEventCreateWithFlags(&ctx->evInfer, hipEventDisableTiming));
-  }
+  if (int rc = EnsureSideStream(ctx)) return rc;
   HIP_TRY(ctx, hipEventRecord(ctx->evFork, ctx->stream));
   HIP_TRY(ctx, hipStreamWaitEvent(ctx->side, ctx->evFork, 0));
   const hipStream_t trainStream = ctx->side;
@@ -666,9 +679,25 @@ int PrepareWavefront(gpugi_ctx* ctx, const DrawParams& p, WavefrontTrace* tr) {
     HIP_TRY(ctx, hipMalloc(&ctx->dWave, bytes));
     ctx->waveCap = bytes;
   }
+  if (int rc = EnsureSideStream(ctx)) return rc;
   tr->wide = p.useWide;
-  if (tr->wide)
+  if (tr->wide) {
     if (int rc = PrepareTraceWide(ctx, ctx->wideStack, &tr->ws)) return rc;
+    // the shadow traces' own spill stacks and batch counters (the grid's size)
+    const size_t spill = std::max<size_t>(8, (size_t)tr->ws.numThreads * tr->ws.spillDepth * sizeof(uint2));
+    if (ctx->waveSpillBytes < spill) {
+      HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+      HIP_TRY(ctx, hipStreamSynchronize(ctx->side));
+      HIP_TRY(ctx, EnsureBuffer(&ctx->dWaveSpill, &ctx->waveSpillBytes, spill));
+    }
+    if (!ctx->dWaveQueue) HIP_TRY(ctx, hipMalloc(&ctx->dWaveQueue, kWideQueueWords * sizeof(uint32_t)));
+    tr->wsShadow = tr->ws;
+    tr->wsShadow.spill = (uint2*)ctx->dWaveSpill;
+    tr->wsShadow.queue = (uint32_t*)ctx->dWaveQueue;
+  }
+  tr->side = ctx->side;
+  tr->evShade = ctx->evShade;
+  tr->evResolve = ctx->evResolve;
   tr->stack = ctx->wideStack;
   tr->refill = ctx->wideRefill;
   tr->wait = ctx->wideWait;

@@ -23,18 +23,26 @@ struct WavefrontBuffers {
 
 // How the passes trace: the wide kernel (with its launch parameters) or the
 // reference walk; passBlocks: the grid of the shade and resolve passes.
+// With `side` set, each bounce's shadow trace and resolve pass run on that
+// stream, beside the next bounce's closest-hit trace on the main stream; the
+// shadow traces then use their own batch counters and spill stacks (wsShadow).
+// evShade orders a bounce's shade pass before its shadow trace, evResolve its
+// resolve pass before the next shade pass (both write the path's color).
 struct WavefrontTrace {
   bool wide = false;
-  WideScene ws;
+  WideScene ws, wsShadow;
   int stack = 10, refill = 8, wait = 16;
   uint32_t passBlocks = 0;
+  hipStream_t side = nullptr;
+  hipEvent_t evShade = nullptr, evResolve = nullptr;
 };
 
 size_t WavefrontBytes(size_t numPixels);
 WavefrontBuffers CarveWavefront(void* base, size_t numPixels);
-// One PT draw: maxPathLength bounces of passes enqueued on `stream`, each
-// reading its queue length on the device (no host synchronization, so the
-// draw can be captured into a graph).
+// One PT draw: maxPathLength bounces of passes enqueued on `stream` (and on
+// tr.side, forked from and joined back into `stream`), each reading its queue
+// length on the device (no host synchronization, so the draw can be captured
+// into a graph).
 hipError_t DrawPTWavefront(const DrawParams& p, const WavefrontTrace& tr, const WavefrontBuffers& b,
                            hipStream_t stream);
 

@@ -16,6 +16,9 @@
 // maxPathLength bounces without waiting, so a frame can be captured in a
 // graph; passes over an empty queue return at once.  The shade and resolve
 // passes run a fixed grid of resident blocks that strides over the queue.
+// On a second stream, the shadow trace and resolve of one bounce run beside
+// the next bounce's closest-hit trace (they touch disjoint buffers); the next
+// shade pass waits for the resolve, so each pixel's terms keep their order.
 #include "integrator_common.h"
 #include "launch.h"
 #include "wavefront_pt.h"
@@ -234,26 +237,42 @@ hipError_t DrawPTWavefront(const DrawParams& p, const WavefrontTrace& tr, const 
   hipLaunchKernelGGL(k_wf_init_counts, dim3(1), dim3(64), 0, stream, b.counters, numPaths);
   // resident blocks of the shade / resolve passes, at most one per 256 paths
   const dim3 passGrid(std::min<uint32_t>(tr.passBlocks, (numPaths + kWfBlock - 1) / kWfBlock));
+  // with a side stream the shadow trace and resolve pass of bounce k overlap
+  // the closest-hit trace of bounce k + 1: the two launches fill each other's
+  // drain (the last waves of a persistent trace leave most CUs idle)
+  const bool fork = tr.side != nullptr;
+  const hipStream_t shadowStream = fork ? tr.side : stream;
+  const WideScene& shadowScene = fork ? tr.wsShadow : tr.ws;
+  hipError_t e = hipSuccess;
   int cur = 0;
   for (int bounce = 0; bounce < p.maxPathLength; ++bounce) {
     const uint32_t* count = b.counters + cur;  // this bounce's queue length, on the device
     const gpugi_ray* rays = (const gpugi_ray*)b.rays[cur];
-    hipError_t e = tr.wide ? LaunchTraceWide(tr.ws, tr.stack, GPUGI_CLOSEST_HIT, rays, (gpugi_hit*)b.hits, numPaths,
-                                             tr.refill, tr.wait, p.watertight, stream, count)
-                           : LaunchTraceRef(p.ref, GPUGI_CLOSEST_HIT, rays, (gpugi_hit*)b.hits, numPaths,
-                                            p.watertight, stream, count);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_wf_zero, dim3(1), dim3(1), 0, stream, b.counters + (1 - cur));
-    hipLaunchKernelGGL(k_wf_shade, passGrid, dim3(kWfBlock), 0, stream, p, b, cur);
-    const gpugi_ray* shadow = (const gpugi_ray*)b.shadowRays;
-    e = tr.wide ? LaunchTraceWide(tr.ws, tr.stack, GPUGI_ANY_HIT, shadow, (gpugi_hit*)b.occluded, numPaths,
+    e = tr.wide ? LaunchTraceWide(tr.ws, tr.stack, GPUGI_CLOSEST_HIT, rays, (gpugi_hit*)b.hits, numPaths,
                                   tr.refill, tr.wait, p.watertight, stream, count)
-                : LaunchTraceRef(p.ref, GPUGI_ANY_HIT, shadow, (gpugi_hit*)b.occluded, numPaths, p.watertight,
+                : LaunchTraceRef(p.ref, GPUGI_CLOSEST_HIT, rays, (gpugi_hit*)b.hits, numPaths, p.watertight,
                                  stream, count);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_wf_resolve, passGrid, dim3(kWfBlock), 0, stream, b, cur);
+    // the previous bounce's resolve pass reads the queue, NEE records and color
+    // that this shade pass overwrites (and the counter k_wf_zero clears)
+    if (fork && bounce > 0 && (e = hipStreamWaitEvent(stream, tr.evResolve, 0)) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_wf_zero, dim3(1), dim3(1), 0, stream, b.counters + (1 - cur));
+    hipLaunchKernelGGL(k_wf_shade, passGrid, dim3(kWfBlock), 0, stream, p, b, cur);
+    if (fork) {
+      if ((e = hipEventRecord(tr.evShade, stream)) != hipSuccess) return e;
+      if ((e = hipStreamWaitEvent(tr.side, tr.evShade, 0)) != hipSuccess) return e;
+    }
+    const gpugi_ray* shadow = (const gpugi_ray*)b.shadowRays;
+    e = tr.wide ? LaunchTraceWide(shadowScene, tr.stack, GPUGI_ANY_HIT, shadow, (gpugi_hit*)b.occluded, numPaths,
+                                  tr.refill, tr.wait, p.watertight, shadowStream, count)
+                : LaunchTraceRef(p.ref, GPUGI_ANY_HIT, shadow, (gpugi_hit*)b.occluded, numPaths, p.watertight,
+                                 shadowStream, count);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_wf_resolve, passGrid, dim3(kWfBlock), 0, shadowStream, b, cur);
+    if (fork && (e = hipEventRecord(tr.evResolve, tr.side)) != hipSuccess) return e;
     cur = 1 - cur;
   }
+  if (fork && (e = hipStreamWaitEvent(stream, tr.evResolve, 0)) != hipSuccess) return e;  // join
   hipLaunchKernelGGL(k_wf_finish, Grid(numPaths), dim3(kWfBlock), 0, stream, p, b, tilesW, numPaths);
   return hipGetLastError();
 }

@@ -210,7 +210,9 @@ int gpugi_set_integrator_kernel(gpugi_ctx* ctx, int kernel);
  * pathtracer.comp dispatch does) or GPUGI_PT_WAVEFRONT (one pass per bounce
  * over the live paths: closest-hit and shadow rays traced by the gpugi_trace
  * kernels on compact device queues whose lengths every pass reads on the
- * device, so the host never waits; see gpugi_capture_draw).  Both give the
+ * device, so the host never waits; see gpugi_capture_draw; each bounce's
+ * shadow trace runs on a second stream of the context beside the next
+ * closest-hit trace, joined back into the context's stream).  Both give the
  * same image with the reference walk; the wavefront schedule with the
  * reference walk and GPUGI_TRIANGLE_WATERTIGHT rebuilds b_x as 1 - (b_y + b_z)
  * from the hit record.  No counterpart in the reference. */
