@@ -63,7 +63,8 @@ def parse():
     p.add_argument("--summarize-mlp", nargs="+", metavar="KT_DIR_THEN_PMC_DIRS",
                    help="write profiles/mlp_kernels.json from a kernel trace and --pmc runs of --mlp-kernels and exit")
     p.add_argument("--sweep", action="store_true",
-                   help="print Grays/s of every wide-kernel configuration on the three ray sets and exit")
+                   help="print Grays/s of every wide-kernel configuration on the three ray sets and exit "
+                        "(heightfield; with --clutter, the clutter room)")
     p.add_argument("--mlp", action="store_true",
                    help="time-to-equal-RMSE of the cached path tracer against the plain one (heightfield, 1920x1080)")
     p.add_argument("--mlp-ref-spp", type=int, default=4096, help="samples per pixel of the plain reference render")
@@ -371,6 +372,40 @@ def run_valu_probe(ctx):
     out.free()
     return {"blocks": blocks, "iters": iters, "ms": [round(x, 4) for x in probe],
             "fma_tflops": round(blocks * 256 * iters * 64 * 2 / (min(probe) * 1e-3) / 1e12, 2)}
+
+
+def sweep_wide_configs(ctx, R, n, grid):
+    """Prints one JSON line per wide-kernel configuration (max_leaf_triangles, lds_stack_entries,
+    refill_lanes, wait_lanes) of `grid`: Grays/s of the three ray sets of R, 1 warm-up + 3 timed launches."""
+    from gpugi_amd import rt
+    sets = (("incoherent", R.incoherent, rt.CLOSEST_HIT), ("coherent", R.primary, rt.CLOSEST_HIT),
+            ("shadow_any", R.shadow, rt.ANY_HIT))
+    for leaf, stack, refill, wait in grid:
+        ctx.set_wide_config(leaf, stack, refill, wait)
+        row = {}
+        for name, buf, q in sets:
+            ctx.trace_device(buf, R.hits, n, q, rt.KERNEL_WIDE)
+            ctx.sync()
+            ctx.timer_begin()
+            for _ in range(3):
+                ctx.trace_device(buf, R.hits, n, q, rt.KERNEL_WIDE)
+            row[name] = round(n / (ctx.timer_end() / 3 * 1e-3) / 1e9, 3)
+        print(json.dumps(dict(ctx.wide_config(), **row)), flush=True)
+
+
+def clutter_sweep():
+    """--clutter --sweep: the wide-kernel configurations on the clutter room's three 2^24-ray sets."""
+    from gpugi_amd import rt, workloads
+    hs = rt.HostScene.clutter(workloads.CLUTTER_SPHERES, workloads.MASTER_SEED)
+    ctx = rt.Context(0)
+    ctx.load_scene(hs)
+    n = 1 << 24
+    R = workloads.SceneRays(ctx, workloads.clutter_camera(workloads.HF_WIDTH, workloads.HF_HEIGHT),
+                            workloads.CLUTTER_LIGHTS, n, workloads.MASTER_SEED, workloads.HF_WIDTH,
+                            workloads.HF_HEIGHT)
+    sweep_wide_configs(ctx, R, n, itertools.product((2, 4), (8, 10, 12, 16, 24), (8, 16), (16, 24)))
+    R.free()
+    ctx.close()
 
 
 def clutter_benchmark(steps, iterations, incoherent_only=False, calibrate=False, valu_probe=False):
@@ -916,6 +951,9 @@ def main():
     if args.batch:
         print(json.dumps(batch_benchmark(args.batch_frames)))
         return
+    if args.clutter and args.sweep:
+        clutter_sweep()
+        return
     if args.clutter:
         print(json.dumps(clutter_benchmark(args.steps, args.clutter_iterations, args.clutter_incoherent_only,
                                            args.calibrate, args.valu_probe)))
@@ -931,20 +969,7 @@ def main():
     kernel = rt.KERNEL_WIDE if args.kernel == "wide" else rt.KERNEL_REFERENCE
 
     if args.sweep:
-        sets = (("incoherent", R.incoherent, rt.CLOSEST_HIT), ("coherent", R.primary, rt.CLOSEST_HIT),
-                ("shadow_any", R.shadow, rt.ANY_HIT))
-        for leaf, stack, refill, wait in itertools.product((1, 2, 4), (8, 10, 12, 16), (8, 16, 32),
-                                                           (1, 8, 16, 32)):
-            ctx.set_wide_config(leaf, stack, refill, wait)
-            row = {}
-            for name, buf, q in sets:
-                ctx.trace_device(buf, R.hits, n, q, rt.KERNEL_WIDE)
-                ctx.sync()
-                ctx.timer_begin()
-                for _ in range(3):
-                    ctx.trace_device(buf, R.hits, n, q, rt.KERNEL_WIDE)
-                row[name] = round(n / (ctx.timer_end() / 3 * 1e-3) / 1e9, 3)
-            print(json.dumps(dict(ctx.wide_config(), **row)), flush=True)
+        sweep_wide_configs(ctx, R, n, itertools.product((1, 2, 4), (8, 10, 12, 16), (8, 16, 32), (1, 8, 16, 32)))
         ctx.close()
         return
 
