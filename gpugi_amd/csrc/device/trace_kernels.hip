@@ -89,6 +89,7 @@ hipError_t LaunchTraceRefImportance(const RefScene& s, int query, const gpugi_ra
 hipError_t LaunchTraceRef(const RefScene& s, int query, const gpugi_ray* rays, gpugi_hit* hits, size_t n,
                           bool watertight, hipStream_t stream, const uint32_t* nDevice) {
   if (n == 0) return hipSuccess;
+  if (n > 0xffffffffull - 255) return hipErrorInvalidValue;  // the kernel indexes rays with 32 bits
   dim3 grid((unsigned)((n + 255) / 256));
   const float4* r = (const float4*)rays;
   int4* h = (int4*)hits;
