@@ -479,9 +479,10 @@ def clutter_benchmark(steps, iterations, incoherent_only=False, calibrate=False,
     sampler = rt.LightSampler(hs, 123)
     integ = {}
     for name, integrator, nls in (("pt", rt.PT, 128), ("pt_wavefront", rt.PT, 128), ("lt", rt.LT, 256),
-                                  ("bpt", rt.BPT, 128)):
-        # the path tracer under both schedules: one kernel, and per-bounce passes through the trace kernels
-        ctx.set_pt_schedule(rt.PT_WAVEFRONT if name == "pt_wavefront" else rt.PT_MEGAKERNEL)
+                                  ("bpt", rt.BPT, 128), ("bpt_wavefront", rt.BPT, 128)):
+        # PT and BPT's eye paths under both schedules: one kernel, and per-bounce passes through the
+        # trace kernels
+        ctx.set_pt_schedule(rt.PT_WAVEFRONT if name.endswith("_wavefront") else rt.PT_MEGAKERNEL)
         sampler = rt.LightSampler(hs, 123)  # every integrator sees the same light-sample batches
         if integrator == rt.BPT:
             entries = 0

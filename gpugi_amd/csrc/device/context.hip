@@ -705,11 +705,23 @@ int PrepareWavefront(gpugi_ctx* ctx, const DrawParams& p, WavefrontTrace* tr) {
   return GPUGI_OK;
 }
 
-// GPUGI_PT as per-bounce passes over device ray queues (wavefront_pt.hip).
-int DrawWavefront(gpugi_ctx* ctx, const DrawParams& p) {
+// GPUGI_PT, or the eye paths of GPUGI_BPT after its light pass, as per-bounce
+// passes over device ray queues (wavefront_pt.hip).
+int DrawWavefront(gpugi_ctx* ctx, int integrator, DrawParams p) {
   WavefrontTrace tr;
   if (int rc = PrepareWavefront(ctx, p, &tr)) return rc;
-  HIP_TRY(ctx, DrawPTWavefront(p, tr, CarveWavefront(ctx->dWave, (size_t)ctx->width * ctx->height), ctx->stream));
+  const WavefrontBuffers b = CarveWavefront(ctx->dWave, (size_t)ctx->width * ctx->height);
+  if (integrator == GPUGI_PT) {
+    HIP_TRY(ctx, DrawPTWavefront(p, tr, b, ctx->stream));
+    return GPUGI_OK;
+  }
+  std::string err;
+  const LightCacheEntryDev* cache = nullptr;
+  const uint32_t* total = nullptr;
+  const hipError_t e = BptLightPass(&ctx->integ, &p, ctx->stream, &err, &cache, &total);
+  if (e != hipSuccess) return Fail(ctx, GPUGI_ERR_HIP, std::string("gpugi_draw: ") + hipGetErrorString(e));
+  if (!err.empty()) return Fail(ctx, GPUGI_ERR_STATE, "gpugi_draw: " + err);
+  HIP_TRY(ctx, DrawBPTWavefront(p, tr, b, cache, total, ctx->stream));
   return GPUGI_OK;
 }
 
@@ -730,8 +742,8 @@ int gpugi_draw(gpugi_ctx* ctx, int integrator, uint32_t frame_seed) {
     return GPUGI_OK;
   }
   DrawParams p = MakeDrawParams(ctx, frame_seed);
-  if (integrator == GPUGI_PT && ctx->ptSchedule == GPUGI_PT_WAVEFRONT) {
-    if (int rc = DrawWavefront(ctx, p)) return rc;
+  if ((integrator == GPUGI_PT || integrator == GPUGI_BPT) && ctx->ptSchedule == GPUGI_PT_WAVEFRONT) {
+    if (int rc = DrawWavefront(ctx, integrator, p)) return rc;
     ++ctx->iterations;
     return GPUGI_OK;
   }

@@ -93,6 +93,12 @@ inline hipError_t EnsureBuffer(void** buf, size_t* cap, size_t bytes) {
 
 hipError_t DrawLT(IntegratorState* st, const DrawParams& p, hipStream_t stream);
 hipError_t DrawBPT(IntegratorState* st, const DrawParams& p, hipStream_t stream, std::string* err);
+// The light pass of a BPT draw (light paths into the cache, bidirectionalpathtracer.cpp:188-194):
+// *cache receives the cache, *total the device word with its entry count (both
+// null, with *err set, when the cache capacity is not set); p->wide.spill is
+// pointed at the integrators' spill stacks.
+hipError_t BptLightPass(IntegratorState* st, DrawParams* p, hipStream_t stream, std::string* err,
+                        const LightCacheEntryDev** cache, const uint32_t** total);
 
 // Result of a BPT warm-up light pass (the two SSBO counters read back by
 // CreateLightCacheWithCapacityEstimate, bidirectionalpathtracer.cpp:75-104).

@@ -16,6 +16,7 @@
 // restatement.
 #include <hip/hip_runtime.h>
 
+#include "bpt_eye.h"
 #include "integrator_common.h"
 #include "scan.h"
 
@@ -29,9 +30,6 @@ static_assert(kLightBlock == kTileThreads, "the wide traversal stack is laid out
 __device__ __forceinline__ void splat_add(float* p, float v) {
   (void)__builtin_amdgcn_global_atomic_fadd_f32((__attribute__((address_space(1))) float*)p, v);
 }
-
-__device__ __forceinline__ float mis_heuristic(float p) { return fminf(1e18f, p); }     // lightcache.glsl:61-73
-__device__ __forceinline__ float distance_mis(float d, float c) { return mis_heuristic(d * d); }  // :84-89
 
 // ProjectToScreen without DEPTH_BUFFER_OCCLUSION_TEST / CHAINED_MIS and with
 // the box reconstruction filter (screenreproject.glsl:55-227).
@@ -269,137 +267,21 @@ __global__ __launch_bounds__(256) void k_bpt_scatter(const LightCacheEntryDev* _
   }
 }
 
-// pathtracer_bidir.comp:16-251 without CHAINED_MIS: the eye path of one pixel
-// as a state that advances one bounce at a time (for path regeneration).
-struct BptEyePath {
-  gg::V3 color, tp, o, d;
-  uint32_t seed;
-  int bounce;
-  float lastConnectionProbability, lastCosAbs, mis;
-};
-
-__device__ __forceinline__ void bpt_eye_begin(const DrawParams& p, BptEyePath& s, int gx, int gy) {
-  s.seed = gg::init_random_seed(p.frameSeed, (uint32_t)(gx + gy * p.width));
-  const float j0 = gg::random01(s.seed);
-  const float j1 = gg::random01(s.seed);
-  const gg::V3 U = gg::v3(p.camera.u[0], p.camera.u[1], p.camera.u[2]);
-  const gg::V3 V = gg::v3(p.camera.v[0], p.camera.v[1], p.camera.v[2]);
-  const gg::V3 W = gg::v3(p.camera.w[0], p.camera.w[1], p.camera.w[2]);
-  s.o = gg::v3(p.camera.position[0], p.camera.position[1], p.camera.position[2]);
-  s.d = gg::primary_direction(j0, j1, gx, gy, p.width, p.height, U, V, W);
-  s.color = gg::v3s(0.0f);
-  s.tp = gg::v3s(1.0f);
-  s.lastConnectionProbability = 1.0f;  // InitialMIS
-  s.lastCosAbs = 1.0f;
-  s.mis = 0.0f;
-  s.bounce = 0;
-}
-
-// One bounce; true when the path has ended.  actualN: light-cache entries in use.
+// One bounce of an eye path (bpt_eye.h); true when the path has ended.
+// actualN: light-cache entries in use.
 template <bool WIDE>
 __device__ __forceinline__ bool bpt_eye_bounce(const DrawParams& p, const TraceCtx& c, BptEyePath& s,
                                                const LightCacheEntryDev* __restrict__ cache, uint32_t actualN) {
-  const int i = s.bounce;
-  if (i >= p.maxPathLength) return true;
+  if (s.bounce >= p.maxPathLength) return true;
   const float lightSampleWeight = (float)actualN;
   const ClosestHit h = trace_closest<WIDE>(c, s.o, s.d);
   if (!h.hit) return true;
-  const float rayHit = h.t;
-  s.o = s.o + rayHit * s.d;
-  const gg::V3 n = shading_normal(p.ref.vinfo, h.tri, h.bary);
-  const MaterialData m = load_material_hit(p.materials, p.tex, p.ref.vinfo, h.tri, h.bary);
-  const float incidentSurfaceCosSat = gg::saturate(-gg::dot(n, s.d));
-  const float connectionLightToEyePath = distance_mis(rayHit, s.lastCosAbs) * mis_heuristic(incidentSurfaceCosSat / gg::kPI);
-  const int spl = p.specificPathLength;
-  if (spl < 0 || i == spl) {  // SHOW_SPECIFIC_PATHLENGTH (pathtracer_bidir.comp:75-92)
-    s.mis = (i == 0) ? 1.0f
-                     : (s.lastConnectionProbability /
-                        fmaxf(s.lastConnectionProbability + connectionLightToEyePath, gg::kDivisorEpsilon));
-    s.color = s.color + s.mis * s.tp * m.emissivity;
-  }
-  // Connection to one random light-cache vertex.
-  const uint32_t idx = gg::xorshift32(s.seed) % actualN;
-  const LightCacheEntryDev& e = cache[idx];
-  const int pathLength = e.pathLength + i + 1;
-  // a connection closes a path of pathLength bounces (pathtracer_bidir.comp:101-108)
-  if (pathLength <= p.maxPathLength && (spl < 0 || pathLength == spl)) {
-    const gg::V3 epos = gg::v3(e.position[0], e.position[1], e.position[2]);
-    gg::V3 ld = epos - s.o;
-    const float lightDistSq = gg::dot(ld, ld);
-    const float lightDist = sqrtf(lightDistSq);
-    ld = ld / fmaxf(lightDist, gg::kDivisorEpsilon);
-    const gg::V3 lo = gg::kRayHitEpsilon * ld + s.o;
-    if (!trace_any<WIDE>(c, lo, ld, lightDist - gg::kRayHitEpsilon * 2.0f)) {
-      const int lmat = e.materialIndex;
-      const gg::V3 flux = gg::v3(e.flux[0], e.flux[1], e.flux[2]);
-      gg::V3 lnormal = gg::v3(0.0f, 1.0f, 0.0f);
-      float cosLight = 0.0f;
-      gg::V3 lintensity;
-      const float cosCamera = fabsf(gg::dot(ld, n));
-      if (e.normal1 < 1.5f) {
-        lnormal = gg::unpack_normal_raw(e.normal0, e.normal1);
-        cosLight = fabsf(gg::dot(ld, lnormal));
-      }
-      float pdf;
-      const gg::V3 bsdf = bsdf_eye(s.d, ld, m, n, pdf);
-      float eyeToLight = mis_heuristic(pdf);
-      if (i > 0) {
-        const float distMIS = distance_mis(rayHit, s.lastCosAbs);
-        eyeToLight = eyeToLight * distMIS;
-        s.mis = s.lastConnectionProbability / fmaxf(s.lastConnectionProbability + eyeToLight, gg::kDivisorEpsilon);
-        if (m.refl[3] >= 100.0f) s.mis = 1.0f;
-      } else {
-        s.mis = eyeToLight / fmaxf(s.lastConnectionProbability + eyeToLight, gg::kDivisorEpsilon);
-      }
-      float misLight = 1.0f;
-      if (e.normal1 < 1.5f) {
-        float lightToEye;
-        if (lmat != -1) {
-          // the light vertex's material at its stored texture coordinate (lightcache.glsl:1-16)
-          const MaterialData lm = load_material_uv(p.materials, p.tex, lmat, make_float2(e.texcoord[0], e.texcoord[1]));
-          const gg::V3 incidentDir = gg::unpack_normal(e.incidentPacked[0], e.incidentPacked[1]);
-          const gg::V3 lbsdf = bsdf_adjoint(ld, -incidentDir, lm, lnormal, pdf);
-          lintensity = flux * lbsdf * cosLight;
-          lightToEye = mis_heuristic(pdf) * distance_mis(lightDist, cosCamera);
-        } else {
-          lintensity = flux * (cosLight * gg::kPI_2);
-          lightToEye = mis_heuristic(cosLight / gg::kPI) * distance_mis(lightDist, cosCamera);
-        }
-        misLight = lightToEye / fmaxf(lightToEye + eyeToLight, gg::kDivisorEpsilon);
-      } else {
-        lintensity = flux * gg::kPI_2;
-        misLight = 1.0f;
-      }
-      s.mis = s.mis * misLight;
-      const gg::V3 irradiance = (s.mis * lightSampleWeight * cosCamera / lightDistSq) * lintensity;
-      s.color = s.color + irradiance * s.tp * bsdf;
-    }
-  }
-  float samplePDF;
-  gg::V3 t1 = gg::v3s(1.0f);
-  bool diffuse;
-  s.d = sample_bsdf(s.d, m, s.seed, n, t1, false, samplePDF, diffuse);
-  s.o = s.o + s.d * gg::kRayHitEpsilon;
-  s.tp = s.tp * t1;
-  if (i >= p.rrStartDepth) {  // Russian roulette from rrStartDepth on
-    const float q = gg::saturate(gg::luminance(t1));
-    if (gg::random01(s.seed) >= q) return true;
-    s.tp = s.tp / q;
-  }
-  float reverse = mis_heuristic(samplePDF);
-  if (i == 0) {
-    s.mis = reverse / fmaxf(reverse + s.lastConnectionProbability, gg::kDivisorEpsilon);
-  } else {
-    const float distMIS = distance_mis(rayHit, s.lastCosAbs);
-    reverse = reverse * distMIS;
-    s.mis = s.lastConnectionProbability / fmaxf(s.lastConnectionProbability + distMIS * reverse, gg::kDivisorEpsilon);
-    if (m.refl[3] >= 100.0f) s.mis = 1.0f;
-  }
-  s.tp = s.tp * s.mis;
-  s.lastConnectionProbability = reverse;
-  s.lastCosAbs = fabsf(gg::dot(s.d, n));
-  ++s.bounce;
-  return s.bounce >= p.maxPathLength;
+  BptEyeVertex v;
+  bpt_eye_hit(p, s, h, v);
+  const BptConnection cn = bpt_connect_setup(p, s, cache, actualN);
+  if (cn.test && !trace_any<WIDE>(c, cn.lo, cn.ld, cn.tmax))
+    s.color = s.color + bpt_connect_value(p, s, v, cn, lightSampleWeight);
+  return bpt_eye_continue(p, s, v);
 }
 
 // Eye paths of all pixels of the 8x8 tiles with path regeneration
@@ -503,8 +385,11 @@ hipError_t BptWarmup(IntegratorState* st, const DrawParams& p0, hipStream_t stre
   return hipSuccess;
 }
 
-hipError_t DrawBPT(IntegratorState* st, const DrawParams& p0, hipStream_t stream, std::string* err) {
-  DrawParams p = p0;
+hipError_t BptLightPass(IntegratorState* st, DrawParams* pp, hipStream_t stream, std::string* err,
+                        const LightCacheEntryDev** cacheOut, const uint32_t** totalOut) {
+  DrawParams& p = *pp;
+  *cacheOut = nullptr;
+  *totalOut = nullptr;
   if (p.bptCapacity < 0) {
     *err = "the light cache capacity is not set (run the warm-up first)";
     return hipSuccess;
@@ -537,6 +422,17 @@ hipError_t DrawBPT(IntegratorState* st, const DrawParams& p0, hipStream_t stream
   // NumLightCacheEntries = R*N + appended (resetcounter.comp:9 plus the appends)
   AddConstantU32(total, numPaths, stream);
   st->lastTotal = total;
+  *cacheOut = cache;
+  *totalOut = total;
+  return hipGetLastError();
+}
+
+hipError_t DrawBPT(IntegratorState* st, const DrawParams& p0, hipStream_t stream, std::string* err) {
+  DrawParams p = p0;
+  const LightCacheEntryDev* cache = nullptr;
+  const uint32_t* total = nullptr;
+  hipError_t e = BptLightPass(st, &p, stream, err, &cache, &total);
+  if (e != hipSuccess || !err->empty()) return e;
   const size_t pixels = (size_t)(p.width / kTile) * (p.height / kTile) * kTileThreads;
   if (pixels) {
     const void* fn = p.useWide ? reinterpret_cast<const void*>(&k_bpt_eye<true>)

@@ -11,7 +11,8 @@ namespace gpugi {
 struct WavefrontBuffers {
   uint32_t* counters;  // [0], [1]: lengths of the two path queues
   float4* colorSeed;   // per pixel: color.xyz, w = the random state's bits
-  float4* tpFlags;     // per pixel: throughput.xyz, w = bits of bounce << 1 | didNEE
+  float4* tpFlags;     // per pixel: throughput.xyz, w = bits of bounce << 1 | didNEE (PT)
+  float2* misState;    // per pixel (BPT eye paths): last connection probability, last |cos|
   uint32_t* queue[2];  // pixel indices of the live paths
   float4* rays[2];     // their rays (gpugi_ray layout), in queue order
   int4* hits;          // closest-hit records of the current queue
@@ -45,5 +46,10 @@ WavefrontBuffers CarveWavefront(void* base, size_t numPixels);
 // into a graph).
 hipError_t DrawPTWavefront(const DrawParams& p, const WavefrontTrace& tr, const WavefrontBuffers& b,
                            hipStream_t stream);
+// The eye pass of a BPT draw (pathtracer_bidir.comp) as the same passes, after
+// the light pass filled `cache`; numEntries: device word with the number of
+// entries written (at most p.bptCapacity are used), as k_bpt_eye reads it.
+hipError_t DrawBPTWavefront(const DrawParams& p, const WavefrontTrace& tr, const WavefrontBuffers& b,
+                            const LightCacheEntryDev* cache, const uint32_t* numEntries, hipStream_t stream);
 
 }  // namespace gpugi

@@ -19,6 +19,12 @@
 // On a second stream, the shadow trace and resolve of one bounce run beside
 // the next bounce's closest-hit trace (they touch disjoint buffers); the next
 // shade pass waits for the resolve, so each pixel's terms keep their order.
+//
+// The eye paths of the bidirectional path tracer (pathtracer_bidir.comp) run
+// through the same passes: the any-hit ray of a bounce is the connection to a
+// light-cache vertex, whose value the shade pass computes with the steps of
+// bpt_eye.h and the resolve pass adds when the connection is not blocked.
+#include "bpt_eye.h"
 #include "integrator_common.h"
 #include "launch.h"
 #include "wavefront_pt.h"
@@ -28,6 +34,7 @@ namespace gpugi {
 namespace {
 
 constexpr int kWfBlock = 256;
+constexpr int kWfPT = 0, kWfBPT = 1;  // the integrator whose paths the passes advance
 
 // Pixel of path i: 8x8 tiles in row order, row-major inside a tile.
 __device__ __forceinline__ void tile_pixel(uint32_t i, uint32_t tilesW, int& gx, int& gy) {
@@ -36,6 +43,7 @@ __device__ __forceinline__ void tile_pixel(uint32_t i, uint32_t tilesW, int& gx,
   gy = (int)((tile / tilesW) * kTile + k / kTile);
 }
 
+template <int MODE>
 __global__ __launch_bounds__(kWfBlock) void k_wf_raygen(DrawParams p, WavefrontBuffers b, uint32_t tilesW,
                                                         uint32_t numPaths) {
   const uint32_t i = blockIdx.x * kWfBlock + threadIdx.x;
@@ -45,13 +53,26 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_raygen(DrawParams p, WavefrontB
   int gx, gy;
   tile_pixel(i, tilesW, gx, gy);
   const uint32_t pix = (uint32_t)gy * (uint32_t)p.width + (uint32_t)gx;
-  PtPath s;
-  pt_begin(p, s, gx, gy, p.frameSeed);
-  b.colorSeed[pix] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(s.seed));
-  b.tpFlags[pix] = make_float4(s.throughput.x, s.throughput.y, s.throughput.z, __uint_as_float(0u));  // bounce 0
+  gg::V3 o, d;
+  if (MODE == kWfPT) {
+    PtPath s;
+    pt_begin(p, s, gx, gy, p.frameSeed);
+    b.colorSeed[pix] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(s.seed));
+    b.tpFlags[pix] = make_float4(s.throughput.x, s.throughput.y, s.throughput.z, __uint_as_float(0u));  // bounce 0
+    o = s.o;
+    d = s.d;
+  } else {
+    BptEyePath s;
+    bpt_eye_begin(p, s, gx, gy);
+    b.colorSeed[pix] = make_float4(s.color.x, s.color.y, s.color.z, __uint_as_float(s.seed));
+    b.tpFlags[pix] = make_float4(s.tp.x, s.tp.y, s.tp.z, __uint_as_float(0u));  // bounce 0
+    b.misState[pix] = make_float2(s.lastConnectionProbability, s.lastCosAbs);
+    o = s.o;
+    d = s.d;
+  }
   b.queue[0][i] = pix;
-  b.rays[0][2 * (size_t)i] = make_float4(s.o.x, s.o.y, s.o.z, gg::kRayMax);
-  b.rays[0][2 * (size_t)i + 1] = make_float4(s.d.x, s.d.y, s.d.z, 0.0f);
+  b.rays[0][2 * (size_t)i] = make_float4(o.x, o.y, o.z, gg::kRayMax);
+  b.rays[0][2 * (size_t)i + 1] = make_float4(d.x, d.y, d.z, 0.0f);
 }
 
 // One bounce of pt_bounce for queue entry i (cur: the queue being shaded).
@@ -129,10 +150,73 @@ __device__ __forceinline__ bool wf_shade_one(const DrawParams& p, const Wavefron
   return alive && bounce + 1 < p.maxPathLength;
 }
 
+// The light cache of a BPT draw and its entry count (a device word).
+struct WfBpt {
+  const LightCacheEntryDev* cache = nullptr;
+  const uint32_t* numEntries = nullptr;
+};
+
+// One bounce of bpt_eye_bounce (light_integrators.hip) for queue entry i: the
+// connection ray becomes the entry's any-hit ray and its value the entry's NEE
+// record (w = 1 when a ray is traced).  Returns whether the path goes on; then
+// (o, d) is its next ray.  actualN: light-cache entries in use.
+__device__ __forceinline__ bool wf_bpt_shade_one(const DrawParams& p, const WavefrontBuffers& b, int cur, uint32_t i,
+                                                 const LightCacheEntryDev* __restrict__ cache, uint32_t actualN,
+                                                 uint32_t& pixOut, gg::V3& oOut, gg::V3& dOut) {
+  const uint32_t pix = b.queue[cur][i];
+  pixOut = pix;
+  const float4 r0 = b.rays[cur][2 * (size_t)i], r1 = b.rays[cur][2 * (size_t)i + 1];
+  const int4 rec = b.hits[i];
+  const ClosestHit h = closest_from_record(p, rec.x, __int_as_float(rec.y), __int_as_float(rec.z),
+                                           __int_as_float(rec.w));
+  float4 sr0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), sr1 = make_float4(0.0f, 0.0f, 1.0f, 0.0f);
+  float4 value = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  bool alive = false;
+  BptEyePath s;
+  s.o = gg::v3(r0.x, r0.y, r0.z);
+  s.d = gg::v3(r1.x, r1.y, r1.z);
+  if (h.hit) {  // a miss ends the path and adds nothing (pathtracer_bidir.comp:52-55)
+    const float4 c4 = b.colorSeed[pix], t4 = b.tpFlags[pix];
+    const float2 ms = b.misState[pix];
+    s.color = gg::v3(c4.x, c4.y, c4.z);
+    s.seed = __float_as_uint(c4.w);
+    s.tp = gg::v3(t4.x, t4.y, t4.z);
+    s.bounce = (int)(__float_as_uint(t4.w) >> 1);
+    s.lastConnectionProbability = ms.x;
+    s.lastCosAbs = ms.y;
+    s.mis = 0.0f;
+    BptEyeVertex v;
+    bpt_eye_hit(p, s, h, v);
+    const BptConnection cn = bpt_connect_setup(p, s, cache, actualN);
+    if (cn.test) {
+      sr0 = make_float4(cn.lo.x, cn.lo.y, cn.lo.z, cn.tmax);
+      sr1 = make_float4(cn.ld.x, cn.ld.y, cn.ld.z, 0.0f);
+      const gg::V3 add = bpt_connect_value(p, s, v, cn, (float)actualN);
+      value = make_float4(add.x, add.y, add.z, 1.0f);
+    }
+    alive = !bpt_eye_continue(p, s, v);
+    b.colorSeed[pix] = make_float4(s.color.x, s.color.y, s.color.z, __uint_as_float(s.seed));
+    b.tpFlags[pix] = make_float4(s.tp.x, s.tp.y, s.tp.z, __uint_as_float((uint32_t)s.bounce << 1));
+    b.misState[pix] = make_float2(s.lastConnectionProbability, s.lastCosAbs);
+  }
+  b.shadowRays[2 * (size_t)i] = sr0;
+  b.shadowRays[2 * (size_t)i + 1] = sr1;
+  b.neeValue[i] = value;
+  oOut = s.o;
+  dOut = s.d;
+  return alive;
+}
+
 // The shade pass over the queue's device length; paths that go on are appended
 // to the next queue, one atomic per wave and in lane order within the wave.
-__global__ __launch_bounds__(kWfBlock) void k_wf_shade(DrawParams p, WavefrontBuffers b, int cur) {
+template <int MODE>
+__global__ __launch_bounds__(kWfBlock) void k_wf_shade(DrawParams p, WavefrontBuffers b, int cur, WfBpt bpt) {
   const uint32_t count = b.counters[cur];
+  uint32_t actualN = 0;
+  if (MODE == kWfBPT) {  // light-cache entries in use, as k_bpt_eye reads them
+    const int numEntries = (int)*bpt.numEntries;
+    actualN = (uint32_t)(numEntries < p.bptCapacity ? numEntries : p.bptCapacity);
+  }
   const uint32_t lane = threadIdx.x & 63u;
   const uint64_t below = (1ull << lane) - 1ull;
   // every lane of a wave runs the same trips: i - lane is uniform per wave
@@ -140,7 +224,8 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_shade(DrawParams p, WavefrontBu
     const uint32_t i = base + lane;
     uint32_t pix = 0;
     gg::V3 o = gg::v3s(0.0f), d = gg::v3s(0.0f);
-    const bool push = i < count && wf_shade_one(p, b, cur, i, pix, o, d);
+    const bool push = i < count && (MODE == kWfPT ? wf_shade_one(p, b, cur, i, pix, o, d)
+                                                  : wf_bpt_shade_one(p, b, cur, i, bpt.cache, actualN, pix, o, d));
     const uint64_t m = __ballot(push);
     if (m == 0) continue;
     const int leader = __ffsll((unsigned long long)m) - 1;
@@ -156,13 +241,24 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_shade(DrawParams p, WavefrontBu
   }
 }
 
-// color += radiance * throughput * weight, radiance = 0 when the shadow ray was
-// blocked or there was nothing to connect to (as estimate_direct_light returns).
+// PT: color += radiance * throughput * weight, radiance = 0 when the shadow ray
+// was blocked or there was nothing to connect to (as estimate_direct_light
+// returns).  BPT: color += the connection's value when its ray was not blocked
+// (nothing otherwise, as bpt_eye_bounce).
+template <int MODE>
 __global__ __launch_bounds__(kWfBlock) void k_wf_resolve(WavefrontBuffers b, int cur) {
   const uint32_t count = b.counters[cur];
   for (uint32_t i = blockIdx.x * kWfBlock + threadIdx.x; i < count; i += gridDim.x * kWfBlock) {
     const float4 v = b.neeValue[i];
     if (v.w == 0.0f) continue;  // a miss: no NEE term
+    if (MODE == kWfBPT) {
+      if (b.occluded[i].x != 0) continue;
+      const uint32_t pix = b.queue[cur][i];
+      const float4 c4 = b.colorSeed[pix];
+      const gg::V3 color = gg::v3(c4.x, c4.y, c4.z) + gg::v3(v.x, v.y, v.z);
+      b.colorSeed[pix] = make_float4(color.x, color.y, color.z, c4.w);
+      continue;
+    }
     const bool visible = v.w == 1.0f && b.occluded[i].x == 0;
     const gg::V3 radiance = visible ? gg::v3(v.x, v.y, v.z) : gg::v3s(0.0f);
     const float4 w = b.neeWeight[i];
@@ -173,6 +269,8 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_resolve(WavefrontBuffers b, int
   }
 }
 
+// accum += vec4(color, 0) (PT) or vec4(color, 1) (BPT eye pass)
+template <int MODE>
 __global__ __launch_bounds__(kWfBlock) void k_wf_finish(DrawParams p, WavefrontBuffers b, uint32_t tilesW,
                                                         uint32_t numPaths) {
   const uint32_t i = blockIdx.x * kWfBlock + threadIdx.x;
@@ -183,7 +281,7 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_finish(DrawParams p, WavefrontB
   const float4 c = b.colorSeed[pix];
   float4* px = &p.accum[pix];
   const float4 a = *px;
-  *px = make_float4(c.x + a.x, c.y + a.y, c.z + a.z, 0.0f + a.w);  // vec4(color, 0) + imageLoad
+  *px = make_float4(c.x + a.x, c.y + a.y, c.z + a.z, (MODE == kWfPT ? 0.0f : 1.0f) + a.w);
 }
 
 // counters[0] = the first queue's length, counters[1] = 0
@@ -198,9 +296,10 @@ dim3 Grid(uint32_t n) { return dim3((n + kWfBlock - 1) / kWfBlock); }
 }  // namespace
 
 size_t WavefrontBytes(size_t numPixels) {
-  // color and random state, throughput and flags (16 B each) per pixel; per queue
-  // slot: two queues and two ray arrays, hits, shadow rays, occlusion, NEE value and weight
-  return numPixels * (16 + 16) + numPixels * (4 * 2 + 32 * 2 + 16 + 32 + 16 + 16 + 16) +
+  // color and random state, throughput and flags (16 B each) and the BPT MIS
+  // state (8 B) per pixel; per queue slot: two queues and two ray arrays, hits,
+  // shadow rays, occlusion, NEE value and weight
+  return numPixels * (16 + 16 + 8) + numPixels * (4 * 2 + 32 * 2 + 16 + 32 + 16 + 16 + 16) +
          16 * 256;  // the counters and each buffer's rounding to 256 bytes
 }
 
@@ -215,6 +314,7 @@ WavefrontBuffers CarveWavefront(void* base, size_t numPixels) {
   b.counters = (uint32_t*)take(256);
   b.colorSeed = (float4*)take(numPixels * 16);
   b.tpFlags = (float4*)take(numPixels * 16);
+  b.misState = (float2*)take(numPixels * 8);
   for (int k = 0; k < 2; ++k) {
     b.queue[k] = (uint32_t*)take(numPixels * 4);
     b.rays[k] = (float4*)take(numPixels * 32);
@@ -227,13 +327,16 @@ WavefrontBuffers CarveWavefront(void* base, size_t numPixels) {
   return b;
 }
 
-hipError_t DrawPTWavefront(const DrawParams& p, const WavefrontTrace& tr, const WavefrontBuffers& b,
-                           hipStream_t stream) {
+namespace {
+
+template <int MODE>
+hipError_t DrawWavefront(const DrawParams& p, const WavefrontTrace& tr, const WavefrontBuffers& b, WfBpt bpt,
+                         hipStream_t stream) {
   const uint32_t tilesW = (uint32_t)(p.width / kTile), tilesH = (uint32_t)(p.height / kTile);
   const uint32_t numPaths = tilesW * tilesH * kTileThreads;
   if (numPaths == 0 || p.maxPathLength <= 0) return hipSuccess;
   if (tr.passBlocks == 0) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_wf_raygen, Grid(numPaths), dim3(kWfBlock), 0, stream, p, b, tilesW, numPaths);
+  hipLaunchKernelGGL(k_wf_raygen<MODE>, Grid(numPaths), dim3(kWfBlock), 0, stream, p, b, tilesW, numPaths);
   hipLaunchKernelGGL(k_wf_init_counts, dim3(1), dim3(64), 0, stream, b.counters, numPaths);
   // resident blocks of the shade / resolve passes, at most one per 256 paths
   const dim3 passGrid(std::min<uint32_t>(tr.passBlocks, (numPaths + kWfBlock - 1) / kWfBlock));
@@ -257,7 +360,7 @@ hipError_t DrawPTWavefront(const DrawParams& p, const WavefrontTrace& tr, const 
     // that this shade pass overwrites (and the counter k_wf_zero clears)
     if (fork && bounce > 0 && (e = hipStreamWaitEvent(stream, tr.evResolve, 0)) != hipSuccess) return e;
     hipLaunchKernelGGL(k_wf_zero, dim3(1), dim3(1), 0, stream, b.counters + (1 - cur));
-    hipLaunchKernelGGL(k_wf_shade, passGrid, dim3(kWfBlock), 0, stream, p, b, cur);
+    hipLaunchKernelGGL(k_wf_shade<MODE>, passGrid, dim3(kWfBlock), 0, stream, p, b, cur, bpt);
     if (fork) {
       if ((e = hipEventRecord(tr.evShade, stream)) != hipSuccess) return e;
       if ((e = hipStreamWaitEvent(tr.side, tr.evShade, 0)) != hipSuccess) return e;
@@ -268,13 +371,29 @@ hipError_t DrawPTWavefront(const DrawParams& p, const WavefrontTrace& tr, const 
                 : LaunchTraceRef(p.ref, GPUGI_ANY_HIT, shadow, (gpugi_hit*)b.occluded, numPaths, p.watertight,
                                  shadowStream, count);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_wf_resolve, passGrid, dim3(kWfBlock), 0, shadowStream, b, cur);
+    hipLaunchKernelGGL(k_wf_resolve<MODE>, passGrid, dim3(kWfBlock), 0, shadowStream, b, cur);
     if (fork && (e = hipEventRecord(tr.evResolve, tr.side)) != hipSuccess) return e;
     cur = 1 - cur;
   }
   if (fork && (e = hipStreamWaitEvent(stream, tr.evResolve, 0)) != hipSuccess) return e;  // join
-  hipLaunchKernelGGL(k_wf_finish, Grid(numPaths), dim3(kWfBlock), 0, stream, p, b, tilesW, numPaths);
+  hipLaunchKernelGGL(k_wf_finish<MODE>, Grid(numPaths), dim3(kWfBlock), 0, stream, p, b, tilesW, numPaths);
   return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t DrawPTWavefront(const DrawParams& p, const WavefrontTrace& tr, const WavefrontBuffers& b,
+                           hipStream_t stream) {
+  return DrawWavefront<kWfPT>(p, tr, b, WfBpt{}, stream);
+}
+
+hipError_t DrawBPTWavefront(const DrawParams& p, const WavefrontTrace& tr, const WavefrontBuffers& b,
+                            const LightCacheEntryDev* cache, const uint32_t* numEntries, hipStream_t stream) {
+  if (!cache || !numEntries) return hipErrorInvalidValue;
+  WfBpt bpt;
+  bpt.cache = cache;
+  bpt.numEntries = numEntries;
+  return DrawWavefront<kWfBPT>(p, tr, b, bpt, stream);
 }
 
 }  // namespace gpugi

@@ -205,17 +205,20 @@ int gpugi_set_specific_path_length(gpugi_ctx* ctx, int length);
  * walk) or GPUGI_KERNEL_WIDE / GPUGI_KERNEL_AUTO (4-wide quantized BVH).
  * Replaces the AABOX_BVH define of the shader compile (pathtracer.cpp:55-64). */
 int gpugi_set_integrator_kernel(gpugi_ctx* ctx, int kernel);
-/* How gpugi_draw(GPUGI_PT) schedules the paths: GPUGI_PT_MEGAKERNEL (default;
- * one lane carries a path through all its bounces, as the reference's
- * pathtracer.comp dispatch does) or GPUGI_PT_WAVEFRONT (one pass per bounce
- * over the live paths: closest-hit and shadow rays traced by the gpugi_trace
- * kernels on compact device queues whose lengths every pass reads on the
- * device, so the host never waits; see gpugi_capture_draw; each bounce's
- * shadow trace runs on a second stream of the context beside the next
- * closest-hit trace, joined back into the context's stream).  Both give the
- * same image with the reference walk; the wavefront schedule with the
- * reference walk and GPUGI_TRIANGLE_WATERTIGHT rebuilds b_x as 1 - (b_y + b_z)
- * from the hit record.  No counterpart in the reference. */
+/* How gpugi_draw(GPUGI_PT) schedules the paths, and gpugi_draw(GPUGI_BPT) its
+ * eye paths: GPUGI_PT_MEGAKERNEL (default; one lane carries a path through
+ * all its bounces, as the reference's pathtracer.comp and
+ * pathtracer_bidir.comp dispatches do) or GPUGI_PT_WAVEFRONT (one pass per
+ * bounce over the live paths: closest-hit rays and shadow / light-cache
+ * connection rays traced by the gpugi_trace kernels on compact device queues
+ * whose lengths every pass reads on the device, so the host never waits; see
+ * gpugi_capture_draw; each bounce's shadow or connection trace runs on a
+ * second stream of the context beside the next closest-hit trace, joined back
+ * into the context's stream).  Both give the same image with the reference
+ * walk (BPT: up to the order of the light pass's float-atomic camera
+ * splats); the wavefront schedule with the reference walk and
+ * GPUGI_TRIANGLE_WATERTIGHT rebuilds b_x as 1 - (b_y + b_z) from the hit
+ * record.  No counterpart in the reference. */
 int gpugi_set_pt_schedule(gpugi_ctx* ctx, int schedule);
 /* Measurement helper, no counterpart in the reference: launches the VALU
  * calibration kernel (blocks x 256 lanes, eight independent FP32 FMA chains

@@ -57,7 +57,10 @@ def bpt_capacity(rt, oracle, ctx, sc, sampler, cam):
     return mean + mean // 10
 
 
-def test_bpt_matches_oracle(rt, oracle, ctx):
+@pytest.mark.parametrize("schedule", ["megakernel", "wavefront"])
+def test_bpt_matches_oracle(rt, oracle, ctx, schedule):
+    """The eye paths run as one kernel or as the wavefront passes (their connection
+    rays traced by the any-hit kernel); either way the image matches the oracle."""
     hs = rt.HostScene.cornell()
     setup(rt, ctx, hs, rt.KERNEL_REFERENCE)
     sc = oracle.Scene(hs.arrays())
@@ -66,16 +69,21 @@ def test_bpt_matches_oracle(rt, oracle, ctx):
     cap = bpt_capacity(rt, oracle, ctx, sc, sampler, cam)
     ctx.bpt_set_light_cache(cap)
     ctx.reset_accum()
+    ctx.set_pt_schedule(rt.PT_WAVEFRONT if schedule == "wavefront" else rt.PT_MEGAKERNEL)
+    print("case: BPT %s" % schedule, flush=True)
     acc = np.zeros((H, W, 4), np.float32)
-    for it in range(2):
-        ls = sampler.generate(128)
-        ctx.set_light_samples(ls)
-        ctx.draw(rt.BPT, rt.wanghash(it))
-        acc, cache, ncpu = oracle.bpt_draw(sc, ls, cam, W, H, rt.wanghash(it), L, cap, acc)
-        gcache, ngpu = ctx.bpt_read_light_cache(cap)
-        assert ngpu == ncpu
-        m = min(ngpu, cap)
-        assert np.array_equal(gcache[:m].view(np.uint32).reshape(m, 16), cache[:m].view(np.uint32))
+    try:
+        for it in range(2):
+            ls = sampler.generate(128)
+            ctx.set_light_samples(ls)
+            ctx.draw(rt.BPT, rt.wanghash(it))
+            acc, cache, ncpu = oracle.bpt_draw(sc, ls, cam, W, H, rt.wanghash(it), L, cap, acc)
+            gcache, ngpu = ctx.bpt_read_light_cache(cap)
+            assert ngpu == ncpu
+            m = min(ngpu, cap)
+            assert np.array_equal(gcache[:m].view(np.uint32).reshape(m, 16), cache[:m].view(np.uint32))
+    finally:
+        ctx.set_pt_schedule(rt.PT_MEGAKERNEL)
     img, n = ctx.readback(W, H)
     assert n == 2 and (img[..., 3] == 2.0).all()
     ok = close(img, acc)

@@ -5,8 +5,8 @@ gpugi_set_specific_path_length replaces the reference's SHOW_SPECIFIC_PATHLENGTH
 define (pathtracer.comp:74-84, lighttracer.comp:133-157,
 pathtracer_bidir.comp:75-108); gpugi_set_rr_start_depth starts Russian
 roulette at a later bounce (the clutter room's roulette from bounce 3).  PT is
-bit-exact in both schedules; LT and BPT splat with float atomics and agree to
-1e-4 relative, their light caches bit for bit.
+bit-exact in both schedules; LT and BPT (both eye-path schedules) splat with
+float atomics and agree to 1e-4 relative, their light caches bit for bit.
 """
 import numpy as np
 import pytest
@@ -90,12 +90,14 @@ def test_lt_path_options_match_oracle(rt, oracle, ctx, options, spl, rr):
     assert ok.all(), np.argwhere(~ok)[:5]
 
 
+@pytest.mark.parametrize("schedule", ["megakernel", "wavefront"])
 @pytest.mark.parametrize("spl,rr", [(2, 0), (3, 3)])
-def test_bpt_path_options_match_oracle(rt, oracle, ctx, options, spl, rr):
-    print("case: BPT spl=%d rr=%d" % (spl, rr), flush=True)
+def test_bpt_path_options_match_oracle(rt, oracle, ctx, options, schedule, spl, rr):
+    print("case: BPT %s spl=%d rr=%d" % (schedule, spl, rr), flush=True)
     hs = rt.HostScene.cornell()
     setup(ctx, hs, rt)
     options(spl, rr)
+    ctx.set_pt_schedule(rt.PT_WAVEFRONT if schedule == "wavefront" else rt.PT_MEGAKERNEL)
     sc = oracle.Scene(hs.arrays())
     cam = workloads.cornell_camera(W)
     sampler = rt.LightSampler(hs, 123)

@@ -8,7 +8,9 @@ tie pixels (at least 99 % of the pixels bit for bit, the means within 1e-3).
 With the reference walk, roulette from depth 3, path length 1, the clutter
 room, the environment map on misses, point lights and textured materials
 equal the one-kernel schedule bit for bit.  A frame recorded into a HIP graph
-replays to the eager frame bit for bit (Cornell box, both walks)."""
+replays to the eager frame bit for bit (Cornell box, both walks).  BPT's eye
+paths under this schedule match the one-kernel eye pass on the clutter room
+(per pixel to the float-atomic order of the light pass's camera splats)."""
 import numpy as np
 import pytest
 
@@ -111,6 +113,46 @@ def test_wavefront_depth_settings_match_megakernel(rt, ctx, wctx, path_length, r
 def test_schedule_rejects_unknown_value(rt, wctx):
     with pytest.raises(Exception):
         wctx.set_pt_schedule(7)
+
+
+@pytest.mark.parametrize("kernel", ["reference", "wide"])
+def test_wavefront_bpt_clutter_matches_megakernel(rt, ctx, wctx, kernel):
+    """BPT on the clutter room with its eye paths under both schedules.  The light
+    pass is the same code; its camera splats are float atomics, so the images
+    agree per pixel to 1e-4 relative (reference walk), and on all but tie pixels
+    (wide walk)."""
+    hs = rt.HostScene.clutter(300, workloads.MASTER_SEED)
+    w, h = 192, 112
+    k = rt.KERNEL_REFERENCE if kernel == "reference" else rt.KERNEL_WIDE
+    print("case: wavefront BPT clutter %s" % kernel, flush=True)
+    imgs = []
+    for c in (ctx, wctx):
+        c.load_scene(hs)
+        c.set_screen(w, h)
+        c.set_camera(workloads.clutter_camera(w, h))
+        c.set_max_path_length(workloads.CLUTTER_PATH_LENGTH)
+        c.set_rr_start_depth(workloads.CLUTTER_RR_START)
+        c.set_integrator_kernel(k)
+        sampler = rt.LightSampler(hs, 7)
+        c.set_light_samples(sampler.generate(128))
+        entries = c.bpt_warmup(rt.wanghash(1))[0] & 0xffffffff  # deterministic: the same on both contexts
+        c.bpt_set_light_cache(entries + entries // 10)
+        c.reset_accum()
+        for it in range(2):
+            c.set_light_samples(sampler.generate(128))
+            c.draw(rt.BPT, rt.wanghash(it + 5))
+        imgs.append(c.readback(w, h)[0])
+        c.set_rr_start_depth(0)
+    ctx.set_integrator_kernel(rt.KERNEL_AUTO)
+    a, b = imgs
+    print("means", a[..., :3].mean(), b[..., :3].mean(), flush=True)
+    assert a[..., :3].mean() > 1e-3 and (a[..., 3] == 2.0).all() and (b[..., 3] == 2.0).all()
+    ok = (np.abs(b - a) <= 1e-4 * np.abs(a) + 1e-6).all(-1)
+    if kernel == "reference":
+        assert ok.all(), np.argwhere(~ok)[:5]
+    else:
+        assert ok.mean() > 0.99
+        assert abs(a[..., :3].mean() - b[..., :3].mean()) <= 1e-3 * a[..., :3].mean()
 
 
 @pytest.mark.parametrize("kernel", ["reference", "wide"])
