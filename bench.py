@@ -720,22 +720,19 @@ def summarize_kernels(name, kt_dir, bench_args):
 
 def summarize_wavefront(kt_dir):
     """Writes profiles/wavefront_pt_clutter_kernel_trace.txt: per-kernel totals of a kernel trace of
-    `bench.py --clutter`, and the kernels of its last wavefront path-tracer draw bounce by bounce."""
+    `bench.py --clutter`, and the kernels of its last wavefront path-tracer draw and its last
+    wavefront BPT draw (eye paths) bounce by bounce."""
     stats = _csv_rows(kt_dir, "_kernel_stats.csv")
     stats.sort(key=lambda r: -float(r["TotalDurationNs"]))
     rows = _csv_rows(kt_dir, "_kernel_trace.csv")
     rows.sort(key=lambda r: int(r["Start_Timestamp"]))
     names = [_short_kernel(r["Kernel_Name"]) for r in rows]
-    raygen = [i for i, n in enumerate(names) if n == "k_wf_raygen"]
-    if not raygen:
-        raise SystemExit("bench.py --summarize-wavefront: no wavefront draw in the trace")
-    first = raygen[-1]
-    last = next(i for i in range(first, len(names)) if names[i] == "k_wf_finish")
     dur = [(int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3 for r in rows]
-    lines = ["Wavefront path tracer, clutter room (5,120,016 triangles), 1920x1080, path length 8,",
+    lines = ["Wavefront schedule, clutter room (5,120,016 triangles), 1920x1080, path length 8,",
              "roulette from bounce 3, wide traversal; every pass reads its queue length on the device.",
-             "Each bounce's shadow trace and resolve pass run on a second stream, beside the next",
-             "bounce's closest-hit trace, so kernel durations overlap (their sum exceeds the draw's span).",
+             "Each bounce's shadow (PT) or light-cache connection (BPT) trace and its resolve pass run on",
+             "a second stream, beside the next bounce's closest-hit trace, so kernel durations overlap",
+             "(their sum exceeds the draw's span).  Mode <0> of the pass kernels is PT, <1> BPT.",
              "",
              "Commands (repo root; the first after cd /tmp && export TMPDIR=/tmp):",
              "  rocprofv3 --kernel-trace --stats -d <out>/wfprof -o run --output-format csv -- "
@@ -745,26 +742,38 @@ def summarize_wavefront(kt_dir):
              "Per-kernel totals over the run (k_trace_wide launches include the trace benchmark before the "
              "integrators):",
              "%-60s %6s %10s %10s" % ("kernel", "calls", "total ms", "avg us")]
-    for r in stats[:16]:
+    for r in stats[:20]:
         lines.append("%-60s %6d %10.3f %10.1f" % (_short_kernel(r["Name"])[:60], int(r["Calls"]),
                                                  float(r["TotalDurationNs"]) / 1e6, float(r["AverageNs"]) / 1e3))
-    lines += ["", "The last wavefront draw, per bounce (kernel durations in us; late bounces launch over the",
-              "full queue capacity and their kernels exit once the device count is reached):"]
-    bounce = 0
-    for i in range(first, last):
-        if names[i] != "k_wf_shade":
-            continue
-        ch = max(j for j in range(first, i) if names[j].startswith("k_trace_wide<false") or
-                 names[j].startswith("k_trace_ref<false"))
-        sh = min((j for j in range(i, last) if names[j].startswith("k_trace_wide<true") or
-                  names[j].startswith("k_trace_ref<true")), default=None)
-        rs = min((j for j in range(i, last) if names[j] == "k_wf_resolve"), default=None)
-        lines.append("  bounce %d:  closest-hit trace %8.1f  shade %7.1f  shadow trace %7.1f  resolve %6.1f" % (
-            bounce, dur[ch], dur[i], dur[sh] if sh is not None else 0.0, dur[rs] if rs is not None else 0.0))
-        bounce += 1
-    span = (int(rows[last]["End_Timestamp"]) - int(rows[first]["Start_Timestamp"])) / 1e6
-    busy = sum(dur[first:last + 1]) / 1e3
-    lines.append("  draw: %.2f ms from the first to the last kernel, %.2f ms of kernel time" % (span, busy))
+    for mode, title in ((0, "path-tracer"), (1, "BPT")):
+        raygen = [i for i, n in enumerate(names) if n == "k_wf_raygen<%d>" % mode]
+        if not raygen:
+            raise SystemExit("bench.py --summarize-wavefront: no wavefront %s draw in the trace" % title)
+        first = raygen[-1]
+        last = next(i for i in range(first, len(names)) if names[i] == "k_wf_finish<%d>" % mode)
+        lines += ["", "The last wavefront %s draw, per bounce (kernel durations in us; late bounces launch over the"
+                  % title, "full queue capacity and their kernels exit once the device count is reached):"]
+        if mode == 1:  # the light pass that precedes the eye paths
+            lp = max(j for j in range(first) if names[j].startswith("k_lighttracer<"))
+            lines.append("  light pass (%s): %.1f us, up to the eye paths' ray generation %.2f ms" % (
+                names[lp], dur[lp], (int(rows[first]["Start_Timestamp"]) - int(rows[lp]["Start_Timestamp"])) / 1e6))
+        bounce = 0
+        for i in range(first, last):
+            if names[i] != "k_wf_shade<%d>" % mode:
+                continue
+            ch = max(j for j in range(first, i) if names[j].startswith("k_trace_wide<false") or
+                     names[j].startswith("k_trace_ref<false"))
+            sh = min((j for j in range(i, last) if names[j].startswith("k_trace_wide<true") or
+                      names[j].startswith("k_trace_ref<true")), default=None)
+            rs = min((j for j in range(i, last) if names[j] == "k_wf_resolve<%d>" % mode), default=None)
+            lines.append("  bounce %d:  closest-hit trace %8.1f  shade %7.1f  %s trace %7.1f  resolve %6.1f" % (
+                bounce, dur[ch], dur[i], "shadow" if mode == 0 else "connection",
+                dur[sh] if sh is not None else 0.0, dur[rs] if rs is not None else 0.0))
+            bounce += 1
+        span = (int(rows[last]["End_Timestamp"]) - int(rows[first]["Start_Timestamp"])) / 1e6
+        busy = sum(dur[first:last + 1]) / 1e3
+        lines.append("  %s: %.2f ms from the first to the last kernel, %.2f ms of kernel time" % (
+            "draw" if mode == 0 else "eye paths", span, busy))
     text = "\n".join(lines) + "\n"
     with open(os.path.join(ROOT, "profiles", "wavefront_pt_clutter_kernel_trace.txt"), "w") as f:
         f.write(text)
