@@ -154,6 +154,13 @@ __device__ __forceinline__ gg::V3 bpt_connect_value(const DrawParams& p, const B
   return irradiance * s.tp * bsdf;
 }
 
+// A connection whose value is exactly zero (for example to a light vertex
+// behind an opaque surface, where the BSDF is zero) leaves the color as it is
+// whether or not its ray is blocked: the color starts at +0 and only gains
+// non-negative terms, so adding a zero never changes its bits.  Its ray is not
+// traced.  (A NaN value fails the test and is traced and added as before.)
+__device__ __forceinline__ bool bpt_value_is_zero(gg::V3 v) { return v.x == 0.0f && v.y == 0.0f && v.z == 0.0f; }
+
 // The rest of the bounce (pathtracer_bidir.comp:207-247): the next direction,
 // roulette from rrStartDepth on, the MIS state; true when the path has ended.
 __device__ __forceinline__ bool bpt_eye_continue(const DrawParams& p, BptEyePath& s, const BptEyeVertex& v) {

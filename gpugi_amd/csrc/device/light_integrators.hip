@@ -56,7 +56,6 @@ __device__ __forceinline__ void project_to_screen(const TraceCtx& c, gg::V3 rayO
   const float ppx = (sx * 0.5f + 0.5f) * (float)p.width;
   const float ppy = (sy * 0.5f + 0.5f) * (float)p.height;
   const int ix = (int)roundf(ppx), iy = (int)roundf(ppy);
-  if (trace_any<WIDE>(c, co, crd, cameraDist)) return;
   const float cosAtCamera = gg::saturate(camViewDotRay / cameraDist);
   const float f2i = adjoint_corrected_out_dot_n(rayD, crd, gN, sN) /
                     (cosAtCamera * cosAtCamera * cosAtCamera * p.camera.pixel_area * cameraDistSq);
@@ -69,8 +68,12 @@ __device__ __forceinline__ void project_to_screen(const TraceCtx& c, gg::V3 rayO
     bsdf = bsdf * mis;
   }
   const gg::V3 color = tp * f2i * bsdf;
-  // Splats outside the image are dropped, as image atomics outside the image are.
+  // Splats outside the image are dropped, as image atomics outside the image
+  // are, and a splat of exactly zero changes nothing (the accumulated sums
+  // are never -0): neither needs the camera ray, which is traced last.
   if (ix < 0 || ix >= p.width || iy < 0 || iy >= p.height) return;
+  if (color.x == 0.0f && color.y == 0.0f && color.z == 0.0f) return;
+  if (trace_any<WIDE>(c, co, crd, cameraDist)) return;
   float* px = reinterpret_cast<float*>(&p.accum[(size_t)iy * p.width + ix]);
   splat_add(px + 0, color.x);
   splat_add(px + 1, color.y);
@@ -279,8 +282,10 @@ __device__ __forceinline__ bool bpt_eye_bounce(const DrawParams& p, const TraceC
   BptEyeVertex v;
   bpt_eye_hit(p, s, h, v);
   const BptConnection cn = bpt_connect_setup(p, s, cache, actualN);
-  if (cn.test && !trace_any<WIDE>(c, cn.lo, cn.ld, cn.tmax))
-    s.color = s.color + bpt_connect_value(p, s, v, cn, lightSampleWeight);
+  if (cn.test) {  // the value first: a zero one needs no ray (bpt_value_is_zero)
+    const gg::V3 add = bpt_connect_value(p, s, v, cn, lightSampleWeight);
+    if (!bpt_value_is_zero(add) && !trace_any<WIDE>(c, cn.lo, cn.ld, cn.tmax)) s.color = s.color + add;
+  }
   return bpt_eye_continue(p, s, v);
 }
 

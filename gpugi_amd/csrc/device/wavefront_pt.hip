@@ -189,10 +189,12 @@ __device__ __forceinline__ bool wf_bpt_shade_one(const DrawParams& p, const Wave
     bpt_eye_hit(p, s, h, v);
     const BptConnection cn = bpt_connect_setup(p, s, cache, actualN);
     if (cn.test) {
-      sr0 = make_float4(cn.lo.x, cn.lo.y, cn.lo.z, cn.tmax);
-      sr1 = make_float4(cn.ld.x, cn.ld.y, cn.ld.z, 0.0f);
       const gg::V3 add = bpt_connect_value(p, s, v, cn, (float)actualN);
-      value = make_float4(add.x, add.y, add.z, 1.0f);
+      if (!bpt_value_is_zero(add)) {  // a zero term needs no ray
+        sr0 = make_float4(cn.lo.x, cn.lo.y, cn.lo.z, cn.tmax);
+        sr1 = make_float4(cn.ld.x, cn.ld.y, cn.ld.z, 0.0f);
+        value = make_float4(add.x, add.y, add.z, 1.0f);
+      }
     }
     alive = !bpt_eye_continue(p, s, v);
     b.colorSeed[pix] = make_float4(s.color.x, s.color.y, s.color.z, __uint_as_float(s.seed));
