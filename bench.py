@@ -54,6 +54,9 @@ def parse():
                    help="write the profiles/ summaries from rocprofv3 output directories and exit")
     p.add_argument("--summarize-counters", nargs="+", metavar="NAME_THEN_DIRS",
                    help="write profiles/<NAME>.json from rocprofv3 --pmc output directories and exit")
+    p.add_argument("--summarize-integrators", nargs="+", metavar="PMC_DIR",
+                   help="write profiles/integrators_clutter_counters.json from rocprofv3 --pmc runs of --clutter "
+                        "and exit")
     p.add_argument("--summarize-kernels", nargs=2, metavar=("NAME", "KT_DIR"),
                    help="write profiles/NAME.txt: per-kernel totals of a kernel trace of bench.py with the "
                         "arguments given by --profiled-args=..., and exit")
@@ -275,6 +278,67 @@ def summarize_counters(name, *dirs):
         json.dump(out, f, indent=1)
         f.write("\n")
     print(json.dumps(out))
+
+
+INTEGRATOR_KERNELS = ("k_pathtracer<true>", "k_lighttracer<true, 0>", "k_lighttracer<true, 1>", "k_bpt_eye<true>",
+                      "k_wf_shade<0>", "k_wf_shade<1>", "k_trace_wide<false, 256, 10, false>",
+                      "k_trace_wide<true, 256, 10, false>")
+
+
+def summarize_integrators(*dirs):
+    """Writes profiles/integrators_clutter_counters.json: per-dispatch averages of the counters that
+    rocprofv3 --pmc runs of `bench.py --clutter` collected for each integrator kernel (the one-kernel
+    PT, LT, BPT light pass and eye pass, the wavefront shade passes and the trace kernels they launch),
+    with the utilisation figures derived from them."""
+    per = {k: {} for k in INTEGRATOR_KERNELS}
+    calls = {k: 0 for k in INTEGRATOR_KERNELS}
+    for d in dirs:
+        acc = {k: {} for k in INTEGRATOR_KERNELS}
+        for r in _csv_rows(d, "_counter_collection.csv"):
+            k = _short_kernel(r["Kernel_Name"])
+            if k in acc:
+                acc[k].setdefault(r["Counter_Name"], []).append(float(r["Counter_Value"]))
+        for k, counters in acc.items():
+            for c, v in counters.items():
+                per[k][c] = sum(v) / len(v)
+                calls[k] = max(calls[k], len(v))
+    out = {}
+    for k in INTEGRATOR_KERNELS:
+        c = per[k]
+        if not c:
+            continue
+        dv = {}
+        if "GRBM_GUI_ACTIVE" in c:
+            cu_cycles = c["GRBM_GUI_ACTIVE"] / 8.0 * 256.0  # summed over the 8 XCDs; 256 CUs
+            if "SQ_INSTS_VALU" in c:
+                dv["valu_wave_insts_per_cu_cycle"] = c["SQ_INSTS_VALU"] / cu_cycles
+            for key, label in (("TA_TA_BUSY_sum", "ta_busy_frac"), ("TD_TD_BUSY_sum", "td_busy_frac")):
+                if key in c:
+                    dv[label] = c[key] / cu_cycles
+        if "SQ_THREAD_CYCLES_VALU" in c and "SQ_ACTIVE_INST_VALU" in c:
+            # the VALU calibration kernel (all 64 lanes active) reads 64.0 here
+            dv["valu_active_lanes"] = c["SQ_THREAD_CYCLES_VALU"] / c["SQ_ACTIVE_INST_VALU"]
+        if "SQ_WAIT_INST_ANY" in c and "SQ_WAVE_CYCLES" in c:
+            dv["wave_wait_frac"] = c["SQ_WAIT_INST_ANY"] / c["SQ_WAVE_CYCLES"]
+        if "TCC_HIT_sum" in c and "TCC_MISS_sum" in c:
+            dv["l2_hit_rate"] = c["TCC_HIT_sum"] / (c["TCC_HIT_sum"] + c["TCC_MISS_sum"])
+        if "TCP_TOTAL_CACHE_ACCESSES_sum" in c and "TCP_TCC_READ_REQ_sum" in c:
+            dv["l1_hit_rate"] = 1.0 - c["TCP_TCC_READ_REQ_sum"] / c["TCP_TOTAL_CACHE_ACCESSES_sum"]
+        out[k] = {"dispatches": calls[k], "per_dispatch": c, "derived": dv}
+    doc = {"workload": "clutter room (5,120,016 triangles), 1920x1080, path length 8, roulette from bounce 3, "
+                       "wide traversal; per dispatch (the wavefront kernels and the trace kernels average over "
+                       "every bounce's launch)",
+           "commands": ["cd /tmp && export TMPDIR=/tmp && timeout -s KILL 300 rocprofv3 --pmc <counters of one pass> "
+                        "-d <out>/<pass> -o <pass> --output-format csv -- python3 <repo>/bench.py --clutter "
+                        "--clutter-iterations 1 --steps 1",
+                        "python3 <repo>/bench.py --summarize-integrators " + " ".join(
+                            "<out>/" + os.path.basename(os.path.normpath(d)) for d in dirs)],
+           "kernels": out}
+    with open(os.path.join(ROOT, "profiles", "integrators_clutter_counters.json"), "w") as f:
+        json.dump(doc, f, indent=1)
+        f.write("\n")
+    for k, v in out.items():
+        print(k, json.dumps({a: round(b, 3) for a, b in v["derived"].items()}))
 
 
 def batch_benchmark(frames):
@@ -935,6 +999,9 @@ def main():
         return
     if args.summarize_counters:
         summarize_counters(*args.summarize_counters)
+        return
+    if args.summarize_integrators:
+        summarize_integrators(*args.summarize_integrators)
         return
     if args.summarize_mlp:
         summarize_mlp(*args.summarize_mlp)
