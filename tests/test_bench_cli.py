@@ -77,3 +77,16 @@ def test_integrator_counter_formulas(tmp_path, monkeypatch):
     assert d["l2_hit_rate"] == pytest.approx(0.75)
     assert d["l1_hit_rate"] == pytest.approx(0.75)
     assert set(doc["kernels"]) == {"k_pathtracer<true>"}
+
+
+@pytest.mark.parametrize("profile,kernel_ms", [("trace_wide_heightfield_pmc.json", 2.4),
+                                               ("trace_wide_clutter_pmc.json", 5.7)])
+def test_committed_traffic_is_a_physical_roofline_input(profile, kernel_ms):
+    """The HBM traffic the bench line divides by the launch time comes from the committed
+    counter summary: at least the rays read and hits written (2^24 x 48 B), and no more
+    than the 8 TB/s peak moves in a launch of that kernel (a lower bound on its measured
+    duration), so roofline.frac stays a fraction of the peak."""
+    traffic = bench.traffic_from_profile(profile)
+    assert traffic is not None
+    assert traffic >= (1 << 24) * 48
+    assert traffic / (kernel_ms * 1e-3) <= bench.HBM_PEAK_GBS * 1e9
