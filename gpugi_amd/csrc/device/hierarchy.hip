@@ -48,20 +48,28 @@ struct NodeView {
 constexpr int kImpFracBits = 8;
 static_assert(kHierPathLength <= kImpFracBits, "every addend is a whole number of units");
 
-// acc[key] += w for every calling lane, the lanes that share the lowest active
-// lane's (key, w) combined into one atomic of w times their number: every ray
-// of a wave enters the top nodes, which would otherwise take one contended
-// atomic per ray.
+// acc[key] += w for every calling lane.  Lanes with the same (key, w) are
+// combined into one atomic of w times their number, up to kCombineGroups
+// groups per call (each group led by its lowest lane); the lanes left after
+// that add on their own.  The rays of a wave enter the top nodes together and
+// split into a few groups below them, which would otherwise take one
+// contended atomic per ray.
+constexpr int kCombineGroups = 8;
+
 __device__ __forceinline__ void add_importance(unsigned long long* __restrict__ acc, uint32_t key, float w) {
-  const uint64_t active = __ballot(true);
-  const int leader = __builtin_ctzll(active);  // the calling lane is active, so active != 0
-  const uint32_t lk = (uint32_t)__shfl((int)key, leader);
-  const float lw = __shfl(w, leader);
-  const bool same = key == lk && w == lw;
-  const uint64_t group = __ballot(same);
   const unsigned long long units = (unsigned long long)(w * (float)(1 << kImpFracBits));  // exact: w = 2^-k
-  if (!same) atomicAdd(&acc[key], units);
-  else if ((int)(threadIdx.x & 63) == leader) atomicAdd(&acc[key], units * (unsigned long long)__popcll(group));
+  const int lane = (int)(threadIdx.x & 63);
+  uint64_t remaining = __ballot(true);  // wave-uniform: the lanes whose addition is still to be made
+  for (int g = 0; g < kCombineGroups && remaining != 0; ++g) {
+    const int leader = __builtin_ctzll(remaining);  // a lane still in the loop
+    const uint32_t lk = (uint32_t)__shfl((int)key, leader);
+    const float lw = __shfl(w, leader);
+    const uint64_t group = __ballot(key == lk && w == lw) & remaining;
+    if (lane == leader) atomicAdd(&acc[lk], units * (unsigned long long)__popcll(group));
+    remaining &= ~group;
+    if ((group >> lane) & 1ull) return;  // this lane's addition was made
+  }
+  atomicAdd(&acc[key], units);
 }
 
 // TraceRayCountHits (counthits.glsl:1-71) over the reference layout.
