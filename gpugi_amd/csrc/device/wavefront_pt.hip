@@ -175,7 +175,7 @@ __device__ __forceinline__ bool wf_bpt_shade_one(const DrawParams& p, const Wave
   BptEyePath s;
   s.o = gg::v3(r0.x, r0.y, r0.z);
   s.d = gg::v3(r1.x, r1.y, r1.z);
-  if (h.hit) {  // a miss ends the path and adds nothing (pathtracer_bidir.comp:52-55)
+  if (h.hit) {  // a miss ends the path and adds nothing (pathtracer_bidir.comp:62-63)
     const float4 c4 = b.colorSeed[pix], t4 = b.tpFlags[pix];
     const float2 ms = b.misState[pix];
     s.color = gg::v3(c4.x, c4.y, c4.z);
