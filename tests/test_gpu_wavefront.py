@@ -283,3 +283,5 @@ def test_wavefront_frame_graph_replay(rt, oracle, wctx, kernel):
     with pytest.raises(rt.GpugiError):
         wctx.capture_draw(rt.PT, seed)
     wctx.set_pt_schedule(rt.PT_WAVEFRONT)
+    with pytest.raises(rt.GpugiError):  # only PT frames are recorded, also under this schedule
+        wctx.capture_draw(rt.BPT, seed)
