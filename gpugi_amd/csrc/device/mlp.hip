@@ -43,15 +43,41 @@ __device__ __forceinline__ uint32_t relu_bf2(uint32_t x) {
   return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(__builtin_bit_cast(s16x2, x), (s16x2){0, 0}));
 }
 
-// sin / cos of 2^k pi p for k < 12: sinpi / cospi at k = 0, 3, 6, 9 and two
+// sin(pi x) and cos(pi x) for |x| < 2^31.  x = k + t with k = rint(x) and
+// |t| <= 1/2, both exact in FP32; sin(pi t) = pi t + t^3 Q(t^2) and cos(pi t) =
+// 1 + t^2 R(t^2), where Q (degree 3) and R (degree 4) are minimax fits on
+// t^2 in [0, 1/4] (absolute error 8e-9 and 3e-10 before rounding); the
+// parity of k flips both signs.  The FP32 result is within 1.2e-7 of the
+// exact value.  18 VALU instructions against the 40 of sinpif + cospif,
+// which also handle infinities and the sign of zero.
+__device__ __forceinline__ void sincospi_enc(float x, float& sn, float& cs) {
+  const float k = rintf(x);
+  const float t = x - k;
+  const float z = t * t;
+  float q = fmaf(z, 0.0780123249f, -0.59851259f);
+  q = fmaf(z, q, 2.55012512f);
+  q = fmaf(z, q, -5.16771412f);
+  float r = fmaf(z, -0.0244206991f, 0.234953925f);
+  r = fmaf(z, r, -1.33521605f);
+  r = fmaf(z, r, 4.05870962f);
+  r = fmaf(z, r, -4.93480206f);
+  const float s = fmaf(t, 3.14159274f, (t * z) * q);
+  const float c = fmaf(z, r, 1.0f);
+  const uint32_t flip = (uint32_t)(int)k << 31;
+  sn = __uint_as_float(__float_as_uint(s) ^ flip);
+  cs = __uint_as_float(__float_as_uint(c) ^ flip);
+}
+
+// sin / cos of 2^k pi p for k < 12: sincospi_enc at k = 0, 3, 6, 9 and two
 // double-angle steps from each (sin 2t = 2 sin t cos t, cos 2t = (cos t - sin
-// t)(cos t + sin t)); for p in [0, 1] the error stays below 4e-7, far inside
-// the BF16 rounding of the features.
+// t)(cos t + sin t)), each of which doubles the error; for p in [0, 1] the
+// error stays below 7e-7, inside the BF16 rounding of the features.
 __device__ __forceinline__ void encode_axis(float p, float* sn, float* cs) {
 #pragma unroll
   for (int a = 0; a < kMlpFreqs; a += 3) {
     const float x = ldexpf(p, a);  // exact
-    float s = sinpif(x), c = cospif(x);
+    float s, c;
+    sincospi_enc(x, s, c);
     sn[a] = s;
     cs[a] = c;
 #pragma unroll
@@ -166,26 +192,45 @@ constexpr int kInferGroups = 2;   // 16-sample groups per wave step
 
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
+// Lane q's share of one sample's raw features: the position components of
+// its three (axis, frequency) pairs below (lo for the first, hi for the last)
+// and the 6 values of slots 18..23 (lanes 0 and 1; zeroed later where unused).
+struct InferRaw {
+  float lo, hi;
+  float e[6];
+};
+
+// Samples past the end (n >= 1) read the last one: each sample is its own
+// column of the products, and only the outputs of samples below n are stored.
+__device__ __forceinline__ InferRaw load_raw(const float* __restrict__ raw, uint32_t idx, uint32_t n, int q) {
+  const float* row = raw + (size_t)(idx < n ? idx : n - 1) * 16;
+  InferRaw r;
+  r.lo = row[(3 * q) >> 2];
+  r.hi = row[(3 * q + 2) >> 2];
+  const int off = q == 1 ? 9 : 3;  // features 3..8 (lane 0) or 9..14 (lane 1)
+#pragma unroll
+  for (int k = 0; k < 6; ++k) r.e[k] = row[off + k];
+  return r;
+}
+
 // Layer 0's B operands of one sample (lane q's 24 features, slot order).
 // The 72 sin / cos features come from 12 (axis a, base frequency 3b) pairs
-// P = 4a + b, each a sinpi / cospi and two double-angle steps (the arithmetic
+// P = 4a + b, each a sincospi_enc and two double-angle steps (the arithmetic
 // of encode_axis); lane q evaluates pairs 3q .. 3q + 2, pair m of the lane
 // filling slots 6m .. 6m + 2 (sin at frequencies 3b .. 3b + 2) and
-// 6m + 3 .. 6m + 5 (cos).  Slots 18..23 of lane q hold features 72 + 6q ..
-// 77 + 6q (direction, normal, albedo, exponent feature, zeros).
-__device__ __forceinline__ void encode_b(const float* __restrict__ raw, uint32_t idx, uint32_t n, int q, bf16x8* b) {
-  float4 r[4];  // features 0..15
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-    r[i] = idx < n ? reinterpret_cast<const float4*>(raw)[(size_t)idx * 4 + i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-  const float f[13] = {r[0].x, r[0].y, r[0].z, r[0].w, r[1].x, r[1].y, r[1].z,
-                       r[1].w, r[2].x, r[2].y, r[2].z, r[2].w, r[3].x};
+// 6m + 3 .. 6m + 5 (cos).  The axis of pair 3q + m is that of pair 3q (lo)
+// for m = 0, of pair 3q + 2 (hi) for m = 2, and for m = 1 hi on lane 1, lo
+// elsewhere.  Slots 18..23 of lane q hold features 72 + 6q .. 77 + 6q
+// (direction, normal, albedo, exponent feature, zeros).
+__device__ __forceinline__ void encode_b(const InferRaw& r, int q, bf16x8* b) {
   float v[24];
 #pragma unroll
   for (int m = 0; m < 3; ++m) {
-    const int P = 3 * q + m, a = P >> 2;
-    const float x = ldexpf(a == 0 ? f[0] : (a == 1 ? f[1] : f[2]), 3 * (P & 3));  // exact
-    float sn = sinpif(x), cs = cospif(x);
+    const int P = 3 * q + m;
+    const float p = m == 0 ? r.lo : (m == 2 || q == 1 ? r.hi : r.lo);
+    const float x = ldexpf(p, 3 * (P & 3));  // exact
+    float sn, cs;
+    sincospi_enc(x, sn, cs);
     v[6 * m] = sn;
     v[6 * m + 3] = cs;
 #pragma unroll
@@ -199,7 +244,7 @@ __device__ __forceinline__ void encode_b(const float* __restrict__ raw, uint32_t
     }
   }
 #pragma unroll
-  for (int k = 0; k < 6; ++k) v[18 + k] = q == 0 ? f[3 + k] : (q == 1 && k < 4 ? f[9 + k] : 0.0f);
+  for (int k = 0; k < 6; ++k) v[18 + k] = (q == 0 || (q == 1 && k < 4)) ? r.e[k] : 0.0f;
 #pragma unroll
   for (int ks = 0; ks < 3; ++ks)
     b[ks] = __builtin_bit_cast(bf16x8, (u32x4){pack_bf2(v[8 * ks], v[8 * ks + 1]), pack_bf2(v[8 * ks + 2], v[8 * ks + 3]),
@@ -207,11 +252,21 @@ __device__ __forceinline__ void encode_b(const float* __restrict__ raw, uint32_t
                                                pack_bf2(v[8 * ks + 6], v[8 * ks + 7])});
 }
 
+// The A fragments of a layer with 64 outputs (lane (s, q): 8 inputs of row
+// 16 ob + s), read once from the LDS copy and kept in registers.
+template <int K_IN>
+__device__ __forceinline__ void load_layer(const uint16_t* W, bf16x8 (&a)[K_IN / 32][4], int s, int q) {
+#pragma unroll
+  for (int ks = 0; ks < K_IN / 32; ++ks)
+#pragma unroll
+    for (int ob = 0; ob < 4; ++ob) a[ks][ob] = ld8(W + (16 * ob + s) * (K_IN + 8) + 32 * ks + 8 * q);
+}
+
 // One layer with 64 outputs for kInferGroups groups: K_IN / 32 K-steps of
 // B operands in, ReLU'd BF16 B operands of the next layer out.
 template <int K_IN>
-__device__ __forceinline__ void infer_layer(const uint16_t* __restrict__ W, const bf16x8 (&x)[kInferGroups][K_IN / 32],
-                                            bf16x8 (&h)[kInferGroups][2], int s, int q) {
+__device__ __forceinline__ void infer_layer(const bf16x8 (&A)[K_IN / 32][4], const bf16x8 (&x)[kInferGroups][K_IN / 32],
+                                            bf16x8 (&h)[kInferGroups][2]) {
   f32x4 acc[kInferGroups][4];
 #pragma unroll
   for (int g = 0; g < kInferGroups; ++g)
@@ -220,11 +275,9 @@ __device__ __forceinline__ void infer_layer(const uint16_t* __restrict__ W, cons
 #pragma unroll
   for (int ks = 0; ks < K_IN / 32; ++ks)
 #pragma unroll
-    for (int ob = 0; ob < 4; ++ob) {
-      const bf16x8 a = ld8(W + (16 * ob + s) * (K_IN + 8) + 32 * ks + 8 * q);
+    for (int ob = 0; ob < 4; ++ob)
 #pragma unroll
-      for (int g = 0; g < kInferGroups; ++g) acc[g][ob] = mfma(a, x[g][ks], acc[g][ob]);
-    }
+      for (int g = 0; g < kInferGroups; ++g) acc[g][ob] = mfma(A[ks][ob], x[g][ks], acc[g][ob]);
 #pragma unroll
   for (int g = 0; g < kInferGroups; ++g)
 #pragma unroll
@@ -236,6 +289,16 @@ __device__ __forceinline__ void infer_layer(const uint16_t* __restrict__ W, cons
     }
 }
 
+// Lane (s, 0) of each group writes its sample's rgb (and a 0).
+__device__ __forceinline__ void store_out(float* __restrict__ out, const f32x4 (&y)[kInferGroups], uint32_t u,
+                                          uint32_t n, int s, int q) {
+#pragma unroll
+  for (int g = 0; g < kInferGroups; ++g) {
+    const uint32_t idx = (u * kInferGroups + g) * 16 + s;
+    if (q == 0 && idx < n) reinterpret_cast<float4*>(out)[idx] = make_float4(y[g][0], y[g][1], y[g][2], 0.0f);
+  }
+}
+
 __global__ __launch_bounds__(kInferBlock) void k_mlp_infer(const uint16_t* __restrict__ wbi,
                                                            const float* __restrict__ raw, float* __restrict__ out,
                                                            uint32_t n) {
@@ -243,35 +306,63 @@ __global__ __launch_bounds__(kInferBlock) void k_mlp_infer(const uint16_t* __res
   for (int i = threadIdx.x; i < kMlpInferSize / 8; i += kInferBlock)
     reinterpret_cast<uint4*>(sW)[i] = reinterpret_cast<const uint4*>(wbi)[i];
   __syncthreads();  // the weights are shared and read-only from here on
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  // wave-uniform, so the sample loop runs on scalar registers
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int s = lane & 15, q = lane >> 4;
   const uint32_t steps = (n + 16 * kInferGroups - 1) / (16 * kInferGroups);
   const uint32_t wavesTotal = gridDim.x * (kInferBlock / 64);
-  // each wave walks its own sequence of kInferGroups x 16 samples
-  for (uint32_t u = blockIdx.x * (kInferBlock / 64) + wave; u < steps; u += wavesTotal) {
+  // The A fragments do not depend on the samples: 38 of them (152 VGPRs) stay
+  // in registers for the whole loop.
+  bf16x8 A0[kMlpIn / 32][4], A1[2][4], A2[2][4], A3[2][4], A4[2];
+  load_layer<kMlpIn>(sW + kMlpInferOff[0], A0, s, q);
+  load_layer<kMlpHidden>(sW + kMlpInferOff[1], A1, s, q);
+  load_layer<kMlpHidden>(sW + kMlpInferOff[2], A2, s, q);
+  load_layer<kMlpHidden>(sW + kMlpInferOff[3], A3, s, q);
+#pragma unroll
+  for (int ks = 0; ks < 2; ++ks) A4[ks] = ld8(sW + kMlpInferOff[4] + s * (kMlpHidden + 8) + 32 * ks + 8 * q);
+  // Each wave walks its own sequence of kInferGroups x 16 samples; the raw
+  // features of its next step are loaded while it computes this one.
+  // A step's outputs are stored at the start of the next one, behind that
+  // step's loads, so the wait for those loads never waits for fresh stores.
+  const uint32_t first = blockIdx.x * (kInferBlock / 64) + wave;
+  uint32_t u = first;
+  InferRaw next[kInferGroups];
+#pragma unroll
+  for (int g = 0; g < kInferGroups; ++g) {
+    next[g] = load_raw(raw, (u * kInferGroups + g) * 16 + s, n, q);
+    // The first step's features are waited for here, before the loop: the
+    // wait at the top of a step then never counts that step's own prefetch.
+    asm volatile("" ::"v"(next[g].lo), "v"(next[g].hi), "v"(next[g].e[0]), "v"(next[g].e[1]), "v"(next[g].e[2]),
+                 "v"(next[g].e[3]), "v"(next[g].e[4]), "v"(next[g].e[5]));
+  }
+  f32x4 y[kInferGroups];
+  for (; u < steps; u += wavesTotal) {
+    InferRaw cur[kInferGroups];
+    const uint32_t un = u + wavesTotal;  // past the last step: clamped reads, never used
+#pragma unroll
+    for (int g = 0; g < kInferGroups; ++g) {
+      cur[g] = next[g];
+      next[g] = load_raw(raw, (un * kInferGroups + g) * 16 + s, n, q);
+    }
+    if (u != first) store_out(out, y, u - wavesTotal, n, s, q);
+    // keep the loads and stores here, a whole step ahead of the next wait
+    __builtin_amdgcn_sched_barrier(0);
     bf16x8 x0[kInferGroups][3];
 #pragma unroll
-    for (int g = 0; g < kInferGroups; ++g) encode_b(raw, (u * kInferGroups + g) * 16 + s, n, q, x0[g]);
+    for (int g = 0; g < kInferGroups; ++g) encode_b(cur[g], q, x0[g]);
     bf16x8 h[kInferGroups][2], h2[kInferGroups][2];
-    infer_layer<kMlpIn>(sW + kMlpInferOff[0], x0, h, s, q);
-    infer_layer<kMlpHidden>(sW + kMlpInferOff[1], h, h2, s, q);
-    infer_layer<kMlpHidden>(sW + kMlpInferOff[2], h2, h, s, q);
-    infer_layer<kMlpHidden>(sW + kMlpInferOff[3], h, h2, s, q);
-    f32x4 y[kInferGroups];
+    infer_layer<kMlpIn>(A0, x0, h);
+    infer_layer<kMlpHidden>(A1, h, h2);
+    infer_layer<kMlpHidden>(A2, h2, h);
+    infer_layer<kMlpHidden>(A3, h, h2);
 #pragma unroll
     for (int g = 0; g < kInferGroups; ++g) y[g] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-    for (int ks = 0; ks < 2; ++ks) {
-      const bf16x8 a = ld8(sW + kMlpInferOff[4] + s * (kMlpHidden + 8) + 32 * ks + 8 * q);
+    for (int ks = 0; ks < 2; ++ks)
 #pragma unroll
-      for (int g = 0; g < kInferGroups; ++g) y[g] = mfma(a, h2[g][ks], y[g]);
-    }
-#pragma unroll
-    for (int g = 0; g < kInferGroups; ++g) {
-      const uint32_t idx = (u * kInferGroups + g) * 16 + s;
-      if (q == 0 && idx < n) reinterpret_cast<float4*>(out)[idx] = make_float4(y[g][0], y[g][1], y[g][2], 0.0f);
-    }
+      for (int g = 0; g < kInferGroups; ++g) y[g] = mfma(A4[ks], h2[g][ks], y[g]);
   }
+  if (u != first) store_out(out, y, u - wavesTotal, n, s, q);
 }
 
 // ---------------------------------------------------------------- gradient

@@ -42,9 +42,17 @@ def mctx(rt):
 
 def test_encoding(mctx):
     f = features(1000, 1)
+    # positions where sin or cos of 2^k pi p is 0 or +-1 for every k, or where the
+    # reduction to [-1/2, 1/2] changes sides
+    edge = np.array([0.0, 1.0, 0.5, 0.25, 2.0 ** -9, 2.0 ** -10, 2.0 ** -12, 1.0 - 2.0 ** -24, 2.0 ** -24,
+                     1.0 / 3.0, 0.75 + 2.0 ** -12, 0.5 - 2.0 ** -11], np.float32)
+    f[:len(edge), 0] = edge
+    f[:len(edge), 1] = edge[::-1]
+    f[len(edge):2 * len(edge), 2] = edge
     enc = mlp.from_bf16_bits(mctx.mlp_encode(f))
     ref = mlp.encode(f)
-    # sinpi/cospi on the device are within a couple of FP32 ulp: at most one BF16 step apart
+    # the device's FP32 sin / cos are within 7e-7 of the exact values (a reduced
+    # polynomial and two double-angle steps): at most one BF16 step apart
     assert np.all(np.abs(enc - ref) <= 2.0 ** -7 * np.abs(ref) + 1e-6)
     assert (enc[:, 82:] == 0).all()
 
