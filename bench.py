@@ -809,6 +809,20 @@ def summarize_wavefront(kt_dir):
     for r in stats[:20]:
         lines.append("%-60s %6d %10.3f %10.1f" % (_short_kernel(r["Name"])[:60], int(r["Calls"]),
                                                  float(r["TotalDurationNs"]) / 1e6, float(r["AverageNs"]) / 1e3))
+    resets = {}
+    for r, nm, d in zip(rows, names, dur):
+        if nm == "k_wide_queue_reset":
+            resets.setdefault(r.get("Queue_Id", "?"), []).append(d)
+    if resets:
+        lines += ["", "k_wide_queue_reset (one workgroup zeroing the batch counters before each k_trace_wide) "
+                  "by hardware queue:"]
+        for qid, ds in sorted(resets.items()):
+            ds.sort()
+            lines.append("  queue %s: %d launches, median %.1f us, max %.1f us" % (qid, len(ds), ds[len(ds) // 2],
+                                                                               ds[-1]))
+        lines += ["A reset queued beside the other stream's persistent trace waits for a free wave slot, which",
+                  "opens only as that trace's waves exit; the trace launched behind it waits for the same slots,",
+                  "so its long durations there are queueing, not work."]
     for mode, title in ((0, "path-tracer"), (1, "BPT")):
         raygen = [i for i, n in enumerate(names) if n == "k_wf_raygen<%d>" % mode]
         if not raygen:
