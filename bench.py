@@ -568,6 +568,17 @@ def clutter_benchmark(steps, iterations, incoherent_only=False, calibrate=False,
         img, _ = ctx.readback(W, H)
         integ[name] = {"ms_per_iteration": round(ms / iterations, 3),
                        "image_mean": float(img[..., :3].mean() / (iterations + 1))}
+        if name == "pt_wavefront":
+            # the same frame recorded into a HIP graph and replayed: no host launch per pass
+            ctx.capture_draw(integrator, rt.wanghash(iterations + 1))
+            ctx.replay_draw()
+            ctx.sync()
+            ms = 0.0
+            for k in range(iterations):
+                ctx.timer_begin()
+                ctx.replay_draw()
+                ms += ctx.timer_end()
+            integ[name]["ms_per_iteration_graph_replay"] = round(ms / iterations, 3)
     ctx.close()
     info = hs.info()
     return {
