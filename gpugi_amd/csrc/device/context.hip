@@ -1252,6 +1252,7 @@ int gpugi_hierarchy_update_importance(gpugi_ctx* ctx, int iterations) {
   HIP_TRY(ctx, hipMemsetAsync(acc, 0, entries * 8, ctx->stream));
   for (int i = 1; i <= iterations; ++i)  // NumImportanceIterations = i (hierarchyimportance.cpp:93-103)
     HIP_TRY(ctx, LaunchHierAcquire(p, i, acc, ctx->numNodes, ctx->stream));
+  HIP_TRY(ctx, LaunchHierSubtreeSum(p.ref, ctx->hierTree, acc, ctx->stream));
   HIP_TRY(ctx, LaunchHierToFloat(acc, entries, ctx->stream));
   HIP_TRY(ctx, LaunchHierPropagate(p.ref, ctx->hierTree, iterations, static_cast<float2*>(ctx->dHierImp),
                                    static_cast<float*>(ctx->dHierSubtree), ctx->stream));

@@ -33,6 +33,10 @@ bool BuildHierTree(const float* hierarchy, uint32_t numNodes, std::vector<int32_
 // the first pass).
 hipError_t LaunchHierAcquire(const DrawParams& p, int iteration, unsigned long long* importance, uint32_t numNodes,
                              hipStream_t stream);
+// After the acquisition passes: each node's subtree sum of count_hits' node
+// terms (one launch per tree level, deepest first), in place.
+hipError_t LaunchHierSubtreeSum(const RefScene& s, const HierTree& tree, unsigned long long* importance,
+                                hipStream_t stream);
 // The fixed-point sums of `entries` entries to FP32 {sum, 0} pairs, in place.
 hipError_t LaunchHierToFloat(unsigned long long* importance, size_t entries, hipStream_t stream);
 // Propagation over the tree: importance {x, y} per node (x divided by the

@@ -55,9 +55,15 @@ static_assert(kHierPathLength <= kImpFracBits, "every addend is a whole number o
 // split into a few groups below them, which would otherwise take one
 // contended atomic per ray.
 constexpr int kCombineGroups = 8;
+// children an inner node can have in the count walk's child loop (the
+// reference trees are binary)
+constexpr int kMaxChildren = 16;
 
+// w may be negative (a node's term w * (1 - c) below); the sums wrap around in
+// 64 bits and end non-negative.
 __device__ __forceinline__ void add_importance(unsigned long long* __restrict__ acc, uint32_t key, float w) {
-  const unsigned long long units = (unsigned long long)(w * (float)(1 << kImpFracBits));  // exact: w = 2^-k
+  const unsigned long long mag = (unsigned long long)(fabsf(w) * (float)(1 << kImpFracBits));  // exact: |w| = m 2^-k
+  const unsigned long long units = w < 0.0f ? 0ull - mag : mag;
   const int lane = (int)(threadIdx.x & 63);
   uint64_t remaining = __ballot(true);  // wave-uniform: the lanes whose addition is still to be made
   for (int g = 0; g < kCombineGroups && remaining != 0; ++g) {
@@ -72,7 +78,31 @@ __device__ __forceinline__ void add_importance(unsigned long long* __restrict__ 
   atomicAdd(&acc[key], units);
 }
 
-// TraceRayCountHits (counthits.glsl:1-71) over the reference layout.
+// Whether the count walk enters node n (its box is hit within rayLength); a
+// and b receive the node's first two texels (child code in a.w, escape in b.w).
+__device__ __forceinline__ bool count_enters(const RefScene& s, gg::V3 o, gg::V3 d, gg::V3 inv, float rayLength,
+                                             int n, float4& a, float4& b) {
+  float newHit, exitDist;
+  bool boxHit;
+  if (s.obox) {
+    a = s.nodes[3 * n];
+    b = s.nodes[3 * n + 1];
+    boxHit = intersect_obox(o, d, a, b, s.nodes[3 * n + 2], newHit, exitDist);
+  } else {
+    a = s.nodes[2 * n];
+    b = s.nodes[2 * n + 1];
+    boxHit = intersect_box(o, inv, gg::v3(a.x, a.y, a.z), gg::v3(b.x, b.y, b.z), newHit, exitDist);
+  }
+  return boxHit && newHit <= rayLength;
+}
+
+// TraceRayCountHits (counthits.glsl:1-71) over the reference layout.  The
+// nodes a walk enters are closed under parents, so instead of w at every
+// entered node it adds w * (1 - c), c being how many of the node's children
+// the walk enters (tested here with the walk's own test), and
+// k_hier_subtree_sum later gives each node the sum over its subtree, which is
+// w for each walk that entered it.  Nodes with one entered child, most of a
+// walk, then need no atomic.  Triangles keep their own w.
 __device__ void count_hits(const RefScene& s, gg::V3 o, gg::V3 d, float rayLength, float w,
                            unsigned long long* __restrict__ imp, uint32_t numNodes) {
   int node = 0, leaf = 0;
@@ -80,24 +110,28 @@ __device__ void count_hits(const RefScene& s, gg::V3 o, gg::V3 d, float rayLengt
   const gg::V3 inv = gg::v3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
   do {
     if (!nextIsLeaf) {
-      float newHit, exitDist;
-      bool boxHit;
       float4 a, b;
-      if (s.obox) {
-        a = s.nodes[3 * node];
-        b = s.nodes[3 * node + 1];
-        boxHit = intersect_obox(o, d, a, b, s.nodes[3 * node + 2], newHit, exitDist);
-      } else {
-        a = s.nodes[2 * node];
-        b = s.nodes[2 * node + 1];
-        boxHit = intersect_box(o, inv, gg::v3(a.x, a.y, a.z), gg::v3(b.x, b.y, b.z), newHit, exitDist);
-      }
+      const bool entered = count_enters(s, o, d, inv, rayLength, node, a, b);
       const uint32_t childCode = __float_as_uint(a.w);
       const int escape = __float_as_int(b.w);
-      if (boxHit && newHit <= rayLength) {
-        add_importance(imp, (uint32_t)node, w);
+      if (entered) {
         const int child = (int)(childCode & 0x7fffffffu);
         nextIsLeaf = (uint32_t)child != childCode;
+        int c = 0;
+        if (!nextIsLeaf) {
+          // the children: the first child and the siblings its escape chain
+          // reaches before this node's escape (the parent rule of BuildHierTree)
+          const int end = escape == 0 ? (int)numNodes : escape;
+          int k = child;
+          for (int it = 0; it < kMaxChildren && k > node && k < end; ++it) {
+            float4 ca, cb;
+            if (count_enters(s, o, d, inv, rayLength, k, ca, cb)) ++c;
+            const int next = __float_as_int(cb.w);
+            if (next == 0) break;
+            k = next;
+          }
+        }
+        if (c != 1) add_importance(imp, (uint32_t)node, w * (float)(1 - c));
         if (nextIsLeaf) {
           leaf = 2 * child;
           node = escape;
@@ -183,6 +217,30 @@ __global__ __launch_bounds__(kHierBlock) __attribute__((amdgpu_waves_per_eu(8)))
 // The fixed-point sums to the FP32 {sum, 0} pairs the propagation reads (in
 // place: each entry is 8 bytes either way).  The conversion is exact while a
 // sum stays below 2^24 units.
+// One tree level of the subtree sums of count_hits' node terms: an inner node
+// adds its children's totals (final, as their level ran before).  The
+// children follow the rule of BuildHierTree, which also checked the tree.
+__global__ __launch_bounds__(kHierBlock) void k_hier_subtree_sum(NodeView nv, unsigned long long* __restrict__ acc,
+                                                                 const uint32_t* __restrict__ order, uint32_t count,
+                                                                 uint32_t numNodes) {
+  const uint32_t i = blockIdx.x * kHierBlock + threadIdx.x;
+  if (i >= count) return;
+  const int n = (int)order[i];
+  const uint32_t cc = __float_as_uint(nv.t0(n).w);
+  if (cc & 0x80000000u) return;  // a leaf node: no child nodes
+  const int escape = __float_as_int(nv.t1(n).w);
+  const int end = escape == 0 ? (int)numNodes : escape;
+  unsigned long long sum = 0;
+  int k = (int)cc;
+  for (int it = 0; it < kMaxChildren && k > n && k < end; ++it) {
+    sum += acc[k];
+    const int next = __float_as_int(nv.t1(k).w);
+    if (next == 0) break;
+    k = next;
+  }
+  acc[n] += sum;
+}
+
 __global__ __launch_bounds__(kHierBlock) void k_hier_to_float(unsigned long long* __restrict__ acc, size_t n) {
   const size_t i = (size_t)blockIdx.x * kHierBlock + threadIdx.x;
   if (i >= n) return;
@@ -393,6 +451,18 @@ hipError_t LaunchHierAcquire(const DrawParams& p, int iteration, unsigned long l
   if (p.numLights == 0 || !p.lights) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_hier_acquire, Blocks(numPixels), dim3(kHierBlock), 0, stream, p,
                      gg::wanghash((uint32_t)iteration), importance, numNodes, tilesW, numPixels);
+  return hipGetLastError();
+}
+
+hipError_t LaunchHierSubtreeSum(const RefScene& s, const HierTree& tree, unsigned long long* importance,
+                                hipStream_t stream) {
+  const NodeView nv{s.nodes, s.obox};
+  for (int lv = (int)tree.levelStart.size() - 2; lv >= 0; --lv) {  // deepest level first
+    const uint32_t first = tree.levelStart[lv], count = tree.levelStart[lv + 1] - first;
+    if (count == 0) continue;
+    hipLaunchKernelGGL(k_hier_subtree_sum, Blocks(count), dim3(kHierBlock), 0, stream, nv, importance,
+                       tree.levelOrder + first, count, tree.numNodes);
+  }
   return hipGetLastError();
 }
 
